@@ -1,0 +1,80 @@
+"""Structured quad-mesh generator emitting the reference's unstructured mesh format.
+
+The reference ships no meshes; every workload here is an n x m grid of
+counter-clockwise quads (CHECK_MESH keeps cell order as given, connectivity.f90:264-272),
+with interior vertices optionally jittered by a seeded uniform offset.  Boundary
+vertices stay on the grid so periodic partner faces keep matching midpoints
+(map_cyc_loc.f90:24-28, tolerance 1e-5).
+
+Numbering: vertex (i, j) -> 1 + i + j*(n+1); cell (i, j) -> 1 + i + j*n with
+vertices (i,j), (i+1,j), (i+1,j+1), (i,j+1).  Boundary faces are listed
+counter-clockwise: bottom (left to right), right (bottom to top), top (right to
+left), left (top to bottom).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .formats import Mesh, fmt_d20
+
+
+def quad_grid(n: int, m: int, x0: float, x1: float, y0: float, y1: float,
+              jitter: float = 0.0, seed: int = 0, bc: str = "CYCL") -> Mesh:
+    """n x m cells on [x0,x1] x [y0,y1].
+
+    ``jitter``: interior vertices move by U(-jitter, jitter) * (hx, hy), drawn
+    from ``numpy.random.default_rng(seed)`` (x offsets for all interior vertices
+    in vertex order, then y offsets).  ``bc``: boundary type on all four sides
+    (``CYCL`` periodic or ``FREE`` zero-gradient outflow).
+    Coordinates are rounded through the 20-character vertex-file field, so the
+    returned mesh is exactly what the reference reads back.
+    """
+    if n < 1 or m < 1:
+        raise ValueError("mesh needs at least one cell per direction")
+    if not 0.0 <= jitter < 0.5:
+        raise ValueError("jitter must be in [0, 0.5) of the cell size")
+    hx = (x1 - x0) / n
+    hy = (y1 - y0) / m
+    ii, jj = np.meshgrid(np.arange(n + 1), np.arange(m + 1), indexing="xy")
+    xv = x0 + ii.astype(np.float64) * hx
+    yv = y0 + jj.astype(np.float64) * hy
+    # exact end points
+    xv[:, n] = x1
+    yv[m, :] = y1
+    if jitter > 0.0 and n > 1 and m > 1:
+        rng = np.random.default_rng(seed)
+        ni = (n - 1) * (m - 1)
+        dx = rng.uniform(-jitter, jitter, ni).reshape(m - 1, n - 1)
+        dy = rng.uniform(-jitter, jitter, ni).reshape(m - 1, n - 1)
+        xv[1:m, 1:n] += dx * hx
+        yv[1:m, 1:n] += dy * hy
+    xv = xv.reshape(-1)
+    yv = yv.reshape(-1)
+    # text round trip: what readGRIDinfo.f90:186 reads back
+    xv = np.array([float(fmt_d20(v)) for v in xv])
+    yv = np.array([float(fmt_d20(v)) for v in yv])
+
+    def vid(i, j):
+        return 1 + i + j * (n + 1)
+
+    ci, cj = np.meshgrid(np.arange(n), np.arange(m), indexing="xy")
+    ci = ci.reshape(-1)
+    cj = cj.reshape(-1)
+    cells = np.stack([vid(ci, cj), vid(ci + 1, cj), vid(ci + 1, cj + 1), vid(ci, cj + 1)],
+                     axis=1).astype(np.int32)
+
+    bnd = []
+    for i in range(n):                      # bottom, left to right
+        bnd.append((vid(i, 0), vid(i + 1, 0)))
+    for j in range(m):                      # right, bottom to top
+        bnd.append((vid(n, j), vid(n, j + 1)))
+    for i in range(n - 1, -1, -1):          # top, right to left
+        bnd.append((vid(i + 1, m), vid(i, m)))
+    for j in range(m - 1, -1, -1):          # left, top to bottom
+        bnd.append((vid(0, j + 1), vid(0, j)))
+    bnd = np.array(bnd, dtype=np.int32)
+    btype = [bc] * len(bnd)
+    # DXCYCL/DYCYCL are the periodic shifts and also the AV length scale
+    # h = sqrt(DXCYCL*DYCYCL/NCELLGLOB) (AV.f90:147), so they carry the domain
+    # extents for FREE boundaries too.
+    return Mesh(xv, yv, cells, bnd, btype, x1 - x0, y1 - y0)
