@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""DOF-updates/s per RK stage of the CHORUS-MHD residual + SSP-RK time step on one MI355X.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ot512]
+
+One "step" is one SSP(5,3) time step (five residual evaluations fused with
+their stage updates, then the B := curl A_z correction) of the whole mesh,
+whose state is resident in HBM before timing starts.  DOF = one solution
+point (9 FP64 state values); value = NCELL*N^2 * 5 * K / wall time of the K
+timed steps.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "chorus-mhd_amd"))
+
+from chorus_mhd import QuadInp, quad_grid          # noqa: E402
+from chorus_mhd import ic as ics                    # noqa: E402
+from chorus_mhd.solver import Solver, topology      # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "DOF-updates/s per RK stage"
+
+# workloads (n, m, domain, N, jitter, seed, bc, ic, gamma, AV)
+WORKLOADS = {
+    "ot512": dict(desc="Orszag-Tang 512x512 quads, interior nodes jittered +-10% (seed 7), SD order 4 "
+                       "(N=5), gamma=5/3, SSP(5,3), CFL 0.2",
+                  n=512, m=512, box=(0.0, 2 * math.pi, 0.0, 2 * math.pi), N=5, jitter=0.10, seed=7,
+                  bc="CYCL", ic="ot", gam=5.0 / 3, artiv=1),
+    "ot64": dict(desc="Orszag-Tang 64x64 uniform quads, SD order 3 (N=4), gamma=5/3, SSP(5,3), CFL 0.2, AV off",
+                 n=64, m=64, box=(0.0, 2 * math.pi, 0.0, 2 * math.pi), N=4, jitter=0.0, seed=0,
+                 bc="CYCL", ic="ot", gam=5.0 / 3, artiv=0),
+    "loop2048": dict(desc="field loop 2048x1024 jittered (seed 7), order 4 (N=5), u=(2,1), AV off",
+                     n=2048, m=1024, box=(-1.0, 1.0, -0.5, 0.5), N=5, jitter=0.10, seed=7,
+                     bc="CYCL", ic="loop", gam=5.0 / 3, artiv=0),
+}
+
+
+def b_stage(N: int) -> float:
+    """Algorithmic HBM bytes per DOF per stage: FP64 state + RK registers + metrics
+    (224 + 32(N+1)/N)."""
+    return 224.0 + 32.0 * (N + 1) / N
+
+
+def setup(w, precision, av_supported):
+    mesh = quad_grid(w["n"], w["m"], *w["box"], jitter=w["jitter"], seed=w["seed"], bc=w["bc"])
+    topo = topology(mesh)
+    artiv = w["artiv"] if av_supported else 0
+    s = Solver(mesh, w["N"], 1e-3, gam=w["gam"], artiv=artiv, precision=precision, topo=topo)
+    return mesh, topo, s, artiv
+
+
+def initial_state(w, mesh, s):
+    if w["ic"] == "ot":
+        return ics.orszag_tang(s, mesh, w["N"], w["gam"])
+    if w["ic"] == "loop":
+        return ics.field_loop(s, mesh, w["N"], w["gam"])
+    return ics.rotor(s, mesh, w["N"], w["gam"])
+
+
+def cpu_baseline(w, artiv, budget_s=30.0):
+    """The compiled reference (oracle/_ref/ref_harness) on a bounded sample of
+    the workload on one host core; falls back to the C oracle port."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as orc
+    n = 96 if w["N"] >= 5 else 128
+    steps = 2
+    N = w["N"]
+    sample = "%dx%d %s mesh (jitter %.2f, seed %d), N=%d, AV %s, %d SSP(5,3) steps" % (
+        n, n, w["ic"], w["jitter"], w["seed"], N, "on" if artiv else "off", steps)
+    mesh = quad_grid(n, n, *w["box"], jitter=w["jitter"], seed=w["seed"], bc=w["bc"])
+    dofs = mesh.ncell * N * N * 5 * steps
+    if os.path.exists(orc.REF_HARNESS) and abs(w["gam"] - 5.0 / 3) < 1e-15:
+        with tempfile.TemporaryDirectory() as d:
+            mesh.write(d)
+            QuadInp(N=N, dt=1e-4, MAXITER=steps, DXCYCL=mesh.dxcycl, DYCYCL=mesh.dycycl,
+                    ARTIV=artiv, restart_ord=N).write(os.path.join(d, "QUAD.INP"))
+            r = subprocess.run([orc.REF_HARNESS, "run"], cwd=d, capture_output=True, text=True,
+                               timeout=600, env=dict(os.environ, OMP_NUM_THREADS="1"))
+            if r.returncode == 0:
+                t = open(os.path.join(d, "timing.txt")).read().split()
+                sec = float(t[1])
+                return dict(value=dofs / sec, unit="DOF-updates/s", cores=1, kind="reference",
+                            sample=sample + " (reference Fortran, flang -O3, time loop only)")
+    orc.build()
+    o = orc.Oracle(mesh, N, dt=1e-4, artiv=artiv, gam=w["gam"])
+    o.init_ot()
+    t0 = time.perf_counter()
+    o.steps(steps)
+    sec = time.perf_counter() - t0
+    return dict(value=dofs / sec, unit="DOF-updates/s", cores=1, kind="port",
+                sample=sample + " (C oracle port)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="ot512", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision form")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on exactly one GPU: this build holds the whole mesh as one "
+              "partition on one MI355X (multi-GPU halo exchange is not built)", file=sys.stderr)
+        sys.exit(2)
+    w = WORKLOADS[args.workload]
+    av_supported = False
+    N = w["N"]
+
+    def run(precision, Q0=None):
+        mesh, topo, s, artiv = setup(w, precision, av_supported)
+        if Q0 is None:
+            Q0 = initial_state(w, mesh, s)
+        else:
+            s.set_state(Q0)
+        dt = ics.cfl_dt(mesh, N, Q0, w["gam"], 0.2)
+        s.close()
+        s = Solver(mesh, N, dt, gam=w["gam"], artiv=artiv, precision=precision, topo=topo)
+        s.set_state(Q0)
+        if args.warmup:
+            s.step(args.warmup)
+            s.set_state(Q0)
+        t0 = time.perf_counter()
+        tm = s.time_steps(args.steps)
+        wall = time.perf_counter() - t0
+        Q = s.get_state()
+        d = s.diag()
+        s.close()
+        return dict(mesh=mesh, dt=dt, wall=wall, tm=tm, Q=Q, Q0=Q0, diag=d, artiv=artiv)
+
+    r64 = run("fp64")
+    mesh = r64["mesh"]
+    ndof = mesh.ncell * N * N
+    value = ndof * 5 * args.steps / r64["wall"]
+    stage_s = r64["tm"]["ms_stage_avg"] * 1e-3
+    achieved = ndof * b_stage(N) / stage_s / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r64["wall"] * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": w["desc"] + (", AV on" if r64["artiv"] else ", AV off"),
+            "form": "FP64 drop-in",
+            "ncell": mesh.ncell, "N": N, "dof": ndof, "dt": r64["dt"],
+            "av_requested": bool(w["artiv"]), "av_applied": bool(r64["artiv"]),
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "k_stage_ideal<%d,double>" % N,
+            "stage_kernel_ms": r64["tm"]["ms_stage_avg"], "ct_kernel_ms": r64["tm"]["ms_ct_avg"],
+            "bytes_per_dof_stage": b_stage(N),
+        },
+        "divb": {"rms": r64["diag"]["divbglob"], "max": r64["diag"]["divbmax"]},
+    }
+    if not args.no_mixed:
+        rm = run("mixed", Q0=r64["Q0"])
+        vm = ndof * 5 * args.steps / rm["wall"]
+        a, b = r64["Q"], rm["Q"]
+        out["mixed"] = {
+            "value": vm, "speedup_vs_fp64": vm / value,
+            "rel_l2_vs_fp64": float(np.linalg.norm(b - a) / np.linalg.norm(a)),
+            "linf_vs_fp64": float(np.abs(b - a).max()),
+            "stage_kernel_ms": rm["tm"]["ms_stage_avg"],
+            "roofline_frac": ndof * b_stage(N) / (rm["tm"]["ms_stage_avg"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "divb_max": rm["diag"]["divbmax"],
+        }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, r64["artiv"])
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

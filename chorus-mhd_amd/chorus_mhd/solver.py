@@ -1,0 +1,152 @@
+"""Host-side mirror of the reference's hot-path subroutines over the C-ABI.
+
+``Solver`` keeps the reference's vocabulary: ``compresid`` (compresid.f90),
+``correct_b`` (correct_B_via_curl_Az, setinitialcond.f90:82-92), ``step``
+(the loop body of iterations.f90:20-33 with RK_S5P3 or FiveStageRk) and
+``diag`` (the norms of iterations.f90:35-67).  States are numpy arrays of shape
+(NCELL, N, N, 9), i.e. the memory order of Fortran ``Q(9,N,N,NCELL)``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import lib as _lib
+from .formats import Mesh
+
+BTYPES = {"CYCL": 1, "FREE": 2}
+
+
+class ChorusError(RuntimeError):
+    pass
+
+
+@dataclass
+class Topology:
+    """IF2C(NFACE,4), IBFINTER, CYC2FPAIR(NPAIR,2), IBFREE of one rank (1-based)."""
+    nface: int
+    if2c: np.ndarray        # (NFACE, 4)
+    ibfinter: np.ndarray
+    cyc2fpair: np.ndarray   # (NPAIR, 2)
+    ibfree: np.ndarray
+
+
+def topology(mesh: Mesh) -> Topology:
+    """CONNECTIVITY + CONNECT_BDRY + MAPCYCLOC of the reference, via the library."""
+    L = _lib.load()
+    nc = mesh.ncell
+    iv = np.asfortranarray(mesh.cells, dtype=np.int32)
+    bnd = np.asfortranarray(mesh.bnd, dtype=np.int32)
+    bt = np.array([BTYPES.get(t, 0) for t in mesh.bnd_type], dtype=np.int32)
+    xv = np.ascontiguousarray(mesh.xv, dtype=np.float64)
+    yv = np.ascontiguousarray(mesh.yv, dtype=np.float64)
+    if2c = np.zeros(16 * nc, dtype=np.int32)
+    ibfinter = np.zeros(4 * nc, dtype=np.int32)
+    cyc = np.zeros(8 * nc, dtype=np.int32)
+    ibfree = np.zeros(4 * nc, dtype=np.int32)
+    nface, ninter, npair, nfree = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    ip = _lib._ip
+    rc = L.chorus_topology(nc, mesh.nvert, _lib.ptr(iv, ip), _lib.ptr(xv), _lib.ptr(yv),
+                           bnd.shape[0], _lib.ptr(bnd, ip), _lib.ptr(bt, ip),
+                           mesh.dxcycl, mesh.dycycl, C.byref(nface), _lib.ptr(if2c, ip),
+                           C.byref(ninter), _lib.ptr(ibfinter, ip), C.byref(npair),
+                           _lib.ptr(cyc, ip), C.byref(nfree), _lib.ptr(ibfree, ip))
+    if rc != 0:
+        raise ChorusError("chorus_topology failed (%d)" % rc)
+    nf, npr = nface.value, npair.value
+    return Topology(nf, if2c[:4 * nf].reshape(4, nf).T.copy(), ibfinter[:ninter.value].copy(),
+                    cyc[:2 * npr].reshape(2, npr).T.copy(), ibfree[:nfree.value].copy())
+
+
+class Solver:
+    """One MHD instance resident on the GPU."""
+
+    def __init__(self, mesh: Mesh, N: int, dt: float, *, gam: float = 5.0 / 3, artiv: int = 0,
+                 k_stage: int = 3, mu: float = 0.0, eta0: float = 0.0, precision: str = "fp64",
+                 topo: Topology = None):
+        L = _lib.load()
+        self.mesh, self.N = mesh, N
+        self.topo = topo if topo is not None else topology(mesh)
+        t = self.topo
+        # keep every array alive while the context is created (it copies them)
+        self._arrs = dict(
+            xv=np.ascontiguousarray(mesh.xv, dtype=np.float64),
+            yv=np.ascontiguousarray(mesh.yv, dtype=np.float64),
+            iv=np.asfortranarray(mesh.cells, dtype=np.int32).ravel(order="F"),
+            if2c=np.asfortranarray(t.if2c, dtype=np.int32).ravel(order="F"),
+            ibfinter=np.ascontiguousarray(t.ibfinter, dtype=np.int32),
+            cyc=np.asfortranarray(t.cyc2fpair, dtype=np.int32).ravel(order="F"),
+            ibfree=np.ascontiguousarray(t.ibfree, dtype=np.int32),
+        )
+        a = self._arrs
+        ip = _lib._ip
+        m = _lib.ChorusMesh(mesh.ncell, mesh.nvert, N, _lib.ptr(a["xv"]), _lib.ptr(a["yv"]),
+                            _lib.ptr(a["iv"], ip), t.nface, _lib.ptr(a["if2c"], ip),
+                            len(t.ibfinter), _lib.ptr(a["ibfinter"], ip),
+                            t.cyc2fpair.shape[0], _lib.ptr(a["cyc"], ip),
+                            len(t.ibfree), _lib.ptr(a["ibfree"], ip))
+        prec = {"fp64": _lib.CHORUS_FP64, "mixed": _lib.CHORUS_MIXED}[precision]
+        p = _lib.ChorusParams(1, artiv, k_stage, mu, eta0, gam, dt, mesh.dxcycl, mesh.dycycl,
+                              mesh.ncell, prec)
+        self._ctx = C.c_void_p()
+        rc = L.chorus_init(C.byref(m), C.byref(p), C.byref(self._ctx))
+        if rc != 0:
+            msg = L.chorus_last_error(self._ctx).decode()
+            L.chorus_free(self._ctx)
+            self._ctx = None
+            raise ChorusError("chorus_init: %s (%d)" % (msg, rc))
+
+    # ---- plumbing -------------------------------------------------------
+    def _check(self, rc, what):
+        if rc != 0:
+            raise ChorusError("%s: %s (%d)" % (what, _lib.load().chorus_last_error(self._ctx).decode(), rc))
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            _lib.load().chorus_free(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def shape(self):
+        return (self.mesh.ncell, self.N, self.N, 9)
+
+    # ---- reference subroutines -----------------------------------------
+    def set_state(self, Q: np.ndarray) -> None:
+        Q = np.ascontiguousarray(Q, dtype=np.float64).reshape(self.shape)
+        self._check(_lib.load().chorus_set_state(self._ctx, _lib.ptr(Q)), "chorus_set_state")
+
+    def get_state(self) -> np.ndarray:
+        Q = np.empty(self.shape)
+        self._check(_lib.load().chorus_get_state(self._ctx, _lib.ptr(Q)), "chorus_get_state")
+        return Q
+
+    def compresid(self) -> np.ndarray:
+        R = np.empty(self.shape)
+        self._check(_lib.load().chorus_compresid(self._ctx, _lib.ptr(R)), "chorus_compresid")
+        return R
+
+    def correct_b(self) -> None:
+        self._check(_lib.load().chorus_correct_b(self._ctx), "chorus_correct_b")
+
+    def step(self, nsteps: int = 1) -> None:
+        self._check(_lib.load().chorus_step(self._ctx, nsteps), "chorus_step")
+
+    def diag(self) -> dict:
+        v = [C.c_double() for _ in range(5)]
+        self._check(_lib.load().chorus_diag(self._ctx, *[C.byref(x) for x in v]), "chorus_diag")
+        return dict(zip(["residglob", "divbglob", "divbmax", "eigvmax", "epsilon0"],
+                        [x.value for x in v]))
+
+    def time_steps(self, nsteps: int) -> dict:
+        tot, st, ct = C.c_double(), C.c_double(), C.c_double()
+        self._check(_lib.load().chorus_time_steps(self._ctx, nsteps, C.byref(tot), C.byref(st),
+                                                  C.byref(ct)), "chorus_time_steps")
+        return dict(ms_total=tot.value, ms_stage_avg=st.value, ms_ct_avg=ct.value)

@@ -1,0 +1,572 @@
+// C-ABI layer (include/chorus_mhd.h): context, device memory, on-device face
+// connectivity, the SSP steppers and the host-side topology helper.
+#include "chorus_mhd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "chorus_internal.hpp"
+
+using namespace chorus;
+
+struct chorus_ctx {
+    std::string err;
+    chorus_params p{};
+    int N = 0, ncell = 0, ncellglob = 0;
+    bool ideal = true;             // mu = eta0 = 0 and no AV: viscous terms vanish exactly
+    hipStream_t stream = nullptr;
+    double *cellxy = nullptr;
+    int2 *nbr = nullptr;
+    double *X[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *R = nullptr;           // residual output / FiveStageRk resid0
+    double *partial = nullptr;     // per-block reduction partials
+    int nblk = 0;
+    int cur = 0;                   // buffer holding the current state
+    int last_in = -1;              // buffer holding the last stage's input state
+    std::vector<hipEvent_t> ev;    // timing events (chorus_time_steps)
+    bool timing = false;
+    size_t ev_used = 0;
+};
+
+namespace {
+
+int fail(chorus_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIP_TRY(c, expr)                                                                   \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(c, CHORUS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+size_t state_len(const chorus_ctx *c) { return (size_t)c->ncell * 9 * c->N * c->N; }
+
+// reference Q(9,N,N,NCELL) <-> device Q[cell][var][js][is]
+void to_device_layout(const chorus_ctx *c, const double *q, std::vector<double> &out) {
+    const int N = c->N, NP = N * N;
+    out.resize(state_len(c));
+    for (int ic = 0; ic < c->ncell; ic++)
+        for (int p = 0; p < NP; p++)
+            for (int v = 0; v < 9; v++) out[((size_t)ic * 9 + v) * NP + p] = q[((size_t)ic * NP + p) * 9 + v];
+}
+void from_device_layout(const chorus_ctx *c, const std::vector<double> &in, double *q) {
+    const int N = c->N, NP = N * N;
+    for (int ic = 0; ic < c->ncell; ic++)
+        for (int p = 0; p < NP; p++)
+            for (int v = 0; v < 9; v++) q[((size_t)ic * NP + p) * 9 + v] = in[((size_t)ic * 9 + v) * NP + p];
+}
+
+bool g_ops_uploaded = false;
+
+int launch_stage(chorus_ctx *c, const StageArgs &a) {
+    hipError_t e;
+    bool t = c->timing && c->ev_used + 2 <= c->ev.size();
+    if (t) (void)hipEventRecord(c->ev[c->ev_used], c->stream);
+    if (c->p.precision == CHORUS_MIXED) e = launch_stage_ideal_mixed(c->N, a, c->stream);
+    else e = launch_stage_ideal_fp64(c->N, a, c->stream);
+    if (t) { (void)hipEventRecord(c->ev[c->ev_used + 1], c->stream); c->ev_used += 2; }
+    if (e != hipSuccess) return fail(c, CHORUS_ERR_HIP, std::string("stage launch: ") + hipGetErrorString(e));
+    return 0;
+}
+
+StageArgs base_args(chorus_ctx *c) {
+    StageArgs a{};
+    a.ncell = c->ncell;
+    a.cellxy = c->cellxy;
+    a.nbr = c->nbr;
+    a.gam = c->p.gam;
+    return a;
+}
+
+int run_ct(chorus_ctx *c, double *Q, int which) {
+    CTArgs a{};
+    a.ncell = c->ncell;
+    a.cellxy = c->cellxy;
+    a.nbr = c->nbr;
+    a.Q = Q;
+    a.which = which;
+    a.partial = c->partial;
+    hipError_t e = launch_ct(c->N, a, c->stream);
+    if (e != hipSuccess) return fail(c, CHORUS_ERR_HIP, std::string("ct launch: ") + hipGetErrorString(e));
+    return 0;
+}
+
+// one RK_S5P3 step (iterations.f90:162-217) + correct_B_via_curl_Az
+int step_s5p3(chorus_ctx *c) {
+    const double a30 = 0.355909775063327, a32 = 0.644090224936674;
+    const double a40 = 0.367933791638137, a43 = 0.632066208361863;
+    const double a52 = 0.237593836598569, a54 = 0.762406163401431;
+    const double b10 = 0.377268915331368, b21 = 0.377268915331368, b32 = 0.242995220537396;
+    const double b43 = 0.238458932846290, b54 = 0.287632146308408;
+    const double dt = c->p.dt;
+    const int ia = c->cur, ib = (c->cur + 1) % 3, ic = (c->cur + 2) % 3;
+    double *A = c->X[ia], *B = c->X[ib], *C = c->X[ic];
+    StageArgs s = base_args(c);
+    int rc;
+    s.Qin = A; s.Qout = B; s.kind = KIND_AXPY; s.c0 = b10 * dt;
+    if ((rc = launch_stage(c, s))) return rc;
+    s.Qin = B; s.Qout = C; s.kind = KIND_AXPY; s.c0 = b21 * dt;
+    if ((rc = launch_stage(c, s))) return rc;
+    s.Qin = C; s.Qout = B; s.Qa = A; s.kind = KIND_3A; s.c1 = a30; s.c2 = a32; s.c0 = b32 * dt;
+    if ((rc = launch_stage(c, s))) return rc;
+    s.Qin = B; s.Qout = A; s.Qa = A; s.kind = KIND_3A; s.c1 = a40; s.c2 = a43; s.c0 = b43 * dt;
+    if ((rc = launch_stage(c, s))) return rc;
+    s.Qin = A; s.Qout = C; s.Qa = C; s.kind = KIND_3A; s.c1 = a52; s.c2 = a54; s.c0 = b54 * dt;
+    if ((rc = launch_stage(c, s))) return rc;
+    c->last_in = ia;
+    c->cur = ic;
+    return 0;
+}
+
+// one FiveStageRk step (iterations.f90:92-160)
+int step_s54(chorus_ctx *c) {
+    const double a11 = 0.39175222700392;
+    const double a21 = 0.44437049406734, a22 = 0.55562950593266, a23 = 0.36841059262959;
+    const double a31 = 0.6201018513854, a32 = 0.3798981486146, a33 = 0.25189177424738;
+    const double a41 = 0.17807995410773, a42 = 0.82192004589227, a43 = 0.54497475021237;
+    const double a51 = 0.00683325884039, a52 = 0.51723167208978, a53 = 0.12759831133288;
+    const double a54 = 0.34833675773694, a55 = 0.08460416338212, a56 = 0.22600748319395;
+    const double dt = c->p.dt;
+    const int ia = c->cur, ib = (c->cur + 1) % 4, ic = (c->cur + 2) % 4, id = (c->cur + 3) % 4;
+    double *A = c->X[ia], *B = c->X[ib], *C = c->X[ic], *D = c->X[id];
+    StageArgs s = base_args(c);
+    int rc;
+    s.Qin = A; s.Qout = B; s.kind = KIND_AXPY; s.c0 = dt * a11;
+    if ((rc = launch_stage(c, s))) return rc;
+    s.Qin = B; s.Qout = C; s.Qa = A; s.kind = KIND_3B; s.c1 = a21; s.c2 = a22; s.c3 = a23; s.c4 = dt;
+    if ((rc = launch_stage(c, s))) return rc;
+    s.Qin = C; s.Qout = D; s.Qa = A; s.kind = KIND_3A; s.c1 = a31; s.c2 = a32; s.c0 = a33 * dt;
+    if ((rc = launch_stage(c, s))) return rc;
+    s.Qin = D; s.Qout = B; s.Qa = A; s.Rout = c->R; s.kind = KIND_S54_4; s.c1 = a41; s.c2 = a42; s.c0 = a43 * dt;
+    if ((rc = launch_stage(c, s))) return rc;
+    s.Qin = B; s.Qout = A; s.Qa = A; s.Qb = C; s.Qc = D; s.Rprev = c->R; s.Rout = nullptr;
+    s.kind = KIND_S54_5; s.c1 = a51; s.c2 = a52; s.c3 = a53; s.c4 = a54; s.c5 = a55 * dt; s.c0 = a56 * dt;
+    if ((rc = launch_stage(c, s))) return rc;
+    c->last_in = ib;
+    c->cur = ia;
+    return 0;
+}
+
+int one_step(chorus_ctx *c) {
+    int rc = c->p.k_stage == 5 ? step_s54(c) : step_s5p3(c);
+    if (rc) return rc;
+    bool t = c->timing && c->ev_used + 2 <= c->ev.size();
+    if (t) (void)hipEventRecord(c->ev[c->ev_used], c->stream);
+    rc = run_ct(c, c->X[c->cur], 0);
+    if (t) { (void)hipEventRecord(c->ev[c->ev_used + 1], c->stream); c->ev_used += 2; }
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// host topology: CONNECTIVITY (connectivity.f90:1-233), CONNECT_BDRY
+// (connect_bdry.f90:1-354), MAPCYCLOC (map_cyc_loc.f90:1-42), one rank
+// ---------------------------------------------------------------------------
+struct Topo {
+    std::vector<int> if2c, if2v, ibfinter, cyc, ibfree;   // column-major as the reference
+    int nface = 0, npair = 0;
+};
+
+int build_topology(int ncell, int nvert, const int *ivcell, const double *xv, const double *yv, int nbnd,
+                   const int *bnd, const int *btype, double dxcycl, double dycycl, Topo &t, std::string &err) {
+    static const int IVF[4][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
+    const double tol = 1e-5;   // tolCYC, setup2d.f90:64
+    auto IVC = [&](int ic, int k) { return ivcell[(size_t)k * ncell + ic]; };
+    std::vector<int> fill(nvert, 0), sta(nvert + 1, 0);
+    for (int ic = 0; ic < ncell; ic++)
+        for (int k = 0; k < 4; k++) {
+            int v = IVC(ic, k);
+            if (v < 1 || v > nvert) { err = "IVCELL vertex id out of range"; return CHORUS_ERR_TOPOLOGY; }
+            fill[v - 1]++;
+        }
+    for (int v = 0; v < nvert; v++)
+        if (!fill[v]) { err = "vertex " + std::to_string(v + 1) + " belongs to no cell"; return CHORUS_ERR_TOPOLOGY; }
+    for (int v = 0; v < nvert; v++) sta[v + 1] = sta[v] + fill[v];
+    std::vector<int> pos(sta.begin(), sta.end() - 1), icvert(sta[nvert]);
+    for (int ic = 0; ic < ncell; ic++)
+        for (int k = 0; k < 4; k++) icvert[pos[IVC(ic, k) - 1]++] = ic;
+    std::vector<int> ic2f((size_t)ncell * 4, 0), f2c, f2v;
+    f2c.reserve((size_t)ncell * 4 * 2);
+    f2v.reserve((size_t)ncell * 4 * 2);
+    int nface = 0;
+    for (int ic = 0; ic < ncell; ic++)
+        for (int fa = 0; fa < 4; fa++) {
+            if (ic2f[ic * 4 + fa]) continue;
+            int vb = IVC(ic, IVF[fa][0]), v1 = IVC(ic, IVF[fa][1]);
+            bool found = false;
+            for (int k = sta[vb - 1]; k < sta[vb] && !found; k++) {
+                int icc = icvert[k];
+                if (icc == ic) continue;
+                for (int ffa = 0; ffa < 4; ffa++) {
+                    int w1 = IVC(icc, IVF[ffa][0]), w2 = IVC(icc, IVF[ffa][1]);
+                    if ((vb == w1 && v1 == w2) || (vb == w2 && v1 == w1)) {
+                        nface++;
+                        ic2f[icc * 4 + ffa] = nface;
+                        f2c.push_back(ic + 1); f2c.push_back(icc + 1);
+                        f2v.push_back(vb); f2v.push_back(v1);
+                        found = true;
+                        break;
+                    }
+                }
+            }
+            if (!found) {
+                nface++;
+                f2c.push_back(ic + 1); f2c.push_back(0);
+                f2v.push_back(vb); f2v.push_back(v1);
+            }
+            ic2f[ic * 4 + fa] = nface;
+        }
+    t.nface = nface;
+    t.if2c.assign((size_t)nface * 4, 0);
+    t.if2v.assign((size_t)nface * 2, 0);
+    for (int f = 0; f < nface; f++) {
+        t.if2c[f] = f2c[2 * f]; t.if2c[nface + f] = f2c[2 * f + 1];
+        t.if2v[f] = f2v[2 * f]; t.if2v[nface + f] = f2v[2 * f + 1];
+        int c1 = f2c[2 * f] - 1, c2 = f2c[2 * f + 1] - 1, v1 = f2v[2 * f], v2 = f2v[2 * f + 1];
+        int a = IVC(c1, 0), b = IVC(c1, 1), cc = IVC(c1, 2), d = IVC(c1, 3), lf = 0;
+        if (a == v1 && b == v2) lf = 1;
+        else if (b == v1 && cc == v2) lf = 2;
+        else if (cc == v1 && d == v2) lf = 3;
+        else if (d == v1 && a == v2) lf = 4;
+        if (!lf) { err = "connectivity: face " + std::to_string(f + 1) + " not on its left cell"; return CHORUS_ERR_TOPOLOGY; }
+        t.if2c[2 * nface + f] = lf;
+        if (c2 >= 0) {
+            a = IVC(c2, 0); b = IVC(c2, 1); cc = IVC(c2, 2); d = IVC(c2, 3);
+            int rf = 0;
+            if (b == v1 && a == v2) rf = 1;
+            else if (cc == v1 && b == v2) rf = 2;
+            else if (d == v1 && cc == v2) rf = 3;
+            else if (a == v1 && d == v2) rf = 4;
+            t.if2c[3 * nface + f] = rf;
+        }
+    }
+    std::vector<int> cycl;
+    for (int ib = 0; ib < nbnd; ib++) {
+        int b1 = bnd[ib], b3 = bnd[nbnd + ib], icc = -1;
+        if (b1 < 1 || b1 > nvert || b3 < 1 || b3 > nvert) continue;
+        for (int k1 = sta[b1 - 1]; k1 < sta[b1]; k1++)
+            for (int k3 = sta[b3 - 1]; k3 < sta[b3]; k3++)
+                if (icvert[k1] == icvert[k3]) icc = icvert[k1];
+        if (icc < 0) continue;
+        int fa;
+        for (fa = 0; fa < 4; fa++) {
+            int v1 = IVC(icc, IVF[fa][0]), v2 = IVC(icc, IVF[fa][1]);
+            if ((v1 == b1 || v2 == b1) && (v1 == b3 || v2 == b3)) break;
+        }
+        if (fa == 4) continue;
+        int k = ic2f[icc * 4 + fa];
+        if (btype[ib] == 1) cycl.push_back(k);
+        else if (btype[ib] == 2) t.ibfree.push_back(k);
+    }
+    for (int f = 0; f < nface; f++)
+        if (t.if2c[nface + f]) t.ibfinter.push_back(f + 1);
+    auto mid = [&](int K, double &x, double &y) {
+        x = (xv[t.if2v[K] - 1] + xv[t.if2v[nface + K] - 1]) / 2;
+        y = (yv[t.if2v[K] - 1] + yv[t.if2v[nface + K] - 1]) / 2;
+    };
+    std::vector<int> cycloc;
+    for (size_t ib = 0; ib < cycl.size(); ib++) {
+        double xi, yi;
+        mid(cycl[ib] - 1, xi, yi);
+        bool hit = false;
+        for (size_t ib2 = 0; ib2 < cycl.size() && !hit; ib2++) {
+            double xo, yo;
+            mid(cycl[ib2] - 1, xo, yo);
+            double ex = std::fabs(std::fabs(xi - xo) - dxcycl), ey = std::fabs(std::fabs(yi - yo) - dycycl);
+            if ((std::fabs(xi - xo) < tol && ey < tol) || (std::fabs(yi - yo) < tol && ex < tol)) hit = true;
+        }
+        if (!hit) { err = "periodic face without a partner (remote periodic faces need more than one rank)"; return CHORUS_ERR_TOPOLOGY; }
+        cycloc.push_back(cycl[ib]);
+    }
+    std::vector<int> p1, p2;
+    for (size_t nf = 0; nf < cycloc.size(); nf++) {
+        double xi, yi;
+        mid(cycloc[nf] - 1, xi, yi);
+        for (size_t nf2 = nf + 1; nf2 < cycloc.size(); nf2++) {
+            double xo, yo;
+            mid(cycloc[nf2] - 1, xo, yo);
+            double ex = std::fabs(std::fabs(xi - xo) - dxcycl), ey = std::fabs(std::fabs(yi - yo) - dycycl);
+            if ((ex < tol && std::fabs(yi - yo) < tol) || (ey < tol && std::fabs(xi - xo) < tol)) {
+                p1.push_back(cycloc[nf]);
+                p2.push_back(cycloc[nf2]);
+            }
+        }
+    }
+    if (p1.size() * 2 != cycloc.size()) { err = "error in matching CYCLOC faces (map_cyc_loc.f90:37-40)"; return CHORUS_ERR_TOPOLOGY; }
+    t.npair = (int)p1.size();
+    t.cyc = p1;
+    t.cyc.insert(t.cyc.end(), p2.begin(), p2.end());
+    return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+const char *chorus_last_error(const chorus_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int chorus_topology(int ncell, int nvert, const int *ivcell, const double *xv, const double *yv, int nbnd,
+                    const int *bnd, const int *btype, double dxcycl, double dycycl, int *nface, int *if2c,
+                    int *ninter, int *ibfinter, int *npair, int *cyc2fpair, int *nfree, int *ibfree) {
+    if (ncell <= 0 || nvert <= 0 || !ivcell || !xv || !yv || (nbnd > 0 && (!bnd || !btype)))
+        return CHORUS_ERR_ARG;
+    Topo t;
+    std::string err;
+    int rc = build_topology(ncell, nvert, ivcell, xv, yv, nbnd, bnd, btype, dxcycl, dycycl, t, err);
+    if (rc) {
+        std::fprintf(stderr, "chorus_topology: %s\n", err.c_str());
+        return rc;
+    }
+    *nface = t.nface;
+    *ninter = (int)t.ibfinter.size();
+    *npair = t.npair;
+    *nfree = (int)t.ibfree.size();
+    std::copy(t.if2c.begin(), t.if2c.end(), if2c);
+    std::copy(t.ibfinter.begin(), t.ibfinter.end(), ibfinter);
+    std::copy(t.cyc.begin(), t.cyc.end(), cyc2fpair);
+    std::copy(t.ibfree.begin(), t.ibfree.end(), ibfree);
+    return 0;
+}
+
+int chorus_init(const chorus_mesh *m, const chorus_params *p, chorus_ctx **out) {
+    if (!out) return CHORUS_ERR_ARG;
+    *out = nullptr;
+    chorus_ctx *c = new chorus_ctx();
+    *out = c;
+    if (!m || !p) return fail(c, CHORUS_ERR_ARG, "null mesh or params");
+    if (m->N < 2 || m->N > kMaxN) return fail(c, CHORUS_ERR_ARG, "N must be in 2..8");
+    if (m->ncell <= 0) return fail(c, CHORUS_ERR_ARG, "empty mesh");
+    if (p->vismode != 1)
+        return fail(c, CHORUS_ERR_ARG, "vismode must be 1: with vismode=0 the reference reads uninitialised "
+                                      "flux-point A_z (compflux.f90:72-75, magnetic_potential.f90:17)");
+    if (p->k_stage != 3 && p->k_stage != 5)
+        return fail(c, CHORUS_ERR_ARG, "k_stage can be only 3 or 5 (iterations.f90:22-31)");
+    if (p->precision != CHORUS_FP64 && p->precision != CHORUS_MIXED)
+        return fail(c, CHORUS_ERR_ARG, "precision must be CHORUS_FP64 or CHORUS_MIXED");
+    if (p->artiv != 0 || p->mu != 0.0 || p->eta0 != 0.0)
+        return fail(c, CHORUS_ERR_ARG, "viscous/AV path not available in this build");
+    c->p = *p;
+    c->N = m->N;
+    c->ncell = m->ncell;
+    c->ncellglob = p->ncellglob > 0 ? p->ncellglob : m->ncell;
+
+    // on-device face connectivity: per cell and local face, the partner cell,
+    // its local face and whether this cell is the reference's left or right cell
+    std::vector<int2> nbr((size_t)m->ncell * 4, make_int2(-2, 0));
+    auto IF = [&](int f, int k) { return m->if2c[(size_t)k * m->nface + f]; };
+    auto link = [&](int l, int lfl, int r, int lfr) -> bool {
+        if (l < 0 || l >= m->ncell || r < 0 || r >= m->ncell || lfl < 1 || lfl > 4 || lfr < 1 || lfr > 4) return false;
+        nbr[(size_t)l * 4 + lfl - 1] = make_int2(r, lfr);
+        nbr[(size_t)r * 4 + lfr - 1] = make_int2(l, lfl | (1 << 3));
+        return true;
+    };
+    for (int k = 0; k < m->ninter; k++) {
+        int f = m->ibfinter[k] - 1;
+        if (f < 0 || f >= m->nface || !link(IF(f, 0) - 1, IF(f, 2), IF(f, 1) - 1, IF(f, 3)))
+            return fail(c, CHORUS_ERR_TOPOLOGY, "bad interior face " + std::to_string(k + 1));
+    }
+    for (int k = 0; k < m->npair; k++) {
+        int f1 = m->cyc2fpair[k] - 1, f2 = m->cyc2fpair[m->npair + k] - 1;
+        if (f1 < 0 || f1 >= m->nface || f2 < 0 || f2 >= m->nface || !link(IF(f1, 0) - 1, IF(f1, 2), IF(f2, 0) - 1, IF(f2, 2)))
+            return fail(c, CHORUS_ERR_TOPOLOGY, "bad periodic pair " + std::to_string(k + 1));
+    }
+    for (int k = 0; k < m->nfree; k++) {
+        int f = m->ibfree[k] - 1;
+        if (f < 0 || f >= m->nface) return fail(c, CHORUS_ERR_TOPOLOGY, "bad FREE face");
+        int l = IF(f, 0) - 1, lf = IF(f, 2);
+        nbr[(size_t)l * 4 + lf - 1] = make_int2(-1, 1 << 4);
+    }
+    for (size_t i = 0; i < nbr.size(); i++)
+        if (nbr[i].x == -2)
+            return fail(c, CHORUS_ERR_TOPOLOGY,
+                        "cell " + std::to_string(i / 4 + 1) + " face " + std::to_string(i % 4 + 1) +
+                            " has no flux: only interior, CYCL and FREE faces carry fluxes (connect_bdry.f90:88-110)");
+    std::vector<double> xy((size_t)m->ncell * 8);
+    for (int ic = 0; ic < m->ncell; ic++)
+        for (int k = 0; k < 4; k++) {
+            int v = m->ivcell[(size_t)k * m->ncell + ic];
+            if (v < 1 || v > m->nvert) return fail(c, CHORUS_ERR_ARG, "IVCELL vertex out of range");
+            xy[(size_t)ic * 8 + k] = m->xv[v - 1];
+            xy[(size_t)ic * 8 + 4 + k] = m->yv[v - 1];
+        }
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(c, CHORUS_ERR_HIP, "no HIP device: this library has no CPU path");
+    if (!g_ops_uploaded) {
+        HIP_TRY(c, upload_operators_fp64());
+        HIP_TRY(c, upload_operators_mixed());
+        g_ops_uploaded = true;
+    }
+    HIP_TRY(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    const size_t ns = state_len(c);
+    HIP_TRY(c, hipMalloc(&c->cellxy, xy.size() * sizeof(double)));
+    HIP_TRY(c, hipMalloc(&c->nbr, nbr.size() * sizeof(int2)));
+    for (int i = 0; i < 4; i++) HIP_TRY(c, hipMalloc(&c->X[i], ns * sizeof(double)));
+    HIP_TRY(c, hipMalloc(&c->R, ns * sizeof(double)));
+    c->nblk = stage_blocks(c->N, c->ncell);
+    HIP_TRY(c, hipMalloc(&c->partial, (size_t)c->nblk * 2 * sizeof(double)));
+    HIP_TRY(c, hipMemcpyAsync(c->cellxy, xy.data(), xy.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->nbr, nbr.data(), nbr.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+void chorus_free(chorus_ctx *c) {
+    if (!c) return;
+    for (auto e : c->ev) (void)hipEventDestroy(e);
+    (void)hipFree(c->cellxy);
+    (void)hipFree(c->nbr);
+    for (int i = 0; i < 4; i++) (void)hipFree(c->X[i]);
+    (void)hipFree(c->R);
+    (void)hipFree(c->partial);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int chorus_set_state(chorus_ctx *c, const double *q) {
+    if (!c || !q) return fail(c, CHORUS_ERR_ARG, "null argument");
+    std::vector<double> h;
+    to_device_layout(c, q, h);
+    HIP_TRY(c, hipMemcpyAsync(c->X[c->cur], h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->last_in = -1;
+    return 0;
+}
+
+int chorus_get_state(chorus_ctx *c, double *q) {
+    if (!c || !q) return fail(c, CHORUS_ERR_ARG, "null argument");
+    std::vector<double> h(state_len(c));
+    HIP_TRY(c, hipMemcpyAsync(h.data(), c->X[c->cur], h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    from_device_layout(c, h, q);
+    return 0;
+}
+
+int chorus_compresid(chorus_ctx *c, double *resid) {
+    if (!c) return CHORUS_ERR_ARG;
+    StageArgs s = base_args(c);
+    s.Qin = c->X[c->cur];
+    s.Rout = c->R;
+    s.kind = KIND_RESID;
+    int rc = launch_stage(c, s);
+    if (rc) return rc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (resid) {
+        std::vector<double> h(state_len(c));
+        HIP_TRY(c, hipMemcpy(h.data(), c->R, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+        from_device_layout(c, h, resid);
+    }
+    return 0;
+}
+
+int chorus_correct_b(chorus_ctx *c) {
+    if (!c) return CHORUS_ERR_ARG;
+    int rc = run_ct(c, c->X[c->cur], 0);
+    if (rc) return rc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int chorus_step(chorus_ctx *c, int nsteps) {
+    if (!c || nsteps < 0) return fail(c, CHORUS_ERR_ARG, "bad arguments");
+    for (int i = 0; i < nsteps; i++) {
+        int rc = one_step(c);
+        if (rc) return rc;
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int chorus_diag(chorus_ctx *c, double *residglob, double *divbglob, double *divbmax, double *eigvmax,
+                double *epsilon0) {
+    if (!c) return CHORUS_ERR_ARG;
+    const int N = c->N;
+    const double npts = (double)c->ncellglob * N * N;
+    if (residglob) {
+        // the reference's norm is over the residual of the last stage evaluated
+        // (iterations.f90:42-45): re-evaluate it from that stage's input state
+        *residglob = NAN;
+        if (c->last_in >= 0) {
+            StageArgs s = base_args(c);
+            s.Qin = c->X[c->last_in];
+            s.Rout = c->R;
+            s.kind = KIND_RESID;
+            int rc = launch_stage(c, s);
+            if (rc) return rc;
+            std::vector<double> h(state_len(c));
+            HIP_TRY(c, hipMemcpyAsync(h.data(), c->R, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+            // cell, point, variable order of the reference's loop
+            const int NP = N * N;
+            double s2 = 0.0;
+            for (int ic = 0; ic < c->ncell; ic++)
+                for (int p = 0; p < NP; p++)
+                    for (int v = 0; v < 9; v++) {
+                        double r = h[((size_t)ic * 9 + v) * NP + p];
+                        s2 = s2 + r * r;
+                    }
+            *residglob = std::sqrt(s2 / (npts * 5));
+        }
+    }
+    if (divbglob || divbmax) {
+        int rc = run_ct(c, c->X[c->cur], 1);
+        if (rc) return rc;
+        std::vector<double> part((size_t)c->nblk * 2);
+        HIP_TRY(c, hipMemcpyAsync(part.data(), c->partial, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        double s2 = 0.0, mx = 0.0;
+        for (int b = 0; b < c->nblk; b++) { s2 += part[2 * b]; mx = std::max(mx, part[2 * b + 1]); }
+        if (divbglob) *divbglob = std::sqrt(s2 / npts);
+        if (divbmax) *divbmax = mx;
+    }
+    if (eigvmax) *eigvmax = NAN;
+    if (epsilon0) *epsilon0 = NAN;
+    return 0;
+}
+
+int chorus_time_steps(chorus_ctx *c, int nsteps, double *ms_total, double *ms_stage_avg, double *ms_ct_avg) {
+    if (!c || nsteps < 1) return fail(c, CHORUS_ERR_ARG, "bad arguments");
+    const size_t need = (size_t)nsteps * 12 + 2;
+    while (c->ev.size() < need) {
+        hipEvent_t e;
+        HIP_TRY(c, hipEventCreate(&e));
+        c->ev.push_back(e);
+    }
+    hipEvent_t t0 = c->ev[need - 2], t1 = c->ev[need - 1];
+    c->ev_used = 0;
+    c->timing = true;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipEventRecord(t0, c->stream));
+    int rc = 0;
+    for (int i = 0; i < nsteps && !rc; i++) rc = one_step(c);
+    c->timing = false;
+    if (rc) return rc;
+    HIP_TRY(c, hipEventRecord(t1, c->stream));
+    HIP_TRY(c, hipEventSynchronize(t1));
+    float ms = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&ms, t0, t1));
+    if (ms_total) *ms_total = ms;
+    // per step: 5 stage launches then 1 CT launch, each bracketed by an event pair
+    double st = 0.0, ct = 0.0;
+    int nst = 0, nct = 0;
+    for (size_t i = 0; i + 1 < c->ev_used; i += 2) {
+        float d = 0.f;
+        HIP_TRY(c, hipEventElapsedTime(&d, c->ev[i], c->ev[i + 1]));
+        if ((i / 2) % 6 == 5) { ct += d; nct++; } else { st += d; nst++; }
+    }
+    if (ms_stage_avg) *ms_stage_avg = nst ? st / nst : 0.0;
+    if (ms_ct_avg) *ms_ct_avg = nct ? ct / nct : 0.0;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,53 @@
+// FP64 drop-in kernels.  Built with -ffp-contract=off: no FMA is formed, so each
+// product and sum rounds separately, exactly like the reference build.
+#include "kernels.hpp"
+
+namespace chorus {
+
+hipError_t upload_operators_fp64() { return upload_tables(); }
+
+template <int N> static int nblocks(int ncell) { return (ncell + Shape<N>::CPB - 1) / Shape<N>::CPB; }
+
+#define CHORUS_DISPATCH_N(N, CALL) \
+    switch (N) {                   \
+    case 2: CALL(2);               \
+    case 3: CALL(3);               \
+    case 4: CALL(4);               \
+    case 5: CALL(5);               \
+    case 6: CALL(6);               \
+    case 7: CALL(7);               \
+    case 8: CALL(8);               \
+    default: return hipErrorInvalidValue; \
+    }
+
+hipError_t launch_stage_ideal_fp64(int N, const StageArgs &a, hipStream_t s) {
+#define L_(n) \
+    hipLaunchKernelGGL((k_stage_ideal<n, double>), dim3(nblocks<n>(a.ncell)), dim3(Shape<n>::THREADS), 0, s, a); \
+    return hipGetLastError();
+    CHORUS_DISPATCH_N(N, L_)
+#undef L_
+}
+
+hipError_t launch_ct(int N, const CTArgs &a, hipStream_t s) {
+#define L_(n) \
+    hipLaunchKernelGGL((k_ct<n>), dim3(nblocks<n>(a.ncell)), dim3(Shape<n>::THREADS), 0, s, a); \
+    return hipGetLastError();
+    CHORUS_DISPATCH_N(N, L_)
+#undef L_
+}
+
+int stage_blocks(int N, int ncell) {
+    switch (N) {
+    case 2: return nblocks<2>(ncell);
+    case 3: return nblocks<3>(ncell);
+    case 4: return nblocks<4>(ncell);
+    case 5: return nblocks<5>(ncell);
+    case 6: return nblocks<6>(ncell);
+    case 7: return nblocks<7>(ncell);
+    case 8: return nblocks<8>(ncell);
+    default: return 0;
+    }
+}
+int ct_blocks(int N, int ncell) { return stage_blocks(N, ncell); }
+
+}  // namespace chorus

@@ -1,0 +1,81 @@
+// Internal interfaces shared by the kernel translation units and the C-ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+
+namespace chorus {
+
+constexpr int kMaxN = 8;       // solution points per direction supported by the kernels
+
+// stage combination kinds (FP64 in both precision forms; the association of
+// each expression is the reference's, iterations.f90:92-217)
+enum StageKind : int {
+    KIND_RESID = 0,   // write the residual only (compresid)
+    KIND_AXPY = 1,    // Q + c0*R                           c0 = b*dt   (RK_S5P3 s1,s2; FiveStageRk s1)
+    KIND_3A = 2,      // (c1*Qa + c2*Q) + c0*R              c0 = b*dt   (RK_S5P3 s3..s5; FiveStageRk s3)
+    KIND_3B = 3,      // (c1*Qa + c2*Q) + (c3*R)*c4         c4 = dt     (FiveStageRk s2)
+    KIND_S54_4 = 4,   // KIND_3A and Rout = R                           (FiveStageRk s4, keeps resid0)
+    KIND_S54_5 = 5,   // ((((c1*Qa + c2*Qb) + c3*Qc) + c4*Q) + c5*Rprev) + c0*R  (FiveStageRk s5)
+};
+
+struct StageArgs {
+    int ncell;
+    const double *cellxy;
+    const int2 *nbr;
+    const double *Qin;
+    double *Qout;
+    const double *Qa, *Qb, *Qc, *Rprev;
+    double *Rout;
+    int kind;
+    double c0, c1, c2, c3, c4, c5;
+    double gam;
+};
+
+struct CTArgs {
+    int ncell;
+    const double *cellxy;
+    const int2 *nbr;
+    double *Q;
+    int which;          // 0: correct B from curl A_z; 1: div B diagnostic
+    double *partial;    // which=1: per block {sum divB^2, max |divB|}
+};
+
+template <class R>
+__device__ __forceinline__ void stage_combine(const StageArgs &a, size_t base, int NP, const double *qs,
+                                              const R *res) {
+#pragma unroll
+    for (int v = 0; v < 9; v++) {
+        const double q = qs[v * NP];
+        const double r = double(res[v]);
+        const size_t o = base + (size_t)v * NP;
+        double qn;
+        switch (a.kind) {
+        case KIND_RESID: a.Rout[o] = r; continue;
+        case KIND_AXPY: qn = q + a.c0 * r; break;
+        case KIND_3A: qn = a.c1 * a.Qa[o] + a.c2 * q + a.c0 * r; break;
+        case KIND_3B: qn = a.c1 * a.Qa[o] + a.c2 * q + a.c3 * r * a.c4; break;
+        case KIND_S54_4: qn = a.c1 * a.Qa[o] + a.c2 * q + a.c0 * r; a.Rout[o] = r; break;
+        default: qn = a.c1 * a.Qa[o] + a.c2 * a.Qb[o] + a.c3 * a.Qc[o] + a.c4 * q + a.c5 * a.Rprev[o] + a.c0 * r; break;
+        }
+        a.Qout[o] = qn;
+    }
+}
+
+// launchers (defined in the kernel translation units); return hipError_t
+hipError_t launch_stage_ideal_fp64(int N, const StageArgs &a, hipStream_t s);
+hipError_t launch_stage_ideal_mixed(int N, const StageArgs &a, hipStream_t s);
+hipError_t launch_ct(int N, const CTArgs &a, hipStream_t s);
+int ct_blocks(int N, int ncell);
+int stage_blocks(int N, int ncell);
+hipError_t upload_operators_fp64();
+hipError_t upload_operators_mixed();
+
+// Xs, Xf, Lmat, Mmat of the reference for N points (setsandpoints,
+// computeLandM, initsetup.f90:167-315), computed on the host in the
+// reference's operation order; L(ifp,is) at [ifp*N+is], M(r,is) at [r*N+is].
+void host_operators(int N, double *Xs, double *Xf, double *L, double *M);
+// Xs for N points (used for N-1 by the AV sensor, initsetup.f90:44-45)
+void host_solution_points(int N, double *Xs);
+
+}  // namespace chorus

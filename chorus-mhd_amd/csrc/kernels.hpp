@@ -1,0 +1,441 @@
+// Stage, constrained-transport and diagnostic kernels for gfx950.
+//
+// Included by two translation units: chorus_fp64.hip (Real = double, built
+// with -ffp-contract=off so every operation rounds as in the reference) and
+// chorus_mixed.hip (Real = float for the flux/contraction arithmetic).
+//
+// Data layout in HBM (one cell = one contiguous block):
+//   Q[cell][var][js][is]       FP64 state, 9*N*N doubles per cell
+//   cellxy[cell][8]            vertex coordinates x1..x4, y1..y4 (IVCELL order)
+//   nbr[cell][lf] (int2)       .x neighbour cell across local face lf (-1: FREE)
+//                              .y bits 0-2 neighbour's local face (1..4),
+//                                 bit 3 this cell is the reference's RIGHT cell,
+//                                 bit 4 FREE boundary face
+// A workgroup owns CPB whole cells: it stages their Q in LDS with contiguous
+// loads, evaluates every flux point of its cells (face points read the
+// neighbour's trace through L2), then every solution point.
+#pragma once
+
+#include "physics.hpp"
+#include "chorus_internal.hpp"
+
+namespace chorus {
+
+// operator tables per N, one private copy per translation unit (no
+// relocatable device code); filled by upload_tables() from host_operators()
+static __constant__ double c_L[kMaxN + 1][(kMaxN + 1) * kMaxN];   // L(ifp,is) at [ifp*N+is]
+static __constant__ double c_M[kMaxN + 1][(kMaxN + 1) * kMaxN];   // M(r,is) at [r*N+is]
+static __constant__ double c_Xs[kMaxN + 1][kMaxN];
+static __constant__ double c_Xf[kMaxN + 1][kMaxN + 1];
+static __constant__ float c_Lf[kMaxN + 1][(kMaxN + 1) * kMaxN];
+static __constant__ float c_Mf[kMaxN + 1][(kMaxN + 1) * kMaxN];
+
+static hipError_t upload_tables() {
+    static double L[kMaxN + 1][(kMaxN + 1) * kMaxN], M[kMaxN + 1][(kMaxN + 1) * kMaxN];
+    static double Xs[kMaxN + 1][kMaxN], Xf[kMaxN + 1][kMaxN + 1];
+    static float Lf[kMaxN + 1][(kMaxN + 1) * kMaxN], Mf[kMaxN + 1][(kMaxN + 1) * kMaxN];
+    for (int n = 2; n <= kMaxN; n++) {
+        host_operators(n, Xs[n], Xf[n], L[n], M[n]);
+        for (int i = 0; i < (n + 1) * n; i++) { Lf[n][i] = float(L[n][i]); Mf[n][i] = float(M[n][i]); }
+    }
+    hipError_t e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_L), L, sizeof(L))) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_M), M, sizeof(M))) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_Xs), Xs, sizeof(Xs))) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_Xf), Xf, sizeof(Xf))) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_Lf), Lf, sizeof(Lf))) != hipSuccess) return e;
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_Mf), Mf, sizeof(Mf));
+}
+
+template <int N> struct Shape {
+    static constexpr int NP = N * N;          // solution points per cell
+    static constexpr int NF = N * (N + 1);    // flux points per family
+    static constexpr int NQ = NV * NP;        // doubles of state per cell
+    static constexpr int CPB = (256 / NP) > 0 ? (256 / NP) : 1;   // cells per block
+    static constexpr int THREADS = ((CPB * NP + 63) / 64) * 64;
+};
+
+// XCD-aware block order: the hardware deals consecutive workgroups round-robin
+// over the 8 XCDs; give each XCD a contiguous run of cell blocks so the
+// neighbour traces a block reads were staged by a block on the same L2.
+__device__ __forceinline__ int xcd_block(int b, int nblk) {
+    int q = nblk >> 3, r = nblk & 7;
+    int x = b & 7, k = b >> 3;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+// local face -> flux-point index of face point nfp (face2fpconnec, initsetup.f90:343-379)
+template <int N> __device__ __forceinline__ void face2fp(int lf, int nfp, int &ifp, int &jfp) {
+    switch (lf) {
+    case 1: ifp = nfp; jfp = 0; break;
+    case 2: ifp = N; jfp = nfp; break;
+    case 3: ifp = N - 1 - nfp; jfp = N; break;
+    default: ifp = 0; jfp = N - 1 - nfp; break;
+    }
+}
+
+// face family of a local face (interfaceflux_all.f90:26-27): 0 = xi (F1), 1 = eta (G2)
+__device__ __forceinline__ int lf_family(int lf) { return (lf & 1) ? 1 : 0; }
+
+// contravariant metric row used by the flux at a point of family fam
+// (calcjacob.f90:139-170): fam 0 -> (S1(1,1), S1(1,2)) = (yeta, -xeta) at
+// (Xf(ifp), Xs(jfp)); fam 1 -> (S2(2,1), S2(2,2)) = (-ypsi, xpsi) at (Xs(ifp), Xf(jfp))
+template <int N>
+__device__ __forceinline__ void metric_row(const double *xy, int fam, int ifp, int jfp, double &s1,
+                                           double &s2) {
+    double xpsi, ypsi, xeta, yeta;
+    if (fam == 0) {
+        map_derivs(xy, xy + 4, c_Xf[N][ifp], c_Xs[N][jfp], xpsi, ypsi, xeta, yeta);
+        s1 = yeta; s2 = -xeta;
+    } else {
+        map_derivs(xy, xy + 4, c_Xs[N][ifp], c_Xf[N][jfp], xpsi, ypsi, xeta, yeta);
+        s1 = -ypsi; s2 = xpsi;
+    }
+}
+
+template <int N> __device__ __forceinline__ double jacobian(const double *xy, int is, int js) {
+    double xpsi, ypsi, xeta, yeta;
+    map_derivs(xy, xy + 4, c_Xs[N][is], c_Xs[N][js], xpsi, ypsi, xeta, yeta);
+    return xpsi * yeta - xeta * ypsi;
+}
+
+template <class R> __device__ __forceinline__ R Lop(int N, int ifp, int is);
+template <> __device__ __forceinline__ double Lop<double>(int N, int ifp, int is) { return c_L[N][ifp * N + is]; }
+template <> __device__ __forceinline__ float Lop<float>(int N, int ifp, int is) { return c_Lf[N][ifp * N + is]; }
+template <class R> __device__ __forceinline__ R Mop(int N, int r, int is);
+template <> __device__ __forceinline__ double Mop<double>(int N, int r, int is) { return c_M[N][r * N + is]; }
+template <> __device__ __forceinline__ float Mop<float>(int N, int r, int is) { return c_Mf[N][r * N + is]; }
+
+// trace of all 9 variables at flux point (ifp,jfp) of family fam of one cell
+// whose state block q[var][js][is] is given (inviscid_flux_interior,
+// compflux.f90:59-94 and the trace loops of interfaceflux_all.f90:40-73)
+template <int N, class R>
+__device__ __forceinline__ void trace9(const double *q, int fam, int ifp, int jfp, R *Qf) {
+    constexpr int NP = N * N;
+#pragma unroll
+    for (int v = 0; v < NV; v++) Qf[v] = R(0);
+    if (fam == 0) {
+#pragma unroll
+        for (int sp = 0; sp < N; sp++) {
+            R l = Lop<R>(N, ifp, sp);
+#pragma unroll
+            for (int v = 0; v < NV; v++) Qf[v] = Qf[v] + R(q[v * NP + jfp * N + sp]) * l;
+        }
+    } else {
+#pragma unroll
+        for (int sp = 0; sp < N; sp++) {
+            R l = Lop<R>(N, jfp, sp);
+#pragma unroll
+            for (int v = 0; v < NV; v++) Qf[v] = Qf[v] + R(q[v * NP + sp * N + ifp]) * l;
+        }
+    }
+}
+
+template <int N>
+__device__ __forceinline__ double trace1(const double *q1, int fam, int ifp, int jfp) {
+    double a = 0.0;
+    if (fam == 0) {
+#pragma unroll
+        for (int sp = 0; sp < N; sp++) a = a + q1[jfp * N + sp] * c_L[N][ifp * N + sp];
+    } else {
+#pragma unroll
+        for (int sp = 0; sp < N; sp++) a = a + q1[sp * N + ifp] * c_L[N][jfp * N + sp];
+    }
+    return a;
+}
+
+// Common face-point resolution: which cell is the reference's left cell of
+// this face and where the partner point lies (interfaceflux_all.f90:16-37,
+// cyclocflux_all.f90:15-77, bcflux.f90:19-53).
+struct FacePoint {
+    int nb;          // partner cell (-1 for FREE)
+    int nb_fam, nb_ifp, nb_jfp;
+    bool right;      // this cell is the right cell
+    bool free_face;
+    int sign_l, sign_r;
+};
+
+template <int N>
+__device__ __forceinline__ FacePoint resolve_face(int2 info, int fam, int r, int line) {
+    FacePoint fp;
+    int lf, nfp_my;
+    if (fam == 0) { lf = (r == 0) ? 4 : 2; nfp_my = (r == 0) ? (N - 1 - line) : line; }
+    else          { lf = (r == 0) ? 1 : 3; nfp_my = (r == 0) ? line : (N - 1 - line); }
+    (void)lf;
+    fp.free_face = (info.y >> 4) & 1;
+    fp.right = (info.y >> 3) & 1;
+    fp.nb = info.x;
+    int own_sign_left = (r == 0) ? -1 : 1;     // sign_l if this point is the left point
+    int own_sign_right = (r == N) ? -1 : 1;    // sign_r if this point is the right point
+    if (fp.free_face) {
+        fp.nb_fam = fam; fp.nb_ifp = 0; fp.nb_jfp = 0;
+        fp.sign_l = own_sign_left; fp.sign_r = 1;
+        return fp;
+    }
+    int lf_nb = info.y & 7;
+    fp.nb_fam = lf_family(lf_nb);
+    int nfp_nb = N - 1 - nfp_my;               // partner index runs the other way
+    face2fp<N>(lf_nb, nfp_nb, fp.nb_ifp, fp.nb_jfp);
+    int nbr = fp.nb_fam == 0 ? fp.nb_ifp : fp.nb_jfp;
+    if (!fp.right) { fp.sign_l = own_sign_left; fp.sign_r = (nbr == N) ? -1 : 1; }
+    else           { fp.sign_l = (nbr == 0) ? -1 : 1; fp.sign_r = own_sign_right; }
+    return fp;
+}
+
+// ---------------------------------------------------------------------------
+// Stage kernel of the ideal path (vismode=1, mu=eta0=0, ARTIV=0): compresid
+// (compresid.f90:1-67 via compflux.f90:1-104, interfaceflux_all.f90:1-116,
+// cyclocflux_all.f90:1-116, bcflux.f90:1-70, magnetic_potential.f90:1-42)
+// fused with one SSP stage combination (iterations.f90:92-217).  With
+// mu=eta0=0 and no AV the viscous terms of compresid are exact zeros, so they
+// are not evaluated.
+// ---------------------------------------------------------------------------
+template <int N, class R>
+__global__ void __launch_bounds__(256) k_stage_ideal(StageArgs a) {
+    using S = Shape<N>;
+    constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
+    __shared__ double sQ[CPB * NQ];
+    __shared__ double sX[CPB * 8];
+    __shared__ R sF[CPB * 8 * NF];       // [c][v][line][r], one family at a time
+    __shared__ R sA[CPB * NF];           // [c][line][r]
+    __shared__ R sS[CPB * NF * 2];       // [c][line][r][2]
+    const int tid = threadIdx.x;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int c0 = blk * CPB;
+    const int ncl = min(CPB, a.ncell - c0);
+
+    // A: stage the block's cells (contiguous) and their vertex coordinates
+    {
+        const double *src = a.Qin + (size_t)c0 * NQ;
+        for (int i = tid; i < ncl * NQ; i += blockDim.x) sQ[i] = src[i];
+        const double *xs = a.cellxy + (size_t)c0 * 8;
+        for (int i = tid; i < ncl * 8; i += blockDim.x) sX[i] = xs[i];
+    }
+    __syncthreads();
+
+    // thread <-> solution point for the C phases
+    const int pc = tid / NP, pk = tid % NP;
+    const int pjs = pk / N, pis = pk % N;
+    const bool pact = (tid < CPB * NP) && (pc < ncl);
+    R dF[8], dA1 = R(0), dA2 = R(0);
+
+#pragma unroll 1
+    for (int fam = 0; fam < 2; fam++) {
+        // B: flux points of this family
+        for (int it = tid; it < ncl * NF; it += blockDim.x) {
+            const int c = it / NF, k = it % NF, line = k / (N + 1), r = k % (N + 1);
+            const int cell = c0 + c;
+            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+            const double *xy = sX + c * 8;
+            double s1d, s2d;
+            metric_row<N>(xy, fam, ifp, jfp, s1d, s2d);
+            R Qown[NV];
+            trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown);
+            R out[8], az;
+            if (r > 0 && r < N) {
+                R F[8], G[8];
+                iflux<R>(R(a.gam), Qown, F, G);
+                R s1 = R(s1d), s2 = R(s2d);
+#pragma unroll
+                for (int v = 0; v < 8; v++) out[v] = F[v] * s1 + G[v] * s2;
+                az = Qown[8];
+            } else {
+                const FacePoint fp = resolve_face<N>(a.nbr[(size_t)cell * 4 + (fam == 0 ? (r == 0 ? 3 : 1) : (r == 0 ? 0 : 2))], fam, r, line);
+                R Qnb[NV];
+                R nx, ny;
+                if (fp.free_face) {
+#pragma unroll
+                    for (int v = 0; v < NV; v++) Qnb[v] = Qown[v];
+                    nx = R(s1d); ny = R(s2d);
+                } else {
+                    trace9<N, R>(a.Qin + (size_t)fp.nb * NQ, fp.nb_fam, fp.nb_ifp, fp.nb_jfp, Qnb);
+                    if (!fp.right) { nx = R(s1d); ny = R(s2d); }
+                    else {
+                        double xyn[8];
+#pragma unroll
+                        for (int q = 0; q < 8; q++) xyn[q] = a.cellxy[(size_t)fp.nb * 8 + q];
+                        double n1, n2;
+                        metric_row<N>(xyn, fp.nb_fam, fp.nb_ifp, fp.nb_jfp, n1, n2);
+                        nx = R(n1); ny = R(n2);
+                    }
+                }
+                R eig;
+                if (!fp.right) {
+                    rusanov<R>(R(a.gam), Qown, Qnb, nx, ny, fp.sign_l, out, az, eig);
+                } else {
+                    R Fnl[8];
+                    rusanov<R>(R(a.gam), Qnb, Qown, nx, ny, fp.sign_l, Fnl, az, eig);
+                    const R slr = R(fp.sign_l * fp.sign_r);
+#pragma unroll
+                    for (int v = 0; v < 8; v++) out[v] = slr * Fnl[v];
+                }
+            }
+            R *f = sF + c * 8 * NF + line * (N + 1) + r;
+#pragma unroll
+            for (int v = 0; v < 8; v++) f[v * NF] = out[v];
+            sA[c * NF + line * (N + 1) + r] = az;
+            sS[(c * NF + line * (N + 1) + r) * 2 + 0] = R(s1d);
+            sS[(c * NF + line * (N + 1) + r) * 2 + 1] = R(s2d);
+        }
+        __syncthreads();
+        // C: derivative along this family at the thread's solution point
+        if (pact) {
+            const int line = fam == 0 ? pjs : pis;    // the line through the point
+            const int pos = fam == 0 ? pis : pjs;     // position along the line
+            const R *f = sF + pc * 8 * NF + line * (N + 1);
+            const R *aa = sA + pc * NF + line * (N + 1);
+            const R *ss = sS + (pc * NF + line * (N + 1)) * 2;
+            R d[8];
+#pragma unroll
+            for (int v = 0; v < 8; v++) d[v] = R(0);
+            R e1 = R(0), e2 = R(0);
+#pragma unroll
+            for (int r = 0; r <= N; r++) {
+                R m = Mop<R>(N, r, pos);
+#pragma unroll
+                for (int v = 0; v < 8; v++) d[v] = d[v] + f[v * NF + r] * m;
+            }
+#pragma unroll
+            for (int r = 0; r <= N; r++) e1 = e1 + aa[r] * Mop<R>(N, r, pos) * ss[r * 2 + 0];
+#pragma unroll
+            for (int r = 0; r <= N; r++) e2 = e2 + aa[r] * Mop<R>(N, r, pos) * ss[r * 2 + 1];
+            if (fam == 0) {
+#pragma unroll
+                for (int v = 0; v < 8; v++) dF[v] = d[v];
+                dA1 = e1; dA2 = e2;
+            } else {
+                // residual at the solution point (compresid.f90:12-65)
+                const double Jd = jacobian<N>(sX + pc * 8, pis, pjs);
+                const R J = R(Jd);
+                const double *qs = sQ + pc * NQ + pk;
+                const R q0 = R(qs[0]), q1 = R(qs[NP]), q2 = R(qs[2 * NP]);
+                const R au = q1 / q0, av = q2 / q0;
+                const R nA1 = (dA1 + e1) / J, nA2 = (dA2 + e2) / J;
+                R res[NV];
+#pragma unroll
+                for (int v = 0; v < 8; v++) res[v] = -(dF[v] + d[v]);
+                res[8] = -(au * nA1) - av * nA2;
+                res[8] = res[8] * J;
+#pragma unroll
+                for (int v = 0; v < NV; v++) res[v] = res[v] / J;
+                const size_t base = (size_t)(c0 + pc) * NQ + pk;
+                stage_combine(a, base, NP, qs, res);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// correct_B_via_curl_Az (setinitialcond.f90:82-92): central A_z at every flux
+// point (interior: L extrapolation, compflux.f90:69-75/88-94; faces: average of
+// both traces, interfaceflux_all.f90:79-91), nablaQsp(9,9) (compflux.f90:107-157)
+// and update_B (magnetic_potential.f90:44-67).  FP64 in both precision forms.
+// In place: reads var 9 (also of neighbours), writes vars 6 and 7.
+// With which=1 it instead evaluates div B = d(Bx)/dx + d(By)/dy at the
+// solution points (iterations.f90:35-48) and writes per-block sum of squares
+// and max |div B| (no state change).
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(256) k_ct(CTArgs a) {
+    using S = Shape<N>;
+    constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
+    __shared__ double sV[2][CPB * NP];   // variable planes staged (A_z, or Bx and By)
+    __shared__ double sX[CPB * 8];
+    __shared__ double sF[2][CPB * NF];   // central flux-point values, one family at a time
+    __shared__ double sS[CPB * NF * 2];
+    __shared__ double red[2][256];
+    const int tid = threadIdx.x;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int c0 = blk * CPB;
+    const int ncl = min(CPB, a.ncell - c0);
+    const int nvar = a.which == 0 ? 1 : 2;
+    const int var0 = a.which == 0 ? 8 : 5;
+    for (int i = tid; i < ncl * NP; i += blockDim.x) {
+        int c = i / NP, k = i % NP;
+        for (int w = 0; w < nvar; w++) sV[w][i] = a.Q[(size_t)(c0 + c) * NQ + (var0 + w) * NP + k];
+    }
+    for (int i = tid; i < ncl * 8; i += blockDim.x) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
+    __syncthreads();
+    const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
+    const bool pact = (tid < CPB * NP) && (pc < ncl);
+    double dpsi[2][2] = {{0.0, 0.0}, {0.0, 0.0}};   // [w][component]
+    double out[2][2];
+#pragma unroll 1
+    for (int fam = 0; fam < 2; fam++) {
+        for (int it = tid; it < ncl * NF; it += blockDim.x) {
+            const int c = it / NF, k = it % NF, line = k / (N + 1), r = k % (N + 1);
+            const int cell = c0 + c;
+            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+            double s1, s2;
+            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1, s2);
+            for (int w = 0; w < nvar; w++) {
+                double own = trace1<N>(sV[w] + c * NP, fam, ifp, jfp);
+                double val = own;
+                if (r == 0 || r == N) {
+                    const int lfi = fam == 0 ? (r == 0 ? 3 : 1) : (r == 0 ? 0 : 2);
+                    const FacePoint fp = resolve_face<N>(a.nbr[(size_t)cell * 4 + lfi], fam, r, line);
+                    double other = own;
+                    if (!fp.free_face)
+                        other = trace1<N>(a.Q + (size_t)fp.nb * NQ + (var0 + w) * NP, fp.nb_fam,
+                                          fp.nb_ifp, fp.nb_jfp);
+                    val = 0.5 * own + 0.5 * other;
+                }
+                sF[w][c * NF + line * (N + 1) + r] = val;
+            }
+            sS[(c * NF + line * (N + 1) + r) * 2 + 0] = s1;
+            sS[(c * NF + line * (N + 1) + r) * 2 + 1] = s2;
+        }
+        __syncthreads();
+        if (pact) {
+            const int line = fam == 0 ? pjs : pis, pos = fam == 0 ? pis : pjs;
+            for (int w = 0; w < nvar; w++) {
+                const double *f = sF[w] + pc * NF + line * (N + 1);
+                const double *ss = sS + (pc * NF + line * (N + 1)) * 2;
+                for (int d = 0; d < 2; d++) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int r = 0; r <= N; r++) acc = acc + f[r] * c_M[N][r * N + pos] * ss[r * 2 + d];
+                    if (fam == 0) dpsi[w][d] = acc;
+                    else out[w][d] = acc;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double sumsq = 0.0, dmax = 0.0;
+    if (pact) {
+        const double J = jacobian<N>(sX + pc * 8, pis, pjs);
+        const size_t base = (size_t)(c0 + pc) * NQ + pk;
+        if (a.which == 0) {
+            const double n1 = (dpsi[0][0] + out[0][0]) / J;   // nablaQs(9,1)
+            const double n2 = (dpsi[0][1] + out[0][1]) / J;   // nablaQs(9,2)
+            a.Q[base + 5 * NP] = n2;
+            a.Q[base + 6 * NP] = -n1;
+        } else {
+            const double dbx = (dpsi[0][0] + out[0][0]) / J;  // nablaQs(6,1)
+            const double dby = (dpsi[1][1] + out[1][1]) / J;  // nablaQs(7,2)
+            const double db = dbx + dby;
+            sumsq = db * db;
+            dmax = __builtin_fabs(db);
+        }
+    }
+    if (a.which != 0) {
+        red[0][tid] = sumsq;
+        red[1][tid] = dmax;
+        __syncthreads();
+        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+            if (tid < s) {
+                red[0][tid] += red[0][tid + s];
+                red[1][tid] = fmax(red[1][tid], red[1][tid + s]);
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            a.partial[blk * 2 + 0] = red[0][0];
+            a.partial[blk * 2 + 1] = red[1][0];
+        }
+    }
+}
+
+}  // namespace chorus

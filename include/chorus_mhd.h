@@ -1,0 +1,119 @@
+/* chorus_mhd.h -- C-ABI of the MI355X-native CHORUS-MHD residual + time step.
+ *
+ * The reference exposes no plugin or FFI API: its hot path is a set of
+ * argument-less Fortran subroutines operating on `module setup2d`
+ * (setup2d.f90:1-99).  Each entry point below replaces one of them and is what
+ * an `iso_c_binding` interface in the reference's Fortran driver binds to
+ * (chorus-mhd_amd/fortran/chorus_hip.f90).  All arrays cross the boundary in
+ * the reference's own layouts: Fortran column-major, 1-based connectivity,
+ * Q(9,N,N,NCELL) FP64.  Host arrays are borrowed for the duration of a call
+ * (copied); the context owns all device memory and its HIP stream.
+ *
+ * Every function returns 0 on success and a negative code on error; the
+ * message is available from chorus_last_error().  A context is not
+ * thread-safe; calls are synchronous with respect to the host.
+ */
+#ifndef CHORUS_MHD_H
+#define CHORUS_MHD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CHORUS_OK            0
+#define CHORUS_ERR_ARG      -1   /* invalid argument / unsupported configuration */
+#define CHORUS_ERR_HIP      -2   /* HIP runtime error (no device, out of memory, launch) */
+#define CHORUS_ERR_TOPOLOGY -3   /* connectivity / periodic matching failure */
+
+/* precision forms */
+#define CHORUS_FP64   0   /* FP64 drop-in: every operation in FP64, reference order */
+#define CHORUS_MIXED  1   /* FP32 flux/contraction arithmetic; FP64 RK sums, CT, metrics */
+
+typedef struct chorus_ctx chorus_ctx;
+
+/* Mesh after the reference's set-up chain (main.f90:16-42): vertex
+ * coordinates XV/YV (readGRIDinfo.f90:283-290), IVCELL(NCELL,4)
+ * (counter-clockwise, connectivity.f90:236-283), IF2C(NFACE,4) left cell,
+ * right cell, left local face, right local face (connectivity.f90:156-215),
+ * IBFINTER(NINTER) (connect_bdry.f90:228-244), CYC2FPAIR(NPAIR,2)
+ * (map_cyc_loc.f90:12-35) and IBFREE(NFREE) (connect_bdry.f90:217-221). */
+typedef struct {
+    int ncell, nvert, N;          /* N = solution points per direction (2..8) */
+    const double *xv, *yv;        /* XV(NVERT), YV(NVERT) */
+    const int *ivcell;            /* IVCELL(NCELL,4) */
+    int nface;
+    const int *if2c;              /* IF2C(NFACE,4) */
+    int ninter;
+    const int *ibfinter;          /* IBFINTER(NINTER) */
+    int npair;
+    const int *cyc2fpair;         /* CYC2FPAIR(NPAIR,2) */
+    int nfree;
+    const int *ibfree;            /* IBFREE(NFREE) */
+} chorus_mesh;
+
+/* Run parameters of QUAD.INP (readinput.f90:16-33) plus the compile-time
+ * constants the reference fixes in setup2d.f90:64-71 (gam). */
+typedef struct {
+    int vismode;                  /* must be 1 (the reference reads Qv/Az flux-point data only then) */
+    int artiv;                    /* 0|1 artificial viscosity (AV.f90) */
+    int k_stage;                  /* 3 -> RK_S5P3, 5 -> FiveStageRk (iterations.f90:22-31) */
+    double mu, eta0;              /* viscosity, resistivity */
+    double gam;                   /* ratio of specific heats (setup2d.f90:68 uses 5/3) */
+    double dt;
+    double dxcycl, dycycl;        /* periodic shifts; also the AV length scale (AV.f90:147) */
+    int ncellglob;                /* NCELLGLOB (AV.f90:147, iterations.f90:66-67); 0 -> ncell */
+    int precision;                /* CHORUS_FP64 | CHORUS_MIXED */
+} chorus_params;
+
+/* INIT_SETUP replacement (initsetup.f90:1-153): uploads the mesh, builds the
+ * on-device face connectivity, operators and metrics.  Replaces the set-up half
+ * of initsetup.f90; the initial state is supplied with chorus_set_state. */
+int chorus_init(const chorus_mesh *mesh, const chorus_params *params, chorus_ctx **out);
+void chorus_free(chorus_ctx *ctx);
+const char *chorus_last_error(const chorus_ctx *ctx);
+
+/* Q(9,N,N,NCELL) in the reference layout (setup2d.f90:54).  Upload/download. */
+int chorus_set_state(chorus_ctx *ctx, const double *q);
+int chorus_get_state(chorus_ctx *ctx, double *q);
+
+/* compresid (compresid.f90:1-67): one residual evaluation of the current
+ * state; resid(9,N,N,NCELL) copied out when resid != NULL. */
+int chorus_compresid(chorus_ctx *ctx, double *resid);
+
+/* correct_B_via_curl_Az (setinitialcond.f90:82-92 -> magnetic_potential.f90:44-67). */
+int chorus_correct_b(chorus_ctx *ctx);
+
+/* nsteps iterations of the time loop body (iterations.f90:22-33): the SSP
+ * stepper of k_stage (RK_S5P3, iterations.f90:162-217, or FiveStageRk,
+ * iterations.f90:92-160) followed by correct_B_via_curl_Az. */
+int chorus_step(chorus_ctx *ctx, int nsteps);
+
+/* Diagnostics of iterations.f90:35-67 for the current state: RMS residual of
+ * the last stage evaluated, RMS div B (reference definitions), max |div B|,
+ * the last stage's maximum Rusanov wave speed and AV amplitude epsilon0
+ * (getrusanov.f90:105-107, AV.f90:145-148).  Any pointer may be NULL. */
+int chorus_diag(chorus_ctx *ctx, double *residglob, double *divbglob, double *divbmax,
+                double *eigvmax, double *epsilon0);
+
+/* Host-side topology helper for drivers that do not run the reference's own
+ * connectivity chain: CONNECTIVITY + CONNECT_BDRY + MAPCYCLOC
+ * (connectivity.f90:1-233, connect_bdry.f90:1-354, map_cyc_loc.f90:1-42) for
+ * one rank.  btype[NBND]: 1 CYCL, 2 FREE, 0 other.  Output arrays are sized by
+ * the caller: if2c >= 4*4*NCELL ints, ibfinter/ibfree >= 4*NCELL,
+ * cyc2fpair >= 2*4*NCELL.  Counts are returned through nface..nfree. */
+int chorus_topology(int ncell, int nvert, const int *ivcell, const double *xv, const double *yv,
+                    int nbnd, const int *bnd, const int *btype, double dxcycl, double dycycl,
+                    int *nface, int *if2c, int *ninter, int *ibfinter, int *npair,
+                    int *cyc2fpair, int *nfree, int *ibfree);
+
+/* Measurement hook (no reference counterpart): runs nsteps steps on the
+ * context's stream bracketed by HIP events and returns the elapsed time of the
+ * whole loop and the average duration of one stage-kernel launch, timed by
+ * events recorded around every stage launch on that same stream. */
+int chorus_time_steps(chorus_ctx *ctx, int nsteps, double *ms_total, double *ms_stage_avg,
+                      double *ms_ct_avg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,55 @@
+import glob
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(REPO, "chorus-mhd_amd"))
+sys.path.insert(0, os.path.join(REPO, "oracle"))
+GOLDEN = os.path.join(REPO, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
+
+
+def golden_names():
+    return sorted(os.path.splitext(os.path.basename(p))[0] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+
+
+def load_golden(name):
+    """Fixture arrays plus the mesh rebuilt from its recipe (checked against the stored coordinates)."""
+    from chorus_mhd import quad_grid
+    g = dict(np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False))
+    n = int(g["n"])
+    mesh = quad_grid(n, n, 0.0, 2 * math.pi, 0.0, 2 * math.pi, jitter=float(g["jitter"]),
+                     seed=int(g["seed"]), bc=str(g["bc"]))
+    assert np.array_equal(mesh.xv, g["xv"]) and np.array_equal(mesh.yv, g["yv"])
+    g["mesh"] = mesh
+    return g
+
+
+def split_topo(t):
+    """Unpack topo.i32 written by oracle/ref_harness.f90 (mode setup)."""
+    nc, nv, nf, ni, npair, nfree = [int(x) for x in t[:6]]
+    off = 6 + 4 * nc
+    if2c = t[off:off + 4 * nf].reshape(4, nf).T
+    off += 4 * nf
+    ibfinter = t[off:off + ni]
+    off += ni
+    cyc = t[off:off + 2 * npair].reshape(2, npair).T
+    off += 2 * npair
+    ibfree = t[off:off + nfree]
+    return dict(nface=nf, if2c=if2c, ibfinter=ibfinter, cyc2fpair=cyc, ibfree=ibfree)
+
+
+@pytest.fixture(scope="session")
+def hip_available():
+    try:
+        import torch  # noqa: F401  (device probe only)
+        return torch.cuda.is_available()
+    except Exception:
+        return False

@@ -1,0 +1,117 @@
+"""Write the golden fixtures of tests/golden/*.npz from the compiled reference.
+
+Runs the unmodified reference Fortran (built by ``make -C oracle ref`` into
+oracle/_ref/ref_harness) on small meshes produced by chorus_mhd.quad_grid and
+stores inputs (mesh recipe, initial state) and outputs (topology, residual,
+state after k steps, reference diagnostics) as compressed numpy archives.
+
+Only this script touches the reference; tests read the .npz files.
+Usage:  python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import math
+import os
+import shutil
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(REPO, "chorus-mhd_amd"))
+sys.path.insert(0, os.path.join(REPO, "oracle"))
+
+from chorus_mhd import QuadInp, quad_grid, write_restart   # noqa: E402
+import oracle as orc                                        # noqa: E402
+
+TWO_PI = 2 * math.pi
+
+# name: (n, N, jitter, bc, artiv, k_stage, steps, ic)
+CASES = {
+    "ot_n8_N4_uniform":      (8, 4, 0.00, "CYCL", 0, 3, 3, "ot"),
+    "ot_n8_N4_jitter_av":    (8, 4, 0.10, "CYCL", 1, 3, 3, "ot"),
+    "ot_n6_N5_jitter_s54":   (6, 5, 0.10, "CYCL", 1, 5, 2, "ot"),
+    "ot_n5_N6_free":         (5, 6, 0.12, "FREE", 0, 3, 2, "ot"),
+    "ot_n6_N3_free_av":      (6, 3, 0.10, "FREE", 1, 3, 3, "ot"),
+    "smooth_n6_N4_jitter":   (6, 4, 0.10, "CYCL", 0, 3, 2, "smooth"),
+    "smooth_n6_N5_free_av":  (6, 5, 0.08, "FREE", 1, 5, 2, "smooth"),
+}
+DT = 1.0e-3
+SEED = 7
+
+
+def smooth_state(mesh, N, sol_xy):
+    """A smooth state with every component non-zero (w and Bz included), so
+    every term of the fluxes contributes.  sol_xy: (NCELL, N, N, 2) points."""
+    x = sol_xy[..., 0]
+    y = sol_xy[..., 1]
+    gam = 5.0 / 3
+    rho = 1.0 + 0.2 * np.sin(x) * np.cos(y)
+    u = 0.5 * np.sin(y) + 0.1
+    v = -0.3 * np.cos(x)
+    w = 0.2 * np.sin(x + y)
+    p = 1.0 + 0.1 * np.cos(x - y)
+    Bx = 0.4 * np.cos(y)
+    By = 0.3 * np.sin(x)
+    Bz = 0.25 * np.cos(x + 2 * y)
+    Az = 0.4 * np.sin(y) + 0.3 * np.cos(x)
+    E = p / (gam - 1) + 0.5 * rho * (u * u + v * v + w * w) + 0.5 * (Bx * Bx + By * By + Bz * Bz)
+    return np.stack([rho, rho * u, rho * v, rho * w, E, Bx, By, Bz, Az], axis=-1)
+
+
+def run_case(name, spec, outdir):
+    n, N, jit, bc, av, kst, steps, ic = spec
+    mesh = quad_grid(n, n, 0.0, TWO_PI, 0.0, TWO_PI, jitter=jit, seed=SEED, bc=bc)
+    work = tempfile.mkdtemp()
+    try:
+        mesh.write(work)
+        q = QuadInp(N=N, dt=DT, MAXITER=steps, DXCYCL=mesh.dxcycl, DYCYCL=mesh.dycycl,
+                    ARTIV=av, k_stage=kst, restart_ord=N)
+        if ic == "smooth":
+            # solution-point coordinates from the reference itself, then a restart
+            q.write(os.path.join(work, "QUAD.INP"))
+            orc.run_reference(work, "setup")
+            geom = np.fromfile(os.path.join(work, "geom.f64"))
+            nv, nc = mesh.nvert, mesh.ncell
+            off = 2 * nv + nc * N * N + 2 * nc * 4 * (N + 1) * N
+            sxy = geom[off:off + 2 * nc * N * N].reshape(nc, N, N, 2)
+            Q0 = smooth_state(mesh, N, sxy)
+            write_restart(os.path.join(work, "restart.in"), 0, 0.0, Q0)
+            q.restart = 1
+            q.NAMERESTART = "restart.in"
+        q.write(os.path.join(work, "QUAD.INP"))
+        orc.run_reference(work, "setup")
+        topo = np.fromfile(os.path.join(work, "topo.i32"), dtype="<i4")
+        q0 = np.fromfile(os.path.join(work, "q0.f64")).reshape(mesh.ncell, N, N, 9)
+        orc.run_reference(work, "resid")
+        resid = np.fromfile(os.path.join(work, "resid.f64")).reshape(mesh.ncell, N, N, 9)
+        extra = {}
+        if av:
+            a = np.fromfile(os.path.join(work, "av.f64"))
+            extra = dict(eigvmax=a[0], epsilon0=a[2],
+                         sm_ind=a[3:3 + 9 * mesh.ncell].reshape(mesh.ncell, 9),
+                         muav=a[3 + 9 * mesh.ncell:3 + 18 * mesh.ncell].reshape(mesh.ncell, 9))
+        orc.run_reference(work, "run")
+        qend = np.fromfile(os.path.join(work, "qend.f64")).reshape(mesh.ncell, N, N, 9)
+        d = open(os.path.join(work, "diag.txt")).read().split()
+        np.savez_compressed(
+            os.path.join(outdir, name + ".npz"),
+            n=n, N=N, jitter=jit, bc=bc, artiv=av, k_stage=kst, steps=steps, ic=ic, dt=DT,
+            seed=SEED, xv=mesh.xv, yv=mesh.yv, topo=topo, q0=q0, resid=resid, qend=qend,
+            diag_iter=int(d[0]), diag_resid=float(d[2]), diag_divb=float(d[3]), **extra)
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def main():
+    if not os.path.exists(orc.REF_HARNESS):
+        raise SystemExit("build the reference first: make -C oracle ref")
+    for name, spec in CASES.items():
+        run_case(name, spec, HERE)
+        print("wrote", name)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,169 @@
+"""GPU parity: the HIP path (through the C-ABI) against the reference's golden
+fixtures and against the pinned oracle on the same seeded inputs.
+
+FP64 drop-in: bit-for-bit (every kernel keeps the reference's operation order
+and is built without FMA contraction).  Mixed form: relative L2 against FP64
+within the 1e-6 budget, div B at round-off on affine meshes.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import os
+
+from conftest import GOLDEN, golden_names, load_golden
+
+import oracle as orc
+from chorus_mhd import quad_grid
+from chorus_mhd.solver import Solver
+
+pytestmark = pytest.mark.gpu
+
+IDEAL = [n for n in golden_names() if int(np.load(os.path.join(GOLDEN, n + ".npz"))["artiv"]) == 0]
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _solver(g, precision="fp64"):
+    return Solver(g["mesh"], int(g["N"]), float(g["dt"]), artiv=int(g["artiv"]),
+                  k_stage=int(g["k_stage"]), precision=precision)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _build():
+    orc.build()
+
+
+@pytest.mark.parametrize("name", IDEAL)
+def test_residual_bitwise_vs_reference(name):
+    _say("case", name)
+    g = load_golden(name)
+    s = _solver(g)
+    s.set_state(g["q0"])
+    r = s.compresid()
+    assert np.array_equal(r, g["resid"]), np.abs(r - g["resid"]).max()
+
+
+@pytest.mark.parametrize("name", IDEAL)
+def test_steps_bitwise_vs_reference(name):
+    _say("case", name)
+    g = load_golden(name)
+    s = _solver(g)
+    s.set_state(g["q0"])
+    s.step(int(g["steps"]))
+    q = s.get_state()
+    assert np.array_equal(q, g["qend"]), np.abs(q - g["qend"]).max()
+
+
+@pytest.mark.parametrize("name", IDEAL)
+def test_diag_first_iteration_vs_reference(name):
+    _say("case", name)
+    g = load_golden(name)
+    s = _solver(g)
+    s.set_state(g["q0"])
+    s.step(1)
+    d = s.diag()
+    assert d["residglob"] == pytest.approx(float(g["diag_resid"]), rel=1e-13)
+    assert d["divbglob"] == pytest.approx(float(g["diag_divb"]), rel=1e-10, abs=1e-15)
+
+
+# (n, m, N, jitter, bc, k_stage)
+RANDOM_CASES = [
+    (5, 4, 2, 0.10, "CYCL", 3),
+    (7, 6, 3, 0.12, "FREE", 3),
+    (9, 7, 4, 0.10, "CYCL", 5),
+    (6, 9, 5, 0.14, "CYCL", 3),
+    (5, 5, 6, 0.10, "FREE", 5),
+    (4, 6, 7, 0.08, "CYCL", 3),
+    (4, 3, 8, 0.10, "CYCL", 3),
+    (31, 3, 5, 0.10, "CYCL", 3),      # ragged: the last block is partly empty
+    (1, 1, 4, 0.0, "FREE", 3),        # a single cell
+]
+
+
+def _smooth_state(o, mesh, N, seed):
+    o.init_ot()
+    Q = o.get_q()
+    rng = np.random.default_rng(seed)
+    Q[..., 3] += 0.1 * rng.standard_normal(Q.shape[:3])     # w != 0
+    Q[..., 7] += 0.1 * rng.standard_normal(Q.shape[:3])     # Bz != 0
+    Q[..., 4] += 0.5
+    return Q
+
+
+@pytest.mark.parametrize("case", RANDOM_CASES, ids=lambda c: "n%dx%d_N%d_%s_k%d" % (c[0], c[1], c[2], c[4], c[5]))
+def test_fp64_bitwise_vs_oracle(case):
+    n, m, N, jit, bc, kst = case
+    _say("case", case)
+    mesh = quad_grid(n, m, 0.0, 2 * math.pi, 0.0, 2 * math.pi, jitter=jit, seed=n * 31 + m, bc=bc)
+    o = orc.Oracle(mesh, N, dt=2e-3, k_stage=kst)
+    Q0 = _smooth_state(o, mesh, N, n + m)
+    s = Solver(mesh, N, 2e-3, k_stage=kst)
+    o.set_q(Q0)
+    s.set_state(Q0)
+    assert np.array_equal(s.compresid(), o.compresid())
+    s.correct_b()
+    o.correct_b()
+    assert np.array_equal(s.get_state(), o.get_q())
+    o.steps(3)
+    s.step(3)
+    assert np.array_equal(s.get_state(), o.get_q())
+
+
+def test_orszag_tang_64_ten_steps_bitwise():
+    """Config-1 mesh (64x64 uniform, N=4, AV off) for 10 steps against the oracle."""
+    _say("case OT 64x64 N=4")
+    mesh = quad_grid(64, 64, 0.0, 2 * math.pi, 0.0, 2 * math.pi)
+    N = 4
+    o = orc.Oracle(mesh, N, dt=1e-3)
+    o.init_ot()
+    s = Solver(mesh, N, 1e-3)
+    s.set_state(o.get_q())
+    o.steps(10)
+    s.step(10)
+    assert np.array_equal(s.get_state(), o.get_q())
+
+
+def test_mixed_precision_error_budget():
+    """Mixed form vs FP64 drop-in on a jittered OT mesh: relative L2 <= 1e-6 after
+    20 steps; div B of the FP64 CT stays at round-off on an affine mesh."""
+    _say("case mixed 24x24 N=5")
+    mesh = quad_grid(24, 24, 0.0, 2 * math.pi, 0.0, 2 * math.pi, jitter=0.1, seed=2)
+    N = 5
+    o = orc.Oracle(mesh, N, dt=1e-3)
+    o.init_ot()
+    Q0 = o.get_q()
+    ref = Solver(mesh, N, 1e-3)
+    mix = Solver(mesh, N, 1e-3, precision="mixed")
+    ref.set_state(Q0)
+    mix.set_state(Q0)
+    r64 = ref.compresid()
+    r32 = mix.compresid()
+    assert np.linalg.norm(r32 - r64) / np.linalg.norm(r64) < 1e-5
+    ref.step(20)
+    mix.step(20)
+    a, b = ref.get_state(), mix.get_state()
+    rel = np.linalg.norm(b - a) / np.linalg.norm(a)
+    _say("mixed rel L2 after 20 steps", rel)
+    assert rel < 1e-6
+    uni = quad_grid(24, 24, 0.0, 2 * math.pi, 0.0, 2 * math.pi)
+    ou = orc.Oracle(uni, N, dt=1e-3)
+    ou.init_ot()
+    mu = Solver(uni, N, 1e-3, precision="mixed")
+    mu.set_state(ou.get_q())
+    mu.step(5)
+    d = mu.diag()
+    assert d["divbmax"] < 1e-12
+
+
+def test_errors_are_reported():
+    mesh = quad_grid(4, 4, 0.0, 1.0, 0.0, 1.0)
+    with pytest.raises(Exception, match="N must be"):
+        Solver(mesh, 9, 1e-3)
+    with pytest.raises(Exception, match="k_stage"):
+        Solver(mesh, 4, 1e-3, k_stage=4)

@@ -1,0 +1,121 @@
+"""Host-side logic (no GPU): mesh generator, reference file formats, the C-ABI
+library's exports and its host-side topology helper against the reference."""
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import REPO, golden_names, load_golden, split_topo
+
+from chorus_mhd import QuadInp, quad_grid, read_mesh, read_restart, write_restart, fmt_d20
+
+
+def test_quad_grid_counts_and_orientation():
+    m = quad_grid(7, 5, 0.0, 2.0, -1.0, 0.5, jitter=0.15, seed=11)
+    assert m.ncell == 35 and m.nvert == 48 and m.bnd.shape[0] == 2 * (7 + 5)
+    x = m.xv[m.cells - 1]
+    y = m.yv[m.cells - 1]
+    # counter-clockwise convex quads (CHECK_MESH, connectivity.f90:255-263)
+    for k in range(4):
+        a, b, c = k, (k + 1) % 4, (k + 2) % 4
+        cross = (x[:, b] - x[:, a]) * (y[:, c] - y[:, b]) - (y[:, b] - y[:, a]) * (x[:, c] - x[:, b])
+        assert (cross > 0).all()
+    # boundary vertices stay on the grid: periodic midpoints match within tolCYC
+    hx, hy = 2.0 / 7, 1.5 / 5
+    for iv in np.unique(m.bnd):
+        xx, yy = m.xv[iv - 1], m.yv[iv - 1]
+        assert abs(xx / hx - round(xx / hx)) < 1e-12 and abs((yy + 1.0) / hy - round((yy + 1.0) / hy)) < 1e-12
+
+
+def test_jitter_bounds_and_seed():
+    a = quad_grid(6, 6, 0.0, 1.0, 0.0, 1.0, jitter=0.1, seed=5)
+    b = quad_grid(6, 6, 0.0, 1.0, 0.0, 1.0, jitter=0.1, seed=5)
+    u = quad_grid(6, 6, 0.0, 1.0, 0.0, 1.0)
+    assert np.array_equal(a.xv, b.xv)
+    d = np.abs(a.xv - u.xv)
+    assert d.max() <= 0.1 / 6 + 1e-15 and d.max() > 0
+
+
+def test_files_round_trip(tmp_path):
+    m = quad_grid(4, 3, 0.0, 2 * math.pi, 0.0, 2 * math.pi, jitter=0.1, seed=1)
+    m.write(str(tmp_path))
+    q = QuadInp(N=5, dt=1.2345678901234567e-3, MAXITER=7, DXCYCL=m.dxcycl, DYCYCL=m.dycycl, ARTIV=1)
+    q.write(str(tmp_path / "QUAD.INP"))
+    q2 = QuadInp.read(str(tmp_path / "QUAD.INP"))
+    assert q2 == q
+    m2 = read_mesh(str(tmp_path), q2)
+    assert np.array_equal(m2.xv, m.xv) and np.array_equal(m2.cells, m.cells)
+    assert m2.bnd_type == m.bnd_type
+    lines = open(tmp_path / "verts.dat").read().splitlines()
+    assert all(len(l) == 9 + 6 + 3 * 21 for l in lines)     # (I9,6X,3(D20.12,1X))
+    with pytest.raises(ValueError):
+        fmt_d20(1.0e30)
+
+
+def test_restart_round_trip(tmp_path):
+    Q = np.random.default_rng(0).standard_normal((5, 3, 3, 9))
+    p = str(tmp_path / "restart.00000012.dat")
+    write_restart(p, 12, 0.5, Q)
+    it, t, Q2 = read_restart(p, 3)
+    assert it == 12 and t == 0.5 and np.array_equal(Q, Q2)
+    assert os.path.getsize(p) == 4 + 8 + 5 * (4 + 81 * 8)
+
+
+# ------------------------------------------------------------------ C-ABI library
+def _header_symbols():
+    txt = open(os.path.join(REPO, "include", "chorus_mhd.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*(chorus_\w+)\s*\(", txt, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from chorus_mhd import lib
+    L = lib.load()
+    syms = _header_symbols()
+    assert len(syms) >= 11
+    for s in syms:
+        assert hasattr(L, s), s
+    assert set(syms) == set(lib.SIGNATURES)
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_library_topology_matches_reference(name):
+    """chorus_topology (host code of the library) == the reference's connectivity."""
+    from chorus_mhd.solver import topology
+    g = load_golden(name)
+    t = split_topo(g["topo"])
+    mine = topology(g["mesh"])
+    assert mine.nface == t["nface"]
+    assert np.array_equal(mine.if2c, t["if2c"])
+    assert np.array_equal(mine.ibfinter, t["ibfinter"])
+    assert np.array_equal(mine.cyc2fpair, t["cyc2fpair"])
+    assert np.array_equal(mine.ibfree, t["ibfree"])
+
+
+def test_topology_rejects_unpaired_periodic_face():
+    from chorus_mhd.solver import ChorusError, topology
+    m = quad_grid(4, 4, 0.0, 1.0, 0.0, 1.0)
+    m.dxcycl = 0.9            # shifts no longer match: map_cyc_loc.f90:37-40 stops
+    with pytest.raises(ChorusError):
+        topology(m)
+
+
+def test_product_path_has_no_cpu_fallback():
+    """Without a HIP device chorus_init fails loudly instead of computing on the CPU."""
+    from chorus_mhd import lib
+    from chorus_mhd.solver import ChorusError, Solver
+    import ctypes
+    L = lib.load()
+    n = ctypes.c_int(0)
+    try:
+        import torch
+        has_gpu = torch.cuda.is_available()
+    except Exception:
+        has_gpu = False
+    if has_gpu:
+        pytest.skip("a HIP device is present")
+    m = quad_grid(4, 4, 0.0, 1.0, 0.0, 1.0)
+    with pytest.raises(ChorusError, match="no HIP device"):
+        Solver(m, 4, 1e-3)
+    del L, n
