@@ -26,6 +26,13 @@ struct chorus_ctx {
     double *X[4] = {nullptr, nullptr, nullptr, nullptr};
     double *R = nullptr;           // residual output / FiveStageRk resid0
     double *partial = nullptr;     // per-block reduction partials
+    void *qvf = nullptr;           // viscous path: face-averaged Qv
+    void *gtr = nullptr;           // viscous path: own nablaQs face traces
+    double *sm = nullptr;          // viscous path: AV indicator per cell
+    double *eig = nullptr;         // two slots of the per-stage maximum wave speed
+    int stage_parity = 0;          // slot used by the next stage
+    int last_parity = -1;          // slot written by the last stage
+    double h_msh = 0.0;            // AV length scale (AV.f90:147)
     int nblk = 0;
     int cur = 0;                   // buffer holding the current state
     int last_in = -1;              // buffer holding the last stage's input state
@@ -71,8 +78,31 @@ int launch_stage(chorus_ctx *c, const StageArgs &a) {
     hipError_t e;
     bool t = c->timing && c->ev_used + 2 <= c->ev.size();
     if (t) (void)hipEventRecord(c->ev[c->ev_used], c->stream);
-    if (c->p.precision == CHORUS_MIXED) e = launch_stage_ideal_mixed(c->N, a, c->stream);
-    else e = launch_stage_ideal_fp64(c->N, a, c->stream);
+    if (c->ideal) {
+        if (c->p.precision == CHORUS_MIXED) e = launch_stage_ideal_mixed(c->N, a, c->stream);
+        else e = launch_stage_ideal_fp64(c->N, a, c->stream);
+    } else {
+        ViscArgs v{};
+        v.ncell = c->ncell;
+        v.cellxy = c->cellxy;
+        v.nbr = c->nbr;
+        v.Qin = a.Qin;
+        v.qvf = c->qvf;
+        v.gtr = c->gtr;
+        v.sm = c->sm;
+        v.eig = c->eig + c->stage_parity;
+        v.eig_next = c->eig + (1 - c->stage_parity);
+        v.artiv = c->p.artiv;
+        v.mu = c->p.mu;
+        v.eta0 = c->p.eta0;
+        v.gam = c->p.gam;
+        v.h_msh = c->h_msh;
+        v.pi = std::atan(1.0) * 4.0;
+        c->last_parity = c->stage_parity;
+        c->stage_parity = 1 - c->stage_parity;
+        if (c->p.precision == CHORUS_MIXED) e = launch_stage_visc_mixed(c->N, a, v, c->stream);
+        else e = launch_stage_visc_fp64(c->N, a, v, c->stream);
+    }
     if (t) { (void)hipEventRecord(c->ev[c->ev_used + 1], c->stream); c->ev_used += 2; }
     if (e != hipSuccess) return fail(c, CHORUS_ERR_HIP, std::string("stage launch: ") + hipGetErrorString(e));
     return 0;
@@ -352,12 +382,16 @@ int chorus_init(const chorus_mesh *m, const chorus_params *p, chorus_ctx **out) 
         return fail(c, CHORUS_ERR_ARG, "k_stage can be only 3 or 5 (iterations.f90:22-31)");
     if (p->precision != CHORUS_FP64 && p->precision != CHORUS_MIXED)
         return fail(c, CHORUS_ERR_ARG, "precision must be CHORUS_FP64 or CHORUS_MIXED");
-    if (p->artiv != 0 || p->mu != 0.0 || p->eta0 != 0.0)
-        return fail(c, CHORUS_ERR_ARG, "viscous/AV path not available in this build");
+    if (p->artiv != 0 && p->artiv != 1) return fail(c, CHORUS_ERR_ARG, "ARTIV must be 0 or 1");
+    if (p->artiv == 1 && m->N < 2) return fail(c, CHORUS_ERR_ARG, "AV needs N >= 2");
     c->p = *p;
+    // with mu = eta0 = 0 and no AV every viscous term is an exact zero
+    // (getrusanov.f90:218-247), so the stage needs no gradient pass
+    c->ideal = (p->artiv == 0 && p->mu == 0.0 && p->eta0 == 0.0);
     c->N = m->N;
     c->ncell = m->ncell;
     c->ncellglob = p->ncellglob > 0 ? p->ncellglob : m->ncell;
+    c->h_msh = std::sqrt(p->dxcycl * p->dycycl / (double)c->ncellglob);
 
     // on-device face connectivity: per cell and local face, the partner cell,
     // its local face and whether this cell is the reference's left or right cell
@@ -415,6 +449,14 @@ int chorus_init(const chorus_mesh *m, const chorus_params *p, chorus_ctx **out) 
     HIP_TRY(c, hipMalloc(&c->R, ns * sizeof(double)));
     c->nblk = stage_blocks(c->N, c->ncell);
     HIP_TRY(c, hipMalloc(&c->partial, (size_t)c->nblk * 2 * sizeof(double)));
+    HIP_TRY(c, hipMalloc(&c->eig, 2 * sizeof(double)));
+    HIP_TRY(c, hipMemsetAsync(c->eig, 0, 2 * sizeof(double), c->stream));
+    if (!c->ideal) {
+        const size_t nf = (size_t)c->ncell * 4 * c->N;
+        HIP_TRY(c, hipMalloc(&c->qvf, nf * 8 * sizeof(double)));
+        HIP_TRY(c, hipMalloc(&c->gtr, nf * 16 * sizeof(double)));
+        HIP_TRY(c, hipMalloc(&c->sm, (size_t)c->ncell * sizeof(double)));
+    }
     HIP_TRY(c, hipMemcpyAsync(c->cellxy, xy.data(), xy.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->nbr, nbr.data(), nbr.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -429,6 +471,10 @@ void chorus_free(chorus_ctx *c) {
     for (int i = 0; i < 4; i++) (void)hipFree(c->X[i]);
     (void)hipFree(c->R);
     (void)hipFree(c->partial);
+    (void)hipFree(c->eig);
+    (void)hipFree(c->qvf);
+    (void)hipFree(c->gtr);
+    (void)hipFree(c->sm);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -529,8 +575,15 @@ int chorus_diag(chorus_ctx *c, double *residglob, double *divbglob, double *divb
         if (divbglob) *divbglob = std::sqrt(s2 / npts);
         if (divbmax) *divbmax = mx;
     }
-    if (eigvmax) *eigvmax = NAN;
-    if (epsilon0) *epsilon0 = NAN;
+    if (eigvmax || epsilon0) {
+        double e = NAN;
+        if (!c->ideal && c->last_parity >= 0) {
+            HIP_TRY(c, hipMemcpyAsync(&e, c->eig + c->last_parity, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+        }
+        if (eigvmax) *eigvmax = e;
+        if (epsilon0) *epsilon0 = (c->p.artiv == 1) ? e * c->h_msh / (double)(N - 1) * 1.0 : NAN;
+    }
     return 0;
 }
 

@@ -1,10 +1,13 @@
 // FP64 drop-in kernels.  Built with -ffp-contract=off: no FMA is formed, so each
 // product and sum rounds separately, exactly like the reference build.
-#include "kernels.hpp"
+#include "kernels_visc.hpp"
 
 namespace chorus {
 
-hipError_t upload_operators_fp64() { return upload_tables(); }
+hipError_t upload_operators_fp64() {
+    hipError_t e = upload_tables();
+    return e != hipSuccess ? e : upload_av_tables();
+}
 
 template <int N> static int nblocks(int ncell) { return (ncell + Shape<N>::CPB - 1) / Shape<N>::CPB; }
 
@@ -49,5 +52,20 @@ int stage_blocks(int N, int ncell) {
     }
 }
 int ct_blocks(int N, int ncell) { return stage_blocks(N, ncell); }
+
+template <int N> static int vblocks(int ncell) { return (ncell + VShape<N>::CPB - 1) / VShape<N>::CPB; }
+
+hipError_t launch_stage_visc_fp64(int N, const StageArgs &a, const ViscArgs &v, hipStream_t s) {
+#define V_(n)                                                                                         \
+    case n:                                                                                           \
+        hipLaunchKernelGGL((k_grad<n, double>), dim3(vblocks<n>(a.ncell)), dim3(VShape<n>::THREADS), 0, s, v); \
+        hipLaunchKernelGGL((k_stage_visc<n, double>), dim3(vblocks<n>(a.ncell)), dim3(VShape<n>::THREADS), 0, s, a, v); \
+        return hipGetLastError();
+    switch (N) {
+        V_(2) V_(3) V_(4) V_(5) V_(6) V_(7) V_(8)
+    default: return hipErrorInvalidValue;
+    }
+#undef V_
+}
 
 }  // namespace chorus

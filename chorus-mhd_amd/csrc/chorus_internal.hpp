@@ -32,6 +32,23 @@ struct StageArgs {
     double gam;
 };
 
+// viscous / AV path (k_grad + k_stage_visc)
+struct ViscArgs {
+    int ncell;
+    const double *cellxy;
+    const int2 *nbr;
+    const double *Qin;
+    void *qvf;          // face-averaged Qv   [cell][lf][nfp][8]   (Real)
+    void *gtr;          // own nablaQs traces [cell][lf][nfp][16]  (Real)
+    double *sm;         // log10 smoothness indicator of density per cell
+    double *eig;        // this stage's maximum Rusanov wave speed (atomic max)
+    double *eig_next;   // the next stage's slot, zeroed by k_stage_visc
+    int artiv;
+    double mu, eta0, gam, h_msh, pi;
+    template <class R> __host__ __device__ R *qvf_r() const { return (R *)qvf; }
+    template <class R> __host__ __device__ R *gtr_r() const { return (R *)gtr; }
+};
+
 struct CTArgs {
     int ncell;
     const double *cellxy;
@@ -66,6 +83,8 @@ __device__ __forceinline__ void stage_combine(const StageArgs &a, size_t base, i
 hipError_t launch_stage_ideal_fp64(int N, const StageArgs &a, hipStream_t s);
 hipError_t launch_stage_ideal_mixed(int N, const StageArgs &a, hipStream_t s);
 hipError_t launch_ct(int N, const CTArgs &a, hipStream_t s);
+hipError_t launch_stage_visc_fp64(int N, const StageArgs &a, const ViscArgs &v, hipStream_t s);
+hipError_t launch_stage_visc_mixed(int N, const StageArgs &a, const ViscArgs &v, hipStream_t s);
 int ct_blocks(int N, int ncell);
 int stage_blocks(int N, int ncell);
 hipError_t upload_operators_fp64();
@@ -77,5 +96,8 @@ hipError_t upload_operators_mixed();
 void host_operators(int N, double *Xs, double *Xf, double *L, double *M);
 // Xs for N points (used for N-1 by the AV sensor, initsetup.f90:44-45)
 void host_solution_points(int N, double *Xs);
+// AV interpolation tables (AV.f90:19-58): H1[isl*N+is] = hhval(is, Xs_low(isl)),
+// H2[is*(N-1)+isl] = hhval_low(isl, Xs(is))
+void host_av_tables(int N, double *H1, double *H2);
 
 }  // namespace chorus

@@ -1,10 +1,13 @@
 // Mixed-precision kernels: FP32 traces, fluxes, Rusanov and derivative
 // contractions; the state, the RK stage sums and the metrics stay FP64.
-#include "kernels.hpp"
+#include "kernels_visc.hpp"
 
 namespace chorus {
 
-hipError_t upload_operators_mixed() { return upload_tables(); }
+hipError_t upload_operators_mixed() {
+    hipError_t e = upload_tables();
+    return e != hipSuccess ? e : upload_av_tables();
+}
 
 template <int N> static int nblocks(int ncell) { return (ncell + Shape<N>::CPB - 1) / Shape<N>::CPB; }
 
@@ -18,6 +21,21 @@ hipError_t launch_stage_ideal_mixed(int N, const StageArgs &a, hipStream_t s) {
     default: return hipErrorInvalidValue;
     }
 #undef L_
+}
+
+template <int N> static int vblocks(int ncell) { return (ncell + VShape<N>::CPB - 1) / VShape<N>::CPB; }
+
+hipError_t launch_stage_visc_mixed(int N, const StageArgs &a, const ViscArgs &v, hipStream_t s) {
+#define V_(n)                                                                                         \
+    case n:                                                                                           \
+        hipLaunchKernelGGL((k_grad<n, float>), dim3(vblocks<n>(a.ncell)), dim3(VShape<n>::THREADS), 0, s, v); \
+        hipLaunchKernelGGL((k_stage_visc<n, float>), dim3(vblocks<n>(a.ncell)), dim3(VShape<n>::THREADS), 0, s, a, v); \
+        return hipGetLastError();
+    switch (N) {
+        V_(2) V_(3) V_(4) V_(5) V_(6) V_(7) V_(8)
+    default: return hipErrorInvalidValue;
+    }
+#undef V_
 }
 
 }  // namespace chorus

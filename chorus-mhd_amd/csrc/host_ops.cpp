@@ -76,6 +76,16 @@ double lagrange(int np, const double *xs, int i, double xval) {
 
 void host_solution_points(int N, double *Xs) { points(N, Xs, nullptr); }
 
+void host_av_tables(int N, double *H1, double *H2) {
+    double xs[kMaxN], xl[kMaxN];
+    points(N, xs, nullptr);
+    points(N - 1, xl, nullptr);
+    for (int isl = 0; isl < N - 1; isl++)
+        for (int is = 0; is < N; is++) H1[isl * N + is] = lagrange(N, xs, is, xl[isl]);
+    for (int is = 0; is < N; is++)
+        for (int isl = 0; isl < N - 1; isl++) H2[is * (N - 1) + isl] = lagrange(N - 1, xl, isl, xs[is]);
+}
+
 // computeLandM (initsetup.f90:258-315)
 void host_operators(int N, double *Xs, double *Xf, double *L, double *M) {
     points(N, Xs, Xf);

@@ -1,0 +1,488 @@
+// Viscous / artificial-viscosity path (vismode=1 with ARTIV=1 or mu, eta0 != 0).
+//
+// The reference stage (compflux.f90:1-31) has three neighbour-coupled phases
+// and one global reduction; on the device they are two kernels per stage:
+//
+//   k_grad       per cell: central Qv at every flux point (face values average
+//                the neighbour's trace, interfaceflux_all.f90:79-91), nablaQs at
+//                the solution points (nablaQsp, compflux.f90:107-157), its own
+//                traces at the four faces (for the neighbours' averages,
+//                interfaceVISflux), the Qv face averages, the AV smoothness
+//                indicator of density (AV.f90:15-82) and the maximum Rusanov
+//                wave speed of the stage (getrusanov.f90:105-107, atomic max)
+//   k_stage_visc per cell: the inviscid stage of k_stage_ideal plus the viscous
+//                and AV fluxes (comp_viscous_flux, compflux.f90:198-258) from
+//                the stored face data, the residual (compresid.f90:1-67) and
+//                the SSP stage combination
+//
+// Face data in HBM, per cell, local face lf and face point nfp (this cell's
+// own numbering along the face, face2fpconnec order):
+//   qvf[cell][lf][nfp][8]   face-averaged Qv (variables 1..8)
+//   gtr[cell][lf][nfp][16]  this cell's trace of nablaQs (vars 1..8, x then y)
+//   sm[cell]                log10 smoothness indicator of density (SM_IND(1,:))
+// Only muAV(1,:) reaches the flux (mu_art = epsilon_loc(1), getrusanov.f90:279),
+// so only density's indicator is evaluated.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace chorus {
+
+// AV interpolation tables per N: H1[isl*N+is] = hhval(is, Xs_low(isl)),
+// H2[is*(N-1)+isl] = hhval_low(isl, Xs(is))  (AV.f90:19-58)
+static __constant__ double c_H1[kMaxN + 1][kMaxN * kMaxN];
+static __constant__ double c_H2[kMaxN + 1][kMaxN * kMaxN];
+
+static hipError_t upload_av_tables() {
+    static double H1[kMaxN + 1][kMaxN * kMaxN], H2[kMaxN + 1][kMaxN * kMaxN];
+    for (int n = 2; n <= kMaxN; n++) host_av_tables(n, H1[n], H2[n]);
+    hipError_t e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_H1), H1, sizeof(H1))) != hipSuccess) return e;
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_H2), H2, sizeof(H2));
+}
+
+template <int N> struct VShape {
+    static constexpr int NP = N * N, NF = N * (N + 1), NQ = NV * NP;
+    static constexpr int CPB = (128 / NP) > 0 ? (128 / NP) : 1;
+    static constexpr int THREADS = ((CPB * NP + 63) / 64) * 64;
+};
+
+// local-face index (0..3) and own face-point number of flux point (fam, r, line)
+template <int N> __device__ __forceinline__ void face_of(int fam, int r, int line, int &lfi, int &nfp) {
+    if (fam == 0) { lfi = (r == 0) ? 3 : 1; nfp = (r == 0) ? (N - 1 - line) : line; }
+    else          { lfi = (r == 0) ? 0 : 2; nfp = (r == 0) ? line : (N - 1 - line); }
+}
+
+// partner's own face-point number of the same physical point
+template <int N> __device__ __forceinline__ int partner_nfp(int nfp) { return N - 1 - nfp; }
+
+__device__ __forceinline__ void atomic_max_nonneg(double *p, double v) {
+    // non-negative doubles order like their bit patterns
+    atomicMax(reinterpret_cast<unsigned long long *>(p), (unsigned long long)__double_as_longlong(v));
+}
+
+// ---------------------------------------------------------------------------
+template <int N, class R>
+__global__ void __launch_bounds__(128) k_grad(ViscArgs a) {
+    using S = VShape<N>;
+    constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
+    __shared__ double sQ[CPB * NQ];
+    __shared__ double sX[CPB * 8];
+    __shared__ R sQv[CPB * NF * 8];      // [c][line][r][v], one family at a time
+    __shared__ R sS[CPB * NF * 2];
+    __shared__ R sG[CPB * 16 * NP];      // nablaQs [c][d*8+v][p]
+    __shared__ double sAV[CPB * NP];     // sensor work
+    __shared__ double sRed[128];
+    const int tid = threadIdx.x;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int c0 = blk * CPB;
+    const int ncl = min(CPB, a.ncell - c0);
+    for (int i = tid; i < ncl * NQ; i += blockDim.x) sQ[i] = a.Qin[(size_t)c0 * NQ + i];
+    for (int i = tid; i < ncl * 8; i += blockDim.x) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
+    __syncthreads();
+    const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
+    const bool pact = (tid < CPB * NP) && (pc < ncl);
+    R gp[16];
+    double emax = 0.0;
+#pragma unroll 1
+    for (int fam = 0; fam < 2; fam++) {
+        for (int it = tid; it < ncl * NF; it += blockDim.x) {
+            const int c = it / NF, k = it % NF, line = k / (N + 1), r = k % (N + 1);
+            const int cell = c0 + c;
+            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+            double s1d, s2d;
+            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+            R Qown[NV], qv[8];
+            trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown);
+            if (r > 0 && r < N) {
+#pragma unroll
+                for (int v = 0; v < 8; v++) qv[v] = Qown[v];
+            } else {
+                int lfi, nfp;
+                face_of<N>(fam, r, line, lfi, nfp);
+                const FacePoint fp = resolve_face<N>(a.nbr[(size_t)cell * 4 + lfi], fam, r, line);
+                R Qnb[NV], nx, ny;
+                if (fp.free_face) {
+#pragma unroll
+                    for (int v = 0; v < NV; v++) Qnb[v] = Qown[v];
+                    nx = R(s1d); ny = R(s2d);
+                } else {
+                    trace9<N, R>(a.Qin + (size_t)fp.nb * NQ, fp.nb_fam, fp.nb_ifp, fp.nb_jfp, Qnb);
+                    if (!fp.right) { nx = R(s1d); ny = R(s2d); }
+                    else {
+                        double xyn[8], n1, n2;
+#pragma unroll
+                        for (int q = 0; q < 8; q++) xyn[q] = a.cellxy[(size_t)fp.nb * 8 + q];
+                        metric_row<N>(xyn, fp.nb_fam, fp.nb_ifp, fp.nb_jfp, n1, n2);
+                        nx = R(n1); ny = R(n2);
+                    }
+                }
+                // central average (0.5*Qfl + 0.5*Qfr is symmetric in its operands)
+#pragma unroll
+                for (int v = 0; v < 8; v++) qv[v] = R(0.5) * Qown[v] + R(0.5) * Qnb[v];
+                R* dst = a.qvf_r<R>() + (((size_t)cell * 4 + lfi) * N + nfp) * 8;
+#pragma unroll
+                for (int v = 0; v < 8; v++) dst[v] = qv[v];
+                if (a.artiv) {
+                    R Fn[8], az, eig;
+                    if (!fp.right) rusanov<R>(R(a.gam), Qown, Qnb, nx, ny, fp.sign_l, Fn, az, eig);
+                    else rusanov<R>(R(a.gam), Qnb, Qown, nx, ny, fp.sign_l, Fn, az, eig);
+                    emax = fmax(emax, double(eig));
+                }
+            }
+            R *q = sQv + (c * NF + line * (N + 1) + r) * 8;
+#pragma unroll
+            for (int v = 0; v < 8; v++) q[v] = qv[v];
+            sS[(c * NF + line * (N + 1) + r) * 2 + 0] = R(s1d);
+            sS[(c * NF + line * (N + 1) + r) * 2 + 1] = R(s2d);
+        }
+        __syncthreads();
+        if (pact) {
+            const int line = fam == 0 ? pjs : pis, pos = fam == 0 ? pis : pjs;
+            const R *q = sQv + (pc * NF + line * (N + 1)) * 8;
+            const R *ss = sS + (pc * NF + line * (N + 1)) * 2;
+            R J = R(0);
+            if (fam == 1) J = R(jacobian<N>(sX + pc * 8, pis, pjs));
+#pragma unroll
+            for (int d = 0; d < 2; d++)
+#pragma unroll
+                for (int v = 0; v < 8; v++) {
+                    R acc = R(0);
+#pragma unroll
+                    for (int r = 0; r <= N; r++) acc = acc + q[r * 8 + v] * Mop<R>(N, r, pos) * ss[r * 2 + d];
+                    if (fam == 0) gp[d * 8 + v] = acc;
+                    else sG[(pc * 16 + d * 8 + v) * NP + pk] = (gp[d * 8 + v] + acc) / J;
+                }
+        }
+        __syncthreads();
+    }
+    // own traces of nablaQs at the four faces (interfaceVISflux, interfaceflux_all.f90:499-536)
+    for (int it = tid; it < ncl * 4 * N; it += blockDim.x) {
+        const int c = it / (4 * N), k = it % (4 * N), lfi = k / N, nfp = k % N;
+        int ifp, jfp;
+        face2fp<N>(lfi + 1, nfp, ifp, jfp);
+        const int fam = lf_family(lfi + 1);
+        R *dst = a.gtr_r<R>() + (((size_t)(c0 + c) * 4 + lfi) * N + nfp) * 16;
+        for (int g = 0; g < 16; g++) {
+            const R *gq = sG + (c * 16 + g) * NP;
+            R acc = R(0);
+            if (fam == 0) {
+#pragma unroll
+                for (int sp = 0; sp < N; sp++) acc = acc + gq[jfp * N + sp] * Lop<R>(N, ifp, sp);
+            } else {
+#pragma unroll
+                for (int sp = 0; sp < N; sp++) acc = acc + gq[sp * N + ifp] * Lop<R>(N, jfp, sp);
+            }
+            dst[g] = acc;
+        }
+    }
+    if (a.artiv) {
+        // smoothness indicator of density (AV.f90:15-82), FP64 in both forms:
+        // N -> N-1 -> N interpolation round trip and log10 of the relative energy
+        constexpr int NL = N - 1;
+        double *low = sAV;   // [c][jsl*NL+isl] uses the first NL*NL entries of each cell
+        for (int it = tid; it < ncl * NL * NL; it += blockDim.x) {
+            const int c = it / (NL * NL), k = it % (NL * NL), jsl = k / NL, isl = k % NL;
+            double acc = 0.0;
+            for (int js = 0; js < N; js++)
+                for (int is = 0; is < N; is++)
+                    acc = acc + sQ[c * NQ + js * N + is] * c_H1[N][isl * N + is] * c_H1[N][jsl * N + js];
+            low[c * NP + k] = acc;
+        }
+        __syncthreads();
+        double dev = 0.0;
+        if (pact) {
+            double acc = 0.0;
+            for (int jsl = 0; jsl < NL; jsl++)
+                for (int isl = 0; isl < NL; isl++)
+                    acc = acc + low[pc * NP + jsl * NL + isl] * c_H2[N][pis * NL + isl] * c_H2[N][pjs * NL + jsl];
+            const double q = sQ[pc * NQ + pk];
+            dev = (acc - q) * (acc - q) / (q * q);
+        }
+        __syncthreads();
+        if (pact) sAV[pc * NP + pk] = dev;
+        __syncthreads();
+        if (tid < ncl) {
+            double se = 0.0;
+            for (int p = 0; p < NP; p++) se = se + sAV[tid * NP + p];
+            a.sm[c0 + tid] = log10(se);
+        }
+        // block maximum of the face wave speeds, one atomic per block
+        sRed[tid] = emax;
+        __syncthreads();
+        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+            if (tid < s) sRed[tid] = fmax(sRed[tid], sRed[tid + s]);
+            __syncthreads();
+        }
+        if (tid == 0) atomic_max_nonneg(a.eig, sRed[0]);
+    }
+}
+
+// muAV(1,:) of one cell from its indicator and epsilon0 (AV.f90:120-172)
+__device__ __forceinline__ double mu_av(double sm, double eps0, double pi) {
+    const double se_low = -6.0, se_high = -1.0;
+    const double k_range = (se_high - se_low) * 0.5, s0 = (se_high + se_low) * 0.5;
+    if (sm < se_low) return 0.0;
+    if (sm > se_high) return eps0;
+    return eps0 / 2.0 * (1.0 + sin(pi * (sm - s0) / 2.0 / k_range));
+}
+
+// ---------------------------------------------------------------------------
+template <int N, class R>
+__global__ void __launch_bounds__(128) k_stage_visc(StageArgs a, ViscArgs v) {
+    using S = VShape<N>;
+    constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
+    __shared__ double sQ[CPB * NQ];
+    __shared__ double sX[CPB * 8];
+    __shared__ R sG[CPB * 16 * NP];
+    __shared__ R sU[CPB * NF * 17];      // Qv (8) in the gradient phase; F (8) + Fv (9) in the flux phase
+    __shared__ R sA[CPB * NF];
+    __shared__ R sS[CPB * NF * 2];
+    __shared__ double sMu[CPB * 4];      // muAVfi/fj(1,side) per local face
+    const int tid = threadIdx.x;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int c0 = blk * CPB;
+    const int ncl = min(CPB, a.ncell - c0);
+    for (int i = tid; i < ncl * NQ; i += blockDim.x) sQ[i] = a.Qin[(size_t)c0 * NQ + i];
+    for (int i = tid; i < ncl * 8; i += blockDim.x) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
+    if (v.artiv) {
+        // epsilon0 = eigvmax_g * h / (N-1) (AV.f90:145-148); face averages of muAV(1)
+        // (compAVflux/interfaceAVflux/cyclocAVflux, AV.f90:199-303, 494-564)
+        const double eps0 = *v.eig * v.h_msh / (double)(N - 1) * 1.0;
+        for (int i = tid; i < ncl * 4; i += blockDim.x) {
+            const int c = i / 4, lfi = i % 4, cell = c0 + c;
+            const int2 info = a.nbr[(size_t)cell * 4 + lfi];
+            const double mine = mu_av(v.sm[cell], eps0, v.pi);
+            double val = mine;
+            if (!((info.y >> 4) & 1)) val = 0.5 * (mine + mu_av(v.sm[info.x], eps0, v.pi));
+            sMu[i] = val;
+        }
+        if (blockIdx.x == 0 && tid == 0) *v.eig_next = 0.0;
+    }
+    __syncthreads();
+    const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
+    const bool pact = (tid < CPB * NP) && (pc < ncl);
+
+    // 1. nablaQs from own Q and the stored face averages (same operations as k_grad)
+    {
+        R gp[16];
+#pragma unroll 1
+        for (int fam = 0; fam < 2; fam++) {
+            for (int it = tid; it < ncl * NF; it += blockDim.x) {
+                const int c = it / NF, k = it % NF, line = k / (N + 1), r = k % (N + 1);
+                const int cell = c0 + c;
+                const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+                double s1d, s2d;
+                metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+                R *q = sU + (c * NF + line * (N + 1) + r) * 8;
+                if (r > 0 && r < N) {
+                    R Qown[NV];
+                    trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown);
+#pragma unroll
+                    for (int w = 0; w < 8; w++) q[w] = Qown[w];
+                } else {
+                    int lfi, nfp;
+                    face_of<N>(fam, r, line, lfi, nfp);
+                    const R *src = v.qvf_r<R>() + (((size_t)cell * 4 + lfi) * N + nfp) * 8;
+#pragma unroll
+                    for (int w = 0; w < 8; w++) q[w] = src[w];
+                }
+                sS[(c * NF + line * (N + 1) + r) * 2 + 0] = R(s1d);
+                sS[(c * NF + line * (N + 1) + r) * 2 + 1] = R(s2d);
+            }
+            __syncthreads();
+            if (pact) {
+                const int line = fam == 0 ? pjs : pis, pos = fam == 0 ? pis : pjs;
+                const R *q = sU + (pc * NF + line * (N + 1)) * 8;
+                const R *ss = sS + (pc * NF + line * (N + 1)) * 2;
+                R J = R(0);
+                if (fam == 1) J = R(jacobian<N>(sX + pc * 8, pis, pjs));
+#pragma unroll
+                for (int d = 0; d < 2; d++)
+#pragma unroll
+                    for (int w = 0; w < 8; w++) {
+                        R acc = R(0);
+#pragma unroll
+                        for (int r = 0; r <= N; r++) acc = acc + q[r * 8 + w] * Mop<R>(N, r, pos) * ss[r * 2 + d];
+                        if (fam == 0) gp[d * 8 + w] = acc;
+                        else sG[(pc * 16 + d * 8 + w) * NP + pk] = (gp[d * 8 + w] + acc) / J;
+                    }
+            }
+            __syncthreads();
+        }
+    }
+
+    // 2. inviscid + viscous fluxes per family, then the derivatives at the solution points
+    R dF[8], dFv[NV], dA1 = R(0), dA2 = R(0);
+    const R gam = R(a.gam), lam = R(-2.0 / 3.0), prandt = R(0.72);
+    const bool visc_mu = v.mu != 0.0 || v.eta0 != 0.0;
+#pragma unroll 1
+    for (int fam = 0; fam < 2; fam++) {
+        for (int it = tid; it < ncl * NF; it += blockDim.x) {
+            const int c = it / NF, k = it % NF, line = k / (N + 1), r = k % (N + 1);
+            const int cell = c0 + c;
+            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+            double s1d, s2d;
+            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+            const R s1 = R(s1d), s2 = R(s2d);
+            R Qown[NV];
+            trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown);
+            // gradient at this flux point: own trace of nablaQs (nablaQ_interior_FP)
+            R g[2 * NV];
+            {
+                const R *gq = sG + c * 16 * NP;
+#pragma unroll
+                for (int d = 0; d < 2; d++) {
+#pragma unroll
+                    for (int w = 0; w < 8; w++) {
+                        R acc = R(0);
+                        if (fam == 0) {
+#pragma unroll
+                            for (int sp = 0; sp < N; sp++) acc = acc + gq[(d * 8 + w) * NP + jfp * N + sp] * Lop<R>(N, ifp, sp);
+                        } else {
+#pragma unroll
+                            for (int sp = 0; sp < N; sp++) acc = acc + gq[(d * 8 + w) * NP + sp * N + ifp] * Lop<R>(N, jfp, sp);
+                        }
+                        g[d * NV + w] = acc;
+                    }
+                    g[d * NV + 8] = R(0);
+                }
+            }
+            R out[8], az, qv[NV];
+#pragma unroll
+            for (int w = 0; w < NV; w++) qv[w] = Qown[w];
+            if (r > 0 && r < N) {
+                R F[8], G[8];
+                iflux<R>(gam, Qown, F, G);
+#pragma unroll
+                for (int w = 0; w < 8; w++) out[w] = F[w] * s1 + G[w] * s2;
+                az = Qown[8];
+            } else {
+                int lfi, nfp;
+                face_of<N>(fam, r, line, lfi, nfp);
+                const FacePoint fp = resolve_face<N>(a.nbr[(size_t)cell * 4 + lfi], fam, r, line);
+                R Qnb[NV], nx, ny;
+                if (fp.free_face) {
+#pragma unroll
+                    for (int w = 0; w < NV; w++) Qnb[w] = Qown[w];
+                    nx = s1; ny = s2;
+                } else {
+                    trace9<N, R>(a.Qin + (size_t)fp.nb * NQ, fp.nb_fam, fp.nb_ifp, fp.nb_jfp, Qnb);
+                    if (!fp.right) { nx = s1; ny = s2; }
+                    else {
+                        double xyn[8], n1, n2;
+#pragma unroll
+                        for (int q = 0; q < 8; q++) xyn[q] = a.cellxy[(size_t)fp.nb * 8 + q];
+                        metric_row<N>(xyn, fp.nb_fam, fp.nb_ifp, fp.nb_jfp, n1, n2);
+                        nx = R(n1); ny = R(n2);
+                    }
+                }
+                R eig;
+                if (!fp.right) {
+                    rusanov<R>(gam, Qown, Qnb, nx, ny, fp.sign_l, out, az, eig);
+                } else {
+                    R Fnl[8];
+                    rusanov<R>(gam, Qnb, Qown, nx, ny, fp.sign_l, Fnl, az, eig);
+                    const R slr = R(fp.sign_l * fp.sign_r);
+#pragma unroll
+                    for (int w = 0; w < 8; w++) out[w] = slr * Fnl[w];
+                }
+                // face-averaged Qv and gradient (interfaceVISflux / BCVISFLUX)
+                const R *src = v.qvf_r<R>() + (((size_t)cell * 4 + lfi) * N + nfp) * 8;
+#pragma unroll
+                for (int w = 0; w < 8; w++) qv[w] = src[w];
+                const R *ot = fp.free_face ? nullptr
+                                           : v.gtr_r<R>() + (((size_t)fp.nb * 4 + (a.nbr[(size_t)cell * 4 + lfi].y & 7) - 1) * N +
+                                                             partner_nfp<N>(nfp)) * 16;
+#pragma unroll
+                for (int d = 0; d < 2; d++)
+#pragma unroll
+                    for (int w = 0; w < 8; w++) {
+                        const R mine = g[d * NV + w];
+                        const R other = ot ? ot[d * 8 + w] : mine;
+                        g[d * NV + w] = R(0.5) * mine + R(0.5) * other;
+                    }
+            }
+            // viscous flux (comp_viscous_flux, compflux.f90:205-256)
+            R fv[NV];
+            {
+                R Fv[NV], Gv[NV];
+                if (visc_mu) {
+                    vflux<R>(gam, lam, prandt, R(v.mu), R(v.eta0), qv, g, Fv, Gv);
+#pragma unroll
+                    for (int w = 0; w < NV; w++) fv[w] = Fv[w] * s1 + Gv[w] * s2;
+                } else {
+#pragma unroll
+                    for (int w = 0; w < NV; w++) fv[w] = R(0);
+                }
+                if (v.artiv) {
+                    int lo = fam == 0 ? 3 : 0, hi = fam == 0 ? 1 : 2;   // sides 1 and 2 of the family
+                    const double x = fam == 0 ? c_Xf[N][ifp] : c_Xf[N][jfp];
+                    const R mu_art = R(sMu[c * 4 + lo] * (1.0 - x) + sMu[c * 4 + hi] * x);
+                    vflux<R>(gam, lam, prandt, mu_art, R(0), qv, g, Fv, Gv);
+#pragma unroll
+                    for (int w = 0; w < NV; w++) fv[w] = fv[w] + Fv[w] * s1 + Gv[w] * s2;
+                }
+            }
+            const int o = c * NF + line * (N + 1) + r;
+            R *f = sU + c * 17 * NF + line * (N + 1) + r;
+#pragma unroll
+            for (int w = 0; w < 8; w++) f[w * NF] = out[w];
+#pragma unroll
+            for (int w = 0; w < NV; w++) f[(8 + w) * NF] = fv[w];
+            sA[o] = az;
+            sS[o * 2 + 0] = s1;
+            sS[o * 2 + 1] = s2;
+        }
+        __syncthreads();
+        if (pact) {
+            const int line = fam == 0 ? pjs : pis, pos = fam == 0 ? pis : pjs;
+            const R *f = sU + pc * 17 * NF + line * (N + 1);
+            const R *aa = sA + pc * NF + line * (N + 1);
+            const R *ss = sS + (pc * NF + line * (N + 1)) * 2;
+            R d[8], dv[NV], e1 = R(0), e2 = R(0);
+#pragma unroll
+            for (int w = 0; w < 8; w++) d[w] = R(0);
+#pragma unroll
+            for (int w = 0; w < NV; w++) dv[w] = R(0);
+#pragma unroll
+            for (int r = 0; r <= N; r++) {
+                const R m = Mop<R>(N, r, pos);
+#pragma unroll
+                for (int w = 0; w < 8; w++) d[w] = d[w] + f[w * NF + r] * m;
+#pragma unroll
+                for (int w = 0; w < NV; w++) dv[w] = dv[w] + f[(8 + w) * NF + r] * m;
+            }
+#pragma unroll
+            for (int r = 0; r <= N; r++) e1 = e1 + aa[r] * Mop<R>(N, r, pos) * ss[r * 2 + 0];
+#pragma unroll
+            for (int r = 0; r <= N; r++) e2 = e2 + aa[r] * Mop<R>(N, r, pos) * ss[r * 2 + 1];
+            if (fam == 0) {
+#pragma unroll
+                for (int w = 0; w < 8; w++) dF[w] = d[w];
+#pragma unroll
+                for (int w = 0; w < NV; w++) dFv[w] = dv[w];
+                dA1 = e1; dA2 = e2;
+            } else {
+                const R J = R(jacobian<N>(sX + pc * 8, pis, pjs));
+                const double *qs = sQ + pc * NQ + pk;
+                const R q0 = R(qs[0]), q1 = R(qs[NP]), q2 = R(qs[2 * NP]);
+                const R au = q1 / q0, av = q2 / q0;
+                const R nA1 = (dA1 + e1) / J, nA2 = (dA2 + e2) / J;
+                R res[NV];
+#pragma unroll
+                for (int w = 0; w < 8; w++) res[w] = -(dF[w] + d[w]);
+                res[8] = -(au * nA1) - av * nA2;
+                res[8] = res[8] * J;
+#pragma unroll
+                for (int w = 0; w < NV; w++) res[w] = res[w] + (dFv[w] + dv[w]);
+#pragma unroll
+                for (int w = 0; w < NV; w++) res[w] = res[w] / J;
+                stage_combine(a, (size_t)(c0 + pc) * NQ + pk, NP, qs, res);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace chorus

@@ -21,7 +21,17 @@ from chorus_mhd.solver import Solver
 
 pytestmark = pytest.mark.gpu
 
-IDEAL = [n for n in golden_names() if int(np.load(os.path.join(GOLDEN, n + ".npz"))["artiv"]) == 0]
+ALL = golden_names()
+
+
+def _close(a, b, g):
+    """Bitwise for the ideal path; with AV the sensor's log10 and sin come from
+    the device math library, which may round differently from the host libm in
+    the last bit, so AV cases allow 1e-13 relative to the field's magnitude."""
+    if int(g["artiv"]) == 0:
+        return np.array_equal(a, b)
+    scale = np.abs(b).max(axis=(0, 1, 2))
+    return bool((np.abs(a - b).max(axis=(0, 1, 2)) <= 1e-13 * scale).all())
 
 
 def _say(*a):
@@ -39,28 +49,32 @@ def _build():
     orc.build()
 
 
-@pytest.mark.parametrize("name", IDEAL)
-def test_residual_bitwise_vs_reference(name):
+@pytest.mark.parametrize("name", ALL)
+def test_residual_vs_reference(name):
     _say("case", name)
     g = load_golden(name)
     s = _solver(g)
     s.set_state(g["q0"])
     r = s.compresid()
-    assert np.array_equal(r, g["resid"]), np.abs(r - g["resid"]).max()
+    assert _close(r, g["resid"], g), np.abs(r - g["resid"]).max()
+    if int(g["artiv"]):
+        d = s.diag()
+        assert d["eigvmax"] == float(g["eigvmax"])
+        assert d["epsilon0"] == pytest.approx(float(g["epsilon0"]), rel=1e-15)
 
 
-@pytest.mark.parametrize("name", IDEAL)
-def test_steps_bitwise_vs_reference(name):
+@pytest.mark.parametrize("name", ALL)
+def test_steps_vs_reference(name):
     _say("case", name)
     g = load_golden(name)
     s = _solver(g)
     s.set_state(g["q0"])
     s.step(int(g["steps"]))
     q = s.get_state()
-    assert np.array_equal(q, g["qend"]), np.abs(q - g["qend"]).max()
+    assert _close(q, g["qend"], g), np.abs(q - g["qend"]).max()
 
 
-@pytest.mark.parametrize("name", IDEAL)
+@pytest.mark.parametrize("name", ALL)
 def test_diag_first_iteration_vs_reference(name):
     _say("case", name)
     g = load_golden(name)
@@ -112,6 +126,58 @@ def test_fp64_bitwise_vs_oracle(case):
     assert np.array_equal(s.get_state(), o.get_q())
     o.steps(3)
     s.step(3)
+    assert np.array_equal(s.get_state(), o.get_q())
+
+
+AV_CASES = [
+    (8, 7, 4, 0.10, "CYCL", 3),
+    (7, 6, 3, 0.12, "FREE", 5),
+    (6, 6, 5, 0.10, "CYCL", 3),
+    (4, 5, 6, 0.10, "FREE", 3),
+    (3, 3, 8, 0.05, "CYCL", 3),
+]
+
+
+@pytest.mark.parametrize("case", AV_CASES, ids=lambda c: "n%dx%d_N%d_%s_k%d" % (c[0], c[1], c[2], c[4], c[5]))
+def test_av_fp64_vs_oracle(case):
+    """ARTIV=1 path (k_grad + k_stage_visc) against the oracle on a state with
+    sharp features so the sensor ramp is active in some cells."""
+    n, m, N, jit, bc, kst = case
+    _say("case AV", case)
+    mesh = quad_grid(n, m, 0.0, 2 * math.pi, 0.0, 2 * math.pi, jitter=jit, seed=n * 7 + m, bc=bc)
+    o = orc.Oracle(mesh, N, dt=2e-3, k_stage=kst, artiv=1)
+    Q0 = _smooth_state(o, mesh, N, n * m)
+    Q0[..., 0] *= 1.0 + 0.3 * (np.random.default_rng(1).random(Q0.shape[:3]) > 0.8)
+    Q0[..., 4] += 1.0
+    s = Solver(mesh, N, 2e-3, k_stage=kst, artiv=1)
+    o.set_q(Q0)
+    s.set_state(Q0)
+    r, ro = s.compresid(), o.compresid()
+    tol = 1e-13 * np.abs(ro).max()
+    assert np.abs(r - ro).max() <= tol
+    d = s.diag()
+    assert d["eigvmax"] == o.scalar("eigvmax")
+    mu = o.array("muAV").reshape(mesh.ncell, 9)[:, 0]
+    assert (mu > 0).any(), "the sensor ramp should be active somewhere"
+    o.steps(2)
+    s.step(2)
+    q, qo = s.get_state(), o.get_q()
+    assert np.abs(q - qo).max() <= 1e-13 * np.abs(qo).max()
+
+
+def test_viscous_resistive_fp64_vs_oracle():
+    """mu, eta0 != 0 (getVfluxvectors) with AV off, bitwise."""
+    _say("case viscous")
+    mesh = quad_grid(6, 5, 0.0, 2 * math.pi, 0.0, 2 * math.pi, jitter=0.1, seed=4)
+    N = 4
+    o = orc.Oracle(mesh, N, dt=1e-3, mu=0.01, eta0=0.02)
+    Q0 = _smooth_state(o, mesh, N, 3)
+    s = Solver(mesh, N, 1e-3, mu=0.01, eta0=0.02)
+    o.set_q(Q0)
+    s.set_state(Q0)
+    assert np.array_equal(s.compresid(), o.compresid())
+    o.steps(2)
+    s.step(2)
     assert np.array_equal(s.get_state(), o.get_q())
 
 
