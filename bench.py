@@ -117,7 +117,7 @@ def main():
               "partition on one MI355X (multi-GPU halo exchange is not built)", file=sys.stderr)
         sys.exit(2)
     w = WORKLOADS[args.workload]
-    av_supported = False
+    av_supported = True
     N = w["N"]
 
     def run(precision, Q0=None):
@@ -169,7 +169,8 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "kernel": "k_stage_ideal<%d,double>" % N,
+            "kernel": ("k_grad<%d,double> + k_stage_visc<%d,double>" % (N, N)) if r64["artiv"]
+                      else "k_stage_ideal<%d,double>" % N,
             "stage_kernel_ms": r64["tm"]["ms_stage_avg"], "ct_kernel_ms": r64["tm"]["ms_ct_avg"],
             "bytes_per_dof_stage": b_stage(N),
         },
