@@ -1,0 +1,173 @@
+! iso_c_binding interface of libchorus_mhd.so (include/chorus_mhd.h) for the
+! reference's Fortran driver.
+!
+! The reference's hot path is a set of argument-less subroutines over
+! `module setup2d` (compresid.f90, iterations.f90:92-217,
+! setinitialcond.f90:82-92).  This module keeps that shape: after INIT_SETUP
+! the driver calls chorus_gpu_setup once; each iteration of the time loop calls
+! chorus_gpu_step (stepper + CT on the GPU) instead of RK_S5P3/FiveStageRk and
+! correct_B_via_curl_Az; chorus_gpu_fetch_state copies Q back into setup2d's
+! array before any output.  The state stays resident on the GPU in between.
+module chorus_hip
+    use iso_c_binding
+    implicit none
+    private
+    public :: chorus_gpu_setup, chorus_gpu_step, chorus_gpu_compresid, chorus_gpu_correct_b, &
+              chorus_gpu_fetch_state, chorus_gpu_diag, chorus_gpu_free
+
+    integer(c_int), parameter, public :: CHORUS_FP64 = 0, CHORUS_MIXED = 1
+
+    type, bind(c) :: chorus_mesh_t
+        integer(c_int) :: ncell, nvert, n
+        type(c_ptr) :: xv, yv, ivcell
+        integer(c_int) :: nface
+        type(c_ptr) :: if2c
+        integer(c_int) :: ninter
+        type(c_ptr) :: ibfinter
+        integer(c_int) :: npair
+        type(c_ptr) :: cyc2fpair
+        integer(c_int) :: nfree
+        type(c_ptr) :: ibfree
+    end type
+
+    type, bind(c) :: chorus_params_t
+        integer(c_int) :: vismode, artiv, k_stage
+        real(c_double) :: mu, eta0, gam, dt, dxcycl, dycycl
+        integer(c_int) :: ncellglob, precision
+    end type
+
+    interface
+        integer(c_int) function c_chorus_init(mesh, params, ctx) bind(c, name='chorus_init')
+            import :: c_int, c_ptr, chorus_mesh_t, chorus_params_t
+            type(chorus_mesh_t), intent(in) :: mesh
+            type(chorus_params_t), intent(in) :: params
+            type(c_ptr), intent(out) :: ctx
+        end function
+        subroutine c_chorus_free(ctx) bind(c, name='chorus_free')
+            import :: c_ptr
+            type(c_ptr), value :: ctx
+        end subroutine
+        type(c_ptr) function c_chorus_last_error(ctx) bind(c, name='chorus_last_error')
+            import :: c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function c_chorus_set_state(ctx, q) bind(c, name='chorus_set_state')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(in) :: q(*)
+        end function
+        integer(c_int) function c_chorus_get_state(ctx, q) bind(c, name='chorus_get_state')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: q(*)
+        end function
+        integer(c_int) function c_chorus_compresid(ctx, resid) bind(c, name='chorus_compresid')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: resid(*)
+        end function
+        integer(c_int) function c_chorus_correct_b(ctx) bind(c, name='chorus_correct_b')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function c_chorus_step(ctx, nsteps) bind(c, name='chorus_step')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nsteps
+        end function
+        integer(c_int) function c_chorus_diag(ctx, residglob, divbglob, divbmax, eigvmax, eps0) &
+                bind(c, name='chorus_diag')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: residglob, divbglob, divbmax, eigvmax, eps0
+        end function
+        integer(c_size_t) function c_strlen(s) bind(c, name='strlen')
+            import :: c_size_t, c_ptr
+            type(c_ptr), value :: s
+        end function
+    end interface
+
+    type(c_ptr), save :: ctx = c_null_ptr
+    ! host copies handed to chorus_init (c_loc needs TARGET arrays)
+    real(c_double), allocatable, target, save :: h_xv(:), h_yv(:)
+    integer(c_int), allocatable, target, save :: h_iv(:,:), h_if2c(:,:), h_inter(:), h_cyc(:,:), h_free(:)
+
+contains
+
+    subroutine check(rc, what)
+        integer(c_int), intent(in) :: rc
+        character(*), intent(in) :: what
+        character(kind=c_char), pointer :: msg(:)
+        integer :: n
+        if (rc /= 0) then
+            n = int(c_strlen(c_chorus_last_error(ctx)))
+            call c_f_pointer(c_chorus_last_error(ctx), msg, [n])
+            print *, 'chorus: ', what, ' failed (', rc, '): ', msg(1:n)
+            stop
+        end if
+    end subroutine
+
+    ! INIT_SETUP done: hand setup2d's mesh, connectivity, parameters and Q to the GPU
+    subroutine chorus_gpu_setup(gam, precision)
+        use setup2d, only: NCELL, NVERT, N, XV, YV, IVCELL, NFACE, IF2C, NINTER, IBFINTER, NPAIR, &
+                           CYC2FPAIR, NFREE, IBFREE, vismode, ARTIV, k_stage, mu, eta0, dt, DXCYCL, &
+                           DYCYCL, NCELLGLOB, Q
+        real(c_double), intent(in) :: gam
+        integer(c_int), intent(in) :: precision
+        type(chorus_mesh_t) :: m
+        type(chorus_params_t) :: p
+        h_xv = XV(1:NVERT)
+        h_yv = YV(1:NVERT)
+        h_iv = IVCELL(1:NCELL, 1:4)
+        h_if2c = IF2C(1:NFACE, 1:4)
+        h_inter = IBFINTER(1:NINTER)
+        allocate(h_cyc(max(NPAIR, 1), 2), h_free(max(NFREE, 1)))
+        if (NPAIR > 0) h_cyc(1:NPAIR, :) = CYC2FPAIR(1:NPAIR, :)
+        if (NFREE > 0) h_free(1:NFREE) = IBFREE(1:NFREE)
+        m%ncell = NCELL; m%nvert = NVERT; m%n = N
+        m%xv = c_loc(h_xv); m%yv = c_loc(h_yv); m%ivcell = c_loc(h_iv)
+        m%nface = NFACE; m%if2c = c_loc(h_if2c)
+        m%ninter = NINTER; m%ibfinter = c_loc(h_inter)
+        m%npair = NPAIR; m%cyc2fpair = c_loc(h_cyc)
+        m%nfree = NFREE; m%ibfree = c_loc(h_free)
+        p%vismode = vismode; p%artiv = ARTIV; p%k_stage = k_stage
+        p%mu = mu; p%eta0 = eta0; p%gam = gam; p%dt = dt
+        p%dxcycl = DXCYCL; p%dycycl = DYCYCL; p%ncellglob = NCELLGLOB; p%precision = precision
+        call check(c_chorus_init(m, p, ctx), 'chorus_init')
+        call check(c_chorus_set_state(ctx, Q), 'chorus_set_state')
+    end subroutine
+
+    ! loop body of iterations.f90:22-33 on the GPU: RK_S5P3 / FiveStageRk + correct_B_via_curl_Az
+    subroutine chorus_gpu_step(nsteps)
+        integer, intent(in) :: nsteps
+        call check(c_chorus_step(ctx, int(nsteps, c_int)), 'chorus_step')
+    end subroutine
+
+    ! compresid (compresid.f90:1-67) of the resident state, into setup2d's resid
+    subroutine chorus_gpu_compresid()
+        use setup2d, only: resid
+        call check(c_chorus_compresid(ctx, resid), 'chorus_compresid')
+    end subroutine
+
+    subroutine chorus_gpu_correct_b()
+        call check(c_chorus_correct_b(ctx), 'chorus_correct_b')
+    end subroutine
+
+    ! copy the resident state into setup2d's Q (before Tecplot/restart output)
+    subroutine chorus_gpu_fetch_state()
+        use setup2d, only: Q
+        call check(c_chorus_get_state(ctx, Q), 'chorus_get_state')
+    end subroutine
+
+    ! residual and div B norms of iterations.f90:66-67, plus max |div B|, eigvmax, epsilon0
+    subroutine chorus_gpu_diag(residglob, divbglob, divbmax, eigvmax, eps0)
+        real(c_double), intent(out) :: residglob, divbglob, divbmax, eigvmax, eps0
+        call check(c_chorus_diag(ctx, residglob, divbglob, divbmax, eigvmax, eps0), 'chorus_diag')
+    end subroutine
+
+    subroutine chorus_gpu_free()
+        call c_chorus_free(ctx)
+        ctx = c_null_ptr
+    end subroutine
+
+end module chorus_hip

@@ -1,0 +1,43 @@
+"""The Fortran drop-in end to end: the reference's own set-up chain, restart
+reader/writer and loop structure, with the time step on the GPU through the
+iso_c_binding module (chorus-mhd_amd/fortran).  Its restart dump after k steps
+must equal the compiled reference's state (golden fixtures)."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import REPO, golden_names, load_golden
+
+from chorus_mhd import QuadInp, read_restart, write_restart
+
+pytestmark = pytest.mark.gpu
+
+DROPIN = os.path.join(REPO, "chorus-mhd_amd", "fortran", "_build", "sd2d_mhd_hip")
+
+
+@pytest.mark.skipif(not os.path.exists(DROPIN), reason="drop-in driver not built (needs the reference sources)")
+@pytest.mark.parametrize("name", golden_names())
+def test_fortran_dropin_matches_reference(name, tmp_path):
+    print("case dropin", name, flush=True)
+    g = load_golden(name)
+    mesh, N, steps = g["mesh"], int(g["N"]), int(g["steps"])
+    d = str(tmp_path)
+    mesh.write(d)
+    write_restart(os.path.join(d, "restart.in"), 0, 0.0, g["q0"])
+    QuadInp(N=N, dt=float(g["dt"]), MAXITER=steps, DXCYCL=mesh.dxcycl, DYCYCL=mesh.dycycl,
+            ARTIV=int(g["artiv"]), k_stage=int(g["k_stage"]), restart=1, restart_ord=N,
+            NAMERESTART="restart.in", nwrite=1000000).write(os.path.join(d, "QUAD.INP"))
+    r = subprocess.run([DROPIN], cwd=d, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    it, t, Q = read_restart(os.path.join(d, "restart.%08d.dat" % steps), N)
+    assert it == steps
+    if int(g["artiv"]) == 0:
+        assert np.array_equal(Q, g["qend"])
+    else:
+        assert np.abs(Q - g["qend"]).max() <= 1e-13 * np.abs(g["qend"]).max()
+    lines = open(os.path.join(d, "resid.out")).read().split("\n")
+    row = lines[1].split()
+    assert int(row[0]) == 1
