@@ -28,6 +28,7 @@ struct chorus_ctx {
     double *partial = nullptr;     // per-block reduction partials
     void *qvf = nullptr;           // viscous path: face-averaged Qv
     void *gtr = nullptr;           // viscous path: own nablaQs face traces
+    void *gsol = nullptr;          // viscous path: nablaQs at the solution points
     double *sm = nullptr;          // viscous path: AV indicator per cell
     double *eig = nullptr;         // two slots of the per-stage maximum wave speed
     int stage_parity = 0;          // slot used by the next stage
@@ -89,6 +90,7 @@ int launch_stage(chorus_ctx *c, const StageArgs &a) {
         v.Qin = a.Qin;
         v.qvf = c->qvf;
         v.gtr = c->gtr;
+        v.gsol = c->gsol;
         v.sm = c->sm;
         v.eig = c->eig + c->stage_parity;
         v.eig_next = c->eig + (1 - c->stage_parity);
@@ -455,6 +457,7 @@ int chorus_init(const chorus_mesh *m, const chorus_params *p, chorus_ctx **out) 
         const size_t nf = (size_t)c->ncell * 4 * c->N;
         HIP_TRY(c, hipMalloc(&c->qvf, nf * 8 * sizeof(double)));
         HIP_TRY(c, hipMalloc(&c->gtr, nf * 16 * sizeof(double)));
+        HIP_TRY(c, hipMalloc(&c->gsol, (size_t)c->ncell * 16 * c->N * c->N * sizeof(double)));
         HIP_TRY(c, hipMalloc(&c->sm, (size_t)c->ncell * sizeof(double)));
     }
     HIP_TRY(c, hipMemcpyAsync(c->cellxy, xy.data(), xy.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -474,6 +477,7 @@ void chorus_free(chorus_ctx *c) {
     (void)hipFree(c->eig);
     (void)hipFree(c->qvf);
     (void)hipFree(c->gtr);
+    (void)hipFree(c->gsol);
     (void)hipFree(c->sm);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;

@@ -40,6 +40,7 @@ struct ViscArgs {
     const double *Qin;
     void *qvf;          // face-averaged Qv   [cell][lf][nfp][8]   (Real)
     void *gtr;          // own nablaQs traces [cell][lf][nfp][16]  (Real)
+    void *gsol;         // nablaQs at the solution points [cell][16][N*N] (Real)
     double *sm;         // log10 smoothness indicator of density per cell
     double *eig;        // this stage's maximum Rusanov wave speed (atomic max)
     double *eig_next;   // the next stage's slot, zeroed by k_stage_visc
@@ -47,6 +48,7 @@ struct ViscArgs {
     double mu, eta0, gam, h_msh, pi;
     template <class R> __host__ __device__ R *qvf_r() const { return (R *)qvf; }
     template <class R> __host__ __device__ R *gtr_r() const { return (R *)gtr; }
+    template <class R> __host__ __device__ R *gsol_r() const { return (R *)gsol; }
 };
 
 struct CTArgs {

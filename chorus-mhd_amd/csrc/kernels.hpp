@@ -194,11 +194,12 @@ template <int N, class R>
 __global__ void __launch_bounds__(256) k_stage_ideal(StageArgs a) {
     using S = Shape<N>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
+    constexpr int FST = 8 * NF + 1;      // per-cell stride of sF (odd: cells land on different banks)
+    constexpr int AST = 3 * NF + 1;      // per-cell stride of sA
     __shared__ double sQ[CPB * NQ];
     __shared__ double sX[CPB * 8];
-    __shared__ R sF[CPB * 8 * NF];       // [c][v][line][r], one family at a time
-    __shared__ R sA[CPB * NF];           // [c][line][r]
-    __shared__ R sS[CPB * NF * 2];       // [c][line][r][2]
+    __shared__ R sF[CPB * FST];          // [c][v][line][r] contravariant flux, one family at a time
+    __shared__ R sA[CPB * AST];          // [c][A_z | metric 1 | metric 2][line][r]
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = blk * CPB;
@@ -221,70 +222,82 @@ __global__ void __launch_bounds__(256) k_stage_ideal(StageArgs a) {
 
 #pragma unroll 1
     for (int fam = 0; fam < 2; fam++) {
-        // B: flux points of this family
-        for (int it = tid; it < ncl * NF; it += blockDim.x) {
-            const int c = it / NF, k = it % NF, line = k / (N + 1), r = k % (N + 1);
+        // B1: interior flux points, one uniform path per wave
+        // (inviscid_flux_interior, compflux.f90:59-102)
+        for (int it = tid; it < ncl * N * (N - 1); it += blockDim.x) {
+            const int c = it / (N * (N - 1)), k = it % (N * (N - 1));
+            const int line = k / (N - 1), r = 1 + k % (N - 1);
+            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+            double s1d, s2d;
+            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+            R Qown[NV], F[8], G[8];
+            trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown);
+            iflux<R>(R(a.gam), Qown, F, G);
+            const R s1 = R(s1d), s2 = R(s2d);
+            R *f = sF + c * FST + line * (N + 1) + r;
+#pragma unroll
+            for (int v = 0; v < 8; v++) f[v * NF] = F[v] * s1 + G[v] * s2;
+            R *aa = sA + c * AST + line * (N + 1) + r;
+            aa[0] = Qown[8];
+            aa[NF] = s1;
+            aa[2 * NF] = s2;
+        }
+        // B2: face points, one uniform path per wave: both traces and the
+        // Rusanov flux (interfaceflux_all.f90:16-114, cyclocflux_all.f90:15-114,
+        // bcflux.f90:19-73)
+        for (int it = tid; it < ncl * 2 * N; it += blockDim.x) {
+            const int c = it / (2 * N), k = it % (2 * N);
+            const int line = k >> 1, r = (k & 1) ? N : 0;
             const int cell = c0 + c;
             const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
-            const double *xy = sX + c * 8;
             double s1d, s2d;
-            metric_row<N>(xy, fam, ifp, jfp, s1d, s2d);
+            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
             R Qown[NV];
             trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown);
-            R out[8], az;
-            if (r > 0 && r < N) {
-                R F[8], G[8];
-                iflux<R>(R(a.gam), Qown, F, G);
-                R s1 = R(s1d), s2 = R(s2d);
+            const FacePoint fp = resolve_face<N>(a.nbr[(size_t)cell * 4 + (fam == 0 ? (r == 0 ? 3 : 1) : (r == 0 ? 0 : 2))], fam, r, line);
+            R Qnb[NV];
+            R nx, ny;
+            if (fp.free_face) {
 #pragma unroll
-                for (int v = 0; v < 8; v++) out[v] = F[v] * s1 + G[v] * s2;
-                az = Qown[8];
+                for (int v = 0; v < NV; v++) Qnb[v] = Qown[v];
+                nx = R(s1d); ny = R(s2d);
             } else {
-                const FacePoint fp = resolve_face<N>(a.nbr[(size_t)cell * 4 + (fam == 0 ? (r == 0 ? 3 : 1) : (r == 0 ? 0 : 2))], fam, r, line);
-                R Qnb[NV];
-                R nx, ny;
-                if (fp.free_face) {
+                trace9<N, R>(a.Qin + (size_t)fp.nb * NQ, fp.nb_fam, fp.nb_ifp, fp.nb_jfp, Qnb);
+                if (!fp.right) { nx = R(s1d); ny = R(s2d); }
+                else {
+                    double xyn[8];
 #pragma unroll
-                    for (int v = 0; v < NV; v++) Qnb[v] = Qown[v];
-                    nx = R(s1d); ny = R(s2d);
-                } else {
-                    trace9<N, R>(a.Qin + (size_t)fp.nb * NQ, fp.nb_fam, fp.nb_ifp, fp.nb_jfp, Qnb);
-                    if (!fp.right) { nx = R(s1d); ny = R(s2d); }
-                    else {
-                        double xyn[8];
-#pragma unroll
-                        for (int q = 0; q < 8; q++) xyn[q] = a.cellxy[(size_t)fp.nb * 8 + q];
-                        double n1, n2;
-                        metric_row<N>(xyn, fp.nb_fam, fp.nb_ifp, fp.nb_jfp, n1, n2);
-                        nx = R(n1); ny = R(n2);
-                    }
-                }
-                R eig;
-                if (!fp.right) {
-                    rusanov<R>(R(a.gam), Qown, Qnb, nx, ny, fp.sign_l, out, az, eig);
-                } else {
-                    R Fnl[8];
-                    rusanov<R>(R(a.gam), Qnb, Qown, nx, ny, fp.sign_l, Fnl, az, eig);
-                    const R slr = R(fp.sign_l * fp.sign_r);
-#pragma unroll
-                    for (int v = 0; v < 8; v++) out[v] = slr * Fnl[v];
+                    for (int q = 0; q < 8; q++) xyn[q] = a.cellxy[(size_t)fp.nb * 8 + q];
+                    double n1, n2;
+                    metric_row<N>(xyn, fp.nb_fam, fp.nb_ifp, fp.nb_jfp, n1, n2);
+                    nx = R(n1); ny = R(n2);
                 }
             }
-            R *f = sF + c * 8 * NF + line * (N + 1) + r;
+            R out[8], az, eig;
+            if (!fp.right) {
+                rusanov<R>(R(a.gam), Qown, Qnb, nx, ny, fp.sign_l, out, az, eig);
+            } else {
+                R Fnl[8];
+                rusanov<R>(R(a.gam), Qnb, Qown, nx, ny, fp.sign_l, Fnl, az, eig);
+                const R slr = R(fp.sign_l * fp.sign_r);
+#pragma unroll
+                for (int v = 0; v < 8; v++) out[v] = slr * Fnl[v];
+            }
+            R *f = sF + c * FST + line * (N + 1) + r;
 #pragma unroll
             for (int v = 0; v < 8; v++) f[v * NF] = out[v];
-            sA[c * NF + line * (N + 1) + r] = az;
-            sS[(c * NF + line * (N + 1) + r) * 2 + 0] = R(s1d);
-            sS[(c * NF + line * (N + 1) + r) * 2 + 1] = R(s2d);
+            R *aa = sA + c * AST + line * (N + 1) + r;
+            aa[0] = az;
+            aa[NF] = R(s1d);
+            aa[2 * NF] = R(s2d);
         }
         __syncthreads();
         // C: derivative along this family at the thread's solution point
         if (pact) {
             const int line = fam == 0 ? pjs : pis;    // the line through the point
             const int pos = fam == 0 ? pis : pjs;     // position along the line
-            const R *f = sF + pc * 8 * NF + line * (N + 1);
-            const R *aa = sA + pc * NF + line * (N + 1);
-            const R *ss = sS + (pc * NF + line * (N + 1)) * 2;
+            const R *f = sF + pc * FST + line * (N + 1);
+            const R *aa = sA + pc * AST + line * (N + 1);
             R d[8];
 #pragma unroll
             for (int v = 0; v < 8; v++) d[v] = R(0);
@@ -296,9 +309,9 @@ __global__ void __launch_bounds__(256) k_stage_ideal(StageArgs a) {
                 for (int v = 0; v < 8; v++) d[v] = d[v] + f[v * NF + r] * m;
             }
 #pragma unroll
-            for (int r = 0; r <= N; r++) e1 = e1 + aa[r] * Mop<R>(N, r, pos) * ss[r * 2 + 0];
+            for (int r = 0; r <= N; r++) e1 = e1 + aa[r] * Mop<R>(N, r, pos) * aa[NF + r];
 #pragma unroll
-            for (int r = 0; r <= N; r++) e2 = e2 + aa[r] * Mop<R>(N, r, pos) * ss[r * 2 + 1];
+            for (int r = 0; r <= N; r++) e2 = e2 + aa[r] * Mop<R>(N, r, pos) * aa[2 * NF + r];
             if (fam == 0) {
 #pragma unroll
                 for (int v = 0; v < 8; v++) dF[v] = d[v];
@@ -343,7 +356,7 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
     __shared__ double sV[2][CPB * NP];   // variable planes staged (A_z, or Bx and By)
     __shared__ double sX[CPB * 8];
     __shared__ double sF[2][CPB * NF];   // central flux-point values, one family at a time
-    __shared__ double sS[CPB * NF * 2];
+    __shared__ double sS[2][CPB * NF];   // metric rows, same layout
     __shared__ double red[2][256];
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -383,19 +396,19 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
                 }
                 sF[w][c * NF + line * (N + 1) + r] = val;
             }
-            sS[(c * NF + line * (N + 1) + r) * 2 + 0] = s1;
-            sS[(c * NF + line * (N + 1) + r) * 2 + 1] = s2;
+            sS[0][c * NF + line * (N + 1) + r] = s1;
+            sS[1][c * NF + line * (N + 1) + r] = s2;
         }
         __syncthreads();
         if (pact) {
             const int line = fam == 0 ? pjs : pis, pos = fam == 0 ? pis : pjs;
             for (int w = 0; w < nvar; w++) {
                 const double *f = sF[w] + pc * NF + line * (N + 1);
-                const double *ss = sS + (pc * NF + line * (N + 1)) * 2;
                 for (int d = 0; d < 2; d++) {
+                    const double *ss = sS[d] + pc * NF + line * (N + 1);
                     double acc = 0.0;
 #pragma unroll
-                    for (int r = 0; r <= N; r++) acc = acc + f[r] * c_M[N][r * N + pos] * ss[r * 2 + d];
+                    for (int r = 0; r <= N; r++) acc = acc + f[r] * c_M[N][r * N + pos] * ss[r];
                     if (fam == 0) dpsi[w][d] = acc;
                     else out[w][d] = acc;
                 }

@@ -62,14 +62,60 @@ __device__ __forceinline__ void atomic_max_nonneg(double *p, double v) {
 }
 
 // ---------------------------------------------------------------------------
+// Central Qv at flux point (fam, r, line) of cell c (interior: own trace;
+// face: average with the partner's trace, interfaceflux_all.f90:79-91,
+// bcflux.f90:55-61).  Face points also return the Rusanov wave speed.
+// ---------------------------------------------------------------------------
+template <int N, class R>
+__device__ __forceinline__ void grad_face_point(const ViscArgs &a, const double *sQc, const double *xy, int cell,
+                                                int fam, int r, int line, double s1d, double s2d, R *qv,
+                                                double &emax) {
+    constexpr int NQ = NV * N * N;
+    const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+    R Qown[NV];
+    trace9<N, R>(sQc, fam, ifp, jfp, Qown);
+    int lfi, nfp;
+    face_of<N>(fam, r, line, lfi, nfp);
+    const FacePoint fp = resolve_face<N>(a.nbr[(size_t)cell * 4 + lfi], fam, r, line);
+    R Qnb[NV], nx, ny;
+    if (fp.free_face) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) Qnb[v] = Qown[v];
+        nx = R(s1d); ny = R(s2d);
+    } else {
+        trace9<N, R>(a.Qin + (size_t)fp.nb * NQ, fp.nb_fam, fp.nb_ifp, fp.nb_jfp, Qnb);
+        if (!fp.right) { nx = R(s1d); ny = R(s2d); }
+        else {
+            double xyn[8], n1, n2;
+#pragma unroll
+            for (int q = 0; q < 8; q++) xyn[q] = a.cellxy[(size_t)fp.nb * 8 + q];
+            metric_row<N>(xyn, fp.nb_fam, fp.nb_ifp, fp.nb_jfp, n1, n2);
+            nx = R(n1); ny = R(n2);
+        }
+    }
+    // central average (0.5*Qfl + 0.5*Qfr is symmetric in its operands)
+#pragma unroll
+    for (int v = 0; v < 8; v++) qv[v] = R(0.5) * Qown[v] + R(0.5) * Qnb[v];
+    R *dst = a.qvf_r<R>() + (((size_t)cell * 4 + lfi) * N + nfp) * 8;
+#pragma unroll
+    for (int v = 0; v < 8; v++) dst[v] = qv[v];
+    if (a.artiv) {
+        R Fn[8], az, eig;
+        if (!fp.right) rusanov<R>(R(a.gam), Qown, Qnb, nx, ny, fp.sign_l, Fn, az, eig);
+        else rusanov<R>(R(a.gam), Qnb, Qown, nx, ny, fp.sign_l, Fn, az, eig);
+        emax = fmax(emax, double(eig));
+    }
+}
+
+// ---------------------------------------------------------------------------
 template <int N, class R>
 __global__ void __launch_bounds__(128) k_grad(ViscArgs a) {
     using S = VShape<N>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
+    constexpr int VST = 10 * NF + 1;     // per-cell stride: Qv (8 planes) + metric rows (2 planes)
     __shared__ double sQ[CPB * NQ];
     __shared__ double sX[CPB * 8];
-    __shared__ R sQv[CPB * NF * 8];      // [c][line][r][v], one family at a time
-    __shared__ R sS[CPB * NF * 2];
+    __shared__ R sV[CPB * VST];          // [c][v | s1 | s2][line][r], one family at a time
     __shared__ R sG[CPB * 16 * NP];      // nablaQs [c][d*8+v][p]
     __shared__ double sAV[CPB * NP];     // sensor work
     __shared__ double sRed[128];
@@ -86,61 +132,41 @@ __global__ void __launch_bounds__(128) k_grad(ViscArgs a) {
     double emax = 0.0;
 #pragma unroll 1
     for (int fam = 0; fam < 2; fam++) {
-        for (int it = tid; it < ncl * NF; it += blockDim.x) {
-            const int c = it / NF, k = it % NF, line = k / (N + 1), r = k % (N + 1);
-            const int cell = c0 + c;
+        // interior flux points: Qv = own trace
+        for (int it = tid; it < ncl * N * (N - 1); it += blockDim.x) {
+            const int c = it / (N * (N - 1)), k = it % (N * (N - 1));
+            const int line = k / (N - 1), r = 1 + k % (N - 1);
             const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
             double s1d, s2d;
             metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
-            R Qown[NV], qv[8];
+            R Qown[NV];
             trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown);
-            if (r > 0 && r < N) {
+            R *q = sV + c * VST + line * (N + 1) + r;
 #pragma unroll
-                for (int v = 0; v < 8; v++) qv[v] = Qown[v];
-            } else {
-                int lfi, nfp;
-                face_of<N>(fam, r, line, lfi, nfp);
-                const FacePoint fp = resolve_face<N>(a.nbr[(size_t)cell * 4 + lfi], fam, r, line);
-                R Qnb[NV], nx, ny;
-                if (fp.free_face) {
+            for (int v = 0; v < 8; v++) q[v * NF] = Qown[v];
+            q[8 * NF] = R(s1d);
+            q[9 * NF] = R(s2d);
+        }
+        // face points: central average with the partner's trace
+        for (int it = tid; it < ncl * 2 * N; it += blockDim.x) {
+            const int c = it / (2 * N), k = it % (2 * N);
+            const int line = k >> 1, r = (k & 1) ? N : 0;
+            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+            double s1d, s2d;
+            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+            R qv[8];
+            grad_face_point<N, R>(a, sQ + c * NQ, sX + c * 8, c0 + c, fam, r, line, s1d, s2d, qv, emax);
+            R *q = sV + c * VST + line * (N + 1) + r;
 #pragma unroll
-                    for (int v = 0; v < NV; v++) Qnb[v] = Qown[v];
-                    nx = R(s1d); ny = R(s2d);
-                } else {
-                    trace9<N, R>(a.Qin + (size_t)fp.nb * NQ, fp.nb_fam, fp.nb_ifp, fp.nb_jfp, Qnb);
-                    if (!fp.right) { nx = R(s1d); ny = R(s2d); }
-                    else {
-                        double xyn[8], n1, n2;
-#pragma unroll
-                        for (int q = 0; q < 8; q++) xyn[q] = a.cellxy[(size_t)fp.nb * 8 + q];
-                        metric_row<N>(xyn, fp.nb_fam, fp.nb_ifp, fp.nb_jfp, n1, n2);
-                        nx = R(n1); ny = R(n2);
-                    }
-                }
-                // central average (0.5*Qfl + 0.5*Qfr is symmetric in its operands)
-#pragma unroll
-                for (int v = 0; v < 8; v++) qv[v] = R(0.5) * Qown[v] + R(0.5) * Qnb[v];
-                R* dst = a.qvf_r<R>() + (((size_t)cell * 4 + lfi) * N + nfp) * 8;
-#pragma unroll
-                for (int v = 0; v < 8; v++) dst[v] = qv[v];
-                if (a.artiv) {
-                    R Fn[8], az, eig;
-                    if (!fp.right) rusanov<R>(R(a.gam), Qown, Qnb, nx, ny, fp.sign_l, Fn, az, eig);
-                    else rusanov<R>(R(a.gam), Qnb, Qown, nx, ny, fp.sign_l, Fn, az, eig);
-                    emax = fmax(emax, double(eig));
-                }
-            }
-            R *q = sQv + (c * NF + line * (N + 1) + r) * 8;
-#pragma unroll
-            for (int v = 0; v < 8; v++) q[v] = qv[v];
-            sS[(c * NF + line * (N + 1) + r) * 2 + 0] = R(s1d);
-            sS[(c * NF + line * (N + 1) + r) * 2 + 1] = R(s2d);
+            for (int v = 0; v < 8; v++) q[v * NF] = qv[v];
+            q[8 * NF] = R(s1d);
+            q[9 * NF] = R(s2d);
         }
         __syncthreads();
+        // nablaQs (nablaQsp, compflux.f90:107-157)
         if (pact) {
             const int line = fam == 0 ? pjs : pis, pos = fam == 0 ? pis : pjs;
-            const R *q = sQv + (pc * NF + line * (N + 1)) * 8;
-            const R *ss = sS + (pc * NF + line * (N + 1)) * 2;
+            const R *q = sV + pc * VST + line * (N + 1);
             R J = R(0);
             if (fam == 1) J = R(jacobian<N>(sX + pc * 8, pis, pjs));
 #pragma unroll
@@ -149,32 +175,32 @@ __global__ void __launch_bounds__(128) k_grad(ViscArgs a) {
                 for (int v = 0; v < 8; v++) {
                     R acc = R(0);
 #pragma unroll
-                    for (int r = 0; r <= N; r++) acc = acc + q[r * 8 + v] * Mop<R>(N, r, pos) * ss[r * 2 + d];
+                    for (int r = 0; r <= N; r++) acc = acc + q[v * NF + r] * Mop<R>(N, r, pos) * q[(8 + d) * NF + r];
                     if (fam == 0) gp[d * 8 + v] = acc;
                     else sG[(pc * 16 + d * 8 + v) * NP + pk] = (gp[d * 8 + v] + acc) / J;
                 }
         }
         __syncthreads();
     }
+    // nablaQs to HBM for the stage kernel
+    for (int i = tid; i < ncl * 16 * NP; i += blockDim.x) a.gsol_r<R>()[(size_t)c0 * 16 * NP + i] = sG[i];
     // own traces of nablaQs at the four faces (interfaceVISflux, interfaceflux_all.f90:499-536)
-    for (int it = tid; it < ncl * 4 * N; it += blockDim.x) {
-        const int c = it / (4 * N), k = it % (4 * N), lfi = k / N, nfp = k % N;
+    for (int it = tid; it < ncl * 4 * N * 16; it += blockDim.x) {
+        const int c = it / (4 * N * 16), k = it % (4 * N * 16), fpi = k / 16, g = k % 16;
+        const int lfi = fpi / N, nfp = fpi % N;
         int ifp, jfp;
         face2fp<N>(lfi + 1, nfp, ifp, jfp);
         const int fam = lf_family(lfi + 1);
-        R *dst = a.gtr_r<R>() + (((size_t)(c0 + c) * 4 + lfi) * N + nfp) * 16;
-        for (int g = 0; g < 16; g++) {
-            const R *gq = sG + (c * 16 + g) * NP;
-            R acc = R(0);
-            if (fam == 0) {
+        const R *gq = sG + (c * 16 + g) * NP;
+        R acc = R(0);
+        if (fam == 0) {
 #pragma unroll
-                for (int sp = 0; sp < N; sp++) acc = acc + gq[jfp * N + sp] * Lop<R>(N, ifp, sp);
-            } else {
+            for (int sp = 0; sp < N; sp++) acc = acc + gq[jfp * N + sp] * Lop<R>(N, ifp, sp);
+        } else {
 #pragma unroll
-                for (int sp = 0; sp < N; sp++) acc = acc + gq[sp * N + ifp] * Lop<R>(N, jfp, sp);
-            }
-            dst[g] = acc;
+            for (int sp = 0; sp < N; sp++) acc = acc + gq[sp * N + ifp] * Lop<R>(N, jfp, sp);
         }
+        a.gtr_r<R>()[((size_t)(c0 + c) * 4 * N + fpi) * 16 + g] = acc;
     }
     if (a.artiv) {
         // smoothness indicator of density (AV.f90:15-82), FP64 in both forms:
@@ -227,17 +253,63 @@ __device__ __forceinline__ double mu_av(double sm, double eps0, double pi) {
     return eps0 / 2.0 * (1.0 + sin(pi * (sm - s0) / 2.0 / k_range));
 }
 
+// gradient of the 8 variables at flux point (ifp,jfp) of family fam: own trace
+// of nablaQs (nablaQ_interior_FP, compflux.f90:159-196)
+template <int N, class R>
+__device__ __forceinline__ void grad_trace(const R *gq, int fam, int ifp, int jfp, R *g) {
+    constexpr int NP = N * N;
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+#pragma unroll
+        for (int w = 0; w < 8; w++) {
+            R acc = R(0);
+            if (fam == 0) {
+#pragma unroll
+                for (int sp = 0; sp < N; sp++) acc = acc + gq[(d * 8 + w) * NP + jfp * N + sp] * Lop<R>(N, ifp, sp);
+            } else {
+#pragma unroll
+                for (int sp = 0; sp < N; sp++) acc = acc + gq[(d * 8 + w) * NP + sp * N + ifp] * Lop<R>(N, jfp, sp);
+            }
+            g[d * NV + w] = acc;
+        }
+        g[d * NV + 8] = R(0);
+    }
+}
+
+// viscous + AV contravariant flux at one flux point (comp_viscous_flux, compflux.f90:205-256)
+template <int N, class R>
+__device__ __forceinline__ void visc_point(const ViscArgs &v, const double *mu4, int fam, int ifp, int jfp, R s1,
+                                           R s2, const R *qv, const R *g, R *fv) {
+    const R gam = R(v.gam), lam = R(-2.0 / 3.0), prandt = R(0.72);
+    R Fv[NV], Gv[NV];
+    if (v.mu != 0.0 || v.eta0 != 0.0) {
+        vflux<R>(gam, lam, prandt, R(v.mu), R(v.eta0), qv, g, Fv, Gv);
+#pragma unroll
+        for (int w = 0; w < NV; w++) fv[w] = Fv[w] * s1 + Gv[w] * s2;
+    } else {
+#pragma unroll
+        for (int w = 0; w < NV; w++) fv[w] = R(0);
+    }
+    if (v.artiv) {
+        const int lo = fam == 0 ? 3 : 0, hi = fam == 0 ? 1 : 2;   // sides 1 and 2 of the family
+        const double x = fam == 0 ? c_Xf[N][ifp] : c_Xf[N][jfp];
+        const R mu_art = R(mu4[lo] * (1.0 - x) + mu4[hi] * x);
+        vflux<R>(gam, lam, prandt, mu_art, R(0), qv, g, Fv, Gv);
+#pragma unroll
+        for (int w = 0; w < NV; w++) fv[w] = fv[w] + Fv[w] * s1 + Gv[w] * s2;
+    }
+}
+
 // ---------------------------------------------------------------------------
 template <int N, class R>
 __global__ void __launch_bounds__(128) k_stage_visc(StageArgs a, ViscArgs v) {
     using S = VShape<N>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
+    constexpr int UST = 20 * NF + 1;     // per-cell stride: F (8) + Fv (9) + A_z + metric rows (2)
     __shared__ double sQ[CPB * NQ];
     __shared__ double sX[CPB * 8];
-    __shared__ R sG[CPB * 16 * NP];
-    __shared__ R sU[CPB * NF * 17];      // Qv (8) in the gradient phase; F (8) + Fv (9) in the flux phase
-    __shared__ R sA[CPB * NF];
-    __shared__ R sS[CPB * NF * 2];
+    __shared__ R sG[CPB * 16 * NP];      // nablaQs from k_grad
+    __shared__ R sU[CPB * UST];          // [c][plane][line][r], one family at a time
     __shared__ double sMu[CPB * 4];      // muAVfi/fj(1,side) per local face
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -245,6 +317,7 @@ __global__ void __launch_bounds__(128) k_stage_visc(StageArgs a, ViscArgs v) {
     const int ncl = min(CPB, a.ncell - c0);
     for (int i = tid; i < ncl * NQ; i += blockDim.x) sQ[i] = a.Qin[(size_t)c0 * NQ + i];
     for (int i = tid; i < ncl * 8; i += blockDim.x) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
+    for (int i = tid; i < ncl * 16 * NP; i += blockDim.x) sG[i] = v.gsol_r<R>()[(size_t)c0 * 16 * NP + i];
     if (v.artiv) {
         // epsilon0 = eigvmax_g * h / (N-1) (AV.f90:145-148); face averages of muAV(1)
         // (compAVflux/interfaceAVflux/cyclocAVflux, AV.f90:199-303, 494-564)
@@ -263,105 +336,52 @@ __global__ void __launch_bounds__(128) k_stage_visc(StageArgs a, ViscArgs v) {
     const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
     const bool pact = (tid < CPB * NP) && (pc < ncl);
 
-    // 1. nablaQs from own Q and the stored face averages (same operations as k_grad)
-    {
-        R gp[16];
-#pragma unroll 1
-        for (int fam = 0; fam < 2; fam++) {
-            for (int it = tid; it < ncl * NF; it += blockDim.x) {
-                const int c = it / NF, k = it % NF, line = k / (N + 1), r = k % (N + 1);
-                const int cell = c0 + c;
-                const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
-                double s1d, s2d;
-                metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
-                R *q = sU + (c * NF + line * (N + 1) + r) * 8;
-                if (r > 0 && r < N) {
-                    R Qown[NV];
-                    trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown);
-#pragma unroll
-                    for (int w = 0; w < 8; w++) q[w] = Qown[w];
-                } else {
-                    int lfi, nfp;
-                    face_of<N>(fam, r, line, lfi, nfp);
-                    const R *src = v.qvf_r<R>() + (((size_t)cell * 4 + lfi) * N + nfp) * 8;
-#pragma unroll
-                    for (int w = 0; w < 8; w++) q[w] = src[w];
-                }
-                sS[(c * NF + line * (N + 1) + r) * 2 + 0] = R(s1d);
-                sS[(c * NF + line * (N + 1) + r) * 2 + 1] = R(s2d);
-            }
-            __syncthreads();
-            if (pact) {
-                const int line = fam == 0 ? pjs : pis, pos = fam == 0 ? pis : pjs;
-                const R *q = sU + (pc * NF + line * (N + 1)) * 8;
-                const R *ss = sS + (pc * NF + line * (N + 1)) * 2;
-                R J = R(0);
-                if (fam == 1) J = R(jacobian<N>(sX + pc * 8, pis, pjs));
-#pragma unroll
-                for (int d = 0; d < 2; d++)
-#pragma unroll
-                    for (int w = 0; w < 8; w++) {
-                        R acc = R(0);
-#pragma unroll
-                        for (int r = 0; r <= N; r++) acc = acc + q[r * 8 + w] * Mop<R>(N, r, pos) * ss[r * 2 + d];
-                        if (fam == 0) gp[d * 8 + w] = acc;
-                        else sG[(pc * 16 + d * 8 + w) * NP + pk] = (gp[d * 8 + w] + acc) / J;
-                    }
-            }
-            __syncthreads();
-        }
-    }
-
-    // 2. inviscid + viscous fluxes per family, then the derivatives at the solution points
     R dF[8], dFv[NV], dA1 = R(0), dA2 = R(0);
-    const R gam = R(a.gam), lam = R(-2.0 / 3.0), prandt = R(0.72);
-    const bool visc_mu = v.mu != 0.0 || v.eta0 != 0.0;
 #pragma unroll 1
     for (int fam = 0; fam < 2; fam++) {
-        for (int it = tid; it < ncl * NF; it += blockDim.x) {
-            const int c = it / NF, k = it % NF, line = k / (N + 1), r = k % (N + 1);
-            const int cell = c0 + c;
+        // interior flux points: inviscid + viscous from own data only
+        for (int it = tid; it < ncl * N * (N - 1); it += blockDim.x) {
+            const int c = it / (N * (N - 1)), k = it % (N * (N - 1));
+            const int line = k / (N - 1), r = 1 + k % (N - 1);
             const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
             double s1d, s2d;
             metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
             const R s1 = R(s1d), s2 = R(s2d);
             R Qown[NV];
             trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown);
-            // gradient at this flux point: own trace of nablaQs (nablaQ_interior_FP)
-            R g[2 * NV];
+            R *u = sU + c * UST + line * (N + 1) + r;
             {
-                const R *gq = sG + c * 16 * NP;
-#pragma unroll
-                for (int d = 0; d < 2; d++) {
-#pragma unroll
-                    for (int w = 0; w < 8; w++) {
-                        R acc = R(0);
-                        if (fam == 0) {
-#pragma unroll
-                            for (int sp = 0; sp < N; sp++) acc = acc + gq[(d * 8 + w) * NP + jfp * N + sp] * Lop<R>(N, ifp, sp);
-                        } else {
-#pragma unroll
-                            for (int sp = 0; sp < N; sp++) acc = acc + gq[(d * 8 + w) * NP + sp * N + ifp] * Lop<R>(N, jfp, sp);
-                        }
-                        g[d * NV + w] = acc;
-                    }
-                    g[d * NV + 8] = R(0);
-                }
-            }
-            R out[8], az, qv[NV];
-#pragma unroll
-            for (int w = 0; w < NV; w++) qv[w] = Qown[w];
-            if (r > 0 && r < N) {
                 R F[8], G[8];
-                iflux<R>(gam, Qown, F, G);
+                iflux<R>(R(a.gam), Qown, F, G);
 #pragma unroll
-                for (int w = 0; w < 8; w++) out[w] = F[w] * s1 + G[w] * s2;
-                az = Qown[8];
-            } else {
-                int lfi, nfp;
-                face_of<N>(fam, r, line, lfi, nfp);
-                const FacePoint fp = resolve_face<N>(a.nbr[(size_t)cell * 4 + lfi], fam, r, line);
-                R Qnb[NV], nx, ny;
+                for (int w = 0; w < 8; w++) u[w * NF] = F[w] * s1 + G[w] * s2;
+            }
+            u[17 * NF] = Qown[8];
+            u[18 * NF] = s1;
+            u[19 * NF] = s2;
+            R g[2 * NV], fv[NV];
+            grad_trace<N, R>(sG + c * 16 * NP, fam, ifp, jfp, g);
+            visc_point<N, R>(v, sMu + c * 4, fam, ifp, jfp, s1, s2, Qown, g, fv);
+#pragma unroll
+            for (int w = 0; w < NV; w++) u[(8 + w) * NF] = fv[w];
+        }
+        // face points: Rusanov with the partner's trace, face-averaged Qv and gradient
+        for (int it = tid; it < ncl * 2 * N; it += blockDim.x) {
+            const int c = it / (2 * N), k = it % (2 * N);
+            const int line = k >> 1, r = (k & 1) ? N : 0;
+            const int cell = c0 + c;
+            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+            double s1d, s2d;
+            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+            const R s1 = R(s1d), s2 = R(s2d);
+            int lfi, nfp;
+            face_of<N>(fam, r, line, lfi, nfp);
+            const int2 info = a.nbr[(size_t)cell * 4 + lfi];
+            const FacePoint fp = resolve_face<N>(info, fam, r, line);
+            R *u = sU + c * UST + line * (N + 1) + r;
+            {
+                R Qown[NV], Qnb[NV], nx, ny;
+                trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown);
                 if (fp.free_face) {
 #pragma unroll
                     for (int w = 0; w < NV; w++) Qnb[w] = Qown[w];
@@ -377,69 +397,47 @@ __global__ void __launch_bounds__(128) k_stage_visc(StageArgs a, ViscArgs v) {
                         nx = R(n1); ny = R(n2);
                     }
                 }
-                R eig;
+                R out[8], az, eig;
                 if (!fp.right) {
-                    rusanov<R>(gam, Qown, Qnb, nx, ny, fp.sign_l, out, az, eig);
+                    rusanov<R>(R(a.gam), Qown, Qnb, nx, ny, fp.sign_l, out, az, eig);
                 } else {
                     R Fnl[8];
-                    rusanov<R>(gam, Qnb, Qown, nx, ny, fp.sign_l, Fnl, az, eig);
+                    rusanov<R>(R(a.gam), Qnb, Qown, nx, ny, fp.sign_l, Fnl, az, eig);
                     const R slr = R(fp.sign_l * fp.sign_r);
 #pragma unroll
                     for (int w = 0; w < 8; w++) out[w] = slr * Fnl[w];
                 }
-                // face-averaged Qv and gradient (interfaceVISflux / BCVISFLUX)
-                const R *src = v.qvf_r<R>() + (((size_t)cell * 4 + lfi) * N + nfp) * 8;
 #pragma unroll
-                for (int w = 0; w < 8; w++) qv[w] = src[w];
-                const R *ot = fp.free_face ? nullptr
-                                           : v.gtr_r<R>() + (((size_t)fp.nb * 4 + (a.nbr[(size_t)cell * 4 + lfi].y & 7) - 1) * N +
-                                                             partner_nfp<N>(nfp)) * 16;
-#pragma unroll
-                for (int d = 0; d < 2; d++)
-#pragma unroll
-                    for (int w = 0; w < 8; w++) {
-                        const R mine = g[d * NV + w];
-                        const R other = ot ? ot[d * 8 + w] : mine;
-                        g[d * NV + w] = R(0.5) * mine + R(0.5) * other;
-                    }
+                for (int w = 0; w < 8; w++) u[w * NF] = out[w];
+                u[17 * NF] = az;
             }
-            // viscous flux (comp_viscous_flux, compflux.f90:205-256)
-            R fv[NV];
-            {
-                R Fv[NV], Gv[NV];
-                if (visc_mu) {
-                    vflux<R>(gam, lam, prandt, R(v.mu), R(v.eta0), qv, g, Fv, Gv);
+            u[18 * NF] = s1;
+            u[19 * NF] = s2;
+            // face-averaged Qv and gradient (interfaceVISflux / BCVISFLUX)
+            R qv[NV], g[2 * NV], fv[NV];
+            const R *src = v.qvf_r<R>() + (((size_t)cell * 4 + lfi) * N + nfp) * 8;
 #pragma unroll
-                    for (int w = 0; w < NV; w++) fv[w] = Fv[w] * s1 + Gv[w] * s2;
-                } else {
+            for (int w = 0; w < 8; w++) qv[w] = src[w];
+            qv[8] = R(0);
+            grad_trace<N, R>(sG + c * 16 * NP, fam, ifp, jfp, g);
+            const R *ot = fp.free_face ? nullptr
+                                       : v.gtr_r<R>() + (((size_t)fp.nb * 4 + (info.y & 7) - 1) * N + partner_nfp<N>(nfp)) * 16;
 #pragma unroll
-                    for (int w = 0; w < NV; w++) fv[w] = R(0);
+            for (int d = 0; d < 2; d++)
+#pragma unroll
+                for (int w = 0; w < 8; w++) {
+                    const R mine = g[d * NV + w];
+                    const R other = ot ? ot[d * 8 + w] : mine;
+                    g[d * NV + w] = R(0.5) * mine + R(0.5) * other;
                 }
-                if (v.artiv) {
-                    int lo = fam == 0 ? 3 : 0, hi = fam == 0 ? 1 : 2;   // sides 1 and 2 of the family
-                    const double x = fam == 0 ? c_Xf[N][ifp] : c_Xf[N][jfp];
-                    const R mu_art = R(sMu[c * 4 + lo] * (1.0 - x) + sMu[c * 4 + hi] * x);
-                    vflux<R>(gam, lam, prandt, mu_art, R(0), qv, g, Fv, Gv);
+            visc_point<N, R>(v, sMu + c * 4, fam, ifp, jfp, s1, s2, qv, g, fv);
 #pragma unroll
-                    for (int w = 0; w < NV; w++) fv[w] = fv[w] + Fv[w] * s1 + Gv[w] * s2;
-                }
-            }
-            const int o = c * NF + line * (N + 1) + r;
-            R *f = sU + c * 17 * NF + line * (N + 1) + r;
-#pragma unroll
-            for (int w = 0; w < 8; w++) f[w * NF] = out[w];
-#pragma unroll
-            for (int w = 0; w < NV; w++) f[(8 + w) * NF] = fv[w];
-            sA[o] = az;
-            sS[o * 2 + 0] = s1;
-            sS[o * 2 + 1] = s2;
+            for (int w = 0; w < NV; w++) u[(8 + w) * NF] = fv[w];
         }
         __syncthreads();
         if (pact) {
             const int line = fam == 0 ? pjs : pis, pos = fam == 0 ? pis : pjs;
-            const R *f = sU + pc * 17 * NF + line * (N + 1);
-            const R *aa = sA + pc * NF + line * (N + 1);
-            const R *ss = sS + (pc * NF + line * (N + 1)) * 2;
+            const R *f = sU + pc * UST + line * (N + 1);
             R d[8], dv[NV], e1 = R(0), e2 = R(0);
 #pragma unroll
             for (int w = 0; w < 8; w++) d[w] = R(0);
@@ -454,9 +452,9 @@ __global__ void __launch_bounds__(128) k_stage_visc(StageArgs a, ViscArgs v) {
                 for (int w = 0; w < NV; w++) dv[w] = dv[w] + f[(8 + w) * NF + r] * m;
             }
 #pragma unroll
-            for (int r = 0; r <= N; r++) e1 = e1 + aa[r] * Mop<R>(N, r, pos) * ss[r * 2 + 0];
+            for (int r = 0; r <= N; r++) e1 = e1 + f[17 * NF + r] * Mop<R>(N, r, pos) * f[18 * NF + r];
 #pragma unroll
-            for (int r = 0; r <= N; r++) e2 = e2 + aa[r] * Mop<R>(N, r, pos) * ss[r * 2 + 1];
+            for (int r = 0; r <= N; r++) e2 = e2 + f[17 * NF + r] * Mop<R>(N, r, pos) * f[19 * NF + r];
             if (fam == 0) {
 #pragma unroll
                 for (int w = 0; w < 8; w++) dF[w] = d[w];
