@@ -8,6 +8,14 @@ namespace chorus {
 
 constexpr int kMaxN = 8;       // solution points per direction supported by the kernels
 
+// minimum waves per SIMD requested from the register allocator (launch bounds)
+#ifndef CHORUS_VISC_MINW
+#define CHORUS_VISC_MINW 3
+#endif
+#ifndef CHORUS_IDEAL_MINW
+#define CHORUS_IDEAL_MINW 3
+#endif
+
 // stage combination kinds (FP64 in both precision forms; the association of
 // each expression is the reference's, iterations.f90:92-217)
 enum StageKind : int {

@@ -1,6 +1,7 @@
 // Mixed-precision kernels: FP32 traces, fluxes, Rusanov and derivative
-// contractions; the state, the RK stage sums and the metrics stay FP64.
-#include "kernels_visc.hpp"
+// contractions in per-cell delta form; the state, the RK stage sums, the
+// metrics, the AV indicator and the CT update stay FP64.
+#include "kernels_mixed.hpp"
 
 namespace chorus {
 
@@ -14,7 +15,7 @@ template <int N> static int nblocks(int ncell) { return (ncell + Shape<N>::CPB -
 hipError_t launch_stage_ideal_mixed(int N, const StageArgs &a, hipStream_t s) {
 #define L_(n)                                                                                     \
     case n:                                                                                       \
-        hipLaunchKernelGGL((k_stage_ideal<n, float>), dim3(nblocks<n>(a.ncell)), dim3(Shape<n>::THREADS), 0, s, a); \
+        hipLaunchKernelGGL((k_stage_ideal_mx<n>), dim3(nblocks<n>(a.ncell)), dim3(Shape<n>::THREADS), 0, s, a); \
         return hipGetLastError();
     switch (N) {
         L_(2) L_(3) L_(4) L_(5) L_(6) L_(7) L_(8)
@@ -28,8 +29,8 @@ template <int N> static int vblocks(int ncell) { return (ncell + VShape<N>::CPB 
 hipError_t launch_stage_visc_mixed(int N, const StageArgs &a, const ViscArgs &v, hipStream_t s) {
 #define V_(n)                                                                                         \
     case n:                                                                                           \
-        hipLaunchKernelGGL((k_grad<n, float>), dim3(vblocks<n>(a.ncell)), dim3(VShape<n>::THREADS), 0, s, v); \
-        hipLaunchKernelGGL((k_stage_visc<n, float>), dim3(vblocks<n>(a.ncell)), dim3(VShape<n>::THREADS), 0, s, a, v); \
+        hipLaunchKernelGGL((k_grad_mx<n>), dim3(vblocks<n>(a.ncell)), dim3(VShape<n>::THREADS), 0, s, v); \
+        hipLaunchKernelGGL((k_stage_visc_mx<n>), dim3(vblocks<n>(a.ncell)), dim3(VShape<n>::THREADS), 0, s, a, v); \
         return hipGetLastError();
     switch (N) {
         V_(2) V_(3) V_(4) V_(5) V_(6) V_(7) V_(8)

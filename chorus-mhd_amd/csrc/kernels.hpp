@@ -191,7 +191,7 @@ __device__ __forceinline__ FacePoint resolve_face(int2 info, int fam, int r, int
 // are not evaluated.
 // ---------------------------------------------------------------------------
 template <int N, class R>
-__global__ void __launch_bounds__(256) k_stage_ideal(StageArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CHORUS_IDEAL_MINW))) k_stage_ideal(StageArgs a) {
     using S = Shape<N>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
     constexpr int FST = 8 * NF + 1;      // per-cell stride of sF (odd: cells land on different banks)

@@ -1,0 +1,539 @@
+// Mixed-precision stage kernels: FP32 flux arithmetic in delta form.
+//
+// Plain FP32 flux values of size |F| lose log10(N^2/h) digits when the
+// divergence differences them over a cell.  Here every FP32 quantity on a flux
+// point is a deviation from the cell's FP64 mean state Qbar:
+//   dQf   = sum_sp (Q(sp) - Qbar) L(r,sp)                 (FP64 difference, FP32 MAC)
+//   dF    = F(Qbar + dQf) - F(Qbar)                       (exact difference formulas)
+//   dF1   = dF*S11 + dG*S12                               (contravariant)
+//   face: dF* = 1/2 (dF(Qin)+dF(Qout)).S - 1/2 lambda |S| sigma (dQout - dQin)
+// and the divergence is taken of dF1 only: the constant part F(Qbar).S has a
+// vanishing discrete divergence on straight-sided bilinear cells (the metric
+// identity behind free-stream preservation: d/dpsi(y_eta) - d/deta(y_psi) = 0,
+// exact for the SD derivative of the linear metric terms), so dropping it
+// changes the residual only at FP64 round-off.  The same holds for A_z and for
+// the central Qv of the gradient.  FP32 errors are then relative to O(h)
+// deviations, i.e. eps32*N^2 relative to the residual, independent of h.
+// The state, the RK sums, the metrics, the AV indicator and the CT update stay
+// FP64.
+#pragma once
+
+#include "kernels_visc.hpp"
+
+namespace chorus {
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// reference (mean) state of a cell and the derived quantities the delta fluxes need
+struct Ref {
+    float q[NV];          // Qbar
+    float u[3];           // m/rho
+    float b2, p, H, ub;   // |B|^2, pressure, E + p + |B|^2/2, u.B
+};
+
+template <int N>
+__device__ __forceinline__ void make_ref(const double *q, double gam, Ref &s, double *qbar) {
+    constexpr int NP = N * N;
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        double a = 0.0;
+        for (int p = 0; p < NP; p++) a += q[v * NP + p];
+        qbar[v] = a / NP;
+        s.q[v] = float(qbar[v]);
+    }
+    const double ux = qbar[1] / qbar[0], uy = qbar[2] / qbar[0], uz = qbar[3] / qbar[0];
+    const double b2 = qbar[5] * qbar[5] + qbar[6] * qbar[6] + qbar[7] * qbar[7];
+    const double p = (gam - 1.0) * (qbar[4] - 0.5 * (qbar[1] * ux + qbar[2] * uy + qbar[3] * uz) - 0.5 * b2);
+    s.u[0] = float(ux); s.u[1] = float(uy); s.u[2] = float(uz);
+    s.b2 = float(b2);
+    s.p = float(p);
+    s.H = float(qbar[4] + p + 0.5 * b2);
+    s.ub = float(ux * qbar[5] + uy * qbar[6] + uz * qbar[7]);
+}
+
+// F(Qbar+d) - F(Qbar), G(Qbar+d) - G(Qbar) for the ideal-MHD fluxes of
+// getIfluxvectors (getrusanov.f90:111-153), written as exact differences
+__device__ __forceinline__ void dflux(const Ref &s, const float *d, float gm1, float *dF, float *dG) {
+    const float rho = s.q[0] + d[0];
+    const float ir = frcp(rho);
+    const float du = (d[1] - s.u[0] * d[0]) * ir;
+    const float dv = (d[2] - s.u[1] * d[0]) * ir;
+    const float dw = (d[3] - s.u[2] * d[0]) * ir;
+    const float ux = s.u[0] + du, uy = s.u[1] + dv, uz = s.u[2] + dw;
+    const float bx = s.q[5] + d[5], by = s.q[6] + d[6], bz = s.q[7] + d[7];
+    const float dB2 = (2.f * s.q[5] + d[5]) * d[5] + (2.f * s.q[6] + d[6]) * d[6] + (2.f * s.q[7] + d[7]) * d[7];
+    const float dmu = s.q[1] * du + d[1] * ux + s.q[2] * dv + d[2] * uy + s.q[3] * dw + d[3] * uz;
+    const float dp = gm1 * (d[4] - 0.5f * dmu - 0.5f * dB2);
+    const float dH = d[4] + dp + 0.5f * dB2;
+    const float H = s.H + dH;
+    const float dub = s.u[0] * d[5] + du * bx + s.u[1] * d[6] + dv * by + s.u[2] * d[7] + dw * bz;
+    const float pm = dp + 0.5f * dB2;
+    dF[0] = d[1];
+    dF[1] = s.q[1] * du + d[1] * ux + pm - (2.f * s.q[5] + d[5]) * d[5];
+    dF[2] = s.q[1] * dv + d[1] * uy - (s.q[5] * d[6] + d[5] * by);
+    dF[3] = s.q[1] * dw + d[1] * uz - (s.q[5] * d[7] + d[5] * bz);
+    dF[4] = s.u[0] * dH + du * H - (s.ub * d[5] + dub * bx);
+    dF[5] = 0.f;
+    dF[6] = s.u[0] * d[6] + du * by - (s.u[1] * d[5] + dv * bx);
+    dF[7] = s.u[0] * d[7] + du * bz - (s.u[2] * d[5] + dw * bx);
+    dG[0] = d[2];
+    dG[1] = dF[2];
+    dG[2] = s.q[2] * dv + d[2] * uy + pm - (2.f * s.q[6] + d[6]) * d[6];
+    dG[3] = s.q[2] * dw + d[2] * uz - (s.q[6] * d[7] + d[6] * bz);
+    dG[4] = s.u[1] * dH + dv * H - (s.ub * d[6] + dub * by);
+    dG[5] = -dF[6];
+    dG[6] = 0.f;
+    dG[7] = s.u[1] * d[7] + dv * bz - (s.u[2] * d[6] + dw * by);
+}
+
+// fast-magnetosonic Rusanov wave speed of getrusanov.f90:74-87 from the full
+// FP32 states; nx, ny is a unit normal (orientation irrelevant)
+__device__ __forceinline__ float wave_speed(const float *Ql, const float *Qr, float nx, float ny, float gam) {
+    const float il = frcp(Ql[0]), ir = frcp(Qr[0]);
+    const float ul = Ql[1] * il, vl = Ql[2] * il, wl = Ql[3] * il;
+    const float ur = Qr[1] * ir, vr = Qr[2] * ir, wr = Qr[3] * ir;
+    const float b2l = Ql[5] * Ql[5] + Ql[6] * Ql[6] + Ql[7] * Ql[7];
+    const float b2r = Qr[5] * Qr[5] + Qr[6] * Qr[6] + Qr[7] * Qr[7];
+    const float pl = (gam - 1.f) * (Ql[4] - 0.5f * Ql[0] * (ul * ul + vl * vl + wl * wl) - 0.5f * b2l);
+    const float pr = (gam - 1.f) * (Qr[4] - 0.5f * Qr[0] * (ur * ur + vr * vr + wr * wr) - 0.5f * b2r);
+    const float Bx = 0.5f * (Ql[5] + Qr[5]), By = 0.5f * (Ql[6] + Qr[6]), Bz = 0.5f * (Ql[7] + Qr[7]);
+    const float irs = frcp(Ql[0] + Qr[0]);
+    const float c_a = gam * (pl + pr) * irs;
+    const float c_d = (Bx * Bx + By * By + Bz * Bz) * 2.f * irs;
+    const float bn = Bx * nx + By * ny;
+    const float c_n = bn * bn * 2.f * irs;
+    const float c_tmp = fsqrt(fmaxf((c_a + c_d) * (c_a + c_d) - 4.f * c_a * c_n, 0.f));
+    const float c_f = fsqrt(0.5f * (c_a + c_d + c_tmp));
+    return 0.5f * fabsf((ul + ur) * nx + (vl + vr) * ny) + c_f;
+}
+
+// delta Rusanov flux in this cell's contravariant frame, A_z upwinding as
+// getrusanov.f90:98-103 (by the reference-left cell's normal velocity)
+__device__ __forceinline__ void drusanov(const Ref &s, const float *dqi, const float *dqo, float s1, float s2,
+                                         int sigma, bool right, float gam, float *dfn, float &daz, float &eig) {
+    float dFi[8], dGi[8], dFo[8], dGo[8], Qi[NV], Qo[NV];
+    dflux(s, dqi, gam - 1.f, dFi, dGi);
+    dflux(s, dqo, gam - 1.f, dFo, dGo);
+#pragma unroll
+    for (int v = 0; v < NV; v++) { Qi[v] = s.q[v] + dqi[v]; Qo[v] = s.q[v] + dqo[v]; }
+    const float mag = fsqrt(s1 * s1 + s2 * s2);
+    const float im = frcp(mag);
+    const float nx = s1 * im * float(sigma), ny = s2 * im * float(sigma);   // outward unit normal
+    eig = wave_speed(Qi, Qo, nx, ny, gam);
+    const float diss = 0.5f * eig * mag * float(sigma);
+#pragma unroll
+    for (int v = 0; v < 8; v++)
+        dfn[v] = 0.5f * ((dFi[v] + dFo[v]) * s1 + (dGi[v] + dGo[v]) * s2) - diss * (dqo[v] - dqi[v]);
+    // left cell's outward normal velocity: ours if we are left, the partner's
+    // velocity against our inward normal if we are right
+    const float vn = right ? -(Qo[1] * nx + Qo[2] * ny) * frcp(Qo[0]) : (Qi[1] * nx + Qi[2] * ny) * frcp(Qi[0]);
+    const bool take_left = vn > 0.f;
+    daz = (take_left != right) ? dqi[8] : dqo[8];
+}
+
+// delta trace: sum_sp float(q(sp) - qbar) L(r,sp) of all 9 variables
+template <int N>
+__device__ __forceinline__ void dtrace9(const double *q, const double *qbar, int fam, int ifp, int jfp, float *d) {
+    constexpr int NP = N * N;
+#pragma unroll
+    for (int v = 0; v < NV; v++) d[v] = 0.f;
+#pragma unroll
+    for (int sp = 0; sp < N; sp++) {
+        const float l = fam == 0 ? c_Lf[N][ifp * N + sp] : c_Lf[N][jfp * N + sp];
+        const int idx = fam == 0 ? jfp * N + sp : sp * N + ifp;
+#pragma unroll
+        for (int v = 0; v < NV; v++) d[v] += float(q[v * NP + idx] - qbar[v]) * l;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Ideal-path stage kernel, mixed form
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(256) k_stage_ideal_mx(StageArgs a) {
+    using S = Shape<N>;
+    constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
+    constexpr int FST = 11 * NF + 1;     // per-cell stride: dF1 (8) + dA_z + metric rows (2)
+    __shared__ double sQ[CPB * NQ];
+    __shared__ double sX[CPB * 8];
+    __shared__ double sBar[CPB * NV];
+    __shared__ Ref sRef[CPB];
+    __shared__ float sF[CPB * FST];
+    const int tid = threadIdx.x;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int c0 = blk * CPB;
+    const int ncl = min(CPB, a.ncell - c0);
+    for (int i = tid; i < ncl * NQ; i += blockDim.x) sQ[i] = a.Qin[(size_t)c0 * NQ + i];
+    for (int i = tid; i < ncl * 8; i += blockDim.x) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
+    __syncthreads();
+    if (tid < ncl) make_ref<N>(sQ + tid * NQ, a.gam, sRef[tid], sBar + tid * NV);
+    __syncthreads();
+    const float gam = float(a.gam);
+    const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
+    const bool pact = (tid < CPB * NP) && (pc < ncl);
+    float dF[8], dA1 = 0.f, dA2 = 0.f;
+#pragma unroll 1
+    for (int fam = 0; fam < 2; fam++) {
+        for (int it = tid; it < ncl * N * (N - 1); it += blockDim.x) {
+            const int c = it / (N * (N - 1)), k = it % (N * (N - 1));
+            const int line = k / (N - 1), r = 1 + k % (N - 1);
+            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+            double s1d, s2d;
+            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+            const float s1 = float(s1d), s2 = float(s2d);
+            float d[NV], F[8], G[8];
+            dtrace9<N>(sQ + c * NQ, sBar + c * NV, fam, ifp, jfp, d);
+            dflux(sRef[c], d, gam - 1.f, F, G);
+            float *f = sF + c * FST + line * (N + 1) + r;
+#pragma unroll
+            for (int v = 0; v < 8; v++) f[v * NF] = F[v] * s1 + G[v] * s2;
+            f[8 * NF] = d[8];
+            f[9 * NF] = s1;
+            f[10 * NF] = s2;
+        }
+        for (int it = tid; it < ncl * 2 * N; it += blockDim.x) {
+            const int c = it / (2 * N), k = it % (2 * N);
+            const int line = k >> 1, r = (k & 1) ? N : 0;
+            const int cell = c0 + c;
+            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+            double s1d, s2d;
+            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+            const float s1 = float(s1d), s2 = float(s2d);
+            const int lfi = fam == 0 ? (r == 0 ? 3 : 1) : (r == 0 ? 0 : 2);
+            const FacePoint fp = resolve_face<N>(a.nbr[(size_t)cell * 4 + lfi], fam, r, line);
+            float di[NV], dq[NV];
+            dtrace9<N>(sQ + c * NQ, sBar + c * NV, fam, ifp, jfp, di);
+            if (fp.free_face) {
+#pragma unroll
+                for (int v = 0; v < NV; v++) dq[v] = di[v];
+            } else {
+                dtrace9<N>(a.Qin + (size_t)fp.nb * NQ, sBar + c * NV, fp.nb_fam, fp.nb_ifp, fp.nb_jfp, dq);
+            }
+            float fn[8], daz, eig;
+            drusanov(sRef[c], di, dq, s1, s2, r == N ? 1 : -1, fp.right && !fp.free_face, gam, fn, daz, eig);
+            float *f = sF + c * FST + line * (N + 1) + r;
+#pragma unroll
+            for (int v = 0; v < 8; v++) f[v * NF] = fn[v];
+            f[8 * NF] = daz;
+            f[9 * NF] = s1;
+            f[10 * NF] = s2;
+        }
+        __syncthreads();
+        if (pact) {
+            const int line = fam == 0 ? pjs : pis, pos = fam == 0 ? pis : pjs;
+            const float *f = sF + pc * FST + line * (N + 1);
+            float d[8], e1 = 0.f, e2 = 0.f;
+#pragma unroll
+            for (int v = 0; v < 8; v++) d[v] = 0.f;
+#pragma unroll
+            for (int r = 0; r <= N; r++) {
+                const float m = c_Mf[N][r * N + pos];
+#pragma unroll
+                for (int v = 0; v < 8; v++) d[v] += f[v * NF + r] * m;
+                e1 += f[8 * NF + r] * m * f[9 * NF + r];
+                e2 += f[8 * NF + r] * m * f[10 * NF + r];
+            }
+            if (fam == 0) {
+#pragma unroll
+                for (int v = 0; v < 8; v++) dF[v] = d[v];
+                dA1 = e1; dA2 = e2;
+            } else {
+                const double *qs = sQ + pc * NQ + pk;
+                const float iJ = float(1.0 / jacobian<N>(sX + pc * 8, pis, pjs));
+                const float iq0 = frcp(float(qs[0]));
+                const float au = float(qs[NP]) * iq0, av = float(qs[2 * NP]) * iq0;
+                float res[NV];
+#pragma unroll
+                for (int v = 0; v < 8; v++) res[v] = -(dF[v] + d[v]) * iJ;
+                res[8] = -(au * (dA1 + e1) + av * (dA2 + e2)) * iJ;
+                stage_combine(a, (size_t)(c0 + pc) * NQ + pk, NP, qs, res);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Viscous / AV path, mixed form: k_grad_mx + k_stage_visc_mx (same data flow
+// as k_grad + k_stage_visc; the stored face averages are deviations from the
+// owning cell's mean state)
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(128) k_grad_mx(ViscArgs a) {
+    using S = VShape<N>;
+    constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
+    constexpr int VST = 10 * NF + 1;
+    __shared__ double sQ[CPB * NQ];
+    __shared__ double sX[CPB * 8];
+    __shared__ double sBar[CPB * NV];
+    __shared__ Ref sRef[CPB];
+    __shared__ float sV[CPB * VST];
+    __shared__ float sG[CPB * 16 * NP];
+    __shared__ double sAV[CPB * NP];
+    __shared__ double sRed[128];
+    const int tid = threadIdx.x;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int c0 = blk * CPB;
+    const int ncl = min(CPB, a.ncell - c0);
+    for (int i = tid; i < ncl * NQ; i += blockDim.x) sQ[i] = a.Qin[(size_t)c0 * NQ + i];
+    for (int i = tid; i < ncl * 8; i += blockDim.x) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
+    __syncthreads();
+    if (tid < ncl) make_ref<N>(sQ + tid * NQ, a.gam, sRef[tid], sBar + tid * NV);
+    __syncthreads();
+    const float gam = float(a.gam);
+    const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
+    const bool pact = (tid < CPB * NP) && (pc < ncl);
+    double emax = 0.0;
+#pragma unroll 1
+    for (int fam = 0; fam < 2; fam++) {
+        for (int it = tid; it < ncl * N * (N - 1); it += blockDim.x) {
+            const int c = it / (N * (N - 1)), k = it % (N * (N - 1));
+            const int line = k / (N - 1), r = 1 + k % (N - 1);
+            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+            double s1d, s2d;
+            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+            float d[NV];
+            dtrace9<N>(sQ + c * NQ, sBar + c * NV, fam, ifp, jfp, d);
+            float *q = sV + c * VST + line * (N + 1) + r;
+#pragma unroll
+            for (int v = 0; v < 8; v++) q[v * NF] = d[v];
+            q[8 * NF] = float(s1d);
+            q[9 * NF] = float(s2d);
+        }
+        for (int it = tid; it < ncl * 2 * N; it += blockDim.x) {
+            const int c = it / (2 * N), k = it % (2 * N);
+            const int line = k >> 1, r = (k & 1) ? N : 0;
+            const int cell = c0 + c;
+            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+            double s1d, s2d;
+            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+            int lfi, nfp;
+            face_of<N>(fam, r, line, lfi, nfp);
+            const FacePoint fp = resolve_face<N>(a.nbr[(size_t)cell * 4 + lfi], fam, r, line);
+            float di[NV], dq[NV], qv[8];
+            dtrace9<N>(sQ + c * NQ, sBar + c * NV, fam, ifp, jfp, di);
+            if (fp.free_face) {
+#pragma unroll
+                for (int v = 0; v < NV; v++) dq[v] = di[v];
+            } else {
+                dtrace9<N>(a.Qin + (size_t)fp.nb * NQ, sBar + c * NV, fp.nb_fam, fp.nb_ifp, fp.nb_jfp, dq);
+            }
+#pragma unroll
+            for (int v = 0; v < 8; v++) qv[v] = 0.5f * di[v] + 0.5f * dq[v];
+            float *dst = a.qvf_r<float>() + (((size_t)cell * 4 + lfi) * N + nfp) * 8;
+#pragma unroll
+            for (int v = 0; v < 8; v++) dst[v] = qv[v];
+            if (a.artiv) {
+                float Qi[NV], Qo[NV];
+#pragma unroll
+                for (int v = 0; v < NV; v++) { Qi[v] = sRef[c].q[v] + di[v]; Qo[v] = sRef[c].q[v] + dq[v]; }
+                const float s1 = float(s1d), s2 = float(s2d);
+                const float im = frcp(fsqrt(s1 * s1 + s2 * s2));
+                emax = fmax(emax, double(wave_speed(Qi, Qo, s1 * im, s2 * im, gam)));
+            }
+            float *q = sV + c * VST + line * (N + 1) + r;
+#pragma unroll
+            for (int v = 0; v < 8; v++) q[v * NF] = qv[v];
+            q[8 * NF] = float(s1d);
+            q[9 * NF] = float(s2d);
+        }
+        __syncthreads();
+        if (pact) {
+            const int line = fam == 0 ? pjs : pis, pos = fam == 0 ? pis : pjs;
+            const float *q = sV + pc * VST + line * (N + 1);
+            const float iJ = fam == 1 ? float(1.0 / jacobian<N>(sX + pc * 8, pis, pjs)) : 0.f;
+#pragma unroll
+            for (int d = 0; d < 2; d++)
+#pragma unroll
+                for (int v = 0; v < 8; v++) {
+                    float acc = 0.f;
+#pragma unroll
+                    for (int r = 0; r <= N; r++) acc += q[v * NF + r] * c_Mf[N][r * N + pos] * q[(8 + d) * NF + r];
+                    float *gdst = &sG[(pc * 16 + d * 8 + v) * NP + pk];
+                    if (fam == 0) *gdst = acc;
+                    else *gdst = (*gdst + acc) * iJ;
+                }
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < ncl * 16 * NP; i += blockDim.x) a.gsol_r<float>()[(size_t)c0 * 16 * NP + i] = sG[i];
+    for (int it = tid; it < ncl * 4 * N * 16; it += blockDim.x) {
+        const int c = it / (4 * N * 16), k = it % (4 * N * 16), fpi = k / 16, g = k % 16;
+        const int lfi = fpi / N, nfp = fpi % N;
+        int ifp, jfp;
+        face2fp<N>(lfi + 1, nfp, ifp, jfp);
+        const int fam = lf_family(lfi + 1);
+        const float *gq = sG + (c * 16 + g) * NP;
+        float acc = 0.f;
+        if (fam == 0) {
+#pragma unroll
+            for (int sp = 0; sp < N; sp++) acc += gq[jfp * N + sp] * c_Lf[N][ifp * N + sp];
+        } else {
+#pragma unroll
+            for (int sp = 0; sp < N; sp++) acc += gq[sp * N + ifp] * c_Lf[N][jfp * N + sp];
+        }
+        a.gtr_r<float>()[((size_t)(c0 + c) * 4 * N + fpi) * 16 + g] = acc;
+    }
+    if (a.artiv) av_sensor_and_eigmax<N>(a, sQ, sAV, sRed, ncl, c0, emax);
+}
+
+template <int N>
+__global__ void __launch_bounds__(128) k_stage_visc_mx(StageArgs a, ViscArgs v) {
+    using S = VShape<N>;
+    constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
+    constexpr int UST = 20 * NF + 1;     // dF1 (8) + Fv1 (9) + dA_z + metric rows (2)
+    __shared__ double sQ[CPB * NQ];
+    __shared__ double sX[CPB * 8];
+    __shared__ double sBar[CPB * NV];
+    __shared__ Ref sRef[CPB];
+    __shared__ float sG[CPB * 16 * NP];
+    __shared__ float sU[CPB * UST];
+    __shared__ double sMu[CPB * 4];
+    const int tid = threadIdx.x;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int c0 = blk * CPB;
+    const int ncl = min(CPB, a.ncell - c0);
+    for (int i = tid; i < ncl * NQ; i += blockDim.x) sQ[i] = a.Qin[(size_t)c0 * NQ + i];
+    for (int i = tid; i < ncl * 8; i += blockDim.x) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
+    for (int i = tid; i < ncl * 16 * NP; i += blockDim.x) sG[i] = v.gsol_r<float>()[(size_t)c0 * 16 * NP + i];
+    if (v.artiv) {
+        const double eps0 = *v.eig * v.h_msh / (double)(N - 1) * 1.0;
+        for (int i = tid; i < ncl * 4; i += blockDim.x) {
+            const int c = i / 4, lfi = i % 4, cell = c0 + c;
+            const int2 info = a.nbr[(size_t)cell * 4 + lfi];
+            const double mine = mu_av(v.sm[cell], eps0, v.pi);
+            double val = mine;
+            if (!((info.y >> 4) & 1)) val = 0.5 * (mine + mu_av(v.sm[info.x], eps0, v.pi));
+            sMu[i] = val;
+        }
+        if (blockIdx.x == 0 && tid == 0) *v.eig_next = 0.0;
+    }
+    __syncthreads();
+    if (tid < ncl) make_ref<N>(sQ + tid * NQ, a.gam, sRef[tid], sBar + tid * NV);
+    __syncthreads();
+    const float gam = float(a.gam);
+    const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
+    const bool pact = (tid < CPB * NP) && (pc < ncl);
+    float dF[8], dFv[NV], dA1 = 0.f, dA2 = 0.f;
+#pragma unroll 1
+    for (int fam = 0; fam < 2; fam++) {
+        for (int it = tid; it < ncl * N * (N - 1); it += blockDim.x) {
+            const int c = it / (N * (N - 1)), k = it % (N * (N - 1));
+            const int line = k / (N - 1), r = 1 + k % (N - 1);
+            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+            double s1d, s2d;
+            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+            const float s1 = float(s1d), s2 = float(s2d);
+            float d[NV];
+            dtrace9<N>(sQ + c * NQ, sBar + c * NV, fam, ifp, jfp, d);
+            float *u = sU + c * UST + line * (N + 1) + r;
+            {
+                float F[8], G[8];
+                dflux(sRef[c], d, gam - 1.f, F, G);
+#pragma unroll
+                for (int w = 0; w < 8; w++) u[w * NF] = F[w] * s1 + G[w] * s2;
+            }
+            u[17 * NF] = d[8];
+            u[18 * NF] = s1;
+            u[19 * NF] = s2;
+            float qv[NV], g[2 * NV], fv[NV];
+#pragma unroll
+            for (int w = 0; w < NV; w++) qv[w] = sRef[c].q[w] + d[w];
+            grad_trace<N, float>(sG + c * 16 * NP, fam, ifp, jfp, g);
+            visc_point<N, float>(v, sMu + c * 4, fam, ifp, jfp, s1, s2, qv, g, fv);
+#pragma unroll
+            for (int w = 0; w < NV; w++) u[(8 + w) * NF] = fv[w];
+        }
+        for (int it = tid; it < ncl * 2 * N; it += blockDim.x) {
+            const int c = it / (2 * N), k = it % (2 * N);
+            const int line = k >> 1, r = (k & 1) ? N : 0;
+            const int cell = c0 + c;
+            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+            double s1d, s2d;
+            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+            const float s1 = float(s1d), s2 = float(s2d);
+            int lfi, nfp;
+            face_of<N>(fam, r, line, lfi, nfp);
+            const int2 info = a.nbr[(size_t)cell * 4 + lfi];
+            const FacePoint fp = resolve_face<N>(info, fam, r, line);
+            float *u = sU + c * UST + line * (N + 1) + r;
+            {
+                float di[NV], dq[NV];
+                dtrace9<N>(sQ + c * NQ, sBar + c * NV, fam, ifp, jfp, di);
+                if (fp.free_face) {
+#pragma unroll
+                    for (int w = 0; w < NV; w++) dq[w] = di[w];
+                } else {
+                    dtrace9<N>(a.Qin + (size_t)fp.nb * NQ, sBar + c * NV, fp.nb_fam, fp.nb_ifp, fp.nb_jfp, dq);
+                }
+                float fn[8], daz, eig;
+                drusanov(sRef[c], di, dq, s1, s2, r == N ? 1 : -1, fp.right && !fp.free_face, gam, fn, daz, eig);
+#pragma unroll
+                for (int w = 0; w < 8; w++) u[w * NF] = fn[w];
+                u[17 * NF] = daz;
+            }
+            u[18 * NF] = s1;
+            u[19 * NF] = s2;
+            float qv[NV], g[2 * NV], fv[NV];
+            const float *src = v.qvf_r<float>() + (((size_t)cell * 4 + lfi) * N + nfp) * 8;
+#pragma unroll
+            for (int w = 0; w < 8; w++) qv[w] = sRef[c].q[w] + src[w];
+            qv[8] = 0.f;
+            grad_trace<N, float>(sG + c * 16 * NP, fam, ifp, jfp, g);
+            const float *ot = fp.free_face ? nullptr
+                                           : v.gtr_r<float>() + (((size_t)fp.nb * 4 + (info.y & 7) - 1) * N + partner_nfp<N>(nfp)) * 16;
+#pragma unroll
+            for (int d = 0; d < 2; d++)
+#pragma unroll
+                for (int w = 0; w < 8; w++) {
+                    const float mine = g[d * NV + w];
+                    const float other = ot ? ot[d * 8 + w] : mine;
+                    g[d * NV + w] = 0.5f * mine + 0.5f * other;
+                }
+            visc_point<N, float>(v, sMu + c * 4, fam, ifp, jfp, s1, s2, qv, g, fv);
+#pragma unroll
+            for (int w = 0; w < NV; w++) u[(8 + w) * NF] = fv[w];
+        }
+        __syncthreads();
+        if (pact) {
+            const int line = fam == 0 ? pjs : pis, pos = fam == 0 ? pis : pjs;
+            const float *f = sU + pc * UST + line * (N + 1);
+            float d[8], dv[NV], e1 = 0.f, e2 = 0.f;
+#pragma unroll
+            for (int w = 0; w < 8; w++) d[w] = 0.f;
+#pragma unroll
+            for (int w = 0; w < NV; w++) dv[w] = 0.f;
+#pragma unroll
+            for (int r = 0; r <= N; r++) {
+                const float m = c_Mf[N][r * N + pos];
+#pragma unroll
+                for (int w = 0; w < 8; w++) d[w] += f[w * NF + r] * m;
+#pragma unroll
+                for (int w = 0; w < NV; w++) dv[w] += f[(8 + w) * NF + r] * m;
+                e1 += f[17 * NF + r] * m * f[18 * NF + r];
+                e2 += f[17 * NF + r] * m * f[19 * NF + r];
+            }
+            if (fam == 0) {
+#pragma unroll
+                for (int w = 0; w < 8; w++) dF[w] = d[w];
+#pragma unroll
+                for (int w = 0; w < NV; w++) dFv[w] = dv[w];
+                dA1 = e1; dA2 = e2;
+            } else {
+                const double *qs = sQ + pc * NQ + pk;
+                const float iJ = float(1.0 / jacobian<N>(sX + pc * 8, pis, pjs));
+                const float iq0 = frcp(float(qs[0]));
+                const float au = float(qs[NP]) * iq0, av = float(qs[2 * NP]) * iq0;
+                float res[NV];
+#pragma unroll
+                for (int w = 0; w < 8; w++) res[w] = (-(dF[w] + d[w]) + (dFv[w] + dv[w])) * iJ;
+                res[8] = (-(au * (dA1 + e1) + av * (dA2 + e2)) + (dFv[8] + dv[8])) * iJ;
+                stage_combine(a, (size_t)(c0 + pc) * NQ + pk, NP, qs, res);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace chorus

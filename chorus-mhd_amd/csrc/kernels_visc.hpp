@@ -61,6 +61,53 @@ __device__ __forceinline__ void atomic_max_nonneg(double *p, double v) {
     atomicMax(reinterpret_cast<unsigned long long *>(p), (unsigned long long)__double_as_longlong(v));
 }
 
+// AV smoothness indicator of density (AV.f90:15-82, FP64 in both forms: the
+// N -> N-1 -> N interpolation round trip and log10 of the relative energy) and
+// the block maximum of the face wave speeds folded into the stage maximum with
+// one atomic per block.  Called by all threads of the block.
+template <int N>
+__device__ __forceinline__ void av_sensor_and_eigmax(const ViscArgs &a, const double *sQ, double *sAV, double *sRed,
+                                                     int ncl, int c0, double emax) {
+    constexpr int NP = N * N, NQ = NV * NP, NL = N - 1;
+    const int tid = threadIdx.x;
+    const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
+    const bool pact = (tid < VShape<N>::CPB * NP) && (pc < ncl);
+    double *low = sAV;   // [c][jsl*NL+isl] uses the first NL*NL entries of each cell
+    for (int it = tid; it < ncl * NL * NL; it += blockDim.x) {
+        const int c = it / (NL * NL), k = it % (NL * NL), jsl = k / NL, isl = k % NL;
+        double acc = 0.0;
+        for (int js = 0; js < N; js++)
+            for (int is = 0; is < N; is++)
+                acc = acc + sQ[c * NQ + js * N + is] * c_H1[N][isl * N + is] * c_H1[N][jsl * N + js];
+        low[c * NP + k] = acc;
+    }
+    __syncthreads();
+    double dev = 0.0;
+    if (pact) {
+        double acc = 0.0;
+        for (int jsl = 0; jsl < NL; jsl++)
+            for (int isl = 0; isl < NL; isl++)
+                acc = acc + low[pc * NP + jsl * NL + isl] * c_H2[N][pis * NL + isl] * c_H2[N][pjs * NL + jsl];
+        const double q = sQ[pc * NQ + pk];
+        dev = (acc - q) * (acc - q) / (q * q);
+    }
+    __syncthreads();
+    if (pact) sAV[pc * NP + pk] = dev;
+    __syncthreads();
+    if (tid < ncl) {
+        double se = 0.0;
+        for (int p = 0; p < NP; p++) se = se + sAV[tid * NP + p];
+        a.sm[c0 + tid] = log10(se);
+    }
+    sRed[tid] = emax;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (tid < s) sRed[tid] = fmax(sRed[tid], sRed[tid + s]);
+        __syncthreads();
+    }
+    if (tid == 0) atomic_max_nonneg(a.eig, sRed[0]);
+}
+
 // ---------------------------------------------------------------------------
 // Central Qv at flux point (fam, r, line) of cell c (interior: own trace;
 // face: average with the partner's trace, interfaceflux_all.f90:79-91,
@@ -109,7 +156,7 @@ __device__ __forceinline__ void grad_face_point(const ViscArgs &a, const double 
 
 // ---------------------------------------------------------------------------
 template <int N, class R>
-__global__ void __launch_bounds__(128) k_grad(ViscArgs a) {
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(CHORUS_VISC_MINW))) k_grad(ViscArgs a) {
     using S = VShape<N>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
     constexpr int VST = 10 * NF + 1;     // per-cell stride: Qv (8 planes) + metric rows (2 planes)
@@ -202,46 +249,7 @@ __global__ void __launch_bounds__(128) k_grad(ViscArgs a) {
         }
         a.gtr_r<R>()[((size_t)(c0 + c) * 4 * N + fpi) * 16 + g] = acc;
     }
-    if (a.artiv) {
-        // smoothness indicator of density (AV.f90:15-82), FP64 in both forms:
-        // N -> N-1 -> N interpolation round trip and log10 of the relative energy
-        constexpr int NL = N - 1;
-        double *low = sAV;   // [c][jsl*NL+isl] uses the first NL*NL entries of each cell
-        for (int it = tid; it < ncl * NL * NL; it += blockDim.x) {
-            const int c = it / (NL * NL), k = it % (NL * NL), jsl = k / NL, isl = k % NL;
-            double acc = 0.0;
-            for (int js = 0; js < N; js++)
-                for (int is = 0; is < N; is++)
-                    acc = acc + sQ[c * NQ + js * N + is] * c_H1[N][isl * N + is] * c_H1[N][jsl * N + js];
-            low[c * NP + k] = acc;
-        }
-        __syncthreads();
-        double dev = 0.0;
-        if (pact) {
-            double acc = 0.0;
-            for (int jsl = 0; jsl < NL; jsl++)
-                for (int isl = 0; isl < NL; isl++)
-                    acc = acc + low[pc * NP + jsl * NL + isl] * c_H2[N][pis * NL + isl] * c_H2[N][pjs * NL + jsl];
-            const double q = sQ[pc * NQ + pk];
-            dev = (acc - q) * (acc - q) / (q * q);
-        }
-        __syncthreads();
-        if (pact) sAV[pc * NP + pk] = dev;
-        __syncthreads();
-        if (tid < ncl) {
-            double se = 0.0;
-            for (int p = 0; p < NP; p++) se = se + sAV[tid * NP + p];
-            a.sm[c0 + tid] = log10(se);
-        }
-        // block maximum of the face wave speeds, one atomic per block
-        sRed[tid] = emax;
-        __syncthreads();
-        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-            if (tid < s) sRed[tid] = fmax(sRed[tid], sRed[tid + s]);
-            __syncthreads();
-        }
-        if (tid == 0) atomic_max_nonneg(a.eig, sRed[0]);
-    }
+    if (a.artiv) av_sensor_and_eigmax<N>(a, sQ, sAV, sRed, ncl, c0, emax);
 }
 
 // muAV(1,:) of one cell from its indicator and epsilon0 (AV.f90:120-172)
@@ -302,7 +310,7 @@ __device__ __forceinline__ void visc_point(const ViscArgs &v, const double *mu4,
 
 // ---------------------------------------------------------------------------
 template <int N, class R>
-__global__ void __launch_bounds__(128) k_stage_visc(StageArgs a, ViscArgs v) {
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(CHORUS_VISC_MINW))) k_stage_visc(StageArgs a, ViscArgs v) {
     using S = VShape<N>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
     constexpr int UST = 20 * NF + 1;     // per-cell stride: F (8) + Fv (9) + A_z + metric rows (2)

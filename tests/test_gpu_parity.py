@@ -195,8 +195,6 @@ def test_orszag_tang_64_ten_steps_bitwise():
     assert np.array_equal(s.get_state(), o.get_q())
 
 
-@pytest.mark.xfail(reason="plain FP32 fluxes lose log10(N^2/h) digits in the divergence; the "
-                          "per-cell delta formulation is not built yet", strict=False)
 def test_mixed_precision_error_budget():
     """Mixed form vs FP64 drop-in on a jittered OT mesh: relative L2 <= 1e-6 after
     20 steps; div B of the FP64 CT stays at round-off on an affine mesh."""
@@ -227,6 +225,30 @@ def test_mixed_precision_error_budget():
     mu.step(5)
     d = mu.diag()
     assert d["divbmax"] < 1e-12
+
+
+def test_mixed_precision_av_error_budget():
+    """Mixed form with AV on (k_grad_mx + k_stage_visc_mx) vs the FP64 drop-in."""
+    _say("case mixed AV 32x32 N=5")
+    mesh = quad_grid(32, 32, 0.0, 2 * math.pi, 0.0, 2 * math.pi, jitter=0.1, seed=5)
+    N = 5
+    o = orc.Oracle(mesh, N, dt=5e-4)
+    o.init_ot()
+    Q0 = o.get_q()
+    ref = Solver(mesh, N, 5e-4, artiv=1)
+    mix = Solver(mesh, N, 5e-4, artiv=1, precision="mixed")
+    ref.set_state(Q0)
+    mix.set_state(Q0)
+    r64, r32 = ref.compresid(), mix.compresid()
+    rr = np.linalg.norm(r32 - r64) / np.linalg.norm(r64)
+    _say("mixed AV residual rel L2", rr)
+    assert rr < 1e-5
+    ref.step(20)
+    mix.step(20)
+    a, b = ref.get_state(), mix.get_state()
+    rel = np.linalg.norm(b - a) / np.linalg.norm(a)
+    _say("mixed AV rel L2 after 20 steps", rel)
+    assert rel < 1e-6
 
 
 def test_errors_are_reported():
