@@ -216,16 +216,25 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(CHORUS
             const R *q = sV + pc * VST + line * (N + 1);
             R J = R(0);
             if (fam == 1) J = R(jacobian<N>(sX + pc * 8, pis, pjs));
+            R acc[16];
 #pragma unroll
-            for (int d = 0; d < 2; d++)
+            for (int g = 0; g < 16; g++) acc[g] = R(0);
+#pragma unroll 1
+            for (int r = 0; r <= N; r++) {
+                const R m = Mop<R>(N, r, pos);
+                const R m1 = q[8 * NF + r], m2 = q[9 * NF + r];
 #pragma unroll
                 for (int v = 0; v < 8; v++) {
-                    R acc = R(0);
-#pragma unroll
-                    for (int r = 0; r <= N; r++) acc = acc + q[v * NF + r] * Mop<R>(N, r, pos) * q[(8 + d) * NF + r];
-                    if (fam == 0) gp[d * 8 + v] = acc;
-                    else sG[(pc * 16 + d * 8 + v) * NP + pk] = (gp[d * 8 + v] + acc) / J;
+                    const R t = q[v * NF + r] * m;
+                    acc[v] = acc[v] + t * m1;
+                    acc[8 + v] = acc[8 + v] + t * m2;
                 }
+            }
+#pragma unroll
+            for (int g = 0; g < 16; g++) {
+                if (fam == 0) gp[g] = acc[g];
+                else sG[(pc * 16 + g) * NP + pk] = (gp[g] + acc[g]) / J;
+            }
         }
         __syncthreads();
     }
@@ -269,18 +278,18 @@ __device__ __forceinline__ void grad_trace(const R *gq, int fam, int ifp, int jf
 #pragma unroll
     for (int d = 0; d < 2; d++) {
 #pragma unroll
-        for (int w = 0; w < 8; w++) {
-            R acc = R(0);
-            if (fam == 0) {
-#pragma unroll
-                for (int sp = 0; sp < N; sp++) acc = acc + gq[(d * 8 + w) * NP + jfp * N + sp] * Lop<R>(N, ifp, sp);
-            } else {
-#pragma unroll
-                for (int sp = 0; sp < N; sp++) acc = acc + gq[(d * 8 + w) * NP + sp * N + ifp] * Lop<R>(N, jfp, sp);
-            }
-            g[d * NV + w] = acc;
-        }
+        for (int w = 0; w < 8; w++) g[d * NV + w] = R(0);
         g[d * NV + 8] = R(0);
+    }
+    // contraction index outermost and not unrolled: 16 loads in flight per step
+#pragma unroll 1
+    for (int sp = 0; sp < N; sp++) {
+        const R l = fam == 0 ? Lop<R>(N, ifp, sp) : Lop<R>(N, jfp, sp);
+        const int idx = fam == 0 ? jfp * N + sp : sp * N + ifp;
+#pragma unroll
+        for (int d = 0; d < 2; d++)
+#pragma unroll
+            for (int w = 0; w < 8; w++) g[d * NV + w] = g[d * NV + w] + gq[(d * 8 + w) * NP + idx] * l;
     }
 }
 
@@ -367,13 +376,24 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(CHORUS
             u[17 * NF] = Qown[8];
             u[18 * NF] = s1;
             u[19 * NF] = s2;
-            R g[2 * NV], fv[NV];
+        }
+        // interior flux points: viscous / AV flux (a separate pass keeps the
+        // register footprint of each pass small)
+        for (int it = tid; it < ncl * N * (N - 1); it += blockDim.x) {
+            const int c = it / (N * (N - 1)), k = it % (N * (N - 1));
+            const int line = k / (N - 1), r = 1 + k % (N - 1);
+            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+            double s1d, s2d;
+            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+            R Qown[NV], g[2 * NV], fv[NV];
+            trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown);
             grad_trace<N, R>(sG + c * 16 * NP, fam, ifp, jfp, g);
-            visc_point<N, R>(v, sMu + c * 4, fam, ifp, jfp, s1, s2, Qown, g, fv);
+            visc_point<N, R>(v, sMu + c * 4, fam, ifp, jfp, R(s1d), R(s2d), Qown, g, fv);
+            R *u = sU + c * UST + line * (N + 1) + r;
 #pragma unroll
             for (int w = 0; w < NV; w++) u[(8 + w) * NF] = fv[w];
         }
-        // face points: Rusanov with the partner's trace, face-averaged Qv and gradient
+        // face points: Rusanov with the partner's trace
         for (int it = tid; it < ncl * 2 * N; it += blockDim.x) {
             const int c = it / (2 * N), k = it % (2 * N);
             const int line = k >> 1, r = (k & 1) ? N : 0;
@@ -421,15 +441,30 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(CHORUS
             }
             u[18 * NF] = s1;
             u[19 * NF] = s2;
-            // face-averaged Qv and gradient (interfaceVISflux / BCVISFLUX)
+        }
+        // face points: viscous / AV flux from the face-averaged Qv and gradient
+        // (interfaceVISflux / BCVISFLUX)
+        for (int it = tid; it < ncl * 2 * N; it += blockDim.x) {
+            const int c = it / (2 * N), k = it % (2 * N);
+            const int line = k >> 1, r = (k & 1) ? N : 0;
+            const int cell = c0 + c;
+            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+            double s1d, s2d;
+            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+            const R s1 = R(s1d), s2 = R(s2d);
+            int lfi, nfp;
+            face_of<N>(fam, r, line, lfi, nfp);
+            const int2 info = a.nbr[(size_t)cell * 4 + lfi];
+            const bool free_face = (info.y >> 4) & 1;
+            R *u = sU + c * UST + line * (N + 1) + r;
             R qv[NV], g[2 * NV], fv[NV];
             const R *src = v.qvf_r<R>() + (((size_t)cell * 4 + lfi) * N + nfp) * 8;
 #pragma unroll
             for (int w = 0; w < 8; w++) qv[w] = src[w];
             qv[8] = R(0);
             grad_trace<N, R>(sG + c * 16 * NP, fam, ifp, jfp, g);
-            const R *ot = fp.free_face ? nullptr
-                                       : v.gtr_r<R>() + (((size_t)fp.nb * 4 + (info.y & 7) - 1) * N + partner_nfp<N>(nfp)) * 16;
+            const R *ot = free_face ? nullptr
+                                    : v.gtr_r<R>() + (((size_t)info.x * 4 + (info.y & 7) - 1) * N + partner_nfp<N>(nfp)) * 16;
 #pragma unroll
             for (int d = 0; d < 2; d++)
 #pragma unroll
@@ -451,7 +486,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(CHORUS
             for (int w = 0; w < 8; w++) d[w] = R(0);
 #pragma unroll
             for (int w = 0; w < NV; w++) dv[w] = R(0);
-#pragma unroll
+#pragma unroll 1
             for (int r = 0; r <= N; r++) {
                 const R m = Mop<R>(N, r, pos);
 #pragma unroll
@@ -459,9 +494,9 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(CHORUS
 #pragma unroll
                 for (int w = 0; w < NV; w++) dv[w] = dv[w] + f[(8 + w) * NF + r] * m;
             }
-#pragma unroll
+#pragma unroll 1
             for (int r = 0; r <= N; r++) e1 = e1 + f[17 * NF + r] * Mop<R>(N, r, pos) * f[18 * NF + r];
-#pragma unroll
+#pragma unroll 1
             for (int r = 0; r <= N; r++) e2 = e2 + f[17 * NF + r] * Mop<R>(N, r, pos) * f[19 * NF + r];
             if (fam == 0) {
 #pragma unroll

@@ -224,7 +224,15 @@ def test_mixed_precision_error_budget():
     mu.set_state(ou.get_q())
     mu.step(5)
     d = mu.diag()
-    assert d["divbmax"] < 1e-12
+    ru = Solver(uni, N, 1e-3)
+    ru.set_state(ou.get_q())
+    ru.step(5)
+    d64 = ru.diag()
+    _say("uniform-mesh div B max: mixed", d["divbmax"], "fp64", d64["divbmax"])
+    # B comes from the FP64 CT in both forms: div B stays at the FP64 drop-in's
+    # own round-off level (conditioned by N^4/h^2), far below truncation
+    assert d["divbmax"] <= 10 * d64["divbmax"] + 1e-13
+    assert d["divbmax"] < 1e-9
 
 
 def test_mixed_precision_av_error_budget():
