@@ -41,6 +41,10 @@ WORKLOADS = {
     "ot64": dict(desc="Orszag-Tang 64x64 uniform quads, SD order 3 (N=4), gamma=5/3, SSP(5,3), CFL 0.2, AV off",
                  n=64, m=64, box=(0.0, 2 * math.pi, 0.0, 2 * math.pi), N=4, jitter=0.0, seed=0,
                  bc="CYCL", ic="ot", gam=5.0 / 3, artiv=0),
+    "ot512ideal": dict(desc="Orszag-Tang 512x512 quads, interior nodes jittered +-10% (seed 7), SD order 4 "
+                            "(N=5), gamma=5/3, SSP(5,3), CFL 0.2",
+                       n=512, m=512, box=(0.0, 2 * math.pi, 0.0, 2 * math.pi), N=5, jitter=0.10, seed=7,
+                       bc="CYCL", ic="ot", gam=5.0 / 3, artiv=0),
     "loop2048": dict(desc="field loop 2048x1024 jittered (seed 7), order 4 (N=5), u=(2,1), AV off",
                      n=2048, m=1024, box=(-1.0, 1.0, -0.5, 0.5), N=5, jitter=0.10, seed=7,
                      bc="CYCL", ic="loop", gam=5.0 / 3, artiv=0),

@@ -35,8 +35,8 @@ UNITS = {
 }
 
 
-def _compile(src: str, flags: list) -> str:
-    obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
+def _compile(src: str, flags: list, objdir: str = OBJDIR) -> str:
+    obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
     lang = ["-x", "hip"] if src.endswith(".hip") else []
     if src.endswith(".cpp"):
         lang = ["-D__HIP_PLATFORM_AMD__"]
@@ -50,23 +50,32 @@ def _compile(src: str, flags: list) -> str:
     return obj
 
 
-def build(verbose: bool = False) -> str:
-    """Compile every translation unit for gfx950 and link the shared library."""
+def build(verbose: bool = False, out: str = LIB, defines: tuple = ()) -> str:
+    """Compile every translation unit for gfx950 and link the shared library.
+    ``defines``: extra -D tuning macros (variants go to another ``out`` path)."""
     if not shutil.which(HIPCC) and not os.path.exists(HIPCC):
         raise RuntimeError("hipcc not found (%s)" % HIPCC)
-    os.makedirs(OBJDIR, exist_ok=True)
+    objdir = OBJDIR if out == LIB else os.path.join(OBJDIR, os.path.basename(out))
+    os.makedirs(objdir, exist_ok=True)
+    extra = ["-D" + d for d in defines]
     with ThreadPoolExecutor(max_workers=4) as ex:
-        objs = list(ex.map(lambda kv: _compile(*kv), UNITS.items()))
-    tmp = LIB + ".tmp"
+        objs = list(ex.map(lambda kv: _compile(kv[0], kv[1] + extra, objdir), UNITS.items()))
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    tmp = out + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-4000:]))
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
     if verbose:
-        print("built", LIB)
-    return LIB
+        print("built", out)
+    return out
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=LIB)
+    ap.add_argument("-D", dest="defines", action="append", default=[])
+    args = ap.parse_args()
+    build(verbose=True, out=os.path.abspath(args.out), defines=tuple(args.defines))

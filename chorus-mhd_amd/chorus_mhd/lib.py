@@ -12,7 +12,9 @@ import os
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(PKG), "libchorus_mhd.so")
+# CHORUS_MHD_LIB selects another in-tree build of the same library (e.g. a
+# variant built with different tuning macros for a measurement)
+LIB_PATH = os.environ.get("CHORUS_MHD_LIB") or os.path.join(os.path.dirname(PKG), "libchorus_mhd.so")
 
 CHORUS_FP64 = 0
 CHORUS_MIXED = 1

@@ -15,6 +15,14 @@ constexpr int kMaxN = 8;       // solution points per direction supported by the
 #ifndef CHORUS_IDEAL_MINW
 #define CHORUS_IDEAL_MINW 3
 #endif
+// unroll factor of the contraction index in FP64 traces / gradient traces /
+// derivative sums: full unrolling hoists every LDS load and exhausts the
+// register file in the FP64 kernels (3 measured fastest at N=5 against 1, 2
+// and full); FP32 kernels unroll fully
+#ifndef CHORUS_F64_UNROLL
+#define CHORUS_F64_UNROLL 3
+#endif
+template <class R, int N> constexpr int contraction_unroll() { return sizeof(R) == 8 ? CHORUS_F64_UNROLL : N + 1; }
 
 // stage combination kinds (FP64 in both precision forms; the association of
 // each expression is the reference's, iterations.f90:92-217)

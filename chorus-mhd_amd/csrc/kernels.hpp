@@ -114,7 +114,8 @@ __device__ __forceinline__ void trace9(const double *q, int fam, int ifp, int jf
     constexpr int NP = N * N;
 #pragma unroll
     for (int v = 0; v < NV; v++) Qf[v] = R(0);
-#pragma unroll 1
+    constexpr int U = contraction_unroll<R, N>();
+#pragma unroll U
     for (int sp = 0; sp < N; sp++) {
         const R l = fam == 0 ? Lop<R>(N, ifp, sp) : Lop<R>(N, jfp, sp);
         const int idx = fam == 0 ? jfp * N + sp : sp * N + ifp;
@@ -294,15 +295,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CHORUS
 #pragma unroll
             for (int v = 0; v < 8; v++) d[v] = R(0);
             R e1 = R(0), e2 = R(0);
-#pragma unroll 1
+            constexpr int U = contraction_unroll<R, N>();
+#pragma unroll U
             for (int r = 0; r <= N; r++) {
                 R m = Mop<R>(N, r, pos);
 #pragma unroll
                 for (int v = 0; v < 8; v++) d[v] = d[v] + f[v * NF + r] * m;
             }
-#pragma unroll 1
+#pragma unroll U
             for (int r = 0; r <= N; r++) e1 = e1 + aa[r] * Mop<R>(N, r, pos) * aa[NF + r];
-#pragma unroll 1
+#pragma unroll U
             for (int r = 0; r <= N; r++) e2 = e2 + aa[r] * Mop<R>(N, r, pos) * aa[2 * NF + r];
             if (fam == 0) {
 #pragma unroll

@@ -219,7 +219,8 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(CHORUS
             R acc[16];
 #pragma unroll
             for (int g = 0; g < 16; g++) acc[g] = R(0);
-#pragma unroll 1
+            constexpr int U = contraction_unroll<R, N>();
+#pragma unroll U
             for (int r = 0; r <= N; r++) {
                 const R m = Mop<R>(N, r, pos);
                 const R m1 = q[8 * NF + r], m2 = q[9 * NF + r];
@@ -281,8 +282,9 @@ __device__ __forceinline__ void grad_trace(const R *gq, int fam, int ifp, int jf
         for (int w = 0; w < 8; w++) g[d * NV + w] = R(0);
         g[d * NV + 8] = R(0);
     }
-    // contraction index outermost and not unrolled: 16 loads in flight per step
-#pragma unroll 1
+    // contraction index outermost: 16 loads in flight per step
+    constexpr int U = contraction_unroll<R, N>();
+#pragma unroll U
     for (int sp = 0; sp < N; sp++) {
         const R l = fam == 0 ? Lop<R>(N, ifp, sp) : Lop<R>(N, jfp, sp);
         const int idx = fam == 0 ? jfp * N + sp : sp * N + ifp;
@@ -486,7 +488,8 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(CHORUS
             for (int w = 0; w < 8; w++) d[w] = R(0);
 #pragma unroll
             for (int w = 0; w < NV; w++) dv[w] = R(0);
-#pragma unroll 1
+            constexpr int U = contraction_unroll<R, N>();
+#pragma unroll U
             for (int r = 0; r <= N; r++) {
                 const R m = Mop<R>(N, r, pos);
 #pragma unroll
@@ -494,9 +497,9 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(CHORUS
 #pragma unroll
                 for (int w = 0; w < NV; w++) dv[w] = dv[w] + f[(8 + w) * NF + r] * m;
             }
-#pragma unroll 1
+#pragma unroll U
             for (int r = 0; r <= N; r++) e1 = e1 + f[17 * NF + r] * Mop<R>(N, r, pos) * f[18 * NF + r];
-#pragma unroll 1
+#pragma unroll U
             for (int r = 0; r <= N; r++) e2 = e2 + f[17 * NF + r] * Mop<R>(N, r, pos) * f[19 * NF + r];
             if (fam == 0) {
 #pragma unroll
