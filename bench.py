@@ -115,6 +115,9 @@ def main():
     ap.add_argument("--workload", default="ot512", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision form")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run one device-to-device copy of 256 MiB (known HBM bytes) so that "
+                         "hardware byte counters collected over this run can be scaled")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: this build holds the whole mesh as one "
@@ -122,6 +125,13 @@ def main():
         sys.exit(2)
     w = WORKLOADS[args.workload]
     av_supported = True
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(1 << 25, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     N = w["N"]
 
     def run(precision, Q0=None):

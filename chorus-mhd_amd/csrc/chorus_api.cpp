@@ -26,6 +26,11 @@ struct chorus_ctx {
     double *X[4] = {nullptr, nullptr, nullptr, nullptr};
     double *R = nullptr;           // residual output / FiveStageRk resid0
     double *partial = nullptr;     // per-block reduction partials
+    int nface = 0;                 // faces carrying a flux (interior, CYCL pairs, FREE)
+    int4 *faces = nullptr;         // {cellL, lfiL, cellR (-1: FREE), lfiR} ordered by left cell
+    double *TQ = nullptr;          // traces of the current state at the face points
+    void *FF = nullptr;            // per cell face point: common inviscid flux + A_z
+    void *FFv = nullptr;           // viscous path: per cell face point: common viscous flux
     void *qvf = nullptr;           // viscous path: face-averaged Qv
     void *gtr = nullptr;           // viscous path: own nablaQs face traces
     void *gsol = nullptr;          // viscous path: nablaQs at the solution points
@@ -75,13 +80,33 @@ void from_device_layout(const chorus_ctx *c, const std::vector<double> &in, doub
 
 bool g_ops_uploaded = false;
 
+FaceArgs face_args(chorus_ctx *c) {
+    FaceArgs f{};
+    f.nface = c->nface;
+    f.faces = c->faces;
+    f.cellxy = c->cellxy;
+    f.TQ = c->TQ;
+    f.FF = c->FF;
+    f.FFv = c->FFv;
+    f.gtr = c->gtr;
+    f.sm = c->sm;
+    f.artiv = c->p.artiv;
+    f.gam = c->p.gam;
+    f.mu = c->p.mu;
+    f.eta0 = c->p.eta0;
+    f.h_msh = c->h_msh;
+    f.pi = std::atan(1.0) * 4.0;
+    return f;
+}
+
 int launch_stage(chorus_ctx *c, const StageArgs &a) {
     hipError_t e;
     bool t = c->timing && c->ev_used + 2 <= c->ev.size();
     if (t) (void)hipEventRecord(c->ev[c->ev_used], c->stream);
+    FaceArgs f = face_args(c);
     if (c->ideal) {
         if (c->p.precision == CHORUS_MIXED) e = launch_stage_ideal_mixed(c->N, a, c->stream);
-        else e = launch_stage_ideal_fp64(c->N, a, c->stream);
+        else e = launch_stage_ideal_fp64(c->N, a, f, c->stream);
     } else {
         ViscArgs v{};
         v.ncell = c->ncell;
@@ -94,16 +119,19 @@ int launch_stage(chorus_ctx *c, const StageArgs &a) {
         v.sm = c->sm;
         v.eig = c->eig + c->stage_parity;
         v.eig_next = c->eig + (1 - c->stage_parity);
+        v.TQ = c->TQ;
         v.artiv = c->p.artiv;
         v.mu = c->p.mu;
         v.eta0 = c->p.eta0;
         v.gam = c->p.gam;
         v.h_msh = c->h_msh;
         v.pi = std::atan(1.0) * 4.0;
+        f.eig = v.eig;
+        f.eig_now = v.eig;
         c->last_parity = c->stage_parity;
         c->stage_parity = 1 - c->stage_parity;
         if (c->p.precision == CHORUS_MIXED) e = launch_stage_visc_mixed(c->N, a, v, c->stream);
-        else e = launch_stage_visc_fp64(c->N, a, v, c->stream);
+        else e = launch_stage_visc_fp64(c->N, a, v, f, c->stream);
     }
     if (t) { (void)hipEventRecord(c->ev[c->ev_used + 1], c->stream); c->ev_used += 2; }
     if (e != hipSuccess) return fail(c, CHORUS_ERR_HIP, std::string("stage launch: ") + hipGetErrorString(e));
@@ -116,7 +144,16 @@ StageArgs base_args(chorus_ctx *c) {
     a.cellxy = c->cellxy;
     a.nbr = c->nbr;
     a.gam = c->p.gam;
+    a.FF = c->FF;
+    a.FFv = c->FFv;
+    a.TQ = c->TQ;
     return a;
+}
+
+int run_trace(chorus_ctx *c, const double *Q) {
+    hipError_t e = launch_trace(c->N, c->ncell, Q, c->TQ, c->stream);
+    if (e != hipSuccess) return fail(c, CHORUS_ERR_HIP, std::string("trace launch: ") + hipGetErrorString(e));
+    return 0;
 }
 
 int run_ct(chorus_ctx *c, double *Q, int which) {
@@ -127,6 +164,8 @@ int run_ct(chorus_ctx *c, double *Q, int which) {
     a.Q = Q;
     a.which = which;
     a.partial = c->partial;
+    a.TQ = c->TQ;
+    a.TQout = c->TQ;
     hipError_t e = launch_ct(c->N, a, c->stream);
     if (e != hipSuccess) return fail(c, CHORUS_ERR_HIP, std::string("ct launch: ") + hipGetErrorString(e));
     return 0;
@@ -426,6 +465,15 @@ int chorus_init(const chorus_mesh *m, const chorus_params *p, chorus_ctx **out) 
             return fail(c, CHORUS_ERR_TOPOLOGY,
                         "cell " + std::to_string(i / 4 + 1) + " face " + std::to_string(i % 4 + 1) +
                             " has no flux: only interior, CYCL and FREE faces carry fluxes (connect_bdry.f90:88-110)");
+    // face list for the face-ordered passes: each face once, from its left cell
+    std::vector<int4> faces;
+    faces.reserve((size_t)m->ncell * 2 + 16);
+    for (int ic = 0; ic < m->ncell; ic++)
+        for (int lfi = 0; lfi < 4; lfi++) {
+            const int2 in = nbr[(size_t)ic * 4 + lfi];
+            if ((in.y >> 4) & 1) faces.push_back(make_int4(ic, lfi, -1, 0));
+            else if (!((in.y >> 3) & 1)) faces.push_back(make_int4(ic, lfi, in.x, (in.y & 7) - 1));
+        }
     std::vector<double> xy((size_t)m->ncell * 8);
     for (int ic = 0; ic < m->ncell; ic++)
         for (int k = 0; k < 4; k++) {
@@ -453,8 +501,15 @@ int chorus_init(const chorus_mesh *m, const chorus_params *p, chorus_ctx **out) 
     HIP_TRY(c, hipMalloc(&c->partial, (size_t)c->nblk * 2 * sizeof(double)));
     HIP_TRY(c, hipMalloc(&c->eig, 2 * sizeof(double)));
     HIP_TRY(c, hipMemsetAsync(c->eig, 0, 2 * sizeof(double), c->stream));
+    c->nface = (int)faces.size();
+    const size_t nfp = (size_t)c->ncell * 4 * c->N;
+    HIP_TRY(c, hipMalloc(&c->faces, faces.size() * sizeof(int4)));
+    HIP_TRY(c, hipMalloc(&c->TQ, nfp * 9 * sizeof(double)));
+    HIP_TRY(c, hipMalloc(&c->FF, nfp * 9 * sizeof(double)));
+    if (!c->ideal) HIP_TRY(c, hipMalloc(&c->FFv, nfp * 9 * sizeof(double)));
+    HIP_TRY(c, hipMemcpyAsync(c->faces, faces.data(), faces.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
     if (!c->ideal) {
-        const size_t nf = (size_t)c->ncell * 4 * c->N;
+        const size_t nf = nfp;
         HIP_TRY(c, hipMalloc(&c->qvf, nf * 8 * sizeof(double)));
         HIP_TRY(c, hipMalloc(&c->gtr, nf * 16 * sizeof(double)));
         HIP_TRY(c, hipMalloc(&c->gsol, (size_t)c->ncell * 16 * c->N * c->N * sizeof(double)));
@@ -475,6 +530,10 @@ void chorus_free(chorus_ctx *c) {
     (void)hipFree(c->R);
     (void)hipFree(c->partial);
     (void)hipFree(c->eig);
+    (void)hipFree(c->faces);
+    (void)hipFree(c->TQ);
+    (void)hipFree(c->FF);
+    (void)hipFree(c->FFv);
     (void)hipFree(c->qvf);
     (void)hipFree(c->gtr);
     (void)hipFree(c->gsol);
@@ -488,6 +547,8 @@ int chorus_set_state(chorus_ctx *c, const double *q) {
     std::vector<double> h;
     to_device_layout(c, q, h);
     HIP_TRY(c, hipMemcpyAsync(c->X[c->cur], h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    int rc = run_trace(c, c->X[c->cur]);
+    if (rc) return rc;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->last_in = -1;
     return 0;
@@ -551,7 +612,11 @@ int chorus_diag(chorus_ctx *c, double *residglob, double *divbglob, double *divb
             s.Qin = c->X[c->last_in];
             s.Rout = c->R;
             s.kind = KIND_RESID;
-            int rc = launch_stage(c, s);
+            // the face passes read the traces of that state; those of the
+            // current state are restored afterwards
+            int rc = run_trace(c, s.Qin);
+            if (!rc) rc = launch_stage(c, s);
+            if (!rc) rc = run_trace(c, c->X[c->cur]);
             if (rc) return rc;
             std::vector<double> h(state_len(c));
             HIP_TRY(c, hipMemcpyAsync(h.data(), c->R, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));

@@ -10,6 +10,7 @@ hipError_t upload_operators_fp64() {
 }
 
 template <int N> static int nblocks(int ncell) { return (ncell + Shape<N>::CPB - 1) / Shape<N>::CPB; }
+template <class S> static int blocks_of(int ncell) { return (ncell + S::CPB - 1) / S::CPB; }
 
 #define CHORUS_DISPATCH_N(N, CALL) \
     switch (N) {                   \
@@ -23,17 +24,31 @@ template <int N> static int nblocks(int ncell) { return (ncell + Shape<N>::CPB -
     default: return hipErrorInvalidValue; \
     }
 
-hipError_t launch_stage_ideal_fp64(int N, const StageArgs &a, hipStream_t s) {
+static int fblocks(int N, int nface) { return (nface * N + 255) / 256; }
+
+hipError_t launch_stage_ideal_fp64(int N, const StageArgs &a, const FaceArgs &f, hipStream_t s) {
+#define L_(n)                                                                                        \
+    hipLaunchKernelGGL((k_face_inv<n, double>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);      \
+    hipLaunchKernelGGL((k_stage_ideal<n, double>), dim3(blocks_of<IShape<n>>(a.ncell)), dim3(IShape<n>::THREADS), 0, s, a); \
+    return hipGetLastError();
+    CHORUS_DISPATCH_N(N, L_)
+#undef L_
+}
+
+hipError_t launch_trace(int N, int ncell, const double *Q, double *TQ, hipStream_t s) {
 #define L_(n) \
-    hipLaunchKernelGGL((k_stage_ideal<n, double>), dim3(nblocks<n>(a.ncell)), dim3(Shape<n>::THREADS), 0, s, a); \
+    hipLaunchKernelGGL((k_trace<n>), dim3(nblocks<n>(ncell)), dim3(Shape<n>::THREADS), 0, s, ncell, Q, TQ); \
     return hipGetLastError();
     CHORUS_DISPATCH_N(N, L_)
 #undef L_
 }
 
 hipError_t launch_ct(int N, const CTArgs &a, hipStream_t s) {
-#define L_(n) \
-    hipLaunchKernelGGL((k_ct<n>), dim3(nblocks<n>(a.ncell)), dim3(Shape<n>::THREADS), 0, s, a); \
+#define L_(n)                                                                                            \
+    if (a.which == 0)                                                                                    \
+        hipLaunchKernelGGL((k_ct<n, 0>), dim3(nblocks<n>(a.ncell)), dim3(Shape<n>::THREADS), 0, s, a); \
+    else                                                                                                 \
+        hipLaunchKernelGGL((k_ct<n, 1>), dim3(nblocks<n>(a.ncell)), dim3(Shape<n>::THREADS), 0, s, a); \
     return hipGetLastError();
     CHORUS_DISPATCH_N(N, L_)
 #undef L_
@@ -55,10 +70,12 @@ int ct_blocks(int N, int ncell) { return stage_blocks(N, ncell); }
 
 template <int N> static int vblocks(int ncell) { return (ncell + VShape<N>::CPB - 1) / VShape<N>::CPB; }
 
-hipError_t launch_stage_visc_fp64(int N, const StageArgs &a, const ViscArgs &v, hipStream_t s) {
+hipError_t launch_stage_visc_fp64(int N, const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s) {
 #define V_(n)                                                                                         \
     case n:                                                                                           \
+        hipLaunchKernelGGL((k_face_inv<n, double>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);    \
         hipLaunchKernelGGL((k_grad<n, double>), dim3(vblocks<n>(a.ncell)), dim3(VShape<n>::THREADS), 0, s, v); \
+        hipLaunchKernelGGL((k_face_visc<n, double>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);   \
         hipLaunchKernelGGL((k_stage_visc<n, double>), dim3(vblocks<n>(a.ncell)), dim3(VShape<n>::THREADS), 0, s, a, v); \
         return hipGetLastError();
     switch (N) {

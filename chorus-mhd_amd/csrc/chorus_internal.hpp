@@ -46,6 +46,35 @@ struct StageArgs {
     int kind;
     double c0, c1, c2, c3, c4, c5;
     double gam;
+    const void *FF;     // common face fluxes of each cell's face points (k_face_inv)
+    const void *FFv;    // viscous face fluxes (k_face_visc)
+    double *TQ;         // traces of Qout at the face points, written by the stage kernel
+};
+
+// Face-ordered pass: one thread per face point.
+//   faces[f] = {cellL, lfiL, cellR, lfiR}: the reference's left and right cell
+//   of face f with their local face index 0..3; cellR = -1 on a FREE face.
+//   Face point k of face f is point k of the left cell's face numbering and
+//   point N-1-k of the right cell's.
+// Per cell, local face lfi and own face point nfp (face2fpconnec order):
+//   TQ [cell][lfi][nfp][9]   trace of Q (FP64)
+//   FF [cell][lfi][nfp][9]   this cell's common inviscid flux (contravariant,
+//                            signed as the reference hands it to the cell) + A_z
+//   FFv[cell][lfi][nfp][9]   this cell's common viscous + AV flux
+//   gtr[cell][lfi][nfp][16]  own trace of nablaQs (k_grad)
+struct FaceArgs {
+    int nface;
+    const int4 *faces;
+    const double *cellxy;
+    const double *TQ;
+    void *FF;
+    void *FFv;
+    const void *gtr;
+    const double *sm;        // AV indicator per cell
+    double *eig;             // k_face_inv: atomic max of the Rusanov wave speed (AV)
+    const double *eig_now;   // k_face_visc: this stage's maximum (epsilon0)
+    int artiv;
+    double gam, mu, eta0, h_msh, pi;
 };
 
 // viscous / AV path (k_grad + k_stage_visc)
@@ -60,6 +89,7 @@ struct ViscArgs {
     double *sm;         // log10 smoothness indicator of density per cell
     double *eig;        // this stage's maximum Rusanov wave speed (atomic max)
     double *eig_next;   // the next stage's slot, zeroed by k_stage_visc
+    const double *TQ;   // traces of Qin at the face points
     int artiv;
     double mu, eta0, gam, h_msh, pi;
     template <class R> __host__ __device__ R *qvf_r() const { return (R *)qvf; }
@@ -74,10 +104,14 @@ struct CTArgs {
     double *Q;
     int which;          // 0: correct B from curl A_z; 1: div B diagnostic
     double *partial;    // which=1: per block {sum divB^2, max |divB|}
+    const double *TQ;   // traces of A_z (which=0) / Bx, By (which=1) at the face points
+    double *TQout;      // which=0: the traces of the corrected Bx, By are written here
 };
 
+// SSP stage combination at one solution point; the new state also replaces
+// the point's entries of qs (the block's LDS copy, read by no other thread)
 template <class R>
-__device__ __forceinline__ void stage_combine(const StageArgs &a, size_t base, int NP, const double *qs,
+__device__ __forceinline__ void stage_combine(const StageArgs &a, size_t base, int NP, double *qs,
                                               const R *res) {
 #pragma unroll
     for (int v = 0; v < 9; v++) {
@@ -94,14 +128,16 @@ __device__ __forceinline__ void stage_combine(const StageArgs &a, size_t base, i
         default: qn = a.c1 * a.Qa[o] + a.c2 * a.Qb[o] + a.c3 * a.Qc[o] + a.c4 * q + a.c5 * a.Rprev[o] + a.c0 * r; break;
         }
         a.Qout[o] = qn;
+        qs[v * NP] = qn;
     }
 }
 
 // launchers (defined in the kernel translation units); return hipError_t
-hipError_t launch_stage_ideal_fp64(int N, const StageArgs &a, hipStream_t s);
+hipError_t launch_stage_ideal_fp64(int N, const StageArgs &a, const FaceArgs &f, hipStream_t s);
+hipError_t launch_trace(int N, int ncell, const double *Q, double *TQ, hipStream_t s);
 hipError_t launch_stage_ideal_mixed(int N, const StageArgs &a, hipStream_t s);
 hipError_t launch_ct(int N, const CTArgs &a, hipStream_t s);
-hipError_t launch_stage_visc_fp64(int N, const StageArgs &a, const ViscArgs &v, hipStream_t s);
+hipError_t launch_stage_visc_fp64(int N, const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s);
 hipError_t launch_stage_visc_mixed(int N, const StageArgs &a, const ViscArgs &v, hipStream_t s);
 int ct_blocks(int N, int ncell);
 int stage_blocks(int N, int ncell);

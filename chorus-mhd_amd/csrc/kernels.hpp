@@ -47,13 +47,19 @@ static hipError_t upload_tables() {
     return hipMemcpyToSymbol(HIP_SYMBOL(c_Mf), Mf, sizeof(Mf));
 }
 
-template <int N> struct Shape {
+// a workgroup of about TB threads owns CPB whole cells, one thread per solution point
+template <int N, int TB> struct CellShape {
     static constexpr int NP = N * N;          // solution points per cell
     static constexpr int NF = N * (N + 1);    // flux points per family
     static constexpr int NQ = NV * NP;        // doubles of state per cell
-    static constexpr int CPB = (256 / NP) > 0 ? (256 / NP) : 1;   // cells per block
+    static constexpr int CPB = (TB / NP) > 0 ? (TB / NP) : 1;   // cells per block
     static constexpr int THREADS = ((CPB * NP + 63) / 64) * 64;
 };
+template <int N> using Shape = CellShape<N, 256>;
+#ifndef CHORUS_IDEAL_TB
+#define CHORUS_IDEAL_TB 256
+#endif
+template <int N> using IShape = CellShape<N, CHORUS_IDEAL_TB>;
 
 // XCD-aware block order: the hardware deals consecutive workgroups round-robin
 // over the 8 XCDs; give each XCD a contiguous run of cell blocks so the
@@ -175,17 +181,125 @@ __device__ __forceinline__ FacePoint resolve_face(int2 info, int fam, int r, int
     return fp;
 }
 
+// reference sign of a cell's contribution at its local face lfi (0..3):
+// -1 on faces 1 and 4 (flux point index 0 of the family), +1 on faces 2 and 3
+// (interfaceflux_all.f90:28-37)
+__device__ __forceinline__ int side_sign(int lfi) { return (lfi == 0 || lfi == 3) ? -1 : 1; }
+
+// traces of all 9 variables of the block's cells at their 4N face points, from
+// the LDS copy q[c][var][js][is]; same summation order as trace9.  Called by
+// all threads of the block.  TQ[cell][lfi][nfp][var].
+template <int N>
+__device__ __forceinline__ void store_traces(const double *q, int ncl, int c0, double *TQ) {
+    constexpr int NP = N * N, NQ = NV * NP, PT = 4 * N * NV;
+    for (int it = threadIdx.x; it < ncl * PT; it += blockDim.x) {
+        const int c = it / PT, k = it % PT, fpi = k / NV, v = k % NV;
+        const int lfi = fpi / N, nfp = fpi % N;
+        int ifp, jfp;
+        face2fp<N>(lfi + 1, nfp, ifp, jfp);
+        const double *qv = q + c * NQ + v * NP;
+        double acc = 0.0;
+        if (lf_family(lfi + 1) == 0) {
+#pragma unroll
+            for (int sp = 0; sp < N; sp++) acc = acc + qv[jfp * N + sp] * c_L[N][ifp * N + sp];
+        } else {
+#pragma unroll
+            for (int sp = 0; sp < N; sp++) acc = acc + qv[sp * N + ifp] * c_L[N][jfp * N + sp];
+        }
+        TQ[(size_t)c0 * PT + it] = acc;
+    }
+}
+
+// traces of a state (after set-up, or for a state not produced by a stage)
+template <int N>
+__global__ void __launch_bounds__(256) k_trace(int ncell, const double *Q, double *TQ) {
+    using S = Shape<N>;
+    constexpr int CPB = S::CPB, NQ = S::NQ;
+    __shared__ double sQ[CPB * NQ];
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int c0 = blk * CPB;
+    const int ncl = min(CPB, ncell - c0);
+    for (int i = threadIdx.x; i < ncl * NQ; i += blockDim.x) sQ[i] = Q[(size_t)c0 * NQ + i];
+    __syncthreads();
+    store_traces<N>(sQ, ncl, c0, TQ);
+}
+
+// block maximum (any block size) folded into *dst with one atomic
+template <class R>
+__device__ __forceinline__ void block_max_atomic(double v, double *red, double *dst) {
+    const int tid = threadIdx.x, n = blockDim.x;
+    red[tid] = v;
+    __syncthreads();
+    int p = 1;
+    while (p < n) p <<= 1;
+    for (int s = p >> 1; s > 0; s >>= 1) {
+        if (tid < s && tid + s < n) red[tid] = fmax(red[tid], red[tid + s]);
+        __syncthreads();
+    }
+    if (tid == 0) {
+        // non-negative doubles order like their bit patterns
+        atomicMax(reinterpret_cast<unsigned long long *>(dst), (unsigned long long)__double_as_longlong(red[0]));
+    }
+}
+
 // ---------------------------------------------------------------------------
-// Stage kernel of the ideal path (vismode=1, mu=eta0=0, ARTIV=0): compresid
-// (compresid.f90:1-67 via compflux.f90:1-104, interfaceflux_all.f90:1-116,
-// cyclocflux_all.f90:1-116, bcflux.f90:1-70, magnetic_potential.f90:1-42)
-// fused with one SSP stage combination (iterations.f90:92-217).  With
-// mu=eta0=0 and no AV the viscous terms of compresid are exact zeros, so they
-// are not evaluated.
+// Rusanov flux of every face point, once per face (getrusanov.f90:1-109 called
+// from interfaceflux_all.f90:16-114, cyclocflux_all.f90:15-114, bcflux.f90:19-73):
+// the left cell's trace and metric, the right cell's trace (a FREE face
+// mirrors the left one).  The left cell receives Fnl, the right one
+// (sign_l*sign_r)*Fnl; both receive the upwind A_z.  With ARTIV the block
+// maximum of the wave speeds goes to the stage maximum (getrusanov.f90:105-107).
 // ---------------------------------------------------------------------------
 template <int N, class R>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CHORUS_IDEAL_MINW))) k_stage_ideal(StageArgs a) {
-    using S = Shape<N>;
+__global__ void __launch_bounds__(256) k_face_inv(FaceArgs a) {
+    __shared__ double red[256];
+    const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    double emax = 0.0;
+    if (t < a.nface * N) {
+        const int f = t / N, k = t % N;
+        const int4 fc = a.faces[f];
+        const bool free_face = fc.z < 0;
+        int ifp, jfp;
+        face2fp<N>(fc.y + 1, k, ifp, jfp);
+        double xy[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) xy[q] = a.cellxy[(size_t)fc.x * 8 + q];
+        double s1, s2;
+        metric_row<N>(xy, lf_family(fc.y + 1), ifp, jfp, s1, s2);
+        const size_t pl = (((size_t)fc.x * 4 + fc.y) * N + k) * NV;
+        const size_t pr = free_face ? pl : (((size_t)fc.z * 4 + fc.w) * N + (N - 1 - k)) * NV;
+        R Ql[NV], Qr[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) { Ql[v] = R(a.TQ[pl + v]); Qr[v] = R(a.TQ[pr + v]); }
+        const int sl = side_sign(fc.y);
+        R Fn[8], az, eig;
+        rusanov<R>(R(a.gam), Ql, Qr, R(s1), R(s2), sl, Fn, az, eig);
+        R *FF = (R *)a.FF;
+#pragma unroll
+        for (int v = 0; v < 8; v++) FF[pl + v] = Fn[v];
+        FF[pl + 8] = az;
+        if (!free_face) {
+            const R slr = R(sl * -side_sign(fc.w));
+#pragma unroll
+            for (int v = 0; v < 8; v++) FF[pr + v] = slr * Fn[v];
+            FF[pr + 8] = az;
+        }
+        emax = double(eig);
+    }
+    if (a.artiv) block_max_atomic<R>(emax, red, a.eig);
+}
+
+// ---------------------------------------------------------------------------
+// Stage kernel of the ideal path (vismode=1, mu=eta0=0, ARTIV=0): compresid
+// (compresid.f90:1-67 via compflux.f90:1-104 and magnetic_potential.f90:1-42)
+// with the face fluxes of k_face_inv, fused with one SSP stage combination
+// (iterations.f90:92-217); the traces of the new state are stored for the next
+// stage's face pass.  With mu=eta0=0 and no AV the viscous terms of compresid
+// are exact zeros, so they are not evaluated.
+// ---------------------------------------------------------------------------
+template <int N, class R>
+__global__ void __launch_bounds__(IShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(CHORUS_IDEAL_MINW))) k_stage_ideal(StageArgs a) {
+    using S = IShape<N>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
     constexpr int FST = 8 * NF + 1;      // per-cell stride of sF (odd: cells land on different banks)
     constexpr int AST = 3 * NF + 1;      // per-cell stride of sA
@@ -211,12 +325,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CHORUS
     const int pc = tid / NP, pk = tid % NP;
     const int pjs = pk / N, pis = pk % N;
     const bool pact = (tid < CPB * NP) && (pc < ncl);
+    const R *FF = (const R *)a.FF;
     R dF[8], dA1 = R(0), dA2 = R(0);
 
 #pragma unroll 1
     for (int fam = 0; fam < 2; fam++) {
-        // B1: interior flux points, one uniform path per wave
-        // (inviscid_flux_interior, compflux.f90:59-102)
+        // B1: interior flux points (inviscid_flux_interior, compflux.f90:59-102)
         for (int it = tid; it < ncl * N * (N - 1); it += blockDim.x) {
             const int c = it / (N * (N - 1)), k = it % (N * (N - 1));
             const int line = k / (N - 1), r = 1 + k % (N - 1);
@@ -235,52 +349,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CHORUS
             aa[NF] = s1;
             aa[2 * NF] = s2;
         }
-        // B2: face points, one uniform path per wave: both traces and the
-        // Rusanov flux (interfaceflux_all.f90:16-114, cyclocflux_all.f90:15-114,
-        // bcflux.f90:19-73)
+        // B2: face points: the common flux and A_z of k_face_inv
         for (int it = tid; it < ncl * 2 * N; it += blockDim.x) {
             const int c = it / (2 * N), k = it % (2 * N);
             const int line = k >> 1, r = (k & 1) ? N : 0;
-            const int cell = c0 + c;
             const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+            int lfi, nfp;
+            if (fam == 0) { lfi = (r == 0) ? 3 : 1; nfp = (r == 0) ? (N - 1 - line) : line; }
+            else          { lfi = (r == 0) ? 0 : 2; nfp = (r == 0) ? line : (N - 1 - line); }
+            const R *src = FF + (((size_t)(c0 + c) * 4 + lfi) * N + nfp) * NV;
             double s1d, s2d;
             metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
-            R Qown[NV];
-            trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown);
-            const FacePoint fp = resolve_face<N>(a.nbr[(size_t)cell * 4 + (fam == 0 ? (r == 0 ? 3 : 1) : (r == 0 ? 0 : 2))], fam, r, line);
-            R Qnb[NV];
-            R nx, ny;
-            if (fp.free_face) {
-#pragma unroll
-                for (int v = 0; v < NV; v++) Qnb[v] = Qown[v];
-                nx = R(s1d); ny = R(s2d);
-            } else {
-                trace9<N, R>(a.Qin + (size_t)fp.nb * NQ, fp.nb_fam, fp.nb_ifp, fp.nb_jfp, Qnb);
-                if (!fp.right) { nx = R(s1d); ny = R(s2d); }
-                else {
-                    double xyn[8];
-#pragma unroll
-                    for (int q = 0; q < 8; q++) xyn[q] = a.cellxy[(size_t)fp.nb * 8 + q];
-                    double n1, n2;
-                    metric_row<N>(xyn, fp.nb_fam, fp.nb_ifp, fp.nb_jfp, n1, n2);
-                    nx = R(n1); ny = R(n2);
-                }
-            }
-            R out[8], az, eig;
-            if (!fp.right) {
-                rusanov<R>(R(a.gam), Qown, Qnb, nx, ny, fp.sign_l, out, az, eig);
-            } else {
-                R Fnl[8];
-                rusanov<R>(R(a.gam), Qnb, Qown, nx, ny, fp.sign_l, Fnl, az, eig);
-                const R slr = R(fp.sign_l * fp.sign_r);
-#pragma unroll
-                for (int v = 0; v < 8; v++) out[v] = slr * Fnl[v];
-            }
             R *f = sF + c * FST + line * (N + 1) + r;
 #pragma unroll
-            for (int v = 0; v < 8; v++) f[v * NF] = out[v];
+            for (int v = 0; v < 8; v++) f[v * NF] = src[v];
             R *aa = sA + c * AST + line * (N + 1) + r;
-            aa[0] = az;
+            aa[0] = src[8];
             aa[NF] = R(s1d);
             aa[2 * NF] = R(s2d);
         }
@@ -314,7 +398,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CHORUS
                 // residual at the solution point (compresid.f90:12-65)
                 const double Jd = jacobian<N>(sX + pc * 8, pis, pjs);
                 const R J = R(Jd);
-                const double *qs = sQ + pc * NQ + pk;
+                double *qs = sQ + pc * NQ + pk;
                 const R q0 = R(qs[0]), q1 = R(qs[NP]), q2 = R(qs[2 * NP]);
                 const R au = q1 / q0, av = q2 / q0;
                 const R nA1 = (dA1 + e1) / J, nA2 = (dA2 + e2) / J;
@@ -331,6 +415,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CHORUS
         }
         __syncthreads();
     }
+    // D: traces of the new state for the next stage's face pass
+    if (a.kind != KIND_RESID) store_traces<N>(sQ, ncl, c0, a.TQ);
 }
 
 // ---------------------------------------------------------------------------
@@ -338,15 +424,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CHORUS
 // point (interior: L extrapolation, compflux.f90:69-75/88-94; faces: average of
 // both traces, interfaceflux_all.f90:79-91), nablaQsp(9,9) (compflux.f90:107-157)
 // and update_B (magnetic_potential.f90:44-67).  FP64 in both precision forms.
-// In place: reads var 9 (also of neighbours), writes vars 6 and 7.
-// With which=1 it instead evaluates div B = d(Bx)/dx + d(By)/dy at the
-// solution points (iterations.f90:35-48) and writes per-block sum of squares
-// and max |div B| (no state change).
+// In place: reads var 9, writes vars 6 and 7 and their face traces.
+// WHICH=1 instead evaluates div B = d(Bx)/dx + d(By)/dy at the solution points
+// (iterations.f90:35-48) and writes per-block sum of squares and max |div B|
+// (no state change).  The partner's trace comes from the stored traces.
 // ---------------------------------------------------------------------------
-template <int N>
+template <int N, int WHICH>
 __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
     using S = Shape<N>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
+    constexpr int NVAR = WHICH == 0 ? 1 : 2, VAR0 = WHICH == 0 ? 8 : 5;
     __shared__ double sV[2][CPB * NP];   // variable planes staged (A_z, or Bx and By)
     __shared__ double sX[CPB * 8];
     __shared__ double sF[2][CPB * NF];   // central flux-point values, one family at a time
@@ -356,18 +443,16 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = blk * CPB;
     const int ncl = min(CPB, a.ncell - c0);
-    const int nvar = a.which == 0 ? 1 : 2;
-    const int var0 = a.which == 0 ? 8 : 5;
     for (int i = tid; i < ncl * NP; i += blockDim.x) {
         int c = i / NP, k = i % NP;
-        for (int w = 0; w < nvar; w++) sV[w][i] = a.Q[(size_t)(c0 + c) * NQ + (var0 + w) * NP + k];
+#pragma unroll
+        for (int w = 0; w < NVAR; w++) sV[w][i] = a.Q[(size_t)(c0 + c) * NQ + (VAR0 + w) * NP + k];
     }
     for (int i = tid; i < ncl * 8; i += blockDim.x) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
     __syncthreads();
     const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
     const bool pact = (tid < CPB * NP) && (pc < ncl);
-    double dpsi[2][2] = {{0.0, 0.0}, {0.0, 0.0}};   // [w][component]
-    double out[2][2];
+    double dpsi[NVAR][2], out[NVAR][2];   // [w][component]
 #pragma unroll 1
     for (int fam = 0; fam < 2; fam++) {
         for (int it = tid; it < ncl * NF; it += blockDim.x) {
@@ -376,16 +461,21 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
             const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
             double s1, s2;
             metric_row<N>(sX + c * 8, fam, ifp, jfp, s1, s2);
-            for (int w = 0; w < nvar; w++) {
+            int2 info = make_int2(-1, 1 << 4);
+            int lfi = 0, nfp = 0;
+            if (r == 0 || r == N) {
+                if (fam == 0) { lfi = (r == 0) ? 3 : 1; nfp = (r == 0) ? (N - 1 - line) : line; }
+                else          { lfi = (r == 0) ? 0 : 2; nfp = (r == 0) ? line : (N - 1 - line); }
+                info = a.nbr[(size_t)cell * 4 + lfi];
+            }
+#pragma unroll
+            for (int w = 0; w < NVAR; w++) {
                 double own = trace1<N>(sV[w] + c * NP, fam, ifp, jfp);
                 double val = own;
                 if (r == 0 || r == N) {
-                    const int lfi = fam == 0 ? (r == 0 ? 3 : 1) : (r == 0 ? 0 : 2);
-                    const FacePoint fp = resolve_face<N>(a.nbr[(size_t)cell * 4 + lfi], fam, r, line);
                     double other = own;
-                    if (!fp.free_face)
-                        other = trace1<N>(a.Q + (size_t)fp.nb * NQ + (var0 + w) * NP, fp.nb_fam,
-                                          fp.nb_ifp, fp.nb_jfp);
+                    if (!((info.y >> 4) & 1))
+                        other = a.TQ[(((size_t)info.x * 4 + (info.y & 7) - 1) * N + (N - 1 - nfp)) * NV + VAR0 + w];
                     val = 0.5 * own + 0.5 * other;
                 }
                 sF[w][c * NF + line * (N + 1) + r] = val;
@@ -396,8 +486,10 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
         __syncthreads();
         if (pact) {
             const int line = fam == 0 ? pjs : pis, pos = fam == 0 ? pis : pjs;
-            for (int w = 0; w < nvar; w++) {
+#pragma unroll
+            for (int w = 0; w < NVAR; w++) {
                 const double *f = sF[w] + pc * NF + line * (N + 1);
+#pragma unroll
                 for (int d = 0; d < 2; d++) {
                     const double *ss = sS[d] + pc * NF + line * (N + 1);
                     double acc = 0.0;
@@ -410,24 +502,37 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
         }
         __syncthreads();
     }
-    double sumsq = 0.0, dmax = 0.0;
-    if (pact) {
-        const double J = jacobian<N>(sX + pc * 8, pis, pjs);
-        const size_t base = (size_t)(c0 + pc) * NQ + pk;
-        if (a.which == 0) {
+    if (WHICH == 0) {
+        // update_B, then the traces of the new Bx, By for the next face pass
+        if (pact) {
+            const double J = jacobian<N>(sX + pc * 8, pis, pjs);
+            const size_t base = (size_t)(c0 + pc) * NQ + pk;
             const double n1 = (dpsi[0][0] + out[0][0]) / J;   // nablaQs(9,1)
             const double n2 = (dpsi[0][1] + out[0][1]) / J;   // nablaQs(9,2)
             a.Q[base + 5 * NP] = n2;
             a.Q[base + 6 * NP] = -n1;
-        } else {
+            sV[0][pc * NP + pk] = n2;
+            sV[1][pc * NP + pk] = -n1;
+        }
+        __syncthreads();
+        for (int it = tid; it < ncl * 4 * N * 2; it += blockDim.x) {
+            const int c = it / (8 * N), k = it % (8 * N), fpi = k / 2, w = k % 2;
+            const int lfi = fpi / N, nfp = fpi % N;
+            int ifp, jfp;
+            face2fp<N>(lfi + 1, nfp, ifp, jfp);
+            const double v = trace1<N>(sV[w] + c * NP, lf_family(lfi + 1), ifp, jfp);
+            a.TQout[(((size_t)(c0 + c) * 4 + lfi) * N + nfp) * NV + 5 + w] = v;
+        }
+    } else {
+        double sumsq = 0.0, dmax = 0.0;
+        if (pact) {
+            const double J = jacobian<N>(sX + pc * 8, pis, pjs);
             const double dbx = (dpsi[0][0] + out[0][0]) / J;  // nablaQs(6,1)
-            const double dby = (dpsi[1][1] + out[1][1]) / J;  // nablaQs(7,2)
+            const double dby = (dpsi[NVAR - 1][1] + out[NVAR - 1][1]) / J;  // nablaQs(7,2)
             const double db = dbx + dby;
             sumsq = db * db;
             dmax = __builtin_fabs(db);
         }
-    }
-    if (a.which != 0) {
         red[0][tid] = sumsq;
         red[1][tid] = dmax;
         __syncthreads();

@@ -239,7 +239,7 @@ __global__ void __launch_bounds__(256) k_stage_ideal_mx(StageArgs a) {
                 for (int v = 0; v < 8; v++) dF[v] = d[v];
                 dA1 = e1; dA2 = e2;
             } else {
-                const double *qs = sQ + pc * NQ + pk;
+                double *qs = sQ + pc * NQ + pk;
                 const float iJ = float(1.0 / jacobian<N>(sX + pc * 8, pis, pjs));
                 const float iq0 = frcp(float(qs[0]));
                 const float au = float(qs[NP]) * iq0, av = float(qs[2 * NP]) * iq0;
@@ -252,6 +252,8 @@ __global__ void __launch_bounds__(256) k_stage_ideal_mx(StageArgs a) {
         }
         __syncthreads();
     }
+    // traces of the new state (the constrained-transport pass reads them)
+    if (a.kind != KIND_RESID) store_traces<N>(sQ, ncl, c0, a.TQ);
 }
 
 // ---------------------------------------------------------------------------
@@ -260,7 +262,7 @@ __global__ void __launch_bounds__(256) k_stage_ideal_mx(StageArgs a) {
 // owning cell's mean state)
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ void __launch_bounds__(128) k_grad_mx(ViscArgs a) {
+__global__ void __launch_bounds__(VShape<N>::THREADS) k_grad_mx(ViscArgs a) {
     using S = VShape<N>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
     constexpr int VST = 10 * NF + 1;
@@ -271,7 +273,7 @@ __global__ void __launch_bounds__(128) k_grad_mx(ViscArgs a) {
     __shared__ float sV[CPB * VST];
     __shared__ float sG[CPB * 16 * NP];
     __shared__ double sAV[CPB * NP];
-    __shared__ double sRed[128];
+    __shared__ double sRed[VShape<N>::THREADS];
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = blk * CPB;
@@ -379,7 +381,7 @@ __global__ void __launch_bounds__(128) k_grad_mx(ViscArgs a) {
 }
 
 template <int N>
-__global__ void __launch_bounds__(128) k_stage_visc_mx(StageArgs a, ViscArgs v) {
+__global__ void __launch_bounds__(VShape<N>::THREADS) k_stage_visc_mx(StageArgs a, ViscArgs v) {
     using S = VShape<N>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
     constexpr int UST = 20 * NF + 1;     // dF1 (8) + Fv1 (9) + dA_z + metric rows (2)
@@ -521,7 +523,7 @@ __global__ void __launch_bounds__(128) k_stage_visc_mx(StageArgs a, ViscArgs v) 
                 for (int w = 0; w < NV; w++) dFv[w] = dv[w];
                 dA1 = e1; dA2 = e2;
             } else {
-                const double *qs = sQ + pc * NQ + pk;
+                double *qs = sQ + pc * NQ + pk;
                 const float iJ = float(1.0 / jacobian<N>(sX + pc * 8, pis, pjs));
                 const float iq0 = frcp(float(qs[0]));
                 const float au = float(qs[NP]) * iq0, av = float(qs[2 * NP]) * iq0;
@@ -534,6 +536,8 @@ __global__ void __launch_bounds__(128) k_stage_visc_mx(StageArgs a, ViscArgs v) 
         }
         __syncthreads();
     }
+    // traces of the new state (the constrained-transport pass reads them)
+    if (a.kind != KIND_RESID) store_traces<N>(sQ, ncl, c0, a.TQ);
 }
 
 }  // namespace chorus

@@ -1,23 +1,23 @@
 // Viscous / artificial-viscosity path (vismode=1 with ARTIV=1 or mu, eta0 != 0).
 //
 // The reference stage (compflux.f90:1-31) has three neighbour-coupled phases
-// and one global reduction; on the device they are two kernels per stage:
+// and one global reduction; on the device they are four launches per stage:
 //
+//   k_face_inv   per face point: Rusanov flux and the stage maximum of its wave
+//                speed (getrusanov.f90:1-109, 105-107)
 //   k_grad       per cell: central Qv at every flux point (face values average
-//                the neighbour's trace, interfaceflux_all.f90:79-91), nablaQs at
+//                both stored traces, interfaceflux_all.f90:79-91), nablaQs at
 //                the solution points (nablaQsp, compflux.f90:107-157), its own
-//                traces at the four faces (for the neighbours' averages,
-//                interfaceVISflux), the Qv face averages, the AV smoothness
-//                indicator of density (AV.f90:15-82) and the maximum Rusanov
-//                wave speed of the stage (getrusanov.f90:105-107, atomic max)
-//   k_stage_visc per cell: the inviscid stage of k_stage_ideal plus the viscous
-//                and AV fluxes (comp_viscous_flux, compflux.f90:198-258) from
-//                the stored face data, the residual (compresid.f90:1-67) and
-//                the SSP stage combination
+//                traces at the four faces and the AV smoothness indicator of
+//                density (AV.f90:15-82)
+//   k_face_visc  per face point: the viscous and AV flux from the face-averaged
+//                Qv and gradient (interfaceVISflux, interfaceAVflux, BCVISFLUX,
+//                comp_viscous_flux) for both cells, each in its own metric
+//   k_stage_visc per cell: interior inviscid, viscous and AV fluxes, the stored
+//                face fluxes, the residual (compresid.f90:1-67), the SSP stage
+//                combination and the traces of the new state
 //
-// Face data in HBM, per cell, local face lf and face point nfp (this cell's
-// own numbering along the face, face2fpconnec order):
-//   qvf[cell][lf][nfp][8]   face-averaged Qv (variables 1..8)
+// Per cell, local face lf and face point nfp (own numbering, face2fpconnec order):
 //   gtr[cell][lf][nfp][16]  this cell's trace of nablaQs (vars 1..8, x then y)
 //   sm[cell]                log10 smoothness indicator of density (SM_IND(1,:))
 // Only muAV(1,:) reaches the flux (mu_art = epsilon_loc(1), getrusanov.f90:279),
@@ -41,11 +41,10 @@ static hipError_t upload_av_tables() {
     return hipMemcpyToSymbol(HIP_SYMBOL(c_H2), H2, sizeof(H2));
 }
 
-template <int N> struct VShape {
-    static constexpr int NP = N * N, NF = N * (N + 1), NQ = NV * NP;
-    static constexpr int CPB = (128 / NP) > 0 ? (128 / NP) : 1;
-    static constexpr int THREADS = ((CPB * NP + 63) / 64) * 64;
-};
+#ifndef CHORUS_VISC_TB
+#define CHORUS_VISC_TB 128
+#endif
+template <int N> using VShape = CellShape<N, CHORUS_VISC_TB>;
 
 // local-face index (0..3) and own face-point number of flux point (fam, r, line)
 template <int N> __device__ __forceinline__ void face_of(int fam, int r, int line, int &lfi, int &nfp) {
@@ -56,18 +55,11 @@ template <int N> __device__ __forceinline__ void face_of(int fam, int r, int lin
 // partner's own face-point number of the same physical point
 template <int N> __device__ __forceinline__ int partner_nfp(int nfp) { return N - 1 - nfp; }
 
-__device__ __forceinline__ void atomic_max_nonneg(double *p, double v) {
-    // non-negative doubles order like their bit patterns
-    atomicMax(reinterpret_cast<unsigned long long *>(p), (unsigned long long)__double_as_longlong(v));
-}
-
 // AV smoothness indicator of density (AV.f90:15-82, FP64 in both forms: the
-// N -> N-1 -> N interpolation round trip and log10 of the relative energy) and
-// the block maximum of the face wave speeds folded into the stage maximum with
-// one atomic per block.  Called by all threads of the block.
+// N -> N-1 -> N interpolation round trip and log10 of the relative energy).
+// Called by all threads of the block.
 template <int N>
-__device__ __forceinline__ void av_sensor_and_eigmax(const ViscArgs &a, const double *sQ, double *sAV, double *sRed,
-                                                     int ncl, int c0, double emax) {
+__device__ __forceinline__ void av_sensor(double *sm, const double *sQ, double *sAV, int ncl, int c0) {
     constexpr int NP = N * N, NQ = NV * NP, NL = N - 1;
     const int tid = threadIdx.x;
     const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
@@ -97,66 +89,21 @@ __device__ __forceinline__ void av_sensor_and_eigmax(const ViscArgs &a, const do
     if (tid < ncl) {
         double se = 0.0;
         for (int p = 0; p < NP; p++) se = se + sAV[tid * NP + p];
-        a.sm[c0 + tid] = log10(se);
+        sm[c0 + tid] = log10(se);
     }
-    sRed[tid] = emax;
+}
+
+template <int N>
+__device__ __forceinline__ void av_sensor_and_eigmax(const ViscArgs &a, const double *sQ, double *sAV, double *sRed,
+                                                     int ncl, int c0, double emax) {
+    av_sensor<N>(a.sm, sQ, sAV, ncl, c0);
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (tid < s) sRed[tid] = fmax(sRed[tid], sRed[tid + s]);
-        __syncthreads();
-    }
-    if (tid == 0) atomic_max_nonneg(a.eig, sRed[0]);
-}
-
-// ---------------------------------------------------------------------------
-// Central Qv at flux point (fam, r, line) of cell c (interior: own trace;
-// face: average with the partner's trace, interfaceflux_all.f90:79-91,
-// bcflux.f90:55-61).  Face points also return the Rusanov wave speed.
-// ---------------------------------------------------------------------------
-template <int N, class R>
-__device__ __forceinline__ void grad_face_point(const ViscArgs &a, const double *sQc, const double *xy, int cell,
-                                                int fam, int r, int line, double s1d, double s2d, R *qv,
-                                                double &emax) {
-    constexpr int NQ = NV * N * N;
-    const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
-    R Qown[NV];
-    trace9<N, R>(sQc, fam, ifp, jfp, Qown);
-    int lfi, nfp;
-    face_of<N>(fam, r, line, lfi, nfp);
-    const FacePoint fp = resolve_face<N>(a.nbr[(size_t)cell * 4 + lfi], fam, r, line);
-    R Qnb[NV], nx, ny;
-    if (fp.free_face) {
-#pragma unroll
-        for (int v = 0; v < NV; v++) Qnb[v] = Qown[v];
-        nx = R(s1d); ny = R(s2d);
-    } else {
-        trace9<N, R>(a.Qin + (size_t)fp.nb * NQ, fp.nb_fam, fp.nb_ifp, fp.nb_jfp, Qnb);
-        if (!fp.right) { nx = R(s1d); ny = R(s2d); }
-        else {
-            double xyn[8], n1, n2;
-#pragma unroll
-            for (int q = 0; q < 8; q++) xyn[q] = a.cellxy[(size_t)fp.nb * 8 + q];
-            metric_row<N>(xyn, fp.nb_fam, fp.nb_ifp, fp.nb_jfp, n1, n2);
-            nx = R(n1); ny = R(n2);
-        }
-    }
-    // central average (0.5*Qfl + 0.5*Qfr is symmetric in its operands)
-#pragma unroll
-    for (int v = 0; v < 8; v++) qv[v] = R(0.5) * Qown[v] + R(0.5) * Qnb[v];
-    R *dst = a.qvf_r<R>() + (((size_t)cell * 4 + lfi) * N + nfp) * 8;
-#pragma unroll
-    for (int v = 0; v < 8; v++) dst[v] = qv[v];
-    if (a.artiv) {
-        R Fn[8], az, eig;
-        if (!fp.right) rusanov<R>(R(a.gam), Qown, Qnb, nx, ny, fp.sign_l, Fn, az, eig);
-        else rusanov<R>(R(a.gam), Qnb, Qown, nx, ny, fp.sign_l, Fn, az, eig);
-        emax = fmax(emax, double(eig));
-    }
+    block_max_atomic<double>(emax, sRed, a.eig);
 }
 
 // ---------------------------------------------------------------------------
 template <int N, class R>
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(CHORUS_VISC_MINW))) k_grad(ViscArgs a) {
+__global__ void __launch_bounds__(VShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(CHORUS_VISC_MINW))) k_grad(ViscArgs a) {
     using S = VShape<N>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
     constexpr int VST = 10 * NF + 1;     // per-cell stride: Qv (8 planes) + metric rows (2 planes)
@@ -165,7 +112,6 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(CHORUS
     __shared__ R sV[CPB * VST];          // [c][v | s1 | s2][line][r], one family at a time
     __shared__ R sG[CPB * 16 * NP];      // nablaQs [c][d*8+v][p]
     __shared__ double sAV[CPB * NP];     // sensor work
-    __shared__ double sRed[128];
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = blk * CPB;
@@ -176,7 +122,6 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(CHORUS
     const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
     const bool pact = (tid < CPB * NP) && (pc < ncl);
     R gp[16];
-    double emax = 0.0;
 #pragma unroll 1
     for (int fam = 0; fam < 2; fam++) {
         // interior flux points: Qv = own trace
@@ -194,18 +139,24 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(CHORUS
             q[8 * NF] = R(s1d);
             q[9 * NF] = R(s2d);
         }
-        // face points: central average with the partner's trace
+        // face points: central average of both stored traces (a FREE face
+        // averages the own trace with itself, bcflux.f90:55-61)
         for (int it = tid; it < ncl * 2 * N; it += blockDim.x) {
             const int c = it / (2 * N), k = it % (2 * N);
             const int line = k >> 1, r = (k & 1) ? N : 0;
+            const int cell = c0 + c;
             const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
             double s1d, s2d;
             metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
-            R qv[8];
-            grad_face_point<N, R>(a, sQ + c * NQ, sX + c * 8, c0 + c, fam, r, line, s1d, s2d, qv, emax);
+            int lfi, nfp;
+            face_of<N>(fam, r, line, lfi, nfp);
+            const int2 info = a.nbr[(size_t)cell * 4 + lfi];
+            const double *own = a.TQ + (((size_t)cell * 4 + lfi) * N + nfp) * NV;
+            const double *oth = ((info.y >> 4) & 1) ? own
+                                : a.TQ + (((size_t)info.x * 4 + (info.y & 7) - 1) * N + partner_nfp<N>(nfp)) * NV;
             R *q = sV + c * VST + line * (N + 1) + r;
 #pragma unroll
-            for (int v = 0; v < 8; v++) q[v * NF] = qv[v];
+            for (int v = 0; v < 8; v++) q[v * NF] = R(0.5) * R(own[v]) + R(0.5) * R(oth[v]);
             q[8 * NF] = R(s1d);
             q[9 * NF] = R(s2d);
         }
@@ -259,7 +210,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(CHORUS
         }
         a.gtr_r<R>()[((size_t)(c0 + c) * 4 * N + fpi) * 16 + g] = acc;
     }
-    if (a.artiv) av_sensor_and_eigmax<N>(a, sQ, sAV, sRed, ncl, c0, emax);
+    if (a.artiv) av_sensor<N>(a.sm, sQ, sAV, ncl, c0);
 }
 
 // muAV(1,:) of one cell from its indicator and epsilon0 (AV.f90:120-172)
@@ -320,8 +271,87 @@ __device__ __forceinline__ void visc_point(const ViscArgs &v, const double *mu4,
 }
 
 // ---------------------------------------------------------------------------
+// Viscous and AV flux of every face point, once per face (interfaceVISflux /
+// cyclocVISflux / BCVISFLUX with interfaceAVflux, compflux.f90:198-258,
+// AV.f90:199-303, 494-564): Qv and nablaQ averaged over both stored traces
+// (a FREE face averages its own), muAV(1) averaged over both cells, the
+// Cartesian fluxes of getVfluxvectors / getVfluxvectors_AV evaluated once and
+// contracted with each cell's own metric row in visc_point's order.  At a face
+// point the flux-point coordinate of mu_art's interpolation is exactly 0 or 1,
+// so mu_art is the face average itself.
+// ---------------------------------------------------------------------------
 template <int N, class R>
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(CHORUS_VISC_MINW))) k_stage_visc(StageArgs a, ViscArgs v) {
+__global__ void __launch_bounds__(256) k_face_visc(FaceArgs a) {
+    const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= a.nface * N) return;
+    const int f = t / N, k = t % N;
+    const int4 fc = a.faces[f];
+    const bool free_face = fc.z < 0;
+    const int kr = N - 1 - k;
+    // own metric rows of both cells at the point
+    R sl1, sl2, sr1 = R(0), sr2 = R(0);
+    {
+        int ifp, jfp;
+        double xy[8], m1, m2;
+        face2fp<N>(fc.y + 1, k, ifp, jfp);
+#pragma unroll
+        for (int q = 0; q < 8; q++) xy[q] = a.cellxy[(size_t)fc.x * 8 + q];
+        metric_row<N>(xy, lf_family(fc.y + 1), ifp, jfp, m1, m2);
+        sl1 = R(m1); sl2 = R(m2);
+        if (!free_face) {
+            face2fp<N>(fc.w + 1, kr, ifp, jfp);
+#pragma unroll
+            for (int q = 0; q < 8; q++) xy[q] = a.cellxy[(size_t)fc.z * 8 + q];
+            metric_row<N>(xy, lf_family(fc.w + 1), ifp, jfp, m1, m2);
+            sr1 = R(m1); sr2 = R(m2);
+        }
+    }
+    const size_t pl = ((size_t)fc.x * 4 + fc.y) * N + k;
+    const size_t pr = free_face ? pl : ((size_t)fc.z * 4 + fc.w) * N + kr;
+    R qv[NV], g[2 * NV];
+#pragma unroll
+    for (int v = 0; v < 8; v++) qv[v] = R(0.5) * R(a.TQ[pl * NV + v]) + R(0.5) * R(a.TQ[pr * NV + v]);
+    qv[8] = R(0);
+    const R *gl = (const R *)a.gtr + pl * 16, *gr = (const R *)a.gtr + pr * 16;
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+#pragma unroll
+        for (int w = 0; w < 8; w++) g[d * NV + w] = R(0.5) * gl[d * 8 + w] + R(0.5) * gr[d * 8 + w];
+        g[d * NV + 8] = R(0);
+    }
+    const R gam = R(a.gam), lam = R(-2.0 / 3.0), prandt = R(0.72);
+    R fl[NV], fr[NV], Fv[NV], Gv[NV];
+    if (a.mu != 0.0 || a.eta0 != 0.0) {
+        vflux<R>(gam, lam, prandt, R(a.mu), R(a.eta0), qv, g, Fv, Gv);
+#pragma unroll
+        for (int w = 0; w < NV; w++) { fl[w] = Fv[w] * sl1 + Gv[w] * sl2; fr[w] = Fv[w] * sr1 + Gv[w] * sr2; }
+    } else {
+#pragma unroll
+        for (int w = 0; w < NV; w++) { fl[w] = R(0); fr[w] = R(0); }
+    }
+    if (a.artiv) {
+        const double eps0 = *a.eig_now * a.h_msh / (double)(N - 1) * 1.0;
+        const double ml = mu_av(a.sm[fc.x], eps0, a.pi);
+        const double mf = free_face ? ml : 0.5 * (ml + mu_av(a.sm[fc.z], eps0, a.pi));
+        vflux<R>(gam, lam, prandt, R(mf), R(0), qv, g, Fv, Gv);
+#pragma unroll
+        for (int w = 0; w < NV; w++) {
+            fl[w] = fl[w] + Fv[w] * sl1 + Gv[w] * sl2;
+            fr[w] = fr[w] + Fv[w] * sr1 + Gv[w] * sr2;
+        }
+    }
+    R *FFv = (R *)a.FFv;
+#pragma unroll
+    for (int w = 0; w < NV; w++) FFv[pl * NV + w] = fl[w];
+    if (!free_face) {
+#pragma unroll
+        for (int w = 0; w < NV; w++) FFv[pr * NV + w] = fr[w];
+    }
+}
+
+// ---------------------------------------------------------------------------
+template <int N, class R>
+__global__ void __launch_bounds__(VShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(CHORUS_VISC_MINW))) k_stage_visc(StageArgs a, ViscArgs v) {
     using S = VShape<N>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
     constexpr int UST = 20 * NF + 1;     // per-cell stride: F (8) + Fv (9) + A_z + metric rows (2)
@@ -354,11 +384,12 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(CHORUS
     __syncthreads();
     const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
     const bool pact = (tid < CPB * NP) && (pc < ncl);
+    const R *FF = (const R *)a.FF, *FFv = (const R *)a.FFv;
 
     R dF[8], dFv[NV], dA1 = R(0), dA2 = R(0);
 #pragma unroll 1
     for (int fam = 0; fam < 2; fam++) {
-        // interior flux points: inviscid + viscous from own data only
+        // interior flux points: inviscid flux
         for (int it = tid; it < ncl * N * (N - 1); it += blockDim.x) {
             const int c = it / (N * (N - 1)), k = it % (N * (N - 1));
             const int line = k / (N - 1), r = 1 + k % (N - 1);
@@ -395,89 +426,24 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(CHORUS
 #pragma unroll
             for (int w = 0; w < NV; w++) u[(8 + w) * NF] = fv[w];
         }
-        // face points: Rusanov with the partner's trace
+        // face points: the stored common inviscid, A_z and viscous fluxes
         for (int it = tid; it < ncl * 2 * N; it += blockDim.x) {
             const int c = it / (2 * N), k = it % (2 * N);
             const int line = k >> 1, r = (k & 1) ? N : 0;
-            const int cell = c0 + c;
             const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
             double s1d, s2d;
             metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
-            const R s1 = R(s1d), s2 = R(s2d);
             int lfi, nfp;
             face_of<N>(fam, r, line, lfi, nfp);
-            const int2 info = a.nbr[(size_t)cell * 4 + lfi];
-            const FacePoint fp = resolve_face<N>(info, fam, r, line);
+            const size_t o = (((size_t)(c0 + c) * 4 + lfi) * N + nfp) * NV;
             R *u = sU + c * UST + line * (N + 1) + r;
-            {
-                R Qown[NV], Qnb[NV], nx, ny;
-                trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown);
-                if (fp.free_face) {
 #pragma unroll
-                    for (int w = 0; w < NV; w++) Qnb[w] = Qown[w];
-                    nx = s1; ny = s2;
-                } else {
-                    trace9<N, R>(a.Qin + (size_t)fp.nb * NQ, fp.nb_fam, fp.nb_ifp, fp.nb_jfp, Qnb);
-                    if (!fp.right) { nx = s1; ny = s2; }
-                    else {
-                        double xyn[8], n1, n2;
+            for (int w = 0; w < 8; w++) u[w * NF] = FF[o + w];
+            u[17 * NF] = FF[o + 8];
 #pragma unroll
-                        for (int q = 0; q < 8; q++) xyn[q] = a.cellxy[(size_t)fp.nb * 8 + q];
-                        metric_row<N>(xyn, fp.nb_fam, fp.nb_ifp, fp.nb_jfp, n1, n2);
-                        nx = R(n1); ny = R(n2);
-                    }
-                }
-                R out[8], az, eig;
-                if (!fp.right) {
-                    rusanov<R>(R(a.gam), Qown, Qnb, nx, ny, fp.sign_l, out, az, eig);
-                } else {
-                    R Fnl[8];
-                    rusanov<R>(R(a.gam), Qnb, Qown, nx, ny, fp.sign_l, Fnl, az, eig);
-                    const R slr = R(fp.sign_l * fp.sign_r);
-#pragma unroll
-                    for (int w = 0; w < 8; w++) out[w] = slr * Fnl[w];
-                }
-#pragma unroll
-                for (int w = 0; w < 8; w++) u[w * NF] = out[w];
-                u[17 * NF] = az;
-            }
-            u[18 * NF] = s1;
-            u[19 * NF] = s2;
-        }
-        // face points: viscous / AV flux from the face-averaged Qv and gradient
-        // (interfaceVISflux / BCVISFLUX)
-        for (int it = tid; it < ncl * 2 * N; it += blockDim.x) {
-            const int c = it / (2 * N), k = it % (2 * N);
-            const int line = k >> 1, r = (k & 1) ? N : 0;
-            const int cell = c0 + c;
-            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
-            double s1d, s2d;
-            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
-            const R s1 = R(s1d), s2 = R(s2d);
-            int lfi, nfp;
-            face_of<N>(fam, r, line, lfi, nfp);
-            const int2 info = a.nbr[(size_t)cell * 4 + lfi];
-            const bool free_face = (info.y >> 4) & 1;
-            R *u = sU + c * UST + line * (N + 1) + r;
-            R qv[NV], g[2 * NV], fv[NV];
-            const R *src = v.qvf_r<R>() + (((size_t)cell * 4 + lfi) * N + nfp) * 8;
-#pragma unroll
-            for (int w = 0; w < 8; w++) qv[w] = src[w];
-            qv[8] = R(0);
-            grad_trace<N, R>(sG + c * 16 * NP, fam, ifp, jfp, g);
-            const R *ot = free_face ? nullptr
-                                    : v.gtr_r<R>() + (((size_t)info.x * 4 + (info.y & 7) - 1) * N + partner_nfp<N>(nfp)) * 16;
-#pragma unroll
-            for (int d = 0; d < 2; d++)
-#pragma unroll
-                for (int w = 0; w < 8; w++) {
-                    const R mine = g[d * NV + w];
-                    const R other = ot ? ot[d * 8 + w] : mine;
-                    g[d * NV + w] = R(0.5) * mine + R(0.5) * other;
-                }
-            visc_point<N, R>(v, sMu + c * 4, fam, ifp, jfp, s1, s2, qv, g, fv);
-#pragma unroll
-            for (int w = 0; w < NV; w++) u[(8 + w) * NF] = fv[w];
+            for (int w = 0; w < NV; w++) u[(8 + w) * NF] = FFv[o + w];
+            u[18 * NF] = R(s1d);
+            u[19 * NF] = R(s2d);
         }
         __syncthreads();
         if (pact) {
@@ -509,7 +475,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(CHORUS
                 dA1 = e1; dA2 = e2;
             } else {
                 const R J = R(jacobian<N>(sX + pc * 8, pis, pjs));
-                const double *qs = sQ + pc * NQ + pk;
+                double *qs = sQ + pc * NQ + pk;
                 const R q0 = R(qs[0]), q1 = R(qs[NP]), q2 = R(qs[2 * NP]);
                 const R au = q1 / q0, av = q2 / q0;
                 const R nA1 = (dA1 + e1) / J, nA2 = (dA2 + e2) / J;
@@ -527,6 +493,8 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(CHORUS
         }
         __syncthreads();
     }
+    // traces of the new state for the next stage's face passes
+    if (a.kind != KIND_RESID) store_traces<N>(sQ, ncl, c0, a.TQ);
 }
 
 }  // namespace chorus
