@@ -31,7 +31,6 @@ struct chorus_ctx {
     double *TQ = nullptr;          // traces of the current state at the face points
     void *FF = nullptr;            // per cell face point: common inviscid flux + A_z
     void *FFv = nullptr;           // viscous path: per cell face point: common viscous flux
-    void *qvf = nullptr;           // viscous path: face-averaged Qv
     void *gtr = nullptr;           // viscous path: own nablaQs face traces
     void *gsol = nullptr;          // viscous path: nablaQs at the solution points
     double *sm = nullptr;          // viscous path: AV indicator per cell
@@ -105,7 +104,7 @@ int launch_stage(chorus_ctx *c, const StageArgs &a) {
     if (t) (void)hipEventRecord(c->ev[c->ev_used], c->stream);
     FaceArgs f = face_args(c);
     if (c->ideal) {
-        if (c->p.precision == CHORUS_MIXED) e = launch_stage_ideal_mixed(c->N, a, c->stream);
+        if (c->p.precision == CHORUS_MIXED) e = launch_stage_ideal_mixed(c->N, a, f, c->stream);
         else e = launch_stage_ideal_fp64(c->N, a, f, c->stream);
     } else {
         ViscArgs v{};
@@ -113,7 +112,6 @@ int launch_stage(chorus_ctx *c, const StageArgs &a) {
         v.cellxy = c->cellxy;
         v.nbr = c->nbr;
         v.Qin = a.Qin;
-        v.qvf = c->qvf;
         v.gtr = c->gtr;
         v.gsol = c->gsol;
         v.sm = c->sm;
@@ -130,7 +128,7 @@ int launch_stage(chorus_ctx *c, const StageArgs &a) {
         f.eig_now = v.eig;
         c->last_parity = c->stage_parity;
         c->stage_parity = 1 - c->stage_parity;
-        if (c->p.precision == CHORUS_MIXED) e = launch_stage_visc_mixed(c->N, a, v, c->stream);
+        if (c->p.precision == CHORUS_MIXED) e = launch_stage_visc_mixed(c->N, a, v, f, c->stream);
         else e = launch_stage_visc_fp64(c->N, a, v, f, c->stream);
     }
     if (t) { (void)hipEventRecord(c->ev[c->ev_used + 1], c->stream); c->ev_used += 2; }
@@ -510,7 +508,6 @@ int chorus_init(const chorus_mesh *m, const chorus_params *p, chorus_ctx **out) 
     HIP_TRY(c, hipMemcpyAsync(c->faces, faces.data(), faces.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
     if (!c->ideal) {
         const size_t nf = nfp;
-        HIP_TRY(c, hipMalloc(&c->qvf, nf * 8 * sizeof(double)));
         HIP_TRY(c, hipMalloc(&c->gtr, nf * 16 * sizeof(double)));
         HIP_TRY(c, hipMalloc(&c->gsol, (size_t)c->ncell * 16 * c->N * c->N * sizeof(double)));
         HIP_TRY(c, hipMalloc(&c->sm, (size_t)c->ncell * sizeof(double)));
@@ -534,7 +531,6 @@ void chorus_free(chorus_ctx *c) {
     (void)hipFree(c->TQ);
     (void)hipFree(c->FF);
     (void)hipFree(c->FFv);
-    (void)hipFree(c->qvf);
     (void)hipFree(c->gtr);
     (void)hipFree(c->gsol);
     (void)hipFree(c->sm);

@@ -74,9 +74,9 @@ hipError_t launch_stage_visc_fp64(int N, const StageArgs &a, const ViscArgs &v, 
 #define V_(n)                                                                                         \
     case n:                                                                                           \
         hipLaunchKernelGGL((k_face_inv<n, double>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);    \
-        hipLaunchKernelGGL((k_grad<n, double>), dim3(vblocks<n>(a.ncell)), dim3(VShape<n>::THREADS), 0, s, v); \
+        hipLaunchKernelGGL((k_grad<n, double>), dim3(vblocks<n>(a.ncell)), dim3(V2Shape<n>::THREADS), 0, s, v); \
         hipLaunchKernelGGL((k_face_visc<n, double>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);   \
-        hipLaunchKernelGGL((k_stage_visc<n, double>), dim3(vblocks<n>(a.ncell)), dim3(VShape<n>::THREADS), 0, s, a, v); \
+        hipLaunchKernelGGL((k_stage_visc<n, double>), dim3(vblocks<n>(a.ncell)), dim3(V2Shape<n>::THREADS), 0, s, a, v); \
         return hipGetLastError();
     switch (N) {
         V_(2) V_(3) V_(4) V_(5) V_(6) V_(7) V_(8)

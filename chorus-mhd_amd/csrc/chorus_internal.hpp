@@ -23,6 +23,15 @@ constexpr int kMaxN = 8;       // solution points per direction supported by the
 #define CHORUS_F64_UNROLL 3
 #endif
 template <class R, int N> constexpr int contraction_unroll() { return sizeof(R) == 8 ? CHORUS_F64_UNROLL : N + 1; }
+// mixed form: unroll of the solution-point index in the delta traces (FP64
+// loads) and in the gradient traces; full unrolling hoists every load and
+// exhausts the register file of the viscous stage kernel
+#ifndef CHORUS_MX_TRACE_UNROLL
+#define CHORUS_MX_TRACE_UNROLL 2
+#endif
+#ifndef CHORUS_MX_GRAD_UNROLL
+#define CHORUS_MX_GRAD_UNROLL 2
+#endif
 
 // stage combination kinds (FP64 in both precision forms; the association of
 // each expression is the reference's, iterations.f90:92-217)
@@ -83,7 +92,6 @@ struct ViscArgs {
     const double *cellxy;
     const int2 *nbr;
     const double *Qin;
-    void *qvf;          // face-averaged Qv   [cell][lf][nfp][8]   (Real)
     void *gtr;          // own nablaQs traces [cell][lf][nfp][16]  (Real)
     void *gsol;         // nablaQs at the solution points [cell][16][N*N] (Real)
     double *sm;         // log10 smoothness indicator of density per cell
@@ -92,7 +100,6 @@ struct ViscArgs {
     const double *TQ;   // traces of Qin at the face points
     int artiv;
     double mu, eta0, gam, h_msh, pi;
-    template <class R> __host__ __device__ R *qvf_r() const { return (R *)qvf; }
     template <class R> __host__ __device__ R *gtr_r() const { return (R *)gtr; }
     template <class R> __host__ __device__ R *gsol_r() const { return (R *)gsol; }
 };
@@ -135,10 +142,10 @@ __device__ __forceinline__ void stage_combine(const StageArgs &a, size_t base, i
 // launchers (defined in the kernel translation units); return hipError_t
 hipError_t launch_stage_ideal_fp64(int N, const StageArgs &a, const FaceArgs &f, hipStream_t s);
 hipError_t launch_trace(int N, int ncell, const double *Q, double *TQ, hipStream_t s);
-hipError_t launch_stage_ideal_mixed(int N, const StageArgs &a, hipStream_t s);
+hipError_t launch_stage_ideal_mixed(int N, const StageArgs &a, const FaceArgs &f, hipStream_t s);
 hipError_t launch_ct(int N, const CTArgs &a, hipStream_t s);
 hipError_t launch_stage_visc_fp64(int N, const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s);
-hipError_t launch_stage_visc_mixed(int N, const StageArgs &a, const ViscArgs &v, hipStream_t s);
+hipError_t launch_stage_visc_mixed(int N, const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s);
 int ct_blocks(int N, int ncell);
 int stage_blocks(int N, int ncell);
 hipError_t upload_operators_fp64();

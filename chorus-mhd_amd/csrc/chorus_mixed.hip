@@ -10,12 +10,14 @@ hipError_t upload_operators_mixed() {
     return e != hipSuccess ? e : upload_av_tables();
 }
 
-template <int N> static int nblocks(int ncell) { return (ncell + Shape<N>::CPB - 1) / Shape<N>::CPB; }
+template <class S> static int blocks_of(int ncell) { return (ncell + S::CPB - 1) / S::CPB; }
+static int fblocks(int N, int nface) { return (nface * N + 255) / 256; }
 
-hipError_t launch_stage_ideal_mixed(int N, const StageArgs &a, hipStream_t s) {
+hipError_t launch_stage_ideal_mixed(int N, const StageArgs &a, const FaceArgs &f, hipStream_t s) {
 #define L_(n)                                                                                     \
     case n:                                                                                       \
-        hipLaunchKernelGGL((k_stage_ideal_mx<n>), dim3(nblocks<n>(a.ncell)), dim3(Shape<n>::THREADS), 0, s, a); \
+        hipLaunchKernelGGL((k_face_inv_mx<n>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);    \
+        hipLaunchKernelGGL((k_stage_ideal_mx<n>), dim3(blocks_of<IShape<n>>(a.ncell)), dim3(IShape<n>::THREADS), 0, s, a); \
         return hipGetLastError();
     switch (N) {
         L_(2) L_(3) L_(4) L_(5) L_(6) L_(7) L_(8)
@@ -24,13 +26,13 @@ hipError_t launch_stage_ideal_mixed(int N, const StageArgs &a, hipStream_t s) {
 #undef L_
 }
 
-template <int N> static int vblocks(int ncell) { return (ncell + VShape<N>::CPB - 1) / VShape<N>::CPB; }
-
-hipError_t launch_stage_visc_mixed(int N, const StageArgs &a, const ViscArgs &v, hipStream_t s) {
+hipError_t launch_stage_visc_mixed(int N, const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s) {
 #define V_(n)                                                                                         \
     case n:                                                                                           \
-        hipLaunchKernelGGL((k_grad_mx<n>), dim3(vblocks<n>(a.ncell)), dim3(VShape<n>::THREADS), 0, s, v); \
-        hipLaunchKernelGGL((k_stage_visc_mx<n>), dim3(vblocks<n>(a.ncell)), dim3(VShape<n>::THREADS), 0, s, a, v); \
+        hipLaunchKernelGGL((k_face_inv_mx<n>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);        \
+        hipLaunchKernelGGL((k_grad_mx<n>), dim3(blocks_of<VShape<n>>(a.ncell)), dim3(VShape<n>::THREADS), 0, s, v); \
+        hipLaunchKernelGGL((k_face_visc_mx<n>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);       \
+        hipLaunchKernelGGL((k_stage_visc_mx<n>), dim3(blocks_of<VShape<n>>(a.ncell)), dim3(VShape<n>::THREADS), 0, s, a, v); \
         return hipGetLastError();
     switch (N) {
         V_(2) V_(3) V_(4) V_(5) V_(6) V_(7) V_(8)

@@ -143,44 +143,6 @@ __device__ __forceinline__ double trace1(const double *q1, int fam, int ifp, int
     return a;
 }
 
-// Common face-point resolution: which cell is the reference's left cell of
-// this face and where the partner point lies (interfaceflux_all.f90:16-37,
-// cyclocflux_all.f90:15-77, bcflux.f90:19-53).
-struct FacePoint {
-    int nb;          // partner cell (-1 for FREE)
-    int nb_fam, nb_ifp, nb_jfp;
-    bool right;      // this cell is the right cell
-    bool free_face;
-    int sign_l, sign_r;
-};
-
-template <int N>
-__device__ __forceinline__ FacePoint resolve_face(int2 info, int fam, int r, int line) {
-    FacePoint fp;
-    int lf, nfp_my;
-    if (fam == 0) { lf = (r == 0) ? 4 : 2; nfp_my = (r == 0) ? (N - 1 - line) : line; }
-    else          { lf = (r == 0) ? 1 : 3; nfp_my = (r == 0) ? line : (N - 1 - line); }
-    (void)lf;
-    fp.free_face = (info.y >> 4) & 1;
-    fp.right = (info.y >> 3) & 1;
-    fp.nb = info.x;
-    int own_sign_left = (r == 0) ? -1 : 1;     // sign_l if this point is the left point
-    int own_sign_right = (r == N) ? -1 : 1;    // sign_r if this point is the right point
-    if (fp.free_face) {
-        fp.nb_fam = fam; fp.nb_ifp = 0; fp.nb_jfp = 0;
-        fp.sign_l = own_sign_left; fp.sign_r = 1;
-        return fp;
-    }
-    int lf_nb = info.y & 7;
-    fp.nb_fam = lf_family(lf_nb);
-    int nfp_nb = N - 1 - nfp_my;               // partner index runs the other way
-    face2fp<N>(lf_nb, nfp_nb, fp.nb_ifp, fp.nb_jfp);
-    int nbr = fp.nb_fam == 0 ? fp.nb_ifp : fp.nb_jfp;
-    if (!fp.right) { fp.sign_l = own_sign_left; fp.sign_r = (nbr == N) ? -1 : 1; }
-    else           { fp.sign_l = (nbr == 0) ? -1 : 1; fp.sign_r = own_sign_right; }
-    return fp;
-}
-
 // reference sign of a cell's contribution at its local face lfi (0..3):
 // -1 on faces 1 and 4 (flux point index 0 of the family), +1 on faces 2 and 3
 // (interfaceflux_all.f90:28-37)

@@ -1,4 +1,5 @@
-// Mixed-precision stage kernels: FP32 flux arithmetic in delta form.
+// Mixed-precision kernels: FP32 flux arithmetic in delta form on the
+// face-ordered pipeline of the FP64 path.
 //
 // Plain FP32 flux values of size |F| lose log10(N^2/h) digits when the
 // divergence differences them over a cell.  Here every FP32 quantity on a flux
@@ -6,7 +7,6 @@
 //   dQf   = sum_sp (Q(sp) - Qbar) L(r,sp)                 (FP64 difference, FP32 MAC)
 //   dF    = F(Qbar + dQf) - F(Qbar)                       (exact difference formulas)
 //   dF1   = dF*S11 + dG*S12                               (contravariant)
-//   face: dF* = 1/2 (dF(Qin)+dF(Qout)).S - 1/2 lambda |S| sigma (dQout - dQin)
 // and the divergence is taken of dF1 only: the constant part F(Qbar).S has a
 // vanishing discrete divergence on straight-sided bilinear cells (the metric
 // identity behind free-stream preservation: d/dpsi(y_eta) - d/deta(y_psi) = 0,
@@ -14,34 +14,40 @@
 // changes the residual only at FP64 round-off.  The same holds for A_z and for
 // the central Qv of the gradient.  FP32 errors are then relative to O(h)
 // deviations, i.e. eps32*N^2 relative to the residual, independent of h.
-// The state, the RK sums, the metrics, the AV indicator and the CT update stay
-// FP64.
+//
+// Every flux point, interior or face, evaluates dF1 of its own trace; a face
+// point then adds the jump term of the common flux, computed once per face
+// point by k_face_inv_mx from the two FP64 traces of that point:
+//   Fnl - F(Q_L).S = 1/2 (F(Q_R) - F(Q_L)).S - 1/2 sign_l |S| lambda (Q_R - Q_L)
+//   for the right cell (sign_l sign_r)(Fnl - F(Q_R).S)
+// where F(Q_R) - F(Q_L) is again an exact difference about Q_L, so only jumps
+// and deviations are ever rounded to FP32.
+// The state, the RK sums, the traces, the metrics, the AV indicator and the
+// CT update stay FP64.
 #pragma once
 
 #include "kernels_visc.hpp"
+
+#ifndef CHORUS_MX_CUNROLL
+#define CHORUS_MX_CUNROLL 2
+#endif
 
 namespace chorus {
 
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
-// reference (mean) state of a cell and the derived quantities the delta fluxes need
+// reference state of a cell (or a face point) and the derived quantities the
+// delta fluxes need
 struct Ref {
     float q[NV];          // Qbar
     float u[3];           // m/rho
     float b2, p, H, ub;   // |B|^2, pressure, E + p + |B|^2/2, u.B
 };
 
-template <int N>
-__device__ __forceinline__ void make_ref(const double *q, double gam, Ref &s, double *qbar) {
-    constexpr int NP = N * N;
+__device__ __forceinline__ void ref_from(const double *qbar, double gam, Ref &s) {
 #pragma unroll
-    for (int v = 0; v < NV; v++) {
-        double a = 0.0;
-        for (int p = 0; p < NP; p++) a += q[v * NP + p];
-        qbar[v] = a / NP;
-        s.q[v] = float(qbar[v]);
-    }
+    for (int v = 0; v < NV; v++) s.q[v] = float(qbar[v]);
     const double ux = qbar[1] / qbar[0], uy = qbar[2] / qbar[0], uz = qbar[3] / qbar[0];
     const double b2 = qbar[5] * qbar[5] + qbar[6] * qbar[6] + qbar[7] * qbar[7];
     const double p = (gam - 1.0) * (qbar[4] - 0.5 * (qbar[1] * ux + qbar[2] * uy + qbar[3] * uz) - 0.5 * b2);
@@ -50,6 +56,22 @@ __device__ __forceinline__ void make_ref(const double *q, double gam, Ref &s, do
     s.p = float(p);
     s.H = float(qbar[4] + p + 0.5 * b2);
     s.ub = float(ux * qbar[5] + uy * qbar[6] + uz * qbar[7]);
+}
+
+// the block's cell means (FP64, one thread per cell and variable) and their
+// reference states; called by all threads of the block
+template <int N>
+__device__ __forceinline__ void block_refs(const double *sQ, int ncl, double gam, double *sBar, Ref *sRef) {
+    constexpr int NP = N * N, NQ = NV * NP;
+    for (int i = threadIdx.x; i < ncl * NV; i += blockDim.x) {
+        const double *q = sQ + (i / NV) * NQ + (i % NV) * NP;
+        double a = 0.0;
+        for (int p = 0; p < NP; p++) a += q[p];
+        sBar[i] = a / NP;
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < ncl; c += blockDim.x) ref_from(sBar + c * NV, gam, sRef[c]);
+    __syncthreads();
 }
 
 // F(Qbar+d) - F(Qbar), G(Qbar+d) - G(Qbar) for the ideal-MHD fluxes of
@@ -108,51 +130,109 @@ __device__ __forceinline__ float wave_speed(const float *Ql, const float *Qr, fl
     return 0.5f * fabsf((ul + ur) * nx + (vl + vr) * ny) + c_f;
 }
 
-// delta Rusanov flux in this cell's contravariant frame, A_z upwinding as
-// getrusanov.f90:98-103 (by the reference-left cell's normal velocity)
-__device__ __forceinline__ void drusanov(const Ref &s, const float *dqi, const float *dqo, float s1, float s2,
-                                         int sigma, bool right, float gam, float *dfn, float &daz, float &eig) {
-    float dFi[8], dGi[8], dFo[8], dGo[8], Qi[NV], Qo[NV];
-    dflux(s, dqi, gam - 1.f, dFi, dGi);
-    dflux(s, dqo, gam - 1.f, dFo, dGo);
-#pragma unroll
-    for (int v = 0; v < NV; v++) { Qi[v] = s.q[v] + dqi[v]; Qo[v] = s.q[v] + dqo[v]; }
-    const float mag = fsqrt(s1 * s1 + s2 * s2);
-    const float im = frcp(mag);
-    const float nx = s1 * im * float(sigma), ny = s2 * im * float(sigma);   // outward unit normal
-    eig = wave_speed(Qi, Qo, nx, ny, gam);
-    const float diss = 0.5f * eig * mag * float(sigma);
-#pragma unroll
-    for (int v = 0; v < 8; v++)
-        dfn[v] = 0.5f * ((dFi[v] + dFo[v]) * s1 + (dGi[v] + dGo[v]) * s2) - diss * (dqo[v] - dqi[v]);
-    // left cell's outward normal velocity: ours if we are left, the partner's
-    // velocity against our inward normal if we are right
-    const float vn = right ? -(Qo[1] * nx + Qo[2] * ny) * frcp(Qo[0]) : (Qi[1] * nx + Qi[2] * ny) * frcp(Qi[0]);
-    const bool take_left = vn > 0.f;
-    daz = (take_left != right) ? dqi[8] : dqo[8];
+// deviations of the block's cells from their means, rounded once to FP32:
+// dq[c][var][p] = float(Q - Qbar); called by all threads of the block
+template <int N>
+__device__ __forceinline__ void block_deviations(const double *sQ, const double *sBar, int ncl, float *dq) {
+    constexpr int NP = N * N, NQ = NV * NP;
+    for (int i = threadIdx.x; i < ncl * NQ; i += blockDim.x) {
+        const int c = i / NQ, v = (i % NQ) / NP;
+        dq[i] = float(sQ[i] - sBar[c * NV + v]);
+    }
+    __syncthreads();
 }
 
-// delta trace: sum_sp float(q(sp) - qbar) L(r,sp) of all 9 variables
+// delta trace: sum_sp dq(sp) L(r,sp) of all 9 variables
 template <int N>
-__device__ __forceinline__ void dtrace9(const double *q, const double *qbar, int fam, int ifp, int jfp, float *d) {
+__device__ __forceinline__ void dtrace9(const float *dq, int fam, int ifp, int jfp, float *d) {
     constexpr int NP = N * N;
 #pragma unroll
     for (int v = 0; v < NV; v++) d[v] = 0.f;
-#pragma unroll
+#pragma unroll CHORUS_MX_TRACE_UNROLL
     for (int sp = 0; sp < N; sp++) {
         const float l = fam == 0 ? c_Lf[N][ifp * N + sp] : c_Lf[N][jfp * N + sp];
         const int idx = fam == 0 ? jfp * N + sp : sp * N + ifp;
 #pragma unroll
-        for (int v = 0; v < NV; v++) d[v] += float(q[v * NP + idx] - qbar[v]) * l;
+        for (int v = 0; v < NV; v++) d[v] += dq[v * NP + idx] * l;
     }
 }
 
 // ---------------------------------------------------------------------------
-// Ideal-path stage kernel, mixed form
+// Face pass, mixed form: the jump terms of the Rusanov flux for both cells of
+// each face point (see the header) and the A_z upwind jump; the stage maximum
+// of the FP32 wave speed with ARTIV.  A FREE face has no jump (zeros).
+// FF[cell][lfi][nfp][9] (float): 8 flux jump terms + A_z jump.
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ void __launch_bounds__(256) k_stage_ideal_mx(StageArgs a) {
-    using S = Shape<N>;
+__global__ void __launch_bounds__(256) k_face_inv_mx(FaceArgs a) {
+    __shared__ double red[256];
+    const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    double emax = 0.0;
+    if (t < a.nface * N) {
+        const int f = t / N, k = t % N;
+        const int4 fc = a.faces[f];
+        const bool free_face = fc.z < 0;
+        const size_t pl = (((size_t)fc.x * 4 + fc.y) * N + k) * NV;
+        const size_t pr = free_face ? pl : (((size_t)fc.z * 4 + fc.w) * N + (N - 1 - k)) * NV;
+        double ql[NV];
+        float d[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            ql[v] = a.TQ[pl + v];
+            d[v] = float(a.TQ[pr + v] - ql[v]);
+        }
+        Ref s;
+        ref_from(ql, a.gam, s);
+        int ifp, jfp;
+        face2fp<N>(fc.y + 1, k, ifp, jfp);
+        double xy[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) xy[q] = a.cellxy[(size_t)fc.x * 8 + q];
+        double m1, m2;
+        metric_row<N>(xy, lf_family(fc.y + 1), ifp, jfp, m1, m2);
+        const float s1 = float(m1), s2 = float(m2);
+        const float gam = float(a.gam);
+        float dF[8], dG[8];
+        dflux(s, d, gam - 1.f, dF, dG);
+        const float sl = float(side_sign(fc.y));
+        const float mag = fsqrt(s1 * s1 + s2 * s2);
+        const float im = frcp(mag);
+        float Qr[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) Qr[v] = s.q[v] + d[v];
+        const float eig = wave_speed(s.q, Qr, s1 * im, s2 * im, gam);
+        const float dis = 0.5f * sl * mag * eig;
+        float *FF = (float *)a.FF;
+        if (free_face) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) FF[pl + v] = 0.f;
+        } else {
+            const float slr = sl * float(-side_sign(fc.w));
+#pragma unroll
+            for (int v = 0; v < 8; v++) {
+                const float h = 0.5f * (dF[v] * s1 + dG[v] * s2);
+                const float e = dis * d[v];
+                FF[pl + v] = h - e;
+                FF[pr + v] = slr * (-h - e);
+            }
+            // upwind A_z by the left cell's outward normal velocity (getrusanov.f90:98-103)
+            const bool take_left = (s.q[1] * s1 + s.q[2] * s2) * sl > 0.f;
+            FF[pl + 8] = take_left ? 0.f : d[8];
+            FF[pr + 8] = take_left ? -d[8] : 0.f;
+        }
+        emax = double(eig);
+    }
+    if (a.artiv) block_max_atomic<float>(emax, red, a.eig);
+}
+
+// ---------------------------------------------------------------------------
+// Ideal-path stage kernel, mixed form: every flux point evaluates the delta
+// flux of its own trace, face points add the jump terms of k_face_inv_mx; the
+// residual is formed in FP32 and combined into the FP64 state.
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(IShape<N>::THREADS) k_stage_ideal_mx(StageArgs a) {
+    using S = IShape<N>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
     constexpr int FST = 11 * NF + 1;     // per-cell stride: dF1 (8) + dA_z + metric rows (2)
     __shared__ double sQ[CPB * NQ];
@@ -160,6 +240,7 @@ __global__ void __launch_bounds__(256) k_stage_ideal_mx(StageArgs a) {
     __shared__ double sBar[CPB * NV];
     __shared__ Ref sRef[CPB];
     __shared__ float sF[CPB * FST];
+    __shared__ float sD[CPB * NQ];       // deviations from the cell means
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = blk * CPB;
@@ -167,55 +248,39 @@ __global__ void __launch_bounds__(256) k_stage_ideal_mx(StageArgs a) {
     for (int i = tid; i < ncl * NQ; i += blockDim.x) sQ[i] = a.Qin[(size_t)c0 * NQ + i];
     for (int i = tid; i < ncl * 8; i += blockDim.x) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
     __syncthreads();
-    if (tid < ncl) make_ref<N>(sQ + tid * NQ, a.gam, sRef[tid], sBar + tid * NV);
-    __syncthreads();
+    block_refs<N>(sQ, ncl, a.gam, sBar, sRef);
+    block_deviations<N>(sQ, sBar, ncl, sD);
     const float gam = float(a.gam);
     const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
     const bool pact = (tid < CPB * NP) && (pc < ncl);
+    const float *FF = (const float *)a.FF;
     float dF[8], dA1 = 0.f, dA2 = 0.f;
 #pragma unroll 1
     for (int fam = 0; fam < 2; fam++) {
-        for (int it = tid; it < ncl * N * (N - 1); it += blockDim.x) {
-            const int c = it / (N * (N - 1)), k = it % (N * (N - 1));
-            const int line = k / (N - 1), r = 1 + k % (N - 1);
+        for (int it = tid; it < ncl * NF; it += blockDim.x) {
+            const int c = it / NF, k = it % NF;
+            const int line = k / (N + 1), r = k % (N + 1);
             const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+            float jump[NV];
+#pragma unroll
+            for (int v = 0; v < NV; v++) jump[v] = 0.f;
+            if (r == 0 || r == N) {
+                int lfi, nfp;
+                face_of<N>(fam, r, line, lfi, nfp);
+                const float *src = FF + (((size_t)(c0 + c) * 4 + lfi) * N + nfp) * NV;
+#pragma unroll
+                for (int v = 0; v < NV; v++) jump[v] = src[v];
+            }
             double s1d, s2d;
             metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
             const float s1 = float(s1d), s2 = float(s2d);
             float d[NV], F[8], G[8];
-            dtrace9<N>(sQ + c * NQ, sBar + c * NV, fam, ifp, jfp, d);
+            dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d);
             dflux(sRef[c], d, gam - 1.f, F, G);
             float *f = sF + c * FST + line * (N + 1) + r;
 #pragma unroll
-            for (int v = 0; v < 8; v++) f[v * NF] = F[v] * s1 + G[v] * s2;
-            f[8 * NF] = d[8];
-            f[9 * NF] = s1;
-            f[10 * NF] = s2;
-        }
-        for (int it = tid; it < ncl * 2 * N; it += blockDim.x) {
-            const int c = it / (2 * N), k = it % (2 * N);
-            const int line = k >> 1, r = (k & 1) ? N : 0;
-            const int cell = c0 + c;
-            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
-            double s1d, s2d;
-            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
-            const float s1 = float(s1d), s2 = float(s2d);
-            const int lfi = fam == 0 ? (r == 0 ? 3 : 1) : (r == 0 ? 0 : 2);
-            const FacePoint fp = resolve_face<N>(a.nbr[(size_t)cell * 4 + lfi], fam, r, line);
-            float di[NV], dq[NV];
-            dtrace9<N>(sQ + c * NQ, sBar + c * NV, fam, ifp, jfp, di);
-            if (fp.free_face) {
-#pragma unroll
-                for (int v = 0; v < NV; v++) dq[v] = di[v];
-            } else {
-                dtrace9<N>(a.Qin + (size_t)fp.nb * NQ, sBar + c * NV, fp.nb_fam, fp.nb_ifp, fp.nb_jfp, dq);
-            }
-            float fn[8], daz, eig;
-            drusanov(sRef[c], di, dq, s1, s2, r == N ? 1 : -1, fp.right && !fp.free_face, gam, fn, daz, eig);
-            float *f = sF + c * FST + line * (N + 1) + r;
-#pragma unroll
-            for (int v = 0; v < 8; v++) f[v * NF] = fn[v];
-            f[8 * NF] = daz;
+            for (int v = 0; v < 8; v++) f[v * NF] = F[v] * s1 + G[v] * s2 + jump[v];
+            f[8 * NF] = d[8] + jump[8];
             f[9 * NF] = s1;
             f[10 * NF] = s2;
         }
@@ -226,7 +291,7 @@ __global__ void __launch_bounds__(256) k_stage_ideal_mx(StageArgs a) {
             float d[8], e1 = 0.f, e2 = 0.f;
 #pragma unroll
             for (int v = 0; v < 8; v++) d[v] = 0.f;
-#pragma unroll
+#pragma unroll CHORUS_MX_CUNROLL
             for (int r = 0; r <= N; r++) {
                 const float m = c_Mf[N][r * N + pos];
 #pragma unroll
@@ -252,14 +317,16 @@ __global__ void __launch_bounds__(256) k_stage_ideal_mx(StageArgs a) {
         }
         __syncthreads();
     }
-    // traces of the new state (the constrained-transport pass reads them)
     if (a.kind != KIND_RESID) store_traces<N>(sQ, ncl, c0, a.TQ);
 }
 
 // ---------------------------------------------------------------------------
-// Viscous / AV path, mixed form: k_grad_mx + k_stage_visc_mx (same data flow
-// as k_grad + k_stage_visc; the stored face averages are deviations from the
-// owning cell's mean state)
+// Viscous / AV path, mixed form: the launches of the FP64 path.  The central
+// Qv of the gradient pass is a deviation from the cell mean (own delta trace
+// plus half the jump of the two FP64 traces at face points); nablaQs, its
+// traces and the viscous fluxes are plain FP32 (they carry no large constant
+// part); the viscous face flux is evaluated once per face point from the FP64
+// trace average.
 // ---------------------------------------------------------------------------
 template <int N>
 __global__ void __launch_bounds__(VShape<N>::THREADS) k_grad_mx(ViscArgs a) {
@@ -271,9 +338,9 @@ __global__ void __launch_bounds__(VShape<N>::THREADS) k_grad_mx(ViscArgs a) {
     __shared__ double sBar[CPB * NV];
     __shared__ Ref sRef[CPB];
     __shared__ float sV[CPB * VST];
+    __shared__ float sD[CPB * NQ];       // deviations from the cell means
     __shared__ float sG[CPB * 16 * NP];
     __shared__ double sAV[CPB * NP];
-    __shared__ double sRed[VShape<N>::THREADS];
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = blk * CPB;
@@ -281,62 +348,38 @@ __global__ void __launch_bounds__(VShape<N>::THREADS) k_grad_mx(ViscArgs a) {
     for (int i = tid; i < ncl * NQ; i += blockDim.x) sQ[i] = a.Qin[(size_t)c0 * NQ + i];
     for (int i = tid; i < ncl * 8; i += blockDim.x) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
     __syncthreads();
-    if (tid < ncl) make_ref<N>(sQ + tid * NQ, a.gam, sRef[tid], sBar + tid * NV);
-    __syncthreads();
-    const float gam = float(a.gam);
+    block_refs<N>(sQ, ncl, a.gam, sBar, sRef);
+    block_deviations<N>(sQ, sBar, ncl, sD);
     const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
     const bool pact = (tid < CPB * NP) && (pc < ncl);
-    double emax = 0.0;
 #pragma unroll 1
     for (int fam = 0; fam < 2; fam++) {
-        for (int it = tid; it < ncl * N * (N - 1); it += blockDim.x) {
-            const int c = it / (N * (N - 1)), k = it % (N * (N - 1));
-            const int line = k / (N - 1), r = 1 + k % (N - 1);
+        for (int it = tid; it < ncl * NF; it += blockDim.x) {
+            const int c = it / NF, k = it % NF;
+            const int line = k / (N + 1), r = k % (N + 1);
             const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+            float half_jump[8];
+#pragma unroll
+            for (int v = 0; v < 8; v++) half_jump[v] = 0.f;
+            if (r == 0 || r == N) {
+                int lfi, nfp;
+                face_of<N>(fam, r, line, lfi, nfp);
+                const int cell = c0 + c;
+                const int2 info = a.nbr[(size_t)cell * 4 + lfi];
+                if (!((info.y >> 4) & 1)) {
+                    const double *own = a.TQ + (((size_t)cell * 4 + lfi) * N + nfp) * NV;
+                    const double *oth = a.TQ + (((size_t)info.x * 4 + (info.y & 7) - 1) * N + partner_nfp<N>(nfp)) * NV;
+#pragma unroll
+                    for (int v = 0; v < 8; v++) half_jump[v] = 0.5f * float(oth[v] - own[v]);
+                }
+            }
             double s1d, s2d;
             metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
             float d[NV];
-            dtrace9<N>(sQ + c * NQ, sBar + c * NV, fam, ifp, jfp, d);
+            dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d);
             float *q = sV + c * VST + line * (N + 1) + r;
 #pragma unroll
-            for (int v = 0; v < 8; v++) q[v * NF] = d[v];
-            q[8 * NF] = float(s1d);
-            q[9 * NF] = float(s2d);
-        }
-        for (int it = tid; it < ncl * 2 * N; it += blockDim.x) {
-            const int c = it / (2 * N), k = it % (2 * N);
-            const int line = k >> 1, r = (k & 1) ? N : 0;
-            const int cell = c0 + c;
-            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
-            double s1d, s2d;
-            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
-            int lfi, nfp;
-            face_of<N>(fam, r, line, lfi, nfp);
-            const FacePoint fp = resolve_face<N>(a.nbr[(size_t)cell * 4 + lfi], fam, r, line);
-            float di[NV], dq[NV], qv[8];
-            dtrace9<N>(sQ + c * NQ, sBar + c * NV, fam, ifp, jfp, di);
-            if (fp.free_face) {
-#pragma unroll
-                for (int v = 0; v < NV; v++) dq[v] = di[v];
-            } else {
-                dtrace9<N>(a.Qin + (size_t)fp.nb * NQ, sBar + c * NV, fp.nb_fam, fp.nb_ifp, fp.nb_jfp, dq);
-            }
-#pragma unroll
-            for (int v = 0; v < 8; v++) qv[v] = 0.5f * di[v] + 0.5f * dq[v];
-            float *dst = a.qvf_r<float>() + (((size_t)cell * 4 + lfi) * N + nfp) * 8;
-#pragma unroll
-            for (int v = 0; v < 8; v++) dst[v] = qv[v];
-            if (a.artiv) {
-                float Qi[NV], Qo[NV];
-#pragma unroll
-                for (int v = 0; v < NV; v++) { Qi[v] = sRef[c].q[v] + di[v]; Qo[v] = sRef[c].q[v] + dq[v]; }
-                const float s1 = float(s1d), s2 = float(s2d);
-                const float im = frcp(fsqrt(s1 * s1 + s2 * s2));
-                emax = fmax(emax, double(wave_speed(Qi, Qo, s1 * im, s2 * im, gam)));
-            }
-            float *q = sV + c * VST + line * (N + 1) + r;
-#pragma unroll
-            for (int v = 0; v < 8; v++) q[v * NF] = qv[v];
+            for (int v = 0; v < 8; v++) q[v * NF] = d[v] + half_jump[v];
             q[8 * NF] = float(s1d);
             q[9 * NF] = float(s2d);
         }
@@ -377,7 +420,76 @@ __global__ void __launch_bounds__(VShape<N>::THREADS) k_grad_mx(ViscArgs a) {
         }
         a.gtr_r<float>()[((size_t)(c0 + c) * 4 * N + fpi) * 16 + g] = acc;
     }
-    if (a.artiv) av_sensor_and_eigmax<N>(a, sQ, sAV, sRed, ncl, c0, emax);
+    if (a.artiv) av_sensor<N>(a.sm, sQ, sAV, ncl, c0);
+}
+
+// viscous and AV face flux, mixed form: k_face_visc in FP32 from the FP64
+// trace average
+template <int N>
+__global__ void __launch_bounds__(256) k_face_visc_mx(FaceArgs a) {
+    const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= a.nface * N) return;
+    const int f = t / N, k = t % N;
+    const int4 fc = a.faces[f];
+    const bool free_face = fc.z < 0;
+    const int kr = N - 1 - k;
+    float sl1, sl2, sr1 = 0.f, sr2 = 0.f;
+    {
+        int ifp, jfp;
+        double xy[8], m1, m2;
+        face2fp<N>(fc.y + 1, k, ifp, jfp);
+#pragma unroll
+        for (int q = 0; q < 8; q++) xy[q] = a.cellxy[(size_t)fc.x * 8 + q];
+        metric_row<N>(xy, lf_family(fc.y + 1), ifp, jfp, m1, m2);
+        sl1 = float(m1); sl2 = float(m2);
+        if (!free_face) {
+            face2fp<N>(fc.w + 1, kr, ifp, jfp);
+#pragma unroll
+            for (int q = 0; q < 8; q++) xy[q] = a.cellxy[(size_t)fc.z * 8 + q];
+            metric_row<N>(xy, lf_family(fc.w + 1), ifp, jfp, m1, m2);
+            sr1 = float(m1); sr2 = float(m2);
+        }
+    }
+    const size_t pl = ((size_t)fc.x * 4 + fc.y) * N + k;
+    const size_t pr = free_face ? pl : ((size_t)fc.z * 4 + fc.w) * N + kr;
+    float qv[NV], g[2 * NV];
+#pragma unroll
+    for (int v = 0; v < 8; v++) qv[v] = float(0.5 * a.TQ[pl * NV + v] + 0.5 * a.TQ[pr * NV + v]);
+    qv[8] = 0.f;
+    const float *gl = (const float *)a.gtr + pl * 16, *gr = (const float *)a.gtr + pr * 16;
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+#pragma unroll
+        for (int w = 0; w < 8; w++) g[d * NV + w] = 0.5f * (gl[d * 8 + w] + gr[d * 8 + w]);
+        g[d * NV + 8] = 0.f;
+    }
+    const float gam = float(a.gam), lam = -2.f / 3.f, prandt = 0.72f;
+    float fl[NV], fr[NV], Fv[NV], Gv[NV];
+#pragma unroll
+    for (int w = 0; w < NV; w++) { fl[w] = 0.f; fr[w] = 0.f; }
+    if (a.mu != 0.0 || a.eta0 != 0.0) {
+        vflux<float>(gam, lam, prandt, float(a.mu), float(a.eta0), qv, g, Fv, Gv);
+#pragma unroll
+        for (int w = 0; w < NV; w++) { fl[w] = Fv[w] * sl1 + Gv[w] * sl2; fr[w] = Fv[w] * sr1 + Gv[w] * sr2; }
+    }
+    if (a.artiv) {
+        const double eps0 = *a.eig_now * a.h_msh / (double)(N - 1);
+        const double ml = mu_av(a.sm[fc.x], eps0, a.pi);
+        const double mf = free_face ? ml : 0.5 * (ml + mu_av(a.sm[fc.z], eps0, a.pi));
+        vflux<float>(gam, lam, prandt, float(mf), 0.f, qv, g, Fv, Gv);
+#pragma unroll
+        for (int w = 0; w < NV; w++) {
+            fl[w] += Fv[w] * sl1 + Gv[w] * sl2;
+            fr[w] += Fv[w] * sr1 + Gv[w] * sr2;
+        }
+    }
+    float *FFv = (float *)a.FFv;
+#pragma unroll
+    for (int w = 0; w < NV; w++) FFv[pl * NV + w] = fl[w];
+    if (!free_face) {
+#pragma unroll
+        for (int w = 0; w < NV; w++) FFv[pr * NV + w] = fr[w];
+    }
 }
 
 template <int N>
@@ -391,6 +503,7 @@ __global__ void __launch_bounds__(VShape<N>::THREADS) k_stage_visc_mx(StageArgs 
     __shared__ Ref sRef[CPB];
     __shared__ float sG[CPB * 16 * NP];
     __shared__ float sU[CPB * UST];
+    __shared__ float sD[CPB * NQ];       // deviations from the cell means
     __shared__ double sMu[CPB * 4];
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -400,7 +513,7 @@ __global__ void __launch_bounds__(VShape<N>::THREADS) k_stage_visc_mx(StageArgs 
     for (int i = tid; i < ncl * 8; i += blockDim.x) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
     for (int i = tid; i < ncl * 16 * NP; i += blockDim.x) sG[i] = v.gsol_r<float>()[(size_t)c0 * 16 * NP + i];
     if (v.artiv) {
-        const double eps0 = *v.eig * v.h_msh / (double)(N - 1) * 1.0;
+        const double eps0 = *v.eig * v.h_msh / (double)(N - 1);
         for (int i = tid; i < ncl * 4; i += blockDim.x) {
             const int c = i / 4, lfi = i % 4, cell = c0 + c;
             const int2 info = a.nbr[(size_t)cell * 4 + lfi];
@@ -412,90 +525,70 @@ __global__ void __launch_bounds__(VShape<N>::THREADS) k_stage_visc_mx(StageArgs 
         if (blockIdx.x == 0 && tid == 0) *v.eig_next = 0.0;
     }
     __syncthreads();
-    if (tid < ncl) make_ref<N>(sQ + tid * NQ, a.gam, sRef[tid], sBar + tid * NV);
-    __syncthreads();
+    block_refs<N>(sQ, ncl, a.gam, sBar, sRef);
+    block_deviations<N>(sQ, sBar, ncl, sD);
     const float gam = float(a.gam);
     const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
     const bool pact = (tid < CPB * NP) && (pc < ncl);
+    const float *FF = (const float *)a.FF, *FFv = (const float *)a.FFv;
     float dF[8], dFv[NV], dA1 = 0.f, dA2 = 0.f;
 #pragma unroll 1
     for (int fam = 0; fam < 2; fam++) {
+        // every flux point: inviscid delta flux of the own trace (+ face jump terms)
+        for (int it = tid; it < ncl * NF; it += blockDim.x) {
+            const int c = it / NF, k = it % NF;
+            const int line = k / (N + 1), r = k % (N + 1);
+            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+            float jump[NV];
+#pragma unroll
+            for (int w = 0; w < NV; w++) jump[w] = 0.f;
+            if (r == 0 || r == N) {
+                int lfi, nfp;
+                face_of<N>(fam, r, line, lfi, nfp);
+                const float *src = FF + (((size_t)(c0 + c) * 4 + lfi) * N + nfp) * NV;
+#pragma unroll
+                for (int w = 0; w < NV; w++) jump[w] = src[w];
+            }
+            double s1d, s2d;
+            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+            const float s1 = float(s1d), s2 = float(s2d);
+            float d[NV], F[8], G[8];
+            dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d);
+            dflux(sRef[c], d, gam - 1.f, F, G);
+            float *u = sU + c * UST + line * (N + 1) + r;
+#pragma unroll
+            for (int w = 0; w < 8; w++) u[w * NF] = F[w] * s1 + G[w] * s2 + jump[w];
+            u[17 * NF] = d[8] + jump[8];
+            u[18 * NF] = s1;
+            u[19 * NF] = s2;
+        }
+        // interior flux points: viscous / AV flux of the own trace
         for (int it = tid; it < ncl * N * (N - 1); it += blockDim.x) {
             const int c = it / (N * (N - 1)), k = it % (N * (N - 1));
             const int line = k / (N - 1), r = 1 + k % (N - 1);
             const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
             double s1d, s2d;
             metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
-            const float s1 = float(s1d), s2 = float(s2d);
-            float d[NV];
-            dtrace9<N>(sQ + c * NQ, sBar + c * NV, fam, ifp, jfp, d);
-            float *u = sU + c * UST + line * (N + 1) + r;
-            {
-                float F[8], G[8];
-                dflux(sRef[c], d, gam - 1.f, F, G);
-#pragma unroll
-                for (int w = 0; w < 8; w++) u[w * NF] = F[w] * s1 + G[w] * s2;
-            }
-            u[17 * NF] = d[8];
-            u[18 * NF] = s1;
-            u[19 * NF] = s2;
-            float qv[NV], g[2 * NV], fv[NV];
+            float d[NV], qv[NV], g[2 * NV], fv[NV];
+            dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d);
 #pragma unroll
             for (int w = 0; w < NV; w++) qv[w] = sRef[c].q[w] + d[w];
             grad_trace<N, float>(sG + c * 16 * NP, fam, ifp, jfp, g);
-            visc_point<N, float>(v, sMu + c * 4, fam, ifp, jfp, s1, s2, qv, g, fv);
+            visc_point<N, float>(v, sMu + c * 4, fam, ifp, jfp, float(s1d), float(s2d), qv, g, fv);
+            float *u = sU + c * UST + line * (N + 1) + r;
 #pragma unroll
             for (int w = 0; w < NV; w++) u[(8 + w) * NF] = fv[w];
         }
+        // face points: the viscous face flux of k_face_visc_mx
         for (int it = tid; it < ncl * 2 * N; it += blockDim.x) {
             const int c = it / (2 * N), k = it % (2 * N);
             const int line = k >> 1, r = (k & 1) ? N : 0;
-            const int cell = c0 + c;
-            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
-            double s1d, s2d;
-            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
-            const float s1 = float(s1d), s2 = float(s2d);
             int lfi, nfp;
             face_of<N>(fam, r, line, lfi, nfp);
-            const int2 info = a.nbr[(size_t)cell * 4 + lfi];
-            const FacePoint fp = resolve_face<N>(info, fam, r, line);
+            const float *src = FFv + (((size_t)(c0 + c) * 4 + lfi) * N + nfp) * NV;
             float *u = sU + c * UST + line * (N + 1) + r;
-            {
-                float di[NV], dq[NV];
-                dtrace9<N>(sQ + c * NQ, sBar + c * NV, fam, ifp, jfp, di);
-                if (fp.free_face) {
 #pragma unroll
-                    for (int w = 0; w < NV; w++) dq[w] = di[w];
-                } else {
-                    dtrace9<N>(a.Qin + (size_t)fp.nb * NQ, sBar + c * NV, fp.nb_fam, fp.nb_ifp, fp.nb_jfp, dq);
-                }
-                float fn[8], daz, eig;
-                drusanov(sRef[c], di, dq, s1, s2, r == N ? 1 : -1, fp.right && !fp.free_face, gam, fn, daz, eig);
-#pragma unroll
-                for (int w = 0; w < 8; w++) u[w * NF] = fn[w];
-                u[17 * NF] = daz;
-            }
-            u[18 * NF] = s1;
-            u[19 * NF] = s2;
-            float qv[NV], g[2 * NV], fv[NV];
-            const float *src = v.qvf_r<float>() + (((size_t)cell * 4 + lfi) * N + nfp) * 8;
-#pragma unroll
-            for (int w = 0; w < 8; w++) qv[w] = sRef[c].q[w] + src[w];
-            qv[8] = 0.f;
-            grad_trace<N, float>(sG + c * 16 * NP, fam, ifp, jfp, g);
-            const float *ot = fp.free_face ? nullptr
-                                           : v.gtr_r<float>() + (((size_t)fp.nb * 4 + (info.y & 7) - 1) * N + partner_nfp<N>(nfp)) * 16;
-#pragma unroll
-            for (int d = 0; d < 2; d++)
-#pragma unroll
-                for (int w = 0; w < 8; w++) {
-                    const float mine = g[d * NV + w];
-                    const float other = ot ? ot[d * 8 + w] : mine;
-                    g[d * NV + w] = 0.5f * mine + 0.5f * other;
-                }
-            visc_point<N, float>(v, sMu + c * 4, fam, ifp, jfp, s1, s2, qv, g, fv);
-#pragma unroll
-            for (int w = 0; w < NV; w++) u[(8 + w) * NF] = fv[w];
+            for (int w = 0; w < NV; w++) u[(8 + w) * NF] = src[w];
         }
         __syncthreads();
         if (pact) {
@@ -506,7 +599,7 @@ __global__ void __launch_bounds__(VShape<N>::THREADS) k_stage_visc_mx(StageArgs 
             for (int w = 0; w < 8; w++) d[w] = 0.f;
 #pragma unroll
             for (int w = 0; w < NV; w++) dv[w] = 0.f;
-#pragma unroll
+#pragma unroll CHORUS_MX_CUNROLL
             for (int r = 0; r <= N; r++) {
                 const float m = c_Mf[N][r * N + pos];
 #pragma unroll
@@ -536,7 +629,6 @@ __global__ void __launch_bounds__(VShape<N>::THREADS) k_stage_visc_mx(StageArgs 
         }
         __syncthreads();
     }
-    // traces of the new state (the constrained-transport pass reads them)
     if (a.kind != KIND_RESID) store_traces<N>(sQ, ncl, c0, a.TQ);
 }
 

@@ -46,6 +46,26 @@ static hipError_t upload_av_tables() {
 #endif
 template <int N> using VShape = CellShape<N, CHORUS_VISC_TB>;
 
+// Two threads per solution point for the viscous kernels (k_grad,
+// k_stage_visc): in the flux
+// phase each thread takes one task of the family (inviscid interior, viscous
+// interior or face point; the inviscid and viscous segments start on wave
+// boundaries and the face points fill the gaps), in the derivative phase group
+// 0 sums the inviscid and A_z terms and group 1 the viscous ones.
+template <int N> struct V2Shape {
+    using S = VShape<N>;
+    static constexpr int CPB = S::CPB, NP = S::NP;
+    static constexpr int NIT = N * (N - 1);                  // interior points per family and cell
+    static constexpr int NI = CPB * NIT;                     // inviscid (and viscous) tasks
+    static constexpr int S1 = ((NI + 63) / 64) * 64;         // first viscous task
+    static constexpr int NFC = CPB * 2 * N;                  // face tasks
+    static constexpr int FB = NFC > (S1 - NI) ? NFC - (S1 - NI) : 0;   // face tasks after the viscous ones
+    static constexpr int BT = S1 + NI + FB;                  // flux-phase task slots
+    static constexpr int G1 = ((CPB * NP + 63) / 64) * 64;   // first thread of derivative group 1
+    static constexpr int CT = G1 + CPB * NP;
+    static constexpr int THREADS = (((BT > CT ? BT : CT) + 63) / 64) * 64;
+};
+
 // local-face index (0..3) and own face-point number of flux point (fam, r, line)
 template <int N> __device__ __forceinline__ void face_of(int fam, int r, int line, int &lfi, int &nfp) {
     if (fam == 0) { lfi = (r == 0) ? 3 : 1; nfp = (r == 0) ? (N - 1 - line) : line; }
@@ -93,19 +113,16 @@ __device__ __forceinline__ void av_sensor(double *sm, const double *sQ, double *
     }
 }
 
-template <int N>
-__device__ __forceinline__ void av_sensor_and_eigmax(const ViscArgs &a, const double *sQ, double *sAV, double *sRed,
-                                                     int ncl, int c0, double emax) {
-    av_sensor<N>(a.sm, sQ, sAV, ncl, c0);
-    __syncthreads();
-    block_max_atomic<double>(emax, sRed, a.eig);
-}
-
 // ---------------------------------------------------------------------------
+// Two threads per solution point (V2Shape): the flux phase has one task per
+// flux point, the derivative phase splits the x (group 0) and y (group 1)
+// components of nablaQs.
 template <int N, class R>
-__global__ void __launch_bounds__(VShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(CHORUS_VISC_MINW))) k_grad(ViscArgs a) {
+__global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(CHORUS_VISC_MINW))) k_grad(ViscArgs a) {
     using S = VShape<N>;
+    using T = V2Shape<N>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
+    constexpr int NIT = T::NIT;
     constexpr int VST = 10 * NF + 1;     // per-cell stride: Qv (8 planes) + metric rows (2 planes)
     __shared__ double sQ[CPB * NQ];
     __shared__ double sX[CPB * 8];
@@ -119,73 +136,80 @@ __global__ void __launch_bounds__(VShape<N>::THREADS) __attribute__((amdgpu_wave
     for (int i = tid; i < ncl * NQ; i += blockDim.x) sQ[i] = a.Qin[(size_t)c0 * NQ + i];
     for (int i = tid; i < ncl * 8; i += blockDim.x) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
     __syncthreads();
-    const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
-    const bool pact = (tid < CPB * NP) && (pc < ncl);
-    R gp[16];
+    const int grp = tid >= T::G1 ? 1 : 0;
+    const int pt = tid - grp * T::G1;
+    const int pc = pt / NP, pk = pt % NP, pjs = pk / N, pis = pk % N;
+    const bool pact = (pt < CPB * NP) && (pc < ncl);
+    R gp[8];
 #pragma unroll 1
     for (int fam = 0; fam < 2; fam++) {
-        // interior flux points: Qv = own trace
-        for (int it = tid; it < ncl * N * (N - 1); it += blockDim.x) {
-            const int c = it / (N * (N - 1)), k = it % (N * (N - 1));
-            const int line = k / (N - 1), r = 1 + k % (N - 1);
-            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
-            double s1d, s2d;
-            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
-            R Qown[NV];
-            trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown);
-            R *q = sV + c * VST + line * (N + 1) + r;
+        for (int t = tid; t < T::NI + T::NFC; t += blockDim.x) {
+            if (t < T::NI) {
+                // interior flux point: Qv = own trace
+                const int c = t / NIT, k = t % NIT;
+                if (c >= ncl) continue;
+                const int line = k / (N - 1), r = 1 + k % (N - 1);
+                const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+                double s1d, s2d;
+                metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+                R Qown[NV];
+                trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown);
+                R *q = sV + c * VST + line * (N + 1) + r;
 #pragma unroll
-            for (int v = 0; v < 8; v++) q[v * NF] = Qown[v];
-            q[8 * NF] = R(s1d);
-            q[9 * NF] = R(s2d);
-        }
-        // face points: central average of both stored traces (a FREE face
-        // averages the own trace with itself, bcflux.f90:55-61)
-        for (int it = tid; it < ncl * 2 * N; it += blockDim.x) {
-            const int c = it / (2 * N), k = it % (2 * N);
-            const int line = k >> 1, r = (k & 1) ? N : 0;
-            const int cell = c0 + c;
-            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
-            double s1d, s2d;
-            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
-            int lfi, nfp;
-            face_of<N>(fam, r, line, lfi, nfp);
-            const int2 info = a.nbr[(size_t)cell * 4 + lfi];
-            const double *own = a.TQ + (((size_t)cell * 4 + lfi) * N + nfp) * NV;
-            const double *oth = ((info.y >> 4) & 1) ? own
-                                : a.TQ + (((size_t)info.x * 4 + (info.y & 7) - 1) * N + partner_nfp<N>(nfp)) * NV;
-            R *q = sV + c * VST + line * (N + 1) + r;
+                for (int v = 0; v < 8; v++) q[v * NF] = Qown[v];
+                q[8 * NF] = R(s1d);
+                q[9 * NF] = R(s2d);
+            } else {
+                // face point: central average of both stored traces (a FREE
+                // face averages the own trace with itself, bcflux.f90:55-61)
+                const int item = t - T::NI;
+                const int c = item / (2 * N), k = item % (2 * N);
+                if (c >= ncl) continue;
+                const int line = k >> 1, r = (k & 1) ? N : 0;
+                const int cell = c0 + c;
+                const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+                double s1d, s2d;
+                metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+                int lfi, nfp;
+                face_of<N>(fam, r, line, lfi, nfp);
+                const int2 info = a.nbr[(size_t)cell * 4 + lfi];
+                const double *own = a.TQ + (((size_t)cell * 4 + lfi) * N + nfp) * NV;
+                const double *oth = ((info.y >> 4) & 1) ? own
+                                    : a.TQ + (((size_t)info.x * 4 + (info.y & 7) - 1) * N + partner_nfp<N>(nfp)) * NV;
+                R *q = sV + c * VST + line * (N + 1) + r;
 #pragma unroll
-            for (int v = 0; v < 8; v++) q[v * NF] = R(0.5) * R(own[v]) + R(0.5) * R(oth[v]);
-            q[8 * NF] = R(s1d);
-            q[9 * NF] = R(s2d);
+                for (int v = 0; v < 8; v++) q[v * NF] = R(0.5) * R(own[v]) + R(0.5) * R(oth[v]);
+                q[8 * NF] = R(s1d);
+                q[9 * NF] = R(s2d);
+            }
         }
         __syncthreads();
-        // nablaQs (nablaQsp, compflux.f90:107-157)
+        // nablaQs (nablaQsp, compflux.f90:107-157); group 0: x, group 1: y
         if (pact) {
             const int line = fam == 0 ? pjs : pis, pos = fam == 0 ? pis : pjs;
             const R *q = sV + pc * VST + line * (N + 1);
-            R J = R(0);
-            if (fam == 1) J = R(jacobian<N>(sX + pc * 8, pis, pjs));
-            R acc[16];
+            const R *mq = q + (8 + grp) * NF;
+            R acc[8];
 #pragma unroll
-            for (int g = 0; g < 16; g++) acc[g] = R(0);
+            for (int v = 0; v < 8; v++) acc[v] = R(0);
             constexpr int U = contraction_unroll<R, N>();
 #pragma unroll U
             for (int r = 0; r <= N; r++) {
                 const R m = Mop<R>(N, r, pos);
-                const R m1 = q[8 * NF + r], m2 = q[9 * NF + r];
+                const R mm = mq[r];
 #pragma unroll
                 for (int v = 0; v < 8; v++) {
                     const R t = q[v * NF + r] * m;
-                    acc[v] = acc[v] + t * m1;
-                    acc[8 + v] = acc[8 + v] + t * m2;
+                    acc[v] = acc[v] + t * mm;
                 }
             }
+            if (fam == 0) {
 #pragma unroll
-            for (int g = 0; g < 16; g++) {
-                if (fam == 0) gp[g] = acc[g];
-                else sG[(pc * 16 + g) * NP + pk] = (gp[g] + acc[g]) / J;
+                for (int v = 0; v < 8; v++) gp[v] = acc[v];
+            } else {
+                const R J = R(jacobian<N>(sX + pc * 8, pis, pjs));
+#pragma unroll
+                for (int v = 0; v < 8; v++) sG[(pc * 16 + grp * 8 + v) * NP + pk] = (gp[v] + acc[v]) / J;
             }
         }
         __syncthreads();
@@ -234,7 +258,7 @@ __device__ __forceinline__ void grad_trace(const R *gq, int fam, int ifp, int jf
         g[d * NV + 8] = R(0);
     }
     // contraction index outermost: 16 loads in flight per step
-    constexpr int U = contraction_unroll<R, N>();
+    constexpr int U = sizeof(R) == 8 ? contraction_unroll<R, N>() : CHORUS_MX_GRAD_UNROLL;
 #pragma unroll U
     for (int sp = 0; sp < N; sp++) {
         const R l = fam == 0 ? Lop<R>(N, ifp, sp) : Lop<R>(N, jfp, sp);
@@ -351,15 +375,18 @@ __global__ void __launch_bounds__(256) k_face_visc(FaceArgs a) {
 
 // ---------------------------------------------------------------------------
 template <int N, class R>
-__global__ void __launch_bounds__(VShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(CHORUS_VISC_MINW))) k_stage_visc(StageArgs a, ViscArgs v) {
+__global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(CHORUS_VISC_MINW))) k_stage_visc(StageArgs a, ViscArgs v) {
     using S = VShape<N>;
+    using T = V2Shape<N>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
+    constexpr int NIT = T::NIT;
     constexpr int UST = 20 * NF + 1;     // per-cell stride: F (8) + Fv (9) + A_z + metric rows (2)
     __shared__ double sQ[CPB * NQ];
     __shared__ double sX[CPB * 8];
     __shared__ R sG[CPB * 16 * NP];      // nablaQs from k_grad
     __shared__ R sU[CPB * UST];          // [c][plane][line][r], one family at a time
     __shared__ double sMu[CPB * 4];      // muAVfi/fj(1,side) per local face
+    static_assert(9 * CPB * NP <= CPB * UST, "exchange area");
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = blk * CPB;
@@ -382,117 +409,142 @@ __global__ void __launch_bounds__(VShape<N>::THREADS) __attribute__((amdgpu_wave
         if (blockIdx.x == 0 && tid == 0) *v.eig_next = 0.0;
     }
     __syncthreads();
-    const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
-    const bool pact = (tid < CPB * NP) && (pc < ncl);
+    // derivative-phase role of this thread
+    const int grp = tid >= T::G1 ? 1 : 0;
+    const int pt = tid - grp * T::G1;
+    const int pc = pt / NP, pk = pt % NP, pjs = pk / N, pis = pk % N;
+    const bool pact = (pt < CPB * NP) && (pc < ncl);
     const R *FF = (const R *)a.FF, *FFv = (const R *)a.FFv;
 
-    R dF[8], dFv[NV], dA1 = R(0), dA2 = R(0);
+    R acc8[8], accv[NV], dA1 = R(0), dA2 = R(0);   // group 0: acc8 = dF; group 1: accv = dFv
 #pragma unroll 1
     for (int fam = 0; fam < 2; fam++) {
-        // interior flux points: inviscid flux
-        for (int it = tid; it < ncl * N * (N - 1); it += blockDim.x) {
-            const int c = it / (N * (N - 1)), k = it % (N * (N - 1));
-            const int line = k / (N - 1), r = 1 + k % (N - 1);
-            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
-            double s1d, s2d;
-            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
-            const R s1 = R(s1d), s2 = R(s2d);
-            R Qown[NV];
-            trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown);
-            R *u = sU + c * UST + line * (N + 1) + r;
-            {
-                R F[8], G[8];
-                iflux<R>(R(a.gam), Qown, F, G);
+        for (int t = tid; t < T::BT; t += blockDim.x) {
+            int kind, item;
+            if (t < T::NI) { kind = 0; item = t; }
+            else if (t < T::S1) { kind = 2; item = t - T::NI; }
+            else if (t < T::S1 + T::NI) { kind = 1; item = t - T::S1; }
+            else { kind = 2; item = (T::S1 - T::NI) + (t - T::S1 - T::NI); }
+            if (kind != 2) {
+                // interior flux point
+                const int c = item / NIT, k = item % NIT;
+                if (c >= ncl) continue;
+                const int line = k / (N - 1), r = 1 + k % (N - 1);
+                const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+                double s1d, s2d;
+                metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+                R Qown[NV];
+                trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown);
+                R *u = sU + c * UST + line * (N + 1) + r;
+                if (kind == 0) {
+                    // inviscid flux
+                    const R s1 = R(s1d), s2 = R(s2d);
+                    R F[8], G[8];
+                    iflux<R>(R(a.gam), Qown, F, G);
 #pragma unroll
-                for (int w = 0; w < 8; w++) u[w * NF] = F[w] * s1 + G[w] * s2;
+                    for (int w = 0; w < 8; w++) u[w * NF] = F[w] * s1 + G[w] * s2;
+                    u[17 * NF] = Qown[8];
+                    u[18 * NF] = s1;
+                    u[19 * NF] = s2;
+                } else {
+                    // viscous / AV flux
+                    R g[2 * NV], fv[NV];
+                    grad_trace<N, R>(sG + c * 16 * NP, fam, ifp, jfp, g);
+                    visc_point<N, R>(v, sMu + c * 4, fam, ifp, jfp, R(s1d), R(s2d), Qown, g, fv);
+#pragma unroll
+                    for (int w = 0; w < NV; w++) u[(8 + w) * NF] = fv[w];
+                }
+            } else {
+                // face point: the stored common inviscid, A_z and viscous fluxes
+                const int c = item / (2 * N), k = item % (2 * N);
+                if (item >= T::NFC || c >= ncl) continue;
+                const int line = k >> 1, r = (k & 1) ? N : 0;
+                const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+                int lfi, nfp;
+                face_of<N>(fam, r, line, lfi, nfp);
+                const size_t o = (((size_t)(c0 + c) * 4 + lfi) * N + nfp) * NV;
+                double s1d, s2d;
+                metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+                R *u = sU + c * UST + line * (N + 1) + r;
+#pragma unroll
+                for (int w = 0; w < 8; w++) u[w * NF] = FF[o + w];
+                u[17 * NF] = FF[o + 8];
+#pragma unroll
+                for (int w = 0; w < NV; w++) u[(8 + w) * NF] = FFv[o + w];
+                u[18 * NF] = R(s1d);
+                u[19 * NF] = R(s2d);
             }
-            u[17 * NF] = Qown[8];
-            u[18 * NF] = s1;
-            u[19 * NF] = s2;
-        }
-        // interior flux points: viscous / AV flux (a separate pass keeps the
-        // register footprint of each pass small)
-        for (int it = tid; it < ncl * N * (N - 1); it += blockDim.x) {
-            const int c = it / (N * (N - 1)), k = it % (N * (N - 1));
-            const int line = k / (N - 1), r = 1 + k % (N - 1);
-            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
-            double s1d, s2d;
-            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
-            R Qown[NV], g[2 * NV], fv[NV];
-            trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown);
-            grad_trace<N, R>(sG + c * 16 * NP, fam, ifp, jfp, g);
-            visc_point<N, R>(v, sMu + c * 4, fam, ifp, jfp, R(s1d), R(s2d), Qown, g, fv);
-            R *u = sU + c * UST + line * (N + 1) + r;
-#pragma unroll
-            for (int w = 0; w < NV; w++) u[(8 + w) * NF] = fv[w];
-        }
-        // face points: the stored common inviscid, A_z and viscous fluxes
-        for (int it = tid; it < ncl * 2 * N; it += blockDim.x) {
-            const int c = it / (2 * N), k = it % (2 * N);
-            const int line = k >> 1, r = (k & 1) ? N : 0;
-            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
-            double s1d, s2d;
-            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
-            int lfi, nfp;
-            face_of<N>(fam, r, line, lfi, nfp);
-            const size_t o = (((size_t)(c0 + c) * 4 + lfi) * N + nfp) * NV;
-            R *u = sU + c * UST + line * (N + 1) + r;
-#pragma unroll
-            for (int w = 0; w < 8; w++) u[w * NF] = FF[o + w];
-            u[17 * NF] = FF[o + 8];
-#pragma unroll
-            for (int w = 0; w < NV; w++) u[(8 + w) * NF] = FFv[o + w];
-            u[18 * NF] = R(s1d);
-            u[19 * NF] = R(s2d);
         }
         __syncthreads();
         if (pact) {
             const int line = fam == 0 ? pjs : pis, pos = fam == 0 ? pis : pjs;
             const R *f = sU + pc * UST + line * (N + 1);
-            R d[8], dv[NV], e1 = R(0), e2 = R(0);
-#pragma unroll
-            for (int w = 0; w < 8; w++) d[w] = R(0);
-#pragma unroll
-            for (int w = 0; w < NV; w++) dv[w] = R(0);
             constexpr int U = contraction_unroll<R, N>();
+            if (grp == 0) {
+                R d[8], e1 = R(0), e2 = R(0);
+#pragma unroll
+                for (int w = 0; w < 8; w++) d[w] = R(0);
 #pragma unroll U
-            for (int r = 0; r <= N; r++) {
-                const R m = Mop<R>(N, r, pos);
+                for (int r = 0; r <= N; r++) {
+                    const R m = Mop<R>(N, r, pos);
 #pragma unroll
-                for (int w = 0; w < 8; w++) d[w] = d[w] + f[w * NF + r] * m;
-#pragma unroll
-                for (int w = 0; w < NV; w++) dv[w] = dv[w] + f[(8 + w) * NF + r] * m;
-            }
+                    for (int w = 0; w < 8; w++) d[w] = d[w] + f[w * NF + r] * m;
+                }
 #pragma unroll U
-            for (int r = 0; r <= N; r++) e1 = e1 + f[17 * NF + r] * Mop<R>(N, r, pos) * f[18 * NF + r];
+                for (int r = 0; r <= N; r++) e1 = e1 + f[17 * NF + r] * Mop<R>(N, r, pos) * f[18 * NF + r];
 #pragma unroll U
-            for (int r = 0; r <= N; r++) e2 = e2 + f[17 * NF + r] * Mop<R>(N, r, pos) * f[19 * NF + r];
-            if (fam == 0) {
+                for (int r = 0; r <= N; r++) e2 = e2 + f[17 * NF + r] * Mop<R>(N, r, pos) * f[19 * NF + r];
+                if (fam == 0) {
 #pragma unroll
-                for (int w = 0; w < 8; w++) dF[w] = d[w];
+                    for (int w = 0; w < 8; w++) acc8[w] = d[w];
+                    dA1 = e1; dA2 = e2;
+                } else {
 #pragma unroll
-                for (int w = 0; w < NV; w++) dFv[w] = dv[w];
-                dA1 = e1; dA2 = e2;
+                    for (int w = 0; w < 8; w++) acc8[w] = -(acc8[w] + d[w]);
+                    dA1 = dA1 + e1; dA2 = dA2 + e2;
+                }
             } else {
-                const R J = R(jacobian<N>(sX + pc * 8, pis, pjs));
-                double *qs = sQ + pc * NQ + pk;
-                const R q0 = R(qs[0]), q1 = R(qs[NP]), q2 = R(qs[2 * NP]);
-                const R au = q1 / q0, av = q2 / q0;
-                const R nA1 = (dA1 + e1) / J, nA2 = (dA2 + e2) / J;
-                R res[NV];
+                R dv[NV];
 #pragma unroll
-                for (int w = 0; w < 8; w++) res[w] = -(dF[w] + d[w]);
-                res[8] = -(au * nA1) - av * nA2;
-                res[8] = res[8] * J;
+                for (int w = 0; w < NV; w++) dv[w] = R(0);
+#pragma unroll U
+                for (int r = 0; r <= N; r++) {
+                    const R m = Mop<R>(N, r, pos);
 #pragma unroll
-                for (int w = 0; w < NV; w++) res[w] = res[w] + (dFv[w] + dv[w]);
+                    for (int w = 0; w < NV; w++) dv[w] = dv[w] + f[(8 + w) * NF + r] * m;
+                }
 #pragma unroll
-                for (int w = 0; w < NV; w++) res[w] = res[w] / J;
-                stage_combine(a, (size_t)(c0 + pc) * NQ + pk, NP, qs, res);
+                for (int w = 0; w < NV; w++) accv[w] = fam == 0 ? dv[w] : accv[w] + dv[w];
             }
         }
         __syncthreads();
     }
+    // group 1 hands its viscous sums to group 0 through the (now free) flux area
+    R *xv = sU;
+    if (pact && grp == 1) {
+#pragma unroll
+        for (int w = 0; w < NV; w++) xv[w * CPB * NP + pt] = accv[w];
+    }
+    __syncthreads();
+    if (pact && grp == 0) {
+        // residual at the solution point (compresid.f90:12-65)
+        const R J = R(jacobian<N>(sX + pc * 8, pis, pjs));
+        double *qs = sQ + pc * NQ + pk;
+        const R q0 = R(qs[0]), q1 = R(qs[NP]), q2 = R(qs[2 * NP]);
+        const R au = q1 / q0, av = q2 / q0;
+        const R nA1 = dA1 / J, nA2 = dA2 / J;
+        R res[NV];
+#pragma unroll
+        for (int w = 0; w < 8; w++) res[w] = acc8[w];
+        res[8] = -(au * nA1) - av * nA2;
+        res[8] = res[8] * J;
+#pragma unroll
+        for (int w = 0; w < NV; w++) res[w] = res[w] + xv[w * CPB * NP + pt];
+#pragma unroll
+        for (int w = 0; w < NV; w++) res[w] = res[w] / J;
+        stage_combine(a, (size_t)(c0 + pc) * NQ + pk, NP, qs, res);
+    }
+    __syncthreads();
     // traces of the new state for the next stage's face passes
     if (a.kind != KIND_RESID) store_traces<N>(sQ, ncl, c0, a.TQ);
 }

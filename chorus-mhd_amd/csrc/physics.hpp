@@ -15,6 +15,9 @@ __device__ __forceinline__ double dsqrt_(double x) { return __builtin_sqrt(x); }
 __device__ __forceinline__ float dsqrt_(float x) { return __builtin_sqrtf(x); }
 __device__ __forceinline__ double dabs_(double x) { return __builtin_fabs(x); }
 __device__ __forceinline__ float dabs_(float x) { return __builtin_fabsf(x); }
+// quotient: IEEE division in FP64; reciprocal times numerator in FP32 (mixed form)
+__device__ __forceinline__ double div_(double a, double b) { return a / b; }
+__device__ __forceinline__ float div_(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
 // getIfluxvectors (getrusanov.f90:111-153): ideal-MHD fluxes F, G of Qi(1:8)
 template <class R>
@@ -110,25 +113,25 @@ __device__ __forceinline__ void vflux(R gam, R lambda, R prandt, R m, R e, const
                                       R *Fv, R *Gv) {
     R rho = Qv[0], rhou = Qv[1], rhov = Qv[2], rhow = Qv[3], rhoE = Qv[4];
     R Bx = Qv[5], By = Qv[6], Bz = Qv[7];
-    R u = rhou / rho, v = rhov / rho, w = rhow / rho;
+    R u = div_(rhou, rho), v = div_(rhov, rho), w = div_(rhow, rho);
     R NORM_U = sq(u) + sq(v) + sq(w);
     R NORM_B = sq(Bx) + sq(By) + sq(Bz);
-    R inte = (rhoE - R(0.5) * NORM_U * rho - R(0.5) * NORM_B) / rho;
+    R inte = div_(rhoE - R(0.5) * NORM_U * rho - R(0.5) * NORM_B, rho);
     R drho[2], du[2], dv[2], dw[2], dBx[2], dBy[2], dBz[2], de[2];
 #pragma unroll
     for (int d = 0; d < 2; d++) {
         const R *g = nQ + d * NV;
         drho[d] = g[0];
-        du[d] = (g[1] - drho[d] * u) / rho;
-        dv[d] = (g[2] - drho[d] * v) / rho;
-        dw[d] = (g[3] - drho[d] * w) / rho;
+        du[d] = div_(g[1] - drho[d] * u, rho);
+        dv[d] = div_(g[2] - drho[d] * v, rho);
+        dw[d] = div_(g[3] - drho[d] * w, rho);
         dBx[d] = g[5]; dBy[d] = g[6]; dBz[d] = g[7];
     }
 #pragma unroll
     for (int d = 0; d < 2; d++) {
         R dKE = R(0.5) * NORM_U * drho[d] + rho * (u * du[d] + v * dv[d] + w * dw[d]);
         R dpre_B = Bx * dBx[d] + By * dBy[d] + Bz * dBz[d];
-        de[d] = (nQ[d * NV + 4] - dKE - dpre_B - drho[d] * inte) / rho;
+        de[d] = div_(nQ[d * NV + 4] - dKE - dpre_B - drho[d] * inte, rho);
     }
     R ux = du[0], uy = du[1], vx = dv[0], vy = dv[1], wx = dw[0], wy = dw[1];
     R ex = de[0], ey = de[1];
@@ -137,12 +140,12 @@ __device__ __forceinline__ void vflux(R gam, R lambda, R prandt, R m, R e, const
     Fv[1] = m * (R(2) * ux + lambda * (ux + vy));
     Fv[2] = m * (vx + uy);
     Fv[3] = m * wx;
-    Fv[4] = u * Fv[1] + v * Fv[2] + w * Fv[3] + m * gam * ex / prandt;
+    Fv[4] = u * Fv[1] + v * Fv[2] + w * Fv[3] + div_(m * gam * ex, prandt);
     Gv[0] = R(0);
     Gv[1] = m * (vx + uy);
     Gv[2] = m * (R(2) * vy + lambda * (ux + vy));
     Gv[3] = m * wy;
-    Gv[4] = Gv[1] * u + Gv[2] * v + Gv[3] * w + m * gam * ey / prandt;
+    Gv[4] = Gv[1] * u + Gv[2] * v + Gv[3] * w + div_(m * gam * ey, prandt);
     R Jx = Bzy - R(0), Jy = R(0) - Bzx, Jz = Byx - Bxy;
     Fv[5] = R(0);
     Fv[6] = e * Jz;
