@@ -34,6 +34,7 @@ struct chorus_ctx {
     void *gtr = nullptr;           // viscous path: own nablaQs face traces
     void *gsol = nullptr;          // viscous path: nablaQs at the solution points
     double *sm = nullptr;          // viscous path: AV indicator per cell
+    double *mu4 = nullptr;         // viscous path: face-averaged muAV per cell and local face
     double *eig = nullptr;         // two slots of the per-stage maximum wave speed
     int stage_parity = 0;          // slot used by the next stage
     int last_parity = -1;          // slot written by the last stage
@@ -89,6 +90,7 @@ FaceArgs face_args(chorus_ctx *c) {
     f.FFv = c->FFv;
     f.gtr = c->gtr;
     f.sm = c->sm;
+    f.mu4 = c->mu4;
     f.artiv = c->p.artiv;
     f.gam = c->p.gam;
     f.mu = c->p.mu;
@@ -118,6 +120,7 @@ int launch_stage(chorus_ctx *c, const StageArgs &a) {
         v.eig = c->eig + c->stage_parity;
         v.eig_next = c->eig + (1 - c->stage_parity);
         v.TQ = c->TQ;
+        v.mu4 = c->mu4;
         v.artiv = c->p.artiv;
         v.mu = c->p.mu;
         v.eta0 = c->p.eta0;
@@ -145,11 +148,12 @@ StageArgs base_args(chorus_ctx *c) {
     a.FF = c->FF;
     a.FFv = c->FFv;
     a.TQ = c->TQ;
+    a.sm = c->sm;
     return a;
 }
 
 int run_trace(chorus_ctx *c, const double *Q) {
-    hipError_t e = launch_trace(c->N, c->ncell, Q, c->TQ, c->stream);
+    hipError_t e = launch_trace(c->N, c->ncell, Q, c->TQ, c->p.artiv ? c->sm : nullptr, c->stream);
     if (e != hipSuccess) return fail(c, CHORUS_ERR_HIP, std::string("trace launch: ") + hipGetErrorString(e));
     return 0;
 }
@@ -511,6 +515,7 @@ int chorus_init(const chorus_mesh *m, const chorus_params *p, chorus_ctx **out) 
         HIP_TRY(c, hipMalloc(&c->gtr, nf * 16 * sizeof(double)));
         HIP_TRY(c, hipMalloc(&c->gsol, (size_t)c->ncell * 16 * c->N * c->N * sizeof(double)));
         HIP_TRY(c, hipMalloc(&c->sm, (size_t)c->ncell * sizeof(double)));
+        HIP_TRY(c, hipMalloc(&c->mu4, (size_t)c->ncell * 4 * sizeof(double)));
     }
     HIP_TRY(c, hipMemcpyAsync(c->cellxy, xy.data(), xy.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->nbr, nbr.data(), nbr.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
@@ -534,6 +539,7 @@ void chorus_free(chorus_ctx *c) {
     (void)hipFree(c->gtr);
     (void)hipFree(c->gsol);
     (void)hipFree(c->sm);
+    (void)hipFree(c->mu4);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

@@ -35,9 +35,9 @@ hipError_t launch_stage_ideal_fp64(int N, const StageArgs &a, const FaceArgs &f,
 #undef L_
 }
 
-hipError_t launch_trace(int N, int ncell, const double *Q, double *TQ, hipStream_t s) {
+hipError_t launch_trace(int N, int ncell, const double *Q, double *TQ, double *sm, hipStream_t s) {
 #define L_(n) \
-    hipLaunchKernelGGL((k_trace<n>), dim3(nblocks<n>(ncell)), dim3(Shape<n>::THREADS), 0, s, ncell, Q, TQ); \
+    hipLaunchKernelGGL((k_trace<n>), dim3(nblocks<n>(ncell)), dim3(Shape<n>::THREADS), 0, s, ncell, Q, TQ, sm); \
     return hipGetLastError();
     CHORUS_DISPATCH_N(N, L_)
 #undef L_
@@ -70,18 +70,29 @@ int ct_blocks(int N, int ncell) { return stage_blocks(N, ncell); }
 
 template <int N> static int vblocks(int ncell) { return (ncell + VShape<N>::CPB - 1) / VShape<N>::CPB; }
 
-hipError_t launch_stage_visc_fp64(int N, const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s) {
-#define V_(n)                                                                                         \
-    case n:                                                                                           \
-        hipLaunchKernelGGL((k_face_inv<n, double>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);    \
-        hipLaunchKernelGGL((k_grad<n, double>), dim3(vblocks<n>(a.ncell)), dim3(V2Shape<n>::THREADS), 0, s, v); \
-        hipLaunchKernelGGL((k_face_visc<n, double>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);   \
-        hipLaunchKernelGGL((k_stage_visc<n, double>), dim3(vblocks<n>(a.ncell)), dim3(V2Shape<n>::THREADS), 0, s, a, v); \
-        return hipGetLastError();
-    switch (N) {
-        V_(2) V_(3) V_(4) V_(5) V_(6) V_(7) V_(8)
+template <int n, int VM>
+static hipError_t visc_stage(const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s) {
+    hipLaunchKernelGGL((k_face_inv<n, double>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);
+    hipLaunchKernelGGL((k_grad<n, double>), dim3(vblocks<n>(a.ncell)), dim3(V2Shape<n>::THREADS), 0, s, v);
+    hipLaunchKernelGGL((k_face_visc<n, double, VM>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);
+    hipLaunchKernelGGL((k_stage_visc<n, double, VM>), dim3(vblocks<n>(a.ncell)), dim3(V2Shape<n>::THREADS), 0, s, a, v);
+    return hipGetLastError();
+}
+
+template <int n>
+static hipError_t visc_stage_vm(const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s) {
+    const int vm = ((v.mu != 0.0 || v.eta0 != 0.0) ? 1 : 0) | (v.artiv ? 2 : 0);
+    switch (vm) {
+    case 1: return visc_stage<n, 1>(a, v, f, s);
+    case 2: return visc_stage<n, 2>(a, v, f, s);
+    case 3: return visc_stage<n, 3>(a, v, f, s);
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_stage_visc_fp64(int N, const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s) {
+#define V_(n) return visc_stage_vm<n>(a, v, f, s);
+    CHORUS_DISPATCH_N(N, V_)
 #undef V_
 }
 

@@ -58,6 +58,7 @@ struct StageArgs {
     const void *FF;     // common face fluxes of each cell's face points (k_face_inv)
     const void *FFv;    // viscous face fluxes (k_face_visc)
     double *TQ;         // traces of Qout at the face points, written by the stage kernel
+    double *sm;         // AV: indicator of Qout per cell, written by the viscous stage kernel
 };
 
 // Face-ordered pass: one thread per face point.
@@ -80,6 +81,7 @@ struct FaceArgs {
     void *FFv;
     const void *gtr;
     const double *sm;        // AV indicator per cell
+    double *mu4;             // k_face_visc: face-averaged muAV(1) per cell and local face
     double *eig;             // k_face_inv: atomic max of the Rusanov wave speed (AV)
     const double *eig_now;   // k_face_visc: this stage's maximum (epsilon0)
     int artiv;
@@ -98,6 +100,7 @@ struct ViscArgs {
     double *eig;        // this stage's maximum Rusanov wave speed (atomic max)
     double *eig_next;   // the next stage's slot, zeroed by k_stage_visc
     const double *TQ;   // traces of Qin at the face points
+    const double *mu4;  // face-averaged muAV(1) per cell and local face (k_face_visc)
     int artiv;
     double mu, eta0, gam, h_msh, pi;
     template <class R> __host__ __device__ R *gtr_r() const { return (R *)gtr; }
@@ -116,10 +119,11 @@ struct CTArgs {
 };
 
 // SSP stage combination at one solution point; the new state also replaces
-// the point's entries of qs (the block's LDS copy, read by no other thread)
+// the point's entries of qs (the block's LDS copy, read by no other thread).
+// qa: a register copy of the point's Qa (prefetched) or null.
 template <class R>
 __device__ __forceinline__ void stage_combine(const StageArgs &a, size_t base, int NP, double *qs,
-                                              const R *res) {
+                                              const R *res, const double *qa = nullptr) {
 #pragma unroll
     for (int v = 0; v < 9; v++) {
         const double q = qs[v * NP];
@@ -129,19 +133,26 @@ __device__ __forceinline__ void stage_combine(const StageArgs &a, size_t base, i
         switch (a.kind) {
         case KIND_RESID: a.Rout[o] = r; continue;
         case KIND_AXPY: qn = q + a.c0 * r; break;
-        case KIND_3A: qn = a.c1 * a.Qa[o] + a.c2 * q + a.c0 * r; break;
-        case KIND_3B: qn = a.c1 * a.Qa[o] + a.c2 * q + a.c3 * r * a.c4; break;
-        case KIND_S54_4: qn = a.c1 * a.Qa[o] + a.c2 * q + a.c0 * r; a.Rout[o] = r; break;
-        default: qn = a.c1 * a.Qa[o] + a.c2 * a.Qb[o] + a.c3 * a.Qc[o] + a.c4 * q + a.c5 * a.Rprev[o] + a.c0 * r; break;
+        case KIND_3A: qn = a.c1 * (qa ? qa[v] : a.Qa[o]) + a.c2 * q + a.c0 * r; break;
+        case KIND_3B: qn = a.c1 * (qa ? qa[v] : a.Qa[o]) + a.c2 * q + a.c3 * r * a.c4; break;
+        case KIND_S54_4: qn = a.c1 * (qa ? qa[v] : a.Qa[o]) + a.c2 * q + a.c0 * r; a.Rout[o] = r; break;
+        default: qn = a.c1 * (qa ? qa[v] : a.Qa[o]) + a.c2 * a.Qb[o] + a.c3 * a.Qc[o] + a.c4 * q + a.c5 * a.Rprev[o] + a.c0 * r; break;
         }
         a.Qout[o] = qn;
         qs[v * NP] = qn;
     }
 }
 
+// register prefetch of a point's Qa for the stage kinds that combine with it
+__device__ __forceinline__ void prefetch_qa(const StageArgs &a, size_t base, int NP, double *qa) {
+    const bool need = a.kind != KIND_RESID && a.kind != KIND_AXPY;
+#pragma unroll
+    for (int v = 0; v < 9; v++) qa[v] = need ? a.Qa[base + (size_t)v * NP] : 0.0;
+}
+
 // launchers (defined in the kernel translation units); return hipError_t
 hipError_t launch_stage_ideal_fp64(int N, const StageArgs &a, const FaceArgs &f, hipStream_t s);
-hipError_t launch_trace(int N, int ncell, const double *Q, double *TQ, hipStream_t s);
+hipError_t launch_trace(int N, int ncell, const double *Q, double *TQ, double *sm, hipStream_t s);
 hipError_t launch_stage_ideal_mixed(int N, const StageArgs &a, const FaceArgs &f, hipStream_t s);
 hipError_t launch_ct(int N, const CTArgs &a, hipStream_t s);
 hipError_t launch_stage_visc_fp64(int N, const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s);

@@ -172,21 +172,6 @@ __device__ __forceinline__ void store_traces(const double *q, int ncl, int c0, d
     }
 }
 
-// traces of a state (after set-up, or for a state not produced by a stage)
-template <int N>
-__global__ void __launch_bounds__(256) k_trace(int ncell, const double *Q, double *TQ) {
-    using S = Shape<N>;
-    constexpr int CPB = S::CPB, NQ = S::NQ;
-    __shared__ double sQ[CPB * NQ];
-    const int blk = xcd_block(blockIdx.x, gridDim.x);
-    const int c0 = blk * CPB;
-    const int ncl = min(CPB, ncell - c0);
-    for (int i = threadIdx.x; i < ncl * NQ; i += blockDim.x) sQ[i] = Q[(size_t)c0 * NQ + i];
-    __syncthreads();
-    store_traces<N>(sQ, ncl, c0, TQ);
-}
-
-// block maximum (any block size) folded into *dst with one atomic
 template <class R>
 __device__ __forceinline__ void block_max_atomic(double v, double *red, double *dst) {
     const int tid = threadIdx.x, n = blockDim.x;
@@ -289,6 +274,7 @@ __global__ void __launch_bounds__(IShape<N>::THREADS) __attribute__((amdgpu_wave
     const bool pact = (tid < CPB * NP) && (pc < ncl);
     const R *FF = (const R *)a.FF;
     R dF[8], dA1 = R(0), dA2 = R(0);
+    double qa[NV];
 
 #pragma unroll 1
     for (int fam = 0; fam < 2; fam++) {
@@ -331,6 +317,8 @@ __global__ void __launch_bounds__(IShape<N>::THREADS) __attribute__((amdgpu_wave
             aa[2 * NF] = R(s2d);
         }
         __syncthreads();
+        // the stage's Qa at the point, loaded while the derivative is summed
+        if (fam == 1 && pact) prefetch_qa(a, (size_t)(c0 + pc) * NQ + pk, NP, qa);
         // C: derivative along this family at the thread's solution point
         if (pact) {
             const int line = fam == 0 ? pjs : pis;    // the line through the point
@@ -372,7 +360,7 @@ __global__ void __launch_bounds__(IShape<N>::THREADS) __attribute__((amdgpu_wave
 #pragma unroll
                 for (int v = 0; v < NV; v++) res[v] = res[v] / J;
                 const size_t base = (size_t)(c0 + pc) * NQ + pk;
-                stage_combine(a, base, NP, qs, res);
+                stage_combine(a, base, NP, qs, res, qa);
             }
         }
         __syncthreads();

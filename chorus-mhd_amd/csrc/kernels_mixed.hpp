@@ -130,16 +130,75 @@ __device__ __forceinline__ float wave_speed(const float *Ql, const float *Qr, fl
     return 0.5f * fabsf((ul + ur) * nx + (vl + vr) * ny) + c_f;
 }
 
-// deviations of the block's cells from their means, rounded once to FP32:
-// dq[c][var][p] = float(Q - Qbar); called by all threads of the block
-template <int N>
-__device__ __forceinline__ void block_deviations(const double *sQ, const double *sBar, int ncl, float *dq) {
-    constexpr int NP = N * N, NQ = NV * NP;
-    for (int i = threadIdx.x; i < ncl * NQ; i += blockDim.x) {
-        const int c = i / NQ, v = (i % NQ) / NP;
-        dq[i] = float(sQ[i] - sBar[c * NV + v]);
+// Phase A of the mixed cell kernels: the block's cells (contiguous in HBM)
+// become FP32 deviations from their FP64 means.  Each thread keeps the values
+// it loaded in registers; tmp (at least ncl*NQ doubles of LDS) holds the FP64
+// copy only until the means are taken, so dq may alias it.  rho, if given,
+// receives the FP64 density plane of each cell (for the AV indicator).
+// Called by all THREADS threads of the block.
+template <int N, int CPB, int THREADS>
+__device__ __forceinline__ void stage_deviations(const double *Qg, int ncl, double gam, double *tmp, double *sBar,
+                                                 Ref *sRef, float *dq, double *rho) {
+    constexpr int NP = N * N, NQ = NV * NP, PER = (CPB * NQ + THREADS - 1) / THREADS;
+    const int tid = threadIdx.x, n = ncl * NQ;
+    double held[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = tid + k * THREADS;
+        held[k] = i < n ? Qg[i] : 0.0;
+        if (i < n) tmp[i] = held[k];
     }
     __syncthreads();
+    for (int i = tid; i < ncl * NV; i += THREADS) {
+        const double *q = tmp + (i / NV) * NQ + (i % NV) * NP;
+        double s = 0.0;
+        for (int p = 0; p < NP; p++) s += q[p];
+        sBar[i] = s / NP;
+    }
+    if (rho)
+        for (int i = tid; i < ncl * NP; i += THREADS) rho[i] = tmp[(i / NP) * NQ + i % NP];
+    __syncthreads();
+    for (int c = tid; c < ncl; c += THREADS) ref_from(sBar + c * NV, gam, sRef[c]);
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = tid + k * THREADS;
+        if (i < n) dq[i] = float(held[k] - sBar[(i / NQ) * NV + (i % NQ) / NP]);
+    }
+    __syncthreads();
+}
+
+// SSP stage combination at one solution point from register copies of the
+// point's state q and (when the stage needs it) Qa; the new state is also
+// written to qout[var*NP] (LDS) for the traces
+__device__ __forceinline__ void combine_point(const StageArgs &a, size_t base, int NP, const double *q,
+                                              const double *qa, const float *res, double *qout) {
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        const double r = double(res[v]);
+        const size_t o = base + (size_t)v * NP;
+        double qn;
+        switch (a.kind) {
+        case KIND_RESID: a.Rout[o] = r; continue;
+        case KIND_AXPY: qn = q[v] + a.c0 * r; break;
+        case KIND_3A: qn = a.c1 * qa[v] + a.c2 * q[v] + a.c0 * r; break;
+        case KIND_3B: qn = a.c1 * qa[v] + a.c2 * q[v] + a.c3 * r * a.c4; break;
+        case KIND_S54_4: qn = a.c1 * qa[v] + a.c2 * q[v] + a.c0 * r; a.Rout[o] = r; break;
+        default: qn = a.c1 * qa[v] + a.c2 * a.Qb[o] + a.c3 * a.Qc[o] + a.c4 * q[v] + a.c5 * a.Rprev[o] + a.c0 * r; break;
+        }
+        a.Qout[o] = qn;
+        qout[v * NP] = qn;
+    }
+}
+
+// register prefetch of a point's state and, for the stages that combine with
+// Qa, of Qa
+__device__ __forceinline__ void prefetch_point(const StageArgs &a, size_t base, int NP, double *q, double *qa) {
+    const bool need_a = a.kind != KIND_RESID && a.kind != KIND_AXPY;
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        q[v] = a.Qin[base + (size_t)v * NP];
+        qa[v] = need_a ? a.Qa[base + (size_t)v * NP] : 0.0;
+    }
 }
 
 // delta trace: sum_sp dq(sp) L(r,sp) of all 9 variables
@@ -228,36 +287,38 @@ __global__ void __launch_bounds__(256) k_face_inv_mx(FaceArgs a) {
 // ---------------------------------------------------------------------------
 // Ideal-path stage kernel, mixed form: every flux point evaluates the delta
 // flux of its own trace, face points add the jump terms of k_face_inv_mx; the
-// residual is formed in FP32 and combined into the FP64 state.
+// residual is formed in FP32 and combined into the FP64 state.  LDS holds the
+// FP32 deviations and flux planes only (one union area, reused for the FP64
+// copy at set-up and for the new state at the end).
 // ---------------------------------------------------------------------------
 template <int N>
 __global__ void __launch_bounds__(IShape<N>::THREADS) k_stage_ideal_mx(StageArgs a) {
     using S = IShape<N>;
-    constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
+    constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ, T = S::THREADS;
     constexpr int FST = 11 * NF + 1;     // per-cell stride: dF1 (8) + dA_z + metric rows (2)
-    __shared__ double sQ[CPB * NQ];
+    constexpr int UF = CPB * NQ + CPB * FST;
+    constexpr int UD = (UF + 1) / 2 > CPB * NQ ? (UF + 1) / 2 : CPB * NQ;
+    __shared__ double U[UD];
     __shared__ double sX[CPB * 8];
     __shared__ double sBar[CPB * NV];
     __shared__ Ref sRef[CPB];
-    __shared__ float sF[CPB * FST];
-    __shared__ float sD[CPB * NQ];       // deviations from the cell means
+    float *sD = (float *)U, *sF = sD + CPB * NQ;
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = blk * CPB;
     const int ncl = min(CPB, a.ncell - c0);
-    for (int i = tid; i < ncl * NQ; i += blockDim.x) sQ[i] = a.Qin[(size_t)c0 * NQ + i];
-    for (int i = tid; i < ncl * 8; i += blockDim.x) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
-    __syncthreads();
-    block_refs<N>(sQ, ncl, a.gam, sBar, sRef);
-    block_deviations<N>(sQ, sBar, ncl, sD);
+    for (int i = tid; i < ncl * 8; i += T) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
+    stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, U, sBar, sRef, sD, nullptr);
     const float gam = float(a.gam);
     const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
     const bool pact = (tid < CPB * NP) && (pc < ncl);
+    const size_t base = (size_t)(c0 + pc) * NQ + pk;
     const float *FF = (const float *)a.FF;
-    float dF[8], dA1 = 0.f, dA2 = 0.f;
+    float dF[8], dA1 = 0.f, dA2 = 0.f, res[NV];
+    double qown[NV], qa[NV];
 #pragma unroll 1
     for (int fam = 0; fam < 2; fam++) {
-        for (int it = tid; it < ncl * NF; it += blockDim.x) {
+        for (int it = tid; it < ncl * NF; it += T) {
             const int c = it / NF, k = it % NF;
             const int line = k / (N + 1), r = k % (N + 1);
             const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
@@ -285,6 +346,8 @@ __global__ void __launch_bounds__(IShape<N>::THREADS) k_stage_ideal_mx(StageArgs
             f[10 * NF] = s2;
         }
         __syncthreads();
+        // the point's state for the combination, loaded while the derivative is summed
+        if (fam == 1 && pact) prefetch_point(a, base, NP, qown, qa);
         if (pact) {
             const int line = fam == 0 ? pjs : pis, pos = fam == 0 ? pis : pjs;
             const float *f = sF + pc * FST + line * (N + 1);
@@ -304,75 +367,84 @@ __global__ void __launch_bounds__(IShape<N>::THREADS) k_stage_ideal_mx(StageArgs
                 for (int v = 0; v < 8; v++) dF[v] = d[v];
                 dA1 = e1; dA2 = e2;
             } else {
-                double *qs = sQ + pc * NQ + pk;
                 const float iJ = float(1.0 / jacobian<N>(sX + pc * 8, pis, pjs));
-                const float iq0 = frcp(float(qs[0]));
-                const float au = float(qs[NP]) * iq0, av = float(qs[2 * NP]) * iq0;
-                float res[NV];
+                const float iq0 = frcp(float(qown[0]));
+                const float au = float(qown[1]) * iq0, av = float(qown[2]) * iq0;
 #pragma unroll
                 for (int v = 0; v < 8; v++) res[v] = -(dF[v] + d[v]) * iJ;
                 res[8] = -(au * (dA1 + e1) + av * (dA2 + e2)) * iJ;
-                stage_combine(a, (size_t)(c0 + pc) * NQ + pk, NP, qs, res);
             }
         }
         __syncthreads();
     }
-    if (a.kind != KIND_RESID) store_traces<N>(sQ, ncl, c0, a.TQ);
+    if (pact) combine_point(a, base, NP, qown, qa, res, U + pc * NQ + pk);
+    if (a.kind == KIND_RESID) return;
+    __syncthreads();
+    store_traces<N>(U, ncl, c0, a.TQ);
 }
 
 // ---------------------------------------------------------------------------
-// Viscous / AV path, mixed form: the launches of the FP64 path.  The central
-// Qv of the gradient pass is a deviation from the cell mean (own delta trace
-// plus half the jump of the two FP64 traces at face points); nablaQs, its
-// traces and the viscous fluxes are plain FP32 (they carry no large constant
-// part); the viscous face flux is evaluated once per face point from the FP64
-// trace average.
+// Viscous / AV path, mixed form: the launches of the FP64 path, two threads
+// per solution point as there.  The central Qv of the gradient pass is a
+// deviation from the cell mean (own delta trace plus half the jump of the two
+// FP64 traces at face points); nablaQs, its traces and the viscous fluxes are
+// plain FP32 (they carry no large constant part); the viscous face flux is
+// evaluated once per face point from the FP64 trace average.
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ void __launch_bounds__(VShape<N>::THREADS) k_grad_mx(ViscArgs a) {
+__global__ void __launch_bounds__(V2Shape<N>::THREADS) k_grad_mx(ViscArgs a) {
     using S = VShape<N>;
-    constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
+    using TS = V2Shape<N>;
+    constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ, T = TS::THREADS, NIT = TS::NIT;
     constexpr int VST = 10 * NF + 1;
-    __shared__ double sQ[CPB * NQ];
+    constexpr int UF = CPB * NQ + CPB * VST + CPB * 16 * NP;
+    constexpr int UD = (UF + 1) / 2 > CPB * NQ ? (UF + 1) / 2 : CPB * NQ;
+    __shared__ double U[UD];
     __shared__ double sX[CPB * 8];
     __shared__ double sBar[CPB * NV];
     __shared__ Ref sRef[CPB];
-    __shared__ float sV[CPB * VST];
-    __shared__ float sD[CPB * NQ];       // deviations from the cell means
-    __shared__ float sG[CPB * 16 * NP];
-    __shared__ double sAV[CPB * NP];
+    float *sD = (float *)U, *sV = sD + CPB * NQ, *sG = sV + CPB * VST;
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = blk * CPB;
     const int ncl = min(CPB, a.ncell - c0);
-    for (int i = tid; i < ncl * NQ; i += blockDim.x) sQ[i] = a.Qin[(size_t)c0 * NQ + i];
-    for (int i = tid; i < ncl * 8; i += blockDim.x) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
-    __syncthreads();
-    block_refs<N>(sQ, ncl, a.gam, sBar, sRef);
-    block_deviations<N>(sQ, sBar, ncl, sD);
-    const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
-    const bool pact = (tid < CPB * NP) && (pc < ncl);
+    for (int i = tid; i < ncl * 8; i += T) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
+    stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, U, sBar, sRef, sD, nullptr);
+    const int grp = tid >= TS::G1 ? 1 : 0;
+    const int pt = tid - grp * TS::G1;
+    const int pc = pt / NP, pk = pt % NP, pjs = pk / N, pis = pk % N;
+    const bool pact = (pt < CPB * NP) && (pc < ncl);
+    float gp[8];
 #pragma unroll 1
     for (int fam = 0; fam < 2; fam++) {
-        for (int it = tid; it < ncl * NF; it += blockDim.x) {
-            const int c = it / NF, k = it % NF;
-            const int line = k / (N + 1), r = k % (N + 1);
-            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+        for (int t = tid; t < TS::NI + TS::NFC; t += T) {
+            int c, line, r;
             float half_jump[8];
 #pragma unroll
             for (int v = 0; v < 8; v++) half_jump[v] = 0.f;
-            if (r == 0 || r == N) {
-                int lfi, nfp;
-                face_of<N>(fam, r, line, lfi, nfp);
-                const int cell = c0 + c;
-                const int2 info = a.nbr[(size_t)cell * 4 + lfi];
-                if (!((info.y >> 4) & 1)) {
-                    const double *own = a.TQ + (((size_t)cell * 4 + lfi) * N + nfp) * NV;
-                    const double *oth = a.TQ + (((size_t)info.x * 4 + (info.y & 7) - 1) * N + partner_nfp<N>(nfp)) * NV;
+            if (t < TS::NI) {
+                c = t / NIT;
+                const int k = t % NIT;
+                line = k / (N - 1); r = 1 + k % (N - 1);
+            } else {
+                const int item = t - TS::NI, k = item % (2 * N);
+                c = item / (2 * N);
+                line = k >> 1; r = (k & 1) ? N : 0;
+                if (c < ncl) {
+                    int lfi, nfp;
+                    face_of<N>(fam, r, line, lfi, nfp);
+                    const int cell = c0 + c;
+                    const int2 info = a.nbr[(size_t)cell * 4 + lfi];
+                    if (!((info.y >> 4) & 1)) {
+                        const double *own = a.TQ + (((size_t)cell * 4 + lfi) * N + nfp) * NV;
+                        const double *oth = a.TQ + (((size_t)info.x * 4 + (info.y & 7) - 1) * N + partner_nfp<N>(nfp)) * NV;
 #pragma unroll
-                    for (int v = 0; v < 8; v++) half_jump[v] = 0.5f * float(oth[v] - own[v]);
+                        for (int v = 0; v < 8; v++) half_jump[v] = 0.5f * float(oth[v] - own[v]);
+                    }
                 }
             }
+            if (c >= ncl) continue;
+            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
             double s1d, s2d;
             metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
             float d[NV];
@@ -387,23 +459,29 @@ __global__ void __launch_bounds__(VShape<N>::THREADS) k_grad_mx(ViscArgs a) {
         if (pact) {
             const int line = fam == 0 ? pjs : pis, pos = fam == 0 ? pis : pjs;
             const float *q = sV + pc * VST + line * (N + 1);
-            const float iJ = fam == 1 ? float(1.0 / jacobian<N>(sX + pc * 8, pis, pjs)) : 0.f;
+            const float *mq = q + (8 + grp) * NF;
+            float acc[8];
 #pragma unroll
-            for (int d = 0; d < 2; d++)
+            for (int v = 0; v < 8; v++) acc[v] = 0.f;
+#pragma unroll CHORUS_MX_CUNROLL
+            for (int r = 0; r <= N; r++) {
+                const float mm = c_Mf[N][r * N + pos] * mq[r];
 #pragma unroll
-                for (int v = 0; v < 8; v++) {
-                    float acc = 0.f;
+                for (int v = 0; v < 8; v++) acc[v] += q[v * NF + r] * mm;
+            }
+            if (fam == 0) {
 #pragma unroll
-                    for (int r = 0; r <= N; r++) acc += q[v * NF + r] * c_Mf[N][r * N + pos] * q[(8 + d) * NF + r];
-                    float *gdst = &sG[(pc * 16 + d * 8 + v) * NP + pk];
-                    if (fam == 0) *gdst = acc;
-                    else *gdst = (*gdst + acc) * iJ;
-                }
+                for (int v = 0; v < 8; v++) gp[v] = acc[v];
+            } else {
+                const float iJ = float(1.0 / jacobian<N>(sX + pc * 8, pis, pjs));
+#pragma unroll
+                for (int v = 0; v < 8; v++) sG[(pc * 16 + grp * 8 + v) * NP + pk] = (gp[v] + acc[v]) * iJ;
+            }
         }
         __syncthreads();
     }
-    for (int i = tid; i < ncl * 16 * NP; i += blockDim.x) a.gsol_r<float>()[(size_t)c0 * 16 * NP + i] = sG[i];
-    for (int it = tid; it < ncl * 4 * N * 16; it += blockDim.x) {
+    for (int i = tid; i < ncl * 16 * NP; i += T) a.gsol_r<float>()[(size_t)c0 * 16 * NP + i] = sG[i];
+    for (int it = tid; it < ncl * 4 * N * 16; it += T) {
         const int c = it / (4 * N * 16), k = it % (4 * N * 16), fpi = k / 16, g = k % 16;
         const int lfi = fpi / N, nfp = fpi % N;
         int ifp, jfp;
@@ -420,12 +498,11 @@ __global__ void __launch_bounds__(VShape<N>::THREADS) k_grad_mx(ViscArgs a) {
         }
         a.gtr_r<float>()[((size_t)(c0 + c) * 4 * N + fpi) * 16 + g] = acc;
     }
-    if (a.artiv) av_sensor<N>(a.sm, sQ, sAV, ncl, c0);
 }
 
 // viscous and AV face flux, mixed form: k_face_visc in FP32 from the FP64
 // trace average
-template <int N>
+template <int N, int VM>
 __global__ void __launch_bounds__(256) k_face_visc_mx(FaceArgs a) {
     const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= a.nface * N) return;
@@ -467,15 +544,19 @@ __global__ void __launch_bounds__(256) k_face_visc_mx(FaceArgs a) {
     float fl[NV], fr[NV], Fv[NV], Gv[NV];
 #pragma unroll
     for (int w = 0; w < NV; w++) { fl[w] = 0.f; fr[w] = 0.f; }
-    if (a.mu != 0.0 || a.eta0 != 0.0) {
+    if constexpr ((VM & 1) != 0) {
         vflux<float>(gam, lam, prandt, float(a.mu), float(a.eta0), qv, g, Fv, Gv);
 #pragma unroll
         for (int w = 0; w < NV; w++) { fl[w] = Fv[w] * sl1 + Gv[w] * sl2; fr[w] = Fv[w] * sr1 + Gv[w] * sr2; }
     }
-    if (a.artiv) {
+    if constexpr ((VM & 2) != 0) {
         const double eps0 = *a.eig_now * a.h_msh / (double)(N - 1);
         const double ml = mu_av(a.sm[fc.x], eps0, a.pi);
         const double mf = free_face ? ml : 0.5 * (ml + mu_av(a.sm[fc.z], eps0, a.pi));
+        if (k == 0) {
+            a.mu4[(size_t)fc.x * 4 + fc.y] = mf;
+            if (!free_face) a.mu4[(size_t)fc.z * 4 + fc.w] = mf;
+        }
         vflux<float>(gam, lam, prandt, float(mf), 0.f, qv, g, Fv, Gv);
 #pragma unroll
         for (int w = 0; w < NV; w++) {
@@ -492,144 +573,175 @@ __global__ void __launch_bounds__(256) k_face_visc_mx(FaceArgs a) {
     }
 }
 
-template <int N>
-__global__ void __launch_bounds__(VShape<N>::THREADS) k_stage_visc_mx(StageArgs a, ViscArgs v) {
+// task layout of the mixed viscous stage kernel: inviscid delta flux at every
+// flux point (face points also copy the viscous face flux), then the viscous
+// flux at the interior points; derivative groups as in V2Shape
+template <int N> struct MxVShape {
     using S = VShape<N>;
-    constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
+    static constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF;
+    static constexpr int NIT = N * (N - 1);
+    static constexpr int NA = CPB * NF;                      // inviscid tasks (all flux points)
+    static constexpr int S1 = NA;
+    static constexpr int NI = CPB * NIT;                     // viscous interior tasks
+    static constexpr int BT = S1 + NI;
+    static constexpr int G1 = ((CPB * NP + 63) / 64) * 64;
+    static constexpr int CT = G1 + CPB * NP;
+    static constexpr int THREADS = (((BT > CT ? BT : CT) + 63) / 64) * 64;
+};
+
+#ifndef CHORUS_MX_VISC_MINW
+#define CHORUS_MX_VISC_MINW 3
+#endif
+template <int N, int VM>
+__global__ void __launch_bounds__(MxVShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(CHORUS_MX_VISC_MINW))) k_stage_visc_mx(StageArgs a, ViscArgs v) {
+    using S = VShape<N>;
+    using TS = MxVShape<N>;
+    constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ, T = TS::THREADS, NIT = TS::NIT;
     constexpr int UST = 20 * NF + 1;     // dF1 (8) + Fv1 (9) + dA_z + metric rows (2)
-    __shared__ double sQ[CPB * NQ];
+    constexpr int UF = CPB * NQ + CPB * UST + CPB * 16 * NP;
+    constexpr int UD = (UF + 1) / 2 > CPB * NQ ? (UF + 1) / 2 : CPB * NQ;
+    __shared__ double U[UD];
     __shared__ double sX[CPB * 8];
     __shared__ double sBar[CPB * NV];
     __shared__ Ref sRef[CPB];
-    __shared__ float sG[CPB * 16 * NP];
-    __shared__ float sU[CPB * UST];
-    __shared__ float sD[CPB * NQ];       // deviations from the cell means
     __shared__ double sMu[CPB * 4];
+    __shared__ double sAV[CPB * NP];
+    float *sD = (float *)U, *sU = sD + CPB * NQ, *sG = sU + CPB * UST;
+    static_assert(9 * CPB * NP <= CPB * UST, "exchange area");
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = blk * CPB;
     const int ncl = min(CPB, a.ncell - c0);
-    for (int i = tid; i < ncl * NQ; i += blockDim.x) sQ[i] = a.Qin[(size_t)c0 * NQ + i];
-    for (int i = tid; i < ncl * 8; i += blockDim.x) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
-    for (int i = tid; i < ncl * 16 * NP; i += blockDim.x) sG[i] = v.gsol_r<float>()[(size_t)c0 * 16 * NP + i];
-    if (v.artiv) {
-        const double eps0 = *v.eig * v.h_msh / (double)(N - 1);
-        for (int i = tid; i < ncl * 4; i += blockDim.x) {
-            const int c = i / 4, lfi = i % 4, cell = c0 + c;
-            const int2 info = a.nbr[(size_t)cell * 4 + lfi];
-            const double mine = mu_av(v.sm[cell], eps0, v.pi);
-            double val = mine;
-            if (!((info.y >> 4) & 1)) val = 0.5 * (mine + mu_av(v.sm[info.x], eps0, v.pi));
-            sMu[i] = val;
-        }
+    for (int i = tid; i < ncl * 8; i += T) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
+    if constexpr ((VM & 2) != 0) {
+        for (int i = tid; i < ncl * 4; i += T) sMu[i] = v.mu4[(size_t)c0 * 4 + i];
         if (blockIdx.x == 0 && tid == 0) *v.eig_next = 0.0;
     }
+    stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, U, sBar, sRef, sD, nullptr);
+    for (int i = tid; i < ncl * 16 * NP; i += T) sG[i] = v.gsol_r<float>()[(size_t)c0 * 16 * NP + i];
     __syncthreads();
-    block_refs<N>(sQ, ncl, a.gam, sBar, sRef);
-    block_deviations<N>(sQ, sBar, ncl, sD);
     const float gam = float(a.gam);
-    const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
-    const bool pact = (tid < CPB * NP) && (pc < ncl);
+    const int grp = tid >= TS::G1 ? 1 : 0;
+    const int pt = tid - grp * TS::G1;
+    const int pc = pt / NP, pk = pt % NP, pjs = pk / N, pis = pk % N;
+    const bool pact = (pt < CPB * NP) && (pc < ncl);
+    const size_t base = (size_t)(c0 + pc) * NQ + pk;
     const float *FF = (const float *)a.FF, *FFv = (const float *)a.FFv;
-    float dF[8], dFv[NV], dA1 = 0.f, dA2 = 0.f;
+    float acc8[8], accv[NV], dA1 = 0.f, dA2 = 0.f;
+    double qown[NV], qa[NV];
 #pragma unroll 1
     for (int fam = 0; fam < 2; fam++) {
-        // every flux point: inviscid delta flux of the own trace (+ face jump terms)
-        for (int it = tid; it < ncl * NF; it += blockDim.x) {
-            const int c = it / NF, k = it % NF;
-            const int line = k / (N + 1), r = k % (N + 1);
-            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
-            float jump[NV];
+        for (int t = tid; t < TS::BT; t += T) {
+            if (t < TS::NA) {
+                // inviscid delta flux of the own trace, + jump terms and the
+                // viscous face flux at face points
+                const int c = t / NF, k = t % NF;
+                if (c >= ncl) continue;
+                const int line = k / (N + 1), r = k % (N + 1);
+                const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+                float *u = sU + c * UST + line * (N + 1) + r;
+                float jump[NV];
 #pragma unroll
-            for (int w = 0; w < NV; w++) jump[w] = 0.f;
-            if (r == 0 || r == N) {
-                int lfi, nfp;
-                face_of<N>(fam, r, line, lfi, nfp);
-                const float *src = FF + (((size_t)(c0 + c) * 4 + lfi) * N + nfp) * NV;
+                for (int w = 0; w < NV; w++) jump[w] = 0.f;
+                if (r == 0 || r == N) {
+                    int lfi, nfp;
+                    face_of<N>(fam, r, line, lfi, nfp);
+                    const size_t o = (((size_t)(c0 + c) * 4 + lfi) * N + nfp) * NV;
 #pragma unroll
-                for (int w = 0; w < NV; w++) jump[w] = src[w];
+                    for (int w = 0; w < NV; w++) jump[w] = FF[o + w];
+#pragma unroll
+                    for (int w = 0; w < NV; w++) u[(8 + w) * NF] = FFv[o + w];
+                }
+                double s1d, s2d;
+                metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+                const float s1 = float(s1d), s2 = float(s2d);
+                float d[NV], F[8], G[8];
+                dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d);
+                dflux(sRef[c], d, gam - 1.f, F, G);
+#pragma unroll
+                for (int w = 0; w < 8; w++) u[w * NF] = F[w] * s1 + G[w] * s2 + jump[w];
+                u[17 * NF] = d[8] + jump[8];
+                u[18 * NF] = s1;
+                u[19 * NF] = s2;
+            } else if (t >= TS::S1) {
+                // viscous / AV flux of the own trace at an interior point
+                const int item = t - TS::S1;
+                const int c = item / NIT, k = item % NIT;
+                if (c >= ncl) continue;
+                const int line = k / (N - 1), r = 1 + k % (N - 1);
+                const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+                double s1d, s2d;
+                metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+                float d[NV], qv[NV], g[2 * NV], fv[NV];
+                dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d);
+#pragma unroll
+                for (int w = 0; w < NV; w++) qv[w] = sRef[c].q[w] + d[w];
+                grad_trace<N, float>(sG + c * 16 * NP, fam, ifp, jfp, g);
+                visc_point<N, float, VM>(v, sMu + c * 4, fam, ifp, jfp, float(s1d), float(s2d), qv, g, fv);
+                float *u = sU + c * UST + line * (N + 1) + r;
+#pragma unroll
+                for (int w = 0; w < NV; w++) u[(8 + w) * NF] = fv[w];
             }
-            double s1d, s2d;
-            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
-            const float s1 = float(s1d), s2 = float(s2d);
-            float d[NV], F[8], G[8];
-            dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d);
-            dflux(sRef[c], d, gam - 1.f, F, G);
-            float *u = sU + c * UST + line * (N + 1) + r;
-#pragma unroll
-            for (int w = 0; w < 8; w++) u[w * NF] = F[w] * s1 + G[w] * s2 + jump[w];
-            u[17 * NF] = d[8] + jump[8];
-            u[18 * NF] = s1;
-            u[19 * NF] = s2;
-        }
-        // interior flux points: viscous / AV flux of the own trace
-        for (int it = tid; it < ncl * N * (N - 1); it += blockDim.x) {
-            const int c = it / (N * (N - 1)), k = it % (N * (N - 1));
-            const int line = k / (N - 1), r = 1 + k % (N - 1);
-            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
-            double s1d, s2d;
-            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
-            float d[NV], qv[NV], g[2 * NV], fv[NV];
-            dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d);
-#pragma unroll
-            for (int w = 0; w < NV; w++) qv[w] = sRef[c].q[w] + d[w];
-            grad_trace<N, float>(sG + c * 16 * NP, fam, ifp, jfp, g);
-            visc_point<N, float>(v, sMu + c * 4, fam, ifp, jfp, float(s1d), float(s2d), qv, g, fv);
-            float *u = sU + c * UST + line * (N + 1) + r;
-#pragma unroll
-            for (int w = 0; w < NV; w++) u[(8 + w) * NF] = fv[w];
-        }
-        // face points: the viscous face flux of k_face_visc_mx
-        for (int it = tid; it < ncl * 2 * N; it += blockDim.x) {
-            const int c = it / (2 * N), k = it % (2 * N);
-            const int line = k >> 1, r = (k & 1) ? N : 0;
-            int lfi, nfp;
-            face_of<N>(fam, r, line, lfi, nfp);
-            const float *src = FFv + (((size_t)(c0 + c) * 4 + lfi) * N + nfp) * NV;
-            float *u = sU + c * UST + line * (N + 1) + r;
-#pragma unroll
-            for (int w = 0; w < NV; w++) u[(8 + w) * NF] = src[w];
         }
         __syncthreads();
         if (pact) {
             const int line = fam == 0 ? pjs : pis, pos = fam == 0 ? pis : pjs;
             const float *f = sU + pc * UST + line * (N + 1);
-            float d[8], dv[NV], e1 = 0.f, e2 = 0.f;
+            if (grp == 0) {
+                float d[8], e1 = 0.f, e2 = 0.f;
 #pragma unroll
-            for (int w = 0; w < 8; w++) d[w] = 0.f;
-#pragma unroll
-            for (int w = 0; w < NV; w++) dv[w] = 0.f;
+                for (int w = 0; w < 8; w++) d[w] = 0.f;
 #pragma unroll CHORUS_MX_CUNROLL
-            for (int r = 0; r <= N; r++) {
-                const float m = c_Mf[N][r * N + pos];
+                for (int r = 0; r <= N; r++) {
+                    const float m = c_Mf[N][r * N + pos];
 #pragma unroll
-                for (int w = 0; w < 8; w++) d[w] += f[w * NF + r] * m;
+                    for (int w = 0; w < 8; w++) d[w] += f[w * NF + r] * m;
+                    e1 += f[17 * NF + r] * m * f[18 * NF + r];
+                    e2 += f[17 * NF + r] * m * f[19 * NF + r];
+                }
 #pragma unroll
-                for (int w = 0; w < NV; w++) dv[w] += f[(8 + w) * NF + r] * m;
-                e1 += f[17 * NF + r] * m * f[18 * NF + r];
-                e2 += f[17 * NF + r] * m * f[19 * NF + r];
-            }
-            if (fam == 0) {
-#pragma unroll
-                for (int w = 0; w < 8; w++) dF[w] = d[w];
-#pragma unroll
-                for (int w = 0; w < NV; w++) dFv[w] = dv[w];
-                dA1 = e1; dA2 = e2;
+                for (int w = 0; w < 8; w++) acc8[w] = fam == 0 ? d[w] : acc8[w] + d[w];
+                dA1 = dA1 + e1; dA2 = dA2 + e2;
             } else {
-                double *qs = sQ + pc * NQ + pk;
-                const float iJ = float(1.0 / jacobian<N>(sX + pc * 8, pis, pjs));
-                const float iq0 = frcp(float(qs[0]));
-                const float au = float(qs[NP]) * iq0, av = float(qs[2 * NP]) * iq0;
-                float res[NV];
+                float dv[NV];
 #pragma unroll
-                for (int w = 0; w < 8; w++) res[w] = (-(dF[w] + d[w]) + (dFv[w] + dv[w])) * iJ;
-                res[8] = (-(au * (dA1 + e1) + av * (dA2 + e2)) + (dFv[8] + dv[8])) * iJ;
-                stage_combine(a, (size_t)(c0 + pc) * NQ + pk, NP, qs, res);
+                for (int w = 0; w < NV; w++) dv[w] = 0.f;
+#pragma unroll CHORUS_MX_CUNROLL
+                for (int r = 0; r <= N; r++) {
+                    const float m = c_Mf[N][r * N + pos];
+#pragma unroll
+                    for (int w = 0; w < NV; w++) dv[w] += f[(8 + w) * NF + r] * m;
+                }
+#pragma unroll
+                for (int w = 0; w < NV; w++) accv[w] = fam == 0 ? dv[w] : accv[w] + dv[w];
             }
         }
         __syncthreads();
     }
-    if (a.kind != KIND_RESID) store_traces<N>(sQ, ncl, c0, a.TQ);
+    // group 1 hands its viscous sums to group 0 through the flux area
+    float *xv = sU;
+    if (pact && grp == 1) {
+#pragma unroll
+        for (int w = 0; w < NV; w++) xv[w * CPB * NP + pt] = accv[w];
+    }
+    __syncthreads();
+    float res[NV];
+    if (pact && grp == 0) {
+        prefetch_point(a, base, NP, qown, qa);
+        const float iJ = float(1.0 / jacobian<N>(sX + pc * 8, pis, pjs));
+        const float iq0 = frcp(float(qown[0]));
+        const float au = float(qown[1]) * iq0, av = float(qown[2]) * iq0;
+#pragma unroll
+        for (int w = 0; w < 8; w++) res[w] = (-acc8[w] + xv[w * CPB * NP + pt]) * iJ;
+        res[8] = (-(au * dA1 + av * dA2) + xv[8 * CPB * NP + pt]) * iJ;
+    }
+    __syncthreads();
+    // the new state replaces the compute layout for the traces
+    if (pact && grp == 0) combine_point(a, base, NP, qown, qa, res, U + pc * NQ + pk);
+    if (a.kind == KIND_RESID) return;
+    __syncthreads();
+    store_traces<N>(U, ncl, c0, a.TQ);
+    if constexpr ((VM & 2) != 0) av_sensor<N, CPB>(a.sm, U, sAV, ncl, c0);
 }
 
 }  // namespace chorus

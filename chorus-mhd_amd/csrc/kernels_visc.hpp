@@ -78,12 +78,14 @@ template <int N> __device__ __forceinline__ int partner_nfp(int nfp) { return N 
 // AV smoothness indicator of density (AV.f90:15-82, FP64 in both forms: the
 // N -> N-1 -> N interpolation round trip and log10 of the relative energy).
 // Called by all threads of the block.
-template <int N>
-__device__ __forceinline__ void av_sensor(double *sm, const double *sQ, double *sAV, int ncl, int c0) {
-    constexpr int NP = N * N, NQ = NV * NP, NL = N - 1;
+// rho[c * CS + p]: the block's densities (cell stride CS); CPB cells per block
+template <int N, int CPB, int CS = NV * N * N>
+__device__ __forceinline__ void av_sensor(double *sm, const double *rho, double *sAV, int ncl, int c0) {
+    constexpr int NP = N * N, NQ = CS, NL = N - 1;
+    const double *sQ = rho;
     const int tid = threadIdx.x;
     const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
-    const bool pact = (tid < VShape<N>::CPB * NP) && (pc < ncl);
+    const bool pact = (tid < CPB * NP) && (pc < ncl);
     double *low = sAV;   // [c][jsl*NL+isl] uses the first NL*NL entries of each cell
     for (int it = tid; it < ncl * NL * NL; it += blockDim.x) {
         const int c = it / (NL * NL), k = it % (NL * NL), jsl = k / NL, isl = k % NL;
@@ -113,6 +115,23 @@ __device__ __forceinline__ void av_sensor(double *sm, const double *sQ, double *
     }
 }
 
+// traces of a state (after set-up, or for a state not produced by a stage) and,
+// with sm, its AV indicator
+template <int N>
+__global__ void __launch_bounds__(256) k_trace(int ncell, const double *Q, double *TQ, double *sm) {
+    using S = Shape<N>;
+    constexpr int CPB = S::CPB, NP = S::NP, NQ = S::NQ;
+    __shared__ double sQ[CPB * NQ];
+    __shared__ double sAV[CPB * NP];
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int c0 = blk * CPB;
+    const int ncl = min(CPB, ncell - c0);
+    for (int i = threadIdx.x; i < ncl * NQ; i += blockDim.x) sQ[i] = Q[(size_t)c0 * NQ + i];
+    __syncthreads();
+    store_traces<N>(sQ, ncl, c0, TQ);
+    if (sm) av_sensor<N, CPB>(sm, sQ, sAV, ncl, c0);
+}
+
 // ---------------------------------------------------------------------------
 // Two threads per solution point (V2Shape): the flux phase has one task per
 // flux point, the derivative phase splits the x (group 0) and y (group 1)
@@ -128,7 +147,6 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
     __shared__ double sX[CPB * 8];
     __shared__ R sV[CPB * VST];          // [c][v | s1 | s2][line][r], one family at a time
     __shared__ R sG[CPB * 16 * NP];      // nablaQs [c][d*8+v][p]
-    __shared__ double sAV[CPB * NP];     // sensor work
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = blk * CPB;
@@ -208,14 +226,17 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
                 for (int v = 0; v < 8; v++) gp[v] = acc[v];
             } else {
                 const R J = R(jacobian<N>(sX + pc * 8, pis, pjs));
+                R *gs = a.gsol_r<R>() + (size_t)(c0 + pc) * 16 * NP;
 #pragma unroll
-                for (int v = 0; v < 8; v++) sG[(pc * 16 + grp * 8 + v) * NP + pk] = (gp[v] + acc[v]) / J;
+                for (int v = 0; v < 8; v++) {
+                    const R g = (gp[v] + acc[v]) / J;
+                    sG[(pc * 16 + grp * 8 + v) * NP + pk] = g;
+                    gs[(grp * 8 + v) * NP + pk] = g;
+                }
             }
         }
         __syncthreads();
     }
-    // nablaQs to HBM for the stage kernel
-    for (int i = tid; i < ncl * 16 * NP; i += blockDim.x) a.gsol_r<R>()[(size_t)c0 * 16 * NP + i] = sG[i];
     // own traces of nablaQs at the four faces (interfaceVISflux, interfaceflux_all.f90:499-536)
     for (int it = tid; it < ncl * 4 * N * 16; it += blockDim.x) {
         const int c = it / (4 * N * 16), k = it % (4 * N * 16), fpi = k / 16, g = k % 16;
@@ -234,7 +255,6 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
         }
         a.gtr_r<R>()[((size_t)(c0 + c) * 4 * N + fpi) * 16 + g] = acc;
     }
-    if (a.artiv) av_sensor<N>(a.sm, sQ, sAV, ncl, c0);
 }
 
 // muAV(1,:) of one cell from its indicator and epsilon0 (AV.f90:120-172)
@@ -271,12 +291,13 @@ __device__ __forceinline__ void grad_trace(const R *gq, int fam, int ifp, int jf
 }
 
 // viscous + AV contravariant flux at one flux point (comp_viscous_flux, compflux.f90:205-256)
-template <int N, class R>
+// VM: bit 0 physical viscosity/resistivity (mu or eta0 nonzero), bit 1 AV
+template <int N, class R, int VM>
 __device__ __forceinline__ void visc_point(const ViscArgs &v, const double *mu4, int fam, int ifp, int jfp, R s1,
                                            R s2, const R *qv, const R *g, R *fv) {
     const R gam = R(v.gam), lam = R(-2.0 / 3.0), prandt = R(0.72);
     R Fv[NV], Gv[NV];
-    if (v.mu != 0.0 || v.eta0 != 0.0) {
+    if constexpr ((VM & 1) != 0) {
         vflux<R>(gam, lam, prandt, R(v.mu), R(v.eta0), qv, g, Fv, Gv);
 #pragma unroll
         for (int w = 0; w < NV; w++) fv[w] = Fv[w] * s1 + Gv[w] * s2;
@@ -284,7 +305,7 @@ __device__ __forceinline__ void visc_point(const ViscArgs &v, const double *mu4,
 #pragma unroll
         for (int w = 0; w < NV; w++) fv[w] = R(0);
     }
-    if (v.artiv) {
+    if constexpr ((VM & 2) != 0) {
         const int lo = fam == 0 ? 3 : 0, hi = fam == 0 ? 1 : 2;   // sides 1 and 2 of the family
         const double x = fam == 0 ? c_Xf[N][ifp] : c_Xf[N][jfp];
         const R mu_art = R(mu4[lo] * (1.0 - x) + mu4[hi] * x);
@@ -304,7 +325,7 @@ __device__ __forceinline__ void visc_point(const ViscArgs &v, const double *mu4,
 // point the flux-point coordinate of mu_art's interpolation is exactly 0 or 1,
 // so mu_art is the face average itself.
 // ---------------------------------------------------------------------------
-template <int N, class R>
+template <int N, class R, int VM>
 __global__ void __launch_bounds__(256) k_face_visc(FaceArgs a) {
     const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= a.nface * N) return;
@@ -345,7 +366,7 @@ __global__ void __launch_bounds__(256) k_face_visc(FaceArgs a) {
     }
     const R gam = R(a.gam), lam = R(-2.0 / 3.0), prandt = R(0.72);
     R fl[NV], fr[NV], Fv[NV], Gv[NV];
-    if (a.mu != 0.0 || a.eta0 != 0.0) {
+    if constexpr ((VM & 1) != 0) {
         vflux<R>(gam, lam, prandt, R(a.mu), R(a.eta0), qv, g, Fv, Gv);
 #pragma unroll
         for (int w = 0; w < NV; w++) { fl[w] = Fv[w] * sl1 + Gv[w] * sl2; fr[w] = Fv[w] * sr1 + Gv[w] * sr2; }
@@ -353,10 +374,14 @@ __global__ void __launch_bounds__(256) k_face_visc(FaceArgs a) {
 #pragma unroll
         for (int w = 0; w < NV; w++) { fl[w] = R(0); fr[w] = R(0); }
     }
-    if (a.artiv) {
+    if constexpr ((VM & 2) != 0) {
         const double eps0 = *a.eig_now * a.h_msh / (double)(N - 1) * 1.0;
         const double ml = mu_av(a.sm[fc.x], eps0, a.pi);
         const double mf = free_face ? ml : 0.5 * (ml + mu_av(a.sm[fc.z], eps0, a.pi));
+        if (k == 0) {
+            a.mu4[(size_t)fc.x * 4 + fc.y] = mf;
+            if (!free_face) a.mu4[(size_t)fc.z * 4 + fc.w] = mf;
+        }
         vflux<R>(gam, lam, prandt, R(mf), R(0), qv, g, Fv, Gv);
 #pragma unroll
         for (int w = 0; w < NV; w++) {
@@ -374,7 +399,7 @@ __global__ void __launch_bounds__(256) k_face_visc(FaceArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-template <int N, class R>
+template <int N, class R, int VM>
 __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(CHORUS_VISC_MINW))) k_stage_visc(StageArgs a, ViscArgs v) {
     using S = VShape<N>;
     using T = V2Shape<N>;
@@ -386,6 +411,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
     __shared__ R sG[CPB * 16 * NP];      // nablaQs from k_grad
     __shared__ R sU[CPB * UST];          // [c][plane][line][r], one family at a time
     __shared__ double sMu[CPB * 4];      // muAVfi/fj(1,side) per local face
+    __shared__ double sAV[CPB * NP];     // AV indicator work
     static_assert(9 * CPB * NP <= CPB * UST, "exchange area");
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -394,18 +420,10 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
     for (int i = tid; i < ncl * NQ; i += blockDim.x) sQ[i] = a.Qin[(size_t)c0 * NQ + i];
     for (int i = tid; i < ncl * 8; i += blockDim.x) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
     for (int i = tid; i < ncl * 16 * NP; i += blockDim.x) sG[i] = v.gsol_r<R>()[(size_t)c0 * 16 * NP + i];
-    if (v.artiv) {
-        // epsilon0 = eigvmax_g * h / (N-1) (AV.f90:145-148); face averages of muAV(1)
-        // (compAVflux/interfaceAVflux/cyclocAVflux, AV.f90:199-303, 494-564)
-        const double eps0 = *v.eig * v.h_msh / (double)(N - 1) * 1.0;
-        for (int i = tid; i < ncl * 4; i += blockDim.x) {
-            const int c = i / 4, lfi = i % 4, cell = c0 + c;
-            const int2 info = a.nbr[(size_t)cell * 4 + lfi];
-            const double mine = mu_av(v.sm[cell], eps0, v.pi);
-            double val = mine;
-            if (!((info.y >> 4) & 1)) val = 0.5 * (mine + mu_av(v.sm[info.x], eps0, v.pi));
-            sMu[i] = val;
-        }
+    if constexpr ((VM & 2) != 0) {
+        // face averages of muAV(1) (compAVflux/interfaceAVflux/cyclocAVflux,
+        // AV.f90:199-303, 494-564), evaluated by k_face_visc
+        for (int i = tid; i < ncl * 4; i += blockDim.x) sMu[i] = v.mu4[(size_t)c0 * 4 + i];
         if (blockIdx.x == 0 && tid == 0) *v.eig_next = 0.0;
     }
     __syncthreads();
@@ -417,6 +435,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
     const R *FF = (const R *)a.FF, *FFv = (const R *)a.FFv;
 
     R acc8[8], accv[NV], dA1 = R(0), dA2 = R(0);   // group 0: acc8 = dF; group 1: accv = dFv
+    const size_t base = (size_t)(c0 + pc) * NQ + pk;
 #pragma unroll 1
     for (int fam = 0; fam < 2; fam++) {
         for (int t = tid; t < T::BT; t += blockDim.x) {
@@ -450,7 +469,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
                     // viscous / AV flux
                     R g[2 * NV], fv[NV];
                     grad_trace<N, R>(sG + c * 16 * NP, fam, ifp, jfp, g);
-                    visc_point<N, R>(v, sMu + c * 4, fam, ifp, jfp, R(s1d), R(s2d), Qown, g, fv);
+                    visc_point<N, R, VM>(v, sMu + c * 4, fam, ifp, jfp, R(s1d), R(s2d), Qown, g, fv);
 #pragma unroll
                     for (int w = 0; w < NV; w++) u[(8 + w) * NF] = fv[w];
                 }
@@ -542,11 +561,14 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
         for (int w = 0; w < NV; w++) res[w] = res[w] + xv[w * CPB * NP + pt];
 #pragma unroll
         for (int w = 0; w < NV; w++) res[w] = res[w] / J;
-        stage_combine(a, (size_t)(c0 + pc) * NQ + pk, NP, qs, res);
+        stage_combine(a, base, NP, qs, res);
     }
+    if (a.kind == KIND_RESID) return;
     __syncthreads();
-    // traces of the new state for the next stage's face passes
-    if (a.kind != KIND_RESID) store_traces<N>(sQ, ncl, c0, a.TQ);
+    // traces of the new state for the next stage's face passes, and its AV
+    // indicator (AV.f90:15-82) for the next stage
+    store_traces<N>(sQ, ncl, c0, a.TQ);
+    if constexpr ((VM & 2) != 0) av_sensor<N, CPB>(a.sm, sQ, sAV, ncl, c0);
 }
 
 }  // namespace chorus
