@@ -60,6 +60,29 @@ def topology(mesh: Mesh) -> Topology:
                     cyc[:2 * npr].reshape(2, npr).T.copy(), ibfree[:nfree.value].copy())
 
 
+PRECISION = {"fp64": _lib.CHORUS_FP64, "mixed": _lib.CHORUS_MIXED}
+
+
+def _c_mesh(mesh: Mesh, N: int, t: Topology):
+    """ChorusMesh over arrays kept alive in the returned dict (the library copies them)."""
+    a = dict(
+        xv=np.ascontiguousarray(mesh.xv, dtype=np.float64),
+        yv=np.ascontiguousarray(mesh.yv, dtype=np.float64),
+        iv=np.asfortranarray(mesh.cells, dtype=np.int32).ravel(order="F"),
+        if2c=np.asfortranarray(t.if2c, dtype=np.int32).ravel(order="F"),
+        ibfinter=np.ascontiguousarray(t.ibfinter, dtype=np.int32),
+        cyc=np.asfortranarray(t.cyc2fpair, dtype=np.int32).ravel(order="F"),
+        ibfree=np.ascontiguousarray(t.ibfree, dtype=np.int32),
+    )
+    ip = _lib._ip
+    m = _lib.ChorusMesh(mesh.ncell, mesh.nvert, N, _lib.ptr(a["xv"]), _lib.ptr(a["yv"]),
+                        _lib.ptr(a["iv"], ip), t.nface, _lib.ptr(a["if2c"], ip),
+                        len(t.ibfinter), _lib.ptr(a["ibfinter"], ip),
+                        t.cyc2fpair.shape[0], _lib.ptr(a["cyc"], ip),
+                        len(t.ibfree), _lib.ptr(a["ibfree"], ip))
+    return m, a
+
+
 class Solver:
     """One MHD instance resident on the GPU."""
 
@@ -150,3 +173,106 @@ class Solver:
         self._check(_lib.load().chorus_time_steps(self._ctx, nsteps, C.byref(tot), C.byref(st),
                                                   C.byref(ct)), "chorus_time_steps")
         return dict(ms_total=tot.value, ms_stage_avg=st.value, ms_ct_avg=ct.value)
+
+
+@dataclass
+class Instance:
+    """One member of a batched ensemble."""
+    mesh: Mesh
+    N: int
+    dt: float
+    gam: float = 5.0 / 3
+    artiv: int = 0
+    k_stage: int = 3
+    mu: float = 0.0
+    eta0: float = 0.0
+    precision: str = "mixed"
+    topo: Topology = None
+
+
+class Batch:
+    """Many independent instances advanced together (chorus_batch_*): instances
+    with equal numerics share launches; one time step is one hipGraph launch."""
+
+    def __init__(self, members):
+        L = _lib.load()
+        self.members = list(members)
+        n = len(self.members)
+        meshes = (_lib.ChorusMesh * n)()
+        params = (_lib.ChorusParams * n)()
+        self._keep = []
+        for i, m in enumerate(self.members):
+            t = m.topo if m.topo is not None else topology(m.mesh)
+            cm, arrs = _c_mesh(m.mesh, m.N, t)
+            self._keep.append(arrs)
+            meshes[i] = cm
+            params[i] = _lib.ChorusParams(1, m.artiv, m.k_stage, m.mu, m.eta0, m.gam, m.dt, m.mesh.dxcycl,
+                                          m.mesh.dycycl, m.mesh.ncell, PRECISION[m.precision])
+        self._b = C.c_void_p()
+        rc = L.chorus_batch_init(n, meshes, params, C.byref(self._b))
+        self._keep = None
+        if rc != 0:
+            msg = L.chorus_batch_last_error(self._b).decode()
+            L.chorus_batch_free(self._b)
+            self._b = None
+            raise ChorusError("chorus_batch_init: %s (%d)" % (msg, rc))
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise ChorusError("%s: %s (%d)" % (what, _lib.load().chorus_batch_last_error(self._b).decode(), rc))
+
+    def close(self):
+        if getattr(self, "_b", None):
+            _lib.load().chorus_batch_free(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def shape(self, i):
+        m = self.members[i]
+        return (m.mesh.ncell, m.N, m.N, 9)
+
+    def set_state(self, i: int, Q: np.ndarray) -> None:
+        Q = np.ascontiguousarray(Q, dtype=np.float64).reshape(self.shape(i))
+        self._check(_lib.load().chorus_batch_set_state(self._b, i, _lib.ptr(Q)), "chorus_batch_set_state")
+
+    def get_state(self, i: int) -> np.ndarray:
+        Q = np.empty(self.shape(i))
+        self._check(_lib.load().chorus_batch_get_state(self._b, i, _lib.ptr(Q)), "chorus_batch_get_state")
+        return Q
+
+    def compresid(self, i: int) -> np.ndarray:
+        R = np.empty(self.shape(i))
+        self._check(_lib.load().chorus_batch_compresid(self._b, i, _lib.ptr(R)), "chorus_batch_compresid")
+        return R
+
+    def step(self, nsteps: int = 1) -> None:
+        self._check(_lib.load().chorus_batch_step(self._b, nsteps), "chorus_batch_step")
+
+    def divb(self) -> np.ndarray:
+        out = np.empty(len(self.members))
+        self._check(_lib.load().chorus_batch_divb(self._b, _lib.ptr(out)), "chorus_batch_divb")
+        return out
+
+    def groups(self):
+        L = _lib.load()
+        res = []
+        for g in range(L.chorus_batch_ngroups(self._b)):
+            v = [C.c_int() for _ in range(4)]
+            self._check(L.chorus_batch_group(self._b, g, *[C.byref(x) for x in v]), "chorus_batch_group")
+            res.append(dict(zip(["N", "ninst", "ncell", "tile"], [x.value for x in v])))
+        return res
+
+    def autotune(self):
+        tiles = np.zeros(max(1, _lib.load().chorus_batch_ngroups(self._b)), dtype=np.int32)
+        self._check(_lib.load().chorus_batch_autotune(self._b, _lib.ptr(tiles, _lib._ip)), "chorus_batch_autotune")
+        return tiles.tolist()
+
+    def time_steps(self, nsteps: int) -> float:
+        ms = C.c_double()
+        self._check(_lib.load().chorus_batch_time_steps(self._b, nsteps, C.byref(ms)), "chorus_batch_time_steps")
+        return ms.value

@@ -35,7 +35,11 @@ struct chorus_ctx {
     void *gsol = nullptr;          // viscous path: nablaQs at the solution points
     double *sm = nullptr;          // viscous path: AV indicator per cell
     double *mu4 = nullptr;         // viscous path: face-averaged muAV per cell and local face
-    double *eig = nullptr;         // two slots of the per-stage maximum wave speed
+    double *eig = nullptr;         // two slots (of ninst values) of the per-stage maximum wave speed
+    int ninst = 1;                 // instances sharing this context (batched groups)
+    int *inst = nullptr;           // batched groups: instance of each cell (device)
+    double *hinst = nullptr;       // batched groups: AV length scale per instance (device)
+    int tile = 0;                  // mixed form: tile shape of the cell kernels
     int stage_parity = 0;          // slot used by the next stage
     int last_parity = -1;          // slot written by the last stage
     double h_msh = 0.0;            // AV length scale (AV.f90:147)
@@ -91,6 +95,8 @@ FaceArgs face_args(chorus_ctx *c) {
     f.gtr = c->gtr;
     f.sm = c->sm;
     f.mu4 = c->mu4;
+    f.inst = c->inst;
+    f.hinst = c->hinst;
     f.artiv = c->p.artiv;
     f.gam = c->p.gam;
     f.mu = c->p.mu;
@@ -106,7 +112,7 @@ int launch_stage(chorus_ctx *c, const StageArgs &a) {
     if (t) (void)hipEventRecord(c->ev[c->ev_used], c->stream);
     FaceArgs f = face_args(c);
     if (c->ideal) {
-        if (c->p.precision == CHORUS_MIXED) e = launch_stage_ideal_mixed(c->N, a, f, c->stream);
+        if (c->p.precision == CHORUS_MIXED) e = launch_stage_ideal_mixed(c->N, a, f, c->stream, c->tile);
         else e = launch_stage_ideal_fp64(c->N, a, f, c->stream);
     } else {
         ViscArgs v{};
@@ -117,8 +123,9 @@ int launch_stage(chorus_ctx *c, const StageArgs &a) {
         v.gtr = c->gtr;
         v.gsol = c->gsol;
         v.sm = c->sm;
-        v.eig = c->eig + c->stage_parity;
-        v.eig_next = c->eig + (1 - c->stage_parity);
+        v.eig = c->eig + (size_t)c->stage_parity * c->ninst;
+        v.eig_next = c->eig + (size_t)(1 - c->stage_parity) * c->ninst;
+        v.ninst = c->ninst;
         v.TQ = c->TQ;
         v.mu4 = c->mu4;
         v.artiv = c->p.artiv;
@@ -131,7 +138,7 @@ int launch_stage(chorus_ctx *c, const StageArgs &a) {
         f.eig_now = v.eig;
         c->last_parity = c->stage_parity;
         c->stage_parity = 1 - c->stage_parity;
-        if (c->p.precision == CHORUS_MIXED) e = launch_stage_visc_mixed(c->N, a, v, f, c->stream);
+        if (c->p.precision == CHORUS_MIXED) e = launch_stage_visc_mixed(c->N, a, v, f, c->stream, c->tile);
         else e = launch_stage_visc_fp64(c->N, a, v, f, c->stream);
     }
     if (t) { (void)hipEventRecord(c->ev[c->ev_used + 1], c->stream); c->ev_used += 2; }
@@ -382,6 +389,8 @@ int build_topology(int ncell, int nvert, const int *ivcell, const double *xv, co
 
 }  // namespace
 
+int init_ctx(chorus_ctx *c, const chorus_mesh *m, const chorus_params *p);
+
 // ===========================================================================
 extern "C" {
 
@@ -415,6 +424,13 @@ int chorus_init(const chorus_mesh *m, const chorus_params *p, chorus_ctx **out) 
     *out = nullptr;
     chorus_ctx *c = new chorus_ctx();
     *out = c;
+    return init_ctx(c, m, p);
+}
+
+}  // extern "C"
+
+// set-up shared by chorus_init and the batched groups (c->ninst set before)
+int init_ctx(chorus_ctx *c, const chorus_mesh *m, const chorus_params *p) {
     if (!m || !p) return fail(c, CHORUS_ERR_ARG, "null mesh or params");
     if (m->N < 2 || m->N > kMaxN) return fail(c, CHORUS_ERR_ARG, "N must be in 2..8");
     if (m->ncell <= 0) return fail(c, CHORUS_ERR_ARG, "empty mesh");
@@ -501,8 +517,8 @@ int chorus_init(const chorus_mesh *m, const chorus_params *p, chorus_ctx **out) 
     HIP_TRY(c, hipMalloc(&c->R, ns * sizeof(double)));
     c->nblk = stage_blocks(c->N, c->ncell);
     HIP_TRY(c, hipMalloc(&c->partial, (size_t)c->nblk * 2 * sizeof(double)));
-    HIP_TRY(c, hipMalloc(&c->eig, 2 * sizeof(double)));
-    HIP_TRY(c, hipMemsetAsync(c->eig, 0, 2 * sizeof(double), c->stream));
+    HIP_TRY(c, hipMalloc(&c->eig, 2 * (size_t)c->ninst * sizeof(double)));
+    HIP_TRY(c, hipMemsetAsync(c->eig, 0, 2 * (size_t)c->ninst * sizeof(double), c->stream));
     c->nface = (int)faces.size();
     const size_t nfp = (size_t)c->ncell * 4 * c->N;
     HIP_TRY(c, hipMalloc(&c->faces, faces.size() * sizeof(int4)));
@@ -523,6 +539,8 @@ int chorus_init(const chorus_mesh *m, const chorus_params *p, chorus_ctx **out) 
     return 0;
 }
 
+extern "C" {
+
 void chorus_free(chorus_ctx *c) {
     if (!c) return;
     for (auto e : c->ev) (void)hipEventDestroy(e);
@@ -540,6 +558,8 @@ void chorus_free(chorus_ctx *c) {
     (void)hipFree(c->gsol);
     (void)hipFree(c->sm);
     (void)hipFree(c->mu4);
+    (void)hipFree(c->inst);
+    (void)hipFree(c->hinst);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -691,6 +711,390 @@ int chorus_time_steps(chorus_ctx *c, int nsteps, double *ms_total, double *ms_st
     if (ms_stage_avg) *ms_stage_avg = nst ? st / nst : 0.0;
     if (ms_ct_avg) *ms_ct_avg = nct ? ct / nct : 0.0;
     return 0;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Batched instances: groups of instances with equal numerics share one
+// context whose mesh is the disjoint union of their meshes; a per-cell
+// instance index selects the per-instance AV quantities.  One step of the
+// batch is one hipGraph launch whose branches are the groups.
+// ===========================================================================
+#include <array>
+#include <map>
+
+struct chorus_batch {
+    std::string err;
+    int ninst = 0;
+    struct Member {
+        int group, off, ncell, N;
+    };
+    std::vector<Member> mem;
+    std::vector<chorus_ctx *> groups;
+    std::vector<hipStream_t> gstream;   // capture branch of each group
+    hipStream_t stream = nullptr;       // capture and launch stream
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    struct Graph {
+        hipGraphExec_t exec = nullptr;
+        std::vector<std::array<int, 4>> post;   // per group: cur, last_in, stage_parity, last_parity
+    };
+    std::map<std::vector<int>, Graph> graphs;
+    bool stale = true;                  // traces / AV indicators not yet evaluated
+};
+
+namespace {
+
+int bfail(chorus_batch *b, int code, const std::string &msg) {
+    if (b) b->err = msg;
+    return code;
+}
+
+#define HIP_TRYB(b, expr)                                                                    \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return bfail(b, CHORUS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+bool same_numerics(const chorus_params &a, const chorus_params &b, int Na, int Nb) {
+    return Na == Nb && a.precision == b.precision && a.k_stage == b.k_stage && a.vismode == b.vismode &&
+           a.artiv == b.artiv && a.mu == b.mu && a.eta0 == b.eta0 && a.gam == b.gam && a.dt == b.dt;
+}
+
+void drop_graphs(chorus_batch *b) {
+    for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second.exec);
+    b->graphs.clear();
+}
+
+int refresh(chorus_batch *b) {
+    if (!b->stale) return 0;
+    for (chorus_ctx *c : b->groups) {
+        int rc = run_trace(c, c->X[c->cur]);
+        if (rc) return bfail(b, rc, c->err);
+    }
+    for (chorus_ctx *c : b->groups) HIP_TRYB(b, hipStreamSynchronize(c->stream));
+    b->stale = false;
+    return 0;
+}
+
+int batch_one_step(chorus_batch *b) {
+    std::vector<int> key;
+    for (chorus_ctx *c : b->groups) { key.push_back(c->cur); key.push_back(c->stage_parity); }
+    auto it = b->graphs.find(key);
+    if (it == b->graphs.end()) {
+        chorus_batch::Graph g;
+        HIP_TRYB(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        (void)hipEventRecord(b->fork, b->stream);
+        int rc = 0;
+        for (size_t k = 0; k < b->groups.size() && !rc; k++) {
+            chorus_ctx *c = b->groups[k];
+            (void)hipStreamWaitEvent(b->gstream[k], b->fork, 0);
+            hipStream_t own = c->stream;
+            c->stream = b->gstream[k];
+            rc = one_step(c);
+            c->stream = own;
+            if (rc) b->err = c->err;
+            (void)hipEventRecord(b->join[k], b->gstream[k]);
+            (void)hipStreamWaitEvent(b->stream, b->join[k], 0);
+        }
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(b->stream, &graph);
+        if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+        if (e != hipSuccess) return bfail(b, CHORUS_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return bfail(b, CHORUS_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+        for (chorus_ctx *c : b->groups) g.post.push_back({c->cur, c->last_in, c->stage_parity, c->last_parity});
+        // the capture advanced the host-side bookkeeping; rewind it to replay below
+        for (size_t k = 0; k < b->groups.size(); k++) {
+            b->groups[k]->cur = key[2 * k];
+            b->groups[k]->stage_parity = key[2 * k + 1];
+        }
+        it = b->graphs.emplace(key, g).first;
+    }
+    HIP_TRYB(b, hipGraphLaunch(it->second.exec, b->stream));
+    for (size_t k = 0; k < b->groups.size(); k++) {
+        chorus_ctx *c = b->groups[k];
+        const auto &p = it->second.post[k];
+        c->cur = p[0]; c->last_in = p[1]; c->stage_parity = p[2]; c->last_parity = p[3];
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *chorus_batch_last_error(const chorus_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+void chorus_batch_free(chorus_batch *b) {
+    if (!b) return;
+    drop_graphs(b);
+    for (chorus_ctx *c : b->groups) chorus_free(c);
+    for (auto s : b->gstream) (void)hipStreamDestroy(s);
+    for (auto e : b->join) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int chorus_batch_init(int ninst, const chorus_mesh *meshes, const chorus_params *params, chorus_batch **out) {
+    if (!out) return CHORUS_ERR_ARG;
+    *out = nullptr;
+    chorus_batch *b = new chorus_batch();
+    *out = b;
+    if (ninst < 1 || !meshes || !params) return bfail(b, CHORUS_ERR_ARG, "empty batch");
+    b->ninst = ninst;
+    b->mem.resize(ninst);
+    // group assignment in order of first appearance
+    std::vector<std::vector<int>> members;
+    for (int i = 0; i < ninst; i++) {
+        if (params[i].k_stage != params[0].k_stage)
+            return bfail(b, CHORUS_ERR_ARG, "all instances of a batch must use the same k_stage");
+        int g = -1;
+        for (size_t k = 0; k < members.size() && g < 0; k++) {
+            const int j = members[k][0];
+            if (same_numerics(params[i], params[j], meshes[i].N, meshes[j].N)) g = (int)k;
+        }
+        if (g < 0) { g = (int)members.size(); members.emplace_back(); }
+        members[g].push_back(i);
+    }
+    for (size_t g = 0; g < members.size(); g++) {
+        // disjoint union of the members' meshes (reference layouts, offsets applied)
+        int NC = 0, NV_ = 0, NF = 0, NI = 0, NPr = 0, NFr = 0;
+        for (int i : members[g]) {
+            const chorus_mesh &m = meshes[i];
+            if (!m.xv || !m.yv || !m.ivcell || !m.if2c || m.ncell <= 0)
+                return bfail(b, CHORUS_ERR_ARG, "instance " + std::to_string(i) + ": incomplete mesh");
+            NC += m.ncell; NV_ += m.nvert; NF += m.nface; NI += m.ninter; NPr += m.npair; NFr += m.nfree;
+        }
+        std::vector<double> xv(NV_), yv(NV_);
+        std::vector<int> ivc((size_t)NC * 4), if2c((size_t)NF * 4), ibi(NI), cyc((size_t)NPr * 2), ibf(NFr), inst(NC);
+        std::vector<double> hinst;
+        int coff = 0, voff = 0, foff = 0, ioff = 0, poff = 0, froff = 0, li = 0;
+        for (int i : members[g]) {
+            const chorus_mesh &m = meshes[i];
+            const chorus_params &p = params[i];
+            b->mem[i] = {(int)g, coff, m.ncell, m.N};
+            for (int v = 0; v < m.nvert; v++) { xv[voff + v] = m.xv[v]; yv[voff + v] = m.yv[v]; }
+            for (int k = 0; k < 4; k++)
+                for (int c = 0; c < m.ncell; c++) ivc[(size_t)k * NC + coff + c] = m.ivcell[(size_t)k * m.ncell + c] + voff;
+            for (int k = 0; k < 4; k++)
+                for (int f = 0; f < m.nface; f++) {
+                    int v = m.if2c[(size_t)k * m.nface + f];
+                    if (k < 2 && v != 0) v += coff;
+                    if2c[(size_t)k * NF + foff + f] = v;
+                }
+            for (int k = 0; k < m.ninter; k++) ibi[ioff + k] = m.ibfinter[k] + foff;
+            for (int k = 0; k < 2; k++)
+                for (int q = 0; q < m.npair; q++) cyc[(size_t)k * NPr + poff + q] = m.cyc2fpair[(size_t)k * m.npair + q] + foff;
+            for (int k = 0; k < m.nfree; k++) ibf[froff + k] = m.ibfree[k] + foff;
+            for (int c = 0; c < m.ncell; c++) inst[coff + c] = li;
+            const int ncg = p.ncellglob > 0 ? p.ncellglob : m.ncell;
+            hinst.push_back(std::sqrt(p.dxcycl * p.dycycl / (double)ncg));   // AV.f90:147
+            coff += m.ncell; voff += m.nvert; foff += m.nface; ioff += m.ninter; poff += m.npair; froff += m.nfree;
+            li++;
+        }
+        chorus_mesh mm{};
+        mm.ncell = NC; mm.nvert = NV_; mm.N = meshes[members[g][0]].N;
+        mm.xv = xv.data(); mm.yv = yv.data(); mm.ivcell = ivc.data();
+        mm.nface = NF; mm.if2c = if2c.data();
+        mm.ninter = NI; mm.ibfinter = ibi.data();
+        mm.npair = NPr; mm.cyc2fpair = cyc.data();
+        mm.nfree = NFr; mm.ibfree = ibf.data();
+        chorus_params pp = params[members[g][0]];
+        pp.ncellglob = NC;
+        chorus_ctx *c = new chorus_ctx();
+        c->ninst = (int)members[g].size();
+        b->groups.push_back(c);
+        int rc = init_ctx(c, &mm, &pp);
+        if (rc) return bfail(b, rc, "group " + std::to_string(g) + ": " + c->err);
+        if (c->ninst > 1) {
+            HIP_TRYB(b, hipMalloc(&c->inst, inst.size() * sizeof(int)));
+            HIP_TRYB(b, hipMalloc(&c->hinst, hinst.size() * sizeof(double)));
+            HIP_TRYB(b, hipMemcpy(c->inst, inst.data(), inst.size() * sizeof(int), hipMemcpyHostToDevice));
+            HIP_TRYB(b, hipMemcpy(c->hinst, hinst.data(), hinst.size() * sizeof(double), hipMemcpyHostToDevice));
+        } else {
+            c->h_msh = hinst[0];
+        }
+        hipStream_t s;
+        HIP_TRYB(b, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        b->gstream.push_back(s);
+        hipEvent_t e;
+        HIP_TRYB(b, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        b->join.push_back(e);
+    }
+    HIP_TRYB(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    HIP_TRYB(b, hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    return 0;
+}
+
+int chorus_batch_set_state(chorus_batch *b, int i, const double *q) {
+    if (!b || !q || i < 0 || i >= b->ninst) return bfail(b, CHORUS_ERR_ARG, "bad instance");
+    const chorus_batch::Member &m = b->mem[i];
+    chorus_ctx *c = b->groups[m.group];
+    const int NP = m.N * m.N;
+    std::vector<double> h((size_t)m.ncell * 9 * NP);
+    for (int ic = 0; ic < m.ncell; ic++)
+        for (int p = 0; p < NP; p++)
+            for (int v = 0; v < 9; v++) h[((size_t)ic * 9 + v) * NP + p] = q[((size_t)ic * NP + p) * 9 + v];
+    HIP_TRYB(b, hipMemcpy(c->X[c->cur] + (size_t)m.off * 9 * NP, h.data(), h.size() * sizeof(double),
+                          hipMemcpyHostToDevice));
+    c->last_in = -1;
+    b->stale = true;
+    return 0;
+}
+
+int chorus_batch_get_state(chorus_batch *b, int i, double *q) {
+    if (!b || !q || i < 0 || i >= b->ninst) return bfail(b, CHORUS_ERR_ARG, "bad instance");
+    const chorus_batch::Member &m = b->mem[i];
+    chorus_ctx *c = b->groups[m.group];
+    const int NP = m.N * m.N;
+    HIP_TRYB(b, hipStreamSynchronize(b->stream));
+    std::vector<double> h((size_t)m.ncell * 9 * NP);
+    HIP_TRYB(b, hipMemcpy(h.data(), c->X[c->cur] + (size_t)m.off * 9 * NP, h.size() * sizeof(double),
+                          hipMemcpyDeviceToHost));
+    for (int ic = 0; ic < m.ncell; ic++)
+        for (int p = 0; p < NP; p++)
+            for (int v = 0; v < 9; v++) q[((size_t)ic * NP + p) * 9 + v] = h[((size_t)ic * 9 + v) * NP + p];
+    return 0;
+}
+
+int chorus_batch_step(chorus_batch *b, int nsteps) {
+    if (!b || nsteps < 0) return bfail(b, CHORUS_ERR_ARG, "bad arguments");
+    int rc = refresh(b);
+    for (int k = 0; k < nsteps && !rc; k++) rc = batch_one_step(b);
+    if (rc) return rc;
+    HIP_TRYB(b, hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int chorus_batch_time_steps(chorus_batch *b, int nsteps, double *ms_total) {
+    if (!b || nsteps < 1) return bfail(b, CHORUS_ERR_ARG, "bad arguments");
+    int rc = refresh(b);
+    if (rc) return rc;
+    hipEvent_t t0, t1;
+    HIP_TRYB(b, hipEventCreate(&t0));
+    HIP_TRYB(b, hipEventCreate(&t1));
+    HIP_TRYB(b, hipEventRecord(t0, b->stream));
+    for (int k = 0; k < nsteps && !rc; k++) rc = batch_one_step(b);
+    (void)hipEventRecord(t1, b->stream);
+    (void)hipEventSynchronize(t1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (rc) return rc;
+    if (ms_total) *ms_total = ms;
+    return 0;
+}
+
+int chorus_batch_compresid(chorus_batch *b, int i, double *resid) {
+    if (!b || i < 0 || i >= b->ninst) return bfail(b, CHORUS_ERR_ARG, "bad instance");
+    int rc = refresh(b);
+    if (rc) return rc;
+    const chorus_batch::Member &m = b->mem[i];
+    chorus_ctx *c = b->groups[m.group];
+    HIP_TRYB(b, hipStreamSynchronize(b->stream));
+    StageArgs s = base_args(c);
+    s.Qin = c->X[c->cur];
+    s.Rout = c->R;
+    s.kind = KIND_RESID;
+    rc = launch_stage(c, s);
+    if (rc) return bfail(b, rc, c->err);
+    HIP_TRYB(b, hipStreamSynchronize(c->stream));
+    if (resid) {
+        const int NP = m.N * m.N;
+        std::vector<double> h((size_t)m.ncell * 9 * NP);
+        HIP_TRYB(b, hipMemcpy(h.data(), c->R + (size_t)m.off * 9 * NP, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int ic = 0; ic < m.ncell; ic++)
+            for (int p = 0; p < NP; p++)
+                for (int v = 0; v < 9; v++) resid[((size_t)ic * NP + p) * 9 + v] = h[((size_t)ic * 9 + v) * NP + p];
+    }
+    return 0;
+}
+
+int chorus_batch_divb(chorus_batch *b, double *divbmax) {
+    if (!b || !divbmax) return bfail(b, CHORUS_ERR_ARG, "bad arguments");
+    int rc = refresh(b);
+    if (rc) return rc;
+    HIP_TRYB(b, hipStreamSynchronize(b->stream));
+    for (int i = 0; i < b->ninst; i++) {
+        const chorus_batch::Member &m = b->mem[i];
+        chorus_ctx *c = b->groups[m.group];
+        CTArgs a{};
+        a.c_first = m.off;
+        a.ncell = m.off + m.ncell;
+        a.cellxy = c->cellxy;
+        a.nbr = c->nbr;
+        a.Q = c->X[c->cur];
+        a.which = 1;
+        a.partial = c->partial;
+        a.TQ = c->TQ;
+        a.TQout = c->TQ;
+        HIP_TRYB(b, launch_ct(m.N, a, c->stream));
+        const int nb = ct_blocks(m.N, m.ncell);
+        std::vector<double> part((size_t)nb * 2);
+        HIP_TRYB(b, hipMemcpyAsync(part.data(), c->partial, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRYB(b, hipStreamSynchronize(c->stream));
+        double mx = 0.0;
+        for (int k = 0; k < nb; k++) mx = std::max(mx, part[2 * k + 1]);
+        divbmax[i] = mx;
+    }
+    return 0;
+}
+
+int chorus_batch_ngroups(const chorus_batch *b) { return b ? (int)b->groups.size() : 0; }
+
+int chorus_batch_group(const chorus_batch *b, int g, int *N, int *ninst, int *ncell, int *tile) {
+    if (!b || g < 0 || g >= (int)b->groups.size()) return CHORUS_ERR_ARG;
+    const chorus_ctx *c = b->groups[g];
+    if (N) *N = c->N;
+    if (ninst) *ninst = c->ninst;
+    if (ncell) *ncell = c->ncell;
+    if (tile) *tile = c->tile;
+    return 0;
+}
+
+int chorus_batch_autotune(chorus_batch *b, int *tiles) {
+    if (!b) return CHORUS_ERR_ARG;
+    int rc = refresh(b);
+    if (rc) return rc;
+    HIP_TRYB(b, hipStreamSynchronize(b->stream));
+    drop_graphs(b);
+    hipEvent_t t0, t1;
+    HIP_TRYB(b, hipEventCreate(&t0));
+    HIP_TRYB(b, hipEventCreate(&t1));
+    for (size_t g = 0; g < b->groups.size(); g++) {
+        chorus_ctx *c = b->groups[g];
+        int best = 0;
+        if (c->p.precision == CHORUS_MIXED) {
+            float best_ms = 1e30f;
+            for (int t = 0; t < tile_count(); t++) {
+                c->tile = t;
+                StageArgs s = base_args(c);
+                s.Qin = c->X[c->cur];
+                s.Rout = c->R;
+                s.kind = KIND_RESID;
+                if ((rc = launch_stage(c, s))) break;   // warm-up
+                (void)hipEventRecord(t0, c->stream);
+                for (int r = 0; r < 3 && !rc; r++) rc = launch_stage(c, s);
+                (void)hipEventRecord(t1, c->stream);
+                (void)hipEventSynchronize(t1);
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, t0, t1);
+                if (!rc && ms < best_ms) { best_ms = ms; best = t; }
+            }
+        }
+        c->tile = best;
+        if (tiles) tiles[g] = best;
+        if (rc) break;
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    return rc ? bfail(b, rc, "autotune failed") : 0;
 }
 
 }  // extern "C"

@@ -46,9 +46,9 @@ hipError_t launch_trace(int N, int ncell, const double *Q, double *TQ, double *s
 hipError_t launch_ct(int N, const CTArgs &a, hipStream_t s) {
 #define L_(n)                                                                                            \
     if (a.which == 0)                                                                                    \
-        hipLaunchKernelGGL((k_ct<n, 0>), dim3(nblocks<n>(a.ncell)), dim3(Shape<n>::THREADS), 0, s, a); \
+        hipLaunchKernelGGL((k_ct<n, 0>), dim3(nblocks<n>(a.ncell - a.c_first)), dim3(Shape<n>::THREADS), 0, s, a); \
     else                                                                                                 \
-        hipLaunchKernelGGL((k_ct<n, 1>), dim3(nblocks<n>(a.ncell)), dim3(Shape<n>::THREADS), 0, s, a); \
+        hipLaunchKernelGGL((k_ct<n, 1>), dim3(nblocks<n>(a.ncell - a.c_first)), dim3(Shape<n>::THREADS), 0, s, a); \
     return hipGetLastError();
     CHORUS_DISPATCH_N(N, L_)
 #undef L_

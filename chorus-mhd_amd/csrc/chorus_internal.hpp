@@ -82,8 +82,10 @@ struct FaceArgs {
     const void *gtr;
     const double *sm;        // AV indicator per cell
     double *mu4;             // k_face_visc: face-averaged muAV(1) per cell and local face
-    double *eig;             // k_face_inv: atomic max of the Rusanov wave speed (AV)
-    const double *eig_now;   // k_face_visc: this stage's maximum (epsilon0)
+    double *eig;             // k_face_inv: atomic max of the Rusanov wave speed (AV), per instance
+    const double *eig_now;   // k_face_visc: this stage's maximum (epsilon0), per instance
+    const int *inst;         // batched instances: instance of each cell (null: one instance)
+    const double *hinst;     // batched instances: AV length scale per instance
     int artiv;
     double gam, mu, eta0, h_msh, pi;
 };
@@ -98,7 +100,8 @@ struct ViscArgs {
     void *gsol;         // nablaQs at the solution points [cell][16][N*N] (Real)
     double *sm;         // log10 smoothness indicator of density per cell
     double *eig;        // this stage's maximum Rusanov wave speed (atomic max)
-    double *eig_next;   // the next stage's slot, zeroed by k_stage_visc
+    double *eig_next;   // the next stage's slots (one per instance), zeroed by k_stage_visc
+    int ninst;          // instances sharing the launch
     const double *TQ;   // traces of Qin at the face points
     const double *mu4;  // face-averaged muAV(1) per cell and local face (k_face_visc)
     int artiv;
@@ -108,7 +111,8 @@ struct ViscArgs {
 };
 
 struct CTArgs {
-    int ncell;
+    int ncell;          // one past the last cell processed
+    int c_first;        // first cell processed (block b starts at c_first + b*CPB)
     const double *cellxy;
     const int2 *nbr;
     double *Q;
@@ -153,10 +157,12 @@ __device__ __forceinline__ void prefetch_qa(const StageArgs &a, size_t base, int
 // launchers (defined in the kernel translation units); return hipError_t
 hipError_t launch_stage_ideal_fp64(int N, const StageArgs &a, const FaceArgs &f, hipStream_t s);
 hipError_t launch_trace(int N, int ncell, const double *Q, double *TQ, double *sm, hipStream_t s);
-hipError_t launch_stage_ideal_mixed(int N, const StageArgs &a, const FaceArgs &f, hipStream_t s);
+hipError_t launch_stage_ideal_mixed(int N, const StageArgs &a, const FaceArgs &f, hipStream_t s, int tile);
+int tile_count();
 hipError_t launch_ct(int N, const CTArgs &a, hipStream_t s);
 hipError_t launch_stage_visc_fp64(int N, const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s);
-hipError_t launch_stage_visc_mixed(int N, const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s);
+hipError_t launch_stage_visc_mixed(int N, const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s,
+                                   int tile);
 int ct_blocks(int N, int ncell);
 int stage_blocks(int N, int ncell);
 hipError_t upload_operators_fp64();

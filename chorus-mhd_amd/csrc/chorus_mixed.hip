@@ -13,48 +13,81 @@ hipError_t upload_operators_mixed() {
 template <class S> static int blocks_of(int ncell) { return (ncell + S::CPB - 1) / S::CPB; }
 static int fblocks(int N, int nface) { return (nface * N + 255) / 256; }
 
-hipError_t launch_stage_ideal_mixed(int N, const StageArgs &a, const FaceArgs &f, hipStream_t s) {
-#define L_(n)                                                                                     \
-    case n:                                                                                       \
-        hipLaunchKernelGGL((k_face_inv_mx<n>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);    \
-        hipLaunchKernelGGL((k_stage_ideal_mx<n>), dim3(blocks_of<IShape<n>>(a.ncell)), dim3(IShape<n>::THREADS), 0, s, a); \
-        return hipGetLastError();
-    switch (N) {
-        L_(2) L_(3) L_(4) L_(5) L_(6) L_(7) L_(8)
-    default: return hipErrorInvalidValue;
-    }
-#undef L_
-}
+// tile shapes of the mixed cell kernels (threads-per-block targets of the
+// ideal and viscous kernels), chosen per order by the autotuner:
+//   tile 0: 256 / 128 (default), tile 1: 128 / 64, tile 2: 512 / 256
+int tile_count() { return 3; }
 
-template <int n, int VM>
-static hipError_t visc_stage(const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s) {
+template <int n, int TBI>
+static hipError_t ideal_stage(const StageArgs &a, const FaceArgs &f, hipStream_t s) {
+    using S = CellShape<n, TBI>;
     hipLaunchKernelGGL((k_face_inv_mx<n>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);
-    hipLaunchKernelGGL((k_grad_mx<n>), dim3(blocks_of<VShape<n>>(a.ncell)), dim3(V2Shape<n>::THREADS), 0, s, v);
-    hipLaunchKernelGGL((k_face_visc_mx<n, VM>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);
-    hipLaunchKernelGGL((k_stage_visc_mx<n, VM>), dim3(blocks_of<VShape<n>>(a.ncell)), dim3(MxVShape<n>::THREADS), 0, s, a, v);
+    hipLaunchKernelGGL((k_stage_ideal_mx<n, TBI>), dim3(blocks_of<S>(a.ncell)), dim3(S::THREADS), 0, s, a);
     return hipGetLastError();
 }
 
 template <int n>
-static hipError_t visc_stage_vm(const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s) {
-    const int vm = ((v.mu != 0.0 || v.eta0 != 0.0) ? 1 : 0) | (v.artiv ? 2 : 0);
-    switch (vm) {
-    case 1: return visc_stage<n, 1>(a, v, f, s);
-    case 2: return visc_stage<n, 2>(a, v, f, s);
-    case 3: return visc_stage<n, 3>(a, v, f, s);
+static hipError_t ideal_stage_tile(const StageArgs &a, const FaceArgs &f, hipStream_t s, int tile) {
+    switch (tile) {
+    case 1: return ideal_stage<n, 128>(a, f, s);
+    case 2: return ideal_stage<n, 512>(a, f, s);
+    default: return ideal_stage<n, 256>(a, f, s);
+    }
+}
+
+hipError_t launch_stage_ideal_mixed(int N, const StageArgs &a, const FaceArgs &f, hipStream_t s, int tile) {
+    switch (N) {
+    case 2: return ideal_stage_tile<2>(a, f, s, tile);
+    case 3: return ideal_stage_tile<3>(a, f, s, tile);
+    case 4: return ideal_stage_tile<4>(a, f, s, tile);
+    case 5: return ideal_stage_tile<5>(a, f, s, tile);
+    case 6: return ideal_stage_tile<6>(a, f, s, tile);
+    case 7: return ideal_stage_tile<7>(a, f, s, tile);
+    case 8: return ideal_stage_tile<8>(a, f, s, tile);
     default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_stage_visc_mixed(int N, const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s) {
+template <int n, int TBV, int VM>
+static hipError_t visc_stage(const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s) {
+    using S = CellShape<n, TBV>;
+    hipLaunchKernelGGL((k_face_inv_mx<n>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);
+    hipLaunchKernelGGL((k_grad_mx<n, TBV>), dim3(blocks_of<S>(a.ncell)), dim3(V2ShapeT<n, TBV>::THREADS), 0, s, v);
+    hipLaunchKernelGGL((k_face_visc_mx<n, VM>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);
+    hipLaunchKernelGGL((k_stage_visc_mx<n, TBV, VM>), dim3(blocks_of<S>(a.ncell)), dim3(MxVShape<n, TBV>::THREADS), 0, s, a, v);
+    return hipGetLastError();
+}
+
+template <int n, int TBV>
+static hipError_t visc_stage_vm(const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s) {
+    const int vm = ((v.mu != 0.0 || v.eta0 != 0.0) ? 1 : 0) | (v.artiv ? 2 : 0);
+    switch (vm) {
+    case 1: return visc_stage<n, TBV, 1>(a, v, f, s);
+    case 2: return visc_stage<n, TBV, 2>(a, v, f, s);
+    case 3: return visc_stage<n, TBV, 3>(a, v, f, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int n>
+static hipError_t visc_stage_tile(const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s, int tile) {
+    switch (tile) {
+    case 1: return visc_stage_vm<n, 64>(a, v, f, s);
+    case 2: return visc_stage_vm<n, 256>(a, v, f, s);
+    default: return visc_stage_vm<n, 128>(a, v, f, s);
+    }
+}
+
+hipError_t launch_stage_visc_mixed(int N, const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s,
+                                   int tile) {
     switch (N) {
-    case 2: return visc_stage_vm<2>(a, v, f, s);
-    case 3: return visc_stage_vm<3>(a, v, f, s);
-    case 4: return visc_stage_vm<4>(a, v, f, s);
-    case 5: return visc_stage_vm<5>(a, v, f, s);
-    case 6: return visc_stage_vm<6>(a, v, f, s);
-    case 7: return visc_stage_vm<7>(a, v, f, s);
-    case 8: return visc_stage_vm<8>(a, v, f, s);
+    case 2: return visc_stage_tile<2>(a, v, f, s, tile);
+    case 3: return visc_stage_tile<3>(a, v, f, s, tile);
+    case 4: return visc_stage_tile<4>(a, v, f, s, tile);
+    case 5: return visc_stage_tile<5>(a, v, f, s, tile);
+    case 6: return visc_stage_tile<6>(a, v, f, s, tile);
+    case 7: return visc_stage_tile<7>(a, v, f, s, tile);
+    case 8: return visc_stage_tile<8>(a, v, f, s, tile);
     default: return hipErrorInvalidValue;
     }
 }

@@ -189,6 +189,20 @@ __device__ __forceinline__ void block_max_atomic(double v, double *red, double *
     }
 }
 
+// maximum per instance: one atomic per wave when the wave's face points belong
+// to one instance (the usual case: instances are contiguous cell ranges),
+// otherwise one per lane
+__device__ __forceinline__ void instance_max_atomic(double v, int inst, double *dst) {
+    const int first = __builtin_amdgcn_readfirstlane(inst);
+    if (__all(inst == first)) {
+        for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+        if ((threadIdx.x & 63) == 0)
+            atomicMax(reinterpret_cast<unsigned long long *>(dst + first), (unsigned long long)__double_as_longlong(v));
+    } else {
+        atomicMax(reinterpret_cast<unsigned long long *>(dst + inst), (unsigned long long)__double_as_longlong(v));
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Rusanov flux of every face point, once per face (getrusanov.f90:1-109 called
 // from interfaceflux_all.f90:16-114, cyclocflux_all.f90:15-114, bcflux.f90:19-73):
@@ -233,7 +247,15 @@ __global__ void __launch_bounds__(256) k_face_inv(FaceArgs a) {
         }
         emax = double(eig);
     }
-    if (a.artiv) block_max_atomic<R>(emax, red, a.eig);
+    if (a.artiv) {
+        if (a.inst) {
+            const int t0 = xcd_block(blockIdx.x, gridDim.x) * blockDim.x;   // first point of the block
+            const int tt = min(t0 + (int)threadIdx.x, a.nface * N - 1);
+            instance_max_atomic(emax, a.inst[a.faces[tt / N].x], a.eig);
+        } else {
+            block_max_atomic<R>(emax, red, a.eig);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -391,7 +413,7 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
     __shared__ double red[2][256];
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
-    const int c0 = blk * CPB;
+    const int c0 = a.c_first + blk * CPB;
     const int ncl = min(CPB, a.ncell - c0);
     for (int i = tid; i < ncl * NP; i += blockDim.x) {
         int c = i / NP, k = i % NP;

@@ -281,7 +281,15 @@ __global__ void __launch_bounds__(256) k_face_inv_mx(FaceArgs a) {
         }
         emax = double(eig);
     }
-    if (a.artiv) block_max_atomic<float>(emax, red, a.eig);
+    if (a.artiv) {
+        if (a.inst) {
+            const int t0 = xcd_block(blockIdx.x, gridDim.x) * blockDim.x;
+            const int tt = min(t0 + (int)threadIdx.x, a.nface * N - 1);
+            instance_max_atomic(emax, a.inst[a.faces[tt / N].x], a.eig);
+        } else {
+            block_max_atomic<float>(emax, red, a.eig);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -291,9 +299,9 @@ __global__ void __launch_bounds__(256) k_face_inv_mx(FaceArgs a) {
 // FP32 deviations and flux planes only (one union area, reused for the FP64
 // copy at set-up and for the new state at the end).
 // ---------------------------------------------------------------------------
-template <int N>
-__global__ void __launch_bounds__(IShape<N>::THREADS) k_stage_ideal_mx(StageArgs a) {
-    using S = IShape<N>;
+template <int N, int TB>
+__global__ void __launch_bounds__((CellShape<N, TB>::THREADS)) k_stage_ideal_mx(StageArgs a) {
+    using S = CellShape<N, TB>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ, T = S::THREADS;
     constexpr int FST = 11 * NF + 1;     // per-cell stride: dF1 (8) + dA_z + metric rows (2)
     constexpr int UF = CPB * NQ + CPB * FST;
@@ -391,10 +399,10 @@ __global__ void __launch_bounds__(IShape<N>::THREADS) k_stage_ideal_mx(StageArgs
 // plain FP32 (they carry no large constant part); the viscous face flux is
 // evaluated once per face point from the FP64 trace average.
 // ---------------------------------------------------------------------------
-template <int N>
-__global__ void __launch_bounds__(V2Shape<N>::THREADS) k_grad_mx(ViscArgs a) {
-    using S = VShape<N>;
-    using TS = V2Shape<N>;
+template <int N, int TB>
+__global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs a) {
+    using S = CellShape<N, TB>;
+    using TS = V2ShapeT<N, TB>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ, T = TS::THREADS, NIT = TS::NIT;
     constexpr int VST = 10 * NF + 1;
     constexpr int UF = CPB * NQ + CPB * VST + CPB * 16 * NP;
@@ -550,7 +558,8 @@ __global__ void __launch_bounds__(256) k_face_visc_mx(FaceArgs a) {
         for (int w = 0; w < NV; w++) { fl[w] = Fv[w] * sl1 + Gv[w] * sl2; fr[w] = Fv[w] * sr1 + Gv[w] * sr2; }
     }
     if constexpr ((VM & 2) != 0) {
-        const double eps0 = *a.eig_now * a.h_msh / (double)(N - 1);
+        const int in = a.inst ? a.inst[fc.x] : 0;
+        const double eps0 = a.eig_now[in] * (a.inst ? a.hinst[in] : a.h_msh) / (double)(N - 1);
         const double ml = mu_av(a.sm[fc.x], eps0, a.pi);
         const double mf = free_face ? ml : 0.5 * (ml + mu_av(a.sm[fc.z], eps0, a.pi));
         if (k == 0) {
@@ -576,8 +585,8 @@ __global__ void __launch_bounds__(256) k_face_visc_mx(FaceArgs a) {
 // task layout of the mixed viscous stage kernel: inviscid delta flux at every
 // flux point (face points also copy the viscous face flux), then the viscous
 // flux at the interior points; derivative groups as in V2Shape
-template <int N> struct MxVShape {
-    using S = VShape<N>;
+template <int N, int TB> struct MxVShape {
+    using S = CellShape<N, TB>;
     static constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF;
     static constexpr int NIT = N * (N - 1);
     static constexpr int NA = CPB * NF;                      // inviscid tasks (all flux points)
@@ -592,10 +601,10 @@ template <int N> struct MxVShape {
 #ifndef CHORUS_MX_VISC_MINW
 #define CHORUS_MX_VISC_MINW 3
 #endif
-template <int N, int VM>
-__global__ void __launch_bounds__(MxVShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(CHORUS_MX_VISC_MINW))) k_stage_visc_mx(StageArgs a, ViscArgs v) {
-    using S = VShape<N>;
-    using TS = MxVShape<N>;
+template <int N, int TB, int VM>
+__global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amdgpu_waves_per_eu(CHORUS_MX_VISC_MINW))) k_stage_visc_mx(StageArgs a, ViscArgs v) {
+    using S = CellShape<N, TB>;
+    using TS = MxVShape<N, TB>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ, T = TS::THREADS, NIT = TS::NIT;
     constexpr int UST = 20 * NF + 1;     // dF1 (8) + Fv1 (9) + dA_z + metric rows (2)
     constexpr int UF = CPB * NQ + CPB * UST + CPB * 16 * NP;
@@ -615,7 +624,8 @@ __global__ void __launch_bounds__(MxVShape<N>::THREADS) __attribute__((amdgpu_wa
     for (int i = tid; i < ncl * 8; i += T) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
     if constexpr ((VM & 2) != 0) {
         for (int i = tid; i < ncl * 4; i += T) sMu[i] = v.mu4[(size_t)c0 * 4 + i];
-        if (blockIdx.x == 0 && tid == 0) *v.eig_next = 0.0;
+        if (blockIdx.x == 0)
+            for (int i = tid; i < v.ninst; i += T) v.eig_next[i] = 0.0;
     }
     stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, U, sBar, sRef, sD, nullptr);
     for (int i = tid; i < ncl * 16 * NP; i += T) sG[i] = v.gsol_r<float>()[(size_t)c0 * 16 * NP + i];

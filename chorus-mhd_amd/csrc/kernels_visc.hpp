@@ -52,8 +52,8 @@ template <int N> using VShape = CellShape<N, CHORUS_VISC_TB>;
 // interior or face point; the inviscid and viscous segments start on wave
 // boundaries and the face points fill the gaps), in the derivative phase group
 // 0 sums the inviscid and A_z terms and group 1 the viscous ones.
-template <int N> struct V2Shape {
-    using S = VShape<N>;
+template <int N, int TB> struct V2ShapeT {
+    using S = CellShape<N, TB>;
     static constexpr int CPB = S::CPB, NP = S::NP;
     static constexpr int NIT = N * (N - 1);                  // interior points per family and cell
     static constexpr int NI = CPB * NIT;                     // inviscid (and viscous) tasks
@@ -65,6 +65,8 @@ template <int N> struct V2Shape {
     static constexpr int CT = G1 + CPB * NP;
     static constexpr int THREADS = (((BT > CT ? BT : CT) + 63) / 64) * 64;
 };
+template <int N> using V2Shape = V2ShapeT<N, CHORUS_VISC_TB>;
+
 
 // local-face index (0..3) and own face-point number of flux point (fam, r, line)
 template <int N> __device__ __forceinline__ void face_of(int fam, int r, int line, int &lfi, int &nfp) {
@@ -375,7 +377,8 @@ __global__ void __launch_bounds__(256) k_face_visc(FaceArgs a) {
         for (int w = 0; w < NV; w++) { fl[w] = R(0); fr[w] = R(0); }
     }
     if constexpr ((VM & 2) != 0) {
-        const double eps0 = *a.eig_now * a.h_msh / (double)(N - 1) * 1.0;
+        const int in = a.inst ? a.inst[fc.x] : 0;
+        const double eps0 = a.eig_now[in] * (a.inst ? a.hinst[in] : a.h_msh) / (double)(N - 1) * 1.0;
         const double ml = mu_av(a.sm[fc.x], eps0, a.pi);
         const double mf = free_face ? ml : 0.5 * (ml + mu_av(a.sm[fc.z], eps0, a.pi));
         if (k == 0) {
@@ -424,7 +427,8 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
         // face averages of muAV(1) (compAVflux/interfaceAVflux/cyclocAVflux,
         // AV.f90:199-303, 494-564), evaluated by k_face_visc
         for (int i = tid; i < ncl * 4; i += blockDim.x) sMu[i] = v.mu4[(size_t)c0 * 4 + i];
-        if (blockIdx.x == 0 && tid == 0) *v.eig_next = 0.0;
+        if (blockIdx.x == 0)
+            for (int i = tid; i < v.ninst; i += blockDim.x) v.eig_next[i] = 0.0;
     }
     __syncthreads();
     // derivative-phase role of this thread

@@ -113,6 +113,39 @@ int chorus_topology(int ncell, int nvert, const int *ivcell, const double *xv, c
 int chorus_time_steps(chorus_ctx *ctx, int nsteps, double *ms_total, double *ms_stage_avg,
                       double *ms_ct_avg);
 
+/* ---- Batched ensembles (no reference counterpart: the reference runs one
+ * mesh per MPI job; the loop of main.f90:44-50 / iterations.f90:18-70 is what
+ * each instance executes).  ninst independent instances, each with its own
+ * mesh, state and parameters.  Instances whose N, precision, k_stage, vismode,
+ * artiv, mu, eta0, gam and dt agree share one group: their cells are
+ * concatenated, so every launch of a stage covers the whole group, with
+ * per-instance AV quantities (wave-speed maximum, epsilon0, length scale).
+ * All instances must use the same k_stage.  One time step of the whole batch
+ * is one hipGraph launch (captured on first use, one graph per buffer
+ * rotation). */
+typedef struct chorus_batch chorus_batch;
+
+int chorus_batch_init(int ninst, const chorus_mesh *meshes, const chorus_params *params, chorus_batch **out);
+void chorus_batch_free(chorus_batch *b);
+const char *chorus_batch_last_error(const chorus_batch *b);
+/* Q(9,N,N,NCELL) of instance i (reference layout), as chorus_set_state / chorus_get_state */
+int chorus_batch_set_state(chorus_batch *b, int i, const double *q);
+int chorus_batch_get_state(chorus_batch *b, int i, double *q);
+int chorus_batch_step(chorus_batch *b, int nsteps);
+/* residual of instance i's current state (compresid.f90:1-67), as chorus_compresid */
+int chorus_batch_compresid(chorus_batch *b, int i, double *resid);
+/* max |div B| per instance (iterations.f90:35-48), divbmax[ninst] */
+int chorus_batch_divb(chorus_batch *b, double *divbmax);
+/* groups: ngroups, and for group g its N, member count, cells and tile shape */
+int chorus_batch_ngroups(const chorus_batch *b);
+int chorus_batch_group(const chorus_batch *b, int g, int *N, int *ninst, int *ncell, int *tile);
+/* Tile-shape autotuner of the mixed-form cell kernels: per group, times every
+ * compiled tile shape on the group's own cells (residual evaluations, no state
+ * change) and keeps the fastest; tiles[ngroups] receives the choice. */
+int chorus_batch_autotune(chorus_batch *b, int *tiles);
+/* Measurement hook: nsteps steps bracketed by HIP events on the batch stream. */
+int chorus_batch_time_steps(chorus_batch *b, int nsteps, double *ms_total);
+
 #ifdef __cplusplus
 }
 #endif

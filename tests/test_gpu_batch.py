@@ -1,0 +1,113 @@
+"""Batched ensembles (chorus_batch_*): every instance of a batch must evolve
+exactly as the same instance run alone through chorus_init/chorus_step, in
+both precision forms, through several hipGraph replays (the buffer rotation
+of SSP(5,3) cycles through six graphs), with per-instance AV quantities, and
+whatever tile shape the autotuner picks (the tile only decides which block
+evaluates a cell, never the arithmetic of a cell).
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from chorus_mhd import quad_grid
+from chorus_mhd import ic as ics
+from chorus_mhd.solver import Batch, Instance, Solver
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _members(precision, artiv):
+    """Uneven sizes and orders; two Orszag-Tang instances share (N, gamma, dt)
+    and therefore one group; the last instance has gamma 1.4 and a FREE
+    boundary (the Orszag-Tang fields on the unit square: the rotor's sub-cell
+    taper would make the reference's residual NaN on a mesh this coarse)."""
+    two_pi = 2 * math.pi
+    spec = [
+        dict(n=8, m=8, box=(0, two_pi, 0, two_pi), N=4, jit=0.10, seed=1, bc="CYCL", ic="ot", gam=5 / 3),
+        dict(n=10, m=6, box=(0, two_pi, 0, two_pi), N=4, jit=0.05, seed=2, bc="CYCL", ic="ot", gam=5 / 3),
+        dict(n=6, m=6, box=(0, two_pi, 0, two_pi), N=3, jit=0.12, seed=3, bc="CYCL", ic="ot", gam=5 / 3),
+        dict(n=8, m=8, box=(0, 1, 0, 1), N=5, jit=0.08, seed=4, bc="FREE", ic="ot", gam=1.4),
+    ]
+    out = []
+    for s in spec:
+        mesh = quad_grid(s["n"], s["m"], *s["box"], jitter=s["jit"], seed=s["seed"], bc=s["bc"])
+        dt = 2e-4 if s["gam"] > 1.5 else 2e-5
+        out.append((Instance(mesh, s["N"], dt, gam=s["gam"], artiv=artiv, precision=precision), s["ic"]))
+    return out
+
+
+def _initial(inst, kind):
+    s = Solver(inst.mesh, inst.N, inst.dt, gam=inst.gam, artiv=inst.artiv, precision=inst.precision)
+    q = ics.orszag_tang(s, inst.mesh, inst.N, inst.gam) if kind == "ot" else ics.rotor(s, inst.mesh, inst.N, inst.gam)
+    s.close()
+    return q
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("artiv", [0, 1])
+def test_batch_equals_single_instances(precision, artiv):
+    _say("batch", precision, "artiv", artiv)
+    mem = _members(precision, artiv)
+    q0 = [_initial(m, k) for m, k in mem]
+    b = Batch([m for m, _ in mem])
+    groups = b.groups()
+    assert sum(g["ninst"] for g in groups) == len(mem)
+    assert len(groups) == 3          # the two N=4 Orszag-Tang instances share a group
+    for i, q in enumerate(q0):
+        b.set_state(i, q)
+    for i, (m, _) in enumerate(mem):
+        assert np.array_equal(b.compresid(i), _single(m, q0[i], 0, resid=True)), "residual of instance %d" % i
+    b.step(7)
+    for i, (m, _) in enumerate(mem):
+        ref = _single(m, q0[i], 7)
+        got = b.get_state(i)
+        assert np.array_equal(got, ref), "instance %d differs (max %g)" % (i, np.abs(got - ref).max())
+    b.close()
+
+
+def _single(m, q0, nsteps, resid=False):
+    s = Solver(m.mesh, m.N, m.dt, gam=m.gam, artiv=m.artiv, precision=m.precision)
+    s.set_state(q0)
+    if resid:
+        r = s.compresid()
+        s.close()
+        return r
+    s.step(nsteps)
+    q = s.get_state()
+    s.close()
+    return q
+
+
+def test_batch_autotune_and_divb():
+    _say("batch autotune")
+    mem = _members("mixed", 1)
+    q0 = [_initial(m, k) for m, k in mem]
+    b = Batch([m for m, _ in mem])
+    for i, q in enumerate(q0):
+        b.set_state(i, q)
+    tiles = b.autotune()
+    assert len(tiles) == len(b.groups()) and all(0 <= t < 3 for t in tiles)
+    assert [g["tile"] for g in b.groups()] == tiles
+    b.step(3)
+    dv = b.divb()
+    for i, (m, _) in enumerate(mem):
+        s = Solver(m.mesh, m.N, m.dt, gam=m.gam, artiv=m.artiv, precision=m.precision)
+        s.set_state(q0[i])
+        s.step(3)
+        assert np.array_equal(b.get_state(i), s.get_state()), "instance %d after autotune" % i
+        assert dv[i] == s.diag()["divbmax"]
+        s.close()
+    b.close()
+
+
+def test_batch_rejects_mixed_steppers():
+    mesh = quad_grid(4, 4, 0, 1, 0, 1)
+    with pytest.raises(Exception):
+        Batch([Instance(mesh, 3, 1e-4, k_stage=3), Instance(mesh, 3, 1e-4, k_stage=5)])
