@@ -27,7 +27,8 @@ sys.path.insert(0, os.path.join(REPO, "chorus-mhd_amd"))
 
 from chorus_mhd import QuadInp, quad_grid          # noqa: E402
 from chorus_mhd import ic as ics                    # noqa: E402
-from chorus_mhd.solver import Solver, topology      # noqa: E402
+from chorus_mhd.solver import Batch, Instance, Solver, topology   # noqa: E402
+from chorus_mhd.diagnostics import magnetic_energy, point_volumes   # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "DOF-updates/s per RK stage"
@@ -48,7 +49,150 @@ WORKLOADS = {
     "loop2048": dict(desc="field loop 2048x1024 jittered (seed 7), order 4 (N=5), u=(2,1), AV off",
                      n=2048, m=1024, box=(-1.0, 1.0, -0.5, 0.5), N=5, jitter=0.10, seed=7,
                      bc="CYCL", ic="loop", gam=5.0 / 3, artiv=0),
+    "rotor1024": dict(desc="MHD rotor 1024x1024 quads on [0,1]^2, interior nodes jittered +-10% (seed 7), "
+                           "FREE (outflow) boundaries, SD order 5 (N=6), gamma=1.4, SSP(5,3), CFL 0.2, AV on",
+                      n=1024, m=1024, box=(0.0, 1.0, 0.0, 1.0), N=6, jitter=0.10, seed=7,
+                      bc="FREE", ic="rotor", gam=1.4, artiv=1, steps=50),
+    "loop_crossings": dict(desc="field loop 256x128 jittered (seed 7), order 4 (N=5), u=(2,1), AV off, "
+                                "two domain crossings (t=2)",
+                           n=256, m=128, box=(-1.0, 1.0, -0.5, 0.5), N=5, jitter=0.10, seed=7,
+                           bc="CYCL", ic="loop", gam=5.0 / 3, artiv=0, t_end=2.0),
 }
+
+
+ENSEMBLE = dict(
+    desc="batched ensemble: 256 independent Orszag-Tang / rotor instances (seed 7 draws the type, "
+         "B0 ~ U[0.5,1.5] scaling the problem's field, interior node jitter ~ U[0,15%]), meshes uniform "
+         "in {32^2,64^2,128^2,256^2}, SD orders uniform in {2..5} (N=3..6), AV on, SSP(5,3); "
+         "instances of one order and problem type share one CFL-0.2 time step",
+    ninst=256, seed=7)
+
+
+def ensemble_members(ninst, seed):
+    """Meshes, raw fields and per-group time steps of the ensemble workload."""
+    rng = np.random.default_rng(seed)
+    spec = []
+    for i in range(ninst):
+        kind = "ot" if rng.random() < 0.5 else "rotor"
+        b0 = float(rng.uniform(0.5, 1.5))
+        jit = float(rng.uniform(0.0, 0.15))
+        n = int(rng.choice([32, 64, 128, 256]))
+        N = int(rng.integers(2, 6)) + 1
+        spec.append((kind, b0, jit, n, N))
+    raw, dts = [], {}
+    for i, (kind, b0, jit, n, N) in enumerate(spec):
+        if kind == "ot":
+            mesh = quad_grid(n, n, 0.0, 2 * math.pi, 0.0, 2 * math.pi, jitter=jit, seed=1000 + i, bc="CYCL")
+            gam = 5.0 / 3
+            Q, p = ics.orszag_tang_fields(mesh, N, gam, b0)
+        else:
+            mesh = quad_grid(n, n, 0.0, 1.0, 0.0, 1.0, jitter=jit, seed=1000 + i, bc="FREE")
+            gam = 1.4
+            Q, p = ics.rotor_fields(mesh, N, gam, b0 * 5.0 / math.sqrt(4 * math.pi))
+        dt = ics.cfl_dt(mesh, N, Q, gam, 0.2)
+        key = (N, kind)
+        dts[key] = min(dts.get(key, dt), dt)
+        raw.append(dict(mesh=mesh, N=N, gam=gam, Q=Q, p=p, key=key, topo=topology(mesh)))
+    for r in raw:
+        r["dt"] = dts[r["key"]]
+    return raw
+
+
+def run_ensemble(args):
+    w = ENSEMBLE
+    steps = 10 if args.steps == 200 else args.steps
+    t0 = time.perf_counter()
+    raw = ensemble_members(w["ninst"], w["seed"])
+    setup_s = time.perf_counter() - t0
+    results, finals = {}, {}
+    for precision in (["mixed"] if args.no_mixed else ["mixed", "fp64"]):
+        members = [Instance(r["mesh"], r["N"], r["dt"], gam=r["gam"], artiv=1, precision=precision, topo=r["topo"])
+                   for r in raw]
+        b = Batch(members)
+        for i, r in enumerate(raw):
+            b.set_state(i, r["Q"])
+        b.correct_b()
+        for i, r in enumerate(raw):
+            q = b.get_state(i)
+            q[..., 4] = ics.energy(q, r["p"], r["gam"])
+            b.set_state(i, q)
+        t1 = time.perf_counter()
+        tiles = b.autotune()
+        tune_s = time.perf_counter() - t1
+        if args.warmup:
+            b.step(args.warmup)
+        ms = b.time_steps(steps)
+        dof = sum(m.mesh.ncell * m.N * m.N for m in members)
+        finals[precision] = [b.get_state(i) for i in range(len(members))]
+        results[precision] = dict(value=dof * 5 * steps / (ms * 1e-3), ms_per_step=ms / steps, dof=dof,
+                                  groups=b.groups(), tiles=tiles, autotune_s=tune_s,
+                                  divb_max=float(np.nanmax(b.divb())))
+        b.close()
+    r = results["mixed"]
+    out = {
+        "metric": METRIC, "value": r["value"], "unit": "DOF-updates/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": w["desc"], "form": "mixed, batched", "instances": w["ninst"], "dof": r["dof"],
+                   "groups": [dict(g, tile_shape=["256/128", "128/64", "512/256"][g["tile"]]) for g in r["groups"]]},
+        "setup_s": setup_s, "autotune_s": r["autotune_s"], "divb_max": r["divb_max"],
+    }
+    if "fp64" in results:
+        f = results["fp64"]
+        rel = []
+        for a, bq in zip(finals["fp64"], finals["mixed"]):
+            if np.isfinite(a).all() and np.isfinite(bq).all():
+                rel.append(float(np.linalg.norm(bq - a) / np.linalg.norm(a)))
+        out["fp64_batched"] = {"value": f["value"], "ms_per_step": f["ms_per_step"], "divb_max": f["divb_max"]}
+        out["speedup_vs_fp64"] = r["value"] / f["value"]
+        out["rel_l2_vs_fp64_max"] = max(rel) if rel else None
+        out["finite_instances"] = len(rel)
+    print(json.dumps(out), flush=True)
+
+
+def run_crossings(w, args):
+    """Field loop to t_end in both forms: magnetic-energy decay, div B over time,
+    and the mixed form's deviation from FP64 at t_end."""
+    N = w["N"]
+    mesh = quad_grid(w["n"], w["m"], *w["box"], jitter=w["jitter"], seed=w["seed"], bc=w["bc"])
+    topo = topology(mesh)
+    vol = point_volumes(mesh, N)
+    s = Solver(mesh, N, 1e-3, gam=w["gam"], precision="fp64", topo=topo)
+    Q0 = initial_state(w, mesh, s)
+    s.close()
+    dt0 = ics.cfl_dt(mesh, N, Q0, w["gam"], 0.2)
+    nsteps = int(math.ceil(w["t_end"] / dt0))
+    dt = w["t_end"] / nsteps
+    e0 = magnetic_energy(mesh, N, Q0, vol)
+    chunk = max(1, nsteps // 20)
+    res, finals = {}, {}
+    for precision in ("fp64", "mixed"):
+        s = Solver(mesh, N, dt, gam=w["gam"], artiv=w["artiv"], precision=precision, topo=topo)
+        s.set_state(Q0)
+        done, ms, hist = 0, 0.0, []
+        while done < nsteps:
+            k = min(chunk, nsteps - done)
+            ms += s.time_steps(k)["ms_total"]
+            done += k
+            d = s.diag()
+            Q = s.get_state()
+            hist.append((done * dt, magnetic_energy(mesh, N, Q, vol) / e0, d["divbmax"], d["divbglob"]))
+        finals[precision] = s.get_state()
+        s.close()
+        res[precision] = dict(value=mesh.ncell * N * N * 5 * nsteps / (ms * 1e-3), ms_per_step=ms / nsteps,
+                              energy_ratio=hist[-1][1], divb_max=max(h[2] for h in hist),
+                              divb_rms_end=hist[-1][3], history=[[round(h[0], 6), h[1], h[2]] for h in hist])
+    a, b = finals["fp64"], finals["mixed"]
+    out = {
+        "metric": METRIC, "value": res["fp64"]["value"], "unit": "DOF-updates/s", "n_gpus": 1, "steps": nsteps,
+        "warmup": 0, "ms_per_step": res["fp64"]["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": w["desc"], "ncell": mesh.ncell, "N": N, "dt": dt, "t_end": w["t_end"]},
+        "fp64": res["fp64"], "mixed": res["mixed"],
+        "mixed_rel_l2_vs_fp64": float(np.linalg.norm(b - a) / np.linalg.norm(a)),
+        "mixed_energy_ratio_minus_fp64": res["mixed"]["energy_ratio"] - res["fp64"]["energy_ratio"],
+    }
+    print(json.dumps(out), flush=True)
 
 
 def b_stage(N: int) -> float:
@@ -112,7 +256,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="ot512", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="ot512", choices=sorted(WORKLOADS) + ["ensemble"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision form")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -123,7 +267,15 @@ def main():
         print("bench.py runs on exactly one GPU: this build holds the whole mesh as one "
               "partition on one MI355X (multi-GPU halo exchange is not built)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "ensemble":
+        run_ensemble(args)
+        return
     w = WORKLOADS[args.workload]
+    if "t_end" in w:
+        run_crossings(w, args)
+        return
+    if "steps" in w and args.steps == 200:
+        args.steps = w["steps"]
     av_supported = True
     if args.copy_calibration:
         import torch

@@ -49,6 +49,11 @@ def _energy(Q, p, gam):
     return p / (gam - 1) + ke + me
 
 
+def energy(Q, p, gam):
+    """Total energy of Q's density, momentum and B with pressure p."""
+    return _energy(Q, p, gam)
+
+
 def _with_ct(solver, Q, p, gam):
     """B := curl A_z on the device, then E from the corrected B."""
     solver.set_state(Q)
@@ -65,6 +70,11 @@ def orszag_tang(solver, mesh, N, gam=5.0 / 3):
 
 
 def orszag_tang_scaled(solver, mesh, N, gam=5.0 / 3, b0=1.0):
+    return _with_ct(solver, *orszag_tang_fields(mesh, N, gam, b0), gam)
+
+
+def orszag_tang_fields(mesh, N, gam=5.0 / 3, b0=1.0):
+    """(Q before the B := curl A_z correction, p) of the Orszag-Tang vortex."""
     xy = solution_xy(mesh, N)
     x, y = xy[..., 0], xy[..., 1]
     rho = np.full_like(x, gam * gam)
@@ -78,12 +88,17 @@ def orszag_tang_scaled(solver, mesh, N, gam=5.0 / 3, b0=1.0):
     Q[..., 6] = b0 * np.sin(2 * x)
     Q[..., 8] = b0 * (0.5 * np.cos(2 * x) + np.cos(y))
     Q[..., 4] = _energy(Q, p, gam)
-    return _with_ct(solver, Q, p, gam)
+    return Q, p
 
 
 def rotor(solver, mesh, N, gam=1.4, b0=5.0 / math.sqrt(4 * math.pi)):
     """MHD rotor on [0,1]^2: rho=10, u = 20(-(y-1/2), x-1/2) for r<0.1, linear taper
     to rho=1, u=0 at r=0.115; p=1, Bx=B0 (A_z = B0*y), By=0."""
+    return _with_ct(solver, *rotor_fields(mesh, N, gam, b0), gam)
+
+
+def rotor_fields(mesh, N, gam=1.4, b0=5.0 / math.sqrt(4 * math.pi)):
+    """(Q before the B := curl A_z correction, p) of the MHD rotor."""
     xy = solution_xy(mesh, N)
     x, y = xy[..., 0] - 0.5, xy[..., 1] - 0.5
     r = np.sqrt(x * x + y * y)
@@ -100,11 +115,16 @@ def rotor(solver, mesh, N, gam=1.4, b0=5.0 / math.sqrt(4 * math.pi)):
     Q[..., 5] = b0
     Q[..., 8] = b0 * xy[..., 1]
     Q[..., 4] = _energy(Q, p, gam)
-    return _with_ct(solver, Q, p, gam)
+    return Q, p
 
 
 def field_loop(solver, mesh, N, gam=5.0 / 3, a0=1e-3, R=0.3, vel=(2.0, 1.0)):
     """Field-loop advection: rho=1, p=1, u=(2,1), A_z = a0 (R - r) for r < R."""
+    return _with_ct(solver, *field_loop_fields(mesh, N, gam, a0, R, vel), gam)
+
+
+def field_loop_fields(mesh, N, gam=5.0 / 3, a0=1e-3, R=0.3, vel=(2.0, 1.0)):
+    """(Q before the B := curl A_z correction, p) of the field loop."""
     xy = solution_xy(mesh, N)
     x, y = xy[..., 0], xy[..., 1]
     r = np.sqrt(x * x + y * y)
@@ -115,7 +135,7 @@ def field_loop(solver, mesh, N, gam=5.0 / 3, a0=1e-3, R=0.3, vel=(2.0, 1.0)):
     Q[..., 8] = np.where(r < R, a0 * (R - r), 0.0)
     p = np.ones_like(x)
     Q[..., 4] = _energy(Q, p, gam)
-    return _with_ct(solver, Q, p, gam)
+    return Q, p
 
 
 def cfl_dt(mesh, N, Q, gam, cfl=0.2) -> float:

@@ -62,6 +62,7 @@ SIGNATURES = {
     "chorus_batch_step": (C.c_int, [C.c_void_p, C.c_int]),
     "chorus_batch_compresid": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "chorus_batch_divb": (C.c_int, [C.c_void_p, _dp]),
+    "chorus_batch_correct_b": (C.c_int, [C.c_void_p]),
     "chorus_batch_ngroups": (C.c_int, [C.c_void_p]),
     "chorus_batch_group": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _ip, _ip]),
     "chorus_batch_autotune": (C.c_int, [C.c_void_p, _ip]),

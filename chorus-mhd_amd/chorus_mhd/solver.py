@@ -253,6 +253,9 @@ class Batch:
     def step(self, nsteps: int = 1) -> None:
         self._check(_lib.load().chorus_batch_step(self._b, nsteps), "chorus_batch_step")
 
+    def correct_b(self) -> None:
+        self._check(_lib.load().chorus_batch_correct_b(self._b), "chorus_batch_correct_b")
+
     def divb(self) -> np.ndarray:
         out = np.empty(len(self.members))
         self._check(_lib.load().chorus_batch_divb(self._b, _lib.ptr(out)), "chorus_batch_divb")

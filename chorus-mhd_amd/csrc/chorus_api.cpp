@@ -39,7 +39,7 @@ struct chorus_ctx {
     int ninst = 1;                 // instances sharing this context (batched groups)
     int *inst = nullptr;           // batched groups: instance of each cell (device)
     double *hinst = nullptr;       // batched groups: AV length scale per instance (device)
-    int tile = 0;                  // mixed form: tile shape of the cell kernels
+    int tile = 2;                  // mixed form: tile shape of the cell kernels (the autotuner's usual pick)
     int stage_parity = 0;          // slot used by the next stage
     int last_parity = -1;          // slot written by the last stage
     double h_msh = 0.0;            // AV length scale (AV.f90:147)
@@ -1043,6 +1043,19 @@ int chorus_batch_divb(chorus_batch *b, double *divbmax) {
         for (int k = 0; k < nb; k++) mx = std::max(mx, part[2 * k + 1]);
         divbmax[i] = mx;
     }
+    return 0;
+}
+
+int chorus_batch_correct_b(chorus_batch *b) {
+    if (!b) return CHORUS_ERR_ARG;
+    int rc = refresh(b);
+    if (rc) return rc;
+    HIP_TRYB(b, hipStreamSynchronize(b->stream));
+    for (chorus_ctx *c : b->groups) {
+        rc = run_ct(c, c->X[c->cur], 0);
+        if (rc) return bfail(b, rc, c->err);
+    }
+    for (chorus_ctx *c : b->groups) HIP_TRYB(b, hipStreamSynchronize(c->stream));
     return 0;
 }
 

@@ -15,7 +15,7 @@ static int fblocks(int N, int nface) { return (nface * N + 255) / 256; }
 
 // tile shapes of the mixed cell kernels (threads-per-block targets of the
 // ideal and viscous kernels), chosen per order by the autotuner:
-//   tile 0: 256 / 128 (default), tile 1: 128 / 64, tile 2: 512 / 256
+//   tile 0: 256 / 128, tile 1: 128 / 64, tile 2: 512 / 256 (default)
 int tile_count() { return 3; }
 
 template <int n, int TBI>

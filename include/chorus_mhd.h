@@ -134,6 +134,8 @@ int chorus_batch_get_state(chorus_batch *b, int i, double *q);
 int chorus_batch_step(chorus_batch *b, int nsteps);
 /* residual of instance i's current state (compresid.f90:1-67), as chorus_compresid */
 int chorus_batch_compresid(chorus_batch *b, int i, double *resid);
+/* correct_B_via_curl_Az (setinitialcond.f90:82-92) of every instance */
+int chorus_batch_correct_b(chorus_batch *b);
 /* max |div B| per instance (iterations.f90:35-48), divbmax[ninst] */
 int chorus_batch_divb(chorus_batch *b, double *divbmax);
 /* groups: ngroups, and for group g its N, member count, cells and tile shape */

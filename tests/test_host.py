@@ -119,3 +119,21 @@ def test_product_path_has_no_cpu_fallback():
     with pytest.raises(ChorusError, match="no HIP device"):
         Solver(m, 4, 1e-3)
     del L, n
+
+
+def test_quadrature_and_energy_diagnostics():
+    """Solution-point quadrature is exact for degree N-1 polynomials on [0,1]
+    and the cell volumes of a jittered mesh sum to the domain area, so the
+    magnetic energy of a uniform field is |B|^2/2 times the area."""
+    from chorus_mhd.diagnostics import magnetic_energy, point_volumes, quadrature_weights
+    from chorus_mhd.ic import solution_points
+    for N in range(2, 9):
+        w, x = quadrature_weights(N), solution_points(N)
+        for k in range(N):
+            assert abs(np.sum(w * x ** k) - 1.0 / (k + 1)) < 1e-13
+    mesh = quad_grid(9, 6, -1.0, 1.0, -0.5, 0.5, jitter=0.12, seed=3)
+    N = 4
+    assert abs(point_volumes(mesh, N).sum() - 2.0) < 1e-13
+    Q = np.zeros((mesh.ncell, N, N, 9))
+    Q[..., 5], Q[..., 6], Q[..., 7] = 0.3, -0.4, 1.2
+    assert abs(magnetic_energy(mesh, N, Q) - 0.5 * (0.09 + 0.16 + 1.44) * 2.0) < 1e-12
