@@ -31,6 +31,10 @@ from chorus_mhd.solver import Batch, Instance, Solver, topology   # noqa: E402
 from chorus_mhd.diagnostics import magnetic_energy, point_volumes   # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes of one stage (all kernels of the stage), rocprofv3 FETCH_SIZE and WRITE_SIZE per
+# dispatch scaled by kernels of known bytes (FETCH x 2.00, WRITE x 1.00); summaries under profiles/
+MEASURED_STAGE_TRAFFIC = {("ot512", "fp64"): 8.08e9, ("ot512", "mixed"): 6.03e9,
+                          ("ot512ideal", "fp64"): 2.77e9, ("ot512ideal", "mixed"): 2.82e9}
 METRIC = "DOF-updates/s per RK stage"
 
 # workloads (n, m, domain, N, jitter, seed, bc, ic, gamma, AV)
@@ -260,7 +264,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision form")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="first run one device-to-device copy of 256 MiB (known HBM bytes) so that "
+                    help="first run two kernels of known HBM bytes (a 256 MiB fill: 256 MiB written; an "
+                         "in-place add of two 256 MiB arrays: 512 MiB read, 256 MiB written) so that "
                          "hardware byte counters collected over this run can be scaled")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -280,8 +285,8 @@ def main():
     if args.copy_calibration:
         import torch
         src = torch.ones(1 << 25, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
+        dst = torch.full_like(src, 2.0)
+        dst.add_(src)
         torch.cuda.synchronize()
         del src, dst
     N = w["N"]
@@ -334,11 +339,11 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "kernel": ("k_grad<%d,double> + k_stage_visc<%d,double>" % (N, N)) if r64["artiv"]
-                      else "k_stage_ideal<%d,double>" % N,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_STAGE_TRAFFIC.get((args.workload, "fp64")),
+            "kernel": ("one stage: k_face_inv + k_grad + k_face_visc + k_stage_visc <%d,double>" % N)
+                      if r64["artiv"] else "one stage: k_face_inv + k_stage_ideal <%d,double>" % N,
             "stage_kernel_ms": r64["tm"]["ms_stage_avg"], "ct_kernel_ms": r64["tm"]["ms_ct_avg"],
-            "bytes_per_dof_stage": b_stage(N),
+            "bytes_per_dof_stage": b_stage(N), "algorithmic_bytes_per_stage": ndof * b_stage(N),
         },
         "divb": {"rms": r64["diag"]["divbglob"], "max": r64["diag"]["divbmax"]},
     }
@@ -352,6 +357,7 @@ def main():
             "linf_vs_fp64": float(np.abs(b - a).max()),
             "stage_kernel_ms": rm["tm"]["ms_stage_avg"],
             "roofline_frac": ndof * b_stage(N) / (rm["tm"]["ms_stage_avg"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "traffic": MEASURED_STAGE_TRAFFIC.get((args.workload, "mixed")),
             "divb_max": rm["diag"]["divbmax"],
         }
     if not args.no_cpu:
