@@ -30,6 +30,7 @@ struct chorus_ctx {
     int4 *faces = nullptr;         // {cellL, lfiL, cellR (-1: FREE), lfiR} ordered by left cell
     double *TQ = nullptr;          // traces of the current state at the face points
     void *FF = nullptr;            // per cell face point: common inviscid flux + A_z
+    double *FA = nullptr;          // mixed form: per cell face point: FP64 upwind A_z
     void *FFv = nullptr;           // viscous path: per cell face point: common viscous flux
     void *gtr = nullptr;           // viscous path: own nablaQs face traces
     void *gsol = nullptr;          // viscous path: nablaQs at the solution points
@@ -91,6 +92,7 @@ FaceArgs face_args(chorus_ctx *c) {
     f.cellxy = c->cellxy;
     f.TQ = c->TQ;
     f.FF = c->FF;
+    f.FA = c->FA;
     f.FFv = c->FFv;
     f.gtr = c->gtr;
     f.sm = c->sm;
@@ -153,6 +155,7 @@ StageArgs base_args(chorus_ctx *c) {
     a.nbr = c->nbr;
     a.gam = c->p.gam;
     a.FF = c->FF;
+    a.FA = c->FA;
     a.FFv = c->FFv;
     a.TQ = c->TQ;
     a.sm = c->sm;
@@ -524,6 +527,7 @@ int init_ctx(chorus_ctx *c, const chorus_mesh *m, const chorus_params *p) {
     HIP_TRY(c, hipMalloc(&c->faces, faces.size() * sizeof(int4)));
     HIP_TRY(c, hipMalloc(&c->TQ, nfp * 9 * sizeof(double)));
     HIP_TRY(c, hipMalloc(&c->FF, nfp * 9 * sizeof(double)));
+    if (c->p.precision == CHORUS_MIXED) HIP_TRY(c, hipMalloc(&c->FA, nfp * sizeof(double)));
     if (!c->ideal) HIP_TRY(c, hipMalloc(&c->FFv, nfp * 9 * sizeof(double)));
     HIP_TRY(c, hipMemcpyAsync(c->faces, faces.data(), faces.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
     if (!c->ideal) {
@@ -553,6 +557,7 @@ void chorus_free(chorus_ctx *c) {
     (void)hipFree(c->faces);
     (void)hipFree(c->TQ);
     (void)hipFree(c->FF);
+    (void)hipFree(c->FA);
     (void)hipFree(c->FFv);
     (void)hipFree(c->gtr);
     (void)hipFree(c->gsol);

@@ -56,6 +56,7 @@ struct StageArgs {
     double c0, c1, c2, c3, c4, c5;
     double gam;
     const void *FF;     // common face fluxes of each cell's face points (k_face_inv)
+    const double *FA;   // mixed form: the FP64 upwind A_z of each cell's face points
     const void *FFv;    // viscous face fluxes (k_face_visc)
     double *TQ;         // traces of Qout at the face points, written by the stage kernel
     double *sm;         // AV: indicator of Qout per cell, written by the viscous stage kernel
@@ -78,6 +79,7 @@ struct FaceArgs {
     const double *cellxy;
     const double *TQ;
     void *FF;
+    double *FA;              // mixed form: FP64 upwind A_z per cell face point
     void *FFv;
     const void *gtr;
     const double *sm;        // AV indicator per cell

@@ -23,7 +23,9 @@
 // where F(Q_R) - F(Q_L) is again an exact difference about Q_L, so only jumps
 // and deviations are ever rounded to FP32.
 // The state, the RK sums, the traces, the metrics, the AV indicator and the
-// CT update stay FP64.
+// CT update stay FP64, and so does the whole vector-potential equation (A_z at
+// the flux points, its upwind face value, its derivative and residual): the
+// SD div B of B = curl A_z amplifies grid-scale noise in A_z by ~(N^2/h)^2.
 #pragma once
 
 #include "kernels_visc.hpp"
@@ -133,12 +135,12 @@ __device__ __forceinline__ float wave_speed(const float *Ql, const float *Qr, fl
 // Phase A of the mixed cell kernels: the block's cells (contiguous in HBM)
 // become FP32 deviations from their FP64 means.  Each thread keeps the values
 // it loaded in registers; tmp (at least ncl*NQ doubles of LDS) holds the FP64
-// copy only until the means are taken, so dq may alias it.  rho, if given,
-// receives the FP64 density plane of each cell (for the AV indicator).
+// copy only until the means are taken, so dq may alias it.  keep, if given,
+// receives the FP64 plane of variable keep_var of each cell.
 // Called by all THREADS threads of the block.
 template <int N, int CPB, int THREADS>
 __device__ __forceinline__ void stage_deviations(const double *Qg, int ncl, double gam, double *tmp, double *sBar,
-                                                 Ref *sRef, float *dq, double *rho) {
+                                                 Ref *sRef, float *dq, double *keep, int keep_var = 0) {
     constexpr int NP = N * N, NQ = NV * NP, PER = (CPB * NQ + THREADS - 1) / THREADS;
     const int tid = threadIdx.x, n = ncl * NQ;
     double held[PER];
@@ -155,8 +157,8 @@ __device__ __forceinline__ void stage_deviations(const double *Qg, int ncl, doub
         for (int p = 0; p < NP; p++) s += q[p];
         sBar[i] = s / NP;
     }
-    if (rho)
-        for (int i = tid; i < ncl * NP; i += THREADS) rho[i] = tmp[(i / NP) * NQ + i % NP];
+    if (keep)
+        for (int i = tid; i < ncl * NP; i += THREADS) keep[i] = tmp[(i / NP) * NQ + keep_var * NP + i % NP];
     __syncthreads();
     for (int c = tid; c < ncl; c += THREADS) ref_from(sBar + c * NV, gam, sRef[c]);
 #pragma unroll
@@ -171,10 +173,10 @@ __device__ __forceinline__ void stage_deviations(const double *Qg, int ncl, doub
 // point's state q and (when the stage needs it) Qa; the new state is also
 // written to qout[var*NP] (LDS) for the traces
 __device__ __forceinline__ void combine_point(const StageArgs &a, size_t base, int NP, const double *q,
-                                              const double *qa, const float *res, double *qout) {
+                                              const double *qa, const float *res, double res8, double *qout) {
 #pragma unroll
     for (int v = 0; v < NV; v++) {
-        const double r = double(res[v]);
+        const double r = v == 8 ? res8 : double(res[v]);
         const size_t o = base + (size_t)v * NP;
         double qn;
         switch (a.kind) {
@@ -214,6 +216,60 @@ __device__ __forceinline__ void dtrace9(const float *dq, int fam, int ifp, int j
 #pragma unroll
         for (int v = 0; v < NV; v++) d[v] += dq[v * NP + idx] * l;
     }
+}
+
+// FP64 A_z derivative term of one family at position pos of a line: the
+// contravariant A_z flux A*s along the line, differentiated, with A(r) the
+// FP64 A_z at the N+1 flux points of the line.  The metric row s(r) of a
+// bilinear cell is linear along the line and independent of the line, with
+// end values (flux points 0 and N sit at 0 and 1) that are vertex differences:
+//   fam 0: s = (y_eta, -x_eta),  s(0) = (y4-y1, x1-x4), s(N) = (y3-y2, x2-x3)
+//   fam 1: s = (-y_psi, x_psi),  s(0) = (y1-y2, x2-x1), s(N) = (y4-y3, x3-x4)
+// so sum_r A(r) M(r,pos) s(r) = s(0) T0 + s(N) T1 with
+// T0 = sum_r A(r) M(r,pos) (1 - Xf(r)), T1 = sum_r A(r) M(r,pos) Xf(r).
+template <int N>
+__device__ __forceinline__ void az_derivative(const double *A, const double *xy, int fam, int pos, double &e1,
+                                              double &e2) {
+    double T0 = 0.0, T1 = 0.0;
+#pragma unroll
+    for (int r = 0; r <= N; r++) {
+        const double am = A[r] * c_M[N][r * N + pos];
+        T1 += am * c_Xf[N][r];
+        T0 += am - am * c_Xf[N][r];
+    }
+    const double *x = xy, *y = xy + 4;
+    if (fam == 0) {
+        e1 = (y[3] - y[0]) * T0 + (y[2] - y[1]) * T1;
+        e2 = (x[0] - x[3]) * T0 + (x[1] - x[2]) * T1;
+    } else {
+        e1 = (y[0] - y[1]) * T0 + (y[3] - y[2]) * T1;
+        e2 = (x[1] - x[0]) * T0 + (x[2] - x[3]) * T1;
+    }
+}
+
+// -(u J dA/dx + v J dA/dy) at solution point (pis, pjs) from the FP64 A_z
+// flux-point values A[fam * FAMST + line * (N + 1) + r] of the point's cell
+// (magnetic_potential.f90:1-42, compresid.f90:40-47); q is the point's state
+template <int N, int FAMST>
+__device__ __forceinline__ double az_residual(const double *A, const double *xy, int pis, int pjs, const double *q) {
+    double e1a, e2a, e1b, e2b;
+    az_derivative<N>(A + pjs * (N + 1), xy, 0, pis, e1a, e2a);
+    az_derivative<N>(A + FAMST + pis * (N + 1), xy, 1, pjs, e1b, e2b);
+    const double au = q[1] / q[0], av = q[2] / q[0];
+    return -(au * (e1a + e1b) + av * (e2a + e2b));
+}
+
+// FP64 A_z at flux point (fam, r, line) of block cell c: the upwind face value
+// of k_face_inv_mx at face points, the trace of the FP64 plane elsewhere
+template <int N>
+__device__ __forceinline__ double az_at(const double *FA, const double *sAz, int cell, int c, int fam, int r,
+                                        int line) {
+    if (r == 0 || r == N) {
+        int lfi, nfp;
+        face_of<N>(fam, r, line, lfi, nfp);
+        return FA[((size_t)cell * 4 + lfi) * N + nfp];
+    }
+    return trace1<N>(sAz + c * N * N, fam, fam == 0 ? r : line, fam == 0 ? line : r);
 }
 
 // ---------------------------------------------------------------------------
@@ -265,6 +321,7 @@ __global__ void __launch_bounds__(256) k_face_inv_mx(FaceArgs a) {
         if (free_face) {
 #pragma unroll
             for (int v = 0; v < NV; v++) FF[pl + v] = 0.f;
+            a.FA[pl / NV] = ql[8];
         } else {
             const float slr = sl * float(-side_sign(fc.w));
 #pragma unroll
@@ -274,10 +331,16 @@ __global__ void __launch_bounds__(256) k_face_inv_mx(FaceArgs a) {
                 FF[pl + v] = h - e;
                 FF[pr + v] = slr * (-h - e);
             }
-            // upwind A_z by the left cell's outward normal velocity (getrusanov.f90:98-103)
-            const bool take_left = (s.q[1] * s1 + s.q[2] * s2) * sl > 0.f;
-            FF[pl + 8] = take_left ? 0.f : d[8];
-            FF[pr + 8] = take_left ? -d[8] : 0.f;
+            // upwind A_z by the left cell's outward normal velocity, evaluated in FP64
+            // as the FP64 form does (getrusanov.f90:61-66, 98-103): near-zero normal
+            // velocities then pick the same side, so A_z keeps the FP64 form's face values
+            const double rhoi = 1.0 / ql[0];
+            const double mn = sqrt(m1 * m1 + m2 * m2);
+            const double ux = m1 / mn * double(side_sign(fc.y)), uy = m2 / mn * double(side_sign(fc.y));
+            const bool take_left = rhoi * ql[1] * ux + rhoi * ql[2] * uy > 0.0;
+            const double az = take_left ? ql[8] : a.TQ[pr + 8];
+            a.FA[pl / NV] = az;
+            a.FA[pr / NV] = az;
         }
         emax = double(eig);
     }
@@ -299,30 +362,37 @@ __global__ void __launch_bounds__(256) k_face_inv_mx(FaceArgs a) {
 // FP32 deviations and flux planes only (one union area, reused for the FP64
 // copy at set-up and for the new state at the end).
 // ---------------------------------------------------------------------------
+// 4 waves per SIMD (<= 128 VGPRs) where that needs no spills
+#ifndef CHORUS_MX_IDEAL_MINW
+#define CHORUS_MX_IDEAL_MINW(N) ((N) <= 5 ? 4 : 3)
+#endif
 template <int N, int TB>
-__global__ void __launch_bounds__((CellShape<N, TB>::THREADS)) k_stage_ideal_mx(StageArgs a) {
+__global__ void __launch_bounds__((CellShape<N, TB>::THREADS)) __attribute__((amdgpu_waves_per_eu(CHORUS_MX_IDEAL_MINW(N)))) k_stage_ideal_mx(StageArgs a) {
     using S = CellShape<N, TB>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ, T = S::THREADS;
-    constexpr int FST = 11 * NF + 1;     // per-cell stride: dF1 (8) + dA_z + metric rows (2)
+    constexpr int FST = 8 * NF + 1;      // per-cell stride of the dF1 planes
+    constexpr int AST = NF + 1;          // per-cell stride of the FP64 A_z flux-point values
     constexpr int UF = CPB * NQ + CPB * FST;
     constexpr int UD = (UF + 1) / 2 > CPB * NQ ? (UF + 1) / 2 : CPB * NQ;
     __shared__ double U[UD];
     __shared__ double sX[CPB * 8];
     __shared__ double sBar[CPB * NV];
     __shared__ Ref sRef[CPB];
+    __shared__ double sAz[CPB * NP];     // FP64 A_z plane of each cell
+    __shared__ double sA[2 * CPB * AST]; // FP64 A_z at the flux points of both families
     float *sD = (float *)U, *sF = sD + CPB * NQ;
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = blk * CPB;
     const int ncl = min(CPB, a.ncell - c0);
     for (int i = tid; i < ncl * 8; i += T) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
-    stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, U, sBar, sRef, sD, nullptr);
+    stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, U, sBar, sRef, sD, sAz, 8);
     const float gam = float(a.gam);
     const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
     const bool pact = (tid < CPB * NP) && (pc < ncl);
     const size_t base = (size_t)(c0 + pc) * NQ + pk;
     const float *FF = (const float *)a.FF;
-    float dF[8], dA1 = 0.f, dA2 = 0.f, res[NV];
+    float dF[8], res[NV];
     double qown[NV], qa[NV];
 #pragma unroll 1
     for (int fam = 0; fam < 2; fam++) {
@@ -330,15 +400,15 @@ __global__ void __launch_bounds__((CellShape<N, TB>::THREADS)) k_stage_ideal_mx(
             const int c = it / NF, k = it % NF;
             const int line = k / (N + 1), r = k % (N + 1);
             const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
-            float jump[NV];
+            float jump[8];
 #pragma unroll
-            for (int v = 0; v < NV; v++) jump[v] = 0.f;
+            for (int v = 0; v < 8; v++) jump[v] = 0.f;
             if (r == 0 || r == N) {
                 int lfi, nfp;
                 face_of<N>(fam, r, line, lfi, nfp);
                 const float *src = FF + (((size_t)(c0 + c) * 4 + lfi) * N + nfp) * NV;
 #pragma unroll
-                for (int v = 0; v < NV; v++) jump[v] = src[v];
+                for (int v = 0; v < 8; v++) jump[v] = src[v];
             }
             double s1d, s2d;
             metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
@@ -349,9 +419,7 @@ __global__ void __launch_bounds__((CellShape<N, TB>::THREADS)) k_stage_ideal_mx(
             float *f = sF + c * FST + line * (N + 1) + r;
 #pragma unroll
             for (int v = 0; v < 8; v++) f[v * NF] = F[v] * s1 + G[v] * s2 + jump[v];
-            f[8 * NF] = d[8] + jump[8];
-            f[9 * NF] = s1;
-            f[10 * NF] = s2;
+            sA[(fam * CPB + c) * AST + line * (N + 1) + r] = az_at<N>(a.FA, sAz, c0 + c, c, fam, r, line);
         }
         __syncthreads();
         // the point's state for the combination, loaded while the derivative is summed
@@ -359,7 +427,7 @@ __global__ void __launch_bounds__((CellShape<N, TB>::THREADS)) k_stage_ideal_mx(
         if (pact) {
             const int line = fam == 0 ? pjs : pis, pos = fam == 0 ? pis : pjs;
             const float *f = sF + pc * FST + line * (N + 1);
-            float d[8], e1 = 0.f, e2 = 0.f;
+            float d[8];
 #pragma unroll
             for (int v = 0; v < 8; v++) d[v] = 0.f;
 #pragma unroll CHORUS_MX_CUNROLL
@@ -367,25 +435,23 @@ __global__ void __launch_bounds__((CellShape<N, TB>::THREADS)) k_stage_ideal_mx(
                 const float m = c_Mf[N][r * N + pos];
 #pragma unroll
                 for (int v = 0; v < 8; v++) d[v] += f[v * NF + r] * m;
-                e1 += f[8 * NF + r] * m * f[9 * NF + r];
-                e2 += f[8 * NF + r] * m * f[10 * NF + r];
             }
-            if (fam == 0) {
 #pragma unroll
-                for (int v = 0; v < 8; v++) dF[v] = d[v];
-                dA1 = e1; dA2 = e2;
-            } else {
-                const float iJ = float(1.0 / jacobian<N>(sX + pc * 8, pis, pjs));
-                const float iq0 = frcp(float(qown[0]));
-                const float au = float(qown[1]) * iq0, av = float(qown[2]) * iq0;
-#pragma unroll
-                for (int v = 0; v < 8; v++) res[v] = -(dF[v] + d[v]) * iJ;
-                res[8] = -(au * (dA1 + e1) + av * (dA2 + e2)) * iJ;
-            }
+            for (int v = 0; v < 8; v++) dF[v] = fam == 0 ? d[v] : -(dF[v] + d[v]);
         }
         __syncthreads();
     }
-    if (pact) combine_point(a, base, NP, qown, qa, res, U + pc * NQ + pk);
+    // A_z residual in FP64 from the flux-point values of both families
+    double res8 = 0.0;
+    if (pact) {
+        const double iJd = 1.0 / jacobian<N>(sX + pc * 8, pis, pjs);
+        const float iJ = float(iJd);
+#pragma unroll
+        for (int v = 0; v < 8; v++) res[v] = dF[v] * iJ;
+        res[8] = 0.f;
+        res8 = az_residual<N, CPB * AST>(sA + pc * AST, sX + pc * 8, pis, pjs, qown) * iJd;
+    }
+    if (pact) combine_point(a, base, NP, qown, qa, res, res8, U + pc * NQ + pk);
     if (a.kind == KIND_RESID) return;
     __syncthreads();
     store_traces<N>(U, ncl, c0, a.TQ);
@@ -599,14 +665,15 @@ template <int N, int TB> struct MxVShape {
 };
 
 #ifndef CHORUS_MX_VISC_MINW
-#define CHORUS_MX_VISC_MINW 3
+#define CHORUS_MX_VISC_MINW(N) ((N) <= 6 ? 4 : 3)
 #endif
 template <int N, int TB, int VM>
-__global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amdgpu_waves_per_eu(CHORUS_MX_VISC_MINW))) k_stage_visc_mx(StageArgs a, ViscArgs v) {
+__global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amdgpu_waves_per_eu(CHORUS_MX_VISC_MINW(N)))) k_stage_visc_mx(StageArgs a, ViscArgs v) {
     using S = CellShape<N, TB>;
     using TS = MxVShape<N, TB>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ, T = TS::THREADS, NIT = TS::NIT;
-    constexpr int UST = 20 * NF + 1;     // dF1 (8) + Fv1 (9) + dA_z + metric rows (2)
+    constexpr int UST = 17 * NF + 1;     // dF1 (8) + Fv1 (9)
+    constexpr int AST = NF + 1;          // per-cell stride of the FP64 A_z flux-point values
     constexpr int UF = CPB * NQ + CPB * UST + CPB * 16 * NP;
     constexpr int UD = (UF + 1) / 2 > CPB * NQ ? (UF + 1) / 2 : CPB * NQ;
     __shared__ double U[UD];
@@ -615,6 +682,8 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     __shared__ Ref sRef[CPB];
     __shared__ double sMu[CPB * 4];
     __shared__ double sAV[CPB * NP];
+    __shared__ double sAz[CPB * NP];     // FP64 A_z plane of each cell
+    __shared__ double sA[2 * CPB * AST]; // FP64 A_z at the flux points of both families
     float *sD = (float *)U, *sU = sD + CPB * NQ, *sG = sU + CPB * UST;
     static_assert(9 * CPB * NP <= CPB * UST, "exchange area");
     const int tid = threadIdx.x;
@@ -627,7 +696,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
         if (blockIdx.x == 0)
             for (int i = tid; i < v.ninst; i += T) v.eig_next[i] = 0.0;
     }
-    stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, U, sBar, sRef, sD, nullptr);
+    stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, U, sBar, sRef, sD, sAz, 8);
     for (int i = tid; i < ncl * 16 * NP; i += T) sG[i] = v.gsol_r<float>()[(size_t)c0 * 16 * NP + i];
     __syncthreads();
     const float gam = float(a.gam);
@@ -637,7 +706,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     const bool pact = (pt < CPB * NP) && (pc < ncl);
     const size_t base = (size_t)(c0 + pc) * NQ + pk;
     const float *FF = (const float *)a.FF, *FFv = (const float *)a.FFv;
-    float acc8[8], accv[NV], dA1 = 0.f, dA2 = 0.f;
+    float acc8[8], accv[NV];
     double qown[NV], qa[NV];
 #pragma unroll 1
     for (int fam = 0; fam < 2; fam++) {
@@ -650,15 +719,15 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
                 const int line = k / (N + 1), r = k % (N + 1);
                 const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
                 float *u = sU + c * UST + line * (N + 1) + r;
-                float jump[NV];
+                float jump[8];
 #pragma unroll
-                for (int w = 0; w < NV; w++) jump[w] = 0.f;
+                for (int w = 0; w < 8; w++) jump[w] = 0.f;
                 if (r == 0 || r == N) {
                     int lfi, nfp;
                     face_of<N>(fam, r, line, lfi, nfp);
                     const size_t o = (((size_t)(c0 + c) * 4 + lfi) * N + nfp) * NV;
 #pragma unroll
-                    for (int w = 0; w < NV; w++) jump[w] = FF[o + w];
+                    for (int w = 0; w < 8; w++) jump[w] = FF[o + w];
 #pragma unroll
                     for (int w = 0; w < NV; w++) u[(8 + w) * NF] = FFv[o + w];
                 }
@@ -670,9 +739,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
                 dflux(sRef[c], d, gam - 1.f, F, G);
 #pragma unroll
                 for (int w = 0; w < 8; w++) u[w * NF] = F[w] * s1 + G[w] * s2 + jump[w];
-                u[17 * NF] = d[8] + jump[8];
-                u[18 * NF] = s1;
-                u[19 * NF] = s2;
+                sA[(fam * CPB + c) * AST + line * (N + 1) + r] = az_at<N>(a.FA, sAz, c0 + c, c, fam, r, line);
             } else if (t >= TS::S1) {
                 // viscous / AV flux of the own trace at an interior point
                 const int item = t - TS::S1;
@@ -698,7 +765,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
             const int line = fam == 0 ? pjs : pis, pos = fam == 0 ? pis : pjs;
             const float *f = sU + pc * UST + line * (N + 1);
             if (grp == 0) {
-                float d[8], e1 = 0.f, e2 = 0.f;
+                float d[8];
 #pragma unroll
                 for (int w = 0; w < 8; w++) d[w] = 0.f;
 #pragma unroll CHORUS_MX_CUNROLL
@@ -706,12 +773,9 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
                     const float m = c_Mf[N][r * N + pos];
 #pragma unroll
                     for (int w = 0; w < 8; w++) d[w] += f[w * NF + r] * m;
-                    e1 += f[17 * NF + r] * m * f[18 * NF + r];
-                    e2 += f[17 * NF + r] * m * f[19 * NF + r];
                 }
 #pragma unroll
                 for (int w = 0; w < 8; w++) acc8[w] = fam == 0 ? d[w] : acc8[w] + d[w];
-                dA1 = dA1 + e1; dA2 = dA2 + e2;
             } else {
                 float dv[NV];
 #pragma unroll
@@ -736,18 +800,19 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     }
     __syncthreads();
     float res[NV];
+    double res8 = 0.0;
     if (pact && grp == 0) {
         prefetch_point(a, base, NP, qown, qa);
-        const float iJ = float(1.0 / jacobian<N>(sX + pc * 8, pis, pjs));
-        const float iq0 = frcp(float(qown[0]));
-        const float au = float(qown[1]) * iq0, av = float(qown[2]) * iq0;
+        const double iJd = 1.0 / jacobian<N>(sX + pc * 8, pis, pjs);
+        const float iJ = float(iJd);
 #pragma unroll
         for (int w = 0; w < 8; w++) res[w] = (-acc8[w] + xv[w * CPB * NP + pt]) * iJ;
-        res[8] = (-(au * dA1 + av * dA2) + xv[8 * CPB * NP + pt]) * iJ;
+        res[8] = 0.f;
+        res8 = (az_residual<N, CPB * AST>(sA + pc * AST, sX + pc * 8, pis, pjs, qown) + double(xv[8 * CPB * NP + pt])) * iJd;
     }
     __syncthreads();
     // the new state replaces the compute layout for the traces
-    if (pact && grp == 0) combine_point(a, base, NP, qown, qa, res, U + pc * NQ + pk);
+    if (pact && grp == 0) combine_point(a, base, NP, qown, qa, res, res8, U + pc * NQ + pk);
     if (a.kind == KIND_RESID) return;
     __syncthreads();
     store_traces<N>(U, ncl, c0, a.TQ);
