@@ -151,6 +151,8 @@ def run_ensemble(args):
         out["speedup_vs_fp64"] = r["value"] / f["value"]
         out["rel_l2_vs_fp64_max"] = max(rel) if rel else None
         out["finite_instances"] = len(rel)
+        out["finite_fp64"] = sum(bool(np.isfinite(a).all()) for a in finals["fp64"])
+        out["finite_mixed"] = sum(bool(np.isfinite(a).all()) for a in finals["mixed"])
     print(json.dumps(out), flush=True)
 
 
