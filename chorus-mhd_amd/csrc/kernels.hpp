@@ -70,6 +70,30 @@ __device__ __forceinline__ int xcd_block(int b, int nblk) {
     return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 
+// Block-cooperative copy of n <= PER*THREADS consecutive elements from global
+// memory: gather_regs issues every load of the thread into registers before
+// any use, scatter_lds then writes them to LDS.  A strided copy loop would
+// wait for each element's load before the next one is issued (one HBM
+// latency per element a thread copies); this costs one.  blockDim.x must be
+// THREADS.
+template <int PER, int THREADS, class T>
+__device__ __forceinline__ void gather_regs(T *v, const T *src, int n) {
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = (int)threadIdx.x + k * THREADS;
+        v[k] = i < n ? src[i] : T(0);
+    }
+}
+template <int PER, int THREADS, class T>
+__device__ __forceinline__ void scatter_lds(T *dst, const T *v, int n) {
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = (int)threadIdx.x + k * THREADS;
+        if (i < n) dst[i] = v[k];
+    }
+}
+template <int NMAX, int THREADS> constexpr int per_thread() { return (NMAX + THREADS - 1) / THREADS; }
+
 // local face -> flux-point index of face point nfp (face2fpconnec, initsetup.f90:343-379)
 template <int N> __device__ __forceinline__ void face2fp(int lf, int nfp, int &ifp, int &jfp) {
     switch (lf) {
@@ -283,10 +307,12 @@ __global__ void __launch_bounds__(IShape<N>::THREADS) __attribute__((amdgpu_wave
 
     // A: stage the block's cells (contiguous) and their vertex coordinates
     {
-        const double *src = a.Qin + (size_t)c0 * NQ;
-        for (int i = tid; i < ncl * NQ; i += blockDim.x) sQ[i] = src[i];
-        const double *xs = a.cellxy + (size_t)c0 * 8;
-        for (int i = tid; i < ncl * 8; i += blockDim.x) sX[i] = xs[i];
+        constexpr int PQ = per_thread<CPB * NQ, S::THREADS>(), PX = per_thread<CPB * 8, S::THREADS>();
+        double hq[PQ], hx[PX];
+        gather_regs<PQ, S::THREADS>(hq, a.Qin + (size_t)c0 * NQ, ncl * NQ);
+        gather_regs<PX, S::THREADS>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
+        scatter_lds<PQ, S::THREADS>(sQ, hq, ncl * NQ);
+        scatter_lds<PX, S::THREADS>(sX, hx, ncl * 8);
     }
     __syncthreads();
 
@@ -415,12 +441,20 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = a.c_first + blk * CPB;
     const int ncl = min(CPB, a.ncell - c0);
-    for (int i = tid; i < ncl * NP; i += blockDim.x) {
-        int c = i / NP, k = i % NP;
+    {
+        constexpr int PV = per_thread<CPB * NP, S::THREADS>(), PX = per_thread<CPB * 8, S::THREADS>();
+        double hv[NVAR][PV], hx[PX];
 #pragma unroll
-        for (int w = 0; w < NVAR; w++) sV[w][i] = a.Q[(size_t)(c0 + c) * NQ + (VAR0 + w) * NP + k];
+        for (int k = 0; k < PV; k++) {
+            const int i = tid + k * S::THREADS, c = i / NP, p = i % NP;
+#pragma unroll
+            for (int w = 0; w < NVAR; w++) hv[w][k] = i < ncl * NP ? a.Q[(size_t)(c0 + c) * NQ + (VAR0 + w) * NP + p] : 0.0;
+        }
+        gather_regs<PX, S::THREADS>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
+#pragma unroll
+        for (int w = 0; w < NVAR; w++) scatter_lds<PV, S::THREADS>(sV[w], hv[w], ncl * NP);
+        scatter_lds<PX, S::THREADS>(sX, hx, ncl * 8);
     }
-    for (int i = tid; i < ncl * 8; i += blockDim.x) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
     __syncthreads();
     const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
     const bool pact = (tid < CPB * NP) && (pc < ncl);

@@ -697,7 +697,12 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
             for (int i = tid; i < v.ninst; i += T) v.eig_next[i] = 0.0;
     }
     stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, U, sBar, sRef, sD, sAz, 8);
-    for (int i = tid; i < ncl * 16 * NP; i += T) sG[i] = v.gsol_r<float>()[(size_t)c0 * 16 * NP + i];
+    {
+        constexpr int PG = per_thread<CPB * 16 * NP, T>();
+        float hg[PG];
+        gather_regs<PG, T>(hg, v.gsol_r<float>() + (size_t)c0 * 16 * NP, ncl * 16 * NP);
+        scatter_lds<PG, T>(sG, hg, ncl * 16 * NP);
+    }
     __syncthreads();
     const float gam = float(a.gam);
     const int grp = tid >= TS::G1 ? 1 : 0;

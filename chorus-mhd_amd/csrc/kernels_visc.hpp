@@ -128,7 +128,12 @@ __global__ void __launch_bounds__(256) k_trace(int ncell, const double *Q, doubl
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = blk * CPB;
     const int ncl = min(CPB, ncell - c0);
-    for (int i = threadIdx.x; i < ncl * NQ; i += blockDim.x) sQ[i] = Q[(size_t)c0 * NQ + i];
+    {
+        constexpr int PQ = per_thread<CPB * NQ, S::THREADS>();
+        double hq[PQ];
+        gather_regs<PQ, S::THREADS>(hq, Q + (size_t)c0 * NQ, ncl * NQ);
+        scatter_lds<PQ, S::THREADS>(sQ, hq, ncl * NQ);
+    }
     __syncthreads();
     store_traces<N>(sQ, ncl, c0, TQ);
     if (sm) av_sensor<N, CPB>(sm, sQ, sAV, ncl, c0);
@@ -153,8 +158,14 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = blk * CPB;
     const int ncl = min(CPB, a.ncell - c0);
-    for (int i = tid; i < ncl * NQ; i += blockDim.x) sQ[i] = a.Qin[(size_t)c0 * NQ + i];
-    for (int i = tid; i < ncl * 8; i += blockDim.x) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
+    {
+        constexpr int PQ = per_thread<CPB * NQ, T::THREADS>(), PX = per_thread<CPB * 8, T::THREADS>();
+        double hq[PQ], hx[PX];
+        gather_regs<PQ, T::THREADS>(hq, a.Qin + (size_t)c0 * NQ, ncl * NQ);
+        gather_regs<PX, T::THREADS>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
+        scatter_lds<PQ, T::THREADS>(sQ, hq, ncl * NQ);
+        scatter_lds<PX, T::THREADS>(sX, hx, ncl * 8);
+    }
     __syncthreads();
     const int grp = tid >= T::G1 ? 1 : 0;
     const int pt = tid - grp * T::G1;
@@ -420,9 +431,18 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = blk * CPB;
     const int ncl = min(CPB, a.ncell - c0);
-    for (int i = tid; i < ncl * NQ; i += blockDim.x) sQ[i] = a.Qin[(size_t)c0 * NQ + i];
-    for (int i = tid; i < ncl * 8; i += blockDim.x) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
-    for (int i = tid; i < ncl * 16 * NP; i += blockDim.x) sG[i] = v.gsol_r<R>()[(size_t)c0 * 16 * NP + i];
+    {
+        constexpr int PQ = per_thread<CPB * NQ, T::THREADS>(), PX = per_thread<CPB * 8, T::THREADS>();
+        constexpr int PG = per_thread<CPB * 16 * NP, T::THREADS>();
+        double hq[PQ], hx[PX];
+        R hg[PG];
+        gather_regs<PQ, T::THREADS>(hq, a.Qin + (size_t)c0 * NQ, ncl * NQ);
+        gather_regs<PX, T::THREADS>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
+        gather_regs<PG, T::THREADS>(hg, v.gsol_r<R>() + (size_t)c0 * 16 * NP, ncl * 16 * NP);
+        scatter_lds<PQ, T::THREADS>(sQ, hq, ncl * NQ);
+        scatter_lds<PX, T::THREADS>(sX, hx, ncl * 8);
+        scatter_lds<PG, T::THREADS>(sG, hg, ncl * 16 * NP);
+    }
     if constexpr ((VM & 2) != 0) {
         // face averages of muAV(1) (compAVflux/interfaceAVflux/cyclocAVflux,
         // AV.f90:199-303, 494-564), evaluated by k_face_visc
