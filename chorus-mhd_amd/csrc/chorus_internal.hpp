@@ -130,6 +130,12 @@ struct CTArgs {
 template <class R>
 __device__ __forceinline__ void stage_combine(const StageArgs &a, size_t base, int NP, double *qs,
                                               const R *res, const double *qa = nullptr) {
+    // without a prefetched Qa, the nine loads are issued together before any
+    // combination (one memory latency instead of nine)
+    double qv[9];
+    const bool need_a = a.kind != KIND_RESID && a.kind != KIND_AXPY;
+#pragma unroll
+    for (int v = 0; v < 9; v++) qv[v] = qa ? qa[v] : (need_a ? a.Qa[base + (size_t)v * NP] : 0.0);
 #pragma unroll
     for (int v = 0; v < 9; v++) {
         const double q = qs[v * NP];
@@ -139,10 +145,10 @@ __device__ __forceinline__ void stage_combine(const StageArgs &a, size_t base, i
         switch (a.kind) {
         case KIND_RESID: a.Rout[o] = r; continue;
         case KIND_AXPY: qn = q + a.c0 * r; break;
-        case KIND_3A: qn = a.c1 * (qa ? qa[v] : a.Qa[o]) + a.c2 * q + a.c0 * r; break;
-        case KIND_3B: qn = a.c1 * (qa ? qa[v] : a.Qa[o]) + a.c2 * q + a.c3 * r * a.c4; break;
-        case KIND_S54_4: qn = a.c1 * (qa ? qa[v] : a.Qa[o]) + a.c2 * q + a.c0 * r; a.Rout[o] = r; break;
-        default: qn = a.c1 * (qa ? qa[v] : a.Qa[o]) + a.c2 * a.Qb[o] + a.c3 * a.Qc[o] + a.c4 * q + a.c5 * a.Rprev[o] + a.c0 * r; break;
+        case KIND_3A: qn = a.c1 * qv[v] + a.c2 * q + a.c0 * r; break;
+        case KIND_3B: qn = a.c1 * qv[v] + a.c2 * q + a.c3 * r * a.c4; break;
+        case KIND_S54_4: qn = a.c1 * qv[v] + a.c2 * q + a.c0 * r; a.Rout[o] = r; break;
+        default: qn = a.c1 * qv[v] + a.c2 * a.Qb[o] + a.c3 * a.Qc[o] + a.c4 * q + a.c5 * a.Rprev[o] + a.c0 * r; break;
         }
         a.Qout[o] = qn;
         qs[v * NP] = qn;
