@@ -81,7 +81,7 @@ __device__ __forceinline__ void gather_regs(T *v, const T *src, int n) {
 #pragma unroll
     for (int k = 0; k < PER; k++) {
         const int i = (int)threadIdx.x + k * THREADS;
-        v[k] = i < n ? src[i] : T(0);
+        v[k] = i < n ? src[i] : T{};
     }
 }
 template <int PER, int THREADS, class T>
@@ -437,6 +437,7 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
     __shared__ double sF[2][CPB * NF];   // central flux-point values, one family at a time
     __shared__ double sS[2][CPB * NF];   // metric rows, same layout
     __shared__ double red[2][256];
+    __shared__ int2 sNbr[CPB * 4];
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = a.c_first + blk * CPB;
@@ -450,10 +451,14 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
 #pragma unroll
             for (int w = 0; w < NVAR; w++) hv[w][k] = i < ncl * NP ? a.Q[(size_t)(c0 + c) * NQ + (VAR0 + w) * NP + p] : 0.0;
         }
+        constexpr int PN = per_thread<CPB * 4, S::THREADS>();
+        int2 hn[PN];
         gather_regs<PX, S::THREADS>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
+        gather_regs<PN, S::THREADS>(hn, a.nbr + (size_t)c0 * 4, ncl * 4);
 #pragma unroll
         for (int w = 0; w < NVAR; w++) scatter_lds<PV, S::THREADS>(sV[w], hv[w], ncl * NP);
         scatter_lds<PX, S::THREADS>(sX, hx, ncl * 8);
+        scatter_lds<PN, S::THREADS>(sNbr, hn, ncl * 4);
     }
     __syncthreads();
     const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
@@ -472,7 +477,7 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
             if (r == 0 || r == N) {
                 if (fam == 0) { lfi = (r == 0) ? 3 : 1; nfp = (r == 0) ? (N - 1 - line) : line; }
                 else          { lfi = (r == 0) ? 0 : 2; nfp = (r == 0) ? line : (N - 1 - line); }
-                info = a.nbr[(size_t)cell * 4 + lfi];
+                info = sNbr[c * 4 + lfi];
             }
 #pragma unroll
             for (int w = 0; w < NVAR; w++) {

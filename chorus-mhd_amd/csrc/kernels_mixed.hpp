@@ -385,8 +385,15 @@ __global__ void __launch_bounds__((CellShape<N, TB>::THREADS)) __attribute__((am
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = blk * CPB;
     const int ncl = min(CPB, a.ncell - c0);
-    for (int i = tid; i < ncl * 8; i += T) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
-    stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, U, sBar, sRef, sD, sAz, 8);
+    {
+        // the small per-cell arrays are loaded with the state (one memory latency)
+        constexpr int PX = per_thread<CPB * 8, T>();
+        double hx[PX];
+        gather_regs<PX, T>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
+        stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, U, sBar, sRef, sD, sAz, 8);
+        scatter_lds<PX, T>(sX, hx, ncl * 8);
+        __syncthreads();
+    }
     const float gam = float(a.gam);
     const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
     const bool pact = (tid < CPB * NP) && (pc < ncl);
@@ -477,13 +484,23 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
     __shared__ double sX[CPB * 8];
     __shared__ double sBar[CPB * NV];
     __shared__ Ref sRef[CPB];
+    __shared__ int2 sNbr[CPB * 4];
     float *sD = (float *)U, *sV = sD + CPB * NQ, *sG = sV + CPB * VST;
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = blk * CPB;
     const int ncl = min(CPB, a.ncell - c0);
-    for (int i = tid; i < ncl * 8; i += T) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
-    stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, U, sBar, sRef, sD, nullptr);
+    {
+        constexpr int PX = per_thread<CPB * 8, T>(), PN = per_thread<CPB * 4, T>();
+        double hx[PX];
+        int2 hn[PN];
+        gather_regs<PX, T>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
+        gather_regs<PN, T>(hn, a.nbr + (size_t)c0 * 4, ncl * 4);
+        stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, U, sBar, sRef, sD, nullptr);
+        scatter_lds<PX, T>(sX, hx, ncl * 8);
+        scatter_lds<PN, T>(sNbr, hn, ncl * 4);
+        __syncthreads();
+    }
     const int grp = tid >= TS::G1 ? 1 : 0;
     const int pt = tid - grp * TS::G1;
     const int pc = pt / NP, pk = pt % NP, pjs = pk / N, pis = pk % N;
@@ -508,7 +525,7 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
                     int lfi, nfp;
                     face_of<N>(fam, r, line, lfi, nfp);
                     const int cell = c0 + c;
-                    const int2 info = a.nbr[(size_t)cell * 4 + lfi];
+                    const int2 info = sNbr[c * 4 + lfi];
                     if (!((info.y >> 4) & 1)) {
                         const double *own = a.TQ + (((size_t)cell * 4 + lfi) * N + nfp) * NV;
                         const double *oth = a.TQ + (((size_t)info.x * 4 + (info.y & 7) - 1) * N + partner_nfp<N>(nfp)) * NV;
@@ -690,18 +707,23 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = blk * CPB;
     const int ncl = min(CPB, a.ncell - c0);
-    for (int i = tid; i < ncl * 8; i += T) sX[i] = a.cellxy[(size_t)c0 * 8 + i];
     if constexpr ((VM & 2) != 0) {
-        for (int i = tid; i < ncl * 4; i += T) sMu[i] = v.mu4[(size_t)c0 * 4 + i];
         if (blockIdx.x == 0)
             for (int i = tid; i < v.ninst; i += T) v.eig_next[i] = 0.0;
     }
-    stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, U, sBar, sRef, sD, sAz, 8);
     {
-        constexpr int PG = per_thread<CPB * 16 * NP, T>();
+        // the per-cell arrays are loaded with the state (one memory latency)
+        constexpr int PG = per_thread<CPB * 16 * NP, T>(), PX = per_thread<CPB * 8, T>();
+        constexpr int PM = per_thread<CPB * 4, T>();
         float hg[PG];
+        double hx[PX], hm[PM];
         gather_regs<PG, T>(hg, v.gsol_r<float>() + (size_t)c0 * 16 * NP, ncl * 16 * NP);
+        gather_regs<PX, T>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
+        if constexpr ((VM & 2) != 0) gather_regs<PM, T>(hm, v.mu4 + (size_t)c0 * 4, ncl * 4);
+        stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, U, sBar, sRef, sD, sAz, 8);
         scatter_lds<PG, T>(sG, hg, ncl * 16 * NP);
+        scatter_lds<PX, T>(sX, hx, ncl * 8);
+        if constexpr ((VM & 2) != 0) scatter_lds<PM, T>(sMu, hm, ncl * 4);
     }
     __syncthreads();
     const float gam = float(a.gam);

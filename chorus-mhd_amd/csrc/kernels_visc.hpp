@@ -154,17 +154,22 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
     __shared__ double sX[CPB * 8];
     __shared__ R sV[CPB * VST];          // [c][v | s1 | s2][line][r], one family at a time
     __shared__ R sG[CPB * 16 * NP];      // nablaQs [c][d*8+v][p]
+    __shared__ int2 sNbr[CPB * 4];       // face neighbours of the block's cells
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = blk * CPB;
     const int ncl = min(CPB, a.ncell - c0);
     {
         constexpr int PQ = per_thread<CPB * NQ, T::THREADS>(), PX = per_thread<CPB * 8, T::THREADS>();
+        constexpr int PN = per_thread<CPB * 4, T::THREADS>();
         double hq[PQ], hx[PX];
+        int2 hn[PN];
         gather_regs<PQ, T::THREADS>(hq, a.Qin + (size_t)c0 * NQ, ncl * NQ);
         gather_regs<PX, T::THREADS>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
+        gather_regs<PN, T::THREADS>(hn, a.nbr + (size_t)c0 * 4, ncl * 4);
         scatter_lds<PQ, T::THREADS>(sQ, hq, ncl * NQ);
         scatter_lds<PX, T::THREADS>(sX, hx, ncl * 8);
+        scatter_lds<PN, T::THREADS>(sNbr, hn, ncl * 4);
     }
     __syncthreads();
     const int grp = tid >= T::G1 ? 1 : 0;
@@ -203,7 +208,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
                 metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
                 int lfi, nfp;
                 face_of<N>(fam, r, line, lfi, nfp);
-                const int2 info = a.nbr[(size_t)cell * 4 + lfi];
+                const int2 info = sNbr[c * 4 + lfi];
                 const double *own = a.TQ + (((size_t)cell * 4 + lfi) * N + nfp) * NV;
                 const double *oth = ((info.y >> 4) & 1) ? own
                                     : a.TQ + (((size_t)info.x * 4 + (info.y & 7) - 1) * N + partner_nfp<N>(nfp)) * NV;
@@ -433,20 +438,21 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
     const int ncl = min(CPB, a.ncell - c0);
     {
         constexpr int PQ = per_thread<CPB * NQ, T::THREADS>(), PX = per_thread<CPB * 8, T::THREADS>();
-        constexpr int PG = per_thread<CPB * 16 * NP, T::THREADS>();
-        double hq[PQ], hx[PX];
+        constexpr int PG = per_thread<CPB * 16 * NP, T::THREADS>(), PM = per_thread<CPB * 4, T::THREADS>();
+        double hq[PQ], hx[PX], hm[PM];
         R hg[PG];
         gather_regs<PQ, T::THREADS>(hq, a.Qin + (size_t)c0 * NQ, ncl * NQ);
         gather_regs<PX, T::THREADS>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
         gather_regs<PG, T::THREADS>(hg, v.gsol_r<R>() + (size_t)c0 * 16 * NP, ncl * 16 * NP);
+        // face averages of muAV(1) (compAVflux/interfaceAVflux/cyclocAVflux,
+        // AV.f90:199-303, 494-564), evaluated by k_face_visc
+        if constexpr ((VM & 2) != 0) gather_regs<PM, T::THREADS>(hm, v.mu4 + (size_t)c0 * 4, ncl * 4);
         scatter_lds<PQ, T::THREADS>(sQ, hq, ncl * NQ);
         scatter_lds<PX, T::THREADS>(sX, hx, ncl * 8);
         scatter_lds<PG, T::THREADS>(sG, hg, ncl * 16 * NP);
+        if constexpr ((VM & 2) != 0) scatter_lds<PM, T::THREADS>(sMu, hm, ncl * 4);
     }
     if constexpr ((VM & 2) != 0) {
-        // face averages of muAV(1) (compAVflux/interfaceAVflux/cyclocAVflux,
-        // AV.f90:199-303, 494-564), evaluated by k_face_visc
-        for (int i = tid; i < ncl * 4; i += blockDim.x) sMu[i] = v.mu4[(size_t)c0 * 4 + i];
         if (blockIdx.x == 0)
             for (int i = tid; i < v.ninst; i += blockDim.x) v.eig_next[i] = 0.0;
     }
