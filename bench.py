@@ -303,6 +303,7 @@ def main():
         s.close()
         s = Solver(mesh, N, dt, gam=w["gam"], artiv=artiv, precision=precision, topo=topo)
         s.set_state(Q0)
+        tile = s.autotune() if precision == "mixed" else None
         if args.warmup:
             s.step(args.warmup)
             s.set_state(Q0)
@@ -312,7 +313,7 @@ def main():
         Q = s.get_state()
         d = s.diag()
         s.close()
-        return dict(mesh=mesh, dt=dt, wall=wall, tm=tm, Q=Q, Q0=Q0, diag=d, artiv=artiv)
+        return dict(mesh=mesh, dt=dt, wall=wall, tm=tm, Q=Q, Q0=Q0, diag=d, artiv=artiv, tile=tile)
 
     r64 = run("fp64")
     mesh = r64["mesh"]
@@ -361,6 +362,7 @@ def main():
             "roofline_frac": ndof * b_stage(N) / (rm["tm"]["ms_stage_avg"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "traffic": MEASURED_STAGE_TRAFFIC.get((args.workload, "mixed")),
             "divb_max": rm["diag"]["divbmax"],
+            "tile_shape": ["256/128", "128/64", "512/256"][rm["tile"]],
         }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, r64["artiv"])

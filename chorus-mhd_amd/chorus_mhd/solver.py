@@ -168,6 +168,13 @@ class Solver:
         return dict(zip(["residglob", "divbglob", "divbmax", "eigvmax", "epsilon0"],
                         [x.value for x in v]))
 
+    def autotune(self) -> int:
+        """Pick the fastest tile shape of the mixed-form cell kernels on this
+        mesh and state (no state change); returns its index."""
+        t = C.c_int()
+        self._check(_lib.load().chorus_autotune(self._ctx, C.byref(t)), "chorus_autotune")
+        return t.value
+
     def time_steps(self, nsteps: int) -> dict:
         tot, st, ct = C.c_double(), C.c_double(), C.c_double()
         self._check(_lib.load().chorus_time_steps(self._ctx, nsteps, C.byref(tot), C.byref(st),

@@ -1076,6 +1076,52 @@ int chorus_batch_group(const chorus_batch *b, int g, int *N, int *ninst, int *nc
     return 0;
 }
 
+}  // extern "C"
+
+namespace {
+// times KIND_RESID stage launches (the cell kernels of a stage without the
+// state update) of context c for every compiled tile shape of the mixed form
+// and keeps the fastest; the FP64 form has one shape
+int autotune_ctx(chorus_ctx *c, hipEvent_t t0, hipEvent_t t1, int *tile) {
+    int best = 0, rc = 0;
+    if (c->p.precision == CHORUS_MIXED) {
+        float best_ms = 1e30f;
+        for (int t = 0; t < tile_count() && !rc; t++) {
+            c->tile = t;
+            StageArgs s = base_args(c);
+            s.Qin = c->X[c->cur];
+            s.Rout = c->R;
+            s.kind = KIND_RESID;
+            if ((rc = launch_stage(c, s))) break;   // warm-up
+            if (hipEventRecord(t0, c->stream) != hipSuccess) return CHORUS_ERR_HIP;
+            for (int r = 0; r < 3 && !rc; r++) rc = launch_stage(c, s);
+            if (hipEventRecord(t1, c->stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess)
+                return CHORUS_ERR_HIP;
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, t0, t1) != hipSuccess) return CHORUS_ERR_HIP;
+            if (!rc && ms < best_ms) { best_ms = ms; best = t; }
+        }
+    }
+    c->tile = best;
+    if (tile) *tile = best;
+    return rc;
+}
+}  // namespace
+
+extern "C" {
+
+int chorus_autotune(chorus_ctx *c, int *tile) {
+    if (!c) return CHORUS_ERR_ARG;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    hipEvent_t t0, t1;
+    HIP_TRY(c, hipEventCreate(&t0));
+    HIP_TRY(c, hipEventCreate(&t1));
+    const int rc = autotune_ctx(c, t0, t1, tile);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    return rc ? fail(c, rc, "autotune failed") : 0;
+}
+
 int chorus_batch_autotune(chorus_batch *b, int *tiles) {
     if (!b) return CHORUS_ERR_ARG;
     int rc = refresh(b);
@@ -1085,31 +1131,8 @@ int chorus_batch_autotune(chorus_batch *b, int *tiles) {
     hipEvent_t t0, t1;
     HIP_TRYB(b, hipEventCreate(&t0));
     HIP_TRYB(b, hipEventCreate(&t1));
-    for (size_t g = 0; g < b->groups.size(); g++) {
-        chorus_ctx *c = b->groups[g];
-        int best = 0;
-        if (c->p.precision == CHORUS_MIXED) {
-            float best_ms = 1e30f;
-            for (int t = 0; t < tile_count(); t++) {
-                c->tile = t;
-                StageArgs s = base_args(c);
-                s.Qin = c->X[c->cur];
-                s.Rout = c->R;
-                s.kind = KIND_RESID;
-                if ((rc = launch_stage(c, s))) break;   // warm-up
-                (void)hipEventRecord(t0, c->stream);
-                for (int r = 0; r < 3 && !rc; r++) rc = launch_stage(c, s);
-                (void)hipEventRecord(t1, c->stream);
-                (void)hipEventSynchronize(t1);
-                float ms = 0.f;
-                (void)hipEventElapsedTime(&ms, t0, t1);
-                if (!rc && ms < best_ms) { best_ms = ms; best = t; }
-            }
-        }
-        c->tile = best;
-        if (tiles) tiles[g] = best;
-        if (rc) break;
-    }
+    for (size_t g = 0; g < b->groups.size() && !rc; g++)
+        rc = autotune_ctx(b->groups[g], t0, t1, tiles ? tiles + g : nullptr);
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     return rc ? bfail(b, rc, "autotune failed") : 0;

@@ -113,6 +113,14 @@ int chorus_topology(int ncell, int nvert, const int *ivcell, const double *xv, c
 int chorus_time_steps(chorus_ctx *ctx, int nsteps, double *ms_total, double *ms_stage_avg,
                       double *ms_ct_avg);
 
+/* Tile-shape autotuner (no reference counterpart): times the mixed form's
+ * cell kernels in every compiled tile shape on this context's own mesh and
+ * state (residual evaluations, no state change), keeps the fastest for later
+ * steps and returns its index in *tile (0: 256/128, 1: 128/64, 2: 512/256
+ * threads of the ideal/viscous cell kernels).  The FP64 form has one shape
+ * (*tile = 0). */
+int chorus_autotune(chorus_ctx *ctx, int *tile);
+
 /* ---- Batched ensembles (no reference counterpart: the reference runs one
  * mesh per MPI job; the loop of main.f90:44-50 / iterations.f90:18-70 is what
  * each instance executes).  ninst independent instances, each with its own
