@@ -3,7 +3,8 @@
 //
 // Plain FP32 flux values of size |F| lose log10(N^2/h) digits when the
 // divergence differences them over a cell.  Here every FP32 quantity on a flux
-// point is a deviation from the cell's FP64 mean state Qbar:
+// point is a deviation from a FP64 reference state Qbar of the cell (its state
+// at the central solution point, so deviations are O(h)):
 //   dQf   = sum_sp (Q(sp) - Qbar) L(r,sp)                 (FP64 difference, FP32 MAC)
 //   dF    = F(Qbar + dQf) - F(Qbar)                       (exact difference formulas)
 //   dF1   = dF*S11 + dG*S12                               (contravariant)
@@ -47,10 +48,13 @@ struct Ref {
     float b2, p, H, ub;   // |B|^2, pressure, E + p + |B|^2/2, u.B
 };
 
+// (the derived quantities only need FP32 consistency with Qbar: an error in
+// them multiplies a deviation in the exact-difference fluxes)
 __device__ __forceinline__ void ref_from(const double *qbar, double gam, Ref &s) {
 #pragma unroll
     for (int v = 0; v < NV; v++) s.q[v] = float(qbar[v]);
-    const double ux = qbar[1] / qbar[0], uy = qbar[2] / qbar[0], uz = qbar[3] / qbar[0];
+    const double ir = 1.0 / qbar[0];
+    const double ux = qbar[1] * ir, uy = qbar[2] * ir, uz = qbar[3] * ir;
     const double b2 = qbar[5] * qbar[5] + qbar[6] * qbar[6] + qbar[7] * qbar[7];
     const double p = (gam - 1.0) * (qbar[4] - 0.5 * (qbar[1] * ux + qbar[2] * uy + qbar[3] * uz) - 0.5 * b2);
     s.u[0] = float(ux); s.u[1] = float(uy); s.u[2] = float(uz);
@@ -58,22 +62,6 @@ __device__ __forceinline__ void ref_from(const double *qbar, double gam, Ref &s)
     s.p = float(p);
     s.H = float(qbar[4] + p + 0.5 * b2);
     s.ub = float(ux * qbar[5] + uy * qbar[6] + uz * qbar[7]);
-}
-
-// the block's cell means (FP64, one thread per cell and variable) and their
-// reference states; called by all threads of the block
-template <int N>
-__device__ __forceinline__ void block_refs(const double *sQ, int ncl, double gam, double *sBar, Ref *sRef) {
-    constexpr int NP = N * N, NQ = NV * NP;
-    for (int i = threadIdx.x; i < ncl * NV; i += blockDim.x) {
-        const double *q = sQ + (i / NV) * NQ + (i % NV) * NP;
-        double a = 0.0;
-        for (int p = 0; p < NP; p++) a += q[p];
-        sBar[i] = a / NP;
-    }
-    __syncthreads();
-    for (int c = threadIdx.x; c < ncl; c += blockDim.x) ref_from(sBar + c * NV, gam, sRef[c]);
-    __syncthreads();
 }
 
 // F(Qbar+d) - F(Qbar), G(Qbar+d) - G(Qbar) for the ideal-MHD fluxes of
@@ -133,32 +121,30 @@ __device__ __forceinline__ float wave_speed(const float *Ql, const float *Qr, fl
 }
 
 // Phase A of the mixed cell kernels: the block's cells (contiguous in HBM)
-// become FP32 deviations from their FP64 means.  Each thread keeps the values
-// it loaded in registers; tmp (at least ncl*NQ doubles of LDS) holds the FP64
-// copy only until the means are taken, so dq may alias it.  keep, if given,
-// receives the FP64 plane of variable keep_var of each cell.
+// become FP32 deviations from their reference states Qbar (the FP64 state at
+// the central solution point).  Each thread keeps the values it loaded in
+// registers; the threads holding a central point publish it as Qbar, and keep,
+// if given, receives the FP64 plane of variable keep_var of each cell.
 // Called by all THREADS threads of the block.
 template <int N, int CPB, int THREADS>
-__device__ __forceinline__ void stage_deviations(const double *Qg, int ncl, double gam, double *tmp, double *sBar,
-                                                 Ref *sRef, float *dq, double *keep, int keep_var = 0) {
+__device__ __forceinline__ void stage_deviations(const double *Qg, int ncl, double gam, double *sBar, Ref *sRef,
+                                                 float *dq, double *keep, int keep_var = 0) {
     constexpr int NP = N * N, NQ = NV * NP, PER = (CPB * NQ + THREADS - 1) / THREADS;
+    constexpr int PC = (N / 2) * N + N / 2;   // central solution point
     const int tid = threadIdx.x, n = ncl * NQ;
     double held[PER];
 #pragma unroll
     for (int k = 0; k < PER; k++) {
         const int i = tid + k * THREADS;
         held[k] = i < n ? Qg[i] : 0.0;
-        if (i < n) tmp[i] = held[k];
     }
-    __syncthreads();
-    for (int i = tid; i < ncl * NV; i += THREADS) {
-        const double *q = tmp + (i / NV) * NQ + (i % NV) * NP;
-        double s = 0.0;
-        for (int p = 0; p < NP; p++) s += q[p];
-        sBar[i] = s / NP;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = tid + k * THREADS;
+        const int c = i / NQ, r = i % NQ, v = r / NP, p = r % NP;
+        if (i < n && p == PC) sBar[c * NV + v] = held[k];
+        if (keep && i < n && v == keep_var) keep[c * NP + p] = held[k];
     }
-    if (keep)
-        for (int i = tid; i < ncl * NP; i += THREADS) keep[i] = tmp[(i / NP) * NQ + keep_var * NP + i % NP];
     __syncthreads();
     for (int c = tid; c < ncl; c += THREADS) ref_from(sBar + c * NV, gam, sRef[c]);
 #pragma unroll
@@ -390,7 +376,7 @@ __global__ void __launch_bounds__((CellShape<N, TB>::THREADS)) __attribute__((am
         constexpr int PX = per_thread<CPB * 8, T>();
         double hx[PX];
         gather_regs<PX, T>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
-        stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, U, sBar, sRef, sD, sAz, 8);
+        stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, sBar, sRef, sD, sAz, 8);
         scatter_lds<PX, T>(sX, hx, ncl * 8);
         __syncthreads();
     }
@@ -467,7 +453,7 @@ __global__ void __launch_bounds__((CellShape<N, TB>::THREADS)) __attribute__((am
 // ---------------------------------------------------------------------------
 // Viscous / AV path, mixed form: the launches of the FP64 path, two threads
 // per solution point as there.  The central Qv of the gradient pass is a
-// deviation from the cell mean (own delta trace plus half the jump of the two
+// deviation from the cell's reference state (own delta trace plus half the jump of the two
 // FP64 traces at face points); nablaQs, its traces and the viscous fluxes are
 // plain FP32 (they carry no large constant part); the viscous face flux is
 // evaluated once per face point from the FP64 trace average.
@@ -496,7 +482,7 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
         int2 hn[PN];
         gather_regs<PX, T>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
         gather_regs<PN, T>(hn, a.nbr + (size_t)c0 * 4, ncl * 4);
-        stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, U, sBar, sRef, sD, nullptr);
+        stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, sBar, sRef, sD, nullptr);
         scatter_lds<PX, T>(sX, hx, ncl * 8);
         scatter_lds<PN, T>(sNbr, hn, ncl * 4);
         __syncthreads();
@@ -720,7 +706,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
         gather_regs<PG, T>(hg, v.gsol_r<float>() + (size_t)c0 * 16 * NP, ncl * 16 * NP);
         gather_regs<PX, T>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
         if constexpr ((VM & 2) != 0) gather_regs<PM, T>(hm, v.mu4 + (size_t)c0 * 4, ncl * 4);
-        stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, U, sBar, sRef, sD, sAz, 8);
+        stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, sBar, sRef, sD, sAz, 8);
         scatter_lds<PG, T>(sG, hg, ncl * 16 * NP);
         scatter_lds<PX, T>(sX, hx, ncl * 8);
         if constexpr ((VM & 2) != 0) scatter_lds<PM, T>(sMu, hm, ncl * 4);
