@@ -20,7 +20,7 @@ int tile_count() { return 3; }
 
 template <int n, int TBI>
 static hipError_t ideal_stage(const StageArgs &a, const FaceArgs &f, hipStream_t s) {
-    using S = CellShape<n, TBI>;
+    using S = FluxCellShape<n, TBI>;
     hipLaunchKernelGGL((k_face_inv_mx<n>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);
     hipLaunchKernelGGL((k_stage_ideal_mx<n, TBI>), dim3(blocks_of<S>(a.ncell)), dim3(S::THREADS), 0, s, a);
     return hipGetLastError();

@@ -56,6 +56,14 @@ template <int N, int TB> struct CellShape {
     static constexpr int THREADS = ((CPB * NP + 63) / 64) * 64;
 };
 template <int N> using Shape = CellShape<N, 256>;
+// a block of whole cells with one thread per flux point of a family (N(N+1)
+// per cell): a kernel whose flux phase evaluates every flux point runs it in
+// one round; its derivative phase uses the first CPB*N^2 threads
+template <int N, int TB> struct FluxCellShape {
+    static constexpr int NP = N * N, NF = N * (N + 1), NQ = NV * NP;
+    static constexpr int CPB = (TB / NF) > 0 ? (TB / NF) : 1;
+    static constexpr int THREADS = ((CPB * NF + 63) / 64) * 64;
+};
 #ifndef CHORUS_IDEAL_TB
 #define CHORUS_IDEAL_TB 256
 #endif

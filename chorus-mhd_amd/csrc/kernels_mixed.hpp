@@ -344,17 +344,18 @@ __global__ void __launch_bounds__(256) k_face_inv_mx(FaceArgs a) {
 // ---------------------------------------------------------------------------
 // Ideal-path stage kernel, mixed form: every flux point evaluates the delta
 // flux of its own trace, face points add the jump terms of k_face_inv_mx; the
-// residual is formed in FP32 and combined into the FP64 state.  LDS holds the
-// FP32 deviations and flux planes only (one union area, reused for the FP64
-// copy at set-up and for the new state at the end).
+// residual is formed in FP32 (A_z in FP64) and combined into the FP64 state.
+// One thread per flux point of a family (FluxCellShape), so the flux phase is
+// one round.  LDS holds the FP32 deviations and flux planes (one union area,
+// reused for the new state at the end) and the FP64 A_z values.
 // ---------------------------------------------------------------------------
 // 4 waves per SIMD (<= 128 VGPRs) where that needs no spills
 #ifndef CHORUS_MX_IDEAL_MINW
 #define CHORUS_MX_IDEAL_MINW(N) ((N) <= 5 ? 4 : 3)
 #endif
 template <int N, int TB>
-__global__ void __launch_bounds__((CellShape<N, TB>::THREADS)) __attribute__((amdgpu_waves_per_eu(CHORUS_MX_IDEAL_MINW(N)))) k_stage_ideal_mx(StageArgs a) {
-    using S = CellShape<N, TB>;
+__global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__((amdgpu_waves_per_eu(CHORUS_MX_IDEAL_MINW(N)))) k_stage_ideal_mx(StageArgs a) {
+    using S = FluxCellShape<N, TB>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ, T = S::THREADS;
     constexpr int FST = 8 * NF + 1;      // per-cell stride of the dF1 planes
     constexpr int AST = NF + 1;          // per-cell stride of the FP64 A_z flux-point values
