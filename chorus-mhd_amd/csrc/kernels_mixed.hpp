@@ -155,6 +155,23 @@ __device__ __forceinline__ void stage_deviations(const double *Qg, int ncl, doub
     __syncthreads();
 }
 
+// FP32 extrapolation and derivative operators of order N staged in LDS
+// (the per-thread indices of the contractions would otherwise make every
+// coefficient a vector load from the constant tables): gather before the
+// block's first barrier, scatter after it
+template <int N, int THREADS> struct OpTables {
+    static constexpr int K = (N + 1) * N, PO = (K + THREADS - 1) / THREADS;
+    float l[PO], m[PO];
+    __device__ __forceinline__ void gather() {
+        gather_regs<PO, THREADS>(l, &c_Lf[N][0], K);
+        gather_regs<PO, THREADS>(m, &c_Mf[N][0], K);
+    }
+    __device__ __forceinline__ void scatter(float *sL, float *sM) const {
+        scatter_lds<PO, THREADS>(sL, l, K);
+        scatter_lds<PO, THREADS>(sM, m, K);
+    }
+};
+
 // SSP stage combination at one solution point from register copies of the
 // point's state q and (when the stage needs it) Qa; the new state is also
 // written to qout[var*NP] (LDS) for the traces
@@ -191,13 +208,13 @@ __device__ __forceinline__ void prefetch_point(const StageArgs &a, size_t base, 
 
 // delta trace: sum_sp dq(sp) L(r,sp) of all 9 variables
 template <int N>
-__device__ __forceinline__ void dtrace9(const float *dq, int fam, int ifp, int jfp, float *d) {
+__device__ __forceinline__ void dtrace9(const float *dq, int fam, int ifp, int jfp, float *d, const float *Lf) {
     constexpr int NP = N * N;
 #pragma unroll
     for (int v = 0; v < NV; v++) d[v] = 0.f;
 #pragma unroll CHORUS_MX_TRACE_UNROLL
     for (int sp = 0; sp < N; sp++) {
-        const float l = fam == 0 ? c_Lf[N][ifp * N + sp] : c_Lf[N][jfp * N + sp];
+        const float l = fam == 0 ? Lf[ifp * N + sp] : Lf[jfp * N + sp];
         const int idx = fam == 0 ? jfp * N + sp : sp * N + ifp;
 #pragma unroll
         for (int v = 0; v < NV; v++) d[v] += dq[v * NP + idx] * l;
@@ -367,6 +384,7 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
     __shared__ Ref sRef[CPB];
     __shared__ double sAz[CPB * NP];     // FP64 A_z plane of each cell
     __shared__ double sA[2 * CPB * AST]; // FP64 A_z at the flux points of both families
+    __shared__ float sLf[(N + 1) * N], sMf[(N + 1) * N];
     float *sD = (float *)U, *sF = sD + CPB * NQ;
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -376,9 +394,12 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
         // the small per-cell arrays are loaded with the state (one memory latency)
         constexpr int PX = per_thread<CPB * 8, T>();
         double hx[PX];
+        OpTables<N, T> ops;
         gather_regs<PX, T>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
+        ops.gather();
         stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, sBar, sRef, sD, sAz, 8);
         scatter_lds<PX, T>(sX, hx, ncl * 8);
+        ops.scatter(sLf, sMf);
         __syncthreads();
     }
     const float gam = float(a.gam);
@@ -408,7 +429,7 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
             metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
             const float s1 = float(s1d), s2 = float(s2d);
             float d[NV], F[8], G[8];
-            dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d);
+            dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d, sLf);
             dflux(sRef[c], d, gam - 1.f, F, G);
             float *f = sF + c * FST + line * (N + 1) + r;
 #pragma unroll
@@ -426,7 +447,7 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
             for (int v = 0; v < 8; v++) d[v] = 0.f;
 #pragma unroll CHORUS_MX_CUNROLL
             for (int r = 0; r <= N; r++) {
-                const float m = c_Mf[N][r * N + pos];
+                const float m = sMf[r * N + pos];
 #pragma unroll
                 for (int v = 0; v < 8; v++) d[v] += f[v * NF + r] * m;
             }
@@ -472,6 +493,7 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
     __shared__ double sBar[CPB * NV];
     __shared__ Ref sRef[CPB];
     __shared__ int2 sNbr[CPB * 4];
+    __shared__ float sLf[(N + 1) * N], sMf[(N + 1) * N];
     float *sD = (float *)U, *sV = sD + CPB * NQ, *sG = sV + CPB * VST;
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -481,11 +503,14 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
         constexpr int PX = per_thread<CPB * 8, T>(), PN = per_thread<CPB * 4, T>();
         double hx[PX];
         int2 hn[PN];
+        OpTables<N, T> ops;
         gather_regs<PX, T>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
         gather_regs<PN, T>(hn, a.nbr + (size_t)c0 * 4, ncl * 4);
+        ops.gather();
         stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, sBar, sRef, sD, nullptr);
         scatter_lds<PX, T>(sX, hx, ncl * 8);
         scatter_lds<PN, T>(sNbr, hn, ncl * 4);
+        ops.scatter(sLf, sMf);
         __syncthreads();
     }
     const int grp = tid >= TS::G1 ? 1 : 0;
@@ -526,7 +551,7 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
             double s1d, s2d;
             metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
             float d[NV];
-            dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d);
+            dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d, sLf);
             float *q = sV + c * VST + line * (N + 1) + r;
 #pragma unroll
             for (int v = 0; v < 8; v++) q[v * NF] = d[v] + half_jump[v];
@@ -543,7 +568,7 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
             for (int v = 0; v < 8; v++) acc[v] = 0.f;
 #pragma unroll CHORUS_MX_CUNROLL
             for (int r = 0; r <= N; r++) {
-                const float mm = c_Mf[N][r * N + pos] * mq[r];
+                const float mm = sMf[r * N + pos] * mq[r];
 #pragma unroll
                 for (int v = 0; v < 8; v++) acc[v] += q[v * NF + r] * mm;
             }
@@ -569,10 +594,10 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
         float acc = 0.f;
         if (fam == 0) {
 #pragma unroll
-            for (int sp = 0; sp < N; sp++) acc += gq[jfp * N + sp] * c_Lf[N][ifp * N + sp];
+            for (int sp = 0; sp < N; sp++) acc += gq[jfp * N + sp] * sLf[ifp * N + sp];
         } else {
 #pragma unroll
-            for (int sp = 0; sp < N; sp++) acc += gq[sp * N + ifp] * c_Lf[N][jfp * N + sp];
+            for (int sp = 0; sp < N; sp++) acc += gq[sp * N + ifp] * sLf[jfp * N + sp];
         }
         a.gtr_r<float>()[((size_t)(c0 + c) * 4 * N + fpi) * 16 + g] = acc;
     }
@@ -688,6 +713,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     __shared__ double sAV[CPB * NP];
     __shared__ double sAz[CPB * NP];     // FP64 A_z plane of each cell
     __shared__ double sA[2 * CPB * AST]; // FP64 A_z at the flux points of both families
+    __shared__ float sLf[(N + 1) * N], sMf[(N + 1) * N];
     float *sD = (float *)U, *sU = sD + CPB * NQ, *sG = sU + CPB * UST;
     static_assert(9 * CPB * NP <= CPB * UST, "exchange area");
     const int tid = threadIdx.x;
@@ -704,6 +730,8 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
         constexpr int PM = per_thread<CPB * 4, T>();
         float hg[PG];
         double hx[PX], hm[PM];
+        OpTables<N, T> ops;
+        ops.gather();
         gather_regs<PG, T>(hg, v.gsol_r<float>() + (size_t)c0 * 16 * NP, ncl * 16 * NP);
         gather_regs<PX, T>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
         if constexpr ((VM & 2) != 0) gather_regs<PM, T>(hm, v.mu4 + (size_t)c0 * 4, ncl * 4);
@@ -711,6 +739,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
         scatter_lds<PG, T>(sG, hg, ncl * 16 * NP);
         scatter_lds<PX, T>(sX, hx, ncl * 8);
         if constexpr ((VM & 2) != 0) scatter_lds<PM, T>(sMu, hm, ncl * 4);
+        ops.scatter(sLf, sMf);
     }
     __syncthreads();
     const float gam = float(a.gam);
@@ -749,7 +778,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
                 metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
                 const float s1 = float(s1d), s2 = float(s2d);
                 float d[NV], F[8], G[8];
-                dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d);
+                dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d, sLf);
                 dflux(sRef[c], d, gam - 1.f, F, G);
 #pragma unroll
                 for (int w = 0; w < 8; w++) u[w * NF] = F[w] * s1 + G[w] * s2 + jump[w];
@@ -764,10 +793,10 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
                 double s1d, s2d;
                 metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
                 float d[NV], qv[NV], g[2 * NV], fv[NV];
-                dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d);
+                dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d, sLf);
 #pragma unroll
                 for (int w = 0; w < NV; w++) qv[w] = sRef[c].q[w] + d[w];
-                grad_trace<N, float>(sG + c * 16 * NP, fam, ifp, jfp, g);
+                grad_trace<N, float>(sG + c * 16 * NP, fam, ifp, jfp, g, sLf);
                 visc_point<N, float, VM>(v, sMu + c * 4, fam, ifp, jfp, float(s1d), float(s2d), qv, g, fv);
                 float *u = sU + c * UST + line * (N + 1) + r;
 #pragma unroll
@@ -784,7 +813,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
                 for (int w = 0; w < 8; w++) d[w] = 0.f;
 #pragma unroll CHORUS_MX_CUNROLL
                 for (int r = 0; r <= N; r++) {
-                    const float m = c_Mf[N][r * N + pos];
+                    const float m = sMf[r * N + pos];
 #pragma unroll
                     for (int w = 0; w < 8; w++) d[w] += f[w * NF + r] * m;
                 }
@@ -796,7 +825,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
                 for (int w = 0; w < NV; w++) dv[w] = 0.f;
 #pragma unroll CHORUS_MX_CUNROLL
                 for (int r = 0; r <= N; r++) {
-                    const float m = c_Mf[N][r * N + pos];
+                    const float m = sMf[r * N + pos];
 #pragma unroll
                     for (int w = 0; w < NV; w++) dv[w] += f[(8 + w) * NF + r] * m;
                 }

@@ -287,7 +287,7 @@ __device__ __forceinline__ double mu_av(double sm, double eps0, double pi) {
 // gradient of the 8 variables at flux point (ifp,jfp) of family fam: own trace
 // of nablaQs (nablaQ_interior_FP, compflux.f90:159-196)
 template <int N, class R>
-__device__ __forceinline__ void grad_trace(const R *gq, int fam, int ifp, int jfp, R *g) {
+__device__ __forceinline__ void grad_trace(const R *gq, int fam, int ifp, int jfp, R *g, const R *Lt = nullptr) {
     constexpr int NP = N * N;
 #pragma unroll
     for (int d = 0; d < 2; d++) {
@@ -299,7 +299,8 @@ __device__ __forceinline__ void grad_trace(const R *gq, int fam, int ifp, int jf
     constexpr int U = sizeof(R) == 8 ? contraction_unroll<R, N>() : CHORUS_MX_GRAD_UNROLL;
 #pragma unroll U
     for (int sp = 0; sp < N; sp++) {
-        const R l = fam == 0 ? Lop<R>(N, ifp, sp) : Lop<R>(N, jfp, sp);
+        const int row = fam == 0 ? ifp : jfp;
+        const R l = Lt ? Lt[row * N + sp] : Lop<R>(N, row, sp);
         const int idx = fam == 0 ? jfp * N + sp : sp * N + ifp;
 #pragma unroll
         for (int d = 0; d < 2; d++)
