@@ -102,6 +102,28 @@ __device__ __forceinline__ void scatter_lds(T *dst, const T *v, int n) {
 }
 template <int NMAX, int THREADS> constexpr int per_thread() { return (NMAX + THREADS - 1) / THREADS; }
 
+// FP64 extrapolation / derivative operators and point tables of order N staged
+// in LDS (per-thread indices would otherwise make every coefficient a vector
+// load from the constant tables): gather before the block's first barrier,
+// scatter after it.  Layout of the LDS copy: L[(N+1)N], M[(N+1)N], Xf[N+1], Xs[N].
+template <int N> struct Ops64Layout {
+    static constexpr int K = (N + 1) * N, L = 0, M = K, XF = 2 * K, XS = 2 * K + N + 1, SIZE = 2 * K + 2 * N + 1;
+};
+template <int N, int THREADS> struct OpTables64 {
+    using Y = Ops64Layout<N>;
+    static constexpr int PO = (Y::SIZE + THREADS - 1) / THREADS;
+    double v[PO];
+    __device__ __forceinline__ void gather() {
+#pragma unroll
+        for (int k = 0; k < PO; k++) {
+            const int i = (int)threadIdx.x + k * THREADS;
+            v[k] = i < Y::M ? c_L[N][i] : i < Y::XF ? c_M[N][i - Y::M] : i < Y::XS ? c_Xf[N][i - Y::XF]
+                 : i < Y::SIZE ? c_Xs[N][i - Y::XS] : 0.0;
+        }
+    }
+    __device__ __forceinline__ void scatter(double *sOps) const { scatter_lds<PO, THREADS>(sOps, v, Y::SIZE); }
+};
+
 // local face -> flux-point index of face point nfp (face2fpconnec, initsetup.f90:343-379)
 template <int N> __device__ __forceinline__ void face2fp(int lf, int nfp, int &ifp, int &jfp) {
     switch (lf) {
@@ -119,21 +141,23 @@ __device__ __forceinline__ int lf_family(int lf) { return (lf & 1) ? 1 : 0; }
 // (calcjacob.f90:139-170): fam 0 -> (S1(1,1), S1(1,2)) = (yeta, -xeta) at
 // (Xf(ifp), Xs(jfp)); fam 1 -> (S2(2,1), S2(2,2)) = (-ypsi, xpsi) at (Xs(ifp), Xf(jfp))
 template <int N>
+// (Xf, Xs: LDS copies of the point tables, or null for the constant tables)
 __device__ __forceinline__ void metric_row(const double *xy, int fam, int ifp, int jfp, double &s1,
-                                           double &s2) {
+                                           double &s2, const double *Xf = nullptr, const double *Xs = nullptr) {
     double xpsi, ypsi, xeta, yeta;
     if (fam == 0) {
-        map_derivs(xy, xy + 4, c_Xf[N][ifp], c_Xs[N][jfp], xpsi, ypsi, xeta, yeta);
+        map_derivs(xy, xy + 4, Xf ? Xf[ifp] : c_Xf[N][ifp], Xs ? Xs[jfp] : c_Xs[N][jfp], xpsi, ypsi, xeta, yeta);
         s1 = yeta; s2 = -xeta;
     } else {
-        map_derivs(xy, xy + 4, c_Xs[N][ifp], c_Xf[N][jfp], xpsi, ypsi, xeta, yeta);
+        map_derivs(xy, xy + 4, Xs ? Xs[ifp] : c_Xs[N][ifp], Xf ? Xf[jfp] : c_Xf[N][jfp], xpsi, ypsi, xeta, yeta);
         s1 = -ypsi; s2 = xpsi;
     }
 }
 
-template <int N> __device__ __forceinline__ double jacobian(const double *xy, int is, int js) {
+template <int N>
+__device__ __forceinline__ double jacobian(const double *xy, int is, int js, const double *Xs = nullptr) {
     double xpsi, ypsi, xeta, yeta;
-    map_derivs(xy, xy + 4, c_Xs[N][is], c_Xs[N][js], xpsi, ypsi, xeta, yeta);
+    map_derivs(xy, xy + 4, Xs ? Xs[is] : c_Xs[N][is], Xs ? Xs[js] : c_Xs[N][js], xpsi, ypsi, xeta, yeta);
     return xpsi * yeta - xeta * ypsi;
 }
 
@@ -147,15 +171,17 @@ template <> __device__ __forceinline__ float Mop<float>(int N, int r, int is) { 
 // trace of all 9 variables at flux point (ifp,jfp) of family fam of one cell
 // whose state block q[var][js][is] is given (inviscid_flux_interior,
 // compflux.f90:59-94 and the trace loops of interfaceflux_all.f90:40-73)
+// (Lt: LDS copy of the FP64 extrapolation table, or null for the constant one)
 template <int N, class R>
-__device__ __forceinline__ void trace9(const double *q, int fam, int ifp, int jfp, R *Qf) {
+__device__ __forceinline__ void trace9(const double *q, int fam, int ifp, int jfp, R *Qf, const double *Lt = nullptr) {
     constexpr int NP = N * N;
 #pragma unroll
     for (int v = 0; v < NV; v++) Qf[v] = R(0);
     constexpr int U = contraction_unroll<R, N>();
 #pragma unroll U
     for (int sp = 0; sp < N; sp++) {
-        const R l = fam == 0 ? Lop<R>(N, ifp, sp) : Lop<R>(N, jfp, sp);
+        const int row = fam == 0 ? ifp : jfp;
+        const R l = Lt ? R(Lt[row * N + sp]) : Lop<R>(N, row, sp);
         const int idx = fam == 0 ? jfp * N + sp : sp * N + ifp;
 #pragma unroll
         for (int v = 0; v < NV; v++) Qf[v] = Qf[v] + R(q[v * NP + idx]) * l;
@@ -163,14 +189,15 @@ __device__ __forceinline__ void trace9(const double *q, int fam, int ifp, int jf
 }
 
 template <int N>
-__device__ __forceinline__ double trace1(const double *q1, int fam, int ifp, int jfp) {
+__device__ __forceinline__ double trace1(const double *q1, int fam, int ifp, int jfp, const double *Lt = nullptr) {
+    const double *L = Lt ? Lt : c_L[N];
     double a = 0.0;
     if (fam == 0) {
 #pragma unroll
-        for (int sp = 0; sp < N; sp++) a = a + q1[jfp * N + sp] * c_L[N][ifp * N + sp];
+        for (int sp = 0; sp < N; sp++) a = a + q1[jfp * N + sp] * L[ifp * N + sp];
     } else {
 #pragma unroll
-        for (int sp = 0; sp < N; sp++) a = a + q1[sp * N + ifp] * c_L[N][jfp * N + sp];
+        for (int sp = 0; sp < N; sp++) a = a + q1[sp * N + ifp] * L[jfp * N + sp];
     }
     return a;
 }
@@ -184,8 +211,9 @@ __device__ __forceinline__ int side_sign(int lfi) { return (lfi == 0 || lfi == 3
 // the LDS copy q[c][var][js][is]; same summation order as trace9.  Called by
 // all threads of the block.  TQ[cell][lfi][nfp][var].
 template <int N>
-__device__ __forceinline__ void store_traces(const double *q, int ncl, int c0, double *TQ) {
+__device__ __forceinline__ void store_traces(const double *q, int ncl, int c0, double *TQ, const double *Lt = nullptr) {
     constexpr int NP = N * N, NQ = NV * NP, PT = 4 * N * NV;
+    const double *L = Lt ? Lt : c_L[N];
     for (int it = threadIdx.x; it < ncl * PT; it += blockDim.x) {
         const int c = it / PT, k = it % PT, fpi = k / NV, v = k % NV;
         const int lfi = fpi / N, nfp = fpi % N;
@@ -195,10 +223,10 @@ __device__ __forceinline__ void store_traces(const double *q, int ncl, int c0, d
         double acc = 0.0;
         if (lf_family(lfi + 1) == 0) {
 #pragma unroll
-            for (int sp = 0; sp < N; sp++) acc = acc + qv[jfp * N + sp] * c_L[N][ifp * N + sp];
+            for (int sp = 0; sp < N; sp++) acc = acc + qv[jfp * N + sp] * L[ifp * N + sp];
         } else {
 #pragma unroll
-            for (int sp = 0; sp < N; sp++) acc = acc + qv[sp * N + ifp] * c_L[N][jfp * N + sp];
+            for (int sp = 0; sp < N; sp++) acc = acc + qv[sp * N + ifp] * L[jfp * N + sp];
         }
         TQ[(size_t)c0 * PT + it] = acc;
     }
@@ -308,6 +336,8 @@ __global__ void __launch_bounds__(IShape<N>::THREADS) __attribute__((amdgpu_wave
     __shared__ double sX[CPB * 8];
     __shared__ R sF[CPB * FST];          // [c][v][line][r] contravariant flux, one family at a time
     __shared__ R sA[CPB * AST];          // [c][A_z | metric 1 | metric 2][line][r]
+    __shared__ double sOps[Ops64Layout<N>::SIZE];   // FP64 operator tables (OpTables64 layout)
+    constexpr int OM = Ops64Layout<N>::M, OXF = Ops64Layout<N>::XF, OXS = Ops64Layout<N>::XS;
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = blk * CPB;
@@ -319,8 +349,11 @@ __global__ void __launch_bounds__(IShape<N>::THREADS) __attribute__((amdgpu_wave
         double hq[PQ], hx[PX];
         gather_regs<PQ, S::THREADS>(hq, a.Qin + (size_t)c0 * NQ, ncl * NQ);
         gather_regs<PX, S::THREADS>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
+        OpTables64<N, S::THREADS> ops;
+        ops.gather();
         scatter_lds<PQ, S::THREADS>(sQ, hq, ncl * NQ);
         scatter_lds<PX, S::THREADS>(sX, hx, ncl * 8);
+        ops.scatter(sOps);
     }
     __syncthreads();
 
@@ -340,9 +373,9 @@ __global__ void __launch_bounds__(IShape<N>::THREADS) __attribute__((amdgpu_wave
             const int line = k / (N - 1), r = 1 + k % (N - 1);
             const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
             double s1d, s2d;
-            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d, sOps + OXF, sOps + OXS);
             R Qown[NV], F[8], G[8];
-            trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown);
+            trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown, sOps);
             iflux<R>(R(a.gam), Qown, F, G);
             const R s1 = R(s1d), s2 = R(s2d);
             R *f = sF + c * FST + line * (N + 1) + r;
@@ -363,7 +396,7 @@ __global__ void __launch_bounds__(IShape<N>::THREADS) __attribute__((amdgpu_wave
             else          { lfi = (r == 0) ? 0 : 2; nfp = (r == 0) ? line : (N - 1 - line); }
             const R *src = FF + (((size_t)(c0 + c) * 4 + lfi) * N + nfp) * NV;
             double s1d, s2d;
-            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d, sOps + OXF, sOps + OXS);
             R *f = sF + c * FST + line * (N + 1) + r;
 #pragma unroll
             for (int v = 0; v < 8; v++) f[v * NF] = src[v];
@@ -388,21 +421,21 @@ __global__ void __launch_bounds__(IShape<N>::THREADS) __attribute__((amdgpu_wave
             constexpr int U = contraction_unroll<R, N>();
 #pragma unroll U
             for (int r = 0; r <= N; r++) {
-                R m = Mop<R>(N, r, pos);
+                R m = R(sOps[OM + r * N + pos]);
 #pragma unroll
                 for (int v = 0; v < 8; v++) d[v] = d[v] + f[v * NF + r] * m;
             }
 #pragma unroll U
-            for (int r = 0; r <= N; r++) e1 = e1 + aa[r] * Mop<R>(N, r, pos) * aa[NF + r];
+            for (int r = 0; r <= N; r++) e1 = e1 + aa[r] * R(sOps[OM + r * N + pos]) * aa[NF + r];
 #pragma unroll U
-            for (int r = 0; r <= N; r++) e2 = e2 + aa[r] * Mop<R>(N, r, pos) * aa[2 * NF + r];
+            for (int r = 0; r <= N; r++) e2 = e2 + aa[r] * R(sOps[OM + r * N + pos]) * aa[2 * NF + r];
             if (fam == 0) {
 #pragma unroll
                 for (int v = 0; v < 8; v++) dF[v] = d[v];
                 dA1 = e1; dA2 = e2;
             } else {
                 // residual at the solution point (compresid.f90:12-65)
-                const double Jd = jacobian<N>(sX + pc * 8, pis, pjs);
+                const double Jd = jacobian<N>(sX + pc * 8, pis, pjs, sOps + OXS);
                 const R J = R(Jd);
                 double *qs = sQ + pc * NQ + pk;
                 const R q0 = R(qs[0]), q1 = R(qs[NP]), q2 = R(qs[2 * NP]);
@@ -422,7 +455,7 @@ __global__ void __launch_bounds__(IShape<N>::THREADS) __attribute__((amdgpu_wave
         __syncthreads();
     }
     // D: traces of the new state for the next stage's face pass
-    if (a.kind != KIND_RESID) store_traces<N>(sQ, ncl, c0, a.TQ);
+    if (a.kind != KIND_RESID) store_traces<N>(sQ, ncl, c0, a.TQ, sOps);
 }
 
 // ---------------------------------------------------------------------------
@@ -446,6 +479,8 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
     __shared__ double sS[2][CPB * NF];   // metric rows, same layout
     __shared__ double red[2][256];
     __shared__ int2 sNbr[CPB * 4];
+    __shared__ double sOps[Ops64Layout<N>::SIZE];   // FP64 operator tables (OpTables64 layout)
+    constexpr int OXF = Ops64Layout<N>::XF, OXS = Ops64Layout<N>::XS;
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = a.c_first + blk * CPB;
@@ -463,10 +498,13 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
         int2 hn[PN];
         gather_regs<PX, S::THREADS>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
         gather_regs<PN, S::THREADS>(hn, a.nbr + (size_t)c0 * 4, ncl * 4);
+        OpTables64<N, S::THREADS> ops;
+        ops.gather();
 #pragma unroll
         for (int w = 0; w < NVAR; w++) scatter_lds<PV, S::THREADS>(sV[w], hv[w], ncl * NP);
         scatter_lds<PX, S::THREADS>(sX, hx, ncl * 8);
         scatter_lds<PN, S::THREADS>(sNbr, hn, ncl * 4);
+        ops.scatter(sOps);
     }
     __syncthreads();
     const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
@@ -476,10 +514,9 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
     for (int fam = 0; fam < 2; fam++) {
         for (int it = tid; it < ncl * NF; it += blockDim.x) {
             const int c = it / NF, k = it % NF, line = k / (N + 1), r = k % (N + 1);
-            const int cell = c0 + c;
             const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
             double s1, s2;
-            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1, s2);
+            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1, s2, sOps + OXF, sOps + OXS);
             int2 info = make_int2(-1, 1 << 4);
             int lfi = 0, nfp = 0;
             if (r == 0 || r == N) {
@@ -489,7 +526,7 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
             }
 #pragma unroll
             for (int w = 0; w < NVAR; w++) {
-                double own = trace1<N>(sV[w] + c * NP, fam, ifp, jfp);
+                double own = trace1<N>(sV[w] + c * NP, fam, ifp, jfp, sOps);
                 double val = own;
                 if (r == 0 || r == N) {
                     double other = own;
@@ -524,7 +561,7 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
     if (WHICH == 0) {
         // update_B, then the traces of the new Bx, By for the next face pass
         if (pact) {
-            const double J = jacobian<N>(sX + pc * 8, pis, pjs);
+            const double J = jacobian<N>(sX + pc * 8, pis, pjs, sOps + OXS);
             const size_t base = (size_t)(c0 + pc) * NQ + pk;
             const double n1 = (dpsi[0][0] + out[0][0]) / J;   // nablaQs(9,1)
             const double n2 = (dpsi[0][1] + out[0][1]) / J;   // nablaQs(9,2)
@@ -539,13 +576,13 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
             const int lfi = fpi / N, nfp = fpi % N;
             int ifp, jfp;
             face2fp<N>(lfi + 1, nfp, ifp, jfp);
-            const double v = trace1<N>(sV[w] + c * NP, lf_family(lfi + 1), ifp, jfp);
+            const double v = trace1<N>(sV[w] + c * NP, lf_family(lfi + 1), ifp, jfp, sOps);
             a.TQout[(((size_t)(c0 + c) * 4 + lfi) * N + nfp) * NV + 5 + w] = v;
         }
     } else {
         double sumsq = 0.0, dmax = 0.0;
         if (pact) {
-            const double J = jacobian<N>(sX + pc * 8, pis, pjs);
+            const double J = jacobian<N>(sX + pc * 8, pis, pjs, sOps + OXS);
             const double dbx = (dpsi[0][0] + out[0][0]) / J;  // nablaQs(6,1)
             const double dby = (dpsi[NVAR - 1][1] + out[NVAR - 1][1]) / J;  // nablaQs(7,2)
             const double db = dbx + dby;

@@ -232,13 +232,14 @@ __device__ __forceinline__ void dtrace9(const float *dq, int fam, int ifp, int j
 // T0 = sum_r A(r) M(r,pos) (1 - Xf(r)), T1 = sum_r A(r) M(r,pos) Xf(r).
 template <int N>
 __device__ __forceinline__ void az_derivative(const double *A, const double *xy, int fam, int pos, double &e1,
-                                              double &e2) {
+                                              double &e2, const double *ops) {
     double T0 = 0.0, T1 = 0.0;
 #pragma unroll
     for (int r = 0; r <= N; r++) {
-        const double am = A[r] * c_M[N][r * N + pos];
-        T1 += am * c_Xf[N][r];
-        T0 += am - am * c_Xf[N][r];
+        const double am = A[r] * ops[Ops64Layout<N>::M + r * N + pos];
+        const double xf = c_Xf[N][r];   // uniform index: a scalar load
+        T1 += am * xf;
+        T0 += am - am * xf;
     }
     const double *x = xy, *y = xy + 4;
     if (fam == 0) {
@@ -254,10 +255,11 @@ __device__ __forceinline__ void az_derivative(const double *A, const double *xy,
 // flux-point values A[fam * FAMST + line * (N + 1) + r] of the point's cell
 // (magnetic_potential.f90:1-42, compresid.f90:40-47); q is the point's state
 template <int N, int FAMST>
-__device__ __forceinline__ double az_residual(const double *A, const double *xy, int pis, int pjs, const double *q) {
+__device__ __forceinline__ double az_residual(const double *A, const double *xy, int pis, int pjs, const double *q,
+                                              const double *ops) {
     double e1a, e2a, e1b, e2b;
-    az_derivative<N>(A + pjs * (N + 1), xy, 0, pis, e1a, e2a);
-    az_derivative<N>(A + FAMST + pis * (N + 1), xy, 1, pjs, e1b, e2b);
+    az_derivative<N>(A + pjs * (N + 1), xy, 0, pis, e1a, e2a, ops);
+    az_derivative<N>(A + FAMST + pis * (N + 1), xy, 1, pjs, e1b, e2b, ops);
     const double au = q[1] / q[0], av = q[2] / q[0];
     return -(au * (e1a + e1b) + av * (e2a + e2b));
 }
@@ -266,13 +268,13 @@ __device__ __forceinline__ double az_residual(const double *A, const double *xy,
 // of k_face_inv_mx at face points, the trace of the FP64 plane elsewhere
 template <int N>
 __device__ __forceinline__ double az_at(const double *FA, const double *sAz, int cell, int c, int fam, int r,
-                                        int line) {
+                                        int line, const double *ops) {
     if (r == 0 || r == N) {
         int lfi, nfp;
         face_of<N>(fam, r, line, lfi, nfp);
         return FA[((size_t)cell * 4 + lfi) * N + nfp];
     }
-    return trace1<N>(sAz + c * N * N, fam, fam == 0 ? r : line, fam == 0 ? line : r);
+    return trace1<N>(sAz + c * N * N, fam, fam == 0 ? r : line, fam == 0 ? line : r, ops);
 }
 
 // ---------------------------------------------------------------------------
@@ -385,6 +387,7 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
     __shared__ double sAz[CPB * NP];     // FP64 A_z plane of each cell
     __shared__ double sA[2 * CPB * AST]; // FP64 A_z at the flux points of both families
     __shared__ float sLf[(N + 1) * N], sMf[(N + 1) * N];
+    __shared__ double sOps[Ops64Layout<N>::SIZE];   // FP64 operator tables (OpTables64 layout)
     float *sD = (float *)U, *sF = sD + CPB * NQ;
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -397,9 +400,12 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
         OpTables<N, T> ops;
         gather_regs<PX, T>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
         ops.gather();
+        OpTables64<N, T> ops64;
+        ops64.gather();
         stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, sBar, sRef, sD, sAz, 8);
         scatter_lds<PX, T>(sX, hx, ncl * 8);
         ops.scatter(sLf, sMf);
+        ops64.scatter(sOps);
         __syncthreads();
     }
     const float gam = float(a.gam);
@@ -426,7 +432,7 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
                 for (int v = 0; v < 8; v++) jump[v] = src[v];
             }
             double s1d, s2d;
-            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d, sOps + Ops64Layout<N>::XF, sOps + Ops64Layout<N>::XS);
             const float s1 = float(s1d), s2 = float(s2d);
             float d[NV], F[8], G[8];
             dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d, sLf);
@@ -434,7 +440,7 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
             float *f = sF + c * FST + line * (N + 1) + r;
 #pragma unroll
             for (int v = 0; v < 8; v++) f[v * NF] = F[v] * s1 + G[v] * s2 + jump[v];
-            sA[(fam * CPB + c) * AST + line * (N + 1) + r] = az_at<N>(a.FA, sAz, c0 + c, c, fam, r, line);
+            sA[(fam * CPB + c) * AST + line * (N + 1) + r] = az_at<N>(a.FA, sAz, c0 + c, c, fam, r, line, sOps);
         }
         __syncthreads();
         // the point's state for the combination, loaded while the derivative is summed
@@ -459,17 +465,17 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
     // A_z residual in FP64 from the flux-point values of both families
     double res8 = 0.0;
     if (pact) {
-        const double iJd = 1.0 / jacobian<N>(sX + pc * 8, pis, pjs);
+        const double iJd = 1.0 / jacobian<N>(sX + pc * 8, pis, pjs, sOps + Ops64Layout<N>::XS);
         const float iJ = float(iJd);
 #pragma unroll
         for (int v = 0; v < 8; v++) res[v] = dF[v] * iJ;
         res[8] = 0.f;
-        res8 = az_residual<N, CPB * AST>(sA + pc * AST, sX + pc * 8, pis, pjs, qown) * iJd;
+        res8 = az_residual<N, CPB * AST>(sA + pc * AST, sX + pc * 8, pis, pjs, qown, sOps) * iJd;
     }
     if (pact) combine_point(a, base, NP, qown, qa, res, res8, U + pc * NQ + pk);
     if (a.kind == KIND_RESID) return;
     __syncthreads();
-    store_traces<N>(U, ncl, c0, a.TQ);
+    store_traces<N>(U, ncl, c0, a.TQ, sOps);
 }
 
 // ---------------------------------------------------------------------------
@@ -494,6 +500,7 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
     __shared__ Ref sRef[CPB];
     __shared__ int2 sNbr[CPB * 4];
     __shared__ float sLf[(N + 1) * N], sMf[(N + 1) * N];
+    __shared__ double sOps[Ops64Layout<N>::SIZE];   // FP64 operator tables (OpTables64 layout)
     float *sD = (float *)U, *sV = sD + CPB * NQ, *sG = sV + CPB * VST;
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -507,10 +514,13 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
         gather_regs<PX, T>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
         gather_regs<PN, T>(hn, a.nbr + (size_t)c0 * 4, ncl * 4);
         ops.gather();
+        OpTables64<N, T> ops64;
+        ops64.gather();
         stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, sBar, sRef, sD, nullptr);
         scatter_lds<PX, T>(sX, hx, ncl * 8);
         scatter_lds<PN, T>(sNbr, hn, ncl * 4);
         ops.scatter(sLf, sMf);
+        ops64.scatter(sOps);
         __syncthreads();
     }
     const int grp = tid >= TS::G1 ? 1 : 0;
@@ -549,7 +559,7 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
             if (c >= ncl) continue;
             const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
             double s1d, s2d;
-            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d, sOps + Ops64Layout<N>::XF, sOps + Ops64Layout<N>::XS);
             float d[NV];
             dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d, sLf);
             float *q = sV + c * VST + line * (N + 1) + r;
@@ -576,7 +586,7 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
 #pragma unroll
                 for (int v = 0; v < 8; v++) gp[v] = acc[v];
             } else {
-                const float iJ = float(1.0 / jacobian<N>(sX + pc * 8, pis, pjs));
+                const float iJ = float(1.0 / jacobian<N>(sX + pc * 8, pis, pjs, sOps + Ops64Layout<N>::XS));
 #pragma unroll
                 for (int v = 0; v < 8; v++) sG[(pc * 16 + grp * 8 + v) * NP + pk] = (gp[v] + acc[v]) * iJ;
             }
@@ -714,6 +724,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     __shared__ double sAz[CPB * NP];     // FP64 A_z plane of each cell
     __shared__ double sA[2 * CPB * AST]; // FP64 A_z at the flux points of both families
     __shared__ float sLf[(N + 1) * N], sMf[(N + 1) * N];
+    __shared__ double sOps[Ops64Layout<N>::SIZE];   // FP64 operator tables (OpTables64 layout)
     float *sD = (float *)U, *sU = sD + CPB * NQ, *sG = sU + CPB * UST;
     static_assert(9 * CPB * NP <= CPB * UST, "exchange area");
     const int tid = threadIdx.x;
@@ -732,6 +743,8 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
         double hx[PX], hm[PM];
         OpTables<N, T> ops;
         ops.gather();
+        OpTables64<N, T> ops64;
+        ops64.gather();
         gather_regs<PG, T>(hg, v.gsol_r<float>() + (size_t)c0 * 16 * NP, ncl * 16 * NP);
         gather_regs<PX, T>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
         if constexpr ((VM & 2) != 0) gather_regs<PM, T>(hm, v.mu4 + (size_t)c0 * 4, ncl * 4);
@@ -740,6 +753,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
         scatter_lds<PX, T>(sX, hx, ncl * 8);
         if constexpr ((VM & 2) != 0) scatter_lds<PM, T>(sMu, hm, ncl * 4);
         ops.scatter(sLf, sMf);
+        ops64.scatter(sOps);
     }
     __syncthreads();
     const float gam = float(a.gam);
@@ -775,14 +789,14 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
                     for (int w = 0; w < NV; w++) u[(8 + w) * NF] = FFv[o + w];
                 }
                 double s1d, s2d;
-                metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+                metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d, sOps + Ops64Layout<N>::XF, sOps + Ops64Layout<N>::XS);
                 const float s1 = float(s1d), s2 = float(s2d);
                 float d[NV], F[8], G[8];
                 dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d, sLf);
                 dflux(sRef[c], d, gam - 1.f, F, G);
 #pragma unroll
                 for (int w = 0; w < 8; w++) u[w * NF] = F[w] * s1 + G[w] * s2 + jump[w];
-                sA[(fam * CPB + c) * AST + line * (N + 1) + r] = az_at<N>(a.FA, sAz, c0 + c, c, fam, r, line);
+                sA[(fam * CPB + c) * AST + line * (N + 1) + r] = az_at<N>(a.FA, sAz, c0 + c, c, fam, r, line, sOps);
             } else if (t >= TS::S1) {
                 // viscous / AV flux of the own trace at an interior point
                 const int item = t - TS::S1;
@@ -791,7 +805,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
                 const int line = k / (N - 1), r = 1 + k % (N - 1);
                 const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
                 double s1d, s2d;
-                metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+                metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d, sOps + Ops64Layout<N>::XF, sOps + Ops64Layout<N>::XS);
                 float d[NV], qv[NV], g[2 * NV], fv[NV];
                 dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d, sLf);
 #pragma unroll
@@ -846,19 +860,19 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     double res8 = 0.0;
     if (pact && grp == 0) {
         prefetch_point(a, base, NP, qown, qa);
-        const double iJd = 1.0 / jacobian<N>(sX + pc * 8, pis, pjs);
+        const double iJd = 1.0 / jacobian<N>(sX + pc * 8, pis, pjs, sOps + Ops64Layout<N>::XS);
         const float iJ = float(iJd);
 #pragma unroll
         for (int w = 0; w < 8; w++) res[w] = (-acc8[w] + xv[w * CPB * NP + pt]) * iJ;
         res[8] = 0.f;
-        res8 = (az_residual<N, CPB * AST>(sA + pc * AST, sX + pc * 8, pis, pjs, qown) + double(xv[8 * CPB * NP + pt])) * iJd;
+        res8 = (az_residual<N, CPB * AST>(sA + pc * AST, sX + pc * 8, pis, pjs, qown, sOps) + double(xv[8 * CPB * NP + pt])) * iJd;
     }
     __syncthreads();
     // the new state replaces the compute layout for the traces
     if (pact && grp == 0) combine_point(a, base, NP, qown, qa, res, res8, U + pc * NQ + pk);
     if (a.kind == KIND_RESID) return;
     __syncthreads();
-    store_traces<N>(U, ncl, c0, a.TQ);
+    store_traces<N>(U, ncl, c0, a.TQ, sOps);
     if constexpr ((VM & 2) != 0) av_sensor<N, CPB>(a.sm, U, sAV, ncl, c0);
 }
 

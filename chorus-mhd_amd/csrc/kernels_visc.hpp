@@ -125,6 +125,7 @@ __global__ void __launch_bounds__(256) k_trace(int ncell, const double *Q, doubl
     constexpr int CPB = S::CPB, NP = S::NP, NQ = S::NQ;
     __shared__ double sQ[CPB * NQ];
     __shared__ double sAV[CPB * NP];
+    __shared__ double sOps[Ops64Layout<N>::SIZE];   // FP64 operator tables (OpTables64 layout)
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = blk * CPB;
     const int ncl = min(CPB, ncell - c0);
@@ -132,10 +133,13 @@ __global__ void __launch_bounds__(256) k_trace(int ncell, const double *Q, doubl
         constexpr int PQ = per_thread<CPB * NQ, S::THREADS>();
         double hq[PQ];
         gather_regs<PQ, S::THREADS>(hq, Q + (size_t)c0 * NQ, ncl * NQ);
+        OpTables64<N, S::THREADS> ops;
+        ops.gather();
         scatter_lds<PQ, S::THREADS>(sQ, hq, ncl * NQ);
+        ops.scatter(sOps);
     }
     __syncthreads();
-    store_traces<N>(sQ, ncl, c0, TQ);
+    store_traces<N>(sQ, ncl, c0, TQ, sOps);
     if (sm) av_sensor<N, CPB>(sm, sQ, sAV, ncl, c0);
 }
 
@@ -155,6 +159,8 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
     __shared__ R sV[CPB * VST];          // [c][v | s1 | s2][line][r], one family at a time
     __shared__ R sG[CPB * 16 * NP];      // nablaQs [c][d*8+v][p]
     __shared__ int2 sNbr[CPB * 4];       // face neighbours of the block's cells
+    __shared__ double sOps[Ops64Layout<N>::SIZE];   // FP64 operator tables (OpTables64 layout)
+    constexpr int OM = Ops64Layout<N>::M, OXF = Ops64Layout<N>::XF, OXS = Ops64Layout<N>::XS;
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = blk * CPB;
@@ -167,9 +173,12 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
         gather_regs<PQ, T::THREADS>(hq, a.Qin + (size_t)c0 * NQ, ncl * NQ);
         gather_regs<PX, T::THREADS>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
         gather_regs<PN, T::THREADS>(hn, a.nbr + (size_t)c0 * 4, ncl * 4);
+        OpTables64<N, T::THREADS> ops;
+        ops.gather();
         scatter_lds<PQ, T::THREADS>(sQ, hq, ncl * NQ);
         scatter_lds<PX, T::THREADS>(sX, hx, ncl * 8);
         scatter_lds<PN, T::THREADS>(sNbr, hn, ncl * 4);
+        ops.scatter(sOps);
     }
     __syncthreads();
     const int grp = tid >= T::G1 ? 1 : 0;
@@ -187,9 +196,9 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
                 const int line = k / (N - 1), r = 1 + k % (N - 1);
                 const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
                 double s1d, s2d;
-                metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+                metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d, sOps + OXF, sOps + OXS);
                 R Qown[NV];
-                trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown);
+                trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown, sOps);
                 R *q = sV + c * VST + line * (N + 1) + r;
 #pragma unroll
                 for (int v = 0; v < 8; v++) q[v * NF] = Qown[v];
@@ -205,7 +214,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
                 const int cell = c0 + c;
                 const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
                 double s1d, s2d;
-                metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+                metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d, sOps + OXF, sOps + OXS);
                 int lfi, nfp;
                 face_of<N>(fam, r, line, lfi, nfp);
                 const int2 info = sNbr[c * 4 + lfi];
@@ -231,7 +240,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
             constexpr int U = contraction_unroll<R, N>();
 #pragma unroll U
             for (int r = 0; r <= N; r++) {
-                const R m = Mop<R>(N, r, pos);
+                const R m = R(sOps[OM + r * N + pos]);
                 const R mm = mq[r];
 #pragma unroll
                 for (int v = 0; v < 8; v++) {
@@ -243,7 +252,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
 #pragma unroll
                 for (int v = 0; v < 8; v++) gp[v] = acc[v];
             } else {
-                const R J = R(jacobian<N>(sX + pc * 8, pis, pjs));
+                const R J = R(jacobian<N>(sX + pc * 8, pis, pjs, sOps + OXS));
                 R *gs = a.gsol_r<R>() + (size_t)(c0 + pc) * 16 * NP;
 #pragma unroll
                 for (int v = 0; v < 8; v++) {
@@ -266,10 +275,10 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
         R acc = R(0);
         if (fam == 0) {
 #pragma unroll
-            for (int sp = 0; sp < N; sp++) acc = acc + gq[jfp * N + sp] * Lop<R>(N, ifp, sp);
+            for (int sp = 0; sp < N; sp++) acc = acc + gq[jfp * N + sp] * R(sOps[ifp * N + sp]);
         } else {
 #pragma unroll
-            for (int sp = 0; sp < N; sp++) acc = acc + gq[sp * N + ifp] * Lop<R>(N, jfp, sp);
+            for (int sp = 0; sp < N; sp++) acc = acc + gq[sp * N + ifp] * R(sOps[jfp * N + sp]);
         }
         a.gtr_r<R>()[((size_t)(c0 + c) * 4 * N + fpi) * 16 + g] = acc;
     }
@@ -432,6 +441,8 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
     __shared__ R sU[CPB * UST];          // [c][plane][line][r], one family at a time
     __shared__ double sMu[CPB * 4];      // muAVfi/fj(1,side) per local face
     __shared__ double sAV[CPB * NP];     // AV indicator work
+    __shared__ double sOps[Ops64Layout<N>::SIZE];   // FP64 operator tables (OpTables64 layout)
+    constexpr int OM = Ops64Layout<N>::M, OXF = Ops64Layout<N>::XF, OXS = Ops64Layout<N>::XS;
     static_assert(9 * CPB * NP <= CPB * UST, "exchange area");
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -445,12 +456,15 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
         gather_regs<PQ, T::THREADS>(hq, a.Qin + (size_t)c0 * NQ, ncl * NQ);
         gather_regs<PX, T::THREADS>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
         gather_regs<PG, T::THREADS>(hg, v.gsol_r<R>() + (size_t)c0 * 16 * NP, ncl * 16 * NP);
+        OpTables64<N, T::THREADS> ops;
+        ops.gather();
         // face averages of muAV(1) (compAVflux/interfaceAVflux/cyclocAVflux,
         // AV.f90:199-303, 494-564), evaluated by k_face_visc
         if constexpr ((VM & 2) != 0) gather_regs<PM, T::THREADS>(hm, v.mu4 + (size_t)c0 * 4, ncl * 4);
         scatter_lds<PQ, T::THREADS>(sQ, hq, ncl * NQ);
         scatter_lds<PX, T::THREADS>(sX, hx, ncl * 8);
         scatter_lds<PG, T::THREADS>(sG, hg, ncl * 16 * NP);
+        ops.scatter(sOps);
         if constexpr ((VM & 2) != 0) scatter_lds<PM, T::THREADS>(sMu, hm, ncl * 4);
     }
     if constexpr ((VM & 2) != 0) {
@@ -482,9 +496,9 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
                 const int line = k / (N - 1), r = 1 + k % (N - 1);
                 const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
                 double s1d, s2d;
-                metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+                metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d, sOps + OXF, sOps + OXS);
                 R Qown[NV];
-                trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown);
+                trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown, sOps);
                 R *u = sU + c * UST + line * (N + 1) + r;
                 if (kind == 0) {
                     // inviscid flux
@@ -499,7 +513,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
                 } else {
                     // viscous / AV flux
                     R g[2 * NV], fv[NV];
-                    grad_trace<N, R>(sG + c * 16 * NP, fam, ifp, jfp, g);
+                    grad_trace<N, R>(sG + c * 16 * NP, fam, ifp, jfp, g, sOps);
                     visc_point<N, R, VM>(v, sMu + c * 4, fam, ifp, jfp, R(s1d), R(s2d), Qown, g, fv);
 #pragma unroll
                     for (int w = 0; w < NV; w++) u[(8 + w) * NF] = fv[w];
@@ -514,7 +528,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
                 face_of<N>(fam, r, line, lfi, nfp);
                 const size_t o = (((size_t)(c0 + c) * 4 + lfi) * N + nfp) * NV;
                 double s1d, s2d;
-                metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d);
+                metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d, sOps + OXF, sOps + OXS);
                 R *u = sU + c * UST + line * (N + 1) + r;
 #pragma unroll
                 for (int w = 0; w < 8; w++) u[w * NF] = FF[o + w];
@@ -536,14 +550,14 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
                 for (int w = 0; w < 8; w++) d[w] = R(0);
 #pragma unroll U
                 for (int r = 0; r <= N; r++) {
-                    const R m = Mop<R>(N, r, pos);
+                    const R m = R(sOps[OM + r * N + pos]);
 #pragma unroll
                     for (int w = 0; w < 8; w++) d[w] = d[w] + f[w * NF + r] * m;
                 }
 #pragma unroll U
-                for (int r = 0; r <= N; r++) e1 = e1 + f[17 * NF + r] * Mop<R>(N, r, pos) * f[18 * NF + r];
+                for (int r = 0; r <= N; r++) e1 = e1 + f[17 * NF + r] * R(sOps[OM + r * N + pos]) * f[18 * NF + r];
 #pragma unroll U
-                for (int r = 0; r <= N; r++) e2 = e2 + f[17 * NF + r] * Mop<R>(N, r, pos) * f[19 * NF + r];
+                for (int r = 0; r <= N; r++) e2 = e2 + f[17 * NF + r] * R(sOps[OM + r * N + pos]) * f[19 * NF + r];
                 if (fam == 0) {
 #pragma unroll
                     for (int w = 0; w < 8; w++) acc8[w] = d[w];
@@ -559,7 +573,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
                 for (int w = 0; w < NV; w++) dv[w] = R(0);
 #pragma unroll U
                 for (int r = 0; r <= N; r++) {
-                    const R m = Mop<R>(N, r, pos);
+                    const R m = R(sOps[OM + r * N + pos]);
 #pragma unroll
                     for (int w = 0; w < NV; w++) dv[w] = dv[w] + f[(8 + w) * NF + r] * m;
                 }
@@ -578,7 +592,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
     __syncthreads();
     if (pact && grp == 0) {
         // residual at the solution point (compresid.f90:12-65)
-        const R J = R(jacobian<N>(sX + pc * 8, pis, pjs));
+        const R J = R(jacobian<N>(sX + pc * 8, pis, pjs, sOps + OXS));
         double *qs = sQ + pc * NQ + pk;
         const R q0 = R(qs[0]), q1 = R(qs[NP]), q2 = R(qs[2 * NP]);
         const R au = q1 / q0, av = q2 / q0;
@@ -598,7 +612,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
     __syncthreads();
     // traces of the new state for the next stage's face passes, and its AV
     // indicator (AV.f90:15-82) for the next stage
-    store_traces<N>(sQ, ncl, c0, a.TQ);
+    store_traces<N>(sQ, ncl, c0, a.TQ, sOps);
     if constexpr ((VM & 2) != 0) av_sensor<N, CPB>(a.sm, sQ, sAV, ncl, c0);
 }
 
