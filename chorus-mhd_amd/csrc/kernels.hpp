@@ -29,6 +29,10 @@ static __constant__ double c_Xs[kMaxN + 1][kMaxN];
 static __constant__ double c_Xf[kMaxN + 1][kMaxN + 1];
 static __constant__ float c_Lf[kMaxN + 1][(kMaxN + 1) * kMaxN];
 static __constant__ float c_Mf[kMaxN + 1][(kMaxN + 1) * kMaxN];
+// AV interpolation tables per N: H1[isl*N+is] = hhval(is, Xs_low(isl)),
+// H2[is*(N-1)+isl] = hhval_low(isl, Xs(is))  (AV.f90:19-58)
+static __constant__ double c_H1[kMaxN + 1][kMaxN * kMaxN];
+static __constant__ double c_H2[kMaxN + 1][kMaxN * kMaxN];
 
 static hipError_t upload_tables() {
     static double L[kMaxN + 1][(kMaxN + 1) * kMaxN], M[kMaxN + 1][(kMaxN + 1) * kMaxN];
@@ -105,9 +109,11 @@ template <int NMAX, int THREADS> constexpr int per_thread() { return (NMAX + THR
 // FP64 extrapolation / derivative operators and point tables of order N staged
 // in LDS (per-thread indices would otherwise make every coefficient a vector
 // load from the constant tables): gather before the block's first barrier,
-// scatter after it.  Layout of the LDS copy: L[(N+1)N], M[(N+1)N], Xf[N+1], Xs[N].
+// scatter after it.  Layout of the LDS copy: L[(N+1)N], M[(N+1)N], Xf[N+1], Xs[N],
+// and the AV interpolation tables H1[(N-1)N], H2[N(N-1)].
 template <int N> struct Ops64Layout {
-    static constexpr int K = (N + 1) * N, L = 0, M = K, XF = 2 * K, XS = 2 * K + N + 1, SIZE = 2 * K + 2 * N + 1;
+    static constexpr int K = (N + 1) * N, KH = (N - 1) * N;
+    static constexpr int L = 0, M = K, XF = 2 * K, XS = 2 * K + N + 1, H1 = XS + N, H2 = H1 + KH, SIZE = H2 + KH;
 };
 template <int N, int THREADS> struct OpTables64 {
     using Y = Ops64Layout<N>;
@@ -118,7 +124,8 @@ template <int N, int THREADS> struct OpTables64 {
         for (int k = 0; k < PO; k++) {
             const int i = (int)threadIdx.x + k * THREADS;
             v[k] = i < Y::M ? c_L[N][i] : i < Y::XF ? c_M[N][i - Y::M] : i < Y::XS ? c_Xf[N][i - Y::XF]
-                 : i < Y::SIZE ? c_Xs[N][i - Y::XS] : 0.0;
+                 : i < Y::H1 ? c_Xs[N][i - Y::XS] : i < Y::H2 ? c_H1[N][i - Y::H1]
+                 : i < Y::SIZE ? c_H2[N][i - Y::H2] : 0.0;
         }
     }
     __device__ __forceinline__ void scatter(double *sOps) const { scatter_lds<PO, THREADS>(sOps, v, Y::SIZE); }

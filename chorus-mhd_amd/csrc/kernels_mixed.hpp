@@ -873,7 +873,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     if (a.kind == KIND_RESID) return;
     __syncthreads();
     store_traces<N>(U, ncl, c0, a.TQ, sOps);
-    if constexpr ((VM & 2) != 0) av_sensor<N, CPB>(a.sm, U, sAV, ncl, c0);
+    if constexpr ((VM & 2) != 0) av_sensor<N, CPB>(a.sm, U, sAV, ncl, c0, sOps);
 }
 
 }  // namespace chorus

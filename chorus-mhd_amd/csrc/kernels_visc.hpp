@@ -28,11 +28,6 @@
 
 namespace chorus {
 
-// AV interpolation tables per N: H1[isl*N+is] = hhval(is, Xs_low(isl)),
-// H2[is*(N-1)+isl] = hhval_low(isl, Xs(is))  (AV.f90:19-58)
-static __constant__ double c_H1[kMaxN + 1][kMaxN * kMaxN];
-static __constant__ double c_H2[kMaxN + 1][kMaxN * kMaxN];
-
 static hipError_t upload_av_tables() {
     static double H1[kMaxN + 1][kMaxN * kMaxN], H2[kMaxN + 1][kMaxN * kMaxN];
     for (int n = 2; n <= kMaxN; n++) host_av_tables(n, H1[n], H2[n]);
@@ -81,10 +76,13 @@ template <int N> __device__ __forceinline__ int partner_nfp(int nfp) { return N 
 // N -> N-1 -> N interpolation round trip and log10 of the relative energy).
 // Called by all threads of the block.
 // rho[c * CS + p]: the block's densities (cell stride CS); CPB cells per block
+// ops: the block's LDS copy of the FP64 tables (OpTables64 layout)
 template <int N, int CPB, int CS = NV * N * N>
-__device__ __forceinline__ void av_sensor(double *sm, const double *rho, double *sAV, int ncl, int c0) {
+__device__ __forceinline__ void av_sensor(double *sm, const double *rho, double *sAV, int ncl, int c0,
+                                          const double *ops) {
     constexpr int NP = N * N, NQ = CS, NL = N - 1;
     const double *sQ = rho;
+    const double *H1 = ops + Ops64Layout<N>::H1, *H2 = ops + Ops64Layout<N>::H2;
     const int tid = threadIdx.x;
     const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
     const bool pact = (tid < CPB * NP) && (pc < ncl);
@@ -92,18 +90,22 @@ __device__ __forceinline__ void av_sensor(double *sm, const double *rho, double 
     for (int it = tid; it < ncl * NL * NL; it += blockDim.x) {
         const int c = it / (NL * NL), k = it % (NL * NL), jsl = k / NL, isl = k % NL;
         double acc = 0.0;
+#pragma unroll
         for (int js = 0; js < N; js++)
+#pragma unroll
             for (int is = 0; is < N; is++)
-                acc = acc + sQ[c * NQ + js * N + is] * c_H1[N][isl * N + is] * c_H1[N][jsl * N + js];
+                acc = acc + sQ[c * NQ + js * N + is] * H1[isl * N + is] * H1[jsl * N + js];
         low[c * NP + k] = acc;
     }
     __syncthreads();
     double dev = 0.0;
     if (pact) {
         double acc = 0.0;
+#pragma unroll
         for (int jsl = 0; jsl < NL; jsl++)
+#pragma unroll
             for (int isl = 0; isl < NL; isl++)
-                acc = acc + low[pc * NP + jsl * NL + isl] * c_H2[N][pis * NL + isl] * c_H2[N][pjs * NL + jsl];
+                acc = acc + low[pc * NP + jsl * NL + isl] * H2[pis * NL + isl] * H2[pjs * NL + jsl];
         const double q = sQ[pc * NQ + pk];
         dev = (acc - q) * (acc - q) / (q * q);
     }
@@ -140,7 +142,7 @@ __global__ void __launch_bounds__(256) k_trace(int ncell, const double *Q, doubl
     }
     __syncthreads();
     store_traces<N>(sQ, ncl, c0, TQ, sOps);
-    if (sm) av_sensor<N, CPB>(sm, sQ, sAV, ncl, c0);
+    if (sm) av_sensor<N, CPB>(sm, sQ, sAV, ncl, c0, sOps);
 }
 
 // ---------------------------------------------------------------------------
@@ -613,7 +615,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
     // traces of the new state for the next stage's face passes, and its AV
     // indicator (AV.f90:15-82) for the next stage
     store_traces<N>(sQ, ncl, c0, a.TQ, sOps);
-    if constexpr ((VM & 2) != 0) av_sensor<N, CPB>(a.sm, sQ, sAV, ncl, c0);
+    if constexpr ((VM & 2) != 0) av_sensor<N, CPB>(a.sm, sQ, sAV, ncl, c0, sOps);
 }
 
 }  // namespace chorus
