@@ -265,6 +265,9 @@ def main():
     ap.add_argument("--workload", default="ot512", choices=sorted(WORKLOADS) + ["ensemble"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision form")
+    ap.add_argument("--tile", type=int, default=None,
+                    help="fixed tile shape of the mixed form (0: 256/128, 1: 128/64, 2: 512/256) instead of "
+                         "autotuning, e.g. for profiles without the tuner's launches")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run two kernels of known HBM bytes (a 256 MiB fill: 256 MiB written; an "
                          "in-place add of two 256 MiB arrays: 512 MiB read, 256 MiB written) so that "
@@ -303,7 +306,13 @@ def main():
         s.close()
         s = Solver(mesh, N, dt, gam=w["gam"], artiv=artiv, precision=precision, topo=topo)
         s.set_state(Q0)
-        tile = s.autotune() if precision == "mixed" else None
+        tile = None
+        if precision == "mixed":
+            if args.tile is None:
+                tile = s.autotune()
+            else:
+                s.set_tile(args.tile)
+                tile = args.tile
         if args.warmup:
             s.step(args.warmup)
             s.set_state(Q0)

@@ -175,6 +175,10 @@ class Solver:
         self._check(_lib.load().chorus_autotune(self._ctx, C.byref(t)), "chorus_autotune")
         return t.value
 
+    def set_tile(self, tile: int) -> None:
+        """Fix the tile shape of the mixed-form cell kernels (see autotune)."""
+        self._check(_lib.load().chorus_set_tile(self._ctx, int(tile)), "chorus_set_tile")
+
     def time_steps(self, nsteps: int) -> dict:
         tot, st, ct = C.c_double(), C.c_double(), C.c_double()
         self._check(_lib.load().chorus_time_steps(self._ctx, nsteps, C.byref(tot), C.byref(st),

@@ -1122,6 +1122,14 @@ int chorus_autotune(chorus_ctx *c, int *tile) {
     return rc ? fail(c, rc, "autotune failed") : 0;
 }
 
+int chorus_set_tile(chorus_ctx *c, int tile) {
+    if (!c) return CHORUS_ERR_ARG;
+    if (tile < 0 || tile >= tile_count()) return fail(c, CHORUS_ERR_ARG, "tile index out of range");
+    if (c->p.precision != CHORUS_MIXED && tile != 0) return fail(c, CHORUS_ERR_ARG, "the FP64 form has one tile shape");
+    c->tile = tile;
+    return 0;
+}
+
 int chorus_batch_autotune(chorus_batch *b, int *tiles) {
     if (!b) return CHORUS_ERR_ARG;
     int rc = refresh(b);

@@ -121,6 +121,11 @@ int chorus_time_steps(chorus_ctx *ctx, int nsteps, double *ms_total, double *ms_
  * (*tile = 0). */
 int chorus_autotune(chorus_ctx *ctx, int *tile);
 
+/* Fixes the tile shape of the mixed form's cell kernels (an index as returned
+ * by chorus_autotune), e.g. to reproduce a measurement without re-tuning.
+ * CHORUS_ERR_ARG for an index out of range or a nonzero index in the FP64 form. */
+int chorus_set_tile(chorus_ctx *ctx, int tile);
+
 /* ---- Batched ensembles (no reference counterpart: the reference runs one
  * mesh per MPI job; the loop of main.f90:44-50 / iterations.f90:18-70 is what
  * each instance executes).  ninst independent instances, each with its own

@@ -111,3 +111,38 @@ def test_batch_rejects_mixed_steppers():
     mesh = quad_grid(4, 4, 0, 1, 0, 1)
     with pytest.raises(Exception):
         Batch([Instance(mesh, 3, 1e-4, k_stage=3), Instance(mesh, 3, 1e-4, k_stage=5)])
+
+
+@pytest.mark.parametrize("artiv", [0, 1])
+def test_single_context_tiles_agree(artiv):
+    """chorus_set_tile / chorus_autotune on one context: every tile shape gives
+    the same bits (a tile decides which block evaluates a cell, not how)."""
+    _say("single-context tiles, artiv", artiv)
+    mesh = quad_grid(12, 10, 0, 2 * math.pi, 0, 2 * math.pi, jitter=0.1, seed=9)
+    N = 5
+    s0 = Solver(mesh, N, 2e-4, artiv=artiv, precision="mixed")
+    q0 = ics.orszag_tang(s0, mesh, N)
+    s0.close()
+    outs = []
+    for tile in range(3):
+        s = Solver(mesh, N, 2e-4, artiv=artiv, precision="mixed")
+        s.set_tile(tile)
+        s.set_state(q0)
+        s.step(3)
+        outs.append(s.get_state())
+        s.close()
+    for t in (1, 2):
+        assert np.array_equal(outs[0], outs[t]), "tile %d differs" % t
+    s = Solver(mesh, N, 2e-4, artiv=artiv, precision="mixed")
+    s.set_state(q0)
+    assert 0 <= s.autotune() < 3
+    s.step(3)
+    assert np.array_equal(s.get_state(), outs[0])
+    with pytest.raises(Exception):
+        s.set_tile(3)
+    s.close()
+    f = Solver(mesh, N, 2e-4, artiv=artiv)
+    assert f.autotune() == 0
+    with pytest.raises(Exception):
+        f.set_tile(1)
+    f.close()
