@@ -124,11 +124,12 @@ __device__ __forceinline__ float wave_speed(const float *Ql, const float *Qr, fl
 // become FP32 deviations from their reference states Qbar (the FP64 state at
 // the central solution point).  Each thread keeps the values it loaded in
 // registers; the threads holding a central point publish it as Qbar, and keep,
-// if given, receives the FP64 plane of variable keep_var of each cell.
+// if given, receives the FP64 plane of variable keep_var of each cell, and q64,
+// if given, the FP64 state itself (the combination reads it from there).
 // Called by all THREADS threads of the block.
 template <int N, int CPB, int THREADS>
 __device__ __forceinline__ void stage_deviations(const double *Qg, int ncl, double gam, double *sBar, Ref *sRef,
-                                                 float *dq, double *keep, int keep_var = 0) {
+                                                 float *dq, double *keep, int keep_var = 0, double *q64 = nullptr) {
     constexpr int NP = N * N, NQ = NV * NP, PER = (CPB * NQ + THREADS - 1) / THREADS;
     constexpr int PC = (N / 2) * N + N / 2;   // central solution point
     const int tid = threadIdx.x, n = ncl * NQ;
@@ -144,6 +145,7 @@ __device__ __forceinline__ void stage_deviations(const double *Qg, int ncl, doub
         const int c = i / NQ, r = i % NQ, v = r / NP, p = r % NP;
         if (i < n && p == PC) sBar[c * NV + v] = held[k];
         if (keep && i < n && v == keep_var) keep[c * NP + p] = held[k];
+        if (q64 && i < n) q64[i] = held[k];
     }
     __syncthreads();
     for (int c = tid; c < ncl; c += THREADS) ref_from(sBar + c * NV, gam, sRef[c]);
@@ -172,37 +174,27 @@ template <int N, int THREADS> struct OpTables {
     }
 };
 
-// SSP stage combination at one solution point from register copies of the
-// point's state q and (when the stage needs it) Qa; the new state is also
-// written to qout[var*NP] (LDS) for the traces
-__device__ __forceinline__ void combine_point(const StageArgs &a, size_t base, int NP, const double *q,
-                                              const double *qa, const float *res, double res8, double *qout) {
+// SSP stage combination at one solution point: the point's FP64 state qs[v*NP]
+// (the block's LDS copy) and a register copy of Qa (when the stage needs it);
+// the new state replaces the LDS copy, for the traces
+__device__ __forceinline__ void combine_point(const StageArgs &a, size_t base, int NP, double *qs,
+                                              const double *qa, const float *res, double res8) {
 #pragma unroll
     for (int v = 0; v < NV; v++) {
         const double r = v == 8 ? res8 : double(res[v]);
         const size_t o = base + (size_t)v * NP;
+        const double q = qs[v * NP];
         double qn;
         switch (a.kind) {
         case KIND_RESID: a.Rout[o] = r; continue;
-        case KIND_AXPY: qn = q[v] + a.c0 * r; break;
-        case KIND_3A: qn = a.c1 * qa[v] + a.c2 * q[v] + a.c0 * r; break;
-        case KIND_3B: qn = a.c1 * qa[v] + a.c2 * q[v] + a.c3 * r * a.c4; break;
-        case KIND_S54_4: qn = a.c1 * qa[v] + a.c2 * q[v] + a.c0 * r; a.Rout[o] = r; break;
-        default: qn = a.c1 * qa[v] + a.c2 * a.Qb[o] + a.c3 * a.Qc[o] + a.c4 * q[v] + a.c5 * a.Rprev[o] + a.c0 * r; break;
+        case KIND_AXPY: qn = q + a.c0 * r; break;
+        case KIND_3A: qn = a.c1 * qa[v] + a.c2 * q + a.c0 * r; break;
+        case KIND_3B: qn = a.c1 * qa[v] + a.c2 * q + a.c3 * r * a.c4; break;
+        case KIND_S54_4: qn = a.c1 * qa[v] + a.c2 * q + a.c0 * r; a.Rout[o] = r; break;
+        default: qn = a.c1 * qa[v] + a.c2 * a.Qb[o] + a.c3 * a.Qc[o] + a.c4 * q + a.c5 * a.Rprev[o] + a.c0 * r; break;
         }
         a.Qout[o] = qn;
-        qout[v * NP] = qn;
-    }
-}
-
-// register prefetch of a point's state and, for the stages that combine with
-// Qa, of Qa
-__device__ __forceinline__ void prefetch_point(const StageArgs &a, size_t base, int NP, double *q, double *qa) {
-    const bool need_a = a.kind != KIND_RESID && a.kind != KIND_AXPY;
-#pragma unroll
-    for (int v = 0; v < NV; v++) {
-        q[v] = a.Qin[base + (size_t)v * NP];
-        qa[v] = need_a ? a.Qa[base + (size_t)v * NP] : 0.0;
+        qs[v * NP] = qn;
     }
 }
 
@@ -256,11 +248,11 @@ __device__ __forceinline__ void az_derivative(const double *A, const double *xy,
 // (magnetic_potential.f90:1-42, compresid.f90:40-47); q is the point's state
 template <int N, int FAMST>
 __device__ __forceinline__ double az_residual(const double *A, const double *xy, int pis, int pjs, const double *q,
-                                              const double *ops) {
+                                              int qstride, const double *ops) {
     double e1a, e2a, e1b, e2b;
     az_derivative<N>(A + pjs * (N + 1), xy, 0, pis, e1a, e2a, ops);
     az_derivative<N>(A + FAMST + pis * (N + 1), xy, 1, pjs, e1b, e2b, ops);
-    const double au = q[1] / q[0], av = q[2] / q[0];
+    const double au = q[qstride] / q[0], av = q[2 * qstride] / q[0];
     return -(au * (e1a + e1b) + av * (e2a + e2b));
 }
 
@@ -379,8 +371,9 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
     constexpr int FST = 8 * NF + 1;      // per-cell stride of the dF1 planes
     constexpr int AST = NF + 1;          // per-cell stride of the FP64 A_z flux-point values
     constexpr int UF = CPB * NQ + CPB * FST;
-    constexpr int UD = (UF + 1) / 2 > CPB * NQ ? (UF + 1) / 2 : CPB * NQ;
+    constexpr int UD = (UF + 1) / 2;
     __shared__ double U[UD];
+    __shared__ double sQ64[CPB * NQ];    // FP64 state (combined in place into the new state)
     __shared__ double sX[CPB * 8];
     __shared__ double sBar[CPB * NV];
     __shared__ Ref sRef[CPB];
@@ -402,7 +395,7 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
         ops.gather();
         OpTables64<N, T> ops64;
         ops64.gather();
-        stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, sBar, sRef, sD, sAz, 8);
+        stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, sBar, sRef, sD, sAz, 8, sQ64);
         scatter_lds<PX, T>(sX, hx, ncl * 8);
         ops.scatter(sLf, sMf);
         ops64.scatter(sOps);
@@ -414,7 +407,7 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
     const size_t base = (size_t)(c0 + pc) * NQ + pk;
     const float *FF = (const float *)a.FF;
     float dF[8], res[NV];
-    double qown[NV], qa[NV];
+    double qa[NV];
 #pragma unroll 1
     for (int fam = 0; fam < 2; fam++) {
         for (int it = tid; it < ncl * NF; it += T) {
@@ -443,8 +436,8 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
             sA[(fam * CPB + c) * AST + line * (N + 1) + r] = az_at<N>(a.FA, sAz, c0 + c, c, fam, r, line, sOps);
         }
         __syncthreads();
-        // the point's state for the combination, loaded while the derivative is summed
-        if (fam == 1 && pact) prefetch_point(a, base, NP, qown, qa);
+        // the point's Qa for the combination, loaded while the derivative is summed
+        if (fam == 1 && pact) prefetch_qa(a, base, NP, qa);
         if (pact) {
             const int line = fam == 0 ? pjs : pis, pos = fam == 0 ? pis : pjs;
             const float *f = sF + pc * FST + line * (N + 1);
@@ -470,12 +463,12 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
 #pragma unroll
         for (int v = 0; v < 8; v++) res[v] = dF[v] * iJ;
         res[8] = 0.f;
-        res8 = az_residual<N, CPB * AST>(sA + pc * AST, sX + pc * 8, pis, pjs, qown, sOps) * iJd;
+        res8 = az_residual<N, CPB * AST>(sA + pc * AST, sX + pc * 8, pis, pjs, sQ64 + pc * NQ + pk, NP, sOps) * iJd;
     }
-    if (pact) combine_point(a, base, NP, qown, qa, res, res8, U + pc * NQ + pk);
+    if (pact) combine_point(a, base, NP, sQ64 + pc * NQ + pk, qa, res, res8);
     if (a.kind == KIND_RESID) return;
     __syncthreads();
-    store_traces<N>(U, ncl, c0, a.TQ, sOps);
+    store_traces<N>(sQ64, ncl, c0, a.TQ, sOps);
 }
 
 // ---------------------------------------------------------------------------
@@ -714,8 +707,9 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     constexpr int UST = 17 * NF + 1;     // dF1 (8) + Fv1 (9)
     constexpr int AST = NF + 1;          // per-cell stride of the FP64 A_z flux-point values
     constexpr int UF = CPB * NQ + CPB * UST + CPB * 16 * NP;
-    constexpr int UD = (UF + 1) / 2 > CPB * NQ ? (UF + 1) / 2 : CPB * NQ;
+    constexpr int UD = (UF + 1) / 2;
     __shared__ double U[UD];
+    __shared__ double sQ64[CPB * NQ];    // FP64 state (combined in place into the new state)
     __shared__ double sX[CPB * 8];
     __shared__ double sBar[CPB * NV];
     __shared__ Ref sRef[CPB];
@@ -748,7 +742,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
         gather_regs<PG, T>(hg, v.gsol_r<float>() + (size_t)c0 * 16 * NP, ncl * 16 * NP);
         gather_regs<PX, T>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
         if constexpr ((VM & 2) != 0) gather_regs<PM, T>(hm, v.mu4 + (size_t)c0 * 4, ncl * 4);
-        stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, sBar, sRef, sD, sAz, 8);
+        stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, sBar, sRef, sD, sAz, 8, sQ64);
         scatter_lds<PG, T>(sG, hg, ncl * 16 * NP);
         scatter_lds<PX, T>(sX, hx, ncl * 8);
         if constexpr ((VM & 2) != 0) scatter_lds<PM, T>(sMu, hm, ncl * 4);
@@ -764,7 +758,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     const size_t base = (size_t)(c0 + pc) * NQ + pk;
     const float *FF = (const float *)a.FF, *FFv = (const float *)a.FFv;
     float acc8[8], accv[NV];
-    double qown[NV], qa[NV];
+    double qa[NV];
 #pragma unroll 1
     for (int fam = 0; fam < 2; fam++) {
         for (int t = tid; t < TS::BT; t += T) {
@@ -859,21 +853,20 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     float res[NV];
     double res8 = 0.0;
     if (pact && grp == 0) {
-        prefetch_point(a, base, NP, qown, qa);
+        prefetch_qa(a, base, NP, qa);
         const double iJd = 1.0 / jacobian<N>(sX + pc * 8, pis, pjs, sOps + Ops64Layout<N>::XS);
         const float iJ = float(iJd);
 #pragma unroll
         for (int w = 0; w < 8; w++) res[w] = (-acc8[w] + xv[w * CPB * NP + pt]) * iJ;
         res[8] = 0.f;
-        res8 = (az_residual<N, CPB * AST>(sA + pc * AST, sX + pc * 8, pis, pjs, qown, sOps) + double(xv[8 * CPB * NP + pt])) * iJd;
+        res8 = (az_residual<N, CPB * AST>(sA + pc * AST, sX + pc * 8, pis, pjs, sQ64 + pc * NQ + pk, NP, sOps) +
+                double(xv[8 * CPB * NP + pt])) * iJd;
+        combine_point(a, base, NP, sQ64 + pc * NQ + pk, qa, res, res8);
     }
-    __syncthreads();
-    // the new state replaces the compute layout for the traces
-    if (pact && grp == 0) combine_point(a, base, NP, qown, qa, res, res8, U + pc * NQ + pk);
     if (a.kind == KIND_RESID) return;
     __syncthreads();
-    store_traces<N>(U, ncl, c0, a.TQ, sOps);
-    if constexpr ((VM & 2) != 0) av_sensor<N, CPB>(a.sm, U, sAV, ncl, c0, sOps);
+    store_traces<N>(sQ64, ncl, c0, a.TQ, sOps);
+    if constexpr ((VM & 2) != 0) av_sensor<N, CPB>(a.sm, sQ64, sAV, ncl, c0, sOps);
 }
 
 }  // namespace chorus
