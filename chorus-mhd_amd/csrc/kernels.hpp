@@ -119,13 +119,16 @@ template <int N, int THREADS> struct OpTables64 {
     using Y = Ops64Layout<N>;
     static constexpr int PO = (Y::SIZE + THREADS - 1) / THREADS;
     double v[PO];
+    // one load per element from a selected address (a chain of conditional
+    // loads into one register would wait for each load before the next)
     __device__ __forceinline__ void gather() {
 #pragma unroll
         for (int k = 0; k < PO; k++) {
             const int i = (int)threadIdx.x + k * THREADS;
-            v[k] = i < Y::M ? c_L[N][i] : i < Y::XF ? c_M[N][i - Y::M] : i < Y::XS ? c_Xf[N][i - Y::XF]
-                 : i < Y::H1 ? c_Xs[N][i - Y::XS] : i < Y::H2 ? c_H1[N][i - Y::H1]
-                 : i < Y::SIZE ? c_H2[N][i - Y::H2] : 0.0;
+            const double *src = i < Y::M ? &c_L[N][i] : i < Y::XF ? &c_M[N][i - Y::M]
+                              : i < Y::XS ? &c_Xf[N][i - Y::XF] : i < Y::H1 ? &c_Xs[N][i - Y::XS]
+                              : i < Y::H2 ? &c_H1[N][i - Y::H1] : &c_H2[N][min(i, Y::SIZE - 1) - Y::H2];
+            v[k] = *src;
         }
     }
     __device__ __forceinline__ void scatter(double *sOps) const { scatter_lds<PO, THREADS>(sOps, v, Y::SIZE); }
