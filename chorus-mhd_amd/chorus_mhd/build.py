@@ -32,6 +32,7 @@ UNITS = {
     "chorus_mixed.hip": DEVICE + ["-ffp-contract=fast"],
     "host_ops.cpp": ["--offload-arch=" + ARCH, "-ffp-contract=off"],
     "chorus_api.cpp": ["--offload-arch=" + ARCH, "-ffp-contract=off"],
+    "tecplot.cpp": ["--offload-arch=" + ARCH, "-ffp-contract=off"],
 }
 
 

@@ -151,6 +151,25 @@ class Mesh:
                 f.write("%d %d %d 1 1 %s\n" % (ib + 1, self.bnd[ib, 0], self.bnd[ib, 1],
                                                self.bnd_type[ib]))
 
+    def glob2local(self) -> "Mesh":
+        """The mesh in the reference's local numbering of one rank (GLOB2LOCAL,
+        readGRIDinfo.f90:124-199): vertices renumbered by first appearance in
+        the cells, in cell order.  The module arrays XV, YV, IVCELL a driver
+        passes to the library are in this numbering (it matters for the Tecplot
+        output, whose vertex nodes keep the vertex numbers)."""
+        new = np.zeros(self.nvert + 1, dtype=np.int64)
+        order = []
+        for c in self.cells:
+            for v in c:
+                if new[v] == 0:
+                    order.append(v)
+                    new[v] = len(order)
+        idx = np.array(order) - 1
+        cells = new[self.cells].astype(np.int32)
+        bnd = np.where(new[self.bnd] > 0, new[self.bnd], 0).astype(np.int32)
+        return Mesh(self.xv[idx].copy(), self.yv[idx].copy(), cells, bnd, list(self.bnd_type),
+                    self.dxcycl, self.dycycl)
+
     def reparsed(self) -> "Mesh":
         """The mesh as the reference reads it back from the files (text round trip)."""
         xv = np.array([float(fmt_d20(v)) for v in self.xv])

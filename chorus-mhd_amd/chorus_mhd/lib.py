@@ -56,6 +56,9 @@ SIGNATURES = {
     "chorus_time_steps": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
     "chorus_autotune": (C.c_int, [C.c_void_p, _ip]),
     "chorus_set_tile": (C.c_int, [C.c_void_p, C.c_int]),
+    "chorus_plot_setup": (C.c_int, [C.c_void_p, C.POINTER(ChorusMesh), C.c_int]),
+    "chorus_write_tecplot": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
+    "chorus_plot_topology": (C.c_int, [C.POINTER(ChorusMesh), C.c_int, _ip, _ip, _dp, _ip]),
     "chorus_batch_init": (C.c_int, [C.c_int, C.POINTER(ChorusMesh), C.POINTER(ChorusParams), C.POINTER(C.c_void_p)]),
     "chorus_batch_free": (None, [C.c_void_p]),
     "chorus_batch_last_error": (C.c_char_p, [C.c_void_p]),

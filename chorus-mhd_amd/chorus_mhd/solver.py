@@ -63,6 +63,24 @@ def topology(mesh: Mesh) -> Topology:
 PRECISION = {"fp64": _lib.CHORUS_FP64, "mixed": _lib.CHORUS_MIXED}
 
 
+def plot_topology(mesh: Mesh, nre: int):
+    """Host-only plot nodes of the Tecplot output (chorus_plot_topology) of a mesh
+    in file numbering (renumbered as the reference's GLOB2LOCAL does first):
+    (xy (2, nnode), c2n (ncellp, 4) 1-based)."""
+    local = mesh.glob2local()
+    m, arrs = _c_mesh(local, 2, topology(local))
+    cap = (nre + 1) ** 2 * mesh.ncell
+    xy = np.zeros(2 * cap)
+    c2n = np.zeros(4 * nre * nre * mesh.ncell, dtype=np.int32)
+    nn, ncp = C.c_int(), C.c_int()
+    rc = _lib.load().chorus_plot_topology(C.byref(m), nre, C.byref(nn), C.byref(ncp), _lib.ptr(xy),
+                                          _lib.ptr(c2n, _lib._ip))
+    if rc != 0:
+        raise ChorusError("chorus_plot_topology failed (%d)" % rc)
+    n = nn.value
+    return np.stack([xy[:n], xy[n:2 * n]]), c2n[:4 * ncp.value].reshape(ncp.value, 4)
+
+
 def _c_mesh(mesh: Mesh, N: int, t: Topology):
     """ChorusMesh over arrays kept alive in the returned dict (the library copies them)."""
     a = dict(
@@ -167,6 +185,25 @@ class Solver:
         self._check(_lib.load().chorus_diag(self._ctx, *[C.byref(x) for x in v]), "chorus_diag")
         return dict(zip(["residglob", "divbglob", "divbmax", "eigvmax", "epsilon0"],
                         [x.value for x in v]))
+
+    def plot_setup(self, nre: int) -> None:
+        """Plot-node topology of the Tecplot output with NRE x NRE plot cells per
+        cell (tecplotter2dsetup); call before stepping (see chorus_plot_setup)."""
+        local = self.mesh.glob2local()
+        m, arrs = _c_mesh(local, self.N, topology(local))
+        self._plot_arrs = arrs
+        self._check(_lib.load().chorus_plot_setup(self._ctx, C.byref(m), int(nre)), "chorus_plot_setup")
+
+    def write_tecplot(self, path: str, ctime: float) -> None:
+        """Tecplot TDV111 file of the current state (tecplotter2d + tecpost_parallel)."""
+        self._check(_lib.load().chorus_write_tecplot(self._ctx, path.encode(), float(ctime)), "chorus_write_tecplot")
+
+    @staticmethod
+    def tecplot_name(ctime: float) -> str:
+        """The reference's file name for solution time ctime (tecplotter.f90:190-191):
+        'TEC' + int(ctime) as 7 digits + the fraction as '.ddd' + '.plt'."""
+        frac = "%6.3f" % (ctime - int(ctime))
+        return "TEC%07d%s.plt" % (int(ctime), frac[2:6])
 
     def autotune(self) -> int:
         """Pick the fastest tile shape of the mixed-form cell kernels on this

@@ -50,6 +50,12 @@ struct chorus_ctx {
     std::vector<hipEvent_t> ev;    // timing events (chorus_time_steps)
     bool timing = false;
     size_t ev_used = 0;
+    // Tecplot output (chorus_plot_setup / chorus_write_tecplot)
+    PlotTopology *plot = nullptr;
+    int *pl_cstart = nullptr, *pl_contrib = nullptr, *pl_factor = nullptr;
+    double *pl_H = nullptr, *pl_q = nullptr, *pl_av = nullptr;
+    double *mucell = nullptr;      // viscous path with AV: muAV(1) per cell of the last stage's input
+    double *sm6 = nullptr;         // viscous path with AV: By indicator of the last stage's input
 };
 
 namespace {
@@ -97,6 +103,7 @@ FaceArgs face_args(chorus_ctx *c) {
     f.gtr = c->gtr;
     f.sm = c->sm;
     f.mu4 = c->mu4;
+    f.mucell = c->mucell;
     f.inst = c->inst;
     f.hinst = c->hinst;
     f.artiv = c->p.artiv;
@@ -130,6 +137,7 @@ int launch_stage(chorus_ctx *c, const StageArgs &a) {
         v.ninst = c->ninst;
         v.TQ = c->TQ;
         v.mu4 = c->mu4;
+        v.sm6 = c->sm6;
         v.artiv = c->p.artiv;
         v.mu = c->p.mu;
         v.eta0 = c->p.eta0;
@@ -565,6 +573,15 @@ void chorus_free(chorus_ctx *c) {
     (void)hipFree(c->mu4);
     (void)hipFree(c->inst);
     (void)hipFree(c->hinst);
+    (void)hipFree(c->pl_cstart);
+    (void)hipFree(c->pl_contrib);
+    (void)hipFree(c->pl_factor);
+    (void)hipFree(c->pl_H);
+    (void)hipFree(c->pl_q);
+    (void)hipFree(c->pl_av);
+    (void)hipFree(c->mucell);
+    (void)hipFree(c->sm6);
+    delete c->plot;
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -681,6 +698,94 @@ int chorus_diag(chorus_ctx *c, double *residglob, double *divbglob, double *divb
         if (epsilon0) *epsilon0 = (c->p.artiv == 1) ? e * c->h_msh / (double)(N - 1) * 1.0 : NAN;
     }
     return 0;
+}
+
+int chorus_plot_topology(const chorus_mesh *m, int nre, int *nnode, int *ncellp, double *xy, int *c2n) {
+    if (!m || !nnode || !ncellp || !m->xv || !m->yv || !m->ivcell || !m->if2c) return CHORUS_ERR_ARG;
+    PlotTopology t;
+    std::string err;
+    const int rc = build_plot_topology(m->ncell, m->nvert, m->nface, m->xv, m->yv, m->ivcell, m->if2c, nre, t, err);
+    if (rc) return rc;
+    *nnode = t.nnode;
+    *ncellp = t.ncellp;
+    if (xy)
+        for (int n = 0; n < t.nnode; n++) { xy[n] = t.px[n + 1]; xy[t.nnode + n] = t.py[n + 1]; }
+    if (c2n) std::copy(t.c2n.begin(), t.c2n.end(), c2n);
+    return 0;
+}
+
+int chorus_plot_setup(chorus_ctx *c, const chorus_mesh *m, int nre) {
+    if (!c || !m) return fail(c, CHORUS_ERR_ARG, "null argument");
+    if (nre < 1) return fail(c, CHORUS_ERR_ARG, "NRE must be >= 1");
+    if (m->ncell != c->ncell || m->N != c->N || !m->xv || !m->yv || !m->ivcell || !m->if2c)
+        return fail(c, CHORUS_ERR_ARG, "mesh does not match the context");
+    auto *t = new PlotTopology;
+    std::string err;
+    int rc = build_plot_topology(m->ncell, m->nvert, m->nface, m->xv, m->yv, m->ivcell, m->if2c, nre, *t, err);
+    if (rc) { delete t; return fail(c, rc, err); }
+    delete c->plot;
+    c->plot = t;
+    for (void *p : {(void *)c->pl_cstart, (void *)c->pl_contrib, (void *)c->pl_factor, (void *)c->pl_H,
+                    (void *)c->pl_q, (void *)c->pl_av})
+        (void)hipFree(p);
+    c->pl_cstart = c->pl_contrib = c->pl_factor = nullptr;
+    c->pl_H = c->pl_q = c->pl_av = nullptr;
+    const size_t nn = (size_t)t->nnode;
+    std::vector<double> H((size_t)(nre + 1) * c->N);
+    host_plot_basis(c->N, nre, H.data());
+    HIP_TRY(c, hipMalloc(&c->pl_cstart, (nn + 1) * sizeof(int)));
+    HIP_TRY(c, hipMalloc(&c->pl_contrib, t->contrib.size() * sizeof(int)));
+    HIP_TRY(c, hipMalloc(&c->pl_factor, t->factor.size() * sizeof(int)));
+    HIP_TRY(c, hipMalloc(&c->pl_H, H.size() * sizeof(double)));
+    HIP_TRY(c, hipMalloc(&c->pl_q, nn * 9 * sizeof(double)));
+    HIP_TRY(c, hipMalloc(&c->pl_av, nn * 2 * sizeof(double)));
+    HIP_TRY(c, hipMemcpy(c->pl_cstart, t->cstart.data(), (nn + 1) * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->pl_contrib, t->contrib.data(), t->contrib.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->pl_factor, t->factor.data(), t->factor.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->pl_H, H.data(), H.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (!c->ideal && c->p.artiv && !c->mucell) {
+        // from here on every viscous stage also keeps muAV(1) per cell and the By
+        // indicator of its input, which the output of the last stage's values needs
+        HIP_TRY(c, hipMalloc(&c->mucell, (size_t)c->ncell * sizeof(double)));
+        HIP_TRY(c, hipMalloc(&c->sm6, (size_t)c->ncell * sizeof(double)));
+        HIP_TRY(c, hipMemset(c->mucell, 0, (size_t)c->ncell * sizeof(double)));
+        HIP_TRY(c, hipMemset(c->sm6, 0, (size_t)c->ncell * sizeof(double)));
+    }
+    return 0;
+}
+
+int chorus_write_tecplot(chorus_ctx *c, const char *path, double ctime) {
+    if (!c || !path) return fail(c, CHORUS_ERR_ARG, "null argument");
+    if (!c->plot) return fail(c, CHORUS_ERR_ARG, "chorus_plot_setup was not called");
+    const PlotTopology &t = *c->plot;
+    const bool av = !c->ideal && c->p.artiv && c->last_parity >= 0;
+    PlotArgs a{};
+    a.nnode = t.nnode;
+    a.N = c->N;
+    a.nre = t.nre;
+    a.artiv = av ? 1 : 0;
+    a.cstart = c->pl_cstart;
+    a.contrib = c->pl_contrib;
+    a.factor = c->pl_factor;
+    a.Q = c->X[c->cur];
+    a.H = c->pl_H;
+    a.mucell = c->mucell;
+    a.sm6 = c->sm6;
+    a.eig = av ? c->eig + (size_t)c->last_parity * c->ninst : nullptr;
+    a.h_msh = c->h_msh;
+    a.gam = c->p.gam;
+    a.pi = std::atan(1.0) * 4.0;
+    a.q = c->pl_q;
+    a.av = c->pl_av;
+    hipError_t e = launch_plot_nodes(a, c->stream);
+    if (e != hipSuccess) return fail(c, CHORUS_ERR_HIP, std::string("plot launch: ") + hipGetErrorString(e));
+    std::vector<double> q((size_t)t.nnode * 9), avh((size_t)t.nnode * 2);
+    HIP_TRY(c, hipMemcpyAsync(q.data(), c->pl_q, q.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(avh.data(), c->pl_av, avh.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    std::string err;
+    const int rc = write_tecplot_file(path, ctime, t, q.data(), avh.data(), err);
+    return rc ? fail(c, rc, err) : 0;
 }
 
 int chorus_time_steps(chorus_ctx *c, int nsteps, double *ms_total, double *ms_stage_avg, double *ms_ct_avg) {

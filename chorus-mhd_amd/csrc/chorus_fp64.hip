@@ -96,4 +96,64 @@ hipError_t launch_stage_visc_fp64(int N, const StageArgs &a, const ViscArgs &v, 
 #undef V_
 }
 
+// ---------------------------------------------------------------------------
+// Plot-node values of the Tecplot output (tecplotter2d, tecplotter.f90:19-58),
+// one thread per plot node gathering its cells' contributions in the order
+// the reference adds them (cells ascending), so the node sums are the
+// reference's bits: Qs = sum_js sum_is Q(is,js)*h(is,xsi)*h(js,eta), summed
+// over the cells of the node, divided by their count, then the primitive
+// variables; muAV(1) and muAV(6) likewise.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_plot_nodes(PlotArgs p) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= p.nnode) return;
+    const int N = p.N, NP = N * N, NQ = NV * NP, W = p.nre + 1;
+    double sum[NV], a1 = 0.0, a2 = 0.0;
+#pragma unroll
+    for (int v = 0; v < NV; v++) sum[v] = 0.0;
+    const double eps0 = p.artiv ? p.eig[0] * p.h_msh / (double)(N - 1) * 1.0 : 0.0;
+    for (int k = p.cstart[n]; k < p.cstart[n + 1]; k++) {
+        const int c = p.contrib[2 * k], ij = p.contrib[2 * k + 1];
+        const double *hi = p.H + (ij % W) * N, *hj = p.H + (ij / W) * N;
+        const double *q = p.Q + (size_t)c * NQ;
+        double qs[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) qs[v] = 0.0;
+        for (int js = 0; js < N; js++)
+            for (int is = 0; is < N; is++) {
+                const double hh = hi[is], hv = hj[js];
+#pragma unroll
+                for (int v = 0; v < NV; v++) qs[v] = qs[v] + q[v * NP + js * N + is] * hh * hv;
+            }
+#pragma unroll
+        for (int v = 0; v < NV; v++) sum[v] = sum[v] + qs[v];
+        if (p.artiv) {
+            a1 = a1 + p.mucell[c];
+            a2 = a2 + mu_av(p.sm6[c], eps0, p.pi);
+        }
+    }
+    const double f = double(p.factor[n + 1]);
+    double qs[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) qs[v] = sum[v] / f;
+    double o[NV];
+    o[0] = qs[0];
+    o[1] = qs[1] / qs[0];
+    o[2] = qs[2] / qs[0];
+    o[3] = qs[3] / qs[0];
+    o[5] = qs[5]; o[6] = qs[6]; o[7] = qs[7]; o[8] = qs[8];
+    const double nu = o[1] * o[1] + o[2] * o[2] + o[3] * o[3];
+    const double nb = o[5] * o[5] + o[6] * o[6] + o[7] * o[7];
+    o[4] = (qs[4] - 0.5 * qs[0] * nu - 0.5 * nb) * (p.gam - 1.0);
+#pragma unroll
+    for (int v = 0; v < NV; v++) p.q[(size_t)v * p.nnode + n] = o[v];
+    p.av[n] = a1 / f;
+    p.av[(size_t)p.nnode + n] = a2 / f;
+}
+
+hipError_t launch_plot_nodes(const PlotArgs &p, hipStream_t s) {
+    hipLaunchKernelGGL(k_plot_nodes, dim3((p.nnode + 255) / 256), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
 }  // namespace chorus

@@ -1,6 +1,9 @@
 // Internal interfaces shared by the kernel translation units and the C-ABI layer.
 #pragma once
 
+#include <string>
+#include <vector>
+
 #include <hip/hip_runtime.h>
 #include <cstddef>
 
@@ -84,6 +87,7 @@ struct FaceArgs {
     const void *gtr;
     const double *sm;        // AV indicator per cell
     double *mu4;             // k_face_visc: face-averaged muAV(1) per cell and local face
+    double *mucell;          // k_face_visc, when set (Tecplot output): muAV(1) per cell
     double *eig;             // k_face_inv: atomic max of the Rusanov wave speed (AV), per instance
     const double *eig_now;   // k_face_visc: this stage's maximum (epsilon0), per instance
     const int *inst;         // batched instances: instance of each cell (null: one instance)
@@ -106,6 +110,7 @@ struct ViscArgs {
     int ninst;          // instances sharing the launch
     const double *TQ;   // traces of Qin at the face points
     const double *mu4;  // face-averaged muAV(1) per cell and local face (k_face_visc)
+    double *sm6;        // when set (Tecplot output): log10 indicator of By of the stage's input
     int artiv;
     double mu, eta0, gam, h_msh, pi;
     template <class R> __host__ __device__ R *gtr_r() const { return (R *)gtr; }
@@ -175,6 +180,35 @@ int ct_blocks(int N, int ncell);
 int stage_blocks(int N, int ncell);
 hipError_t upload_operators_fp64();
 hipError_t upload_operators_mixed();
+
+// plot-node topology of the Tecplot output (tecplotter2dsetup.f90:1-164)
+struct PlotTopology {
+    int nre = 0, nnode = 0, ncellp = 0;
+    std::vector<double> px, py;     // node coordinates, 1-based [1..nnode]
+    std::vector<int> factor;        // cells sharing each node, 1-based
+    std::vector<int> gnum;          // node of (ii, jj) of each cell: [c][jj-1][ii-1], 1-based ids
+    std::vector<int> c2n;           // 4 nodes (1-based) per plot cell
+    std::vector<int> cstart;        // contributions of node n: [cstart[n], cstart[n+1])
+    std::vector<int> contrib;       // per contribution: cell (0-based), (jj-1)*(nre+1) + (ii-1)
+};
+int build_plot_topology(int ncell, int nvert, int nface, const double *xv, const double *yv, const int *ivcell,
+                        const int *if2c, int nre, PlotTopology &t, std::string &err);
+int write_tecplot_file(const char *path, double ctime, const PlotTopology &t, const double *q, const double *av,
+                       std::string &err);
+// plot-node values (tecplotter.f90:19-58) on the device: 9 primitive plot
+// variables q[v*nnode+n] and muAV(1), muAV(6) in av[k*nnode+n]
+struct PlotArgs {
+    int nnode, N, nre, artiv;
+    const int *cstart, *contrib, *factor;   // factor 1-based
+    const double *Q, *H;                    // state [cell][var][js][is]; H[ii*N+is] = hval(is, ii/nre)
+    const double *mucell, *sm6, *eig;       // muAV(1) per cell, By indicator per cell, stage maximum
+    double h_msh, gam, pi;                  // epsilon0 = eig * h_msh / (N-1) (AV.f90:147-148)
+    double *q, *av;
+};
+hipError_t launch_plot_nodes(const PlotArgs &p, hipStream_t s);
+// hval of the N solution points at the NRE+1 equally spaced plot coordinates
+// (tecplotter.f90:64-82): H[ii*N+is]
+void host_plot_basis(int N, int nre, double *H);
 
 // Xs, Xf, Lmat, Mmat of the reference for N points (setsandpoints,
 // computeLandM, initsetup.f90:167-315), computed on the host in the

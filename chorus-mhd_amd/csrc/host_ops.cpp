@@ -86,6 +86,13 @@ void host_av_tables(int N, double *H1, double *H2) {
         for (int isl = 0; isl < N - 1; isl++) H2[is * (N - 1) + isl] = lagrange(N - 1, xl, isl, xs[is]);
 }
 
+void host_plot_basis(int N, int nre, double *H) {
+    double xs[kMaxN];
+    points(N, xs, nullptr);
+    for (int ii = 0; ii <= nre; ii++)
+        for (int is = 0; is < N; is++) H[ii * N + is] = lagrange(N, xs, is, double(ii) / double(nre));
+}
+
 // computeLandM (initsetup.f90:258-315)
 void host_operators(int N, double *Xs, double *Xf, double *L, double *M) {
     points(N, Xs, Xf);

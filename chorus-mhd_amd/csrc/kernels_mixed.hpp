@@ -659,10 +659,15 @@ __global__ void __launch_bounds__(256) k_face_visc_mx(FaceArgs a) {
         const int in = a.inst ? a.inst[fc.x] : 0;
         const double eps0 = a.eig_now[in] * (a.inst ? a.hinst[in] : a.h_msh) / (double)(N - 1);
         const double ml = mu_av(a.sm[fc.x], eps0, a.pi);
-        const double mf = free_face ? ml : 0.5 * (ml + mu_av(a.sm[fc.z], eps0, a.pi));
+        const double mr = free_face ? ml : mu_av(a.sm[fc.z], eps0, a.pi);
+        const double mf = free_face ? ml : 0.5 * (ml + mr);
         if (k == 0) {
             a.mu4[(size_t)fc.x * 4 + fc.y] = mf;
             if (!free_face) a.mu4[(size_t)fc.z * 4 + fc.w] = mf;
+            if (a.mucell) {
+                a.mucell[fc.x] = ml;
+                if (!free_face) a.mucell[fc.z] = mr;
+            }
         }
         vflux<float>(gam, lam, prandt, float(mf), 0.f, qv, g, Fv, Gv);
 #pragma unroll
@@ -750,6 +755,10 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
         ops64.scatter(sOps);
     }
     __syncthreads();
+    // the By indicator of the input state, for the Tecplot output's muAV(6)
+    if constexpr ((VM & 2) != 0) {
+        if (v.sm6) av_sensor<N, CPB>(v.sm6, sQ64 + 5 * NP, sAV, ncl, c0, sOps);
+    }
     const float gam = float(a.gam);
     const int grp = tid >= TS::G1 ? 1 : 0;
     const int pt = tid - grp * TS::G1;

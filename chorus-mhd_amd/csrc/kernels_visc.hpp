@@ -408,10 +408,15 @@ __global__ void __launch_bounds__(256) k_face_visc(FaceArgs a) {
         const int in = a.inst ? a.inst[fc.x] : 0;
         const double eps0 = a.eig_now[in] * (a.inst ? a.hinst[in] : a.h_msh) / (double)(N - 1) * 1.0;
         const double ml = mu_av(a.sm[fc.x], eps0, a.pi);
-        const double mf = free_face ? ml : 0.5 * (ml + mu_av(a.sm[fc.z], eps0, a.pi));
+        const double mr = free_face ? ml : mu_av(a.sm[fc.z], eps0, a.pi);
+        const double mf = free_face ? ml : 0.5 * (ml + mr);
         if (k == 0) {
             a.mu4[(size_t)fc.x * 4 + fc.y] = mf;
             if (!free_face) a.mu4[(size_t)fc.z * 4 + fc.w] = mf;
+            if (a.mucell) {
+                a.mucell[fc.x] = ml;
+                if (!free_face) a.mucell[fc.z] = mr;
+            }
         }
         vflux<R>(gam, lam, prandt, R(mf), R(0), qv, g, Fv, Gv);
 #pragma unroll
@@ -474,6 +479,11 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
             for (int i = tid; i < v.ninst; i += blockDim.x) v.eig_next[i] = 0.0;
     }
     __syncthreads();
+    // the By indicator of the input state, for the Tecplot output's muAV(6)
+    // (AV.f90:60-80 for variable 6)
+    if constexpr ((VM & 2) != 0) {
+        if (v.sm6) av_sensor<N, CPB>(v.sm6, sQ + 5 * NP, sAV, ncl, c0, sOps);
+    }
     // derivative-phase role of this thread
     const int grp = tid >= T::G1 ? 1 : 0;
     const int pt = tid - grp * T::G1;

@@ -113,6 +113,26 @@ int chorus_topology(int ncell, int nvert, const int *ivcell, const double *xv, c
 int chorus_time_steps(chorus_ctx *ctx, int nsteps, double *ms_total, double *ms_stage_avg,
                       double *ms_ct_avg);
 
+/* Tecplot output, replacing tecplotter2dsetup (tecplotter2dsetup.f90:1-164),
+ * tecplotter2d (tecplotter.f90:1-86) and tecpost_parallel (tecplotter.f90:
+ * 92-418) for one partition.  chorus_plot_setup builds the plot-node topology
+ * of NRE x NRE plot cells per cell from the same mesh as chorus_init; call it
+ * before stepping: with ARTIV=1 the viscous stages then also keep muAV(1) and
+ * the By indicator of their input (the muAV and etaAV columns hold the last
+ * residual evaluation's values, as the reference's module array muAV does;
+ * with ARTIV=0 they are zeros, where the reference reads an unallocated
+ * array).  chorus_write_tecplot forms the node values of the current state on
+ * the device and writes the TDV111 binary file at path (the reference writes
+ * ./TEC/TEC<int(ctime):7><frac(ctime):4>.plt; the caller chooses the path)
+ * with solution time ctime. */
+int chorus_plot_setup(chorus_ctx *ctx, const chorus_mesh *mesh, int nre);
+/* Host only (no device): the plot nodes of chorus_plot_setup.  nnode and
+ * ncellp receive the node and plot-cell counts; xy (2 x (NRE+1)^2 NCELL
+ * doubles, may be null) the node coordinates X then Y; c2n (4 NRE^2 NCELL
+ * ints, may be null) the 1-based nodes of each plot cell. */
+int chorus_plot_topology(const chorus_mesh *mesh, int nre, int *nnode, int *ncellp, double *xy, int *c2n);
+int chorus_write_tecplot(chorus_ctx *ctx, const char *path, double ctime);
+
 /* Tile-shape autotuner (no reference counterpart): times the mixed form's
  * cell kernels in every compiled tile shape on this context's own mesh and
  * state (residual evaluations, no state change), keeps the fastest for later

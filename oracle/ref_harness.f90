@@ -12,6 +12,10 @@
 !   mode "setup": topo.i32, geom.f64, ops.f64, q0.f64
 !   mode "resid": q0.f64, resid.f64 (+ av.f64 when ARTIV=1)
 !   mode "run"  : q0.f64, qend.f64, diag.txt, timing.txt
+!   mode "plot" : as "run", then the reference's own Tecplot output chain
+!                 (tecplotter2dsetup, tecplotter2d, tecpost_parallel, the
+!                 calls of iterations.f90:82-86) writes ./TEC/TEC*.plt; needs
+!                 ARTIV=1 (tecplotter2d reads muAV) and an existing ./TEC
 !
 ! usage:  ref_harness setup|resid|run     (reads QUAD.INP like the reference)
 program ref_harness
@@ -80,7 +84,7 @@ program ref_harness
             close(fh)
         end if
 
-    else if (trim(mode) == 'run') then
+    else if (trim(mode) == 'run' .or. trim(mode) == 'plot') then
         nsteps = MAXITER - iter
         open(newunit=fh, file='diag.txt', status='replace')
         t0 = MPI_WTIME()
@@ -123,8 +127,13 @@ program ref_harness
         open(newunit=fh, file='timing.txt', status='replace')
         write(fh, '(I8,ES26.17E3,I12,I4)') nsteps, t1 - t0, NCELL, N
         close(fh)
+        if (trim(mode) == 'plot') then
+            call tecplotter2dsetup
+            call tecplotter2d
+            call tecpost_parallel
+        end if
     else
-        stop 'mode must be setup, resid or run'
+        stop 'mode must be setup, resid, run or plot'
     end if
 
     call MPI_FINALIZE(ierror)

@@ -5,8 +5,8 @@ oracle/_ref/ref_harness) on small meshes produced by chorus_mhd.quad_grid and
 stores inputs (mesh recipe, initial state) and outputs (topology, residual,
 state after k steps, reference diagnostics) as compressed numpy archives.
 
-Only this script touches the reference; tests read the .npz files.
-Usage:  python tests/golden/make_golden.py
+Only this script touches the reference; tests read the .npz (and .plt) files.
+Usage:  python tests/golden/make_golden.py [case ...]
 """
 from __future__ import annotations
 
@@ -105,11 +105,49 @@ def run_case(name, spec, outdir):
         shutil.rmtree(work, ignore_errors=True)
 
 
+# Tecplot output of the reference (mode "plot": tecplotter2dsetup, tecplotter2d,
+# tecpost_parallel after the steps; needs ARTIV=1, see tecplotter.f90:37-38)
+# name: (nx, ny, N, jitter, bc, steps, NRE)
+PLOT_CASES = {
+    "plot_ot_8x6_N4_nre3": (8, 6, 4, 0.10, "CYCL", 3, 3),
+    "plot_ot_6x6_N3_free_nre2": (6, 6, 3, 0.10, "FREE", 2, 2),
+}
+
+
+def run_plot_case(name, spec, outdir):
+    nx, ny, N, jit, bc, steps, nre = spec
+    mesh = quad_grid(nx, ny, 0.0, TWO_PI, 0.0, TWO_PI, jitter=jit, seed=SEED, bc=bc)
+    work = tempfile.mkdtemp()
+    try:
+        mesh.write(work)
+        QuadInp(N=N, dt=DT, MAXITER=steps, DXCYCL=mesh.dxcycl, DYCYCL=mesh.dycycl, ARTIV=1, NRE=nre,
+                restart_ord=N).write(os.path.join(work, "QUAD.INP"))
+        os.makedirs(os.path.join(work, "TEC"))
+        orc.run_reference(work, "plot")
+        files = os.listdir(os.path.join(work, "TEC"))
+        if len(files) != 1:
+            raise RuntimeError("expected one Tecplot file, got %s" % files)
+        q0 = np.fromfile(os.path.join(work, "q0.f64")).reshape(mesh.ncell, N, N, 9)
+        shutil.copy(os.path.join(work, "TEC", files[0]), os.path.join(outdir, name + ".plt"))
+        np.savez_compressed(os.path.join(outdir, name + ".npz"), nx=nx, ny=ny, N=N, jitter=jit, bc=bc,
+                            steps=steps, nre=nre, dt=DT, seed=SEED, q0=q0, plt_name=files[0])
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 def main():
     if not os.path.exists(orc.REF_HARNESS):
         raise SystemExit("build the reference first: make -C oracle ref")
+    only = sys.argv[1:]
     for name, spec in CASES.items():
+        if only and name not in only:
+            continue
         run_case(name, spec, HERE)
+        print("wrote", name)
+    for name, spec in PLOT_CASES.items():
+        if only and name not in only:
+            continue
+        run_plot_case(name, spec, HERE)
         print("wrote", name)
 
 

@@ -1,0 +1,94 @@
+"""Tecplot output (chorus_plot_setup / chorus_write_tecplot) against the
+reference's own files: tests/golden/plot_*.plt were written by the unmodified
+reference (tecplotter2dsetup, tecplotter2d, tecpost_parallel) after the same
+steps from the same state (tests/golden/make_golden.py).
+
+CPU: the plot-node topology (node numbering, FP32 coordinates, FP64 ranges,
+connectivity) of the library's host builder equals the reference's.
+GPU: the FP64 drop-in writes the reference's file byte for byte (node sums in
+the reference's order, muAV and etaAV of the last stage's input); the mixed
+form's variables agree to its error budget.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from chorus_mhd import quad_grid
+from chorus_mhd.solver import Solver, plot_topology
+from chorus_mhd.tecplot import read_plt
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CASES = ["plot_ot_8x6_N4_nre3", "plot_ot_6x6_N3_free_nre2"]
+
+
+def _case(name):
+    g = np.load(os.path.join(HERE, "golden", name + ".npz"))
+    mesh = quad_grid(int(g["nx"]), int(g["ny"]), 0.0, 2 * math.pi, 0.0, 2 * math.pi, jitter=float(g["jitter"]),
+                     seed=int(g["seed"]), bc=str(g["bc"]))
+    return g, mesh, os.path.join(HERE, "golden", name + ".plt")
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_plot_topology_matches_reference(name):
+    g, mesh, plt = _case(name)
+    f = read_plt(plt)
+    xy, c2n = plot_topology(mesh, int(g["nre"]))
+    assert f["names"] == ["X", "Y", "RHO", "U", "V", "W", "P", "Bx", "By", "Bz", "Az", "muAV", "etaAV"]
+    assert xy.shape[1] == f["nnode"] and c2n.shape[0] == f["ncell"] == mesh.ncell * int(g["nre"]) ** 2
+    assert np.array_equal(c2n - 1, f["conn"])
+    assert np.array_equal(xy.astype(np.float32), f["data"][:2])
+    assert np.array_equal([xy[0].min(), xy[0].max()], f["minmax"][0])
+    assert np.array_equal([xy[1].min(), xy[1].max()], f["minmax"][1])
+
+
+def test_plot_topology_counts():
+    """Closed form on an n x m periodic grid: (n*NRE+1)(m*NRE+1) nodes (periodic
+    sides are distinct boundary faces, so their nodes are not merged)."""
+    mesh = quad_grid(5, 4, 0.0, 1.0, 0.0, 1.0, jitter=0.05, seed=3)
+    for nre in (1, 2, 4):
+        xy, c2n = plot_topology(mesh, nre)
+        assert xy.shape[1] == (5 * nre + 1) * (4 * nre + 1)
+        assert c2n.shape == (20 * nre * nre, 4) and c2n.min() == 1 and c2n.max() == xy.shape[1]
+
+
+def _gpu_file(name, precision, tmp_path):
+    g, mesh, plt = _case(name)
+    s = Solver(mesh, int(g["N"]), float(g["dt"]), artiv=1, precision=precision)
+    s.plot_setup(int(g["nre"]))
+    s.set_state(g["q0"])
+    s.step(int(g["steps"]))
+    ctime = 0.0
+    for _ in range(int(g["steps"])):
+        ctime = ctime + float(g["dt"])     # ctime = ctime0 + dt per step (iterations.f90:184-215)
+    assert Solver.tecplot_name(ctime) == str(g["plt_name"])
+    out = str(tmp_path / Solver.tecplot_name(ctime))
+    s.write_tecplot(out, ctime)
+    s.close()
+    return out, plt
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", CASES)
+def test_tecplot_fp64_byte_identical(name, tmp_path):
+    print("tecplot fp64", name, flush=True)
+    out, plt = _gpu_file(name, "fp64", tmp_path)
+    a, b = open(out, "rb").read(), open(plt, "rb").read()
+    if a != b:
+        fa, fb = read_plt(out), read_plt(plt)
+        bad = [fb["names"][v] for v in range(13) if not np.array_equal(fa["data"][v], fb["data"][v])]
+        raise AssertionError("file differs from the reference's; variables: %s" % bad)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", CASES)
+def test_tecplot_mixed_within_budget(name, tmp_path):
+    print("tecplot mixed", name, flush=True)
+    out, plt = _gpu_file(name, "mixed", tmp_path)
+    fa, fb = read_plt(out), read_plt(plt)
+    assert np.array_equal(fa["conn"], fb["conn"]) and np.array_equal(fa["data"][:2], fb["data"][:2])
+    for v in range(2, 11):
+        ref = fb["data"][v].astype(np.float64)
+        err = np.abs(fa["data"][v] - ref).max() / max(np.abs(ref).max(), 1e-30)
+        assert err < 1e-6, (fb["names"][v], err)
