@@ -7,13 +7,16 @@
 ! the driver calls chorus_gpu_setup once; each iteration of the time loop calls
 ! chorus_gpu_step (stepper + CT on the GPU) instead of RK_S5P3/FiveStageRk and
 ! correct_B_via_curl_Az; chorus_gpu_fetch_state copies Q back into setup2d's
-! array before any output.  The state stays resident on the GPU in between.
+! array before the restart dump; chorus_gpu_plot_setup / chorus_gpu_tecplot
+! replace the Tecplot output chain.  The state stays resident on the GPU in
+! between.
 module chorus_hip
     use iso_c_binding
     implicit none
     private
     public :: chorus_gpu_setup, chorus_gpu_step, chorus_gpu_compresid, chorus_gpu_correct_b, &
-              chorus_gpu_fetch_state, chorus_gpu_diag, chorus_gpu_free
+              chorus_gpu_fetch_state, chorus_gpu_diag, chorus_gpu_free, chorus_gpu_plot_setup, &
+              chorus_gpu_tecplot
 
     integer(c_int), parameter, public :: CHORUS_FP64 = 0, CHORUS_MIXED = 1
 
@@ -81,6 +84,18 @@ module chorus_hip
             type(c_ptr), value :: ctx
             real(c_double), intent(out) :: residglob, divbglob, divbmax, eigvmax, eps0
         end function
+        integer(c_int) function c_chorus_plot_setup(ctx, mesh, nre) bind(c, name='chorus_plot_setup')
+            import :: c_int, c_ptr, chorus_mesh_t
+            type(c_ptr), value :: ctx
+            type(chorus_mesh_t), intent(in) :: mesh
+            integer(c_int), value :: nre
+        end function
+        integer(c_int) function c_chorus_write_tecplot(ctx, path, ctime) bind(c, name='chorus_write_tecplot')
+            import :: c_int, c_ptr, c_char, c_double
+            type(c_ptr), value :: ctx
+            character(kind=c_char), dimension(*), intent(in) :: path
+            real(c_double), value :: ctime
+        end function
         integer(c_size_t) function c_strlen(s) bind(c, name='strlen')
             import :: c_size_t, c_ptr
             type(c_ptr), value :: s
@@ -88,6 +103,7 @@ module chorus_hip
     end interface
 
     type(c_ptr), save :: ctx = c_null_ptr
+    type(chorus_mesh_t), save :: h_mesh
     ! host copies handed to chorus_init (c_loc needs TARGET arrays)
     real(c_double), allocatable, target, save :: h_xv(:), h_yv(:)
     integer(c_int), allocatable, target, save :: h_iv(:,:), h_if2c(:,:), h_inter(:), h_cyc(:,:), h_free(:)
@@ -135,6 +151,25 @@ contains
         p%dxcycl = DXCYCL; p%dycycl = DYCYCL; p%ncellglob = NCELLGLOB; p%precision = precision
         call check(c_chorus_init(m, p, ctx), 'chorus_init')
         call check(c_chorus_set_state(ctx, Q), 'chorus_set_state')
+        h_mesh = m
+    end subroutine
+
+    ! tecplotter2dsetup (tecplotter2dsetup.f90:1-164) on the mesh of chorus_gpu_setup
+    subroutine chorus_gpu_plot_setup(nre)
+        integer, intent(in) :: nre
+        call check(c_chorus_plot_setup(ctx, h_mesh, int(nre, c_int)), 'chorus_plot_setup')
+    end subroutine
+
+    ! tecplotter2d + tecpost_parallel (tecplotter.f90:1-418): ./TEC/TEC<time>.plt
+    ! of the resident state, named as tecplotter.f90:190-191 does
+    subroutine chorus_gpu_tecplot(ctime)
+        double precision, intent(in) :: ctime
+        character(6) :: decimal
+        character(64) :: filename
+        write(decimal, '(f6.3)') ctime - int(ctime)
+        write(filename, '(a,i7.7,a4,a)') 'TEC', int(ctime), decimal(3:6), '.plt'
+        call check(c_chorus_write_tecplot(ctx, './TEC/'//trim(filename)//c_null_char, real(ctime, c_double)), &
+                   'chorus_write_tecplot')
     end subroutine
 
     ! loop body of iterations.f90:22-33 on the GPU: RK_S5P3 / FiveStageRk + correct_B_via_curl_Az

@@ -4,9 +4,12 @@
 ! Program flow follows main.f90:8-65 and iterations.f90:1-89: read QUAD.INP and
 ! the mesh, build connectivity, INIT_SETUP (initial condition or restart), then
 ! per iteration one GPU step (stepper + CT), the residual / div B norms every 10
-! iterations into resid.out, and restart dumps (WRITE_ALL_DATA_PARALLEL) at
-! mod(iter,nwrite)==1 and at MAXITER.  The Tecplot writer is not called: with
-! ARTIV=0 it reads the unallocated muAV (tecplotter.f90:37-38).
+! iterations into resid.out, and at mod(iter,nwrite)==1 and at MAXITER the
+! Tecplot file (the library's device-side replacement of tecplotter2dsetup,
+! tecplotter2d and tecpost_parallel, into ./TEC/, which must exist as for the
+! reference; with ARTIV=0 its muAV/etaAV columns are zeros, where the
+! reference reads the unallocated muAV, tecplotter.f90:37-38) and the restart
+! dump (WRITE_ALL_DATA_PARALLEL).
 !
 ! usage: sd2d_mhd_hip [mixed]      (FP64 drop-in unless "mixed" is given)
 program sd2d_mhd_hip
@@ -49,6 +52,7 @@ program sd2d_mhd_hip
     call INIT_SETUP
 
     call chorus_gpu_setup(gam, prec)
+    call chorus_gpu_plot_setup(NRE)
 
     open(newunit=fh, file='resid.out', status='replace')
     write(fh, *) 'ITER    ctime     RESIDNORM   divB'
@@ -61,6 +65,7 @@ program sd2d_mhd_hip
             write(fh, *) iter, ctime, residglob, divbglob
         end if
         if (mod(iter, nwrite) == 1 .or. iter >= MAXITER) then
+            call chorus_gpu_tecplot(ctime)
             call chorus_gpu_fetch_state()
             call WRITE_ALL_DATA_PARALLEL
         end if

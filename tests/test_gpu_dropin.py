@@ -1,7 +1,10 @@
 """The Fortran drop-in end to end: the reference's own set-up chain, restart
 reader/writer and loop structure, with the time step on the GPU through the
 iso_c_binding module (chorus-mhd_amd/fortran).  Its restart dump after k steps
-must equal the compiled reference's state (golden fixtures)."""
+must equal the compiled reference's state (golden fixtures), and the Tecplot
+file it writes at the last step through the library's device-side output must
+equal the reference's own file byte for byte (tests/golden/plot_*)."""
+import math
 import os
 import subprocess
 import sys
@@ -11,7 +14,7 @@ import pytest
 
 from conftest import REPO, golden_names, load_golden
 
-from chorus_mhd import QuadInp, read_restart, write_restart
+from chorus_mhd import QuadInp, quad_grid, read_restart, write_restart
 
 pytestmark = pytest.mark.gpu
 
@@ -26,6 +29,7 @@ def test_fortran_dropin_matches_reference(name, tmp_path):
     mesh, N, steps = g["mesh"], int(g["N"]), int(g["steps"])
     d = str(tmp_path)
     mesh.write(d)
+    os.makedirs(os.path.join(d, "TEC"))
     write_restart(os.path.join(d, "restart.in"), 0, 0.0, g["q0"])
     QuadInp(N=N, dt=float(g["dt"]), MAXITER=steps, DXCYCL=mesh.dxcycl, DYCYCL=mesh.dycycl,
             ARTIV=int(g["artiv"]), k_stage=int(g["k_stage"]), restart=1, restart_ord=N,
@@ -41,3 +45,32 @@ def test_fortran_dropin_matches_reference(name, tmp_path):
     lines = open(os.path.join(d, "resid.out")).read().split("\n")
     row = lines[1].split()
     assert int(row[0]) == 1
+
+
+PLOT_CASES = ["plot_ot_8x6_N4_nre3", "plot_ot_6x6_N3_free_nre2"]
+
+
+@pytest.mark.skipif(not os.path.exists(DROPIN), reason="drop-in driver not built (needs the reference sources)")
+@pytest.mark.parametrize("name", PLOT_CASES)
+def test_fortran_dropin_tecplot_matches_reference(name, tmp_path):
+    """Same steps from the same state as the reference run that wrote the golden
+    .plt (restart from its initial state; the reference started from its own
+    initial condition, which is that state)."""
+    print("case dropin tecplot", name, flush=True)
+    g = np.load(os.path.join(REPO, "tests", "golden", name + ".npz"))
+    mesh = quad_grid(int(g["nx"]), int(g["ny"]), 0.0, 2 * math.pi, 0.0, 2 * math.pi, jitter=float(g["jitter"]),
+                     seed=int(g["seed"]), bc=str(g["bc"]))
+    N, steps = int(g["N"]), int(g["steps"])
+    d = str(tmp_path)
+    mesh.write(d)
+    os.makedirs(os.path.join(d, "TEC"))
+    write_restart(os.path.join(d, "restart.in"), 0, 0.0, g["q0"])
+    QuadInp(N=N, dt=float(g["dt"]), MAXITER=steps, DXCYCL=mesh.dxcycl, DYCYCL=mesh.dycycl, ARTIV=1,
+            NRE=int(g["nre"]), restart=1, restart_ord=N, NAMERESTART="restart.in",
+            nwrite=1000000).write(os.path.join(d, "QUAD.INP"))
+    r = subprocess.run([DROPIN], cwd=d, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    out = os.path.join(d, "TEC", str(g["plt_name"]))
+    assert os.path.exists(out), sorted(os.listdir(os.path.join(d, "TEC")))
+    ref = os.path.join(REPO, "tests", "golden", name + ".plt")
+    assert open(out, "rb").read() == open(ref, "rb").read()
