@@ -198,6 +198,24 @@ class Solver:
         """Tecplot TDV111 file of the current state (tecplotter2d + tecpost_parallel)."""
         self._check(_lib.load().chorus_write_tecplot(self._ctx, path.encode(), float(ctime)), "chorus_write_tecplot")
 
+    def write_restart(self, path: str, it: int, ctime: float, icg=None) -> None:
+        """Restart dump of the current state (WRITE_ALL_DATA_PARALLEL,
+        write_data.f90:5-83), packed on the device; icg: global cell numbers
+        (default 1..NCELL)."""
+        ip = None
+        if icg is not None:
+            icg = np.ascontiguousarray(icg, dtype=np.int32)
+            if icg.shape != (self.mesh.ncell,):
+                raise ValueError("icg needs one entry per cell")
+            ip = icg.ctypes.data_as(C.POINTER(C.c_int))
+        self._check(_lib.load().chorus_write_restart(self._ctx, path.encode(), int(it), float(ctime), ip),
+                    "chorus_write_restart")
+
+    @staticmethod
+    def restart_name(it: int) -> str:
+        """The reference's restart file name (write_data.f90:44)."""
+        return "restart.%08d.dat" % it
+
     @staticmethod
     def tecplot_name(ctime: float) -> str:
         """The reference's file name for solution time ctime (tecplotter.f90:190-191):
@@ -292,6 +310,17 @@ class Batch:
         Q = np.empty(self.shape(i))
         self._check(_lib.load().chorus_batch_get_state(self._b, i, _lib.ptr(Q)), "chorus_batch_get_state")
         return Q
+
+    def write_restart(self, i: int, path: str, it: int, ctime: float, icg=None) -> None:
+        """Restart dump of instance i (write_data.f90:5-83), as Solver.write_restart."""
+        ip = None
+        if icg is not None:
+            icg = np.ascontiguousarray(icg, dtype=np.int32)
+            if icg.shape != (self.shape(i)[0],):
+                raise ValueError("icg needs one entry per cell")
+            ip = icg.ctypes.data_as(C.POINTER(C.c_int))
+        self._check(_lib.load().chorus_batch_write_restart(self._b, i, path.encode(), int(it), float(ctime), ip),
+                    "chorus_batch_write_restart")
 
     def compresid(self, i: int) -> np.ndarray:
         R = np.empty(self.shape(i))

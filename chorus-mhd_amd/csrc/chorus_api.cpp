@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -56,6 +57,8 @@ struct chorus_ctx {
     double *pl_H = nullptr, *pl_q = nullptr, *pl_av = nullptr;
     double *mucell = nullptr;      // viscous path with AV: muAV(1) per cell of the last stage's input
     double *sm6 = nullptr;         // viscous path with AV: By indicator of the last stage's input
+    // restart dump image (chorus_write_restart), allocated on first use
+    unsigned *rs_img = nullptr;
 };
 
 namespace {
@@ -581,6 +584,7 @@ void chorus_free(chorus_ctx *c) {
     (void)hipFree(c->pl_av);
     (void)hipFree(c->mucell);
     (void)hipFree(c->sm6);
+    (void)hipFree(c->rs_img);
     delete c->plot;
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -785,6 +789,54 @@ int chorus_write_tecplot(chorus_ctx *c, const char *path, double ctime) {
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     std::string err;
     const int rc = write_tecplot_file(path, ctime, t, q.data(), avh.data(), err);
+    return rc ? fail(c, rc, err) : 0;
+}
+
+namespace {
+// restart dump of ncell cells of c's N starting at device state Q; the image
+// buffer c->rs_img holds all of c's cells (allocated on first use)
+int restart_dump(chorus_ctx *c, int ncell, const double *Q, const char *path, int iter, double ctime,
+                 const int *icg, std::string &err) {
+    auto hip_fail = [&](const char *what, hipError_t e) {
+        err = std::string(what) + ": " + hipGetErrorString(e);
+        return CHORUS_ERR_HIP;
+    };
+    const size_t rec = 1 + 2 * (size_t)9 * c->N * c->N, words = rec * ncell;
+    hipError_t e;
+    if (!c->rs_img && (e = hipMalloc(&c->rs_img, rec * c->ncell * sizeof(unsigned))) != hipSuccess)
+        return hip_fail("restart image", e);
+    if ((e = launch_restart_image(ncell, c->N, Q, c->rs_img, c->stream)) != hipSuccess)
+        return hip_fail("restart launch", e);
+    std::vector<unsigned> img(words);
+    if ((e = hipMemcpyAsync(img.data(), c->rs_img, words * sizeof(unsigned), hipMemcpyDeviceToHost, c->stream)) !=
+            hipSuccess ||
+        (e = hipStreamSynchronize(c->stream)) != hipSuccess)
+        return hip_fail("restart copy", e);
+    if (icg)
+        for (int i = 0; i < ncell; i++) img[(size_t)i * rec] = (unsigned)icg[i];
+    // header: int32 iter, FP64 ctime (write_data.f90:47-50), then the records
+    unsigned char head[12];
+    const int32_t it = iter;
+    std::memcpy(head, &it, 4);
+    std::memcpy(head + 4, &ctime, 8);
+    FILE *f = std::fopen(path, "wb");
+    if (!f) {
+        err = std::string("cannot open ") + path;
+        return CHORUS_ERR_ARG;
+    }
+    const bool ok = std::fwrite(head, 1, 12, f) == 12 && std::fwrite(img.data(), sizeof(unsigned), words, f) == words;
+    if (std::fclose(f) != 0 || !ok) {
+        err = std::string("write failed: ") + path;
+        return CHORUS_ERR_ARG;
+    }
+    return 0;
+}
+}  // namespace
+
+int chorus_write_restart(chorus_ctx *c, const char *path, int iter, double ctime, const int *icg) {
+    if (!c || !path) return fail(c, CHORUS_ERR_ARG, "null argument");
+    std::string err;
+    const int rc = restart_dump(c, c->ncell, c->X[c->cur], path, iter, ctime, icg, err);
     return rc ? fail(c, rc, err) : 0;
 }
 
@@ -1070,6 +1122,16 @@ int chorus_batch_get_state(chorus_batch *b, int i, double *q) {
         for (int p = 0; p < NP; p++)
             for (int v = 0; v < 9; v++) q[((size_t)ic * NP + p) * 9 + v] = h[((size_t)ic * 9 + v) * NP + p];
     return 0;
+}
+
+int chorus_batch_write_restart(chorus_batch *b, int i, const char *path, int iter, double ctime, const int *icg) {
+    if (!b || !path || i < 0 || i >= b->ninst) return bfail(b, CHORUS_ERR_ARG, "bad instance");
+    const chorus_batch::Member &m = b->mem[i];
+    chorus_ctx *c = b->groups[m.group];
+    HIP_TRYB(b, hipStreamSynchronize(b->stream));
+    std::string err;
+    const int rc = restart_dump(c, m.ncell, c->X[c->cur] + (size_t)m.off * 9 * m.N * m.N, path, iter, ctime, icg, err);
+    return rc ? bfail(b, rc, err) : 0;
 }
 
 int chorus_batch_step(chorus_batch *b, int nsteps) {

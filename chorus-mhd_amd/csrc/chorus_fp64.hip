@@ -156,4 +156,35 @@ hipError_t launch_plot_nodes(const PlotArgs &p, hipStream_t s) {
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Restart dump image (write_data.f90:44-74): per cell one record of an int32
+// global cell number followed by Q(1:9,1:N,1:N) of the cell in Fortran order,
+// records back to back (4-byte aligned, so each FP64 value is stored as two
+// 32-bit words).  One thread per value: it reads Q[c][v][js][is] of the device
+// layout and writes the value's two words at its place in the record; thread 0
+// of the cell also writes the cell number c+1.  The 12-byte header (iter,
+// ctime) is the host's.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_restart_image(int ncell, int N, const double *__restrict__ Q,
+                                                       unsigned *__restrict__ img) {
+    const int NP = N * N, NQ = 9 * NP;
+    const size_t rec = 1 + 2 * (size_t)NQ;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)ncell * NQ) return;
+    const int c = (int)(t / NQ), d = (int)(t % NQ);
+    const int v = d % 9, p = d / 9;                      // p = is + N*js
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(Q[(size_t)c * NQ + v * NP + p]);
+    unsigned *r = img + (size_t)c * rec;
+    r[1 + 2 * d] = (unsigned)bits;
+    r[2 + 2 * d] = (unsigned)(bits >> 32);
+    if (d == 0) r[0] = (unsigned)(c + 1);
+}
+
+hipError_t launch_restart_image(int ncell, int N, const double *Q, unsigned *img, hipStream_t s) {
+    const size_t n = (size_t)ncell * 9 * N * N;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_restart_image, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ncell, N, Q, img);
+    return hipGetLastError();
+}
+
 }  // namespace chorus

@@ -206,6 +206,9 @@ struct PlotArgs {
     double *q, *av;
 };
 hipError_t launch_plot_nodes(const PlotArgs &p, hipStream_t s);
+// restart dump records (write_data.f90:49-74) of Q[cell][var][js][is]: per cell
+// the 32-bit words {c+1, Q(1:9,1:N,1:N) as FP64 in Fortran order}
+hipError_t launch_restart_image(int ncell, int N, const double *Q, unsigned *img, hipStream_t s);
 // hval of the N solution points at the NRE+1 equally spaced plot coordinates
 // (tecplotter.f90:64-82): H[ii*N+is]
 void host_plot_basis(int N, int nre, double *H);

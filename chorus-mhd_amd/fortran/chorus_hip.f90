@@ -6,17 +6,18 @@
 ! setinitialcond.f90:82-92).  This module keeps that shape: after INIT_SETUP
 ! the driver calls chorus_gpu_setup once; each iteration of the time loop calls
 ! chorus_gpu_step (stepper + CT on the GPU) instead of RK_S5P3/FiveStageRk and
-! correct_B_via_curl_Az; chorus_gpu_fetch_state copies Q back into setup2d's
-! array before the restart dump; chorus_gpu_plot_setup / chorus_gpu_tecplot
-! replace the Tecplot output chain.  The state stays resident on the GPU in
-! between.
+! correct_B_via_curl_Az; chorus_gpu_write_restart replaces the restart dump
+! (WRITE_ALL_DATA_PARALLEL) and chorus_gpu_plot_setup / chorus_gpu_tecplot the
+! Tecplot output chain, both written from the resident state;
+! chorus_gpu_fetch_state copies Q back into setup2d's array for any other use.
+! The state stays resident on the GPU in between.
 module chorus_hip
     use iso_c_binding
     implicit none
     private
     public :: chorus_gpu_setup, chorus_gpu_step, chorus_gpu_compresid, chorus_gpu_correct_b, &
               chorus_gpu_fetch_state, chorus_gpu_diag, chorus_gpu_free, chorus_gpu_plot_setup, &
-              chorus_gpu_tecplot
+              chorus_gpu_tecplot, chorus_gpu_write_restart
 
     integer(c_int), parameter, public :: CHORUS_FP64 = 0, CHORUS_MIXED = 1
 
@@ -96,6 +97,14 @@ module chorus_hip
             character(kind=c_char), dimension(*), intent(in) :: path
             real(c_double), value :: ctime
         end function
+        integer(c_int) function c_chorus_write_restart(ctx, path, iter, ctime, icg) bind(c, name='chorus_write_restart')
+            import :: c_int, c_ptr, c_char, c_double
+            type(c_ptr), value :: ctx
+            character(kind=c_char), dimension(*), intent(in) :: path
+            integer(c_int), value :: iter
+            real(c_double), value :: ctime
+            integer(c_int), intent(in) :: icg(*)
+        end function
         integer(c_size_t) function c_strlen(s) bind(c, name='strlen')
             import :: c_size_t, c_ptr
             type(c_ptr), value :: s
@@ -172,6 +181,18 @@ contains
                    'chorus_write_tecplot')
     end subroutine
 
+    ! WRITE_ALL_DATA_PARALLEL (write_data.f90:5-83) of the resident state:
+    ! restart.<iter:8>.dat with setup2d's iter, ctime and global cell numbers
+    subroutine chorus_gpu_write_restart()
+        use setup2d, only: NCELL, IC2ICG, iter, ctime
+        character(64) :: filename
+        integer(c_int), allocatable :: icg(:)
+        icg = IC2ICG(1:NCELL)
+        write(filename, '(A,I8.8,A)') 'restart.', iter, '.dat'
+        call check(c_chorus_write_restart(ctx, trim(filename)//c_null_char, int(iter, c_int), &
+                   real(ctime, c_double), icg), 'chorus_write_restart')
+    end subroutine
+
     ! loop body of iterations.f90:22-33 on the GPU: RK_S5P3 / FiveStageRk + correct_B_via_curl_Az
     subroutine chorus_gpu_step(nsteps)
         integer, intent(in) :: nsteps
@@ -188,7 +209,7 @@ contains
         call check(c_chorus_correct_b(ctx), 'chorus_correct_b')
     end subroutine
 
-    ! copy the resident state into setup2d's Q (before Tecplot/restart output)
+    ! copy the resident state into setup2d's Q
     subroutine chorus_gpu_fetch_state()
         use setup2d, only: Q
         call check(c_chorus_get_state(ctx, Q), 'chorus_get_state')

@@ -9,9 +9,10 @@
 ! tecplotter2d and tecpost_parallel, into ./TEC/, which must exist as for the
 ! reference; with ARTIV=0 its muAV/etaAV columns are zeros, where the
 ! reference reads the unallocated muAV, tecplotter.f90:37-38) and the restart
-! dump (WRITE_ALL_DATA_PARALLEL).
+! dump (the library's device-side WRITE_ALL_DATA_PARALLEL; with "refio" the
+! state is copied back and the reference's own writer runs).
 !
-! usage: sd2d_mhd_hip [mixed]      (FP64 drop-in unless "mixed" is given)
+! usage: sd2d_mhd_hip [mixed] [refio]   (FP64 drop-in unless "mixed" is given)
 program sd2d_mhd_hip
     use setup2d
     use chorus_hip
@@ -21,6 +22,8 @@ program sd2d_mhd_hip
 
     character(16) :: arg
     integer(c_int) :: prec
+    logical :: refio
+    integer :: ia
     real(c_double) :: residglob, divbglob, divbmax, eigv, eps0
     double precision :: t_start, t_finish
     integer :: fh
@@ -31,8 +34,12 @@ program sd2d_mhd_hip
     NPROC = size
     if (NPROC /= 1) stop 'sd2d_mhd_hip runs one rank: the GPU holds the whole mesh'
     prec = CHORUS_FP64
-    call get_command_argument(1, arg)
-    if (trim(arg) == 'mixed') prec = CHORUS_MIXED
+    refio = .false.
+    do ia = 1, command_argument_count()
+        call get_command_argument(ia, arg)
+        if (trim(arg) == 'mixed') prec = CHORUS_MIXED
+        if (trim(arg) == 'refio') refio = .true.
+    end do
 
     t_start = MPI_WTIME()
     call readinput
@@ -66,8 +73,12 @@ program sd2d_mhd_hip
         end if
         if (mod(iter, nwrite) == 1 .or. iter >= MAXITER) then
             call chorus_gpu_tecplot(ctime)
-            call chorus_gpu_fetch_state()
-            call WRITE_ALL_DATA_PARALLEL
+            if (refio) then
+                call chorus_gpu_fetch_state()
+                call WRITE_ALL_DATA_PARALLEL
+            else
+                call chorus_gpu_write_restart()
+            end if
         end if
     end do
     close(fh)

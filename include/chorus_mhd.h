@@ -133,6 +133,15 @@ int chorus_plot_setup(chorus_ctx *ctx, const chorus_mesh *mesh, int nre);
 int chorus_plot_topology(const chorus_mesh *mesh, int nre, int *nnode, int *ncellp, double *xy, int *c2n);
 int chorus_write_tecplot(chorus_ctx *ctx, const char *path, double ctime);
 
+/* Restart dump, replacing WRITE_ALL_DATA_PARALLEL (write_data.f90:5-83) for one
+ * partition: the current state is packed into the dump's record layout on the
+ * device and written to path (the reference's name is restart.%08d.dat of
+ * iter): int32 iter, FP64 ctime, then per cell an int32 global cell number and
+ * Q(1:9,1:N,1:N) in Fortran order.  icg[NCELL] holds the global cell numbers
+ * (IC2ICG); null writes 1..NCELL.  The file is truncated first (the reference's
+ * MPI_MODE_CREATE leaves the tail of a longer existing file in place). */
+int chorus_write_restart(chorus_ctx *ctx, const char *path, int iter, double ctime, const int *icg);
+
 /* Tile-shape autotuner (no reference counterpart): times the mixed form's
  * cell kernels in every compiled tile shape on this context's own mesh and
  * state (residual evaluations, no state change), keeps the fastest for later
@@ -167,6 +176,8 @@ int chorus_batch_get_state(chorus_batch *b, int i, double *q);
 int chorus_batch_step(chorus_batch *b, int nsteps);
 /* residual of instance i's current state (compresid.f90:1-67), as chorus_compresid */
 int chorus_batch_compresid(chorus_batch *b, int i, double *resid);
+/* restart dump of instance i (write_data.f90:5-83), as chorus_write_restart */
+int chorus_batch_write_restart(chorus_batch *b, int i, const char *path, int iter, double ctime, const int *icg);
 /* correct_B_via_curl_Az (setinitialcond.f90:82-92) of every instance */
 int chorus_batch_correct_b(chorus_batch *b);
 /* max |div B| per instance (iterations.f90:35-48), divbmax[ninst] */

@@ -11,7 +11,7 @@ import sys
 import numpy as np
 import pytest
 
-from chorus_mhd import quad_grid
+from chorus_mhd import quad_grid, write_restart
 from chorus_mhd import ic as ics
 from chorus_mhd.solver import Batch, Instance, Solver
 
@@ -52,7 +52,7 @@ def _initial(inst, kind):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("artiv", [0, 1])
-def test_batch_equals_single_instances(precision, artiv):
+def test_batch_equals_single_instances(precision, artiv, tmp_path):
     _say("batch", precision, "artiv", artiv)
     mem = _members(precision, artiv)
     q0 = [_initial(m, k) for m, k in mem]
@@ -69,6 +69,10 @@ def test_batch_equals_single_instances(precision, artiv):
         ref = _single(m, q0[i], 7)
         got = b.get_state(i)
         assert np.array_equal(got, ref), "instance %d differs (max %g)" % (i, np.abs(got - ref).max())
+        # the instance's restart dump, packed on the device, equals the host-written one
+        b.write_restart(i, str(tmp_path / "dev.dat"), 7, 0.5)
+        write_restart(str(tmp_path / "host.dat"), 7, 0.5, got)
+        assert (tmp_path / "dev.dat").read_bytes() == (tmp_path / "host.dat").read_bytes(), "dump of instance %d" % i
     b.close()
 
 

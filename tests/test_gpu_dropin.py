@@ -1,11 +1,13 @@
 """The Fortran drop-in end to end: the reference's own set-up chain, restart
 reader/writer and loop structure, with the time step on the GPU through the
 iso_c_binding module (chorus-mhd_amd/fortran).  Its restart dump after k steps
-must equal the compiled reference's state (golden fixtures), and the Tecplot
+must equal the compiled reference's state (golden fixtures) and the file the
+reference's own writer makes of the same state, and the Tecplot
 file it writes at the last step through the library's device-side output must
 equal the reference's own file byte for byte (tests/golden/plot_*)."""
 import math
 import os
+import shutil
 import subprocess
 import sys
 
@@ -36,6 +38,19 @@ def test_fortran_dropin_matches_reference(name, tmp_path):
             NAMERESTART="restart.in", nwrite=1000000).write(os.path.join(d, "QUAD.INP"))
     r = subprocess.run([DROPIN], cwd=d, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    # the same run with the reference's own restart writer on the copied-back
+    # state: the device-packed dumps must equal its files byte for byte
+    d2 = os.path.join(d, "refio")
+    os.makedirs(os.path.join(d2, "TEC"))
+    for f in os.listdir(d):
+        if f in ("restart.in", "QUAD.INP") or f.endswith(".dat") and not f.startswith("restart."):
+            shutil.copy(os.path.join(d, f), d2)
+    r = subprocess.run([DROPIN, "refio"], cwd=d2, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    dumps = sorted(f for f in os.listdir(d2) if f.startswith("restart.0"))
+    assert dumps == sorted(f for f in os.listdir(d) if f.startswith("restart.0")) and len(dumps) == 2
+    for f in dumps:
+        assert open(os.path.join(d, f), "rb").read() == open(os.path.join(d2, f), "rb").read(), f
     it, t, Q = read_restart(os.path.join(d, "restart.%08d.dat" % steps), N)
     assert it == steps
     if int(g["artiv"]) == 0:
