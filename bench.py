@@ -201,6 +201,50 @@ def run_crossings(w, args):
     print(json.dumps(out), flush=True)
 
 
+def run_output(w, args):
+    """Output path at the workload (FP64 drop-in state after one step): the
+    restart dump (chorus_write_restart) and the Tecplot file of NRE x NRE plot
+    cells per cell (chorus_write_tecplot), each the mean wall time of a few
+    calls including the file write, beside the host route the driver would
+    otherwise take for the dump (download + transpose + host pack and write)."""
+    from chorus_mhd import write_restart
+    mesh, topo, s, artiv = setup(w, "fp64", True)
+    Q0 = initial_state(w, mesh, s)
+    s.close()
+    N, nre, reps = w["N"], args.nre, 3
+    s = Solver(mesh, N, 1e-4, gam=w["gam"], artiv=artiv, topo=topo)
+    t0 = time.perf_counter()
+    s.plot_setup(nre)
+    t_setup = time.perf_counter() - t0
+    s.set_state(Q0)
+    s.step(1)
+    res = {}
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "out")
+        for name, fn in (("restart", lambda: s.write_restart(path, 1, 1e-4)),
+                         ("tecplot", lambda: s.write_tecplot(path, 1e-4)),
+                         ("restart_host_route", lambda: write_restart(path, 1, 1e-4, s.get_state()))):
+            fn()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn()
+            res[name] = ((time.perf_counter() - t0) / reps, os.path.getsize(path))
+            os.remove(path)
+    s.close()
+    nnode = (w["n"] * nre + 1) * (w["m"] * nre + 1) if w["bc"] == "CYCL" else None
+    out = {
+        "metric": "output wall time per dump", "unit": "s", "higher_is_better": False, "n_gpus": 1,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": w["desc"] + (", AV on" if artiv else ", AV off"), "ncell": mesh.ncell, "N": N,
+                   "nre": nre, "plot_nodes": nnode, "calls_timed": reps},
+        "restart_s": res["restart"][0], "restart_bytes": res["restart"][1],
+        "restart_host_route_s": res["restart_host_route"][0],
+        "tecplot_s": res["tecplot"][0], "tecplot_bytes": res["tecplot"][1],
+        "plot_setup_s": t_setup,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def b_stage(N: int) -> float:
     """Algorithmic HBM bytes per DOF per stage: FP64 state + RK registers + metrics
     (224 + 32(N+1)/N)."""
@@ -272,6 +316,9 @@ def main():
                     help="first run two kernels of known HBM bytes (a 256 MiB fill: 256 MiB written; an "
                          "in-place add of two 256 MiB arrays: 512 MiB read, 256 MiB written) so that "
                          "hardware byte counters collected over this run can be scaled")
+    ap.add_argument("--output", action="store_true",
+                    help="time the output path (restart dump, Tecplot file) at the workload instead of steps")
+    ap.add_argument("--nre", type=int, default=4, help="plot cells per cell edge of the Tecplot file (--output)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: this build holds the whole mesh as one "
@@ -281,6 +328,9 @@ def main():
         run_ensemble(args)
         return
     w = WORKLOADS[args.workload]
+    if args.output:
+        run_output(w, args)
+        return
     if "t_end" in w:
         run_crossings(w, args)
         return
