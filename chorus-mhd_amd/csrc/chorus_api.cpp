@@ -54,7 +54,7 @@ struct chorus_ctx {
     // Tecplot output (chorus_plot_setup / chorus_write_tecplot)
     PlotTopology *plot = nullptr;
     int *pl_cstart = nullptr, *pl_contrib = nullptr, *pl_factor = nullptr;
-    double *pl_H = nullptr, *pl_q = nullptr, *pl_av = nullptr;
+    double *pl_H = nullptr, *pl_q = nullptr, *pl_av = nullptr, *pl_V = nullptr;
     double *mucell = nullptr;      // viscous path with AV: muAV(1) per cell of the last stage's input
     double *sm6 = nullptr;         // viscous path with AV: By indicator of the last stage's input
     // restart dump image (chorus_write_restart), allocated on first use
@@ -582,6 +582,7 @@ void chorus_free(chorus_ctx *c) {
     (void)hipFree(c->pl_H);
     (void)hipFree(c->pl_q);
     (void)hipFree(c->pl_av);
+    (void)hipFree(c->pl_V);
     (void)hipFree(c->mucell);
     (void)hipFree(c->sm6);
     (void)hipFree(c->rs_img);
@@ -730,10 +731,10 @@ int chorus_plot_setup(chorus_ctx *c, const chorus_mesh *m, int nre) {
     delete c->plot;
     c->plot = t;
     for (void *p : {(void *)c->pl_cstart, (void *)c->pl_contrib, (void *)c->pl_factor, (void *)c->pl_H,
-                    (void *)c->pl_q, (void *)c->pl_av})
+                    (void *)c->pl_q, (void *)c->pl_av, (void *)c->pl_V})
         (void)hipFree(p);
     c->pl_cstart = c->pl_contrib = c->pl_factor = nullptr;
-    c->pl_H = c->pl_q = c->pl_av = nullptr;
+    c->pl_H = c->pl_q = c->pl_av = c->pl_V = nullptr;
     const size_t nn = (size_t)t->nnode;
     std::vector<double> H((size_t)(nre + 1) * c->N);
     host_plot_basis(c->N, nre, H.data());
@@ -743,6 +744,7 @@ int chorus_plot_setup(chorus_ctx *c, const chorus_mesh *m, int nre) {
     HIP_TRY(c, hipMalloc(&c->pl_H, H.size() * sizeof(double)));
     HIP_TRY(c, hipMalloc(&c->pl_q, nn * 9 * sizeof(double)));
     HIP_TRY(c, hipMalloc(&c->pl_av, nn * 2 * sizeof(double)));
+    HIP_TRY(c, hipMalloc(&c->pl_V, (size_t)c->ncell * (nre + 1) * (nre + 1) * 9 * sizeof(double)));
     HIP_TRY(c, hipMemcpy(c->pl_cstart, t->cstart.data(), (nn + 1) * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->pl_contrib, t->contrib.data(), t->contrib.size() * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->pl_factor, t->factor.data(), t->factor.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -781,6 +783,8 @@ int chorus_write_tecplot(chorus_ctx *c, const char *path, double ctime) {
     a.pi = std::atan(1.0) * 4.0;
     a.q = c->pl_q;
     a.av = c->pl_av;
+    a.V = c->pl_V;
+    a.ncell = c->ncell;
     hipError_t e = launch_plot_nodes(a, c->stream);
     if (e != hipSuccess) return fail(c, CHORUS_ERR_HIP, std::string("plot launch: ") + hipGetErrorString(e));
     std::vector<double> q((size_t)t.nnode * 9), avh((size_t)t.nnode * 2);

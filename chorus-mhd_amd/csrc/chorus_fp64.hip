@@ -97,34 +97,53 @@ hipError_t launch_stage_visc_fp64(int N, const StageArgs &a, const ViscArgs &v, 
 }
 
 // ---------------------------------------------------------------------------
-// Plot-node values of the Tecplot output (tecplotter2d, tecplotter.f90:19-58),
-// one thread per plot node gathering its cells' contributions in the order
-// the reference adds them (cells ascending), so the node sums are the
-// reference's bits: Qs = sum_js sum_is Q(is,js)*h(is,xsi)*h(js,eta), summed
-// over the cells of the node, divided by their count, then the primitive
-// variables; muAV(1) and muAV(6) likewise.
+// Plot-node values of the Tecplot output (tecplotter2d, tecplotter.f90:19-58)
+// in two passes with the reference's bits:
+//   k_plot_cells: one thread per (cell, variable) holds the variable's N^2
+//     values in registers and forms the cell's contribution to every plot
+//     point, Qs = sum_js sum_is Q(is,js)*h(is,xsi)*h(js,eta) (js outer, is
+//     inner, as the reference), into V[cell][point][var];
+//   k_plot_nodes: one thread per plot node adds its contributions in the order
+//     the reference does (cells ascending), divides by their count and forms
+//     the primitive variables; muAV(1) and muAV(6) likewise.
 // ---------------------------------------------------------------------------
+template <int N> __global__ void __launch_bounds__(256) k_plot_cells(PlotArgs p) {
+    constexpr int NP = N * N;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= p.ncell * NV) return;
+    const int c = t / NV, v = t % NV, W = p.nre + 1;
+    const double *qp = p.Q + (size_t)c * NV * NP + v * NP;
+    double q[NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) q[k] = qp[k];
+    double *out = p.V + (size_t)c * W * W * NV + v;
+    for (int jj = 0; jj < W; jj++) {
+        const double *hj = p.H + jj * N;
+        for (int ii = 0; ii < W; ii++) {
+            const double *hi = p.H + ii * N;
+            double qs = 0.0;
+#pragma unroll
+            for (int js = 0; js < N; js++) {
+                const double hv = hj[js];
+#pragma unroll
+                for (int is = 0; is < N; is++) qs = qs + q[js * N + is] * hi[is] * hv;
+            }
+            out[(size_t)(jj * W + ii) * NV] = qs;
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) k_plot_nodes(PlotArgs p) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= p.nnode) return;
-    const int N = p.N, NP = N * N, NQ = NV * NP, W = p.nre + 1;
+    const int W = p.nre + 1, N = p.N;
     double sum[NV], a1 = 0.0, a2 = 0.0;
 #pragma unroll
     for (int v = 0; v < NV; v++) sum[v] = 0.0;
     const double eps0 = p.artiv ? p.eig[0] * p.h_msh / (double)(N - 1) * 1.0 : 0.0;
     for (int k = p.cstart[n]; k < p.cstart[n + 1]; k++) {
         const int c = p.contrib[2 * k], ij = p.contrib[2 * k + 1];
-        const double *hi = p.H + (ij % W) * N, *hj = p.H + (ij / W) * N;
-        const double *q = p.Q + (size_t)c * NQ;
-        double qs[NV];
-#pragma unroll
-        for (int v = 0; v < NV; v++) qs[v] = 0.0;
-        for (int js = 0; js < N; js++)
-            for (int is = 0; is < N; is++) {
-                const double hh = hi[is], hv = hj[js];
-#pragma unroll
-                for (int v = 0; v < NV; v++) qs[v] = qs[v] + q[v * NP + js * N + is] * hh * hv;
-            }
+        const double *qs = p.V + ((size_t)c * W * W + ij) * NV;
 #pragma unroll
         for (int v = 0; v < NV; v++) sum[v] = sum[v] + qs[v];
         if (p.artiv) {
@@ -152,7 +171,13 @@ __global__ void __launch_bounds__(256) k_plot_nodes(PlotArgs p) {
 }
 
 hipError_t launch_plot_nodes(const PlotArgs &p, hipStream_t s) {
-    hipLaunchKernelGGL(k_plot_nodes, dim3((p.nnode + 255) / 256), dim3(256), 0, s, p);
+    const int nt = p.ncell * NV;
+#define L_(n)                                                                                         \
+    if (nt) hipLaunchKernelGGL(k_plot_cells<n>, dim3((nt + 255) / 256), dim3(256), 0, s, p); \
+    break;
+    CHORUS_DISPATCH_N(p.N, L_)
+#undef L_
+    if (p.nnode) hipLaunchKernelGGL(k_plot_nodes, dim3((p.nnode + 255) / 256), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 

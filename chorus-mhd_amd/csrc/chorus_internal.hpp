@@ -198,11 +198,12 @@ int write_tecplot_file(const char *path, double ctime, const PlotTopology &t, co
 // plot-node values (tecplotter.f90:19-58) on the device: 9 primitive plot
 // variables q[v*nnode+n] and muAV(1), muAV(6) in av[k*nnode+n]
 struct PlotArgs {
-    int nnode, N, nre, artiv;
+    int nnode, N, nre, artiv, ncell;
     const int *cstart, *contrib, *factor;   // factor 1-based
     const double *Q, *H;                    // state [cell][var][js][is]; H[ii*N+is] = hval(is, ii/nre)
     const double *mucell, *sm6, *eig;       // muAV(1) per cell, By indicator per cell, stage maximum
     double h_msh, gam, pi;                  // epsilon0 = eig * h_msh / (N-1) (AV.f90:147-148)
+    double *V;                              // each cell's contribution per plot point [cell][point][var]
     double *q, *av;
 };
 hipError_t launch_plot_nodes(const PlotArgs &p, hipStream_t s);

@@ -124,7 +124,8 @@ int chorus_time_steps(chorus_ctx *ctx, int nsteps, double *ms_total, double *ms_
  * array).  chorus_write_tecplot forms the node values of the current state on
  * the device and writes the TDV111 binary file at path (the reference writes
  * ./TEC/TEC<int(ctime):7><frac(ctime):4>.plt; the caller chooses the path)
- * with solution time ctime. */
+ * with solution time ctime.  The plot set-up holds about NCELL (NRE+1)^2 x 9 + 13 x nnode
+ * doubles of device memory (each cell's contributions, node values). */
 int chorus_plot_setup(chorus_ctx *ctx, const chorus_mesh *mesh, int nre);
 /* Host only (no device): the plot nodes of chorus_plot_setup.  nnode and
  * ncellp receive the node and plot-cell counts; xy (2 x (NRE+1)^2 NCELL

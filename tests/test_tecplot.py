@@ -92,3 +92,38 @@ def test_tecplot_mixed_within_budget(name, tmp_path):
         ref = fb["data"][v].astype(np.float64)
         err = np.abs(fa["data"][v] - ref).max() / max(np.abs(ref).max(), 1e-30)
         assert err < 1e-6, (fb["names"][v], err)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("N,nre,bc", [(2, 1, "FREE"), (5, 3, "CYCL"), (6, 4, "FREE"), (8, 2, "CYCL")])
+def test_tecplot_linear_state_exact(N, nre, bc, tmp_path):
+    """Orders and plot resolutions the reference files do not cover: a state
+    linear in x and y is bilinear in each cell's reference coordinates, so the
+    interpolation to every plot node, the averaging over the cells sharing it
+    and the primitive variables reproduce the closed form at the node's
+    coordinates (to FP32 rounding); the AV columns are zero without AV."""
+    print("tecplot linear", N, nre, bc, flush=True)
+    from chorus_mhd.ic import solution_xy
+    mesh = quad_grid(7, 5, 0.0, 2.0, 0.0, 1.0, jitter=0.12, seed=4, bc=bc)
+    gam = 5.0 / 3
+    xy = solution_xy(mesh, N)
+    coef = np.array([[2.0, 0.1, 0.05], [0.3, 0.2, -0.1], [-0.2, 0.05, 0.1], [0.1, -0.03, 0.02], [9.0, 0.4, 0.3],
+                     [0.5, 0.1, 0.0], [-0.4, 0.0, 0.2], [0.2, 0.05, 0.05], [0.1, 0.3, -0.2]])
+    Q = coef[:, 0] + xy[..., 0:1] * coef[:, 1] + xy[..., 1:2] * coef[:, 2]
+    s = Solver(mesh, N, 1e-4, gam=gam)
+    s.plot_setup(nre)
+    s.set_state(Q)
+    out = str(tmp_path / "lin.plt")
+    s.write_tecplot(out, 0.5)
+    s.close()
+    f = read_plt(out)
+    pxy, c2n = plot_topology(mesh, nre)
+    assert np.array_equal(f["data"][:2], pxy.astype(np.float32)) and np.array_equal(f["conn"], c2n - 1)
+    q = coef[:, 0:1] + pxy[0][None] * coef[:, 1:2] + pxy[1][None] * coef[:, 2:3]
+    u = q[1:4] / q[0]
+    p = (q[4] - 0.5 * q[0] * (u * u).sum(0) - 0.5 * (q[5:8] ** 2).sum(0)) * (gam - 1.0)
+    want = np.concatenate([q[0:1], u, p[None], q[5:9]])
+    for k in range(9):
+        err = np.abs(f["data"][2 + k] - want[k]).max() / np.abs(want[k]).max()
+        assert err < 2e-6, (f["names"][2 + k], err)
+    assert not f["data"][11:].any()
