@@ -302,15 +302,52 @@ __device__ __forceinline__ void instance_max_atomic(double v, int inst, double *
 // (sign_l*sign_r)*Fnl; both receive the upwind A_z.  With ARTIV the block
 // maximum of the wave speeds goes to the stage maximum (getrusanov.f90:105-107).
 // ---------------------------------------------------------------------------
+// The block's 256 face points move through LDS: their 9-value traces (left,
+// right) are gathered with consecutive threads on consecutive values of a
+// record, and the fluxes leave the same way, so every global access of a wave
+// covers a few contiguous runs instead of 64 records 72 bytes apart.
+template <int N> __device__ __forceinline__ size_t face_rec(const int4 &fc, int k, bool right_side) {
+    return (right_side && fc.z >= 0) ? ((size_t)fc.z * 4 + fc.w) * N + (N - 1 - k) : ((size_t)fc.x * 4 + fc.y) * N + k;
+}
+
 template <int N, class R>
-__global__ void __launch_bounds__(256) k_face_inv(FaceArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_face_inv(FaceArgs a) {
+    constexpr int FPB = 256, RW = 2 * NV;          // face points per block, values per point (left, right)
+    constexpr int NFB = FPB / N + 2;               // faces a block's points can touch
+    __shared__ double sT[FPB * RW];               // [p][left 9 | right 9]: traces, then fluxes
+    __shared__ int4 sF[NFB];
     __shared__ double red[256];
-    const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int tid = threadIdx.x;
+    const int p0 = xcd_block(blockIdx.x, gridDim.x) * FPB;
+    const int npt = min(FPB, a.nface * N - p0);
+    const int f0 = p0 / N, nf = (p0 + npt - 1) / N - f0 + 1;
+    if (tid < nf) sF[tid] = a.faces[f0 + tid];
+    __syncthreads();
+    {
+        // value e = p*RW + w of the block: every load of the thread issued first
+        double val[RW];
+#pragma unroll
+        for (int j = 0; j < RW; j++) {
+            const int e = j * FPB + tid, p = e / RW, w = e % RW;
+            if (p < npt) {
+                const int gp = p0 + p;
+                val[j] = a.TQ[face_rec<N>(sF[gp / N - f0], gp % N, w >= NV) * NV + (w % NV)];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < RW; j++) {
+            const int e = j * FPB + tid;
+            if (e / RW < npt) sT[e] = val[j];
+        }
+    }
+    __syncthreads();
     double emax = 0.0;
-    if (t < a.nface * N) {
-        const int f = t / N, k = t % N;
-        const int4 fc = a.faces[f];
-        const bool free_face = fc.z < 0;
+    R Fn[8], az, slr = R(1);
+    bool free_face = true;
+    if (tid < npt) {
+        const int gp = p0 + tid, k = gp % N;
+        const int4 fc = sF[gp / N - f0];
+        free_face = fc.z < 0;
         int ifp, jfp;
         face2fp<N>(fc.y + 1, k, ifp, jfp);
         double xy[8];
@@ -318,30 +355,43 @@ __global__ void __launch_bounds__(256) k_face_inv(FaceArgs a) {
         for (int q = 0; q < 8; q++) xy[q] = a.cellxy[(size_t)fc.x * 8 + q];
         double s1, s2;
         metric_row<N>(xy, lf_family(fc.y + 1), ifp, jfp, s1, s2);
-        const size_t pl = (((size_t)fc.x * 4 + fc.y) * N + k) * NV;
-        const size_t pr = free_face ? pl : (((size_t)fc.z * 4 + fc.w) * N + (N - 1 - k)) * NV;
         R Ql[NV], Qr[NV];
 #pragma unroll
-        for (int v = 0; v < NV; v++) { Ql[v] = R(a.TQ[pl + v]); Qr[v] = R(a.TQ[pr + v]); }
+        for (int v = 0; v < NV; v++) { Ql[v] = R(sT[tid * RW + v]); Qr[v] = R(sT[tid * RW + NV + v]); }
         const int sl = side_sign(fc.y);
-        R Fn[8], az, eig;
+        R eig;
         rusanov<R>(R(a.gam), Ql, Qr, R(s1), R(s2), sl, Fn, az, eig);
+        if (!free_face) slr = R(sl * -side_sign(fc.w));
+        emax = double(eig);
+    }
+    __syncthreads();
+    if (tid < npt) {
+#pragma unroll
+        for (int v = 0; v < 8; v++) sT[tid * RW + v] = double(Fn[v]);
+        sT[tid * RW + 8] = double(az);
+        if (!free_face) {
+#pragma unroll
+            for (int v = 0; v < 8; v++) sT[tid * RW + NV + v] = double(slr * Fn[v]);
+            sT[tid * RW + NV + 8] = double(az);
+        }
+    }
+    __syncthreads();
+    {
         R *FF = (R *)a.FF;
 #pragma unroll
-        for (int v = 0; v < 8; v++) FF[pl + v] = Fn[v];
-        FF[pl + 8] = az;
-        if (!free_face) {
-            const R slr = R(sl * -side_sign(fc.w));
-#pragma unroll
-            for (int v = 0; v < 8; v++) FF[pr + v] = slr * Fn[v];
-            FF[pr + 8] = az;
+        for (int j = 0; j < RW; j++) {
+            const int e = j * FPB + tid, p = e / RW, w = e % RW;
+            if (p < npt) {
+                const int gp = p0 + p;
+                const int4 fc = sF[gp / N - f0];
+                if (w < NV || fc.z >= 0)                 // a FREE face has no right cell
+                    FF[face_rec<N>(fc, gp % N, w >= NV) * NV + (w % NV)] = R(sT[e]);
+            }
         }
-        emax = double(eig);
     }
     if (a.artiv) {
         if (a.inst) {
-            const int t0 = xcd_block(blockIdx.x, gridDim.x) * blockDim.x;   // first point of the block
-            const int tt = min(t0 + (int)threadIdx.x, a.nface * N - 1);
+            const int tt = min(p0 + tid, a.nface * N - 1);
             instance_max_atomic(emax, a.inst[a.faces[tt / N].x], a.eig);
         } else {
             block_max_atomic<R>(emax, red, a.eig);

@@ -276,22 +276,51 @@ __device__ __forceinline__ double az_at(const double *FA, const double *sAz, int
 // FF[cell][lfi][nfp][9] (float): 8 flux jump terms + A_z jump.
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ void __launch_bounds__(256) k_face_inv_mx(FaceArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_face_inv_mx(FaceArgs a) {
+    // the block's 256 face points move through LDS as in k_face_inv: traces in
+    // with every load of a thread issued first, jump terms out, both with
+    // consecutive threads on consecutive values of a record
+    constexpr int FPB = 256, RW = 2 * NV, NFB = FPB / N + 2;
+    __shared__ double sT[FPB * RW];               // [p][left 9 | right 9] traces; then float jump terms
+    __shared__ int4 sF[NFB];
     __shared__ double red[256];
-    const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    float *sO = reinterpret_cast<float *>(sT);
+    const int tid = threadIdx.x;
+    const int p0 = xcd_block(blockIdx.x, gridDim.x) * FPB;
+    const int npt = min(FPB, a.nface * N - p0);
+    const int f0 = p0 / N, nf = (p0 + npt - 1) / N - f0 + 1;
+    if (tid < nf) sF[tid] = a.faces[f0 + tid];
+    __syncthreads();
+    {
+        double val[RW];
+#pragma unroll
+        for (int j = 0; j < RW; j++) {
+            const int e = j * FPB + tid, p = e / RW, w = e % RW;
+            if (p < npt) {
+                const int gp = p0 + p;
+                val[j] = a.TQ[face_rec<N>(sF[gp / N - f0], gp % N, w >= NV) * NV + (w % NV)];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < RW; j++) {
+            const int e = j * FPB + tid;
+            if (e / RW < npt) sT[e] = val[j];
+        }
+    }
+    __syncthreads();
     double emax = 0.0;
-    if (t < a.nface * N) {
-        const int f = t / N, k = t % N;
-        const int4 fc = a.faces[f];
-        const bool free_face = fc.z < 0;
-        const size_t pl = (((size_t)fc.x * 4 + fc.y) * N + k) * NV;
-        const size_t pr = free_face ? pl : (((size_t)fc.z * 4 + fc.w) * N + (N - 1 - k)) * NV;
+    float out[RW];
+    bool free_face = true;
+    if (tid < npt) {
+        const int gp = p0 + tid, k = gp % N;
+        const int4 fc = sF[gp / N - f0];
+        free_face = fc.z < 0;
         double ql[NV];
         float d[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-            ql[v] = a.TQ[pl + v];
-            d[v] = float(a.TQ[pr + v] - ql[v]);
+            ql[v] = sT[tid * RW + v];
+            d[v] = float(sT[tid * RW + NV + v] - ql[v]);
         }
         Ref s;
         ref_from(ql, a.gam, s);
@@ -314,20 +343,21 @@ __global__ void __launch_bounds__(256) k_face_inv_mx(FaceArgs a) {
         for (int v = 0; v < NV; v++) Qr[v] = s.q[v] + d[v];
         const float eig = wave_speed(s.q, Qr, s1 * im, s2 * im, gam);
         const float dis = 0.5f * sl * mag * eig;
-        float *FF = (float *)a.FF;
+        const size_t rl = face_rec<N>(fc, k, false);
         if (free_face) {
 #pragma unroll
-            for (int v = 0; v < NV; v++) FF[pl + v] = 0.f;
-            a.FA[pl / NV] = ql[8];
+            for (int v = 0; v < RW; v++) out[v] = 0.f;
+            a.FA[rl] = ql[8];
         } else {
             const float slr = sl * float(-side_sign(fc.w));
 #pragma unroll
             for (int v = 0; v < 8; v++) {
                 const float h = 0.5f * (dF[v] * s1 + dG[v] * s2);
                 const float e = dis * d[v];
-                FF[pl + v] = h - e;
-                FF[pr + v] = slr * (-h - e);
+                out[v] = h - e;
+                out[NV + v] = slr * (-h - e);
             }
+            out[8] = out[NV + 8] = 0.f;
             // upwind A_z by the left cell's outward normal velocity, evaluated in FP64
             // as the FP64 form does (getrusanov.f90:61-66, 98-103): near-zero normal
             // velocities then pick the same side, so A_z keeps the FP64 form's face values
@@ -335,16 +365,38 @@ __global__ void __launch_bounds__(256) k_face_inv_mx(FaceArgs a) {
             const double mn = sqrt(m1 * m1 + m2 * m2);
             const double ux = m1 / mn * double(side_sign(fc.y)), uy = m2 / mn * double(side_sign(fc.y));
             const bool take_left = rhoi * ql[1] * ux + rhoi * ql[2] * uy > 0.0;
-            const double az = take_left ? ql[8] : a.TQ[pr + 8];
-            a.FA[pl / NV] = az;
-            a.FA[pr / NV] = az;
+            const double az = take_left ? ql[8] : sT[tid * RW + NV + 8];
+            a.FA[rl] = az;
+            a.FA[face_rec<N>(fc, k, true)] = az;
         }
         emax = double(eig);
     }
+    __syncthreads();
+    if (tid < npt) {
+#pragma unroll
+        for (int v = 0; v < RW; v++) sO[tid * RW + v] = out[v];
+    }
+    __syncthreads();
+    {
+        // FF[cell][lfi][nfp][9] (float): the 8 jump terms of each side; a FREE
+        // face writes zeros (all 9) on its only side; the A_z slot of an
+        // interior face is not used
+        float *FF = (float *)a.FF;
+#pragma unroll
+        for (int j = 0; j < RW; j++) {
+            const int e = j * FPB + tid, p = e / RW, w = e % RW;
+            if (p < npt) {
+                const int gp = p0 + p;
+                const int4 fc = sF[gp / N - f0];
+                const bool ff = fc.z < 0;
+                if (ff ? w < NV : (w % NV) < 8)
+                    FF[face_rec<N>(fc, gp % N, w >= NV) * NV + (w % NV)] = sO[e];
+            }
+        }
+    }
     if (a.artiv) {
         if (a.inst) {
-            const int t0 = xcd_block(blockIdx.x, gridDim.x) * blockDim.x;
-            const int tt = min(t0 + (int)threadIdx.x, a.nface * N - 1);
+            const int tt = min(p0 + tid, a.nface * N - 1);
             instance_max_atomic(emax, a.inst[a.faces[tt / N].x], a.eig);
         } else {
             block_max_atomic<float>(emax, red, a.eig);
