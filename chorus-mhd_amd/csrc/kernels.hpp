@@ -310,6 +310,59 @@ template <int N> __device__ __forceinline__ size_t face_rec(const int4 &fc, int 
     return (right_side && fc.z >= 0) ? ((size_t)fc.z * 4 + fc.w) * N + (N - 1 - k) : ((size_t)fc.x * 4 + fc.y) * N + k;
 }
 
+// Slot of a thread in the block-wide copies of 2*NV values per face point
+// (left record, right record): a fixed value w and the points pb, pb+PSTEP, ...
+// so consecutive threads touch consecutive values of a record.
+template <int N, int FPB> struct FaceSlots {
+    static constexpr int RW = 2 * NV, PSTEP = 256 / RW, LANES = PSTEP * RW, NIT = (FPB + PSTEP - 1) / PSTEP;
+    int w, pb;
+    bool on, side;
+    __device__ FaceSlots() : w(threadIdx.x % RW), pb(threadIdx.x / RW), on(threadIdx.x < LANES), side(w >= NV) {}
+    __device__ int wv() const { return side ? w - NV : w; }
+};
+
+// sT[p][w] = TQ value w of face point p0+p (w < NV left record, else right;
+// a FREE face repeats its left record); every load of the thread issued first
+template <int N, int FPB>
+__device__ __forceinline__ void gather_face_traces(const double *__restrict__ TQ, const int4 *sF, int f0, int p0, int npt,
+                                                   double *sT) {
+    using SL = FaceSlots<N, FPB>;
+    const SL sl;
+    if (!sl.on) return;
+    double val[SL::NIT];
+#pragma unroll
+    for (int j = 0; j < SL::NIT; j++) {
+        const int p = sl.pb + j * SL::PSTEP;
+        if (p < npt) {
+            const int gp = p0 + p, f = gp / N;
+            val[j] = TQ[face_rec<N>(sF[f - f0], gp - f * N, sl.side) * NV + sl.wv()];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < SL::NIT; j++) {
+        const int p = sl.pb + j * SL::PSTEP;
+        if (p < npt) sT[p * SL::RW + sl.w] = val[j];
+    }
+}
+
+// out[record value] = sT[p][w] for the slots keep(fc, w) selects
+template <int N, int FPB, class T, class S, class Keep>
+__device__ __forceinline__ void scatter_face_values(T *__restrict__ out, const int4 *sF, int f0, int p0, int npt,
+                                                    const S *sT, Keep keep) {
+    using SL = FaceSlots<N, FPB>;
+    const SL sl;
+    if (!sl.on) return;
+#pragma unroll
+    for (int j = 0; j < SL::NIT; j++) {
+        const int p = sl.pb + j * SL::PSTEP;
+        if (p < npt) {
+            const int gp = p0 + p, f = gp / N;
+            const int4 fc = sF[f - f0];
+            if (keep(fc, sl.w)) out[face_rec<N>(fc, gp - f * N, sl.side) * NV + sl.wv()] = T(sT[p * SL::RW + sl.w]);
+        }
+    }
+}
+
 template <int N, class R>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_face_inv(FaceArgs a) {
     constexpr int FPB = 256, RW = 2 * NV;          // face points per block, values per point (left, right)
@@ -323,23 +376,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     const int f0 = p0 / N, nf = (p0 + npt - 1) / N - f0 + 1;
     if (tid < nf) sF[tid] = a.faces[f0 + tid];
     __syncthreads();
-    {
-        // value e = p*RW + w of the block: every load of the thread issued first
-        double val[RW];
-#pragma unroll
-        for (int j = 0; j < RW; j++) {
-            const int e = j * FPB + tid, p = e / RW, w = e % RW;
-            if (p < npt) {
-                const int gp = p0 + p;
-                val[j] = a.TQ[face_rec<N>(sF[gp / N - f0], gp % N, w >= NV) * NV + (w % NV)];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < RW; j++) {
-            const int e = j * FPB + tid;
-            if (e / RW < npt) sT[e] = val[j];
-        }
-    }
+    gather_face_traces<N, FPB>(a.TQ, sF, f0, p0, npt, sT);
     __syncthreads();
     double emax = 0.0;
     R Fn[8], az, slr = R(1);
@@ -376,19 +413,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         }
     }
     __syncthreads();
-    {
-        R *FF = (R *)a.FF;
-#pragma unroll
-        for (int j = 0; j < RW; j++) {
-            const int e = j * FPB + tid, p = e / RW, w = e % RW;
-            if (p < npt) {
-                const int gp = p0 + p;
-                const int4 fc = sF[gp / N - f0];
-                if (w < NV || fc.z >= 0)                 // a FREE face has no right cell
-                    FF[face_rec<N>(fc, gp % N, w >= NV) * NV + (w % NV)] = R(sT[e]);
-            }
-        }
-    }
+    // a FREE face has no right cell
+    scatter_face_values<N, FPB>((R *)a.FF, sF, f0, p0, npt, sT,
+                                [](const int4 &fc, int w) { return w < NV || fc.z >= 0; });
     if (a.artiv) {
         if (a.inst) {
             const int tt = min(p0 + tid, a.nface * N - 1);

@@ -291,22 +291,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     const int f0 = p0 / N, nf = (p0 + npt - 1) / N - f0 + 1;
     if (tid < nf) sF[tid] = a.faces[f0 + tid];
     __syncthreads();
-    {
-        double val[RW];
-#pragma unroll
-        for (int j = 0; j < RW; j++) {
-            const int e = j * FPB + tid, p = e / RW, w = e % RW;
-            if (p < npt) {
-                const int gp = p0 + p;
-                val[j] = a.TQ[face_rec<N>(sF[gp / N - f0], gp % N, w >= NV) * NV + (w % NV)];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < RW; j++) {
-            const int e = j * FPB + tid;
-            if (e / RW < npt) sT[e] = val[j];
-        }
-    }
+    gather_face_traces<N, FPB>(a.TQ, sF, f0, p0, npt, sT);
     __syncthreads();
     double emax = 0.0;
     float out[RW];
@@ -377,23 +362,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         for (int v = 0; v < RW; v++) sO[tid * RW + v] = out[v];
     }
     __syncthreads();
-    {
-        // FF[cell][lfi][nfp][9] (float): the 8 jump terms of each side; a FREE
-        // face writes zeros (all 9) on its only side; the A_z slot of an
-        // interior face is not used
-        float *FF = (float *)a.FF;
-#pragma unroll
-        for (int j = 0; j < RW; j++) {
-            const int e = j * FPB + tid, p = e / RW, w = e % RW;
-            if (p < npt) {
-                const int gp = p0 + p;
-                const int4 fc = sF[gp / N - f0];
-                const bool ff = fc.z < 0;
-                if (ff ? w < NV : (w % NV) < 8)
-                    FF[face_rec<N>(fc, gp % N, w >= NV) * NV + (w % NV)] = sO[e];
-            }
-        }
-    }
+    // FF[cell][lfi][nfp][9] (float): the 8 jump terms of each side; a FREE face
+    // writes zeros (all 9) on its only side; the A_z slot of an interior face is
+    // not used
+    scatter_face_values<N, FPB>((float *)a.FF, sF, f0, p0, npt, sO,
+                                [](const int4 &fc, int w) { return fc.z < 0 ? w < NV : (w % NV) < 8; });
     if (a.artiv) {
         if (a.inst) {
             const int tt = min(p0 + tid, a.nface * N - 1);
