@@ -33,8 +33,8 @@ from chorus_mhd.diagnostics import magnetic_energy, point_volumes   # noqa: E402
 HBM_PEAK_GBS = 8000.0
 # HBM bytes of one stage (all kernels of the stage), rocprofv3 FETCH_SIZE and WRITE_SIZE per
 # dispatch scaled by kernels of known bytes (FETCH x 2.00, WRITE x 1.00); summaries under profiles/
-MEASURED_STAGE_TRAFFIC = {("ot512", "fp64"): 8.07e9, ("ot512", "mixed"): 5.84e9,
-                          ("ot512ideal", "fp64"): 2.77e9, ("ot512ideal", "mixed"): 2.57e9}
+MEASURED_STAGE_TRAFFIC = {("ot512", "fp64"): 8.07e9, ("ot512", "mixed"): 5.73e9,
+                          ("ot512ideal", "fp64"): 2.76e9, ("ot512ideal", "mixed"): 2.47e9}
 METRIC = "DOF-updates/s per RK stage"
 
 # workloads (n, m, domain, N, jitter, seed, bc, ic, gamma, AV)
