@@ -587,8 +587,9 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
     __shared__ double sS[2][CPB * NF];   // metric rows, same layout
     __shared__ double red[2][256];
     __shared__ int2 sNbr[CPB * 4];
+    __shared__ double sP[NVAR][CPB * 4 * N];   // the partner's trace at each face point [c][lfi][nfp]
     __shared__ double sOps[Ops64Layout<N>::SIZE];   // FP64 operator tables (OpTables64 layout)
-    constexpr int OXF = Ops64Layout<N>::XF, OXS = Ops64Layout<N>::XS;
+    constexpr int OM = Ops64Layout<N>::M, OXF = Ops64Layout<N>::XF, OXS = Ops64Layout<N>::XS;
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = a.c_first + blk * CPB;
@@ -608,6 +609,26 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
         gather_regs<PN, S::THREADS>(hn, a.nbr + (size_t)c0 * 4, ncl * 4);
         OpTables64<N, S::THREADS> ops;
         ops.gather();
+        // the partners' stored traces at the block's face points, fetched with the
+        // staging loads (a FREE face has none: its own trace is used instead)
+        constexpr int PP = per_thread<CPB * 4 * N, S::THREADS>();
+        double hp[NVAR][PP];
+#pragma unroll
+        for (int k = 0; k < PP; k++) {
+            const int i = tid + k * S::THREADS, c = i / (4 * N), lfi = (i / N) % 4, nfp = i % N;
+#pragma unroll
+            for (int w = 0; w < NVAR; w++) hp[w][k] = 0.0;
+            if (i < ncl * 4 * N) {
+                const int2 info = a.nbr[(size_t)(c0 + c) * 4 + lfi];
+                if (!((info.y >> 4) & 1)) {
+                    const double *t = a.TQ + (((size_t)info.x * 4 + (info.y & 7) - 1) * N + (N - 1 - nfp)) * NV + VAR0;
+#pragma unroll
+                    for (int w = 0; w < NVAR; w++) hp[w][k] = t[w];
+                }
+            }
+        }
+#pragma unroll
+        for (int w = 0; w < NVAR; w++) scatter_lds<PP, S::THREADS>(sP[w], hp[w], ncl * 4 * N);
 #pragma unroll
         for (int w = 0; w < NVAR; w++) scatter_lds<PV, S::THREADS>(sV[w], hv[w], ncl * NP);
         scatter_lds<PX, S::THREADS>(sX, hx, ncl * 8);
@@ -637,9 +658,7 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
                 double own = trace1<N>(sV[w] + c * NP, fam, ifp, jfp, sOps);
                 double val = own;
                 if (r == 0 || r == N) {
-                    double other = own;
-                    if (!((info.y >> 4) & 1))
-                        other = a.TQ[(((size_t)info.x * 4 + (info.y & 7) - 1) * N + (N - 1 - nfp)) * NV + VAR0 + w];
+                    const double other = ((info.y >> 4) & 1) ? own : sP[w][(c * 4 + lfi) * N + nfp];
                     val = 0.5 * own + 0.5 * other;
                 }
                 sF[w][c * NF + line * (N + 1) + r] = val;
@@ -658,7 +677,7 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
                     const double *ss = sS[d] + pc * NF + line * (N + 1);
                     double acc = 0.0;
 #pragma unroll
-                    for (int r = 0; r <= N; r++) acc = acc + f[r] * c_M[N][r * N + pos] * ss[r];
+                    for (int r = 0; r <= N; r++) acc = acc + f[r] * sOps[OM + r * N + pos] * ss[r];
                     if (fam == 0) dpsi[w][d] = acc;
                     else out[w][d] = acc;
                 }
