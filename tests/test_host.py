@@ -137,3 +137,34 @@ def test_quadrature_and_energy_diagnostics():
     Q = np.zeros((mesh.ncell, N, N, 9))
     Q[..., 5], Q[..., 6], Q[..., 7] = 0.3, -0.4, 1.2
     assert abs(magnetic_energy(mesh, N, Q) - 0.5 * (0.09 + 0.16 + 1.44) * 2.0) < 1e-12
+
+
+def test_glob2local_renumbers_by_first_appearance():
+    """GLOB2LOCAL (readGRIDinfo.f90:124-199) on one rank: vertices numbered by
+    first appearance in the cells, geometry and boundary edges unchanged."""
+    mesh = quad_grid(5, 4, 0.0, 1.0, 0.0, 1.0, jitter=0.1, seed=2, bc="FREE")
+    rng = np.random.default_rng(0)
+    perm = rng.permutation(mesh.nvert)                 # scramble the file numbering first
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(mesh.nvert)
+    from chorus_mhd.formats import Mesh
+    scr = Mesh(mesh.xv[perm], mesh.yv[perm], (inv[mesh.cells - 1] + 1).astype(np.int32),
+               (inv[mesh.bnd - 1] + 1).astype(np.int32), list(mesh.bnd_type), mesh.dxcycl, mesh.dycycl)
+    loc = scr.glob2local()
+    seen = []
+    for c in loc.cells:
+        for v in c:
+            if v not in seen:
+                seen.append(int(v))
+    assert seen == list(range(1, loc.nvert + 1))
+    assert np.array_equal(loc.xv[loc.cells - 1], scr.xv[scr.cells - 1])
+    assert np.array_equal(loc.yv[loc.cells - 1], scr.yv[scr.cells - 1])
+    assert np.array_equal(loc.xv[loc.bnd - 1], scr.xv[scr.bnd - 1])
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(REPO, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 2 and "one GPU" in r.stderr
