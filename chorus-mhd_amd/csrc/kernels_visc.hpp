@@ -266,23 +266,27 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
         }
         __syncthreads();
     }
-    // own traces of nablaQs at the four faces (interfaceVISflux, interfaceflux_all.f90:499-536)
-    for (int it = tid; it < ncl * 4 * N * 16; it += blockDim.x) {
-        const int c = it / (4 * N * 16), k = it % (4 * N * 16), fpi = k / 16, g = k % 16;
+    // own traces of nablaQs at the four faces (interfaceVISflux, interfaceflux_all.f90:499-536):
+    // a thread per face point and half of the 16 components, the L row in registers
+    for (int it = tid; it < ncl * 4 * N * 2; it += blockDim.x) {
+        const int c = it / (8 * N), k = it % (8 * N), fpi = k >> 1, h = k & 1;
         const int lfi = fpi / N, nfp = fpi % N;
         int ifp, jfp;
         face2fp<N>(lfi + 1, nfp, ifp, jfp);
-        const int fam = lf_family(lfi + 1);
-        const R *gq = sG + (c * 16 + g) * NP;
-        R acc = R(0);
-        if (fam == 0) {
+        const bool fam0 = lf_family(lfi + 1) == 0;
+        const int lrow = fam0 ? ifp : jfp, base = fam0 ? jfp * N : ifp, stride = fam0 ? 1 : N;
+        R Lr[N];
 #pragma unroll
-            for (int sp = 0; sp < N; sp++) acc = acc + gq[jfp * N + sp] * R(sOps[ifp * N + sp]);
-        } else {
+        for (int sp = 0; sp < N; sp++) Lr[sp] = R(sOps[lrow * N + sp]);
+        R *dst = a.gtr_r<R>() + ((size_t)(c0 + c) * 4 * N + fpi) * 16 + h * 8;
 #pragma unroll
-            for (int sp = 0; sp < N; sp++) acc = acc + gq[sp * N + ifp] * R(sOps[jfp * N + sp]);
+        for (int gg = 0; gg < 8; gg++) {
+            const R *gq = sG + (c * 16 + h * 8 + gg) * NP + base;
+            R acc = R(0);
+#pragma unroll
+            for (int sp = 0; sp < N; sp++) acc = acc + gq[sp * stride] * Lr[sp];
+            dst[gg] = acc;
         }
-        a.gtr_r<R>()[((size_t)(c0 + c) * 4 * N + fpi) * 16 + g] = acc;
     }
 }
 
