@@ -612,22 +612,27 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
         __syncthreads();
     }
     for (int i = tid; i < ncl * 16 * NP; i += T) a.gsol_r<float>()[(size_t)c0 * 16 * NP + i] = sG[i];
-    for (int it = tid; it < ncl * 4 * N * 16; it += T) {
-        const int c = it / (4 * N * 16), k = it % (4 * N * 16), fpi = k / 16, g = k % 16;
+    // own face traces of nablaQs: a thread per face point and half of the 16
+    // components, the L row in registers
+    for (int it = tid; it < ncl * 4 * N * 2; it += T) {
+        const int c = it / (8 * N), k = it % (8 * N), fpi = k >> 1, h = k & 1;
         const int lfi = fpi / N, nfp = fpi % N;
         int ifp, jfp;
         face2fp<N>(lfi + 1, nfp, ifp, jfp);
-        const int fam = lf_family(lfi + 1);
-        const float *gq = sG + (c * 16 + g) * NP;
-        float acc = 0.f;
-        if (fam == 0) {
+        const bool fam0 = lf_family(lfi + 1) == 0;
+        const int lrow = fam0 ? ifp : jfp, base = fam0 ? jfp * N : ifp, stride = fam0 ? 1 : N;
+        float Lr[N];
 #pragma unroll
-            for (int sp = 0; sp < N; sp++) acc += gq[jfp * N + sp] * sLf[ifp * N + sp];
-        } else {
+        for (int sp = 0; sp < N; sp++) Lr[sp] = sLf[lrow * N + sp];
+        float *dst = a.gtr_r<float>() + ((size_t)(c0 + c) * 4 * N + fpi) * 16 + h * 8;
 #pragma unroll
-            for (int sp = 0; sp < N; sp++) acc += gq[sp * N + ifp] * sLf[jfp * N + sp];
+        for (int gg = 0; gg < 8; gg++) {
+            const float *gq = sG + (c * 16 + h * 8 + gg) * NP + base;
+            float acc = 0.f;
+#pragma unroll
+            for (int sp = 0; sp < N; sp++) acc += gq[sp * stride] * Lr[sp];
+            dst[gg] = acc;
         }
-        a.gtr_r<float>()[((size_t)(c0 + c) * 4 * N + fpi) * 16 + g] = acc;
     }
 }
 
