@@ -89,3 +89,26 @@ def test_fortran_dropin_tecplot_matches_reference(name, tmp_path):
     assert os.path.exists(out), sorted(os.listdir(os.path.join(d, "TEC")))
     ref = os.path.join(REPO, "tests", "golden", name + ".plt")
     assert open(out, "rb").read() == open(ref, "rb").read()
+
+
+@pytest.mark.skipif(not os.path.exists(DROPIN), reason="drop-in driver not built (needs the reference sources)")
+@pytest.mark.parametrize("name", ["ot_n8_N4_jitter_av", "smooth_n6_N5_free_av"])
+def test_fortran_dropin_mixed_within_budget(name, tmp_path):
+    """The driver's mixed form (argument "mixed") against the reference's FP64
+    state: relative L2 within the 1e-6 budget of the mixed form."""
+    print("case dropin mixed", name, flush=True)
+    g = load_golden(name)
+    mesh, N, steps = g["mesh"], int(g["N"]), int(g["steps"])
+    d = str(tmp_path)
+    mesh.write(d)
+    os.makedirs(os.path.join(d, "TEC"))
+    write_restart(os.path.join(d, "restart.in"), 0, 0.0, g["q0"])
+    QuadInp(N=N, dt=float(g["dt"]), MAXITER=steps, DXCYCL=mesh.dxcycl, DYCYCL=mesh.dycycl,
+            ARTIV=int(g["artiv"]), k_stage=int(g["k_stage"]), restart=1, restart_ord=N,
+            NAMERESTART="restart.in", nwrite=1000000).write(os.path.join(d, "QUAD.INP"))
+    r = subprocess.run([DROPIN, "mixed"], cwd=d, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    it, t, Q = read_restart(os.path.join(d, "restart.%08d.dat" % steps), N)
+    assert it == steps
+    rel = np.linalg.norm(Q - g["qend"]) / np.linalg.norm(g["qend"])
+    assert rel < 1e-6, rel
