@@ -60,6 +60,8 @@ SIGNATURES = {
     "chorus_write_tecplot": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
     "chorus_write_restart": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_double, _ip]),
     "chorus_batch_write_restart": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_int, C.c_double, _ip]),
+    "chorus_batch_plot_setup": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(ChorusMesh), C.c_int]),
+    "chorus_batch_write_tecplot": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_double]),
     "chorus_plot_topology": (C.c_int, [C.POINTER(ChorusMesh), C.c_int, _ip, _ip, _dp, _ip]),
     "chorus_batch_init": (C.c_int, [C.c_int, C.POINTER(ChorusMesh), C.POINTER(ChorusParams), C.POINTER(C.c_void_p)]),
     "chorus_batch_free": (None, [C.c_void_p]),

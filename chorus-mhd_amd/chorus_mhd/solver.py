@@ -311,6 +311,19 @@ class Batch:
         self._check(_lib.load().chorus_batch_get_state(self._b, i, _lib.ptr(Q)), "chorus_batch_get_state")
         return Q
 
+    def plot_setup(self, i: int, nre: int) -> None:
+        """Tecplot plot topology of instance i (as Solver.plot_setup)."""
+        m = self.members[i]
+        local = m.mesh.glob2local()
+        cm, arrs = _c_mesh(local, m.N, topology(local))
+        self._check(_lib.load().chorus_batch_plot_setup(self._b, int(i), C.byref(cm), int(nre)),
+                    "chorus_batch_plot_setup")
+
+    def write_tecplot(self, i: int, path: str, ctime: float) -> None:
+        """Tecplot file of instance i's current state (as Solver.write_tecplot)."""
+        self._check(_lib.load().chorus_batch_write_tecplot(self._b, int(i), path.encode(), float(ctime)),
+                    "chorus_batch_write_tecplot")
+
     def write_restart(self, i: int, path: str, it: int, ctime: float, icg=None) -> None:
         """Restart dump of instance i (write_data.f90:5-83), as Solver.write_restart."""
         ip = None

@@ -16,6 +16,24 @@
 
 using namespace chorus;
 
+namespace {
+// Tecplot output of one mesh (chorus_plot_setup / chorus_batch_plot_setup):
+// the host topology and its device copies, the basis and the work arrays
+struct PlotState {
+    PlotTopology *t = nullptr;
+    int *cstart = nullptr, *contrib = nullptr, *factor = nullptr;
+    double *H = nullptr, *q = nullptr, *av = nullptr, *V = nullptr;
+    void release() {
+        for (void *p : {(void *)cstart, (void *)contrib, (void *)factor, (void *)H, (void *)q, (void *)av, (void *)V})
+            (void)hipFree(p);
+        cstart = contrib = factor = nullptr;
+        H = q = av = V = nullptr;
+        delete t;
+        t = nullptr;
+    }
+};
+}  // namespace
+
 struct chorus_ctx {
     std::string err;
     chorus_params p{};
@@ -52,9 +70,7 @@ struct chorus_ctx {
     bool timing = false;
     size_t ev_used = 0;
     // Tecplot output (chorus_plot_setup / chorus_write_tecplot)
-    PlotTopology *plot = nullptr;
-    int *pl_cstart = nullptr, *pl_contrib = nullptr, *pl_factor = nullptr;
-    double *pl_H = nullptr, *pl_q = nullptr, *pl_av = nullptr, *pl_V = nullptr;
+    PlotState plot;
     double *mucell = nullptr;      // viscous path with AV: muAV(1) per cell of the last stage's input
     double *sm6 = nullptr;         // viscous path with AV: By indicator of the last stage's input
     // restart dump image (chorus_write_restart), allocated on first use
@@ -576,17 +592,10 @@ void chorus_free(chorus_ctx *c) {
     (void)hipFree(c->mu4);
     (void)hipFree(c->inst);
     (void)hipFree(c->hinst);
-    (void)hipFree(c->pl_cstart);
-    (void)hipFree(c->pl_contrib);
-    (void)hipFree(c->pl_factor);
-    (void)hipFree(c->pl_H);
-    (void)hipFree(c->pl_q);
-    (void)hipFree(c->pl_av);
-    (void)hipFree(c->pl_V);
+    c->plot.release();
     (void)hipFree(c->mucell);
     (void)hipFree(c->sm6);
     (void)hipFree(c->rs_img);
-    delete c->plot;
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -719,80 +728,102 @@ int chorus_plot_topology(const chorus_mesh *m, int nre, int *nnode, int *ncellp,
     return 0;
 }
 
+namespace {
+// build the plot topology of mesh m (N points per direction, nre plot cells per
+// edge) into ps, replacing what it held
+int plot_state_setup(PlotState &ps, const chorus_mesh *m, int N, int nre, std::string &err) {
+    auto *t = new PlotTopology;
+    int rc = build_plot_topology(m->ncell, m->nvert, m->nface, m->xv, m->yv, m->ivcell, m->if2c, nre, *t, err);
+    if (rc) { delete t; return rc; }
+    ps.release();
+    ps.t = t;
+    const size_t nn = (size_t)t->nnode;
+    std::vector<double> H((size_t)(nre + 1) * N);
+    host_plot_basis(N, nre, H.data());
+    hipError_t e = hipSuccess;
+    auto ok = [&](hipError_t r) { if (e == hipSuccess) e = r; return e == hipSuccess; };
+    ok(hipMalloc(&ps.cstart, (nn + 1) * sizeof(int))) && ok(hipMalloc(&ps.contrib, t->contrib.size() * sizeof(int))) &&
+        ok(hipMalloc(&ps.factor, t->factor.size() * sizeof(int))) && ok(hipMalloc(&ps.H, H.size() * sizeof(double))) &&
+        ok(hipMalloc(&ps.q, nn * 9 * sizeof(double))) && ok(hipMalloc(&ps.av, nn * 2 * sizeof(double))) &&
+        ok(hipMalloc(&ps.V, (size_t)m->ncell * (nre + 1) * (nre + 1) * 9 * sizeof(double))) &&
+        ok(hipMemcpy(ps.cstart, t->cstart.data(), (nn + 1) * sizeof(int), hipMemcpyHostToDevice)) &&
+        ok(hipMemcpy(ps.contrib, t->contrib.data(), t->contrib.size() * sizeof(int), hipMemcpyHostToDevice)) &&
+        ok(hipMemcpy(ps.factor, t->factor.data(), t->factor.size() * sizeof(int), hipMemcpyHostToDevice)) &&
+        ok(hipMemcpy(ps.H, H.data(), H.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (e != hipSuccess) {
+        ps.release();
+        err = std::string("plot set-up: ") + hipGetErrorString(e);
+        return CHORUS_ERR_HIP;
+    }
+    return 0;
+}
+
+// from here on every viscous stage of c also keeps muAV(1) per cell and the By
+// indicator of its input, which the output of the last stage's values needs
+hipError_t plot_av_columns(chorus_ctx *c) {
+    if (c->ideal || !c->p.artiv || c->mucell) return hipSuccess;
+    hipError_t e = hipMalloc(&c->mucell, (size_t)c->ncell * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&c->sm6, (size_t)c->ncell * sizeof(double));
+    if (e == hipSuccess) e = hipMemset(c->mucell, 0, (size_t)c->ncell * sizeof(double));
+    if (e == hipSuccess) e = hipMemset(c->sm6, 0, (size_t)c->ncell * sizeof(double));
+    return e;
+}
+
+// node values of a.Q (ncell cells of c's order) on the device, then the TDV111
+// file; a carries the per-mesh pointers, the rest comes from ps and c
+int plot_state_write(const PlotState &ps, chorus_ctx *c, PlotArgs a, const char *path, double ctime, std::string &err) {
+    const PlotTopology &t = *ps.t;
+    a.nnode = t.nnode;
+    a.N = c->N;
+    a.nre = t.nre;
+    a.cstart = ps.cstart;
+    a.contrib = ps.contrib;
+    a.factor = ps.factor;
+    a.H = ps.H;
+    a.gam = c->p.gam;
+    a.pi = std::atan(1.0) * 4.0;
+    a.q = ps.q;
+    a.av = ps.av;
+    a.V = ps.V;
+    hipError_t e = launch_plot_nodes(a, c->stream);
+    std::vector<double> q((size_t)t.nnode * 9), avh((size_t)t.nnode * 2);
+    if (e == hipSuccess) e = hipMemcpyAsync(q.data(), ps.q, q.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(avh.data(), ps.av, avh.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        err = std::string("plot: ") + hipGetErrorString(e);
+        return CHORUS_ERR_HIP;
+    }
+    return write_tecplot_file(path, ctime, t, q.data(), avh.data(), err);
+}
+}  // namespace
+
 int chorus_plot_setup(chorus_ctx *c, const chorus_mesh *m, int nre) {
     if (!c || !m) return fail(c, CHORUS_ERR_ARG, "null argument");
     if (nre < 1) return fail(c, CHORUS_ERR_ARG, "NRE must be >= 1");
     if (m->ncell != c->ncell || m->N != c->N || !m->xv || !m->yv || !m->ivcell || !m->if2c)
         return fail(c, CHORUS_ERR_ARG, "mesh does not match the context");
-    auto *t = new PlotTopology;
     std::string err;
-    int rc = build_plot_topology(m->ncell, m->nvert, m->nface, m->xv, m->yv, m->ivcell, m->if2c, nre, *t, err);
-    if (rc) { delete t; return fail(c, rc, err); }
-    delete c->plot;
-    c->plot = t;
-    for (void *p : {(void *)c->pl_cstart, (void *)c->pl_contrib, (void *)c->pl_factor, (void *)c->pl_H,
-                    (void *)c->pl_q, (void *)c->pl_av, (void *)c->pl_V})
-        (void)hipFree(p);
-    c->pl_cstart = c->pl_contrib = c->pl_factor = nullptr;
-    c->pl_H = c->pl_q = c->pl_av = c->pl_V = nullptr;
-    const size_t nn = (size_t)t->nnode;
-    std::vector<double> H((size_t)(nre + 1) * c->N);
-    host_plot_basis(c->N, nre, H.data());
-    HIP_TRY(c, hipMalloc(&c->pl_cstart, (nn + 1) * sizeof(int)));
-    HIP_TRY(c, hipMalloc(&c->pl_contrib, t->contrib.size() * sizeof(int)));
-    HIP_TRY(c, hipMalloc(&c->pl_factor, t->factor.size() * sizeof(int)));
-    HIP_TRY(c, hipMalloc(&c->pl_H, H.size() * sizeof(double)));
-    HIP_TRY(c, hipMalloc(&c->pl_q, nn * 9 * sizeof(double)));
-    HIP_TRY(c, hipMalloc(&c->pl_av, nn * 2 * sizeof(double)));
-    HIP_TRY(c, hipMalloc(&c->pl_V, (size_t)c->ncell * (nre + 1) * (nre + 1) * 9 * sizeof(double)));
-    HIP_TRY(c, hipMemcpy(c->pl_cstart, t->cstart.data(), (nn + 1) * sizeof(int), hipMemcpyHostToDevice));
-    HIP_TRY(c, hipMemcpy(c->pl_contrib, t->contrib.data(), t->contrib.size() * sizeof(int), hipMemcpyHostToDevice));
-    HIP_TRY(c, hipMemcpy(c->pl_factor, t->factor.data(), t->factor.size() * sizeof(int), hipMemcpyHostToDevice));
-    HIP_TRY(c, hipMemcpy(c->pl_H, H.data(), H.size() * sizeof(double), hipMemcpyHostToDevice));
-    if (!c->ideal && c->p.artiv && !c->mucell) {
-        // from here on every viscous stage also keeps muAV(1) per cell and the By
-        // indicator of its input, which the output of the last stage's values needs
-        HIP_TRY(c, hipMalloc(&c->mucell, (size_t)c->ncell * sizeof(double)));
-        HIP_TRY(c, hipMalloc(&c->sm6, (size_t)c->ncell * sizeof(double)));
-        HIP_TRY(c, hipMemset(c->mucell, 0, (size_t)c->ncell * sizeof(double)));
-        HIP_TRY(c, hipMemset(c->sm6, 0, (size_t)c->ncell * sizeof(double)));
-    }
+    const int rc = plot_state_setup(c->plot, m, c->N, nre, err);
+    if (rc) return fail(c, rc, err);
+    HIP_TRY(c, plot_av_columns(c));
     return 0;
 }
 
 int chorus_write_tecplot(chorus_ctx *c, const char *path, double ctime) {
     if (!c || !path) return fail(c, CHORUS_ERR_ARG, "null argument");
-    if (!c->plot) return fail(c, CHORUS_ERR_ARG, "chorus_plot_setup was not called");
-    const PlotTopology &t = *c->plot;
+    if (!c->plot.t) return fail(c, CHORUS_ERR_ARG, "chorus_plot_setup was not called");
     const bool av = !c->ideal && c->p.artiv && c->last_parity >= 0;
     PlotArgs a{};
-    a.nnode = t.nnode;
-    a.N = c->N;
-    a.nre = t.nre;
+    a.ncell = c->ncell;
     a.artiv = av ? 1 : 0;
-    a.cstart = c->pl_cstart;
-    a.contrib = c->pl_contrib;
-    a.factor = c->pl_factor;
     a.Q = c->X[c->cur];
-    a.H = c->pl_H;
     a.mucell = c->mucell;
     a.sm6 = c->sm6;
     a.eig = av ? c->eig + (size_t)c->last_parity * c->ninst : nullptr;
     a.h_msh = c->h_msh;
-    a.gam = c->p.gam;
-    a.pi = std::atan(1.0) * 4.0;
-    a.q = c->pl_q;
-    a.av = c->pl_av;
-    a.V = c->pl_V;
-    a.ncell = c->ncell;
-    hipError_t e = launch_plot_nodes(a, c->stream);
-    if (e != hipSuccess) return fail(c, CHORUS_ERR_HIP, std::string("plot launch: ") + hipGetErrorString(e));
-    std::vector<double> q((size_t)t.nnode * 9), avh((size_t)t.nnode * 2);
-    HIP_TRY(c, hipMemcpyAsync(q.data(), c->pl_q, q.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(avh.data(), c->pl_av, avh.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
     std::string err;
-    const int rc = write_tecplot_file(path, ctime, t, q.data(), avh.data(), err);
+    const int rc = plot_state_write(c->plot, c, a, path, ctime, err);
     return rc ? fail(c, rc, err) : 0;
 }
 
@@ -895,6 +926,9 @@ struct chorus_batch {
     int ninst = 0;
     struct Member {
         int group, off, ncell, N;
+        int li = 0;          // index among the group's instances
+        double h = 0.0;      // AV length scale (AV.f90:147)
+        PlotState plot;      // Tecplot output (chorus_batch_plot_setup)
     };
     std::vector<Member> mem;
     std::vector<chorus_ctx *> groups;
@@ -998,6 +1032,7 @@ const char *chorus_batch_last_error(const chorus_batch *b) { return b ? b->err.c
 void chorus_batch_free(chorus_batch *b) {
     if (!b) return;
     drop_graphs(b);
+    for (auto &m : b->mem) m.plot.release();
     for (chorus_ctx *c : b->groups) chorus_free(c);
     for (auto s : b->gstream) (void)hipStreamDestroy(s);
     for (auto e : b->join) (void)hipEventDestroy(e);
@@ -1060,6 +1095,8 @@ int chorus_batch_init(int ninst, const chorus_mesh *meshes, const chorus_params 
             for (int c = 0; c < m.ncell; c++) inst[coff + c] = li;
             const int ncg = p.ncellglob > 0 ? p.ncellglob : m.ncell;
             hinst.push_back(std::sqrt(p.dxcycl * p.dycycl / (double)ncg));   // AV.f90:147
+            b->mem[i].li = li;
+            b->mem[i].h = hinst.back();
             coff += m.ncell; voff += m.nvert; foff += m.nface; ioff += m.ninter; poff += m.npair; froff += m.nfree;
             li++;
         }
@@ -1135,6 +1172,44 @@ int chorus_batch_write_restart(chorus_batch *b, int i, const char *path, int ite
     HIP_TRYB(b, hipStreamSynchronize(b->stream));
     std::string err;
     const int rc = restart_dump(c, m.ncell, c->X[c->cur] + (size_t)m.off * 9 * m.N * m.N, path, iter, ctime, icg, err);
+    return rc ? bfail(b, rc, err) : 0;
+}
+
+int chorus_batch_plot_setup(chorus_batch *b, int i, const chorus_mesh *mesh, int nre) {
+    if (!b || !mesh || i < 0 || i >= b->ninst) return bfail(b, CHORUS_ERR_ARG, "bad instance");
+    if (nre < 1) return bfail(b, CHORUS_ERR_ARG, "NRE must be >= 1");
+    chorus_batch::Member &m = b->mem[i];
+    if (mesh->ncell != m.ncell || mesh->N != m.N || !mesh->xv || !mesh->yv || !mesh->ivcell || !mesh->if2c)
+        return bfail(b, CHORUS_ERR_ARG, "mesh does not match instance " + std::to_string(i));
+    chorus_ctx *c = b->groups[m.group];
+    HIP_TRYB(b, hipStreamSynchronize(b->stream));
+    std::string err;
+    const int rc = plot_state_setup(m.plot, mesh, m.N, nre, err);
+    if (rc) return bfail(b, rc, err);
+    if (!c->ideal && c->p.artiv && !c->mucell) {
+        HIP_TRYB(b, plot_av_columns(c));
+        drop_graphs(b);   // the captured stages do not yet write the new AV columns
+    }
+    return 0;
+}
+
+int chorus_batch_write_tecplot(chorus_batch *b, int i, const char *path, double ctime) {
+    if (!b || !path || i < 0 || i >= b->ninst) return bfail(b, CHORUS_ERR_ARG, "bad instance");
+    const chorus_batch::Member &m = b->mem[i];
+    if (!m.plot.t) return bfail(b, CHORUS_ERR_ARG, "chorus_batch_plot_setup was not called for this instance");
+    chorus_ctx *c = b->groups[m.group];
+    HIP_TRYB(b, hipStreamSynchronize(b->stream));
+    const bool av = !c->ideal && c->p.artiv && c->last_parity >= 0 && c->mucell;
+    PlotArgs a{};
+    a.ncell = m.ncell;
+    a.artiv = av ? 1 : 0;
+    a.Q = c->X[c->cur] + (size_t)m.off * 9 * m.N * m.N;
+    a.mucell = av ? c->mucell + m.off : nullptr;
+    a.sm6 = av ? c->sm6 + m.off : nullptr;
+    a.eig = av ? c->eig + (size_t)c->last_parity * c->ninst + m.li : nullptr;
+    a.h_msh = m.h;
+    std::string err;
+    const int rc = plot_state_write(m.plot, c, a, path, ctime, err);
     return rc ? bfail(b, rc, err) : 0;
 }
 

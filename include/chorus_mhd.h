@@ -179,6 +179,11 @@ int chorus_batch_step(chorus_batch *b, int nsteps);
 int chorus_batch_compresid(chorus_batch *b, int i, double *resid);
 /* restart dump of instance i (write_data.f90:5-83), as chorus_write_restart */
 int chorus_batch_write_restart(chorus_batch *b, int i, const char *path, int iter, double ctime, const int *icg);
+/* Tecplot output of instance i (tecplotter2dsetup / tecplotter2d / tecpost_parallel),
+ * as chorus_plot_setup / chorus_write_tecplot: mesh is the instance's own (the
+ * one passed to chorus_batch_init); set up before stepping for the muAV columns */
+int chorus_batch_plot_setup(chorus_batch *b, int i, const chorus_mesh *mesh, int nre);
+int chorus_batch_write_tecplot(chorus_batch *b, int i, const char *path, double ctime);
 /* correct_B_via_curl_Az (setinitialcond.f90:82-92) of every instance */
 int chorus_batch_correct_b(chorus_batch *b);
 /* max |div B| per instance (iterations.f90:35-48), divbmax[ninst] */

@@ -150,3 +150,27 @@ def test_single_context_tiles_agree(artiv):
     with pytest.raises(Exception):
         f.set_tile(1)
     f.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_tecplot_equals_single_instances(precision, tmp_path):
+    """Each instance's Tecplot file, written from the batch, is the file the
+    same instance writes alone (AV on: muAV and etaAV of the last stage)."""
+    _say("batch tecplot", precision)
+    mem = _members(precision, 1)[:4]
+    q0 = [_initial(m, k) for m, k in mem]
+    b = Batch([m for m, _ in mem])
+    for i, q in enumerate(q0):
+        b.set_state(i, q)
+        b.plot_setup(i, 2 + i % 2)
+    b.step(3)
+    for i, (m, _) in enumerate(mem):
+        b.write_tecplot(i, str(tmp_path / "b.plt"), 0.125)
+        s = Solver(m.mesh, m.N, m.dt, gam=m.gam, artiv=m.artiv, precision=m.precision)
+        s.plot_setup(2 + i % 2)
+        s.set_state(q0[i])
+        s.step(3)
+        s.write_tecplot(str(tmp_path / "s.plt"), 0.125)
+        s.close()
+        assert (tmp_path / "b.plt").read_bytes() == (tmp_path / "s.plt").read_bytes(), "instance %d" % i
+    b.close()
