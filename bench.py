@@ -232,6 +232,7 @@ def run_output(w, args):
             os.remove(path)
     s.close()
     nnode = (w["n"] * nre + 1) * (w["m"] * nre + 1) if w["bc"] == "CYCL" else None
+    base = None if args.no_cpu else cpu_output_baseline(w, nre)
     out = {
         "metric": "output wall time per dump", "unit": "s", "higher_is_better": False, "n_gpus": 1,
         "dtype": "f64", "data": "synthetic",
@@ -241,8 +242,38 @@ def run_output(w, args):
         "restart_host_route_s": res["restart_host_route"][0],
         "tecplot_s": res["tecplot"][0], "tecplot_bytes": res["tecplot"][1],
         "plot_setup_s": t_setup,
+        "tecplot_nodes_per_s": nnode / res["tecplot"][0] if nnode else None,
     }
+    if base:
+        out["cpu_baseline"] = base
     print(json.dumps(out), flush=True)
+
+
+def cpu_output_baseline(w, nre, n=128):
+    """The reference's own Tecplot chain (tecplotter2dsetup, then tecplotter2d +
+    tecpost_parallel) timed by the compiled reference harness on one host core,
+    on an n x n sample of the workload after one step; plot nodes per second of
+    the output call."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as orc
+    if not os.path.exists(orc.REF_HARNESS) or abs(w["gam"] - 5.0 / 3) > 1e-15 or w["bc"] != "CYCL":
+        return None
+    mesh = quad_grid(n, n, *w["box"], jitter=w["jitter"], seed=w["seed"], bc=w["bc"])
+    with tempfile.TemporaryDirectory() as d:
+        mesh.write(d)
+        os.makedirs(os.path.join(d, "TEC"))
+        QuadInp(N=w["N"], dt=1e-4, MAXITER=1, DXCYCL=mesh.dxcycl, DYCYCL=mesh.dycycl, ARTIV=1, NRE=nre,
+                restart_ord=w["N"]).write(os.path.join(d, "QUAD.INP"))
+        r = subprocess.run([orc.REF_HARNESS, "plot"], cwd=d, capture_output=True, text=True, timeout=600,
+                           env=dict(os.environ, OMP_NUM_THREADS="1"))
+        if r.returncode != 0:
+            return None
+        setup_s, out_s = (float(x) for x in open(os.path.join(d, "plot_timing.txt")).read().split())
+    nodes = (n * nre + 1) ** 2
+    return dict(value=nodes / out_s, unit="plot nodes/s", cores=1, kind="reference", output_s=out_s,
+                setup_s=setup_s, sample="%dx%d %s mesh (jitter %.2f, seed %d), N=%d, NRE=%d, AV on, after one step "
+                "(reference Fortran tecplotter2d + tecpost_parallel, flang -O3)" % (
+                    n, n, w["ic"], w["jitter"], w["seed"], w["N"], nre))
 
 
 def b_stage(N: int) -> float:

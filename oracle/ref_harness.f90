@@ -14,8 +14,9 @@
 !   mode "run"  : q0.f64, qend.f64, diag.txt, timing.txt
 !   mode "plot" : as "run", then the reference's own Tecplot output chain
 !                 (tecplotter2dsetup, tecplotter2d, tecpost_parallel, the
-!                 calls of iterations.f90:82-86) writes ./TEC/TEC*.plt; needs
-!                 ARTIV=1 (tecplotter2d reads muAV) and an existing ./TEC
+!                 calls of iterations.f90:82-86) writes ./TEC/TEC*.plt and the
+!                 chain's timing to plot_timing.txt; needs ARTIV=1
+!                 (tecplotter2d reads muAV) and an existing ./TEC
 !
 ! usage:  ref_harness setup|resid|run     (reads QUAD.INP like the reference)
 program ref_harness
@@ -128,9 +129,15 @@ program ref_harness
         write(fh, '(I8,ES26.17E3,I12,I4)') nsteps, t1 - t0, NCELL, N
         close(fh)
         if (trim(mode) == 'plot') then
+            t0 = MPI_WTIME()
             call tecplotter2dsetup
+            t1 = MPI_WTIME()
             call tecplotter2d
             call tecpost_parallel
+            ! plot_timing.txt: seconds of the plot-node set-up, seconds of one output
+            open(newunit=fh, file='plot_timing.txt', status='replace')
+            write(fh, '(2ES26.17E3)') t1 - t0, MPI_WTIME() - t1
+            close(fh)
         end if
     else
         stop 'mode must be setup, resid, run or plot'
