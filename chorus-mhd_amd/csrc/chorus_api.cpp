@@ -5,10 +5,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -75,6 +77,14 @@ struct chorus_ctx {
     double *sm6 = nullptr;         // viscous path with AV: By indicator of the last stage's input
     // restart dump image (chorus_write_restart), allocated on first use
     unsigned *rs_img = nullptr;
+    // one time step captured as a hipGraph, per buffer rotation, tile shape and
+    // whether the stages write the Tecplot AV columns; post = the bookkeeping
+    // (cur, last_in, stage_parity, last_parity) after the step
+    struct StepGraph {
+        hipGraphExec_t exec = nullptr;
+        std::array<int, 4> post{};
+    };
+    std::map<std::array<int, 4>, StepGraph> step_graphs;
 };
 
 namespace {
@@ -274,6 +284,37 @@ int one_step(chorus_ctx *c) {
     rc = run_ct(c, c->X[c->cur], 0);
     if (t) { (void)hipEventRecord(c->ev[c->ev_used + 1], c->stream); c->ev_used += 2; }
     return rc;
+}
+
+// one_step as one hipGraph launch: the step's 11-26 kernel launches cost one
+// launch, which matters for small meshes; captured on first use per key
+int graph_step(chorus_ctx *c) {
+    const std::array<int, 4> key{c->cur, c->stage_parity, c->tile, c->mucell ? 1 : 0};
+    auto it = c->step_graphs.find(key);
+    if (it == c->step_graphs.end()) {
+        chorus_ctx::StepGraph g;
+        HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = one_step(c);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(c, CHORUS_ERR_HIP, std::string("step capture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(c, CHORUS_ERR_HIP, std::string("step graph: ") + hipGetErrorString(e));
+        g.post = {c->cur, c->last_in, c->stage_parity, c->last_parity};
+        // the capture advanced the host-side bookkeeping; rewind it to replay below
+        c->cur = key[0];
+        c->stage_parity = key[1];
+        it = c->step_graphs.emplace(key, g).first;
+    }
+    HIP_TRY(c, hipGraphLaunch(it->second.exec, c->stream));
+    const auto &p = it->second.post;
+    c->cur = p[0]; c->last_in = p[1]; c->stage_parity = p[2]; c->last_parity = p[3];
+    return 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -596,6 +637,7 @@ void chorus_free(chorus_ctx *c) {
     (void)hipFree(c->mucell);
     (void)hipFree(c->sm6);
     (void)hipFree(c->rs_img);
+    for (auto &g : c->step_graphs) (void)hipGraphExecDestroy(g.second.exec);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -649,7 +691,7 @@ int chorus_correct_b(chorus_ctx *c) {
 int chorus_step(chorus_ctx *c, int nsteps) {
     if (!c || nsteps < 0) return fail(c, CHORUS_ERR_ARG, "bad arguments");
     for (int i = 0; i < nsteps; i++) {
-        int rc = one_step(c);
+        int rc = graph_step(c);
         if (rc) return rc;
     }
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -918,8 +960,6 @@ int chorus_time_steps(chorus_ctx *c, int nsteps, double *ms_total, double *ms_st
 // instance index selects the per-instance AV quantities.  One step of the
 // batch is one hipGraph launch whose branches are the groups.
 // ===========================================================================
-#include <array>
-#include <map>
 
 struct chorus_batch {
     std::string err;

@@ -85,7 +85,9 @@ int chorus_correct_b(chorus_ctx *ctx);
 
 /* nsteps iterations of the time loop body (iterations.f90:22-33): the SSP
  * stepper of k_stage (RK_S5P3, iterations.f90:162-217, or FiveStageRk,
- * iterations.f90:92-160) followed by correct_B_via_curl_Az. */
+ * iterations.f90:92-160) followed by correct_B_via_curl_Az.  Each step is one
+ * hipGraph launch on the context's stream (captured on first use per buffer
+ * rotation). */
 int chorus_step(chorus_ctx *ctx, int nsteps);
 
 /* Diagnostics of iterations.f90:35-67 for the current state: RMS residual of
