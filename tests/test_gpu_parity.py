@@ -274,3 +274,27 @@ def test_errors_are_reported():
         Solver(mesh, 9, 1e-3)
     with pytest.raises(Exception, match="k_stage"):
         Solver(mesh, 4, 1e-3, k_stage=4)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_steps_equal_plain_launches(precision):
+    """chorus_step launches each step as a captured hipGraph (one per buffer
+    rotation, wave-speed slot and tile shape); chorus_time_steps launches the
+    same kernels one by one.  Seven steps cover the whole cycle of graphs; the
+    mixed form also changes its tile shape in between (a new graph)."""
+    _say("graph steps", precision)
+    g = load_golden("ot_n8_N4_jitter_av")
+    a, b = _solver(g, precision), _solver(g, precision)
+    a.set_state(g["q0"])
+    b.set_state(g["q0"])
+    a.step(7)
+    b.time_steps(7)
+    if precision == "mixed":
+        a.set_tile(1)
+        b.set_tile(1)
+        a.step(3)
+        b.time_steps(3)
+    qa, qb = a.get_state(), b.get_state()
+    a.close()
+    b.close()
+    assert np.array_equal(qa, qb)
