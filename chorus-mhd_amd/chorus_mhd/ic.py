@@ -92,8 +92,10 @@ def orszag_tang_fields(mesh, N, gam=5.0 / 3, b0=1.0):
 
 
 def rotor(solver, mesh, N, gam=1.4, b0=5.0 / math.sqrt(4 * math.pi)):
-    """MHD rotor on [0,1]^2: rho=10, u = 20(-(y-1/2), x-1/2) for r<0.1, linear taper
-    to rho=1, u=0 at r=0.115; p=1, Bx=B0 (A_z = B0*y), By=0."""
+    """MHD rotor on [0,1]^2 (Balsara-Spicer form): rho=10, u = 20(-(y-1/2), x-1/2)
+    for r<0.1; in the ring 0.1<r<0.115 with f = (0.115-r)/0.015, rho = 1+9f and
+    u = f*2*(-(y-1/2), x-1/2)/r (the disk's edge speed 2 tapered by f); outside
+    rho=1, u=0.  p=1, Bx=B0 (A_z = B0*y), By=0."""
     return _with_ct(solver, *rotor_fields(mesh, N, gam, b0), gam)
 
 
@@ -102,10 +104,11 @@ def rotor_fields(mesh, N, gam=1.4, b0=5.0 / math.sqrt(4 * math.pi)):
     xy = solution_xy(mesh, N)
     x, y = xy[..., 0] - 0.5, xy[..., 1] - 0.5
     r = np.sqrt(x * x + y * y)
-    r0, r1 = 0.1, 0.115
+    r0, r1, u0 = 0.1, 0.115, 2.0
     fr = np.clip((r1 - r) / (r1 - r0), 0.0, 1.0)
     rho = 1.0 + 9.0 * fr
-    omega = 20.0 * fr
+    # angular speed: u0/r0 inside the disk, f*u0/r in the ring, 0 outside
+    omega = np.where(r < r0, u0 / r0, fr * u0 / np.maximum(r, r0))
     u, v = -omega * y, omega * x
     p = np.ones_like(x)
     Q = np.zeros(x.shape + (9,))

@@ -650,7 +650,10 @@ int chorus_set_state(chorus_ctx *c, const double *q) {
     int rc = run_trace(c, c->X[c->cur]);
     if (rc) return rc;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    // no stage has evaluated the new state yet: the residual norm, the wave
+    // speed and the AV output columns are those of no stage (NaN / zeros)
     c->last_in = -1;
+    c->last_parity = -1;
     return 0;
 }
 
@@ -968,6 +971,7 @@ struct chorus_batch {
         int group, off, ncell, N;
         int li = 0;          // index among the group's instances
         double h = 0.0;      // AV length scale (AV.f90:147)
+        bool av_valid = false;   // a stage has evaluated the member's current state
         PlotState plot;      // Tecplot output (chorus_batch_plot_setup)
     };
     std::vector<Member> mem;
@@ -1186,6 +1190,7 @@ int chorus_batch_set_state(chorus_batch *b, int i, const double *q) {
     HIP_TRYB(b, hipMemcpy(c->X[c->cur] + (size_t)m.off * 9 * NP, h.data(), h.size() * sizeof(double),
                           hipMemcpyHostToDevice));
     c->last_in = -1;
+    b->mem[i].av_valid = false;   // its AV output columns belong to the previous state
     b->stale = true;
     return 0;
 }
@@ -1239,7 +1244,7 @@ int chorus_batch_write_tecplot(chorus_batch *b, int i, const char *path, double 
     if (!m.plot.t) return bfail(b, CHORUS_ERR_ARG, "chorus_batch_plot_setup was not called for this instance");
     chorus_ctx *c = b->groups[m.group];
     HIP_TRYB(b, hipStreamSynchronize(b->stream));
-    const bool av = !c->ideal && c->p.artiv && c->last_parity >= 0 && c->mucell;
+    const bool av = !c->ideal && c->p.artiv && c->last_parity >= 0 && c->mucell && m.av_valid;
     PlotArgs a{};
     a.ncell = m.ncell;
     a.artiv = av ? 1 : 0;
@@ -1259,6 +1264,8 @@ int chorus_batch_step(chorus_batch *b, int nsteps) {
     for (int k = 0; k < nsteps && !rc; k++) rc = batch_one_step(b);
     if (rc) return rc;
     HIP_TRYB(b, hipStreamSynchronize(b->stream));
+    if (nsteps > 0)
+        for (auto &m : b->mem) m.av_valid = true;
     return 0;
 }
 
@@ -1278,6 +1285,7 @@ int chorus_batch_time_steps(chorus_batch *b, int nsteps, double *ms_total) {
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     if (rc) return rc;
+    for (auto &m : b->mem) m.av_valid = true;
     if (ms_total) *ms_total = ms;
     return 0;
 }
@@ -1296,6 +1304,8 @@ int chorus_batch_compresid(chorus_batch *b, int i, double *resid) {
     rc = launch_stage(c, s);
     if (rc) return bfail(b, rc, c->err);
     HIP_TRYB(b, hipStreamSynchronize(c->stream));
+    for (auto &o : b->mem)
+        if (o.group == m.group) o.av_valid = true;
     if (resid) {
         const int NP = m.N * m.N;
         std::vector<double> h((size_t)m.ncell * 9 * NP);

@@ -18,6 +18,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 REF_HARNESS = os.path.join(HERE, "_ref", "ref_harness")
+# the same build with gam = 1.4 (the reference's compile-time PARAMETER,
+# setup2d.f90:68, changed by `make -C oracle ref-g14`)
+REF_HARNESS_G14 = os.path.join(HERE, "_ref", "ref_harness_g14")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -174,11 +177,18 @@ class Oracle:
         return lib().orc_scalar(self._c, name.encode())
 
 
-def run_reference(workdir: str, mode: str, timeout: float = 600.0) -> None:
-    """Run the compiled reference (oracle/_ref/ref_harness) in ``workdir``."""
-    if not os.path.exists(REF_HARNESS):
-        raise FileNotFoundError(REF_HARNESS)
-    r = subprocess.run([REF_HARNESS, mode], cwd=workdir, capture_output=True, text=True,
+def run_reference(workdir: str, mode: str, timeout: float = 600.0, gam: float = 5.0 / 3) -> None:
+    """Run the compiled reference (oracle/_ref/ref_harness) in ``workdir``;
+    gam = 1.4 selects the build with that ratio of specific heats."""
+    if gam == 5.0 / 3:
+        exe = REF_HARNESS
+    elif gam == 1.4:
+        exe = REF_HARNESS_G14
+    else:
+        raise ValueError("the reference is built for gam = 5/3 and 1.4 only, not %r" % gam)
+    if not os.path.exists(exe):
+        raise FileNotFoundError(exe)
+    r = subprocess.run([exe, mode], cwd=workdir, capture_output=True, text=True,
                        timeout=timeout)
     if r.returncode != 0:
         raise RuntimeError("reference harness failed: %s\n%s" % (r.stdout[-2000:], r.stderr[-2000:]))

@@ -23,15 +23,27 @@ def golden_names():
 
 
 def load_golden(name):
-    """Fixture arrays plus the mesh rebuilt from its recipe (checked against the stored coordinates)."""
+    """Fixture arrays plus the mesh rebuilt from its recipe (checked against the stored coordinates).
+    Fixtures without the keys nx/ny/dom/gam/mu/eta0 are n x n on [0,2pi]^2 with gam=5/3, mu=eta0=0."""
     from chorus_mhd import quad_grid
     g = dict(np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False))
     n = int(g["n"])
-    mesh = quad_grid(n, n, 0.0, 2 * math.pi, 0.0, 2 * math.pi, jitter=float(g["jitter"]),
+    nx, ny = int(g.get("nx", n)), int(g.get("ny", n))
+    x0, x1, y0, y1 = g.get("dom", (0.0, 2 * math.pi, 0.0, 2 * math.pi))
+    mesh = quad_grid(nx, ny, float(x0), float(x1), float(y0), float(y1), jitter=float(g["jitter"]),
                      seed=int(g["seed"]), bc=str(g["bc"]))
     assert np.array_equal(mesh.xv, g["xv"]) and np.array_equal(mesh.yv, g["yv"])
     g["mesh"] = mesh
+    g.setdefault("gam", 5.0 / 3)
+    g.setdefault("mu", 0.0)
+    g.setdefault("eta0", 0.0)
     return g
+
+
+def golden_params(g):
+    """Keyword arguments of Solver / Oracle for a fixture."""
+    return dict(artiv=int(g["artiv"]), k_stage=int(g["k_stage"]), gam=float(g["gam"]),
+                mu=float(g["mu"]), eta0=float(g["eta0"]))
 
 
 def split_topo(t):

@@ -28,7 +28,7 @@ import oracle as orc                                        # noqa: E402
 
 TWO_PI = 2 * math.pi
 
-# name: (n, N, jitter, bc, artiv, k_stage, steps, ic)
+# name: (n, N, jitter, bc, artiv, k_stage, steps, ic)   on [0, 2pi]^2, gam = 5/3
 CASES = {
     "ot_n8_N4_uniform":      (8, 4, 0.00, "CYCL", 0, 3, 3, "ot"),
     "ot_n8_N4_jitter_av":    (8, 4, 0.10, "CYCL", 1, 3, 3, "ot"),
@@ -40,14 +40,42 @@ CASES = {
 }
 DT = 1.0e-3
 SEED = 7
+TWO_PI_DOM = (0.0, TWO_PI, 0.0, TWO_PI)
+
+# Cases with more knobs (dict form): nx, ny, domain, gam, mu, eta0 and the time
+# step rule.  "dt" is a number or ("cfl", c): dt = c h_min / (lambda_max N) of
+# the initial state (chorus_mhd.ic.cfl_dt).  gam = 1.4 runs the reference built
+# by `make -C oracle ref-g14` (setup2d.f90:68 is a PARAMETER).
+XCASES = {
+    # the AV ramp active in part of the cells (AV.f90:157-172 between the
+    # thresholds): getVfluxvectors_AV with nonzero mu_art
+    "rough_n6_N4_jitter_av": dict(nx=6, ny=6, N=4, jitter=0.10, bc="CYCL", artiv=1, k_stage=3,
+                                  steps=2, ic="rough"),
+    # physical viscosity and resistivity, AV off: getVfluxvectors
+    # (getrusanov.f90:157-248, the eta0*J terms of :232-247)
+    "visc_n6_N4_jitter": dict(nx=6, ny=6, N=4, jitter=0.10, bc="CYCL", artiv=0, k_stage=3, steps=2,
+                              ic="smooth", mu=0.01, eta0=0.02),
+    # both models at once: comp_viscous_flux takes both branches
+    # (compflux.f90:205-256), AV ramp active, FREE faces, SSP(5,4)
+    "visc_n6_N5_free_av": dict(nx=6, ny=6, N=5, jitter=0.10, bc="FREE", artiv=1, k_stage=5, steps=2,
+                               ic="rough", mu=0.005, eta0=0.01),
+    # MHD rotor, gam = 1.4, N=6, AV, FREE faces, on the central part of the
+    # unit square (16^2 cells of h = 0.025); CFL 0.05: at CFL 0.2 the
+    # reference's own state blows up on a mesh this coarse
+    "rotor_n16_N6_free_av_g14": dict(nx=16, ny=16, dom=(0.3, 0.7, 0.3, 0.7), N=6, jitter=0.10,
+                                     bc="FREE", artiv=1, k_stage=3, steps=3, ic="rotor", gam=1.4,
+                                     dt=("cfl", 0.05)),
+    # field-loop advection, periodic, N=5, AV off, CFL 0.2
+    "loop_n16x8_N5": dict(nx=16, ny=8, dom=(-1.0, 1.0, -0.5, 0.5), N=5, jitter=0.10, bc="CYCL",
+                          artiv=0, k_stage=3, steps=3, ic="loop", dt=("cfl", 0.2)),
+}
 
 
-def smooth_state(mesh, N, sol_xy):
+def smooth_state(mesh, N, sol_xy, gam=5.0 / 3):
     """A smooth state with every component non-zero (w and Bz included), so
     every term of the fluxes contributes.  sol_xy: (NCELL, N, N, 2) points."""
     x = sol_xy[..., 0]
     y = sol_xy[..., 1]
-    gam = 5.0 / 3
     rho = 1.0 + 0.2 * np.sin(x) * np.cos(y)
     u = 0.5 * np.sin(y) + 0.1
     v = -0.3 * np.cos(x)
@@ -61,31 +89,88 @@ def smooth_state(mesh, N, sol_xy):
     return np.stack([rho, rho * u, rho * v, rho * w, E, Bx, By, Bz, Az], axis=-1)
 
 
-def run_case(name, spec, outdir):
-    n, N, jit, bc, av, kst, steps, ic = spec
-    mesh = quad_grid(n, n, 0.0, TWO_PI, 0.0, TWO_PI, jitter=jit, seed=SEED, bc=bc)
+def rough_state(mesh, N, sol_xy, gam=5.0 / 3, seed=11):
+    """The smooth state with density raised by 30 % at a seeded fifth of the
+    solution points (and E raised by 1), so the AV sensor of the cells they
+    fall in lies between its thresholds or above."""
+    Q = smooth_state(mesh, N, sol_xy, gam)
+    jump = np.random.default_rng(seed).random(Q.shape[:3]) > 0.8
+    Q[..., 0] *= 1.0 + 0.3 * jump
+    Q[..., 4] += 1.0
+    return Q
+
+
+class _OracleAsSolver:
+    """The C oracle behind the set_state/get_state/correct_b calls of chorus_mhd.ic."""
+
+    def __init__(self, o):
+        self.o = o
+
+    def set_state(self, Q):
+        self.o.set_q(Q)
+
+    def get_state(self):
+        return self.o.get_q()
+
+    def correct_b(self):
+        self.o.correct_b()
+
+
+def _spec(name):
+    if name in CASES:
+        n, N, jit, bc, av, kst, steps, ic = CASES[name]
+        return dict(nx=n, ny=n, dom=TWO_PI_DOM, N=N, jitter=jit, bc=bc, artiv=av, k_stage=kst,
+                    steps=steps, ic=ic, gam=5.0 / 3, mu=0.0, eta0=0.0, dt=DT)
+    d = dict(dom=TWO_PI_DOM, gam=5.0 / 3, mu=0.0, eta0=0.0, dt=DT)
+    d.update(XCASES[name])
+    return d
+
+
+def run_case(name, outdir):
+    sp = _spec(name)
+    nx, ny, N, jit, bc, av = sp["nx"], sp["ny"], sp["N"], sp["jitter"], sp["bc"], sp["artiv"]
+    gam, ic = sp["gam"], sp["ic"]
+    x0, x1, y0, y1 = sp["dom"]
+    mesh = quad_grid(nx, ny, x0, x1, y0, y1, jitter=jit, seed=SEED, bc=bc)
     work = tempfile.mkdtemp()
     try:
         mesh.write(work)
-        q = QuadInp(N=N, dt=DT, MAXITER=steps, DXCYCL=mesh.dxcycl, DYCYCL=mesh.dycycl,
-                    ARTIV=av, k_stage=kst, restart_ord=N)
-        if ic == "smooth":
+        dt = sp["dt"]
+        Q0 = None
+        if ic in ("rotor", "loop"):
+            # the product's initial conditions (chorus_mhd.ic), B := curl A_z by
+            # the oracle's CT (pinned to the reference's correct_B_via_curl_Az)
+            from chorus_mhd import ic as ics
+            o = orc.Oracle(mesh, N, dt=1.0, artiv=av, gam=gam)
+            if ic == "rotor":
+                Q0 = ics.rotor(_OracleAsSolver(o), mesh, N, gam=gam)
+            else:
+                Q0 = ics.field_loop(_OracleAsSolver(o), mesh, N, gam=gam)
+        if isinstance(dt, tuple):
+            from chorus_mhd import ic as ics
+            dt = ics.cfl_dt(mesh, N, Q0, gam, cfl=dt[1])
+        q = QuadInp(N=N, dt=dt, MAXITER=sp["steps"], DXCYCL=mesh.dxcycl, DYCYCL=mesh.dycycl,
+                    ARTIV=av, k_stage=sp["k_stage"], restart_ord=N, mu=sp["mu"], eta0=sp["eta0"])
+        if ic in ("smooth", "rough"):
             # solution-point coordinates from the reference itself, then a restart
             q.write(os.path.join(work, "QUAD.INP"))
-            orc.run_reference(work, "setup")
+            orc.run_reference(work, "setup", gam=gam)
             geom = np.fromfile(os.path.join(work, "geom.f64"))
             nv, nc = mesh.nvert, mesh.ncell
             off = 2 * nv + nc * N * N + 2 * nc * 4 * (N + 1) * N
             sxy = geom[off:off + 2 * nc * N * N].reshape(nc, N, N, 2)
-            Q0 = smooth_state(mesh, N, sxy)
+            Q0 = (smooth_state if ic == "smooth" else rough_state)(mesh, N, sxy, gam)
+        if Q0 is not None:
             write_restart(os.path.join(work, "restart.in"), 0, 0.0, Q0)
             q.restart = 1
             q.NAMERESTART = "restart.in"
         q.write(os.path.join(work, "QUAD.INP"))
-        orc.run_reference(work, "setup")
+        orc.run_reference(work, "setup", gam=gam)
         topo = np.fromfile(os.path.join(work, "topo.i32"), dtype="<i4")
         q0 = np.fromfile(os.path.join(work, "q0.f64")).reshape(mesh.ncell, N, N, 9)
-        orc.run_reference(work, "resid")
+        if Q0 is not None:
+            assert np.array_equal(q0, Q0), "restart read back differs"
+        orc.run_reference(work, "resid", gam=gam)
         resid = np.fromfile(os.path.join(work, "resid.f64")).reshape(mesh.ncell, N, N, 9)
         extra = {}
         if av:
@@ -93,14 +178,17 @@ def run_case(name, spec, outdir):
             extra = dict(eigvmax=a[0], epsilon0=a[2],
                          sm_ind=a[3:3 + 9 * mesh.ncell].reshape(mesh.ncell, 9),
                          muav=a[3 + 9 * mesh.ncell:3 + 18 * mesh.ncell].reshape(mesh.ncell, 9))
-        orc.run_reference(work, "run")
+        orc.run_reference(work, "run", gam=gam)
         qend = np.fromfile(os.path.join(work, "qend.f64")).reshape(mesh.ncell, N, N, 9)
+        if not (np.isfinite(resid).all() and np.isfinite(qend).all()):
+            raise RuntimeError("%s: the reference's residual or state is not finite" % name)
         d = open(os.path.join(work, "diag.txt")).read().split()
         np.savez_compressed(
             os.path.join(outdir, name + ".npz"),
-            n=n, N=N, jitter=jit, bc=bc, artiv=av, k_stage=kst, steps=steps, ic=ic, dt=DT,
-            seed=SEED, xv=mesh.xv, yv=mesh.yv, topo=topo, q0=q0, resid=resid, qend=qend,
-            diag_iter=int(d[0]), diag_resid=float(d[2]), diag_divb=float(d[3]), **extra)
+            n=nx, nx=nx, ny=ny, dom=np.array([x0, x1, y0, y1]), N=N, jitter=jit, bc=bc, artiv=av,
+            k_stage=sp["k_stage"], steps=sp["steps"], ic=ic, dt=dt, gam=gam, mu=sp["mu"],
+            eta0=sp["eta0"], seed=SEED, xv=mesh.xv, yv=mesh.yv, topo=topo, q0=q0, resid=resid,
+            qend=qend, diag_iter=int(d[0]), diag_resid=float(d[2]), diag_divb=float(d[3]), **extra)
     finally:
         shutil.rmtree(work, ignore_errors=True)
 
@@ -139,10 +227,10 @@ def main():
     if not os.path.exists(orc.REF_HARNESS):
         raise SystemExit("build the reference first: make -C oracle ref")
     only = sys.argv[1:]
-    for name, spec in CASES.items():
+    for name in list(CASES) + list(XCASES):
         if only and name not in only:
             continue
-        run_case(name, spec, HERE)
+        run_case(name, HERE)
         print("wrote", name)
     for name, spec in PLOT_CASES.items():
         if only and name not in only:

@@ -11,6 +11,9 @@ import sys
 import numpy as np
 import pytest
 
+from conftest import golden_params, load_golden
+
+import oracle as orc
 from chorus_mhd import quad_grid, write_restart
 from chorus_mhd import ic as ics
 from chorus_mhd.solver import Batch, Instance, Solver
@@ -108,6 +111,65 @@ def test_batch_autotune_and_divb():
         assert np.array_equal(b.get_state(i), s.get_state()), "instance %d after autotune" % i
         assert dv[i] == s.diag()["divbmax"]
         s.close()
+    b.close()
+
+
+# reference fixtures run as members of one batch (all SSP(5,3)): N = 3..6,
+# periodic and FREE faces, AV on and off, mu/eta0 != 0, the gam = 1.4 rotor
+# and the field loop; the three N=4 AV members with dt = 1e-3 share one group
+BATCH_GOLDENS = ["ot_n6_N3_free_av", "ot_n5_N6_free", "rotor_n16_N6_free_av_g14", "ot_n8_N4_jitter_av",
+                 "rough_n6_N4_jitter_av", "loop_n16x8_N5", "visc_n6_N4_jitter"]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_members_vs_reference(precision):
+    """Every member of a batch against the reference's own fixture (or, for
+    the one member made here, against the oracle), not against another HIP
+    run: FP64 bitwise without AV and within 1e-13 of each field's magnitude
+    with AV; mixed within 1e-6 relative L2 (residuals within 1e-5)."""
+    _say("batch vs reference", precision)
+    gs = [load_golden(n) for n in BATCH_GOLDENS]
+    mem = [Instance(g["mesh"], int(g["N"]), float(g["dt"]), precision=precision, **golden_params(g))
+           for g in gs]
+    # an oracle-made Orszag-Tang member in the group of the N=4 AV fixtures
+    om = quad_grid(10, 6, 0.0, 2 * math.pi, 0.0, 2 * math.pi, jitter=0.12, seed=21)
+    o = orc.Oracle(om, 4, dt=1e-3, artiv=1)
+    o.init_ot()
+    oq0 = o.get_q()
+    o.steps(3)
+    oq3 = o.get_q()
+    mem.append(Instance(om, 4, 1e-3, artiv=1, precision=precision))
+    b = Batch(mem)
+    groups = b.groups()
+    assert sum(x["ninst"] for x in groups) == len(mem)
+    assert max(x["ninst"] for x in groups) == 3
+    assert sorted({x["N"] for x in groups}) == [3, 4, 5, 6]
+    for i, g in enumerate(gs):
+        b.set_state(i, g["q0"])
+    b.set_state(len(gs), oq0)
+
+    def check(i, got, ref, artiv, what, budget=1e-6):
+        if precision == "mixed":
+            rel = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+            _say(what, "mixed rel L2", rel)
+            assert rel <= budget, what
+        elif artiv == 0:
+            assert np.array_equal(got, ref), what
+        else:
+            scale = np.abs(ref).max(axis=(0, 1, 2))
+            assert (np.abs(got - ref).max(axis=(0, 1, 2)) <= 1e-13 * scale).all(), what
+
+    for i, g in enumerate(gs):
+        # the mixed form's residual budget is 1e-5 (its state budget 1e-6)
+        check(i, b.compresid(i), g["resid"], int(g["artiv"]), "residual of " + BATCH_GOLDENS[i], 1e-5)
+    done = 0
+    for target in (2, 3):
+        b.step(target - done)
+        done = target
+        for i, g in enumerate(gs):
+            if int(g["steps"]) == target:
+                check(i, b.get_state(i), g["qend"], int(g["artiv"]), "%s after %d steps" % (BATCH_GOLDENS[i], target))
+    check(len(gs), b.get_state(len(gs)), oq3, 1, "oracle member after 3 steps")
     b.close()
 
 

@@ -21,9 +21,16 @@ from chorus_mhd import QuadInp, quad_grid, read_restart, write_restart
 pytestmark = pytest.mark.gpu
 
 DROPIN = os.path.join(REPO, "chorus-mhd_amd", "fortran", "_build", "sd2d_mhd_hip")
+# the driver linked with the reference objects built with gam = 1.4
+DROPIN_G14 = os.path.join(REPO, "chorus-mhd_amd", "fortran", "_build", "g14", "sd2d_mhd_hip")
 
 
-@pytest.mark.skipif(not os.path.exists(DROPIN), reason="drop-in driver not built (needs the reference sources)")
+def _driver(g):
+    return DROPIN_G14 if float(g["gam"]) == 1.4 else DROPIN
+
+
+@pytest.mark.skipif(not (os.path.exists(DROPIN) and os.path.exists(DROPIN_G14)),
+                    reason="drop-in driver not built (needs the reference sources)")
 @pytest.mark.parametrize("name", golden_names())
 def test_fortran_dropin_matches_reference(name, tmp_path):
     print("case dropin", name, flush=True)
@@ -35,8 +42,9 @@ def test_fortran_dropin_matches_reference(name, tmp_path):
     write_restart(os.path.join(d, "restart.in"), 0, 0.0, g["q0"])
     QuadInp(N=N, dt=float(g["dt"]), MAXITER=steps, DXCYCL=mesh.dxcycl, DYCYCL=mesh.dycycl,
             ARTIV=int(g["artiv"]), k_stage=int(g["k_stage"]), restart=1, restart_ord=N,
-            NAMERESTART="restart.in", nwrite=1000000).write(os.path.join(d, "QUAD.INP"))
-    r = subprocess.run([DROPIN], cwd=d, capture_output=True, text=True, timeout=120)
+            NAMERESTART="restart.in", nwrite=1000000, mu=float(g["mu"]),
+            eta0=float(g["eta0"])).write(os.path.join(d, "QUAD.INP"))
+    r = subprocess.run([_driver(g)], cwd=d, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     # the same run with the reference's own restart writer on the copied-back
     # state: the device-packed dumps must equal its files byte for byte
@@ -45,7 +53,7 @@ def test_fortran_dropin_matches_reference(name, tmp_path):
     for f in os.listdir(d):
         if f in ("restart.in", "QUAD.INP") or f.endswith(".dat") and not f.startswith("restart."):
             shutil.copy(os.path.join(d, f), d2)
-    r = subprocess.run([DROPIN, "refio"], cwd=d2, capture_output=True, text=True, timeout=120)
+    r = subprocess.run([_driver(g), "refio"], cwd=d2, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     dumps = sorted(f for f in os.listdir(d2) if f.startswith("restart.0"))
     assert dumps == sorted(f for f in os.listdir(d) if f.startswith("restart.0")) and len(dumps) == 2
@@ -91,8 +99,10 @@ def test_fortran_dropin_tecplot_matches_reference(name, tmp_path):
     assert open(out, "rb").read() == open(ref, "rb").read()
 
 
-@pytest.mark.skipif(not os.path.exists(DROPIN), reason="drop-in driver not built (needs the reference sources)")
-@pytest.mark.parametrize("name", ["ot_n8_N4_jitter_av", "smooth_n6_N5_free_av"])
+@pytest.mark.skipif(not (os.path.exists(DROPIN) and os.path.exists(DROPIN_G14)),
+                    reason="drop-in driver not built (needs the reference sources)")
+@pytest.mark.parametrize("name", ["ot_n8_N4_jitter_av", "smooth_n6_N5_free_av", "visc_n6_N5_free_av",
+                                  "rotor_n16_N6_free_av_g14"])
 def test_fortran_dropin_mixed_within_budget(name, tmp_path):
     """The driver's mixed form (argument "mixed") against the reference's FP64
     state: relative L2 within the 1e-6 budget of the mixed form."""
@@ -105,8 +115,9 @@ def test_fortran_dropin_mixed_within_budget(name, tmp_path):
     write_restart(os.path.join(d, "restart.in"), 0, 0.0, g["q0"])
     QuadInp(N=N, dt=float(g["dt"]), MAXITER=steps, DXCYCL=mesh.dxcycl, DYCYCL=mesh.dycycl,
             ARTIV=int(g["artiv"]), k_stage=int(g["k_stage"]), restart=1, restart_ord=N,
-            NAMERESTART="restart.in", nwrite=1000000).write(os.path.join(d, "QUAD.INP"))
-    r = subprocess.run([DROPIN, "mixed"], cwd=d, capture_output=True, text=True, timeout=120)
+            NAMERESTART="restart.in", nwrite=1000000, mu=float(g["mu"]),
+            eta0=float(g["eta0"])).write(os.path.join(d, "QUAD.INP"))
+    r = subprocess.run([_driver(g), "mixed"], cwd=d, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     it, t, Q = read_restart(os.path.join(d, "restart.%08d.dat" % steps), N)
     assert it == steps

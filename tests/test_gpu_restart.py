@@ -13,7 +13,7 @@ import struct
 import numpy as np
 import pytest
 
-from conftest import golden_names, load_golden
+from conftest import golden_names, golden_params, load_golden
 
 from chorus_mhd import quad_grid, read_restart, write_restart
 from chorus_mhd.solver import Solver
@@ -27,7 +27,7 @@ def test_restart_dump_matches_format(name, precision, tmp_path):
     print("restart", name, precision, flush=True)
     g = load_golden(name)
     N = int(g["N"])
-    s = Solver(g["mesh"], N, float(g["dt"]), artiv=int(g["artiv"]), k_stage=int(g["k_stage"]), precision=precision)
+    s = Solver(g["mesh"], N, float(g["dt"]), precision=precision, **golden_params(g))
     s.set_state(g["q0"])
     s.step(2)
     ctime = float(g["dt"]) + float(g["dt"])

@@ -12,7 +12,7 @@ import math
 import numpy as np
 import pytest
 
-from conftest import golden_names, load_golden, split_topo
+from conftest import golden_names, golden_params, load_golden, split_topo
 
 import oracle as orc
 
@@ -25,8 +25,7 @@ def _build():
 
 
 def _oracle(g):
-    return orc.Oracle(g["mesh"], int(g["N"]), dt=float(g["dt"]), artiv=int(g["artiv"]),
-                      k_stage=int(g["k_stage"]))
+    return orc.Oracle(g["mesh"], int(g["N"]), dt=float(g["dt"]), **golden_params(g))
 
 
 @pytest.mark.parametrize("name", NAMES)

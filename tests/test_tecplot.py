@@ -127,3 +127,44 @@ def test_tecplot_linear_state_exact(N, nre, bc, tmp_path):
         err = np.abs(f["data"][2 + k] - want[k]).max() / np.abs(want[k]).max()
         assert err < 2e-6, (f["names"][2 + k], err)
     assert not f["data"][11:].any()
+
+
+@pytest.mark.gpu
+def test_av_columns_reset_by_set_state(tmp_path):
+    """A newly loaded state has been evaluated by no stage: its muAV/etaAV
+    columns are zeros and chorus_diag's wave speed is NaN, as before the first
+    stage; the next step fills them again.  In a batch only the reloaded
+    member loses its columns, not the other member of its group."""
+    print("AV columns after set_state", flush=True)
+    from conftest import golden_params, load_golden
+    from chorus_mhd.solver import Batch, Instance
+    g = load_golden("rough_n6_N4_jitter_av")       # AV ramp active in every cell
+    mesh, N, dt = g["mesh"], int(g["N"]), float(g["dt"])
+    s = Solver(mesh, N, dt, **golden_params(g))
+    s.plot_setup(2)
+    s.set_state(g["q0"])
+    out = str(tmp_path / "a.plt")
+    s.step(1)
+    s.write_tecplot(out, 0.0)
+    assert read_plt(out)["data"][11].any()
+    assert math.isfinite(s.diag()["eigvmax"])
+    s.set_state(g["q0"])
+    s.write_tecplot(out, 0.0)
+    assert not read_plt(out)["data"][11:].any()
+    assert math.isnan(s.diag()["eigvmax"])
+    s.step(1)
+    s.write_tecplot(out, 0.0)
+    assert read_plt(out)["data"][11].any()
+    s.close()
+    b = Batch([Instance(mesh, N, dt, precision="fp64", **golden_params(g)) for _ in range(2)])
+    assert len(b.groups()) == 1
+    for i in range(2):
+        b.plot_setup(i, 2)
+        b.set_state(i, g["q0"])
+    b.step(1)
+    b.set_state(0, g["q0"])
+    b.write_tecplot(0, out, 0.0)
+    assert not read_plt(out)["data"][11:].any()
+    b.write_tecplot(1, out, 0.0)
+    assert read_plt(out)["data"][11].any()
+    b.close()
