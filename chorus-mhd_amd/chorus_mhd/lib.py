@@ -50,6 +50,7 @@ SIGNATURES = {
     "chorus_correct_b": (C.c_int, [C.c_void_p]),
     "chorus_step": (C.c_int, [C.c_void_p, C.c_int]),
     "chorus_diag": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp]),
+    "chorus_diag_noalloc": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp]),
     "chorus_topology": (C.c_int, [C.c_int, C.c_int, _ip, _dp, _dp, C.c_int, _ip, _ip,
                                   C.c_double, C.c_double, _ip, _ip, _ip, _ip, _ip, _ip,
                                   _ip, _ip]),

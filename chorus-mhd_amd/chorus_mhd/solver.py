@@ -186,6 +186,14 @@ class Solver:
         return dict(zip(["residglob", "divbglob", "divbmax", "eigvmax", "epsilon0"],
                         [x.value for x in v]))
 
+    def diag_noalloc(self) -> dict:
+        """diag() through chorus_diag_noalloc: no device allocation, partials in
+        the library's fixed per-GPU scratch array."""
+        v = [C.c_double() for _ in range(5)]
+        self._check(_lib.load().chorus_diag_noalloc(self._ctx, *[C.byref(x) for x in v]), "chorus_diag_noalloc")
+        return dict(zip(["residglob", "divbglob", "divbmax", "eigvmax", "epsilon0"],
+                        [x.value for x in v]))
+
     def plot_setup(self, nre: int) -> None:
         """Plot-node topology of the Tecplot output with NRE x NRE plot cells per
         cell (tecplotter2dsetup); call before stepping (see chorus_plot_setup)."""

@@ -66,6 +66,8 @@ struct chorus_ctx {
     int last_parity = -1;          // slot written by the last stage
     double h_msh = 0.0;            // AV length scale (AV.f90:147)
     int nblk = 0;
+    int device = -1;               // the HIP device current at chorus_init
+    double *scratch = nullptr;     // that device's diagnostics scratch array (chorus_diag_noalloc)
     int cur = 0;                   // buffer holding the current state
     int last_in = -1;              // buffer holding the last stage's input state
     std::vector<hipEvent_t> ev;    // timing events (chorus_time_steps)
@@ -205,14 +207,14 @@ int run_trace(chorus_ctx *c, const double *Q) {
     return 0;
 }
 
-int run_ct(chorus_ctx *c, double *Q, int which) {
+int run_ct(chorus_ctx *c, double *Q, int which, double *partial = nullptr) {
     CTArgs a{};
     a.ncell = c->ncell;
     a.cellxy = c->cellxy;
     a.nbr = c->nbr;
     a.Q = Q;
     a.which = which;
-    a.partial = c->partial;
+    a.partial = partial ? partial : c->partial;
     a.TQ = c->TQ;
     a.TQout = c->TQ;
     hipError_t e = launch_ct(c->N, a, c->stream);
@@ -580,6 +582,10 @@ int init_ctx(chorus_ctx *c, const chorus_mesh *m, const chorus_params *p) {
         HIP_TRY(c, upload_operators_mixed());
         g_ops_uploaded = true;
     }
+    // the scratch array of chorus_diag_noalloc on this device, resolved now
+    // with the device current (once per device)
+    HIP_TRY(c, hipGetDevice(&c->device));
+    HIP_TRY(c, diag_scratch(c->device, &c->scratch));
     HIP_TRY(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     const size_t ns = state_len(c);
     HIP_TRY(c, hipMalloc(&c->cellxy, xy.size() * sizeof(double)));
@@ -756,6 +762,50 @@ int chorus_diag(chorus_ctx *c, double *residglob, double *divbglob, double *divb
         if (eigvmax) *eigvmax = e;
         if (epsilon0) *epsilon0 = (c->p.artiv == 1) ? e * c->h_msh / (double)(N - 1) * 1.0 : NAN;
     }
+    return 0;
+}
+
+int chorus_diag_noalloc(chorus_ctx *c, double *residglob, double *divbglob, double *divbmax, double *eigvmax,
+                        double *epsilon0) {
+    if (!c) return CHORUS_ERR_ARG;
+    if (!c->scratch) return fail(c, CHORUS_ERR_HIP, "chorus_diag_noalloc: no scratch array on this device");
+    // scratch: the div B partials of every cell block, the residual partials, 3 results
+    const size_t need = 2 * (size_t)c->nblk + (size_t)kDiagResBlocks + 3;
+    if (need > kDiagScratchDoubles)
+        return fail(c, CHORUS_ERR_ARG, "chorus_diag_noalloc: the mesh needs more scratch than the fixed array "
+                                       "holds (2 x cell blocks + 1027 doubles > 1048576); use chorus_diag");
+    const int N = c->N;
+    const double npts = (double)c->ncellglob * N * N;
+    const bool resid = residglob && c->last_in >= 0;
+    double out[3] = {0.0, 0.0, 0.0};
+    if (resid || divbglob || divbmax) {
+        if (resid) {
+            // the residual of the last stage's input (as chorus_diag); the face
+            // passes read that state's traces, the current state's are restored
+            StageArgs s = base_args(c);
+            s.Qin = c->X[c->last_in];
+            s.Rout = c->R;
+            s.kind = KIND_RESID;
+            int rc = run_trace(c, s.Qin);
+            if (!rc) rc = launch_stage(c, s);
+            if (!rc) rc = run_trace(c, c->X[c->cur]);
+            if (rc) return rc;
+        }
+        int rc = run_ct(c, c->X[c->cur], 1, c->scratch);
+        if (rc) return rc;
+        HIP_TRY(c, launch_diag_reduce(resid ? c->R : nullptr, state_len(c), c->nblk, c->scratch, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(out, c->scratch + 2 * (size_t)c->nblk + kDiagResBlocks, sizeof(out),
+                                  hipMemcpyDeviceToHost, c->stream));
+    }
+    double e = NAN;
+    if ((eigvmax || epsilon0) && !c->ideal && c->last_parity >= 0)
+        HIP_TRY(c, hipMemcpyAsync(&e, c->eig + c->last_parity, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (residglob) *residglob = resid ? std::sqrt(out[2] / (npts * 5)) : NAN;
+    if (divbglob) *divbglob = std::sqrt(out[0] / npts);
+    if (divbmax) *divbmax = out[1];
+    if (eigvmax) *eigvmax = e;
+    if (epsilon0) *epsilon0 = (c->p.artiv == 1) ? e * c->h_msh / (double)(N - 1) * 1.0 : NAN;
     return 0;
 }
 

@@ -1,5 +1,7 @@
 // FP64 drop-in kernels.  Built with -ffp-contract=off: no FMA is formed, so each
 // product and sum rounds separately, exactly like the reference build.
+#include <mutex>
+
 #include "kernels_visc.hpp"
 
 namespace chorus {
@@ -67,6 +69,83 @@ int stage_blocks(int N, int ncell) {
     }
 }
 int ct_blocks(int N, int ncell) { return stage_blocks(N, ncell); }
+
+// ---------------------------------------------------------------------------
+// Scratch of chorus_diag_noalloc: a fixed device array per GPU (held until the
+// process ends) instead of memory the call allocates.  Layout for a context of
+// nblk cell blocks:
+//   [0, 2 nblk)                      k_ct<N,1> partials {sum divB^2, max |divB|} per block
+//   [2 nblk, 2 nblk + kDiagResBlocks) k_sumsq partial sums of resid^2
+//   the next 3 doubles               k_diag_final: sum divB^2, max |divB|, sum resid^2
+// In host code the name `scratch` only stands in for the array: kernels get
+// its device address from diag_scratch().  Every call on one GPU shares the
+// array; calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[kDiagScratchDoubles];
+
+hipError_t diag_scratch(int device, double **ptr) {
+    static std::mutex mu;
+    static double *addr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device) return hipErrorInvalidDevice;   // resolved with that device current
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[device] = static_cast<double *>(p);
+    }
+    *ptr = addr[device];
+    return hipSuccess;
+}
+
+// partial sums of R[i]^2 over a grid-stride of the array, one per block
+__global__ void __launch_bounds__(256) k_sumsq(const double *R, size_t n, double *part) {
+    __shared__ double red[256];
+    const int tid = threadIdx.x;
+    double s = 0.0;
+    for (size_t i = (size_t)blockIdx.x * 256 + tid; i < n; i += (size_t)gridDim.x * 256) s = s + R[i] * R[i];
+    red[tid] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (tid < k) red[tid] = red[tid] + red[tid + k];
+        __syncthreads();
+    }
+    if (tid == 0) part[blockIdx.x] = red[0];
+}
+
+// one thread: the div B block partials added and maxed in block order (the
+// order chorus_diag adds them on the host, so the two agree bit for bit), and
+// the residual partials added
+__global__ void k_diag_final(const double *part, int nblk, const double *rpart, int nres, double *out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double s2 = 0.0, mx = 0.0, r2 = 0.0;
+    for (int b = 0; b < nblk; b++) {
+        s2 += part[2 * b];
+        const double m = part[2 * b + 1];
+        mx = (mx < m) ? m : mx;
+    }
+    for (int b = 0; b < nres; b++) r2 += rpart[b];
+    out[0] = s2;
+    out[1] = mx;
+    out[2] = r2;
+}
+
+hipError_t launch_diag_reduce(const double *R, size_t n, int nblk, double *scr, hipStream_t s) {
+    double *rpart = scr + 2 * (size_t)nblk;
+    int nres = 0;
+    if (R) {
+        const size_t want = (n + 255) / 256;
+        nres = (int)(want < (size_t)kDiagResBlocks ? want : (size_t)kDiagResBlocks);
+        if (nres < 1) nres = 1;
+        hipLaunchKernelGGL(k_sumsq, dim3(nres), dim3(256), 0, s, R, n, rpart);
+    }
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s, scr, nblk, rpart, nres, rpart + kDiagResBlocks);
+    return hipGetLastError();
+}
 
 template <int N> static int vblocks(int ncell) { return (ncell + VShape<N>::CPB - 1) / VShape<N>::CPB; }
 

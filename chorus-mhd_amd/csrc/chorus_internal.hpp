@@ -178,6 +178,19 @@ hipError_t launch_stage_visc_mixed(int N, const StageArgs &a, const ViscArgs &v,
                                    int tile);
 int ct_blocks(int N, int ncell);
 int stage_blocks(int N, int ncell);
+
+// chorus_diag_noalloc: a fixed device array per GPU holds the partials of the
+// diagnostics (see chorus_fp64.hip for its layout).  8 MiB: the div B partials
+// of up to about 520k cell blocks (5.2M cells at N=5) fit.
+constexpr size_t kDiagScratchDoubles = size_t(1) << 20;
+constexpr int kDiagResBlocks = 1024;      // partial sums of the residual norm
+constexpr int kMaxDevices = 64;
+// device address of the array on `device`, which must be the current device
+// on the first call for it (hipGetSymbolAddress, once per device)
+hipError_t diag_scratch(int device, double **ptr);
+// residual norm partials of R[0, n) (when R is set) and the final sums of the
+// div B partials already in scr[0, 2 nblk)
+hipError_t launch_diag_reduce(const double *R, size_t n, int nblk, double *scr, hipStream_t s);
 hipError_t upload_operators_fp64();
 hipError_t upload_operators_mixed();
 

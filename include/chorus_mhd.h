@@ -97,6 +97,22 @@ int chorus_step(chorus_ctx *ctx, int nsteps);
 int chorus_diag(chorus_ctx *ctx, double *residglob, double *divbglob, double *divbmax,
                 double *eigvmax, double *epsilon0);
 
+/* Allocation-free variant of chorus_diag (no counterpart in the reference; for
+ * callers that cannot allocate device memory or use heap-allocating library
+ * routines during the time loop: serving loops and harnesses that forbid
+ * hipMalloc).  Same outputs as chorus_diag: div B RMS and max bit for bit,
+ * the residual norm summed in another order (agreeing to about 1e-15
+ * relative).  The kernels' partials live in a fixed 8 MiB device array per
+ * GPU (held until the process ends) instead of context memory, and only the
+ * final scalars come back to the host.  Limit: 2 x (cell blocks) + 1027
+ * doubles must fit in 1048576 (cell blocks = ceil(NCELL / floor(256/N^2)),
+ * e.g. NCELL <= 5.2M at N=5); larger meshes get CHORUS_ERR_ARG.  Every call
+ * on one GPU shares the array: calls that overlap on one GPU (several host
+ * threads, or contexts whose streams run concurrently) race.  Synchronous, so
+ * not for stream capture. */
+int chorus_diag_noalloc(chorus_ctx *ctx, double *residglob, double *divbglob, double *divbmax,
+                        double *eigvmax, double *epsilon0);
+
 /* Host-side topology helper for drivers that do not run the reference's own
  * connectivity chain: CONNECTIVITY + CONNECT_BDRY + MAPCYCLOC
  * (connectivity.f90:1-233, connect_bdry.f90:1-354, map_cyc_loc.f90:1-42) for
