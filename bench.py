@@ -1,15 +1,18 @@
 #!/usr/bin/env python3
 """DOF-updates/s per RK stage of the CHORUS-MHD residual + SSP-RK time step on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ot512]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ensemble|ot512|...]
 
+Default workload: the batched ensemble of 256 independent Orszag-Tang / rotor
+instances (N = 3..6, meshes 32^2..256^2, AV on), the largest configuration that
+fits one GPU, in three forms: the mixed-precision batched form (the headline
+value), the FP64 batched form, and the FP64 drop-in run one instance at a time.
 One "step" is one SSP(5,3) time step (five residual evaluations fused with
-their stage updates, then the B := curl A_z correction) of the whole mesh,
-whose state is resident in HBM before timing starts.  The timed steps are
-chorus_time_steps: the kernels of chorus_step with HIP events around every
-launch.  DOF = one solution
-point (9 FP64 state values); value = NCELL*N^2 * 5 * K / wall time of the K
-timed steps.  Prints one JSON line.
+their stage updates, then the B := curl A_z correction) of every instance,
+whose states are resident in HBM before timing starts; the timed steps are the
+product path (chorus_batch_step / chorus_step: one hipGraph launch per step).
+DOF = one solution point (9 FP64 state values); value = sum over instances of
+NCELL*N^2 * 5 * K / wall time of the K timed steps.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -17,6 +20,7 @@ import argparse
 import json
 import math
 import os
+import re
 import subprocess
 import sys
 import tempfile
@@ -27,17 +31,55 @@ import numpy as np
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "chorus-mhd_amd"))
 
-from chorus_mhd import QuadInp, quad_grid          # noqa: E402
+from chorus_mhd import QuadInp, quad_grid, write_restart   # noqa: E402
 from chorus_mhd import ic as ics                    # noqa: E402
 from chorus_mhd.solver import Batch, Instance, Solver, topology   # noqa: E402
 from chorus_mhd.diagnostics import magnetic_energy, point_volumes   # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes of one stage (all kernels of the stage), rocprofv3 FETCH_SIZE and WRITE_SIZE per
-# dispatch scaled by kernels of known bytes (FETCH x 2.00, WRITE x 1.00); summaries under profiles/
-MEASURED_STAGE_TRAFFIC = {("ot512", "fp64"): 8.07e9, ("ot512", "mixed"): 5.73e9,
-                          ("ot512ideal", "fp64"): 2.76e9, ("ot512ideal", "mixed"): 2.47e9}
 METRIC = "DOF-updates/s per RK stage"
+# host cores of one GPU's share of the GPU box (the CPU baseline's process count)
+CPU_SHARE = 16
+PROFILES = os.path.join(REPO, "profiles")
+TILES = ["256/128", "128/64", "512/256"]
+
+
+def measured_traffic(key: str):
+    """HBM bytes of one unit of work (a stage or a batched step) from the
+    calibrated rocprofv3 counter summaries under profiles/: the line
+    'traffic[<key>] = <bytes> bytes' of the summary that measured it."""
+    pat = re.compile(r"traffic\[%s\] = ([0-9.eE+]+) bytes" % re.escape(key))
+    for f in sorted(os.listdir(PROFILES)) if os.path.isdir(PROFILES) else []:
+        if f.endswith(".md"):
+            m = pat.search(open(os.path.join(PROFILES, f)).read())
+            if m:
+                return float(m.group(1))
+    return None
+
+
+def host_info() -> dict:
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return dict(cpu_model=model, nproc=os.cpu_count(), affinity=len(os.sched_getaffinity(0)))
+
+
+def b_stage(N: int) -> float:
+    """Algorithmic HBM bytes per DOF per stage: FP64 state + RK registers + metrics
+    (224 + 32(N+1)/N)."""
+    return 224.0 + 32.0 * (N + 1) / N
+
+
+def b_ct(N: int) -> float:
+    """Algorithmic HBM bytes per DOF of the CT correction: A_z read, Bx and By
+    written, metrics (32 + 32(N+1)/N)."""
+    return 32.0 + 32.0 * (N + 1) / N
+
 
 # workloads (n, m, domain, N, jitter, seed, bc, ic, gamma, AV)
 WORKLOADS = {
@@ -70,12 +112,14 @@ ENSEMBLE = dict(
     desc="batched ensemble: 256 independent Orszag-Tang / rotor instances (seed 7 draws the type, "
          "B0 ~ U[0.5,1.5] scaling the problem's field, interior node jitter ~ U[0,15%]), meshes uniform "
          "in {32^2,64^2,128^2,256^2}, SD orders uniform in {2..5} (N=3..6), AV on, SSP(5,3); "
-         "instances of one order and problem type share one CFL-0.2 time step",
-    ninst=256, seed=7)
+         "instances of one order and problem type share one time step (Orszag-Tang CFL 0.2, "
+         "rotor CFL 0.1)",
+    ninst=256, seed=7, cfl={"ot": 0.2, "rotor": 0.1})
 
 
-def ensemble_members(ninst, seed):
+def ensemble_members(ninst, seed, cfl=None):
     """Meshes, raw fields and per-group time steps of the ensemble workload."""
+    cfl = cfl or ENSEMBLE["cfl"]
     rng = np.random.default_rng(seed)
     spec = []
     for i in range(ninst):
@@ -95,66 +139,204 @@ def ensemble_members(ninst, seed):
             mesh = quad_grid(n, n, 0.0, 1.0, 0.0, 1.0, jitter=jit, seed=1000 + i, bc="FREE")
             gam = 1.4
             Q, p = ics.rotor_fields(mesh, N, gam, b0 * 5.0 / math.sqrt(4 * math.pi))
-        dt = ics.cfl_dt(mesh, N, Q, gam, 0.2)
+        dt = ics.cfl_dt(mesh, N, Q, gam, cfl[kind])
         key = (N, kind)
         dts[key] = min(dts.get(key, dt), dt)
-        raw.append(dict(mesh=mesh, N=N, gam=gam, Q=Q, p=p, key=key, topo=topology(mesh)))
+        raw.append(dict(mesh=mesh, N=N, gam=gam, Q=Q, p=p, key=key, kind=kind, n=n, topo=topology(mesh)))
     for r in raw:
         r["dt"] = dts[r["key"]]
     return raw
 
 
+def _members(raw, precision):
+    return [Instance(r["mesh"], r["N"], r["dt"], gam=r["gam"], artiv=1, precision=precision, topo=r["topo"])
+            for r in raw]
+
+
+def _initial_states(raw):
+    """B := curl A_z of every instance on the device (FP64 in both forms), then
+    the energy from the corrected B: the states every form starts from."""
+    b = Batch(_members(raw, "fp64"))
+    for i, r in enumerate(raw):
+        b.set_state(i, r["Q"])
+    b.correct_b()
+    q0 = []
+    for i, r in enumerate(raw):
+        q = b.get_state(i)
+        q[..., 4] = ics.energy(q, r["p"], r["gam"])
+        q0.append(q)
+        r["Q"] = None          # the raw fields are no longer needed
+    b.close()
+    return q0
+
+
+def _timed_batch(raw, q0, precision, args):
+    b = Batch(_members(raw, precision))
+    for i, q in enumerate(q0):
+        b.set_state(i, q)
+    tiles, tune_s = None, 0.0
+    if precision == "mixed":
+        t0 = time.perf_counter()
+        tiles = b.autotune()
+        tune_s = time.perf_counter() - t0
+    if args.warmup:
+        b.step(args.warmup)           # captures the step graphs of every buffer rotation
+    t0 = time.perf_counter()
+    b.step(args.steps)                # synchronous: returns when the last step has finished
+    wall = time.perf_counter() - t0
+    states = [b.get_state(i) for i in range(len(raw))]
+    divb = b.divb()
+    groups = b.groups()
+    b.close()
+    return dict(wall=wall, states=states, divb=divb, groups=groups, tiles=tiles, autotune_s=tune_s)
+
+
+def _dropin_sequential(raw, q0, args):
+    """The FP64 drop-in as a user of chorus_step runs the ensemble: one context
+    per instance, one after the other, K timed steps each (after W warmup
+    steps that capture its step graphs)."""
+    wall = 0.0
+    for i, r in enumerate(raw):
+        s = Solver(r["mesh"], r["N"], r["dt"], gam=r["gam"], artiv=1, precision="fp64", topo=r["topo"])
+        s.set_state(q0[i])
+        if args.warmup:
+            s.step(args.warmup)
+        t0 = time.perf_counter()
+        s.step(args.steps)
+        wall += time.perf_counter() - t0
+        s.close()
+    return wall
+
+
+def _write_instance(d, mesh, N, dt, steps, artiv, q0=None, mu=0.0, eta0=0.0, metis=None):
+    mesh.write(d)
+    q = QuadInp(N=N, dt=dt, MAXITER=steps, DXCYCL=mesh.dxcycl, DYCYCL=mesh.dycycl, ARTIV=artiv,
+                restart_ord=N, mu=mu, eta0=eta0)
+    if q0 is not None:
+        write_restart(os.path.join(d, "restart.in"), 0, 0.0, q0)
+        q.restart = 1
+        q.NAMERESTART = "restart.in"
+    if metis is not None:
+        with open(os.path.join(d, "metis.dat"), "w") as f:
+            f.write("\n".join(str(int(p)) for p in metis) + "\n")
+        q.METIS_CELL = "metis.dat"
+    q.write(os.path.join(d, "QUAD.INP"))
+
+
+def _harness(gam):
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as orc
+    exe = orc.REF_HARNESS if gam == 5.0 / 3 else (orc.REF_HARNESS_G14 if gam == 1.4 else None)
+    return exe if exe and os.path.exists(exe) else None
+
+
+def cpu_ensemble_baseline(raw, q0, P=CPU_SHARE, steps=1):
+    """The compiled reference (oracle/_ref: unmodified sources; the gamma = 1.4
+    build for rotors) on the first P instances of the ensemble, P single-rank
+    processes at once, one per host core of the GPU's share: the aggregate
+    DOF-updates/s of the P concurrent runs (sum of each run's DOF x stages over
+    its own time-loop seconds).  Each instance starts from the GPU runs'
+    initial state (restart file).  The reference loop evaluates a diagnostic
+    flux and gradient every iteration (iterations.f90:35-36); the same runs
+    without it give the overhead."""
+    P = min(P, len(raw), os.cpu_count() or 1)
+    sample = list(range(P))
+    if any(_harness(raw[i]["gam"]) is None for i in sample):
+        return None
+    res = {}
+    with tempfile.TemporaryDirectory() as top:
+        dirs = []
+        for i in sample:
+            d = os.path.join(top, "i%03d" % i)
+            os.makedirs(d)
+            r = raw[i]
+            _write_instance(d, r["mesh"], r["N"], r["dt"], steps, 1, q0=q0[i])
+            dirs.append(d)
+        for mode in ("run", "runnd"):
+            env = dict(os.environ, OMP_NUM_THREADS="1")
+            procs = [subprocess.Popen([_harness(raw[i]["gam"]), mode], cwd=d, stdout=subprocess.DEVNULL,
+                                      stderr=subprocess.DEVNULL, env=env) for i, d in zip(sample, dirs)]
+            t0 = time.perf_counter()
+            rc = [p.wait(timeout=900) for p in procs]
+            wall = time.perf_counter() - t0
+            if any(rc):
+                return None
+            secs = [float(open(os.path.join(d, "timing.txt")).read().split()[1]) for d in dirs]
+            dofs = [raw[i]["mesh"].ncell * raw[i]["N"] ** 2 * 5 * steps for i in sample]
+            res[mode] = dict(value=sum(n / s for n, s in zip(dofs, secs)), loop_s=sum(secs), wall=wall)
+    h = host_info()
+    return dict(value=res["run"]["value"], unit="DOF-updates/s", cores=P, kind="reference",
+                cpu_model=h["cpu_model"], nproc=h["nproc"], processes=P,
+                value_without_diag_flux=res["runnd"]["value"],
+                diag_flux_overhead=res["run"]["loop_s"] / res["runnd"]["loop_s"] - 1.0,
+                sample="the first %d ensemble instances (%s), %d SSP(5,3) step each, as %d concurrent "
+                       "single-rank runs of the reference Fortran (flang -O3, unmodified sources; gamma=1.4 "
+                       "build for rotors), time loop only (setup excluded), including the diagnostic flux "
+                       "the reference loop evaluates every iteration (iterations.f90:35-36)" % (
+                           P, ", ".join("%s %d^2 N=%d" % (raw[i]["kind"], raw[i]["n"], raw[i]["N"]) for i in sample),
+                           steps, P))
+
+
 def run_ensemble(args):
     w = ENSEMBLE
-    steps = 10 if args.steps == 200 else args.steps
     t0 = time.perf_counter()
     raw = ensemble_members(w["ninst"], w["seed"])
+    q0 = _initial_states(raw)
     setup_s = time.perf_counter() - t0
-    results, finals = {}, {}
-    for precision in (["mixed"] if args.no_mixed else ["mixed", "fp64"]):
-        members = [Instance(r["mesh"], r["N"], r["dt"], gam=r["gam"], artiv=1, precision=precision, topo=r["topo"])
-                   for r in raw]
-        b = Batch(members)
-        for i, r in enumerate(raw):
-            b.set_state(i, r["Q"])
-        b.correct_b()
-        for i, r in enumerate(raw):
-            q = b.get_state(i)
-            q[..., 4] = ics.energy(q, r["p"], r["gam"])
-            b.set_state(i, q)
-        t1 = time.perf_counter()
-        tiles = b.autotune()
-        tune_s = time.perf_counter() - t1
-        if args.warmup:
-            b.step(args.warmup)
-        ms = b.time_steps(steps)
-        dof = sum(m.mesh.ncell * m.N * m.N for m in members)
-        finals[precision] = [b.get_state(i) for i in range(len(members))]
-        results[precision] = dict(value=dof * 5 * steps / (ms * 1e-3), ms_per_step=ms / steps, dof=dof,
-                                  groups=b.groups(), tiles=tiles, autotune_s=tune_s,
-                                  divb_max=float(np.nanmax(b.divb())))
-        b.close()
-    r = results["mixed"]
+    ndof = np.array([r["mesh"].ncell * r["N"] ** 2 for r in raw], dtype=np.float64)
+    res = {}
+    for precision in ("mixed", "fp64"):
+        res[precision] = _timed_batch(raw, q0, precision, args)
+    finite = np.array([bool(np.isfinite(a).all() and np.isfinite(b).all())
+                       for a, b in zip(res["fp64"]["states"], res["mixed"]["states"])])
+    rel = [float(np.linalg.norm(b - a) / np.linalg.norm(a))
+           for a, b, f in zip(res["fp64"]["states"], res["mixed"]["states"], finite) if f]
+    dof_all, dof_fin = float(ndof.sum()), float(ndof[finite].sum())
+    K = args.steps
+    mx, f64 = res["mixed"], res["fp64"]
+    value = dof_fin * 5 * K / mx["wall"]
+    ms_step = mx["wall"] * 1e3 / K
+    step_bytes = float(sum(n * (5 * b_stage(r["N"]) + b_ct(r["N"])) for n, r in zip(ndof, raw)))
+    achieved = step_bytes / (ms_step * 1e-3) / 1e9
+    f64_ms = f64["wall"] * 1e3 / K
     out = {
-        "metric": METRIC, "value": r["value"], "unit": "DOF-updates/s", "n_gpus": 1, "steps": steps,
-        "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "metric": METRIC, "value": value, "unit": "DOF-updates/s", "n_gpus": 1, "steps": K,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
-        "config": {"workload": w["desc"], "form": "mixed, batched", "instances": w["ninst"], "dof": r["dof"],
-                   "groups": [dict(g, tile_shape=["256/128", "128/64", "512/256"][g["tile"]]) for g in r["groups"]]},
-        "setup_s": setup_s, "autotune_s": r["autotune_s"], "divb_max": r["divb_max"],
+        "config": {"workload": w["desc"], "form": "mixed precision (FP32 flux arithmetic, FP64 state, RK, CT), "
+                   "batched: one hipGraph launch per step", "instances": w["ninst"],
+                   "dof": dof_all, "dof_finite": dof_fin, "finite_instances": int(finite.sum()),
+                   "groups": [dict(g, tile_shape=TILES[g["tile"]]) for g in mx["groups"]]},
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("ensemble256/mixed/step"),
+            "kernel": "one batched time step (one hipGraph launch: 8 groups' stage kernels x 5 stages + CT, "
+                      "groups concurrent); dominant kernels k_stage_visc_mx<N=4..6> and k_grad_mx",
+            "algorithmic_bytes_per_step": step_bytes,
+            "bytes_per_dof": "5 x (224 + 32(N+1)/N) + 32 + 32(N+1)/N",
+        },
+        "fp64_batched": {"value": dof_fin * 5 * K / f64["wall"], "ms_per_step": f64_ms,
+                         "roofline_frac": step_bytes / (f64_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "traffic": measured_traffic("ensemble256/fp64/step")},
+        "speedup_vs_fp64_batched": f64["wall"] / mx["wall"],
+        "mixed_rel_l2_vs_fp64_max": max(rel) if rel else None,
+        "mixed_rel_l2_vs_fp64_median": float(np.median(rel)) if rel else None,
+        "divb_max": {"fp64_finite": float(np.max(f64["divb"][finite])),
+                     "mixed_finite": float(np.max(mx["divb"][finite])),
+                     "fp64_ot": float(np.max([d for d, r, f in zip(f64["divb"], raw, finite) if f and r["kind"] == "ot"])),
+                     "fp64_rotor": float(np.max([d for d, r, f in zip(f64["divb"], raw, finite) if f and r["kind"] == "rotor"]))},
+        "setup_s": setup_s, "autotune_s": mx["autotune_s"],
     }
-    if "fp64" in results:
-        f = results["fp64"]
-        rel = []
-        for a, bq in zip(finals["fp64"], finals["mixed"]):
-            if np.isfinite(a).all() and np.isfinite(bq).all():
-                rel.append(float(np.linalg.norm(bq - a) / np.linalg.norm(a)))
-        out["fp64_batched"] = {"value": f["value"], "ms_per_step": f["ms_per_step"], "divb_max": f["divb_max"]}
-        out["speedup_vs_fp64"] = r["value"] / f["value"]
-        out["rel_l2_vs_fp64_max"] = max(rel) if rel else None
-        out["finite_instances"] = len(rel)
-        out["finite_fp64"] = sum(bool(np.isfinite(a).all()) for a in finals["fp64"])
-        out["finite_mixed"] = sum(bool(np.isfinite(a).all()) for a in finals["mixed"])
+    if not args.no_dropin:
+        t = _dropin_sequential(raw, q0, args)
+        out["fp64_dropin_sequential"] = {"value": dof_fin * 5 * K / t, "ms_per_step": t * 1e3 / K}
+        out["speedup_vs_fp64_dropin"] = t / mx["wall"]
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_ensemble_baseline(raw, q0)
+    if args.divb_report:
+        out["divb_per_instance"] = [dict(kind=r["kind"], n=r["n"], N=r["N"], finite=bool(f),
+                                         fp64=float(a), mixed=float(b))
+                                    for r, f, a, b in zip(raw, finite, f64["divb"], mx["divb"])]
     print(json.dumps(out), flush=True)
 
 
@@ -278,12 +460,6 @@ def cpu_output_baseline(w, nre, n=128):
                     n, n, w["ic"], w["jitter"], w["seed"], w["N"], nre))
 
 
-def b_stage(N: int) -> float:
-    """Algorithmic HBM bytes per DOF per stage: FP64 state + RK registers + metrics
-    (224 + 32(N+1)/N)."""
-    return 224.0 + 32.0 * (N + 1) / N
-
-
 def setup(w, precision, av_supported):
     mesh = quad_grid(w["n"], w["m"], *w["box"], jitter=w["jitter"], seed=w["seed"], bc=w["bc"])
     topo = topology(mesh)
@@ -300,76 +476,65 @@ def initial_state(w, mesh, s):
     return ics.rotor(s, mesh, w["N"], w["gam"])
 
 
-def cpu_baseline(w, artiv, budget_s=30.0):
-    """The compiled reference (oracle/_ref/ref_harness) on a bounded sample of
-    the workload on one host core; falls back to the C oracle port."""
-    sys.path.insert(0, os.path.join(REPO, "oracle"))
-    import oracle as orc
-    n = 96 if w["N"] >= 5 else 128
-    steps = 2
+def cpu_reference_mpi(w, artiv, P=CPU_SHARE, steps=2, max_dof=8.0e6):
+    """The compiled reference (oracle/_ref: unmodified sources; the gamma = 1.4
+    build for the rotor) with P MPI ranks (mpirun, a strip partition of the
+    cell rows as METIS_CELL, readGRIDinfo.f90:100-107), one per host core of
+    the GPU's share: the full mesh when it has at most max_dof points, else
+    the same workload on a coarser mesh of about that size.  Orszag-Tang starts
+    from the reference's own initial condition, the others from the same
+    initial-condition routine as the GPU run (restart file).  The timed part
+    is the reference's time loop (barrier to barrier), once as the reference
+    runs it (with the diagnostic flux of iterations.f90:35-36 every
+    iteration) and once without that flux."""
+    exe = _harness(w["gam"])
+    mpirun = "/opt/conda/bin/mpirun"
+    if exe is None or not os.path.exists(mpirun):
+        return None
+    P = min(P, os.cpu_count() or 1)
     N = w["N"]
-    sample = "%dx%d %s mesh (jitter %.2f, seed %d), N=%d, AV %s, %d SSP(5,3) steps" % (
-        n, n, w["ic"], w["jitter"], w["seed"], N, "on" if artiv else "off", steps)
-    mesh = quad_grid(n, n, *w["box"], jitter=w["jitter"], seed=w["seed"], bc=w["bc"])
+    n, m = w["n"], w["m"]
+    scale = 1.0
+    while n * m * N * N * scale * scale > max_dof:
+        scale *= 0.5
+    n, m = max(8, int(round(n * scale))), max(8, int(round(m * scale)))
+    mesh = quad_grid(n, m, *w["box"], jitter=w["jitter"], seed=w["seed"], bc=w["bc"])
+    q0 = None
+    if w["ic"] != "ot":
+        s = Solver(mesh, N, 1e-3, gam=w["gam"], artiv=artiv, topo=topology(mesh))
+        q0 = initial_state(w, mesh, s)
+        s.close()
+        dt = ics.cfl_dt(mesh, N, q0, w["gam"], 0.2)
+    else:
+        dt = ics.cfl_dt(mesh, N, ics.orszag_tang_fields(mesh, N, w["gam"])[0], w["gam"], 0.2)
+    part = ((np.arange(mesh.ncell) // n) * P) // m          # strips of cell rows, rank 0..P-1
+    res = {}
+    with tempfile.TemporaryDirectory() as d:
+        _write_instance(d, mesh, N, dt, steps, artiv, q0=q0, metis=part if P > 1 else None)
+        for mode in ("run", "runnd"):
+            r = subprocess.run([mpirun, "-np", str(P), exe, mode], cwd=d, capture_output=True, text=True,
+                               timeout=900, env=dict(os.environ, OMP_NUM_THREADS="1"))
+            if r.returncode != 0:
+                return None
+            t = open(os.path.join(d, "timing.txt")).read().split()
+            res[mode] = float(t[1])
     dofs = mesh.ncell * N * N * 5 * steps
-    if os.path.exists(orc.REF_HARNESS) and abs(w["gam"] - 5.0 / 3) < 1e-15:
-        with tempfile.TemporaryDirectory() as d:
-            mesh.write(d)
-            QuadInp(N=N, dt=1e-4, MAXITER=steps, DXCYCL=mesh.dxcycl, DYCYCL=mesh.dycycl,
-                    ARTIV=artiv, restart_ord=N).write(os.path.join(d, "QUAD.INP"))
-            r = subprocess.run([orc.REF_HARNESS, "run"], cwd=d, capture_output=True, text=True,
-                               timeout=600, env=dict(os.environ, OMP_NUM_THREADS="1"))
-            if r.returncode == 0:
-                t = open(os.path.join(d, "timing.txt")).read().split()
-                sec = float(t[1])
-                return dict(value=dofs / sec, unit="DOF-updates/s", cores=1, kind="reference",
-                            sample=sample + " (reference Fortran, flang -O3, time loop only)")
-    orc.build()
-    o = orc.Oracle(mesh, N, dt=1e-4, artiv=artiv, gam=w["gam"])
-    o.init_ot()
-    t0 = time.perf_counter()
-    o.steps(steps)
-    sec = time.perf_counter() - t0
-    return dict(value=dofs / sec, unit="DOF-updates/s", cores=1, kind="port",
-                sample=sample + " (C oracle port)")
+    h = host_info()
+    return dict(value=dofs / res["run"], unit="DOF-updates/s", cores=P, kind="reference",
+                cpu_model=h["cpu_model"], nproc=h["nproc"], mpi_ranks=P,
+                value_without_diag_flux=dofs / res["runnd"], diag_flux_overhead=res["run"] / res["runnd"] - 1.0,
+                sample="%dx%d %s mesh (jitter %.2f, seed %d), N=%d, AV %s, %d SSP(5,3) steps, %d MPI ranks "
+                       "(strip partition), reference Fortran (flang -O3, unmodified sources), time loop only, "
+                       "including the diagnostic flux the reference loop evaluates every iteration "
+                       "(iterations.f90:35-36)" % (n, m, w["ic"], w["jitter"], w["seed"], N,
+                                                    "on" if artiv else "off", steps, P))
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="ot512", choices=sorted(WORKLOADS) + ["ensemble"])
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision form")
-    ap.add_argument("--tile", type=int, default=None,
-                    help="fixed tile shape of the mixed form (0: 256/128, 1: 128/64, 2: 512/256) instead of "
-                         "autotuning, e.g. for profiles without the tuner's launches")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="first run two kernels of known HBM bytes (a 256 MiB fill: 256 MiB written; an "
-                         "in-place add of two 256 MiB arrays: 512 MiB read, 256 MiB written) so that "
-                         "hardware byte counters collected over this run can be scaled")
-    ap.add_argument("--output", action="store_true",
-                    help="time the output path (restart dump, Tecplot file) at the workload instead of steps")
-    ap.add_argument("--nre", type=int, default=4, help="plot cells per cell edge of the Tecplot file (--output)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py runs on exactly one GPU: this build holds the whole mesh as one "
-              "partition on one MI355X (multi-GPU halo exchange is not built)", file=sys.stderr)
-        sys.exit(2)
-    if args.workload == "ensemble":
-        run_ensemble(args)
-        return
-    w = WORKLOADS[args.workload]
-    if args.output:
-        run_output(w, args)
-        return
-    if "t_end" in w:
-        run_crossings(w, args)
-        return
+def run_single(w, args):
+    """One instance (config 1, 2, 3 or 5): the FP64 drop-in, then the mixed
+    form from the same initial state."""
     if "steps" in w and args.steps == 200:
         args.steps = w["steps"]
-    av_supported = True
     if args.copy_calibration:
         import torch
         src = torch.ones(1 << 25, dtype=torch.float64, device="cuda")
@@ -378,13 +543,12 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     N = w["N"]
+    K = args.steps
 
     def run(precision, Q0=None):
-        mesh, topo, s, artiv = setup(w, precision, av_supported)
+        mesh, topo, s, artiv = setup(w, precision, True)
         if Q0 is None:
             Q0 = initial_state(w, mesh, s)
-        else:
-            s.set_state(Q0)
         dt = ics.cfl_dt(mesh, N, Q0, w["gam"], 0.2)
         s.close()
         s = Solver(mesh, N, dt, gam=w["gam"], artiv=artiv, precision=precision, topo=topo)
@@ -397,70 +561,116 @@ def main():
                 s.set_tile(args.tile)
                 tile = args.tile
         if args.warmup:
-            s.step(args.warmup)
+            s.step(args.warmup)         # captures the step graphs of every buffer rotation
             s.set_state(Q0)
-        # the timed steps carry HIP events around every stage and CT launch
-        # (chorus_time_steps), which give the per-launch kernel times
+        # the timed steps: chorus_step, one hipGraph launch per step (the product path)
         t0 = time.perf_counter()
-        tm = s.time_steps(args.steps)
+        s.step(K)
         wall = time.perf_counter() - t0
         Q = s.get_state()
         d = s.diag()
+        # a separate pass with HIP events around every stage and CT launch
+        # (chorus_time_steps, plain launches) gives the per-launch kernel times
+        tm = None
+        if not args.no_event_pass:
+            s.set_state(Q0)
+            tm = s.time_steps(min(K, 20))
         s.close()
         return dict(mesh=mesh, dt=dt, wall=wall, tm=tm, Q=Q, Q0=Q0, diag=d, artiv=artiv, tile=tile)
 
     r64 = run("fp64")
     mesh = r64["mesh"]
     ndof = mesh.ncell * N * N
-    value = ndof * 5 * args.steps / r64["wall"]
-    stage_s = r64["tm"]["ms_stage_avg"] * 1e-3
-    achieved = ndof * b_stage(N) / stage_s / 1e9
+    value = ndof * 5 * K / r64["wall"]
+    key = args.workload
     out = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "DOF-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": r64["wall"] * 1e3 / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
+        "metric": METRIC, "value": value, "unit": "DOF-updates/s", "n_gpus": 1, "steps": K,
+        "warmup": args.warmup, "ms_per_step": r64["wall"] * 1e3 / K, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {
             "workload": w["desc"] + (", AV on" if r64["artiv"] else ", AV off"),
-            "form": "FP64 drop-in",
+            "form": "FP64 drop-in (chorus_step: one hipGraph launch per step)",
             "ncell": mesh.ncell, "N": N, "dof": ndof, "dt": r64["dt"],
-            "av_requested": bool(w["artiv"]), "av_applied": bool(r64["artiv"]),
         },
-        "roofline": {
+        "divb": {"rms": r64["diag"]["divbglob"], "max": r64["diag"]["divbmax"]},
+    }
+    if r64["tm"]:
+        stage_s = r64["tm"]["ms_stage_avg"] * 1e-3
+        achieved = ndof * b_stage(N) / stage_s / 1e9
+        out["roofline"] = {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_STAGE_TRAFFIC.get((args.workload, "fp64")),
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("%s/fp64/stage" % key),
             "kernel": ("one stage: k_face_inv + k_grad + k_face_visc + k_stage_visc <%d,double>" % N)
                       if r64["artiv"] else "one stage: k_face_inv + k_stage_ideal <%d,double>" % N,
             "stage_kernel_ms": r64["tm"]["ms_stage_avg"], "ct_kernel_ms": r64["tm"]["ms_ct_avg"],
             "bytes_per_dof_stage": b_stage(N), "algorithmic_bytes_per_stage": ndof * b_stage(N),
-        },
-        "divb": {"rms": r64["diag"]["divbglob"], "max": r64["diag"]["divbmax"]},
-    }
+        }
     if not args.no_mixed:
         rm = run("mixed", Q0=r64["Q0"])
-        vm = ndof * 5 * args.steps / rm["wall"]
+        vm = ndof * 5 * K / rm["wall"]
         a, b = r64["Q"], rm["Q"]
         out["mixed"] = {
-            "value": vm, "speedup_vs_fp64": vm / value,
+            "value": vm, "speedup_vs_fp64": vm / value, "ms_per_step": rm["wall"] * 1e3 / K,
             "rel_l2_vs_fp64": float(np.linalg.norm(b - a) / np.linalg.norm(a)),
             "linf_vs_fp64": float(np.abs(b - a).max()),
-            "stage_kernel_ms": rm["tm"]["ms_stage_avg"],
-            "roofline_frac": ndof * b_stage(N) / (rm["tm"]["ms_stage_avg"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "traffic": MEASURED_STAGE_TRAFFIC.get((args.workload, "mixed")),
+            "traffic": measured_traffic("%s/mixed/stage" % key),
             "divb_max": rm["diag"]["divbmax"],
-            "tile_shape": ["256/128", "128/64", "512/256"][rm["tile"]],
+            "tile_shape": TILES[rm["tile"]],
         }
+        if rm["tm"]:
+            out["mixed"]["stage_kernel_ms"] = rm["tm"]["ms_stage_avg"]
+            out["mixed"]["roofline_frac"] = ndof * b_stage(N) / (rm["tm"]["ms_stage_avg"] * 1e-3) / 1e9 / HBM_PEAK_GBS
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w, r64["artiv"])
+        out["cpu_baseline"] = cpu_reference_mpi(w, r64["artiv"])
     print(json.dumps(out), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=6)
+    ap.add_argument("--workload", default="ensemble", choices=sorted(WORKLOADS) + ["ensemble"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-mixed", action="store_true", help="single-instance workloads: skip the mixed form")
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="ensemble: skip the FP64 drop-in run one instance at a time")
+    ap.add_argument("--no-event-pass", action="store_true",
+                    help="single-instance workloads: skip the per-launch event pass (e.g. under a profiler)")
+    ap.add_argument("--divb-report", action="store_true", help="ensemble: max |div B| of every instance")
+    ap.add_argument("--tile", type=int, default=None,
+                    help="fixed tile shape of the mixed form (0: 256/128, 1: 128/64, 2: 512/256) instead of "
+                         "autotuning, e.g. for profiles without the tuner's launches")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run two kernels of known HBM bytes (a 256 MiB fill: 256 MiB written; an "
+                         "in-place add of two 256 MiB arrays: 512 MiB read, 256 MiB written) so that "
+                         "hardware byte counters collected over this run can be scaled")
+    ap.add_argument("--output", action="store_true",
+                    help="time the output path (restart dump, Tecplot file) at the workload instead of steps")
+    ap.add_argument("--nre", type=int, default=4, help="plot cells per cell edge of the Tecplot file (--output)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on exactly one GPU: this build holds a whole mesh (or a whole ensemble) on one "
+              "MI355X; multi-GPU halo exchange is not built", file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "ensemble":
+        if args.copy_calibration:
+            import torch
+            src = torch.ones(1 << 25, dtype=torch.float64, device="cuda")
+            dst = torch.full_like(src, 2.0)
+            dst.add_(src)
+            torch.cuda.synchronize()
+            del src, dst
+        run_ensemble(args)
+        return
+    w = WORKLOADS[args.workload]
+    if args.output:
+        run_output(w, args)
+        return
+    if "t_end" in w:
+        run_crossings(w, args)
+        return
+    run_single(w, args)
 
 
 if __name__ == "__main__":

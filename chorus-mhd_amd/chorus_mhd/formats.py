@@ -109,6 +109,21 @@ def fmt_d20(x: float) -> str:
     return s
 
 
+def text_round_trip(a: np.ndarray) -> np.ndarray:
+    """The doubles the reference reads back from the 20-character fields of
+    ``fmt_d20`` (vectorised).  A field carries 18 - d fractional digits for a
+    value with d integer digits; when |x| >= 2**-4 the double's half-ulp
+    (>= 2**-57 * 10**(d-1)) exceeds half the field's decimal spacing, so the
+    text parses back to x itself.  Only smaller nonzero values go through the
+    formatted text."""
+    a = np.asarray(a, dtype=np.float64)
+    out = a.copy()
+    small = np.flatnonzero((np.abs(a) < 0.0625) & (a != 0.0))
+    for i in small:
+        out[i] = float(fmt_d20(a[i]))
+    return out
+
+
 @dataclass
 class Mesh:
     """Unstructured quad mesh in the reference's global numbering (1-based ids).
@@ -172,8 +187,8 @@ class Mesh:
 
     def reparsed(self) -> "Mesh":
         """The mesh as the reference reads it back from the files (text round trip)."""
-        xv = np.array([float(fmt_d20(v)) for v in self.xv])
-        yv = np.array([float(fmt_d20(v)) for v in self.yv])
+        xv = text_round_trip(self.xv)
+        yv = text_round_trip(self.yv)
         return Mesh(xv, yv, self.cells.copy(), self.bnd.copy(), list(self.bnd_type),
                     self.dxcycl, self.dycycl)
 

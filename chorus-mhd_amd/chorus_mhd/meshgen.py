@@ -15,7 +15,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from .formats import Mesh, fmt_d20
+from .formats import Mesh, text_round_trip
 
 
 def quad_grid(n: int, m: int, x0: float, x1: float, y0: float, y1: float,
@@ -51,8 +51,8 @@ def quad_grid(n: int, m: int, x0: float, x1: float, y0: float, y1: float,
     xv = xv.reshape(-1)
     yv = yv.reshape(-1)
     # text round trip: what readGRIDinfo.f90:186 reads back
-    xv = np.array([float(fmt_d20(v)) for v in xv])
-    yv = np.array([float(fmt_d20(v)) for v in yv])
+    xv = text_round_trip(xv)
+    yv = text_round_trip(yv)
 
     def vid(i, j):
         return 1 + i + j * (n + 1)

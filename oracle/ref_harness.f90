@@ -12,13 +12,20 @@
 !   mode "setup": topo.i32, geom.f64, ops.f64, q0.f64
 !   mode "resid": q0.f64, resid.f64 (+ av.f64 when ARTIV=1)
 !   mode "run"  : q0.f64, qend.f64, diag.txt, timing.txt
+!   mode "runnd": as "run" without the diagnostic flux and gradient that
+!                 iterations.f90:35-36 evaluate every iteration (used only
+!                 every 10th): timing of the loop without that work
 !   mode "plot" : as "run", then the reference's own Tecplot output chain
 !                 (tecplotter2dsetup, tecplotter2d, tecpost_parallel, the
 !                 calls of iterations.f90:82-86) writes ./TEC/TEC*.plt and the
 !                 chain's timing to plot_timing.txt; needs ARTIV=1
 !                 (tecplotter2d reads muAV) and an existing ./TEC
 !
-! usage:  ref_harness setup|resid|run     (reads QUAD.INP like the reference)
+! Under mpirun (NPROC > 1, METIS_CELL in QUAD.INP) only rank 0 writes files,
+! and the loop is timed between barriers: timing.txt holds the steps, the
+! seconds, NCELLGLOB and N.
+!
+! usage:  ref_harness setup|resid|run|runnd|plot   (reads QUAD.INP like the reference)
 program ref_harness
     use setup2d
     implicit none
@@ -52,9 +59,11 @@ program ref_harness
     call MAPCYCLOC
     call INIT_SETUP
 
-    open(newunit=fh, file='q0.f64', access='stream', form='unformatted', status='replace')
-    write(fh) Q
-    close(fh)
+    if (rank == 0) then
+        open(newunit=fh, file='q0.f64', access='stream', form='unformatted', status='replace')
+        write(fh) Q
+        close(fh)
+    end if
 
     if (trim(mode) == 'setup') then
         open(newunit=fh, file='topo.i32', access='stream', form='unformatted', status='replace')
@@ -85,9 +94,10 @@ program ref_harness
             close(fh)
         end if
 
-    else if (trim(mode) == 'run' .or. trim(mode) == 'plot') then
+    else if (trim(mode) == 'run' .or. trim(mode) == 'runnd' .or. trim(mode) == 'plot') then
         nsteps = MAXITER - iter
-        open(newunit=fh, file='diag.txt', status='replace')
+        if (rank == 0) open(newunit=fh, file='diag.txt', status='replace')
+        call MPI_BARRIER(MPI_COMM_WORLD, ierror)
         t0 = MPI_WTIME()
         do while (iter < MAXITER)
             ! loop body of iterations.f90:20-57
@@ -100,9 +110,11 @@ program ref_harness
                 stop 'k_stage must be 3 or 5'
             end if
             call correct_B_via_curl_Az
-            call comp_inviscid_flux
-            call nablaQsp(1,9)
-            if (mod(iter,10) == 1) then
+            if (trim(mode) /= 'runnd') then
+                call comp_inviscid_flux
+                call nablaQsp(1,9)
+            end if
+            if (mod(iter,10) == 1 .and. trim(mode) /= 'runnd' .and. rank == 0) then
                 residnorm = 0.d0
                 divB = 0.d0
                 do ic = 1,NCELL
@@ -120,14 +132,17 @@ program ref_harness
                 write(fh, '(I8,3ES26.17E3)') iter, ctime, residglob, divBglob
             end if
         end do
+        call MPI_BARRIER(MPI_COMM_WORLD, ierror)
         t1 = MPI_WTIME()
-        close(fh)
-        open(newunit=fh, file='qend.f64', access='stream', form='unformatted', status='replace')
-        write(fh) Q
-        close(fh)
-        open(newunit=fh, file='timing.txt', status='replace')
-        write(fh, '(I8,ES26.17E3,I12,I4)') nsteps, t1 - t0, NCELL, N
-        close(fh)
+        if (rank == 0) then
+            close(fh)
+            open(newunit=fh, file='qend.f64', access='stream', form='unformatted', status='replace')
+            write(fh) Q
+            close(fh)
+            open(newunit=fh, file='timing.txt', status='replace')
+            write(fh, '(I8,ES26.17E3,I12,I4)') nsteps, t1 - t0, NCELLGLOB, N
+            close(fh)
+        end if
         if (trim(mode) == 'plot') then
             t0 = MPI_WTIME()
             call tecplotter2dsetup
@@ -140,7 +155,7 @@ program ref_harness
             close(fh)
         end if
     else
-        stop 'mode must be setup, resid, run or plot'
+        stop 'mode must be setup, resid, run, runnd or plot'
     end if
 
     call MPI_FINALIZE(ierror)

@@ -168,3 +168,15 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(REPO, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=300)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_text_round_trip_vectorised():
+    """The vectorised round trip equals formatting every value and parsing it
+    back, over magnitudes from 1e-20 to 1e3 (both signs, and zero)."""
+    from chorus_mhd.formats import text_round_trip
+    rng = np.random.default_rng(2)
+    x = np.concatenate([rng.uniform(-1, 1, 4000) * 10.0 ** rng.integers(-20, 4, 4000),
+                        [0.0, -0.0, 0.0625, -0.0625, np.nextafter(0.0625, 0), 2 * math.pi, 1.0 - 1e-17]])
+    want = np.array([float(fmt_d20(v)) for v in x])
+    got = text_round_trip(x)
+    assert np.array_equal(got.view(np.int64), want.view(np.int64))
