@@ -31,6 +31,11 @@
 
 #include "kernels_visc.hpp"
 
+// prototype: the derivative phase of k_grad_mx on the matrix cores
+#ifndef CHORUS_GRAD_MFMA
+#define CHORUS_GRAD_MFMA 0
+#endif
+typedef float v4f __attribute__((ext_vector_type(4)));
 #ifndef CHORUS_MX_CUNROLL
 #define CHORUS_MX_CUNROLL 2
 #endif
@@ -546,6 +551,14 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
     const int pc = pt / NP, pk = pt % NP, pjs = pk / N, pis = pk % N;
     const bool pact = (pt < CPB * NP) && (pc < ncl);
     float gp[8];
+#if CHORUS_GRAD_MFMA
+    // 1/J of every point of the block, for the MFMA derivative phase
+    __shared__ float sIJ[CPB * NP];
+    for (int i = tid; i < ncl * NP; i += T) {
+        const int k = i % NP;
+        sIJ[i] = float(1.0 / jacobian<N>(sX + (i / NP) * 8, k % N, k / N, sOps + Ops64Layout<N>::XS));
+    }
+#endif
 #pragma unroll 1
     for (int fam = 0; fam < 2; fam++) {
         for (int t = tid; t < TS::NI + TS::NFC; t += T) {
@@ -587,6 +600,44 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
             q[9 * NF] = float(s2d);
         }
         __syncthreads();
+#if CHORUS_GRAD_MFMA
+        // derivative phase on the matrix cores: per (cell, line) one 16x16 tile
+        // D[pos][col] = sum_r M(r,pos) X[r][col], col = (metric row grp, variable v),
+        // X = q_v(r) s_grp(r), with v_mfma_f32_16x16x4f32 over k-blocks of 4 flux
+        // points (rows pos >= N and flux points r > N are zero operands)
+        {
+            constexpr int KB = (N + 4) / 4;
+            const int lane = tid & 63, wv = tid >> 6, kq = lane >> 4, col = lane & 15;
+            const int cgrp = col >> 3, cv = col & 7;
+            float am[KB];
+#pragma unroll
+            for (int kb = 0; kb < KB; kb++) {
+                const int r = kb * 4 + kq;
+                am[kb] = (col < N && r <= N) ? sMf[r * N + col] : 0.f;
+            }
+            for (int tile = wv; tile < ncl * N; tile += T / 64) {
+                const int c = tile / N, line = tile % N;
+                const float *q = sV + c * VST + line * (N + 1);
+                v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int kb = 0; kb < KB; kb++) {
+                    const int r = kb * 4 + kq;
+                    const float b = r <= N ? q[cv * NF + r] * q[(8 + cgrp) * NF + r] : 0.f;
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(am[kb], b, acc, 0, 0, 0);
+                }
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int pos = 4 * kq + j;
+                    if (pos < N) {
+                        const int k = fam == 0 ? line * N + pos : pos * N + line;
+                        float *g = sG + (c * 16 + cgrp * 8 + cv) * NP + k;
+                        *g = fam == 0 ? acc[j] : (*g + acc[j]) * sIJ[c * NP + k];
+                    }
+                }
+            }
+        }
+        (void)gp;
+#else
         if (pact) {
             const int line = fam == 0 ? pjs : pis, pos = fam == 0 ? pis : pjs;
             const float *q = sV + pc * VST + line * (N + 1);
@@ -609,6 +660,7 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
                 for (int v = 0; v < 8; v++) sG[(pc * 16 + grp * 8 + v) * NP + pk] = (gp[v] + acc[v]) * iJ;
             }
         }
+#endif
         __syncthreads();
     }
     for (int i = tid; i < ncl * 16 * NP; i += T) a.gsol_r<float>()[(size_t)c0 * 16 * NP + i] = sG[i];
