@@ -39,9 +39,17 @@ class ChorusParams(C.Structure):
                 ("ncellglob", C.c_int), ("precision", C.c_int)]
 
 
+class ChorusBC(C.Structure):
+    _fields_ = [("ninlet", C.c_int), ("ibfinl", _ip), ("noutlet", C.c_int), ("ibfout", _ip),
+                ("nsymp", C.c_int), ("ibfsymp", _ip), ("nwall", C.c_int), ("ibfwal", _ip),
+                ("qin", C.c_double * 9)]
+
+
 # exported symbols and their signatures (include/chorus_mhd.h)
 SIGNATURES = {
     "chorus_init": (C.c_int, [C.POINTER(ChorusMesh), C.POINTER(ChorusParams), C.POINTER(C.c_void_p)]),
+    "chorus_init_bc": (C.c_int, [C.POINTER(ChorusMesh), C.POINTER(ChorusParams), C.POINTER(ChorusBC),
+                                 C.POINTER(C.c_void_p)]),
     "chorus_free": (None, [C.c_void_p]),
     "chorus_last_error": (C.c_char_p, [C.c_void_p]),
     "chorus_set_state": (C.c_int, [C.c_void_p, _dp]),

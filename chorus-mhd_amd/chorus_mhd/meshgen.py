@@ -25,7 +25,8 @@ def quad_grid(n: int, m: int, x0: float, x1: float, y0: float, y1: float,
     ``jitter``: interior vertices move by U(-jitter, jitter) * (hx, hy), drawn
     from ``numpy.random.default_rng(seed)`` (x offsets for all interior vertices
     in vertex order, then y offsets).  ``bc``: boundary type on all four sides
-    (``CYCL`` periodic or ``FREE`` zero-gradient outflow).
+    (``CYCL`` periodic, ``FREE``, ``INLE``, ``OUTL``, ``SYMP``, ``WALL``), or a
+    tuple of four: bottom, right, top, left (``CYCL`` on opposite sides only).
     Coordinates are rounded through the 20-character vertex-file field, so the
     returned mesh is exactly what the reference reads back.
     """
@@ -73,7 +74,10 @@ def quad_grid(n: int, m: int, x0: float, x1: float, y0: float, y1: float,
     for j in range(m - 1, -1, -1):          # left, top to bottom
         bnd.append((vid(0, j + 1), vid(0, j)))
     bnd = np.array(bnd, dtype=np.int32)
-    btype = [bc] * len(bnd)
+    sides = (bc,) * 4 if isinstance(bc, str) else tuple(bc)
+    if len(sides) != 4 or (sides[0] == "CYCL") != (sides[2] == "CYCL") or (sides[1] == "CYCL") != (sides[3] == "CYCL"):
+        raise ValueError("bc: one type or four (bottom, right, top, left), CYCL on both sides of a pair")
+    btype = [sides[0]] * n + [sides[1]] * m + [sides[2]] * n + [sides[3]] * m
     # DXCYCL/DYCYCL are the periodic shifts and also the AV length scale
     # h = sqrt(DXCYCL*DYCYCL/NCELLGLOB) (AV.f90:147), so they carry the domain
     # extents for FREE boundaries too.

@@ -60,6 +60,27 @@ def topology(mesh: Mesh) -> Topology:
                     cyc[:2 * npr].reshape(2, npr).T.copy(), ibfree[:nfree.value].copy())
 
 
+BC_KINDS = ("INLE", "OUTL", "SYMP", "WALL")
+
+
+def boundary_lists(mesh: Mesh, t: Topology) -> dict:
+    """IBFINL, IBFOUT, IBFSYMP, IBFWAL of CONNECT_BDRY (connect_bdry.f90:186-205):
+    the 1-based face numbers of the mesh's INLE/OUTL/SYMP/WALL boundary
+    entries, in boundary-file order.  A boundary face has no right cell in
+    IF2C; its vertices are those of the left cell's local face (face k joins
+    vertices k and k+1, connectivity.f90:73-83)."""
+    face_of = {}
+    for f in range(t.nface):
+        if t.if2c[f, 1] == 0:
+            lc, lf = int(t.if2c[f, 0]) - 1, int(t.if2c[f, 2])
+            face_of[frozenset((int(mesh.cells[lc, lf - 1]), int(mesh.cells[lc, lf % 4])))] = f + 1
+    out = {k: [] for k in BC_KINDS}
+    for ib, kind in enumerate(mesh.bnd_type):
+        if kind in out:
+            out[kind].append(face_of[frozenset((int(mesh.bnd[ib, 0]), int(mesh.bnd[ib, 1])))])
+    return {k: np.array(v, dtype=np.int32) for k, v in out.items()}
+
+
 PRECISION = {"fp64": _lib.CHORUS_FP64, "mixed": _lib.CHORUS_MIXED}
 
 
@@ -106,7 +127,7 @@ class Solver:
 
     def __init__(self, mesh: Mesh, N: int, dt: float, *, gam: float = 5.0 / 3, artiv: int = 0,
                  k_stage: int = 3, mu: float = 0.0, eta0: float = 0.0, precision: str = "fp64",
-                 topo: Topology = None):
+                 topo: Topology = None, qin=None):
         L = _lib.load()
         self.mesh, self.N = mesh, N
         self.topo = topo if topo is not None else topology(mesh)
@@ -132,7 +153,17 @@ class Solver:
         p = _lib.ChorusParams(1, artiv, k_stage, mu, eta0, gam, dt, mesh.dxcycl, mesh.dycycl,
                               mesh.ncell, prec)
         self._ctx = C.c_void_p()
-        rc = L.chorus_init(C.byref(m), C.byref(p), C.byref(self._ctx))
+        if any(k in BC_KINDS for k in mesh.bnd_type):
+            # INLE / OUTL / SYMP / WALL faces (chorus_init_bc)
+            bl = boundary_lists(mesh, t)
+            self._arrs.update({"bc_" + k: v for k, v in bl.items()})
+            ip = _lib._ip
+            q = (C.c_double * 9)(*([0.0] * 9 if qin is None else [float(x) for x in qin]))
+            b = _lib.ChorusBC(len(bl["INLE"]), _lib.ptr(bl["INLE"], ip), len(bl["OUTL"]), _lib.ptr(bl["OUTL"], ip),
+                              len(bl["SYMP"]), _lib.ptr(bl["SYMP"], ip), len(bl["WALL"]), _lib.ptr(bl["WALL"], ip), q)
+            rc = L.chorus_init_bc(C.byref(m), C.byref(p), C.byref(b), C.byref(self._ctx))
+        else:
+            rc = L.chorus_init(C.byref(m), C.byref(p), C.byref(self._ctx))
         if rc != 0:
             msg = L.chorus_last_error(self._ctx).decode()
             L.chorus_free(self._ctx)

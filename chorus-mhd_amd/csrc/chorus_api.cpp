@@ -65,6 +65,7 @@ struct chorus_ctx {
     int stage_parity = 0;          // slot used by the next stage
     int last_parity = -1;          // slot written by the last stage
     double h_msh = 0.0;            // AV length scale (AV.f90:147)
+    double qin[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // INLE faces: the prescribed state (chorus_bc)
     int nblk = 0;
     int device = -1;               // the HIP device current at chorus_init
     double *scratch = nullptr;     // that device's diagnostics scratch array (chorus_diag_noalloc)
@@ -143,6 +144,7 @@ FaceArgs face_args(chorus_ctx *c) {
     f.eta0 = c->p.eta0;
     f.h_msh = c->h_msh;
     f.pi = std::atan(1.0) * 4.0;
+    for (int v = 0; v < 8; v++) f.qin[v] = c->qin[v];
     return f;
 }
 
@@ -462,7 +464,7 @@ int build_topology(int ncell, int nvert, const int *ivcell, const double *xv, co
 
 }  // namespace
 
-int init_ctx(chorus_ctx *c, const chorus_mesh *m, const chorus_params *p);
+int init_ctx(chorus_ctx *c, const chorus_mesh *m, const chorus_params *p, const chorus_bc *bc = nullptr);
 
 // ===========================================================================
 extern "C" {
@@ -493,17 +495,21 @@ int chorus_topology(int ncell, int nvert, const int *ivcell, const double *xv, c
 }
 
 int chorus_init(const chorus_mesh *m, const chorus_params *p, chorus_ctx **out) {
+    return chorus_init_bc(m, p, nullptr, out);
+}
+
+int chorus_init_bc(const chorus_mesh *m, const chorus_params *p, const chorus_bc *bc, chorus_ctx **out) {
     if (!out) return CHORUS_ERR_ARG;
     *out = nullptr;
     chorus_ctx *c = new chorus_ctx();
     *out = c;
-    return init_ctx(c, m, p);
+    return init_ctx(c, m, p, bc);
 }
 
 }  // extern "C"
 
 // set-up shared by chorus_init and the batched groups (c->ninst set before)
-int init_ctx(chorus_ctx *c, const chorus_mesh *m, const chorus_params *p) {
+int init_ctx(chorus_ctx *c, const chorus_mesh *m, const chorus_params *p, const chorus_bc *bc) {
     if (!m || !p) return fail(c, CHORUS_ERR_ARG, "null mesh or params");
     if (m->N < 2 || m->N > kMaxN) return fail(c, CHORUS_ERR_ARG, "N must be in 2..8");
     if (m->ncell <= 0) return fail(c, CHORUS_ERR_ARG, "empty mesh");
@@ -545,24 +551,46 @@ int init_ctx(chorus_ctx *c, const chorus_mesh *m, const chorus_params *p) {
         if (f1 < 0 || f1 >= m->nface || f2 < 0 || f2 >= m->nface || !link(IF(f1, 0) - 1, IF(f1, 2), IF(f2, 0) - 1, IF(f2, 2)))
             return fail(c, CHORUS_ERR_TOPOLOGY, "bad periodic pair " + std::to_string(k + 1));
     }
-    for (int k = 0; k < m->nfree; k++) {
-        int f = m->ibfree[k] - 1;
-        if (f < 0 || f >= m->nface) return fail(c, CHORUS_ERR_TOPOLOGY, "bad FREE face");
-        int l = IF(f, 0) - 1, lf = IF(f, 2);
-        nbr[(size_t)l * 4 + lf - 1] = make_int2(-1, 1 << 4);
+    // faces without a right cell: FREE, and the kinds of chorus_bc; every pass
+    // but the inviscid face flux treats them alike (own trace)
+    std::vector<int> bkind((size_t)m->ncell * 4, BK_ZEROGRAD);
+    auto boundary = [&](int nlist, const int *list, int kind, const char *what) -> int {
+        if (nlist > 0 && !list) return fail(c, CHORUS_ERR_ARG, std::string("null ") + what + " list");
+        for (int k = 0; k < nlist; k++) {
+            int f = list[k] - 1;
+            if (f < 0 || f >= m->nface) return fail(c, CHORUS_ERR_TOPOLOGY, std::string("bad ") + what + " face");
+            int l = IF(f, 0) - 1, lf = IF(f, 2);
+            if (l < 0 || l >= m->ncell || lf < 1 || lf > 4)
+                return fail(c, CHORUS_ERR_TOPOLOGY, std::string("bad ") + what + " face");
+            nbr[(size_t)l * 4 + lf - 1] = make_int2(-1, 1 << 4);
+            bkind[(size_t)l * 4 + lf - 1] = kind;
+        }
+        return 0;
+    };
+    int rcb = boundary(m->nfree, m->ibfree, BK_ZEROGRAD, "FREE");
+    if (!rcb && bc) {
+        rcb = boundary(bc->noutlet, bc->ibfout, BK_ZEROGRAD, "OUTL");
+        if (!rcb) rcb = boundary(bc->nsymp, bc->ibfsymp, BK_MIRROR, "SYMP");
+        if (!rcb) rcb = boundary(bc->nwall, bc->ibfwal, BK_MIRROR, "WALL");
+        if (!rcb) rcb = boundary(bc->ninlet, bc->ibfinl, BK_INLET, "INLE");
+        if (!rcb && bc->ninlet > 0 && !(bc->qin[0] > 0.0))
+            rcb = fail(c, CHORUS_ERR_ARG, "INLE faces need an inlet state with positive density (chorus_bc.qin)");
+        for (int v = 0; v < 8; v++) c->qin[v] = bc->qin[v];
     }
+    if (rcb) return rcb;
     for (size_t i = 0; i < nbr.size(); i++)
         if (nbr[i].x == -2)
             return fail(c, CHORUS_ERR_TOPOLOGY,
                         "cell " + std::to_string(i / 4 + 1) + " face " + std::to_string(i % 4 + 1) +
-                            " has no flux: only interior, CYCL and FREE faces carry fluxes (connect_bdry.f90:88-110)");
-    // face list for the face-ordered passes: each face once, from its left cell
+                            " has no flux: it is in none of IBFINTER, CYC2FPAIR, IBFREE and the chorus_bc lists");
+    // face list for the face-ordered passes: each face once, from its left cell;
+    // a face without a right cell carries its boundary kind in .w
     std::vector<int4> faces;
     faces.reserve((size_t)m->ncell * 2 + 16);
     for (int ic = 0; ic < m->ncell; ic++)
         for (int lfi = 0; lfi < 4; lfi++) {
             const int2 in = nbr[(size_t)ic * 4 + lfi];
-            if ((in.y >> 4) & 1) faces.push_back(make_int4(ic, lfi, -1, 0));
+            if ((in.y >> 4) & 1) faces.push_back(make_int4(ic, lfi, -1, bkind[(size_t)ic * 4 + lfi]));
             else if (!((in.y >> 3) & 1)) faces.push_back(make_int4(ic, lfi, in.x, (in.y & 7) - 1));
         }
     std::vector<double> xy((size_t)m->ncell * 8);

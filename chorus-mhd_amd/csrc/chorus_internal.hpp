@@ -94,6 +94,15 @@ struct FaceArgs {
     const double *hinst;     // batched instances: AV length scale per instance
     int artiv;
     double gam, mu, eta0, h_msh, pi;
+    double qin[8];           // INLE faces: the prescribed state (chorus_bc)
+};
+
+// Boundary kinds of the faces without a right cell (faces[f].z < 0), held in
+// faces[f].w: the right state of the inviscid flux at the face point
+enum BoundaryKind : int {
+    BK_ZEROGRAD = 0,   // FREE, OUTL: Q_R = Q_L
+    BK_MIRROR = 1,     // SYMP, WALL: normal momentum and normal B reversed
+    BK_INLET = 2,      // INLE: Q_R(1:8) = qin, A_z from the cell
 };
 
 // viscous / AV path (k_grad + k_stage_visc)

@@ -395,6 +395,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         R Ql[NV], Qr[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) { Ql[v] = R(sT[tid * RW + v]); Qr[v] = R(sT[tid * RW + NV + v]); }
+        if (free_face && fc.w != BK_ZEROGRAD) {
+            // INLE / SYMP / WALL: the right state of the boundary kind
+            double qr[NV];
+            boundary_state(fc.w, sT + tid * RW, s1, s2, a.qin, qr);
+#pragma unroll
+            for (int v = 0; v < NV; v++) Qr[v] = R(qr[v]);
+        }
         const int sl = side_sign(fc.y);
         R eig;
         rusanov<R>(R(a.gam), Ql, Qr, R(s1), R(s2), sl, Fn, az, eig);

@@ -305,15 +305,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         const int gp = p0 + tid, k = gp % N;
         const int4 fc = sF[gp / N - f0];
         free_face = fc.z < 0;
-        double ql[NV];
-        float d[NV];
-#pragma unroll
-        for (int v = 0; v < NV; v++) {
-            ql[v] = sT[tid * RW + v];
-            d[v] = float(sT[tid * RW + NV + v] - ql[v]);
-        }
-        Ref s;
-        ref_from(ql, a.gam, s);
         int ifp, jfp;
         face2fp<N>(fc.y + 1, k, ifp, jfp);
         double xy[8];
@@ -322,6 +313,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         double m1, m2;
         metric_row<N>(xy, lf_family(fc.y + 1), ifp, jfp, m1, m2);
         const float s1 = float(m1), s2 = float(m2);
+        // the right state: the other cell's trace; at an INLE / SYMP / WALL face
+        // that of the boundary kind; at a FREE / OUTL face the own trace
+        double ql[NV], qr[NV];
+        float d[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            ql[v] = sT[tid * RW + v];
+            qr[v] = sT[tid * RW + NV + v];
+        }
+        if (free_face && fc.w != BK_ZEROGRAD) boundary_state(fc.w, ql, m1, m2, a.qin, qr);
+#pragma unroll
+        for (int v = 0; v < NV; v++) d[v] = float(qr[v] - ql[v]);
+        Ref s;
+        ref_from(ql, a.gam, s);
         const float gam = float(a.gam);
         float dF[8], dG[8];
         dflux(s, d, gam - 1.f, dF, dG);
@@ -337,6 +342,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         if (free_face) {
 #pragma unroll
             for (int v = 0; v < RW; v++) out[v] = 0.f;
+            if (fc.w != BK_ZEROGRAD) {
+                // the left cell's jump term against the boundary state
+#pragma unroll
+                for (int v = 0; v < 8; v++) out[v] = 0.5f * (dF[v] * s1 + dG[v] * s2) - dis * d[v];
+            }
+            // A_z of the boundary state is the cell's own: the upwind value is too
             a.FA[rl] = ql[8];
         } else {
             const float slr = sl * float(-side_sign(fc.w));

@@ -104,6 +104,28 @@ __device__ __forceinline__ void rusanov(R gam, const R *Qfl, const R *Qfr, R nx,
     Az = (Vn_l > R(0)) ? Qfl[8] : Qfr[8];
 }
 
+// Right state of the inviscid flux at a boundary face point of kind `kind`
+// (BoundaryKind) from the cell's own trace ql and the metric row (s1, s2) of
+// the face point (any orientation: the mirror is symmetric in n).  FP64.
+__device__ __forceinline__ void boundary_state(int kind, const double *ql, double s1, double s2, const double *qin,
+                                               double *qr) {
+#pragma unroll
+    for (int v = 0; v < 9; v++) qr[v] = ql[v];
+    if (kind == 1) {
+        const double mag = sqrt(s1 * s1 + s2 * s2);
+        const double nx = s1 / mag, ny = s2 / mag;
+        const double mn = ql[1] * nx + ql[2] * ny;
+        const double bn = ql[5] * nx + ql[6] * ny;
+        qr[1] = ql[1] - 2.0 * mn * nx;
+        qr[2] = ql[2] - 2.0 * mn * ny;
+        qr[5] = ql[5] - 2.0 * bn * nx;
+        qr[6] = ql[6] - 2.0 * bn * ny;
+    } else if (kind == 2) {
+#pragma unroll
+        for (int v = 0; v < 8; v++) qr[v] = qin[v];
+    }
+}
+
 // getVfluxvectors (getrusanov.f90:157-249) with viscosity m and resistivity e.
 // getVfluxvectors_AV (getrusanov.f90:251-361) is the same body with m = mu_art
 // and e = 0.  nQ[d*9+v] = nablaQ(v,d).  Entries 6..9 of the outputs are the

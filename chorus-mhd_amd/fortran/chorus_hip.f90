@@ -40,11 +40,25 @@ module chorus_hip
         integer(c_int) :: ncellglob, precision
     end type
 
+    ! IBFINL, IBFOUT, IBFSYMP, IBFWAL (connect_bdry.f90:186-205) and the inlet state
+    type, bind(c) :: chorus_bc_t
+        integer(c_int) :: ninlet
+        type(c_ptr) :: ibfinl
+        integer(c_int) :: noutlet
+        type(c_ptr) :: ibfout
+        integer(c_int) :: nsymp
+        type(c_ptr) :: ibfsymp
+        integer(c_int) :: nwall
+        type(c_ptr) :: ibfwal
+        real(c_double) :: qin(9)
+    end type
+
     interface
-        integer(c_int) function c_chorus_init(mesh, params, ctx) bind(c, name='chorus_init')
-            import :: c_int, c_ptr, chorus_mesh_t, chorus_params_t
+        integer(c_int) function c_chorus_init_bc(mesh, params, bc, ctx) bind(c, name='chorus_init_bc')
+            import :: c_int, c_ptr, chorus_mesh_t, chorus_params_t, chorus_bc_t
             type(chorus_mesh_t), intent(in) :: mesh
             type(chorus_params_t), intent(in) :: params
+            type(chorus_bc_t), intent(in) :: bc
             type(c_ptr), intent(out) :: ctx
         end function
         subroutine c_chorus_free(ctx) bind(c, name='chorus_free')
@@ -116,6 +130,7 @@ module chorus_hip
     ! host copies handed to chorus_init (c_loc needs TARGET arrays)
     real(c_double), allocatable, target, save :: h_xv(:), h_yv(:)
     integer(c_int), allocatable, target, save :: h_iv(:,:), h_if2c(:,:), h_inter(:), h_cyc(:,:), h_free(:)
+    integer(c_int), allocatable, target, save :: h_inl(:), h_out(:), h_symp(:), h_wal(:)
 
 contains
 
@@ -132,15 +147,20 @@ contains
         end if
     end subroutine
 
-    ! INIT_SETUP done: hand setup2d's mesh, connectivity, parameters and Q to the GPU
-    subroutine chorus_gpu_setup(gam, precision)
+    ! INIT_SETUP done: hand setup2d's mesh, connectivity, boundary lists,
+    ! parameters and Q to the GPU.  qin: the state of INLE faces (the reference
+    ! reads none; needed only when the mesh has INLE faces)
+    subroutine chorus_gpu_setup(gam, precision, qin)
         use setup2d, only: NCELL, NVERT, N, XV, YV, IVCELL, NFACE, IF2C, NINTER, IBFINTER, NPAIR, &
                            CYC2FPAIR, NFREE, IBFREE, vismode, ARTIV, k_stage, mu, eta0, dt, DXCYCL, &
-                           DYCYCL, NCELLGLOB, Q
+                           DYCYCL, NCELLGLOB, Q, NINLET, NOUTLET, NSYMP, NWALL, IBFINL, IBFOUT, &
+                           IBFSYMP, IBFWAL
         real(c_double), intent(in) :: gam
         integer(c_int), intent(in) :: precision
+        real(c_double), intent(in), optional :: qin(9)
         type(chorus_mesh_t) :: m
         type(chorus_params_t) :: p
+        type(chorus_bc_t) :: b
         h_xv = XV(1:NVERT)
         h_yv = YV(1:NVERT)
         h_iv = IVCELL(1:NCELL, 1:4)
@@ -158,7 +178,18 @@ contains
         p%vismode = vismode; p%artiv = ARTIV; p%k_stage = k_stage
         p%mu = mu; p%eta0 = eta0; p%gam = gam; p%dt = dt
         p%dxcycl = DXCYCL; p%dycycl = DYCYCL; p%ncellglob = NCELLGLOB; p%precision = precision
-        call check(c_chorus_init(m, p, ctx), 'chorus_init')
+        allocate(h_inl(max(NINLET, 1)), h_out(max(NOUTLET, 1)), h_symp(max(NSYMP, 1)), h_wal(max(NWALL, 1)))
+        if (NINLET > 0) h_inl(1:NINLET) = IBFINL(1:NINLET)
+        if (NOUTLET > 0) h_out(1:NOUTLET) = IBFOUT(1:NOUTLET)
+        if (NSYMP > 0) h_symp(1:NSYMP) = IBFSYMP(1:NSYMP)
+        if (NWALL > 0) h_wal(1:NWALL) = IBFWAL(1:NWALL)
+        b%ninlet = NINLET; b%ibfinl = c_loc(h_inl)
+        b%noutlet = NOUTLET; b%ibfout = c_loc(h_out)
+        b%nsymp = NSYMP; b%ibfsymp = c_loc(h_symp)
+        b%nwall = NWALL; b%ibfwal = c_loc(h_wal)
+        b%qin = 0.0_c_double
+        if (present(qin)) b%qin = qin
+        call check(c_chorus_init_bc(m, p, b, ctx), 'chorus_init_bc')
         call check(c_chorus_set_state(ctx, Q), 'chorus_set_state')
         h_mesh = m
     end subroutine

@@ -72,6 +72,35 @@ int chorus_init(const chorus_mesh *mesh, const chorus_params *params, chorus_ctx
 void chorus_free(chorus_ctx *ctx);
 const char *chorus_last_error(const chorus_ctx *ctx);
 
+/* Boundary faces of the four kinds the reference lists (IBFINL, IBFOUT,
+ * IBFSYMP, IBFWAL: connect_bdry.f90:88-110, 186-205) but never gives a flux
+ * (compflux.f90:33-49 treats only interior, CYCL and FREE faces).  This build
+ * states their physics as the right state of the inviscid Rusanov flux
+ * (getrusanov.f90:1-109) at the face point, formed from the cell's own trace
+ * Q_L and the face's unit normal n:
+ *   OUTL        zero gradient: Q_R = Q_L (as FREE, bcflux.f90:46)
+ *   SYMP, WALL  mirror (inviscid slip wall / symmetry plane): Q_R = Q_L with
+ *               the normal momentum and the normal magnetic field reversed,
+ *               m_R = m_L - 2 (m_L.n) n, B_R = B_L - 2 (B_L.n) n, so the
+ *               normal mass and energy fluxes vanish
+ *   INLE        prescribed state: Q_R(1:8) = qin(1:8), A_z from the cell
+ * Everything else at these faces is the FREE treatment (bcflux.f90:73-121,
+ * AV.f90:199-303): the central flux-point values and gradients of the viscous
+ * path, the AV viscosity and the A_z of the CT correction take the cell's own
+ * trace.  Face numbers are 1-based, as in IF2C. */
+typedef struct {
+    int ninlet;  const int *ibfinl;   /* IBFINL(NINLET) */
+    int noutlet; const int *ibfout;   /* IBFOUT(NOUTLET) */
+    int nsymp;   const int *ibfsymp;  /* IBFSYMP(NSYMP) */
+    int nwall;   const int *ibfwal;   /* IBFWAL(NWALL) */
+    double qin[9];                    /* inlet state (conserved variables, Q(1:9) order; qin[8] unused) */
+} chorus_bc;
+
+/* chorus_init for meshes with INLE/OUTL/SYMP/WALL faces (bc may be NULL:
+ * then it is chorus_init).  Single contexts only: the batched API takes
+ * interior, CYCL and FREE faces. */
+int chorus_init_bc(const chorus_mesh *mesh, const chorus_params *params, const chorus_bc *bc, chorus_ctx **out);
+
 /* Q(9,N,N,NCELL) in the reference layout (setup2d.f90:54).  Upload/download. */
 int chorus_set_state(chorus_ctx *ctx, const double *q);
 int chorus_get_state(chorus_ctx *ctx, double *q);

@@ -26,6 +26,9 @@ struct orc_ctx {
     int N, ncell, nvert, nface, ninter, npair, nfree;
     double gam, lambda, prandt, pi;
     int *ivcell, *if2c, *ibfinter, *cyc2fpair, *ibfree;
+    /* every face without a right cell (IBFREE, IBFOUT, IBFSYMP, IBFWAL, IBFINL)
+     * and its kind: 0 zero gradient, 1 mirror, 2 prescribed inlet state */
+    int nbf, *ibf, *bkind;
     double *xv, *yv;
     double *Xs, *Xf, *Lmat, *Mmat, *Xs_low, *Xf_low;
     double *dmdxs, *dmdxf1, *dmdxf2;
@@ -160,6 +163,8 @@ int orc_topology(int ncell, int nvert, const int *ivcell, const double *xv, cons
     int *ibfcyc = malloc(sizeof(int) * ((size_t)nbnd + 1));
     int *ibfree = malloc(sizeof(int) * ((size_t)nbnd + 1));
     int ncycl = 0, nfree = 0;
+    int *lists[4], nlist[4] = {0, 0, 0, 0};   /* INLE, OUTL, SYMP, WALL */
+    for (int k = 0; k < 4; k++) lists[k] = malloc(sizeof(int) * ((size_t)nbnd + 1));
     for (int ib = 0; ib < nbnd; ib++) {
         int ivb1 = bnd[ib], ivb3 = bnd[nbnd + ib];
         int icc = -1;
@@ -176,7 +181,12 @@ int orc_topology(int ncell, int nvert, const int *ivcell, const double *xv, cons
         int k = ic2f[icc * 4 + ifa];
         if (btype[ib] == 1) ibfcyc[ncycl++] = k;
         else if (btype[ib] == 2) ibfree[nfree++] = k;
+        else if (btype[ib] >= 3 && btype[ib] <= 6) lists[btype[ib] - 3][nlist[btype[ib] - 3]++] = k;
     }
+    out->ninl = nlist[0]; out->ibfinl = lists[0];
+    out->nout = nlist[1]; out->ibfout = lists[1];
+    out->nsymp = nlist[2]; out->ibfsymp = lists[2];
+    out->nwal = nlist[3]; out->ibfwal = lists[3];
     int ninter = 0;
     for (int f = 0; f < nface; f++) if (if2c[nface + f] != 0) ninter++;
     int *ibfinter = malloc(sizeof(int) * ((size_t)ninter + 1));
@@ -199,7 +209,11 @@ int orc_topology(int ncell, int nvert, const int *ivcell, const double *xv, cons
             if ((fabs(xfin - xfou) < tolCYC && epsY < tolCYC) ||
                 (fabs(yfin - yfou) < tolCYC && epsX < tolCYC)) hit = 1;
         }
-        if (!hit) { free(ibfcycloc); free(ibfinter); free(if2c); free(if2v); free(ibfcyc); free(ibfree); goto fail; }
+        if (!hit) {
+            free(ibfcycloc); free(ibfinter); free(if2c); free(if2v); free(ibfcyc); free(ibfree);
+            for (int k = 0; k < 4; k++) free(lists[k]);
+            goto fail;
+        }
         ibfcycloc[ncycloc++] = ibfcyc[ib];
     }
     /* MAPCYCLOC (map_cyc_loc.f90:12-40) */
@@ -227,6 +241,7 @@ int orc_topology(int ncell, int nvert, const int *ivcell, const double *xv, cons
     }
     if (npair != ncycloc / 2) {
         free(pair); free(ibfcycloc); free(ibfinter); free(if2c); free(if2v); free(ibfcyc); free(ibfree);
+        for (int k = 0; k < 4; k++) free(lists[k]);
         goto fail;
     }
     out->nface = nface; out->ninter = ninter; out->npair = npair; out->nfree = nfree;
@@ -248,6 +263,7 @@ fail:
 void orc_topo_free(orc_topo *t)
 {
     free(t->if2c); free(t->if2v); free(t->ibfinter); free(t->cyc2fpair); free(t->ibfree);
+    free(t->ibfinl); free(t->ibfout); free(t->ibfsymp); free(t->ibfwal);
     memset(t, 0, sizeof(*t));
 }
 
@@ -450,6 +466,21 @@ orc_ctx *orc_create(const orc_params *p, int ncell, int nvert, const int *ivcell
     memcpy(c->cyc2fpair, t->cyc2fpair, sizeof(int) * (size_t)t->npair * 2);
     c->ibfree = malloc(sizeof(int) * ((size_t)t->nfree + 1));
     memcpy(c->ibfree, t->ibfree, sizeof(int) * (size_t)t->nfree);
+    {
+        const int *src[5] = {t->ibfree, t->ibfout, t->ibfsymp, t->ibfwal, t->ibfinl};
+        const int cnt[5] = {t->nfree, t->nout, t->nsymp, t->nwal, t->ninl};
+        const int kind[5] = {0, 0, 1, 1, 2};
+        int n = 0;
+        for (int k = 0; k < 5; k++) n += cnt[k];
+        c->ibf = malloc(sizeof(int) * ((size_t)n + 1));
+        c->bkind = malloc(sizeof(int) * ((size_t)n + 1));
+        c->nbf = 0;
+        for (int k = 0; k < 5; k++)
+            for (int i = 0; i < cnt[k]; i++) {
+                c->ibf[c->nbf] = src[k][i];
+                c->bkind[c->nbf++] = kind[k];
+            }
+    }
 
     int N = c->N;
     size_t nc = (size_t)ncell;
@@ -495,6 +526,7 @@ void orc_destroy(orc_ctx *c)
                     c->nablaAs, c->muAV, c->muAVfi, c->muAVfj, c->SM_IND};
     for (size_t i = 0; i < sizeof(ds) / sizeof(ds[0]); i++) free(ds[i]);
     free(c->ivcell); free(c->if2c); free(c->ibfinter); free(c->cyc2fpair); free(c->ibfree);
+    free(c->ibf); free(c->bkind);
     free(c);
 }
 
@@ -763,13 +795,37 @@ static void two_sided_face(orc_ctx *c, int icleft, int icright, int ifacelc, int
     }
 }
 
-/* BCFLUX (bcflux.f90:1-70): FREE face, right state = left state */
+/* Right state of a face without a right cell.  FREE (bcflux.f90:46) and OUTL:
+ * the left state.  The reference gives INLE/SYMP/WALL faces no flux
+ * (compflux.f90:33-49); this build states: SYMP and WALL mirror the normal
+ * momentum and the normal magnetic field about the face's unit normal, INLE
+ * takes the prescribed state for Q(1:8) and the cell's own A_z. */
+static void boundary_state(const orc_ctx *c, int kind, const double *Qfl, const double *normf, double *Qfr)
+{
+    for (int v = 0; v < NV; v++) Qfr[v] = Qfl[v];
+    if (kind == 1) {
+        double mag = sqrt(normf[0] * normf[0] + normf[1] * normf[1]);
+        double nx = normf[0] / mag, ny = normf[1] / mag;
+        double mn = Qfl[1] * nx + Qfl[2] * ny;
+        double bn = Qfl[5] * nx + Qfl[6] * ny;
+        Qfr[1] = Qfl[1] - 2.0 * mn * nx;
+        Qfr[2] = Qfl[2] - 2.0 * mn * ny;
+        Qfr[5] = Qfl[5] - 2.0 * bn * nx;
+        Qfr[6] = Qfl[6] - 2.0 * bn * ny;
+    } else if (kind == 2) {
+        for (int v = 0; v < 8; v++) Qfr[v] = c->p.qin[v];
+    }
+}
+
+/* BCFLUX (bcflux.f90:1-70) over every face without a right cell: the
+ * Rusanov flux against the face kind's right state; the central flux-point
+ * values stay the cell's own (Qfr = Qfl, bcflux.f90:46-52) for every kind */
 static void bcflux(orc_ctx *c)
 {
     int N = c->N;
     double Qfl[NV], Qfr[NV], Fnl[8], Fnr[8], normf[2], Az;
-    for (int ifn = 0; ifn < c->nfree; ifn++) {
-        int iface = c->ibfree[ifn] - 1;
+    for (int ifn = 0; ifn < c->nbf; ifn++) {
+        int iface = c->ibf[ifn] - 1;
         int icleft = IF2C(c, iface, 0) - 1, ifacelc = IF2C(c, iface, 2);
         int faml = ifacelc % 2 + 1;
         for (int nfp = 0; nfp < N; nfp++) {
@@ -785,8 +841,8 @@ static void bcflux(orc_ctx *c)
                 normf[0] = c->S2[IDX_S2(c, 1, 0, ifpl, jfpl, icleft)];
                 normf[1] = c->S2[IDX_S2(c, 1, 1, ifpl, jfpl, icleft)];
             }
-            for (int v = 0; v < NV; v++) Qfr[v] = Qfl[v];
-            if (c->p.vismode == 1) store_qv(c, faml, ifpl, jfpl, icleft, Qfl, Qfr);
+            if (c->p.vismode == 1) store_qv(c, faml, ifpl, jfpl, icleft, Qfl, Qfl);
+            boundary_state(c, c->bkind[ifn], Qfl, normf, Qfr);
             rusanov(c, Qfl, Qfr, Fnl, Fnr, &Az, normf, sign_l, sign_r);
             store_face(c, faml, ifpl, jfpl, icleft, Fnl, Az);
         }
@@ -951,13 +1007,13 @@ static void two_sided_visface(orc_ctx *c, int icleft, int icright, int ifacelc, 
     }
 }
 
-/* BCVISFLUX (bcflux.f90:73-121) */
+/* BCVISFLUX (bcflux.f90:73-121), over every face without a right cell */
 static void bcvisflux(orc_ctx *c)
 {
     int N = c->N;
     double l1[NV], l2[NV];
-    for (int ifn = 0; ifn < c->nfree; ifn++) {
-        int iface = c->ibfree[ifn] - 1;
+    for (int ifn = 0; ifn < c->nbf; ifn++) {
+        int iface = c->ibf[ifn] - 1;
         int icleft = IF2C(c, iface, 0) - 1, ifacelc = IF2C(c, iface, 2);
         int faml = ifacelc % 2 + 1;
         for (int nfp = 0; nfp < N; nfp++) {

@@ -22,8 +22,9 @@ typedef struct orc_ctx orc_ctx;
 /* Mesh topology, CONNECTIVITY + CONNECT_BDRY + MAPCYCLOC restated
  * (connectivity.f90:1-233, connect_bdry.f90:1-354, map_cyc_loc.f90:1-42).
  * Inputs (1-based ids): ivcell(NCELL,4) column-major, bnd(NBND,2) column-major
- * vertex pairs, btype[NBND] in {0 other,1 CYCL,2 FREE}.  Outputs are allocated
- * by the oracle and freed with orc_topo_free.  Returns 0 or a negative code. */
+ * vertex pairs, btype[NBND] in {0 other, 1 CYCL, 2 FREE, 3 INLE, 4 OUTL,
+ * 5 SYMP, 6 WALL}.  Outputs are allocated by the oracle and freed with
+ * orc_topo_free.  Returns 0 or a negative code. */
 typedef struct {
     int nface, ninter, npair, nfree;
     int *if2c;       /* IF2C(NFACE,4) */
@@ -31,6 +32,9 @@ typedef struct {
     int *ibfinter;   /* IBFINTER(NINTER) */
     int *cyc2fpair;  /* CYC2FPAIR(NPAIR,2) */
     int *ibfree;     /* IBFREE(NFREE) */
+    /* IBFINL, IBFOUT, IBFSYMP, IBFWAL in boundary-file order (connect_bdry.f90:186-205) */
+    int ninl, nout, nsymp, nwal;
+    int *ibfinl, *ibfout, *ibfsymp, *ibfwal;
 } orc_topo;
 
 int  orc_topology(int ncell, int nvert, const int *ivcell, const double *xv, const double *yv,
@@ -44,6 +48,7 @@ typedef struct {
     int artiv;        /* 0|1 */
     int k_stage;      /* 3 -> RK_S5P3, 5 -> FiveStageRk */
     double mu, eta0, gam, dt, dxcycl, dycycl;
+    double qin[9];    /* INLE faces: the prescribed state (the reference has none, see bcflux) */
 } orc_params;
 
 /* INIT_SETUP restated (initsetup.f90:1-165 minus I/O): operators, metrics,

@@ -29,13 +29,15 @@ _ip = C.POINTER(C.c_int)
 class _Topo(C.Structure):
     _fields_ = [("nface", C.c_int), ("ninter", C.c_int), ("npair", C.c_int), ("nfree", C.c_int),
                 ("if2c", _ip), ("if2v", _ip), ("ibfinter", _ip), ("cyc2fpair", _ip),
-                ("ibfree", _ip)]
+                ("ibfree", _ip), ("ninl", C.c_int), ("nout", C.c_int), ("nsymp", C.c_int),
+                ("nwal", C.c_int), ("ibfinl", _ip), ("ibfout", _ip), ("ibfsymp", _ip), ("ibfwal", _ip)]
 
 
 class _Params(C.Structure):
     _fields_ = [("N", C.c_int), ("vismode", C.c_int), ("artiv", C.c_int), ("k_stage", C.c_int),
                 ("mu", C.c_double), ("eta0", C.c_double), ("gam", C.c_double),
-                ("dt", C.c_double), ("dxcycl", C.c_double), ("dycycl", C.c_double)]
+                ("dt", C.c_double), ("dxcycl", C.c_double), ("dycycl", C.c_double),
+                ("qin", C.c_double * 9)]
 
 
 _lib = None
@@ -77,7 +79,7 @@ def _ptr(a, t):
     return a.ctypes.data_as(t)
 
 
-BTYPES = {"CYCL": 1, "FREE": 2}
+BTYPES = {"CYCL": 1, "FREE": 2, "INLE": 3, "OUTL": 4, "SYMP": 5, "WALL": 6}
 
 
 class Topology:
@@ -105,6 +107,11 @@ class Topology:
         self.cyc2fpair = (np.ctypeslib.as_array(t.cyc2fpair, (max(2 * t.npair, 1),))[:2 * t.npair]
                           .reshape(2, t.npair).T.copy())
         self.ibfree = np.ctypeslib.as_array(t.ibfree, (max(t.nfree, 1),))[:t.nfree].copy()
+        # IBFINL, IBFOUT, IBFSYMP, IBFWAL (connect_bdry.f90:186-205)
+        self.bc_lists = {}
+        for key, n, arr in (("INLE", t.ninl, t.ibfinl), ("OUTL", t.nout, t.ibfout),
+                            ("SYMP", t.nsymp, t.ibfsymp), ("WALL", t.nwal, t.ibfwal)):
+            self.bc_lists[key] = np.ctypeslib.as_array(arr, (max(n, 1),))[:n].copy()
 
     def __del__(self):
         try:
@@ -117,12 +124,13 @@ class Oracle:
     """One FP64 instance: INIT_SETUP, compresid, RK + CT steps (reference semantics)."""
 
     def __init__(self, mesh, N, dt=1e-3, artiv=0, k_stage=3, mu=0.0, eta0=0.0,
-                 gam=5.0 / 3, vismode=1, topo=None):
+                 gam=5.0 / 3, vismode=1, topo=None, qin=None):
         L = lib()
         self.mesh = mesh
         self.N = N
         self.topo = topo or Topology(mesh)
-        p = _Params(N, vismode, artiv, k_stage, mu, eta0, gam, dt, mesh.dxcycl, mesh.dycycl)
+        q = (C.c_double * 9)(*([0.0] * 9 if qin is None else [float(x) for x in qin]))
+        p = _Params(N, vismode, artiv, k_stage, mu, eta0, gam, dt, mesh.dxcycl, mesh.dycycl, q)
         iv = np.asfortranarray(mesh.cells, dtype=np.int32)
         self._keep = (iv,)
         self._c = L.orc_create(C.byref(p), mesh.ncell, mesh.nvert, _ptr(iv, _ip),
