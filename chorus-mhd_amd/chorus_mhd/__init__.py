@@ -6,7 +6,7 @@ FiveStageRk, correct_B_via_curl_Az) over the C-ABI library
 formats and a structured mesh generator.
 """
 from .formats import QuadInp, Mesh, read_mesh, write_restart, read_restart, fmt_d20
-from .meshgen import quad_grid
+from .meshgen import quad_grid, curved_grid
 
 __all__ = ["QuadInp", "Mesh", "read_mesh", "write_restart", "read_restart", "fmt_d20",
-           "quad_grid"]
+           "quad_grid", "curved_grid"]

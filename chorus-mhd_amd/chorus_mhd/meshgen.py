@@ -82,3 +82,40 @@ def quad_grid(n: int, m: int, x0: float, x1: float, y0: float, y1: float,
     # h = sqrt(DXCYCL*DYCYCL/NCELLGLOB) (AV.f90:147), so they carry the domain
     # extents for FREE boundaries too.
     return Mesh(xv, yv, cells, bnd, btype, x1 - x0, y1 - y0)
+
+
+def curved_grid(n: int, m: int, x0: float, x1: float, y0: float, y1: float, amp: float = 0.1,
+                bc="CYCL"):
+    """n x m 8-node (curved) cells on [x0,x1] x [y0,y1]: the uniform grid's
+    vertices and edge midpoints moved by the smooth map
+        (x, y) + amp (hx, hy) sin(2 pi (x-x0)/Lx) sin(2 pi (y-y0)/Ly),
+    which fixes the boundary, so boundary edges stay straight and periodic
+    partners keep matching midpoints.  Returns (mesh, xm, ym): the mesh of the
+    corner vertices (numbered as quad_grid) and the mid-edge nodes 5..8 of each
+    cell, (NCELL, 4) each, in the orientation of the reference's 8-node map
+    (node 5 on edge 1-2, 6 on 2-3, 7 on 3-4, 8 on 4-1; mapder.f90:106-115).
+    Shared edges get the same mid node from both cells."""
+    base = quad_grid(n, m, x0, x1, y0, y1, bc=bc)
+    hx, hy = (x1 - x0) / n, (y1 - y0) / m
+    lx, ly = x1 - x0, y1 - y0
+
+    def warp(x, y):
+        s = amp * np.sin(2 * np.pi * (x - x0) / lx) * np.sin(2 * np.pi * (y - y0) / ly)
+        return x + s * hx, y + s * hy
+
+    ii, jj = np.meshgrid(np.arange(n + 1), np.arange(m + 1), indexing="xy")
+    gx, gy = x0 + ii.reshape(-1) * hx, y0 + jj.reshape(-1) * hy
+    gx[(ii.reshape(-1) == n)] = x1
+    gy[(jj.reshape(-1) == m)] = y1
+    xv, yv = warp(gx, gy)
+    xv, yv = text_round_trip(xv), text_round_trip(yv)
+    ci, cj = np.meshgrid(np.arange(n), np.arange(m), indexing="xy")
+    ci, cj = ci.reshape(-1).astype(np.float64), cj.reshape(-1).astype(np.float64)
+    # grid positions of the mid-edge nodes: (i+1/2, j), (i+1, j+1/2), (i+1/2, j+1), (i, j+1/2)
+    mi = np.stack([ci + 0.5, ci + 1.0, ci + 0.5, ci], axis=1)
+    mj = np.stack([cj, cj + 0.5, cj + 1.0, cj + 0.5], axis=1)
+    xm, ym = warp(x0 + mi * hx, y0 + mj * hy)
+    xm = text_round_trip(xm.reshape(-1)).reshape(-1, 4)
+    ym = text_round_trip(ym.reshape(-1)).reshape(-1, 4)
+    mesh = Mesh(xv, yv, base.cells, base.bnd, list(base.bnd_type), base.dxcycl, base.dycycl)
+    return mesh, xm, ym

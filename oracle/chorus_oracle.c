@@ -38,6 +38,8 @@ struct orc_ctx {
     double *nablaQs, *nablaQvfi, *nablaQvfj, *Azfi, *Azfj, *nablaAs;
     double *muAV, *muAVfi, *muAVfj, *SM_IND;
     double eigvmax, eigvmax_g, epsilon0;
+    /* 8-node cells (orc_set_nodes8): mid-edge nodes 5..8 per cell, XM(NCELL,4) */
+    double *xm8, *ym8;
 };
 
 /* ---- layouts (0-based arguments) --------------------------------------- */
@@ -362,37 +364,106 @@ static void dmdx4(double psi, double eta, double d[4][2])
     d[3][0] = -eta;      d[3][1] = 1.0 - psi;
 }
 
-/* calcjacob (calcjacob.f90:1-86) + xyCoor_atGps (mapder.f90:255-273) */
+/* MAPDER_8_NODE (mapder.f90:117-253): derivatives d/dpsi, d/deta of the
+ * 8-node serendipity shape functions at (psi, eta); the reference's REAL
+ * literals (4., 2., 3., 1.) are exact in double precision */
+static void dmdx8(double psi, double eta, double d[8][2])
+{
+    d[0][0] = (1.0 - eta) * (4.0 * psi + 2.0 * eta - 3.0);
+    d[0][1] = (1.0 - psi) * (4.0 * eta + 2.0 * psi - 3.0);
+    d[1][0] = (1.0 - eta) * (4.0 * psi - 2.0 * eta - 1.0);
+    d[1][1] = psi * (4.0 * eta - 2.0 * psi - 1.0);
+    d[2][0] = eta * (4.0 * psi + 2.0 * eta - 3.0);
+    d[2][1] = psi * (4.0 * eta + 2.0 * psi - 3.0);
+    d[3][0] = eta * (4.0 * psi - 2.0 * eta - 1.0);
+    d[3][1] = (1.0 - psi) * (4.0 * eta - 2.0 * psi - 1.0);
+    d[4][0] = 4.0 * (1.0 - eta) * (1.0 - 2.0 * psi);
+    d[4][1] = 4.0 * psi * (psi - 1.0);
+    d[5][0] = 4.0 * eta * (1.0 - eta);
+    d[5][1] = 4.0 * psi * (1.0 - 2.0 * eta);
+    d[6][0] = 4.0 * eta * (1.0 - 2.0 * psi);
+    d[6][1] = 4.0 * psi * (1.0 - psi);
+    d[7][0] = 4.0 * eta * (eta - 1.0);
+    d[7][1] = 4.0 * (1.0 - psi) * (1.0 - 2.0 * eta);
+}
+
+/* MapBaseFunc8 (mapder.f90:92-104) */
+static void base8(double xi, double eta, double f[8])
+{
+    f[0] = (1.0 - xi) * (1.0 - eta) * (-2.0 * xi - 2.0 * eta + 1.0);
+    f[1] = xi * (1.0 - eta) * (2.0 * xi - 2.0 * eta - 1.0);
+    f[2] = xi * eta * (2.0 * xi + 2.0 * eta - 3.0);
+    f[3] = (1.0 - xi) * eta * (-2.0 * xi + 2.0 * eta - 1.0);
+    f[4] = 4.0 * (1.0 - eta) * (1.0 - xi) * xi;
+    f[5] = 4.0 * eta * (1.0 - eta) * xi;
+    f[6] = 4.0 * (1.0 - xi) * xi * eta;
+    f[7] = 4.0 * (1.0 - eta) * (1.0 - xi) * eta;
+}
+
+void orc_mapder8(int N, double *dmdxs, double *dmdxf1, double *dmdxf2)
+{
+    double pi = atan(1.0) * 4.0, Xs[16], Xf[17], d[8][2];
+    setsandpoints(N, pi, Xs, Xf);
+    for (int js = 0; js < N; js++)
+        for (int is = 0; is < N; is++) {
+            dmdx8(Xs[is], Xs[js], d);
+            for (int k = 0; k < 8; k++)
+                for (int a = 0; a < 2; a++) dmdxs[(((size_t)js * N + is) * 2 + a) * 8 + k] = d[k][a];
+        }
+    for (int jfp = 0; jfp < N; jfp++)
+        for (int ifp = 0; ifp <= N; ifp++) {
+            dmdx8(Xf[ifp], Xs[jfp], d);
+            for (int k = 0; k < 8; k++)
+                for (int a = 0; a < 2; a++) dmdxf1[(((size_t)jfp * (N + 1) + ifp) * 2 + a) * 8 + k] = d[k][a];
+        }
+    for (int jfp = 0; jfp <= N; jfp++)
+        for (int ifp = 0; ifp < N; ifp++) {
+            dmdx8(Xs[ifp], Xf[jfp], d);
+            for (int k = 0; k < 8; k++)
+                for (int a = 0; a < 2; a++) dmdxf2[(((size_t)jfp * N + ifp) * 2 + a) * 8 + k] = d[k][a];
+        }
+}
+
+/* map derivatives at (psi, eta) from the cell's nn nodes X, Y in calcjacob's
+ * order (calcjacob.f90:20-28); nn = 8 takes MAPDER_8_NODE's derivatives */
+static void map_derivs(int nn, const double *X, const double *Y, double psi, double eta, double *xpsi,
+                       double *ypsi, double *xeta, double *yeta)
+{
+    double d[8][2];
+    if (nn == 8) dmdx8(psi, eta, d);
+    else dmdx4(psi, eta, (double (*)[2])d);
+    *xpsi = 0.0; *xeta = 0.0; *ypsi = 0.0; *yeta = 0.0;
+    for (int k = 0; k < nn; k++) {
+        *xpsi = *xpsi + d[k][0] * X[k];
+        *ypsi = *ypsi + d[k][0] * Y[k];
+        *xeta = *xeta + d[k][1] * X[k];
+        *yeta = *yeta + d[k][1] * Y[k];
+    }
+}
+
+/* calcjacob (calcjacob.f90:1-86) + xyCoor_atGps (mapder.f90:255-273); with
+ * mid-edge nodes (orc_set_nodes8) the same sums run over the 8 nodes, which
+ * the reference prepares (MAPDER_8_NODE) but never reaches (initsetup.f90:71-78) */
 static void metrics(orc_ctx *c)
 {
-    int N = c->N;
+    int N = c->N, nn = c->xm8 ? 8 : 4;
     for (int ic = 0; ic < c->ncell; ic++) {
-        double X[4], Y[4];
+        double X[8], Y[8];
         for (int k = 0; k < 4; k++) { X[k] = c->xv[IVCELL(c, ic, k) - 1]; Y[k] = c->yv[IVCELL(c, ic, k) - 1]; }
+        if (nn == 8)
+            for (int k = 0; k < 4; k++) {
+                X[4 + k] = c->xm8[(size_t)k * c->ncell + ic];
+                Y[4 + k] = c->ym8[(size_t)k * c->ncell + ic];
+            }
+        double xpsi, ypsi, xeta, yeta;
         for (int js = 0; js < N; js++)
             for (int is = 0; is < N; is++) {
-                double d[4][2];
-                dmdx4(c->Xs[is], c->Xs[js], d);
-                double xpsi = 0.0, xeta = 0.0, ypsi = 0.0, yeta = 0.0;
-                for (int k = 0; k < 4; k++) {
-                    xpsi = xpsi + d[k][0] * X[k];
-                    ypsi = ypsi + d[k][0] * Y[k];
-                    xeta = xeta + d[k][1] * X[k];
-                    yeta = yeta + d[k][1] * Y[k];
-                }
+                map_derivs(nn, X, Y, c->Xs[is], c->Xs[js], &xpsi, &ypsi, &xeta, &yeta);
                 c->Jac[IDX_JAC(c, is, js, ic)] = xpsi * yeta - xeta * ypsi;
             }
         for (int jfp = 0; jfp < N; jfp++)
             for (int ifp = 0; ifp <= N; ifp++) {
-                double d[4][2];
-                dmdx4(c->Xf[ifp], c->Xs[jfp], d);
-                double xpsi = 0.0, xeta = 0.0, ypsi = 0.0, yeta = 0.0;
-                for (int k = 0; k < 4; k++) {
-                    xpsi = xpsi + d[k][0] * X[k];
-                    ypsi = ypsi + d[k][0] * Y[k];
-                    xeta = xeta + d[k][1] * X[k];
-                    yeta = yeta + d[k][1] * Y[k];
-                }
+                map_derivs(nn, X, Y, c->Xf[ifp], c->Xs[jfp], &xpsi, &ypsi, &xeta, &yeta);
                 c->S1[IDX_S1(c, 0, 0, ifp, jfp, ic)] = yeta;
                 c->S1[IDX_S1(c, 0, 1, ifp, jfp, ic)] = -xeta;
                 c->S1[IDX_S1(c, 1, 0, ifp, jfp, ic)] = -ypsi;
@@ -400,28 +471,21 @@ static void metrics(orc_ctx *c)
             }
         for (int jfp = 0; jfp <= N; jfp++)
             for (int ifp = 0; ifp < N; ifp++) {
-                double d[4][2];
-                dmdx4(c->Xs[ifp], c->Xf[jfp], d);
-                double xpsi = 0.0, xeta = 0.0, ypsi = 0.0, yeta = 0.0;
-                for (int k = 0; k < 4; k++) {
-                    xpsi = xpsi + d[k][0] * X[k];
-                    ypsi = ypsi + d[k][0] * Y[k];
-                    xeta = xeta + d[k][1] * X[k];
-                    yeta = yeta + d[k][1] * Y[k];
-                }
+                map_derivs(nn, X, Y, c->Xs[ifp], c->Xf[jfp], &xpsi, &ypsi, &xeta, &yeta);
                 c->S2[IDX_S2(c, 0, 0, ifp, jfp, ic)] = yeta;
                 c->S2[IDX_S2(c, 0, 1, ifp, jfp, ic)] = -xeta;
                 c->S2[IDX_S2(c, 1, 0, ifp, jfp, ic)] = -ypsi;
                 c->S2[IDX_S2(c, 1, 1, ifp, jfp, ic)] = xpsi;
             }
-        /* solution-point coordinates, MapBaseFunc (mapder.f90:79-87) */
+        /* solution-point coordinates, MapBaseFunc / MapBaseFunc8 (mapder.f90:79-104) */
         for (int js = 0; js < N; js++)
             for (int is = 0; is < N; is++) {
                 double xi = c->Xs[is], eta = c->Xs[js];
-                double fn[4] = {(1.0 - xi) * (1.0 - eta), (1.0 - eta) * xi, xi * eta, (1.0 - xi) * eta};
+                double fn[8] = {(1.0 - xi) * (1.0 - eta), (1.0 - eta) * xi, xi * eta, (1.0 - xi) * eta};
+                if (nn == 8) base8(xi, eta, fn);
                 double xx = 0.0, yy = 0.0;
-                for (int nb = 0; nb < 4; nb++) xx = xx + X[nb] * fn[nb];
-                for (int nb = 0; nb < 4; nb++) yy = yy + Y[nb] * fn[nb];
+                for (int nb = 0; nb < nn; nb++) xx = xx + X[nb] * fn[nb];
+                for (int nb = 0; nb < nn; nb++) yy = yy + Y[nb] * fn[nb];
                 c->XXsolu[(IDX_JAC(c, is, js, ic)) * 2 + 0] = xx;
                 c->XXsolu[(IDX_JAC(c, is, js, ic)) * 2 + 1] = yy;
             }
@@ -517,6 +581,16 @@ orc_ctx *orc_create(const orc_params *p, int ncell, int nvert, const int *ivcell
     return c;
 }
 
+int orc_set_nodes8(orc_ctx *c, const double *xm, const double *ym)
+{
+    if (!c || !xm || !ym) return -1;
+    if (!c->xm8) { c->xm8 = dalloc((size_t)c->ncell * 4); c->ym8 = dalloc((size_t)c->ncell * 4); }
+    memcpy(c->xm8, xm, sizeof(double) * (size_t)c->ncell * 4);
+    memcpy(c->ym8, ym, sizeof(double) * (size_t)c->ncell * 4);
+    metrics(c);
+    return 0;
+}
+
 void orc_destroy(orc_ctx *c)
 {
     if (!c) return;
@@ -527,6 +601,7 @@ void orc_destroy(orc_ctx *c)
     for (size_t i = 0; i < sizeof(ds) / sizeof(ds[0]); i++) free(ds[i]);
     free(c->ivcell); free(c->if2c); free(c->ibfinter); free(c->cyc2fpair); free(c->ibfree);
     free(c->ibf); free(c->bkind);
+    free(c->xm8); free(c->ym8);
     free(c);
 }
 

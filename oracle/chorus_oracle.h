@@ -57,6 +57,16 @@ orc_ctx *orc_create(const orc_params *p, int ncell, int nvert, const int *ivcell
                     const double *xv, const double *yv, const orc_topo *topo);
 void orc_destroy(orc_ctx *c);
 
+/* 8-node cells: the mid-edge nodes 5..8 of every cell (node 5 on edge 1-2, 6 on
+ * 2-3, 7 on 3-4, 8 on 4-1: the orientation of mapder.f90:106-115), XM(NCELL,4)
+ * and YM(NCELL,4) column-major; the metrics and solution-point coordinates are
+ * recomputed with the 8-node map (MAPDER_8_NODE, mapder.f90:117-253, summed as
+ * calcjacob.f90 sums the 4-node map).  Returns 0. */
+int orc_set_nodes8(orc_ctx *c, const double *xm, const double *ym);
+/* MAPDER_8_NODE's tables for N points: dmdxs(8,2,N,N), dmdxf1(8,2,N+1,N),
+ * dmdxf2(8,2,N,N+1), column-major */
+void orc_mapder8(int N, double *dmdxs, double *dmdxf1, double *dmdxf2);
+
 /* state Q(9,N,N,NCELL) */
 void orc_set_q(orc_ctx *c, const double *q);
 void orc_get_q(const orc_ctx *c, double *q);

@@ -71,6 +71,9 @@ def lib():
         L.orc_array.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_long)]
         L.orc_scalar.restype = C.c_double
         L.orc_scalar.argtypes = [C.c_void_p, C.c_char_p]
+        L.orc_set_nodes8.restype = C.c_int
+        L.orc_set_nodes8.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_mapder8.argtypes = [C.c_int, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -183,6 +186,20 @@ class Oracle:
 
     def scalar(self, name):
         return lib().orc_scalar(self._c, name.encode())
+
+    def set_nodes8(self, xm, ym):
+        """Mid-edge nodes 5..8 of every cell, (NCELL, 4) each: 8-node cells."""
+        self._x8 = np.asfortranarray(xm, dtype=np.float64).ravel(order="F")
+        self._y8 = np.asfortranarray(ym, dtype=np.float64).ravel(order="F")
+        if lib().orc_set_nodes8(self._c, _ptr(self._x8, _dp), _ptr(self._y8, _dp)) != 0:
+            raise RuntimeError("orc_set_nodes8 failed")
+
+
+def mapder8(N):
+    """MAPDER_8_NODE's tables (dmdxs(8,2,N,N), dmdxf1(8,2,N+1,N), dmdxf2(8,2,N,N+1)), flat in Fortran order."""
+    a, b, c = np.empty(16 * N * N), np.empty(16 * (N + 1) * N), np.empty(16 * (N + 1) * N)
+    lib().orc_mapder8(N, _ptr(a, _dp), _ptr(b, _dp), _ptr(c, _dp))
+    return a, b, c
 
 
 def run_reference(workdir: str, mode: str, timeout: float = 600.0, gam: float = 5.0 / 3) -> None:

@@ -15,6 +15,12 @@
 !   mode "runnd": as "run" without the diagnostic flux and gradient that
 !                 iterations.f90:35-36 evaluate every iteration (used only
 !                 every 10th): timing of the loop without that work
+!   mode "mapder8": mapder8.f64, the 8-node shape-function derivative tables
+!                 of MAPDER_8_NODE (mapder.f90:117-253) for the run's N:
+!                 dmdxs(8,2,N,N), dmdxf1(8,2,N+1,N), dmdxf2(8,2,N,N+1) (the
+!                 reference's INIT_SETUP stops right after computing them,
+!                 initsetup.f90:71-78, so they are taken here after a
+!                 CURVE_WALL=0 set-up)
 !   mode "plot" : as "run", then the reference's own Tecplot output chain
 !                 (tecplotter2dsetup, tecplotter2d, tecpost_parallel, the
 !                 calls of iterations.f90:82-86) writes ./TEC/TEC*.plt and the
@@ -25,7 +31,7 @@
 ! and the loop is timed between barriers: timing.txt holds the steps, the
 ! seconds, NCELLGLOB and N.
 !
-! usage:  ref_harness setup|resid|run|runnd|plot   (reads QUAD.INP like the reference)
+! usage:  ref_harness setup|resid|run|runnd|mapder8|plot   (reads QUAD.INP like the reference)
 program ref_harness
     use setup2d
     implicit none
@@ -80,6 +86,14 @@ program ref_harness
         close(fh)
         open(newunit=fh, file='ops.f64', access='stream', form='unformatted', status='replace')
         write(fh) Xs, Xf, Lmat, Mmat
+        close(fh)
+
+    else if (trim(mode) == 'mapder8') then
+        deallocate(dmdxs, dmdxf1, dmdxf2)
+        allocate(dmdxs(8,2,N,N), dmdxf1(8,2,N+1,N), dmdxf2(8,2,N,N+1))
+        call MAPDER_8_NODE
+        open(newunit=fh, file='mapder8.f64', access='stream', form='unformatted', status='replace')
+        write(fh) dmdxs, dmdxf1, dmdxf2
         close(fh)
 
     else if (trim(mode) == 'resid') then
@@ -155,7 +169,7 @@ program ref_harness
             close(fh)
         end if
     else
-        stop 'mode must be setup, resid, run, runnd or plot'
+        stop 'mode must be setup, resid, run, runnd, mapder8 or plot'
     end if
 
     call MPI_FINALIZE(ierror)

@@ -17,9 +17,10 @@ def pytest_configure(config):
 
 
 def golden_names():
-    """The solver fixtures (the Tecplot fixtures plot_* have their own tests)."""
+    """The solver fixtures (the Tecplot fixtures plot_* and the metric tables
+    mapder8_* have their own tests)."""
     return sorted(os.path.splitext(os.path.basename(p))[0] for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
-                  if not os.path.basename(p).startswith("plot_"))
+                  if not os.path.basename(p).startswith(("plot_", "mapder8_")))
 
 
 def load_golden(name):

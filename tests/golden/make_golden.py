@@ -223,6 +223,23 @@ def run_plot_case(name, spec, outdir):
         shutil.rmtree(work, ignore_errors=True)
 
 
+def run_mapder8(outdir, Ns=range(2, 9)):
+    """MAPDER_8_NODE's shape-function derivative tables (mapder.f90:117-253),
+    computed by the reference itself (harness mode mapder8) for N = 2..8."""
+    mesh = quad_grid(2, 2, 0.0, 1.0, 0.0, 1.0, bc="FREE")
+    out = {}
+    for N in Ns:
+        work = tempfile.mkdtemp()
+        try:
+            mesh.write(work)
+            QuadInp(N=N, dt=DT, MAXITER=1, DXCYCL=1.0, DYCYCL=1.0, restart_ord=N).write(os.path.join(work, "QUAD.INP"))
+            orc.run_reference(work, "mapder8")
+            out["N%d" % N] = np.fromfile(os.path.join(work, "mapder8.f64"))
+        finally:
+            shutil.rmtree(work, ignore_errors=True)
+    np.savez_compressed(os.path.join(outdir, "mapder8_tables.npz"), **out)
+
+
 def main():
     if not os.path.exists(orc.REF_HARNESS):
         raise SystemExit("build the reference first: make -C oracle ref")
@@ -237,6 +254,9 @@ def main():
             continue
         run_plot_case(name, spec, HERE)
         print("wrote", name)
+    if not only or "mapder8_tables" in only:
+        run_mapder8(HERE)
+        print("wrote mapder8_tables")
 
 
 if __name__ == "__main__":
