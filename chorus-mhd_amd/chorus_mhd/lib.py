@@ -50,6 +50,7 @@ SIGNATURES = {
     "chorus_init": (C.c_int, [C.POINTER(ChorusMesh), C.POINTER(ChorusParams), C.POINTER(C.c_void_p)]),
     "chorus_init_bc": (C.c_int, [C.POINTER(ChorusMesh), C.POINTER(ChorusParams), C.POINTER(ChorusBC),
                                  C.POINTER(C.c_void_p)]),
+    "chorus_set_nodes8": (C.c_int, [C.c_void_p, _dp, _dp]),
     "chorus_free": (None, [C.c_void_p]),
     "chorus_last_error": (C.c_char_p, [C.c_void_p]),
     "chorus_set_state": (C.c_int, [C.c_void_p, _dp]),

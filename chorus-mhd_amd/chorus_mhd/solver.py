@@ -208,6 +208,14 @@ class Solver:
     def correct_b(self) -> None:
         self._check(_lib.load().chorus_correct_b(self._ctx), "chorus_correct_b")
 
+    def set_nodes8(self, xm: np.ndarray, ym: np.ndarray) -> None:
+        """Curved cells: the edge nodes XM(NCELL,4), YM(NCELL,4) of the 8-node
+        map (MAPDER_8_NODE, mapder.f90:117-253), column k the node on edge k."""
+        shape = (self.mesh.ncell, 4)
+        xm = np.asfortranarray(np.asarray(xm, dtype=np.float64).reshape(shape)).ravel(order="F")
+        ym = np.asfortranarray(np.asarray(ym, dtype=np.float64).reshape(shape)).ravel(order="F")
+        self._check(_lib.load().chorus_set_nodes8(self._ctx, _lib.ptr(xm), _lib.ptr(ym)), "chorus_set_nodes8")
+
     def step(self, nsteps: int = 1) -> None:
         self._check(_lib.load().chorus_step(self._ctx, nsteps), "chorus_step")
 

@@ -42,7 +42,8 @@ struct chorus_ctx {
     int N = 0, ncell = 0, ncellglob = 0;
     bool ideal = true;             // mu = eta0 = 0 and no AV: viscous terms vanish exactly
     hipStream_t stream = nullptr;
-    double *cellxy = nullptr;
+    double *cellxy = nullptr;      // per cell: x of the ng geometry nodes, then their y
+    int ng = 4;                    // geometry nodes per cell: 4 vertices, or 8 (chorus_set_nodes8)
     int2 *nbr = nullptr;
     double *X[4] = {nullptr, nullptr, nullptr, nullptr};
     double *R = nullptr;           // residual output / FiveStageRk resid0
@@ -145,6 +146,7 @@ FaceArgs face_args(chorus_ctx *c) {
     f.h_msh = c->h_msh;
     f.pi = std::atan(1.0) * 4.0;
     for (int v = 0; v < 8; v++) f.qin[v] = c->qin[v];
+    f.ng = c->ng;
     return f;
 }
 
@@ -200,6 +202,7 @@ StageArgs base_args(chorus_ctx *c) {
     a.FFv = c->FFv;
     a.TQ = c->TQ;
     a.sm = c->sm;
+    a.ng = c->ng;
     return a;
 }
 
@@ -219,6 +222,7 @@ int run_ct(chorus_ctx *c, double *Q, int which, double *partial = nullptr) {
     a.partial = partial ? partial : c->partial;
     a.TQ = c->TQ;
     a.TQout = c->TQ;
+    a.ng = c->ng;
     hipError_t e = launch_ct(c->N, a, c->stream);
     if (e != hipSuccess) return fail(c, CHORUS_ERR_HIP, std::string("ct launch: ") + hipGetErrorString(e));
     return 0;
@@ -717,6 +721,43 @@ int chorus_compresid(chorus_ctx *c, double *resid) {
     return 0;
 }
 
+int chorus_set_nodes8(chorus_ctx *c, const double *xm, const double *ym) {
+    if (!c || !xm || !ym) return fail(c, CHORUS_ERR_ARG, "null argument");
+    if (c->p.precision != CHORUS_FP64 || !c->ideal)
+        return fail(c, CHORUS_ERR_ARG, "8-node cells need the FP64 form with mu = eta0 = 0 and ARTIV = 0");
+    if (c->plot.t) return fail(c, CHORUS_ERR_ARG, "Tecplot output of 8-node cells is not supported");
+    const size_t nc = (size_t)c->ncell;
+    for (size_t i = 0; i < 4 * nc; i++)
+        if (!std::isfinite(xm[i]) || !std::isfinite(ym[i])) return fail(c, CHORUS_ERR_ARG, "edge node not finite");
+    // the corners stay those of the mesh (the first 4 x and the first 4 y of
+    // each cell's current geometry)
+    std::vector<double> old(nc * 2 * c->ng), xy(nc * 16);
+    HIP_TRY(c, hipMemcpyAsync(old.data(), c->cellxy, old.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (size_t ic = 0; ic < nc; ic++)
+        for (int k = 0; k < 4; k++) {
+            xy[ic * 16 + k] = old[ic * 2 * c->ng + k];
+            xy[ic * 16 + 8 + k] = old[ic * 2 * c->ng + c->ng + k];
+            xy[ic * 16 + 4 + k] = xm[(size_t)k * nc + ic];
+            xy[ic * 16 + 12 + k] = ym[(size_t)k * nc + ic];
+        }
+    double *d = nullptr;
+    HIP_TRY(c, hipMalloc(&d, xy.size() * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(d, xy.data(), xy.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return fail(c, CHORUS_ERR_HIP, std::string("chorus_set_nodes8: ") + hipGetErrorString(e));
+    }
+    (void)hipFree(c->cellxy);
+    c->cellxy = d;
+    c->ng = 8;
+    // the captured steps hold the old geometry and the 4-node kernels
+    for (auto &g : c->step_graphs) (void)hipGraphExecDestroy(g.second.exec);
+    c->step_graphs.clear();
+    return 0;
+}
+
 int chorus_correct_b(chorus_ctx *c) {
     if (!c) return CHORUS_ERR_ARG;
     int rc = run_ct(c, c->X[c->cur], 0);
@@ -924,6 +965,7 @@ int plot_state_write(const PlotState &ps, chorus_ctx *c, PlotArgs a, const char 
 int chorus_plot_setup(chorus_ctx *c, const chorus_mesh *m, int nre) {
     if (!c || !m) return fail(c, CHORUS_ERR_ARG, "null argument");
     if (nre < 1) return fail(c, CHORUS_ERR_ARG, "NRE must be >= 1");
+    if (c->ng == 8) return fail(c, CHORUS_ERR_ARG, "Tecplot output of 8-node cells is not supported");
     if (m->ncell != c->ncell || m->N != c->N || !m->xv || !m->yv || !m->ivcell || !m->if2c)
         return fail(c, CHORUS_ERR_ARG, "mesh does not match the context");
     std::string err;

@@ -28,11 +28,17 @@ template <class S> static int blocks_of(int ncell) { return (ncell + S::CPB - 1)
 
 static int fblocks(int N, int nface) { return (nface * N + 255) / 256; }
 
-hipError_t launch_stage_ideal_fp64(int N, const StageArgs &a, const FaceArgs &f, hipStream_t s) {
-#define L_(n)                                                                                        \
-    hipLaunchKernelGGL((k_face_inv<n, double>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);      \
-    hipLaunchKernelGGL((k_stage_ideal<n, double>), dim3(blocks_of<IShape<n>>(a.ncell)), dim3(IShape<n>::THREADS), 0, s, a); \
+template <int N, int NG> static hipError_t ideal_stage(const StageArgs &a, const FaceArgs &f, hipStream_t s) {
+    hipLaunchKernelGGL((k_face_inv<N, double, NG>), dim3(fblocks(N, f.nface)), dim3(256), 0, s, f);
+    hipLaunchKernelGGL((k_stage_ideal<N, double, NG>), dim3(blocks_of<IShape<N>>(a.ncell)), dim3(IShape<N>::THREADS),
+                       0, s, a);
     return hipGetLastError();
+}
+
+// cells with 8 geometry nodes (curved edges) take the NG = 8 instantiations
+hipError_t launch_stage_ideal_fp64(int N, const StageArgs &a, const FaceArgs &f, hipStream_t s) {
+    if ((a.ng == 8) != (f.ng == 8)) return hipErrorInvalidValue;
+#define L_(n) return a.ng == 8 ? ideal_stage<n, 8>(a, f, s) : ideal_stage<n, 4>(a, f, s);
     CHORUS_DISPATCH_N(N, L_)
 #undef L_
 }
@@ -45,13 +51,16 @@ hipError_t launch_trace(int N, int ncell, const double *Q, double *TQ, double *s
 #undef L_
 }
 
-hipError_t launch_ct(int N, const CTArgs &a, hipStream_t s) {
-#define L_(n)                                                                                            \
-    if (a.which == 0)                                                                                    \
-        hipLaunchKernelGGL((k_ct<n, 0>), dim3(nblocks<n>(a.ncell - a.c_first)), dim3(Shape<n>::THREADS), 0, s, a); \
-    else                                                                                                 \
-        hipLaunchKernelGGL((k_ct<n, 1>), dim3(nblocks<n>(a.ncell - a.c_first)), dim3(Shape<n>::THREADS), 0, s, a); \
+template <int N, int NG> static hipError_t ct_pass(const CTArgs &a, hipStream_t s) {
+    if (a.which == 0)
+        hipLaunchKernelGGL((k_ct<N, 0, NG>), dim3(nblocks<N>(a.ncell - a.c_first)), dim3(Shape<N>::THREADS), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_ct<N, 1, NG>), dim3(nblocks<N>(a.ncell - a.c_first)), dim3(Shape<N>::THREADS), 0, s, a);
     return hipGetLastError();
+}
+
+hipError_t launch_ct(int N, const CTArgs &a, hipStream_t s) {
+#define L_(n) return a.ng == 8 ? ct_pass<n, 8>(a, s) : ct_pass<n, 4>(a, s);
     CHORUS_DISPATCH_N(N, L_)
 #undef L_
 }

@@ -63,6 +63,7 @@ struct StageArgs {
     const void *FFv;    // viscous face fluxes (k_face_visc)
     double *TQ;         // traces of Qout at the face points, written by the stage kernel
     double *sm;         // AV: indicator of Qout per cell, written by the viscous stage kernel
+    int ng;             // geometry nodes per cell in cellxy: 4 (0 taken as 4) or 8
 };
 
 // Face-ordered pass: one thread per face point.
@@ -95,6 +96,7 @@ struct FaceArgs {
     int artiv;
     double gam, mu, eta0, h_msh, pi;
     double qin[8];           // INLE faces: the prescribed state (chorus_bc)
+    int ng;                  // geometry nodes per cell in cellxy: 4 (0 taken as 4) or 8
 };
 
 // Boundary kinds of the faces without a right cell (faces[f].z < 0), held in
@@ -136,6 +138,7 @@ struct CTArgs {
     double *partial;    // which=1: per block {sum divB^2, max |divB|}
     const double *TQ;   // traces of A_z (which=0) / Bx, By (which=1) at the face points
     double *TQout;      // which=0: the traces of the corrected Bx, By are written here
+    int ng;             // geometry nodes per cell in cellxy: 4 (0 taken as 4) or 8
 };
 
 // SSP stage combination at one solution point; the new state also replaces

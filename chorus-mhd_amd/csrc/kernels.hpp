@@ -150,25 +150,46 @@ __device__ __forceinline__ int lf_family(int lf) { return (lf & 1) ? 1 : 0; }
 // contravariant metric row used by the flux at a point of family fam
 // (calcjacob.f90:139-170): fam 0 -> (S1(1,1), S1(1,2)) = (yeta, -xeta) at
 // (Xf(ifp), Xs(jfp)); fam 1 -> (S2(2,1), S2(2,2)) = (-ypsi, xpsi) at (Xs(ifp), Xf(jfp))
-template <int N>
-// (Xf, Xs: LDS copies of the point tables, or null for the constant tables)
-__device__ __forceinline__ void metric_row(const double *xy, int fam, int ifp, int jfp, double &s1,
-                                           double &s2, const double *Xf = nullptr, const double *Xs = nullptr) {
+// Geometry of a cell: NG = 4 corner vertices (x1..x4, y1..y4: the bilinear map
+// of mapder.f90:1-76) or NG = 8 nodes (x1..x8, y1..y8: corners then edge
+// nodes, the 8-node map of MAPDER_8_NODE, mapder.f90:117-253)
+template <int NG>
+__device__ __forceinline__ void map_derivs_g(const double *xy, double psi, double eta, double &xpsi, double &ypsi,
+                                             double &xeta, double &yeta) {
+    if constexpr (NG == 8) map_derivs8(xy, xy + 8, psi, eta, xpsi, ypsi, xeta, yeta);
+    else map_derivs(xy, xy + 4, psi, eta, xpsi, ypsi, xeta, yeta);
+}
+
+template <int N, int NG>
+__device__ __forceinline__ void metric_row_g(const double *xy, int fam, int ifp, int jfp, double &s1, double &s2,
+                                             const double *Xf = nullptr, const double *Xs = nullptr) {
     double xpsi, ypsi, xeta, yeta;
     if (fam == 0) {
-        map_derivs(xy, xy + 4, Xf ? Xf[ifp] : c_Xf[N][ifp], Xs ? Xs[jfp] : c_Xs[N][jfp], xpsi, ypsi, xeta, yeta);
+        map_derivs_g<NG>(xy, Xf ? Xf[ifp] : c_Xf[N][ifp], Xs ? Xs[jfp] : c_Xs[N][jfp], xpsi, ypsi, xeta, yeta);
         s1 = yeta; s2 = -xeta;
     } else {
-        map_derivs(xy, xy + 4, Xs ? Xs[ifp] : c_Xs[N][ifp], Xf ? Xf[jfp] : c_Xf[N][jfp], xpsi, ypsi, xeta, yeta);
+        map_derivs_g<NG>(xy, Xs ? Xs[ifp] : c_Xs[N][ifp], Xf ? Xf[jfp] : c_Xf[N][jfp], xpsi, ypsi, xeta, yeta);
         s1 = -ypsi; s2 = xpsi;
     }
 }
 
+template <int N, int NG>
+__device__ __forceinline__ double jacobian_g(const double *xy, int is, int js, const double *Xs = nullptr) {
+    double xpsi, ypsi, xeta, yeta;
+    map_derivs_g<NG>(xy, Xs ? Xs[is] : c_Xs[N][is], Xs ? Xs[js] : c_Xs[N][js], xpsi, ypsi, xeta, yeta);
+    return xpsi * yeta - xeta * ypsi;
+}
+
+template <int N>
+// (Xf, Xs: LDS copies of the point tables, or null for the constant tables)
+__device__ __forceinline__ void metric_row(const double *xy, int fam, int ifp, int jfp, double &s1,
+                                           double &s2, const double *Xf = nullptr, const double *Xs = nullptr) {
+    metric_row_g<N, 4>(xy, fam, ifp, jfp, s1, s2, Xf, Xs);
+}
+
 template <int N>
 __device__ __forceinline__ double jacobian(const double *xy, int is, int js, const double *Xs = nullptr) {
-    double xpsi, ypsi, xeta, yeta;
-    map_derivs(xy, xy + 4, Xs ? Xs[is] : c_Xs[N][is], Xs ? Xs[js] : c_Xs[N][js], xpsi, ypsi, xeta, yeta);
-    return xpsi * yeta - xeta * ypsi;
+    return jacobian_g<N, 4>(xy, is, js, Xs);
 }
 
 template <class R> __device__ __forceinline__ R Lop(int N, int ifp, int is);
@@ -363,7 +384,7 @@ __device__ __forceinline__ void scatter_face_values(T *__restrict__ out, const i
     }
 }
 
-template <int N, class R>
+template <int N, class R, int NG = 4>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_face_inv(FaceArgs a) {
     constexpr int FPB = 256, RW = 2 * NV;          // face points per block, values per point (left, right)
     constexpr int NFB = FPB / N + 2;               // faces a block's points can touch
@@ -387,11 +408,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         free_face = fc.z < 0;
         int ifp, jfp;
         face2fp<N>(fc.y + 1, k, ifp, jfp);
-        double xy[8];
+        double xy[2 * NG];
 #pragma unroll
-        for (int q = 0; q < 8; q++) xy[q] = a.cellxy[(size_t)fc.x * 8 + q];
+        for (int q = 0; q < 2 * NG; q++) xy[q] = a.cellxy[(size_t)fc.x * (2 * NG) + q];
         double s1, s2;
-        metric_row<N>(xy, lf_family(fc.y + 1), ifp, jfp, s1, s2);
+        metric_row_g<N, NG>(xy, lf_family(fc.y + 1), ifp, jfp, s1, s2);
         R Ql[NV], Qr[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) { Ql[v] = R(sT[tid * RW + v]); Qr[v] = R(sT[tid * RW + NV + v]); }
@@ -441,14 +462,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 // stage's face pass.  With mu=eta0=0 and no AV the viscous terms of compresid
 // are exact zeros, so they are not evaluated.
 // ---------------------------------------------------------------------------
-template <int N, class R>
+template <int N, class R, int NG = 4>
 __global__ void __launch_bounds__(IShape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(CHORUS_IDEAL_MINW))) k_stage_ideal(StageArgs a) {
     using S = IShape<N>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
     constexpr int FST = 8 * NF + 1;      // per-cell stride of sF (odd: cells land on different banks)
     constexpr int AST = 3 * NF + 1;      // per-cell stride of sA
     __shared__ double sQ[CPB * NQ];
-    __shared__ double sX[CPB * 8];
+    __shared__ double sX[CPB * 2 * NG];
     __shared__ R sF[CPB * FST];          // [c][v][line][r] contravariant flux, one family at a time
     __shared__ R sA[CPB * AST];          // [c][A_z | metric 1 | metric 2][line][r]
     __shared__ double sOps[Ops64Layout<N>::SIZE];   // FP64 operator tables (OpTables64 layout)
@@ -460,14 +481,14 @@ __global__ void __launch_bounds__(IShape<N>::THREADS) __attribute__((amdgpu_wave
 
     // A: stage the block's cells (contiguous) and their vertex coordinates
     {
-        constexpr int PQ = per_thread<CPB * NQ, S::THREADS>(), PX = per_thread<CPB * 8, S::THREADS>();
+        constexpr int PQ = per_thread<CPB * NQ, S::THREADS>(), PX = per_thread<CPB * 2 * NG, S::THREADS>();
         double hq[PQ], hx[PX];
         gather_regs<PQ, S::THREADS>(hq, a.Qin + (size_t)c0 * NQ, ncl * NQ);
-        gather_regs<PX, S::THREADS>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
+        gather_regs<PX, S::THREADS>(hx, a.cellxy + (size_t)c0 * 2 * NG, ncl * 2 * NG);
         OpTables64<N, S::THREADS> ops;
         ops.gather();
         scatter_lds<PQ, S::THREADS>(sQ, hq, ncl * NQ);
-        scatter_lds<PX, S::THREADS>(sX, hx, ncl * 8);
+        scatter_lds<PX, S::THREADS>(sX, hx, ncl * 2 * NG);
         ops.scatter(sOps);
     }
     __syncthreads();
@@ -488,7 +509,7 @@ __global__ void __launch_bounds__(IShape<N>::THREADS) __attribute__((amdgpu_wave
             const int line = k / (N - 1), r = 1 + k % (N - 1);
             const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
             double s1d, s2d;
-            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d, sOps + OXF, sOps + OXS);
+            metric_row_g<N, NG>(sX + c * 2 * NG, fam, ifp, jfp, s1d, s2d, sOps + OXF, sOps + OXS);
             R Qown[NV], F[8], G[8];
             trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown, sOps);
             iflux<R>(R(a.gam), Qown, F, G);
@@ -511,7 +532,7 @@ __global__ void __launch_bounds__(IShape<N>::THREADS) __attribute__((amdgpu_wave
             else          { lfi = (r == 0) ? 0 : 2; nfp = (r == 0) ? line : (N - 1 - line); }
             const R *src = FF + (((size_t)(c0 + c) * 4 + lfi) * N + nfp) * NV;
             double s1d, s2d;
-            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d, sOps + OXF, sOps + OXS);
+            metric_row_g<N, NG>(sX + c * 2 * NG, fam, ifp, jfp, s1d, s2d, sOps + OXF, sOps + OXS);
             R *f = sF + c * FST + line * (N + 1) + r;
 #pragma unroll
             for (int v = 0; v < 8; v++) f[v * NF] = src[v];
@@ -550,7 +571,7 @@ __global__ void __launch_bounds__(IShape<N>::THREADS) __attribute__((amdgpu_wave
                 dA1 = e1; dA2 = e2;
             } else {
                 // residual at the solution point (compresid.f90:12-65)
-                const double Jd = jacobian<N>(sX + pc * 8, pis, pjs, sOps + OXS);
+                const double Jd = jacobian_g<N, NG>(sX + pc * 2 * NG, pis, pjs, sOps + OXS);
                 const R J = R(Jd);
                 double *qs = sQ + pc * NQ + pk;
                 const R q0 = R(qs[0]), q1 = R(qs[NP]), q2 = R(qs[2 * NP]);
@@ -583,13 +604,13 @@ __global__ void __launch_bounds__(IShape<N>::THREADS) __attribute__((amdgpu_wave
 // (iterations.f90:35-48) and writes per-block sum of squares and max |div B|
 // (no state change).  The partner's trace comes from the stored traces.
 // ---------------------------------------------------------------------------
-template <int N, int WHICH>
+template <int N, int WHICH, int NG = 4>
 __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
     using S = Shape<N>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
     constexpr int NVAR = WHICH == 0 ? 1 : 2, VAR0 = WHICH == 0 ? 8 : 5;
     __shared__ double sV[2][CPB * NP];   // variable planes staged (A_z, or Bx and By)
-    __shared__ double sX[CPB * 8];
+    __shared__ double sX[CPB * 2 * NG];
     __shared__ double sF[2][CPB * NF];   // central flux-point values, one family at a time
     __shared__ double sS[2][CPB * NF];   // metric rows, same layout
     __shared__ double red[2][256];
@@ -602,7 +623,7 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
     const int c0 = a.c_first + blk * CPB;
     const int ncl = min(CPB, a.ncell - c0);
     {
-        constexpr int PV = per_thread<CPB * NP, S::THREADS>(), PX = per_thread<CPB * 8, S::THREADS>();
+        constexpr int PV = per_thread<CPB * NP, S::THREADS>(), PX = per_thread<CPB * 2 * NG, S::THREADS>();
         double hv[NVAR][PV], hx[PX];
 #pragma unroll
         for (int k = 0; k < PV; k++) {
@@ -612,7 +633,7 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
         }
         constexpr int PN = per_thread<CPB * 4, S::THREADS>();
         int2 hn[PN];
-        gather_regs<PX, S::THREADS>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
+        gather_regs<PX, S::THREADS>(hx, a.cellxy + (size_t)c0 * 2 * NG, ncl * 2 * NG);
         gather_regs<PN, S::THREADS>(hn, a.nbr + (size_t)c0 * 4, ncl * 4);
         OpTables64<N, S::THREADS> ops;
         ops.gather();
@@ -638,7 +659,7 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
         for (int w = 0; w < NVAR; w++) scatter_lds<PP, S::THREADS>(sP[w], hp[w], ncl * 4 * N);
 #pragma unroll
         for (int w = 0; w < NVAR; w++) scatter_lds<PV, S::THREADS>(sV[w], hv[w], ncl * NP);
-        scatter_lds<PX, S::THREADS>(sX, hx, ncl * 8);
+        scatter_lds<PX, S::THREADS>(sX, hx, ncl * 2 * NG);
         scatter_lds<PN, S::THREADS>(sNbr, hn, ncl * 4);
         ops.scatter(sOps);
     }
@@ -652,7 +673,7 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
             const int c = it / NF, k = it % NF, line = k / (N + 1), r = k % (N + 1);
             const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
             double s1, s2;
-            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1, s2, sOps + OXF, sOps + OXS);
+            metric_row_g<N, NG>(sX + c * 2 * NG, fam, ifp, jfp, s1, s2, sOps + OXF, sOps + OXS);
             int2 info = make_int2(-1, 1 << 4);
             int lfi = 0, nfp = 0;
             if (r == 0 || r == N) {
@@ -695,7 +716,7 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
     if (WHICH == 0) {
         // update_B, then the traces of the new Bx, By for the next face pass
         if (pact) {
-            const double J = jacobian<N>(sX + pc * 8, pis, pjs, sOps + OXS);
+            const double J = jacobian_g<N, NG>(sX + pc * 2 * NG, pis, pjs, sOps + OXS);
             const size_t base = (size_t)(c0 + pc) * NQ + pk;
             const double n1 = (dpsi[0][0] + out[0][0]) / J;   // nablaQs(9,1)
             const double n2 = (dpsi[0][1] + out[0][1]) / J;   // nablaQs(9,2)
@@ -716,7 +737,7 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
     } else {
         double sumsq = 0.0, dmax = 0.0;
         if (pact) {
-            const double J = jacobian<N>(sX + pc * 8, pis, pjs, sOps + OXS);
+            const double J = jacobian_g<N, NG>(sX + pc * 2 * NG, pis, pjs, sOps + OXS);
             const double dbx = (dpsi[0][0] + out[0][0]) / J;  // nablaQs(6,1)
             const double dby = (dpsi[NVAR - 1][1] + out[NVAR - 1][1]) / J;  // nablaQs(7,2)
             const double db = dbx + dby;

@@ -197,4 +197,28 @@ __device__ __forceinline__ void map_derivs(const double *X, const double *Y, dou
     }
 }
 
+// The same derivatives for an 8-node cell: nodes 1-4 the corners, 5-8 the
+// edge nodes (5 on edge 1-2, 6 on 2-3, 7 on 3-4, 8 on 4-1), with the
+// derivatives of the 8-node shape functions of MAPDER_8_NODE
+// (mapder.f90:117-253), summed in calcjacob's order (calcjacob.f90:20-28).
+__device__ __forceinline__ void map_derivs8(const double *X, const double *Y, double psi, double eta,
+                                            double &xpsi, double &ypsi, double &xeta, double &yeta) {
+    const double da[8] = {(1.0 - eta) * (4.0 * psi + 2.0 * eta - 3.0), (1.0 - eta) * (4.0 * psi - 2.0 * eta - 1.0),
+                          eta * (4.0 * psi + 2.0 * eta - 3.0),        eta * (4.0 * psi - 2.0 * eta - 1.0),
+                          4.0 * (1.0 - eta) * (1.0 - 2.0 * psi),       4.0 * eta * (1.0 - eta),
+                          4.0 * eta * (1.0 - 2.0 * psi),               4.0 * eta * (eta - 1.0)};
+    const double db[8] = {(1.0 - psi) * (4.0 * eta + 2.0 * psi - 3.0), psi * (4.0 * eta - 2.0 * psi - 1.0),
+                          psi * (4.0 * eta + 2.0 * psi - 3.0),        (1.0 - psi) * (4.0 * eta - 2.0 * psi - 1.0),
+                          4.0 * psi * (psi - 1.0),                     4.0 * psi * (1.0 - 2.0 * eta),
+                          4.0 * psi * (1.0 - psi),                     4.0 * (1.0 - psi) * (1.0 - 2.0 * eta)};
+    xpsi = 0.0; ypsi = 0.0; xeta = 0.0; yeta = 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        xpsi = xpsi + da[k] * X[k];
+        ypsi = ypsi + da[k] * Y[k];
+        xeta = xeta + db[k] * X[k];
+        yeta = yeta + db[k] * Y[k];
+    }
+}
+
 }  // namespace chorus

@@ -101,6 +101,19 @@ typedef struct {
  * interior, CYCL and FREE faces. */
 int chorus_init_bc(const chorus_mesh *mesh, const chorus_params *params, const chorus_bc *bc, chorus_ctx **out);
 
+/* Curved cells (CURVE_WALL = 1): replaces the bilinear map of each cell by
+ * the 8-node serendipity map of MapBaseFunc8 / MAPDER_8_NODE
+ * (mapder.f90:91-253), nodes 1-4 the cell's vertices and 5-8 the edge
+ * nodes xm, ym given here: XM(NCELL,4) column-major, column k the node on
+ * edge k of the cell (1: vertices 1-2, 2: 2-3, 3: 3-4, 4: 4-1).  The metric
+ * terms and Jacobians of the residual and of the CT correction then come
+ * from the 8-node derivatives (the sums of calcjacob.f90:20-28 over 8
+ * nodes).  The reference prepares these tables and stops
+ * (initsetup.f90:71-78); this completes that branch.  FP64 form with
+ * mu = eta0 = 0 and ARTIV = 0 only (else CHORUS_ERR_ARG), no Tecplot output
+ * (chorus_plot_setup returns CHORUS_ERR_ARG); call before the first step. */
+int chorus_set_nodes8(chorus_ctx *ctx, const double *xm, const double *ym);
+
 /* Q(9,N,N,NCELL) in the reference layout (setup2d.f90:54).  Upload/download. */
 int chorus_set_state(chorus_ctx *ctx, const double *q);
 int chorus_get_state(chorus_ctx *ctx, double *q);

@@ -1,0 +1,156 @@
+"""8-node (curved) cells on the GPU (chorus_set_nodes8) against the oracle,
+whose 8-node metric tests/test_curved.py pins: the reference's MAPDER_8_NODE
+tables bit for bit, the bilinear limit of straight cells, a closed-form
+quadratic map and free-stream preservation.  The reference computes those
+tables and stops (initsetup.f90:71-78), so there is no reference trajectory
+to hold the GPU to: the FP64 ideal path equals the oracle bit for bit
+(residual, state after RK_S5P3 / FiveStageRk steps with the CT correction,
+max div B)."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from chorus_mhd import curved_grid, quad_grid
+from chorus_mhd.solver import ChorusError, Solver
+
+pytestmark = pytest.mark.gpu
+GAM = 5.0 / 3
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _prim_to_q(xy, rho, u, v, w, p, bx, by, bz, az):
+    q = np.empty(xy.shape[:-1] + (9,))
+    q[..., 0], q[..., 1], q[..., 2], q[..., 3] = rho, rho * u, rho * v, rho * w
+    q[..., 4] = p / (GAM - 1) + 0.5 * rho * (u * u + v * v + w * w) + 0.5 * (bx * bx + by * by + bz * bz)
+    q[..., 5], q[..., 6], q[..., 7], q[..., 8] = bx, by, bz, az
+    return q
+
+
+def _state(o, mesh, N, lx, ly, walls):
+    """A smooth state at the oracle's (curved) solution points, periodic in x
+    and, without walls, in y; with walls at y = 0 and y = ly the normal
+    velocity and the normal B (B = curl A_z) vanish there."""
+    xy = o.array("XXsolu").reshape(mesh.ncell, N, N, 2)
+    x, y = xy[..., 0], xy[..., 1]
+    kx, ky = 2 * math.pi / lx, (1 if walls else 2) * math.pi / ly
+    rho = 1.2 + 0.2 * np.cos(kx * x) * np.cos(ky * y)
+    p = 1.0 + 0.15 * np.cos(kx * x) * np.sin(ky * y)
+    u = 0.4 + 0.1 * np.cos(ky * y)
+    v = 0.15 * np.sin(ky * y) * np.sin(kx * x)
+    w = 0.1 * np.cos(kx * x + ky * y)
+    bz = 0.2 * np.sin(kx * x) * np.cos(ky * y)
+    c, b0 = 0.05, (0.3 if walls else 0.0)
+    s2 = np.sin(ky * y) ** 2
+    az = b0 * y + c * np.cos(kx * x) * s2
+    bx = b0 + c * np.cos(kx * x) * 2 * ky * np.sin(ky * y) * np.cos(ky * y)      # dA_z/dy
+    by = c * kx * np.sin(kx * x) * s2                                             # -dA_z/dx
+    return _prim_to_q(xy, rho, u, v, w, p, bx, by, bz, az)
+
+
+# (N, k_stage, sides bottom/right/top/left, amplitude of the warp, steps)
+CASES = [
+    (2, 3, "CYCL", 0.15, 4),
+    (3, 5, "CYCL", 0.15, 4),
+    (4, 3, ("WALL", "CYCL", "SYMP", "CYCL"), 0.12, 3),
+    (5, 3, "CYCL", 0.12, 4),
+    (6, 5, ("WALL", "CYCL", "WALL", "CYCL"), 0.1, 3),
+    (8, 3, "CYCL", 0.1, 2),
+]
+
+
+def _id(c):
+    sides = c[2] if isinstance(c[2], str) else "-".join(c[2])
+    return "N%d_k%d_%s_amp%g" % (c[0], c[1], sides, c[3])
+
+
+@pytest.mark.parametrize("case", CASES, ids=_id)
+def test_curved_fp64_vs_oracle(case):
+    N, kst, bc, amp, steps = case
+    _say("case curved", _id(case))
+    lx, ly = 1.4, 1.2
+    mesh, xm, ym = curved_grid(7, 6, 0.0, lx, 0.0, ly, amp=amp, bc=bc)
+    o = orc.Oracle(mesh, N, dt=1e-3, k_stage=kst)
+    o.set_nodes8(xm, ym)
+    q0 = _state(o, mesh, N, lx, ly, walls=not isinstance(bc, str))
+    o.set_q(q0)
+    ro = o.compresid()
+    _, _, divb_o = o.steps(steps)
+    qo = o.get_q()
+    assert np.isfinite(qo).all()
+    s = Solver(mesh, N, 1e-3, k_stage=kst)
+    s.set_nodes8(xm, ym)
+    s.set_state(q0)
+    r = s.compresid()
+    s.step(steps)
+    q = s.get_state()
+    d = s.diag()
+    s.close()
+    assert np.array_equal(r, ro)
+    assert np.array_equal(q, qo)
+    _say(_id(case), "max div B: gpu", d["divbmax"], "oracle", divb_o)
+    assert d["divbmax"] == pytest.approx(divb_o, rel=1e-12)
+
+
+def test_curved_free_stream_on_gpu():
+    """A uniform state on curved periodic cells: resid(1:8) vanishes to round-off."""
+    N = 5
+    mesh, xm, ym = curved_grid(6, 5, 0.0, 1.2, 0.0, 1.0, amp=0.15)
+    s = Solver(mesh, N, 1e-3)
+    s.set_nodes8(xm, ym)
+    xy = np.zeros((mesh.ncell, N, N, 2))
+    q = _prim_to_q(xy, 1.3, 0.7, -0.4, 0.2, 0.9, 0.5, -0.3, 0.25, 0.1)
+    s.set_state(q)
+    r = s.compresid()
+    s.close()
+    _say("curved free stream max |resid(1:8)|", np.abs(r[..., :8]).max())
+    assert np.abs(r[..., :8]).max() < 1e-11
+
+
+def test_straight_edge_nodes_match_bilinear_cells():
+    """Edge midpoints of straight-edged cells as nodes 5..8: the 8-node path's
+    residual equals the 4-node path's to round-off (a different sum of the
+    same map), and a rotated node order does not."""
+    N = 4
+    mesh = quad_grid(6, 5, 0.0, 1.3, 0.0, 1.0, jitter=0.15, seed=4)
+    X, Y = mesh.xv[mesh.cells - 1], mesh.yv[mesh.cells - 1]
+    xm = np.stack([0.5 * (X[:, k] + X[:, (k + 1) % 4]) for k in range(4)], axis=1)
+    ym = np.stack([0.5 * (Y[:, k] + Y[:, (k + 1) % 4]) for k in range(4)], axis=1)
+    o = orc.Oracle(mesh, N)
+    q0 = _state(o, mesh, N, 1.3, 1.0, walls=False)
+    r = {}
+    for label, nodes in (("bilinear", None), ("mid", (xm, ym)),
+                         ("rotated", (np.roll(xm, 1, axis=1), np.roll(ym, 1, axis=1)))):
+        s = Solver(mesh, N, 1e-3)
+        if nodes is not None:
+            s.set_nodes8(*nodes)
+        s.set_state(q0)
+        r[label] = s.compresid()
+        s.close()
+    scale = np.abs(r["bilinear"]).max()
+    assert np.abs(r["mid"] - r["bilinear"]).max() <= 1e-11 * scale
+    assert np.abs(r["rotated"] - r["bilinear"]).max() > 1e-3 * scale
+
+
+def test_set_nodes8_rejects_other_paths():
+    mesh, xm, ym = curved_grid(4, 4, 0.0, 1.0, 0.0, 1.0)
+    for kw in (dict(precision="mixed"), dict(artiv=1), dict(mu=0.01), dict(eta0=0.01)):
+        s = Solver(mesh, 3, 1e-3, **kw)
+        with pytest.raises(ChorusError, match="8-node cells"):
+            s.set_nodes8(xm, ym)
+        s.close()
+    s = Solver(mesh, 3, 1e-3)
+    s.set_nodes8(xm, ym)
+    with pytest.raises(ChorusError, match="8-node"):
+        s.plot_setup(2)
+    bad = xm.copy()
+    bad[1, 2] = np.nan
+    with pytest.raises(ChorusError, match="not finite"):
+        s.set_nodes8(bad, ym)
+    s.close()
