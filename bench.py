@@ -117,6 +117,11 @@ ENSEMBLE = dict(
     ninst=256, seed=7, cfl={"ot": 0.2, "rotor": 0.1})
 
 
+def _progress(msg):
+    """One line per finished phase on stderr (the JSON line stays alone on stdout)."""
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
 def ensemble_members(ninst, seed, cfl=None):
     """Meshes, raw fields and per-group time steps of the ensemble workload."""
     cfl = cfl or ENSEMBLE["cfl"]
@@ -283,10 +288,12 @@ def run_ensemble(args):
     raw = ensemble_members(w["ninst"], w["seed"])
     q0 = _initial_states(raw)
     setup_s = time.perf_counter() - t0
+    _progress("ensemble set up: %d instances, %.1f s" % (len(raw), setup_s))
     ndof = np.array([r["mesh"].ncell * r["N"] ** 2 for r in raw], dtype=np.float64)
     res = {}
     for precision in ("mixed", "fp64"):
         res[precision] = _timed_batch(raw, q0, precision, args)
+        _progress("%s batched: %d steps in %.3f s" % (precision, args.steps, res[precision]["wall"]))
     finite = np.array([bool(np.isfinite(a).all() and np.isfinite(b).all())
                        for a, b in zip(res["fp64"]["states"], res["mixed"]["states"])])
     rel = [float(np.linalg.norm(b - a) / np.linalg.norm(a))
@@ -329,9 +336,11 @@ def run_ensemble(args):
     }
     if not args.no_dropin:
         t = _dropin_sequential(raw, q0, args)
+        _progress("FP64 drop-in one instance at a time: %.3f s" % t)
         out["fp64_dropin_sequential"] = {"value": dof_fin * 5 * K / t, "ms_per_step": t * 1e3 / K}
         out["speedup_vs_fp64_dropin"] = t / mx["wall"]
     if not args.no_cpu:
+        _progress("CPU baseline (reference Fortran on the host cores)")
         out["cpu_baseline"] = cpu_ensemble_baseline(raw, q0)
     if args.divb_report:
         out["divb_per_instance"] = [dict(kind=r["kind"], n=r["n"], N=r["N"], finite=bool(f),
@@ -579,6 +588,7 @@ def run_single(w, args):
         return dict(mesh=mesh, dt=dt, wall=wall, tm=tm, Q=Q, Q0=Q0, diag=d, artiv=artiv, tile=tile)
 
     r64 = run("fp64")
+    _progress("FP64 drop-in: %d steps in %.3f s" % (K, r64["wall"]))
     mesh = r64["mesh"]
     ndof = mesh.ncell * N * N
     value = ndof * 5 * K / r64["wall"]
@@ -607,6 +617,7 @@ def run_single(w, args):
         }
     if not args.no_mixed:
         rm = run("mixed", Q0=r64["Q0"])
+        _progress("mixed: %d steps in %.3f s" % (K, rm["wall"]))
         vm = ndof * 5 * K / rm["wall"]
         a, b = r64["Q"], rm["Q"]
         out["mixed"] = {
@@ -621,6 +632,7 @@ def run_single(w, args):
             out["mixed"]["stage_kernel_ms"] = rm["tm"]["ms_stage_avg"]
             out["mixed"]["roofline_frac"] = ndof * b_stage(N) / (rm["tm"]["ms_stage_avg"] * 1e-3) / 1e9 / HBM_PEAK_GBS
     if not args.no_cpu:
+        _progress("CPU baseline (reference Fortran, MPI on the host cores)")
         out["cpu_baseline"] = cpu_reference_mpi(w, r64["artiv"])
     print(json.dumps(out), flush=True)
 

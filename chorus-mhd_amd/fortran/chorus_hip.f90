@@ -17,7 +17,7 @@ module chorus_hip
     private
     public :: chorus_gpu_setup, chorus_gpu_step, chorus_gpu_compresid, chorus_gpu_correct_b, &
               chorus_gpu_fetch_state, chorus_gpu_diag, chorus_gpu_free, chorus_gpu_plot_setup, &
-              chorus_gpu_tecplot, chorus_gpu_write_restart
+              chorus_gpu_tecplot, chorus_gpu_write_restart, chorus_gpu_set_nodes8
 
     integer(c_int), parameter, public :: CHORUS_FP64 = 0, CHORUS_MIXED = 1
 
@@ -119,6 +119,11 @@ module chorus_hip
             real(c_double), value :: ctime
             integer(c_int), intent(in) :: icg(*)
         end function
+        integer(c_int) function c_chorus_set_nodes8(ctx, xm, ym) bind(c, name='chorus_set_nodes8')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(in) :: xm(*), ym(*)
+        end function
         integer(c_size_t) function c_strlen(s) bind(c, name='strlen')
             import :: c_size_t, c_ptr
             type(c_ptr), value :: s
@@ -192,6 +197,18 @@ contains
         call check(c_chorus_init_bc(m, p, b, ctx), 'chorus_init_bc')
         call check(c_chorus_set_state(ctx, Q), 'chorus_set_state')
         h_mesh = m
+    end subroutine
+
+    ! CURVE_WALL = 1 (initsetup.f90:71-78, where the reference stops): the edge
+    ! nodes 5..8 of every cell, XM(NCELL,4) and YM(NCELL,4), column k on edge k
+    ! (1: vertices 1-2, 2: 2-3, 3: 3-4, 4: 4-1), for the 8-node map of
+    ! MAPDER_8_NODE (mapder.f90:117-253); after chorus_gpu_setup, before stepping
+    subroutine chorus_gpu_set_nodes8(xm, ym)
+        real(c_double), intent(in) :: xm(:,:), ym(:,:)
+        real(c_double), allocatable :: x(:,:), y(:,:)
+        x = xm
+        y = ym
+        call check(c_chorus_set_nodes8(ctx, x, y), 'chorus_set_nodes8')
     end subroutine
 
     ! tecplotter2dsetup (tecplotter2dsetup.f90:1-164) on the mesh of chorus_gpu_setup

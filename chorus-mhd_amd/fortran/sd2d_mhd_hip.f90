@@ -12,6 +12,13 @@
 ! dump (the library's device-side WRITE_ALL_DATA_PARALLEL; with "refio" the
 ! state is copied back and the reference's own writer runs).
 !
+! Curved cells: where the reference stops (CURVE_WALL=1, initsetup.f90:71-78),
+! a file nodes8.dat in the run directory gives every cell's edge nodes 5..8
+! (NCELLGLOB records by global cell number, each the 4 x then the 4 y, edge k
+! between vertices k and k+1); the set-up then runs with CURVE_WALL=0 and the
+! library switches to the 8-node map of MAPDER_8_NODE.  No Tecplot output
+! then (the reference's plot set-up has no 8-node branch either).
+!
 ! usage: sd2d_mhd_hip [mixed] [refio]   (FP64 drop-in unless "mixed" is given)
 program sd2d_mhd_hip
     use setup2d
@@ -22,8 +29,9 @@ program sd2d_mhd_hip
 
     character(16) :: arg
     integer(c_int) :: prec
-    logical :: refio
-    integer :: ia
+    logical :: refio, curved
+    integer :: ia, icl
+    real(c_double), allocatable :: rows8(:,:), xm8(:,:), ym8(:,:)
     real(c_double) :: residglob, divbglob, divbmax, eigv, eps0
     double precision :: t_start, t_finish
     integer :: fh
@@ -59,7 +67,20 @@ program sd2d_mhd_hip
     call INIT_SETUP
 
     call chorus_gpu_setup(gam, prec)
-    call chorus_gpu_plot_setup(NRE)
+    inquire(file='nodes8.dat', exist=curved)
+    if (curved) then
+        allocate(rows8(8, NCELLGLOB), xm8(NCELL, 4), ym8(NCELL, 4))
+        open(newunit=fh, file='nodes8.dat', status='old')
+        read(fh, *) rows8
+        close(fh)
+        do icl = 1, NCELL
+            xm8(icl, :) = rows8(1:4, IC2ICG(icl))
+            ym8(icl, :) = rows8(5:8, IC2ICG(icl))
+        end do
+        call chorus_gpu_set_nodes8(xm8, ym8)
+    else
+        call chorus_gpu_plot_setup(NRE)
+    end if
 
     open(newunit=fh, file='resid.out', status='replace')
     write(fh, *) 'ITER    ctime     RESIDNORM   divB'
@@ -72,7 +93,7 @@ program sd2d_mhd_hip
             write(fh, *) iter, ctime, residglob, divbglob
         end if
         if (mod(iter, nwrite) == 1 .or. iter >= MAXITER) then
-            call chorus_gpu_tecplot(ctime)
+            if (.not. curved) call chorus_gpu_tecplot(ctime)
             if (refio) then
                 call chorus_gpu_fetch_state()
                 call WRITE_ALL_DATA_PARALLEL

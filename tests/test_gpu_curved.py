@@ -7,13 +7,17 @@ to hold the GPU to: the FP64 ideal path equals the oracle bit for bit
 (residual, state after RK_S5P3 / FiveStageRk steps with the CT correction,
 max div B)."""
 import math
+import os
+import subprocess
 import sys
 
 import numpy as np
 import pytest
 
+from conftest import REPO
+
 import oracle as orc
-from chorus_mhd import curved_grid, quad_grid
+from chorus_mhd import QuadInp, curved_grid, quad_grid, read_restart, write_restart
 from chorus_mhd.solver import ChorusError, Solver
 
 pytestmark = pytest.mark.gpu
@@ -154,3 +158,35 @@ def test_set_nodes8_rejects_other_paths():
     with pytest.raises(ChorusError, match="not finite"):
         s.set_nodes8(bad, ym)
     s.close()
+
+
+DROPIN = os.path.join(REPO, "chorus-mhd_amd", "fortran", "_build", "sd2d_mhd_hip")
+
+
+@pytest.mark.skipif(not os.path.exists(DROPIN), reason="drop-in driver not built (needs the reference sources)")
+def test_fortran_dropin_curved(tmp_path):
+    """The Fortran drop-in with nodes8.dat (the edge nodes by global cell):
+    the reference's set-up chain, then chorus_gpu_set_nodes8; its dump after 3
+    steps equals the oracle's 8-node state."""
+    _say("case dropin curved")
+    N, steps, lx, ly = 4, 3, 1.4, 1.2
+    mesh, xm, ym = curved_grid(7, 6, 0.0, lx, 0.0, ly, amp=0.12)
+    o = orc.Oracle(mesh, N, dt=1e-3)
+    o.set_nodes8(xm, ym)
+    q0 = _state(o, mesh, N, lx, ly, walls=False)
+    d = str(tmp_path)
+    mesh.write(d)
+    os.makedirs(os.path.join(d, "TEC"))
+    write_restart(os.path.join(d, "restart.in"), 0, 0.0, q0)
+    with open(os.path.join(d, "nodes8.dat"), "w") as f:
+        for ic in range(mesh.ncell):
+            f.write(" ".join("%.17e" % v for v in list(xm[ic]) + list(ym[ic])) + "\n")
+    QuadInp(N=N, dt=1e-3, MAXITER=steps, DXCYCL=mesh.dxcycl, DYCYCL=mesh.dycycl, restart=1, restart_ord=N,
+            NAMERESTART="restart.in", nwrite=1000000).write(os.path.join(d, "QUAD.INP"))
+    r = subprocess.run([DROPIN], cwd=d, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    it, t, Q = read_restart(os.path.join(d, "restart.%08d.dat" % steps), N)
+    o.set_q(q0)
+    o.steps(steps)
+    assert it == steps
+    assert np.array_equal(Q, o.get_q())
