@@ -158,28 +158,31 @@ hipError_t launch_diag_reduce(const double *R, size_t n, int nblk, double *scr, 
 
 template <int N> static int vblocks(int ncell) { return (ncell + VShape<N>::CPB - 1) / VShape<N>::CPB; }
 
-template <int n, int VM>
+template <int n, int VM, int NG>
 static hipError_t visc_stage(const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s) {
-    hipLaunchKernelGGL((k_face_inv<n, double>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);
-    hipLaunchKernelGGL((k_grad<n, double>), dim3(vblocks<n>(a.ncell)), dim3(V2Shape<n>::THREADS), 0, s, v);
-    hipLaunchKernelGGL((k_face_visc<n, double, VM>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);
-    hipLaunchKernelGGL((k_stage_visc<n, double, VM>), dim3(vblocks<n>(a.ncell)), dim3(V2Shape<n>::THREADS), 0, s, a, v);
+    hipLaunchKernelGGL((k_face_inv<n, double, NG>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);
+    hipLaunchKernelGGL((k_grad<n, double, NG>), dim3(vblocks<n>(a.ncell)), dim3(V2Shape<n>::THREADS), 0, s, v);
+    hipLaunchKernelGGL((k_face_visc<n, double, VM, NG>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);
+    hipLaunchKernelGGL((k_stage_visc<n, double, VM, NG>), dim3(vblocks<n>(a.ncell)), dim3(V2Shape<n>::THREADS), 0, s,
+                       a, v);
     return hipGetLastError();
 }
 
-template <int n>
+template <int n, int NG>
 static hipError_t visc_stage_vm(const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s) {
     const int vm = ((v.mu != 0.0 || v.eta0 != 0.0) ? 1 : 0) | (v.artiv ? 2 : 0);
     switch (vm) {
-    case 1: return visc_stage<n, 1>(a, v, f, s);
-    case 2: return visc_stage<n, 2>(a, v, f, s);
-    case 3: return visc_stage<n, 3>(a, v, f, s);
+    case 1: return visc_stage<n, 1, NG>(a, v, f, s);
+    case 2: return visc_stage<n, 2, NG>(a, v, f, s);
+    case 3: return visc_stage<n, 3, NG>(a, v, f, s);
     default: return hipErrorInvalidValue;
     }
 }
 
+// cells with 8 geometry nodes (curved edges) take the NG = 8 instantiations
 hipError_t launch_stage_visc_fp64(int N, const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s) {
-#define V_(n) return visc_stage_vm<n>(a, v, f, s);
+    if ((a.ng == 8) != (f.ng == 8)) return hipErrorInvalidValue;
+#define V_(n) return a.ng == 8 ? visc_stage_vm<n, 8>(a, v, f, s) : visc_stage_vm<n, 4>(a, v, f, s);
     CHORUS_DISPATCH_N(N, V_)
 #undef V_
 }

@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(256) k_trace(int ncell, const double *Q, doubl
 // Two threads per solution point (V2Shape): the flux phase has one task per
 // flux point, the derivative phase splits the x (group 0) and y (group 1)
 // components of nablaQs.
-template <int N, class R>
+template <int N, class R, int NG = 4>
 __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(CHORUS_VISC_MINW))) k_grad(ViscArgs a) {
     using S = VShape<N>;
     using T = V2Shape<N>;
@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
     constexpr int NIT = T::NIT;
     constexpr int VST = 10 * NF + 1;     // per-cell stride: Qv (8 planes) + metric rows (2 planes)
     __shared__ double sQ[CPB * NQ];
-    __shared__ double sX[CPB * 8];
+    __shared__ double sX[CPB * 2 * NG];
     __shared__ R sV[CPB * VST];          // [c][v | s1 | s2][line][r], one family at a time
     __shared__ R sG[CPB * 16 * NP];      // nablaQs [c][d*8+v][p]
     __shared__ int2 sNbr[CPB * 4];       // face neighbours of the block's cells
@@ -168,17 +168,17 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
     const int c0 = blk * CPB;
     const int ncl = min(CPB, a.ncell - c0);
     {
-        constexpr int PQ = per_thread<CPB * NQ, T::THREADS>(), PX = per_thread<CPB * 8, T::THREADS>();
+        constexpr int PQ = per_thread<CPB * NQ, T::THREADS>(), PX = per_thread<CPB * 2 * NG, T::THREADS>();
         constexpr int PN = per_thread<CPB * 4, T::THREADS>();
         double hq[PQ], hx[PX];
         int2 hn[PN];
         gather_regs<PQ, T::THREADS>(hq, a.Qin + (size_t)c0 * NQ, ncl * NQ);
-        gather_regs<PX, T::THREADS>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
+        gather_regs<PX, T::THREADS>(hx, a.cellxy + (size_t)c0 * 2 * NG, ncl * 2 * NG);
         gather_regs<PN, T::THREADS>(hn, a.nbr + (size_t)c0 * 4, ncl * 4);
         OpTables64<N, T::THREADS> ops;
         ops.gather();
         scatter_lds<PQ, T::THREADS>(sQ, hq, ncl * NQ);
-        scatter_lds<PX, T::THREADS>(sX, hx, ncl * 8);
+        scatter_lds<PX, T::THREADS>(sX, hx, ncl * 2 * NG);
         scatter_lds<PN, T::THREADS>(sNbr, hn, ncl * 4);
         ops.scatter(sOps);
     }
@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
                 const int line = k / (N - 1), r = 1 + k % (N - 1);
                 const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
                 double s1d, s2d;
-                metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d, sOps + OXF, sOps + OXS);
+                metric_row_g<N, NG>(sX + c * 2 * NG, fam, ifp, jfp, s1d, s2d, sOps + OXF, sOps + OXS);
                 R Qown[NV];
                 trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown, sOps);
                 R *q = sV + c * VST + line * (N + 1) + r;
@@ -216,7 +216,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
                 const int cell = c0 + c;
                 const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
                 double s1d, s2d;
-                metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d, sOps + OXF, sOps + OXS);
+                metric_row_g<N, NG>(sX + c * 2 * NG, fam, ifp, jfp, s1d, s2d, sOps + OXF, sOps + OXS);
                 int lfi, nfp;
                 face_of<N>(fam, r, line, lfi, nfp);
                 const int2 info = sNbr[c * 4 + lfi];
@@ -254,7 +254,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
 #pragma unroll
                 for (int v = 0; v < 8; v++) gp[v] = acc[v];
             } else {
-                const R J = R(jacobian<N>(sX + pc * 8, pis, pjs, sOps + OXS));
+                const R J = R(jacobian_g<N, NG>(sX + pc * 2 * NG, pis, pjs, sOps + OXS));
                 R *gs = a.gsol_r<R>() + (size_t)(c0 + pc) * 16 * NP;
 #pragma unroll
                 for (int v = 0; v < 8; v++) {
@@ -359,7 +359,7 @@ __device__ __forceinline__ void visc_point(const ViscArgs &v, const double *mu4,
 // point the flux-point coordinate of mu_art's interpolation is exactly 0 or 1,
 // so mu_art is the face average itself.
 // ---------------------------------------------------------------------------
-template <int N, class R, int VM>
+template <int N, class R, int VM, int NG = 4>
 __global__ void __launch_bounds__(256) k_face_visc(FaceArgs a) {
     const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= a.nface * N) return;
@@ -371,17 +371,17 @@ __global__ void __launch_bounds__(256) k_face_visc(FaceArgs a) {
     R sl1, sl2, sr1 = R(0), sr2 = R(0);
     {
         int ifp, jfp;
-        double xy[8], m1, m2;
+        double xy[2 * NG], m1, m2;
         face2fp<N>(fc.y + 1, k, ifp, jfp);
 #pragma unroll
-        for (int q = 0; q < 8; q++) xy[q] = a.cellxy[(size_t)fc.x * 8 + q];
-        metric_row<N>(xy, lf_family(fc.y + 1), ifp, jfp, m1, m2);
+        for (int q = 0; q < 2 * NG; q++) xy[q] = a.cellxy[(size_t)fc.x * (2 * NG) + q];
+        metric_row_g<N, NG>(xy, lf_family(fc.y + 1), ifp, jfp, m1, m2);
         sl1 = R(m1); sl2 = R(m2);
         if (!free_face) {
             face2fp<N>(fc.w + 1, kr, ifp, jfp);
 #pragma unroll
-            for (int q = 0; q < 8; q++) xy[q] = a.cellxy[(size_t)fc.z * 8 + q];
-            metric_row<N>(xy, lf_family(fc.w + 1), ifp, jfp, m1, m2);
+            for (int q = 0; q < 2 * NG; q++) xy[q] = a.cellxy[(size_t)fc.z * (2 * NG) + q];
+            metric_row_g<N, NG>(xy, lf_family(fc.w + 1), ifp, jfp, m1, m2);
             sr1 = R(m1); sr2 = R(m2);
         }
     }
@@ -439,7 +439,7 @@ __global__ void __launch_bounds__(256) k_face_visc(FaceArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-template <int N, class R, int VM>
+template <int N, class R, int VM, int NG = 4>
 __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(CHORUS_VISC_MINW))) k_stage_visc(StageArgs a, ViscArgs v) {
     using S = VShape<N>;
     using T = V2Shape<N>;
@@ -447,7 +447,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
     constexpr int NIT = T::NIT;
     constexpr int UST = 20 * NF + 1;     // per-cell stride: F (8) + Fv (9) + A_z + metric rows (2)
     __shared__ double sQ[CPB * NQ];
-    __shared__ double sX[CPB * 8];
+    __shared__ double sX[CPB * 2 * NG];
     __shared__ R sG[CPB * 16 * NP];      // nablaQs from k_grad
     __shared__ R sU[CPB * UST];          // [c][plane][line][r], one family at a time
     __shared__ double sMu[CPB * 4];      // muAVfi/fj(1,side) per local face
@@ -460,12 +460,12 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
     const int c0 = blk * CPB;
     const int ncl = min(CPB, a.ncell - c0);
     {
-        constexpr int PQ = per_thread<CPB * NQ, T::THREADS>(), PX = per_thread<CPB * 8, T::THREADS>();
+        constexpr int PQ = per_thread<CPB * NQ, T::THREADS>(), PX = per_thread<CPB * 2 * NG, T::THREADS>();
         constexpr int PG = per_thread<CPB * 16 * NP, T::THREADS>(), PM = per_thread<CPB * 4, T::THREADS>();
         double hq[PQ], hx[PX], hm[PM];
         R hg[PG];
         gather_regs<PQ, T::THREADS>(hq, a.Qin + (size_t)c0 * NQ, ncl * NQ);
-        gather_regs<PX, T::THREADS>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
+        gather_regs<PX, T::THREADS>(hx, a.cellxy + (size_t)c0 * 2 * NG, ncl * 2 * NG);
         gather_regs<PG, T::THREADS>(hg, v.gsol_r<R>() + (size_t)c0 * 16 * NP, ncl * 16 * NP);
         OpTables64<N, T::THREADS> ops;
         ops.gather();
@@ -473,7 +473,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
         // AV.f90:199-303, 494-564), evaluated by k_face_visc
         if constexpr ((VM & 2) != 0) gather_regs<PM, T::THREADS>(hm, v.mu4 + (size_t)c0 * 4, ncl * 4);
         scatter_lds<PQ, T::THREADS>(sQ, hq, ncl * NQ);
-        scatter_lds<PX, T::THREADS>(sX, hx, ncl * 8);
+        scatter_lds<PX, T::THREADS>(sX, hx, ncl * 2 * NG);
         scatter_lds<PG, T::THREADS>(sG, hg, ncl * 16 * NP);
         ops.scatter(sOps);
         if constexpr ((VM & 2) != 0) scatter_lds<PM, T::THREADS>(sMu, hm, ncl * 4);
@@ -512,7 +512,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
                 const int line = k / (N - 1), r = 1 + k % (N - 1);
                 const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
                 double s1d, s2d;
-                metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d, sOps + OXF, sOps + OXS);
+                metric_row_g<N, NG>(sX + c * 2 * NG, fam, ifp, jfp, s1d, s2d, sOps + OXF, sOps + OXS);
                 R Qown[NV];
                 trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown, sOps);
                 R *u = sU + c * UST + line * (N + 1) + r;
@@ -544,7 +544,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
                 face_of<N>(fam, r, line, lfi, nfp);
                 const size_t o = (((size_t)(c0 + c) * 4 + lfi) * N + nfp) * NV;
                 double s1d, s2d;
-                metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d, sOps + OXF, sOps + OXS);
+                metric_row_g<N, NG>(sX + c * 2 * NG, fam, ifp, jfp, s1d, s2d, sOps + OXF, sOps + OXS);
                 R *u = sU + c * UST + line * (N + 1) + r;
 #pragma unroll
                 for (int w = 0; w < 8; w++) u[w * NF] = FF[o + w];
@@ -608,7 +608,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
     __syncthreads();
     if (pact && grp == 0) {
         // residual at the solution point (compresid.f90:12-65)
-        const R J = R(jacobian<N>(sX + pc * 8, pis, pjs, sOps + OXS));
+        const R J = R(jacobian_g<N, NG>(sX + pc * 2 * NG, pis, pjs, sOps + OXS));
         double *qs = sQ + pc * NQ + pk;
         const R q0 = R(qs[0]), q1 = R(qs[NP]), q2 = R(qs[2 * NP]);
         const R au = q1 / q0, av = q2 / q0;
