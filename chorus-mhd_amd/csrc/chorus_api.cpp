@@ -55,7 +55,7 @@ struct chorus_ctx {
     double *FA = nullptr;          // mixed form: per cell face point: FP64 upwind A_z
     void *FFv = nullptr;           // viscous path: per cell face point: common viscous flux
     void *gtr = nullptr;           // viscous path: own nablaQs face traces
-    void *gsol = nullptr;          // viscous path: nablaQs at the solution points
+    void *gsol = nullptr;          // viscous path: nablaQs at the solution points (mixed), interior viscous fluxes (FP64)
     double *sm = nullptr;          // viscous path: AV indicator per cell
     double *mu4 = nullptr;         // viscous path: face-averaged muAV per cell and local face
     double *eig = nullptr;         // two slots (of ninst values) of the per-stage maximum wave speed
@@ -166,6 +166,9 @@ int launch_stage(chorus_ctx *c, const StageArgs &a) {
         v.Qin = a.Qin;
         v.gtr = c->gtr;
         v.gsol = c->gsol;
+        v.fvi = c->gsol;
+        v.inst = c->inst;
+        v.hinst = c->hinst;
         v.sm = c->sm;
         v.eig = c->eig + (size_t)c->stage_parity * c->ninst;
         v.eig_next = c->eig + (size_t)(1 - c->stage_parity) * c->ninst;

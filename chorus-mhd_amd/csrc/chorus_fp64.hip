@@ -161,7 +161,7 @@ template <int N> static int vblocks(int ncell) { return (ncell + VShape<N>::CPB 
 template <int n, int VM, int NG>
 static hipError_t visc_stage(const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s) {
     hipLaunchKernelGGL((k_face_inv<n, double, NG>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);
-    hipLaunchKernelGGL((k_grad<n, double, NG>), dim3(vblocks<n>(a.ncell)), dim3(V2Shape<n>::THREADS), 0, s, v);
+    hipLaunchKernelGGL((k_grad<n, double, VM, NG>), dim3(vblocks<n>(a.ncell)), dim3(V2Shape<n>::THREADS), 0, s, v);
     hipLaunchKernelGGL((k_face_visc<n, double, VM, NG>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);
     hipLaunchKernelGGL((k_stage_visc<n, double, VM, NG>), dim3(vblocks<n>(a.ncell)), dim3(V2Shape<n>::THREADS), 0, s,
                        a, v);

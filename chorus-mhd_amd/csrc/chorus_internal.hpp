@@ -11,9 +11,11 @@ namespace chorus {
 
 constexpr int kMaxN = 8;       // solution points per direction supported by the kernels
 
-// minimum waves per SIMD requested from the register allocator (launch bounds)
+// minimum waves per SIMD requested from the register allocator (launch bounds);
+// FP64 viscous kernels: 4 (at most 128 VGPRs) except at N=3, where k_grad
+// would spill 24 registers
 #ifndef CHORUS_VISC_MINW
-#define CHORUS_VISC_MINW 3
+#define CHORUS_VISC_MINW(N) ((N) == 3 ? 3 : 4)
 #endif
 #ifndef CHORUS_IDEAL_MINW
 #define CHORUS_IDEAL_MINW 3
@@ -114,7 +116,10 @@ struct ViscArgs {
     const int2 *nbr;
     const double *Qin;
     void *gtr;          // own nablaQs traces [cell][lf][nfp][16]  (Real)
-    void *gsol;         // nablaQs at the solution points [cell][16][N*N] (Real)
+    void *gsol;         // mixed form: nablaQs at the solution points [cell][16][N*N] (float)
+    void *fvi;          // FP64 form: interior viscous/AV fluxes [cell][fam][9][N*(N-1)] (the gsol allocation)
+    const int *inst;    // batched instances: instance of each cell (null: one instance)
+    const double *hinst;   // batched instances: AV length scale per instance
     double *sm;         // log10 smoothness indicator of density per cell
     double *eig;        // this stage's maximum Rusanov wave speed (atomic max)
     double *eig_next;   // the next stage's slots (one per instance), zeroed by k_stage_visc
@@ -126,6 +131,7 @@ struct ViscArgs {
     double mu, eta0, gam, h_msh, pi;
     template <class R> __host__ __device__ R *gtr_r() const { return (R *)gtr; }
     template <class R> __host__ __device__ R *gsol_r() const { return (R *)gsol; }
+    template <class R> __host__ __device__ R *fvi_r() const { return (R *)fvi; }
 };
 
 struct CTArgs {

@@ -8,17 +8,21 @@
 //   k_grad       per cell: central Qv at every flux point (face values average
 //                both stored traces, interfaceflux_all.f90:79-91), nablaQs at
 //                the solution points (nablaQsp, compflux.f90:107-157), its own
-//                traces at the four faces and the AV smoothness indicator of
-//                density (AV.f90:15-82)
+//                traces at the four faces, and the viscous and AV fluxes of the
+//                interior flux points (comp_viscous_flux, compflux.f90:205-256;
+//                FP64 form)
 //   k_face_visc  per face point: the viscous and AV flux from the face-averaged
 //                Qv and gradient (interfaceVISflux, interfaceAVflux, BCVISFLUX,
 //                comp_viscous_flux) for both cells, each in its own metric
-//   k_stage_visc per cell: interior inviscid, viscous and AV fluxes, the stored
-//                face fluxes, the residual (compresid.f90:1-67), the SSP stage
-//                combination and the traces of the new state
+//   k_stage_visc per cell: interior inviscid fluxes, the stored interior
+//                viscous and face fluxes, the residual (compresid.f90:1-67),
+//                the SSP stage combination, the traces of the new state and its
+//                AV smoothness indicator of density (AV.f90:15-82)
 //
 // Per cell, local face lf and face point nfp (own numbering, face2fpconnec order):
 //   gtr[cell][lf][nfp][16]  this cell's trace of nablaQs (vars 1..8, x then y)
+//   fvi[cell][fam][9][N(N-1)]  FP64 form: viscous + AV flux (contravariant) of
+//                           the interior flux points, point k = line*(N-1) + r-1
 //   sm[cell]                log10 smoothness indicator of density (SM_IND(1,:))
 // Only muAV(1,:) reaches the flux (mu_art = epsilon_loc(1), getrusanov.f90:279),
 // so only density's indicator is evaluated.
@@ -145,12 +149,71 @@ __global__ void __launch_bounds__(256) k_trace(int ncell, const double *Q, doubl
     if (sm) av_sensor<N, CPB>(sm, sQ, sAV, ncl, c0, sOps);
 }
 
+// muAV(1,:) of one cell from its indicator and epsilon0 (AV.f90:120-172)
+__device__ __forceinline__ double mu_av(double sm, double eps0, double pi) {
+    const double se_low = -6.0, se_high = -1.0;
+    const double k_range = (se_high - se_low) * 0.5, s0 = (se_high + se_low) * 0.5;
+    if (sm < se_low) return 0.0;
+    if (sm > se_high) return eps0;
+    return eps0 / 2.0 * (1.0 + sin(pi * (sm - s0) / 2.0 / k_range));
+}
+
+// gradient of the 8 variables at flux point (ifp,jfp) of family fam: own trace
+// of nablaQs (nablaQ_interior_FP, compflux.f90:159-196)
+template <int N, class R>
+__device__ __forceinline__ void grad_trace(const R *gq, int fam, int ifp, int jfp, R *g, const R *Lt = nullptr) {
+    constexpr int NP = N * N;
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+#pragma unroll
+        for (int w = 0; w < 8; w++) g[d * NV + w] = R(0);
+        g[d * NV + 8] = R(0);
+    }
+    // contraction index outermost: 16 loads in flight per step
+    constexpr int U = sizeof(R) == 8 ? contraction_unroll<R, N>() : CHORUS_MX_GRAD_UNROLL;
+#pragma unroll U
+    for (int sp = 0; sp < N; sp++) {
+        const int row = fam == 0 ? ifp : jfp;
+        const R l = Lt ? Lt[row * N + sp] : Lop<R>(N, row, sp);
+        const int idx = fam == 0 ? jfp * N + sp : sp * N + ifp;
+#pragma unroll
+        for (int d = 0; d < 2; d++)
+#pragma unroll
+            for (int w = 0; w < 8; w++) g[d * NV + w] = g[d * NV + w] + gq[(d * 8 + w) * NP + idx] * l;
+    }
+}
+
+// viscous + AV contravariant flux at one flux point (comp_viscous_flux, compflux.f90:205-256)
+// VM: bit 0 physical viscosity/resistivity (mu or eta0 nonzero), bit 1 AV
+template <int N, class R, int VM>
+__device__ __forceinline__ void visc_point(const ViscArgs &v, const double *mu4, int fam, int ifp, int jfp, R s1,
+                                           R s2, const R *qv, const R *g, R *fv) {
+    const R gam = R(v.gam), lam = R(-2.0 / 3.0), prandt = R(0.72);
+    R Fv[NV], Gv[NV];
+    if constexpr ((VM & 1) != 0) {
+        vflux<R>(gam, lam, prandt, R(v.mu), R(v.eta0), qv, g, Fv, Gv);
+#pragma unroll
+        for (int w = 0; w < NV; w++) fv[w] = Fv[w] * s1 + Gv[w] * s2;
+    } else {
+#pragma unroll
+        for (int w = 0; w < NV; w++) fv[w] = R(0);
+    }
+    if constexpr ((VM & 2) != 0) {
+        const int lo = fam == 0 ? 3 : 0, hi = fam == 0 ? 1 : 2;   // sides 1 and 2 of the family
+        const double x = fam == 0 ? c_Xf[N][ifp] : c_Xf[N][jfp];
+        const R mu_art = R(mu4[lo] * (1.0 - x) + mu4[hi] * x);
+        vflux<R>(gam, lam, prandt, mu_art, R(0), qv, g, Fv, Gv);
+#pragma unroll
+        for (int w = 0; w < NV; w++) fv[w] = fv[w] + Fv[w] * s1 + Gv[w] * s2;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Two threads per solution point (V2Shape): the flux phase has one task per
 // flux point, the derivative phase splits the x (group 0) and y (group 1)
 // components of nablaQs.
-template <int N, class R, int NG = 4>
-__global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(CHORUS_VISC_MINW))) k_grad(ViscArgs a) {
+template <int N, class R, int VM, int NG = 4>
+__global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(CHORUS_VISC_MINW(N)))) k_grad(ViscArgs a) {
     using S = VShape<N>;
     using T = V2Shape<N>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
@@ -161,6 +224,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
     __shared__ R sV[CPB * VST];          // [c][v | s1 | s2][line][r], one family at a time
     __shared__ R sG[CPB * 16 * NP];      // nablaQs [c][d*8+v][p]
     __shared__ int2 sNbr[CPB * 4];       // face neighbours of the block's cells
+    __shared__ double sMu[CPB * 4];      // muAVfi/fj(1,side) per local face
     __shared__ double sOps[Ops64Layout<N>::SIZE];   // FP64 operator tables (OpTables64 layout)
     constexpr int OM = Ops64Layout<N>::M, OXF = Ops64Layout<N>::XF, OXS = Ops64Layout<N>::XS;
     const int tid = threadIdx.x;
@@ -183,6 +247,20 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
         ops.scatter(sOps);
     }
     __syncthreads();
+    // face averages of muAV(1) of every local face of the block's cells
+    // (compAVflux / interfaceAVflux / cyclocAVflux, AV.f90:199-303, 494-564,
+    // with epsilon0 of AV.f90:145-148 from this stage's wave-speed maximum);
+    // k_face_visc forms the same average at the face points
+    if constexpr ((VM & 2) != 0) {
+        for (int i = tid; i < ncl * 4; i += blockDim.x) {
+            const int cell = c0 + i / 4;
+            const int2 info = sNbr[i];
+            const int in = a.inst ? a.inst[cell] : 0;
+            const double eps0 = a.eig[in] * (a.inst ? a.hinst[in] : a.h_msh) / (double)(N - 1) * 1.0;
+            const double mo = mu_av(a.sm[cell], eps0, a.pi);
+            sMu[i] = ((info.y >> 4) & 1) ? mo : 0.5 * (mo + mu_av(a.sm[info.x], eps0, a.pi));
+        }
+    }
     const int grp = tid >= T::G1 ? 1 : 0;
     const int pt = tid - grp * T::G1;
     const int pc = pt / NP, pk = pt % NP, pjs = pk / N, pis = pk % N;
@@ -255,13 +333,8 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
                 for (int v = 0; v < 8; v++) gp[v] = acc[v];
             } else {
                 const R J = R(jacobian_g<N, NG>(sX + pc * 2 * NG, pis, pjs, sOps + OXS));
-                R *gs = a.gsol_r<R>() + (size_t)(c0 + pc) * 16 * NP;
 #pragma unroll
-                for (int v = 0; v < 8; v++) {
-                    const R g = (gp[v] + acc[v]) / J;
-                    sG[(pc * 16 + grp * 8 + v) * NP + pk] = g;
-                    gs[(grp * 8 + v) * NP + pk] = g;
-                }
+                for (int v = 0; v < 8; v++) sG[(pc * 16 + grp * 8 + v) * NP + pk] = (gp[v] + acc[v]) / J;
             }
         }
         __syncthreads();
@@ -288,64 +361,22 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
             dst[gg] = acc;
         }
     }
-}
-
-// muAV(1,:) of one cell from its indicator and epsilon0 (AV.f90:120-172)
-__device__ __forceinline__ double mu_av(double sm, double eps0, double pi) {
-    const double se_low = -6.0, se_high = -1.0;
-    const double k_range = (se_high - se_low) * 0.5, s0 = (se_high + se_low) * 0.5;
-    if (sm < se_low) return 0.0;
-    if (sm > se_high) return eps0;
-    return eps0 / 2.0 * (1.0 + sin(pi * (sm - s0) / 2.0 / k_range));
-}
-
-// gradient of the 8 variables at flux point (ifp,jfp) of family fam: own trace
-// of nablaQs (nablaQ_interior_FP, compflux.f90:159-196)
-template <int N, class R>
-__device__ __forceinline__ void grad_trace(const R *gq, int fam, int ifp, int jfp, R *g, const R *Lt = nullptr) {
-    constexpr int NP = N * N;
+    // viscous / AV flux of the own traces at the interior flux points of both
+    // families (comp_viscous_flux, compflux.f90:205-256), for the stage kernel
+    R *fvi = a.fvi_r<R>();
+    for (int it = tid; it < ncl * 2 * NIT; it += blockDim.x) {
+        const int c = it / (2 * NIT), k2 = it % (2 * NIT), fam = k2 / NIT, k = k2 % NIT;
+        const int line = k / (N - 1), r = 1 + k % (N - 1);
+        const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+        double s1d, s2d;
+        metric_row_g<N, NG>(sX + c * 2 * NG, fam, ifp, jfp, s1d, s2d, sOps + OXF, sOps + OXS);
+        R Qown[NV], g[2 * NV], fv[NV];
+        trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown, sOps);
+        grad_trace<N, R>(sG + c * 16 * NP, fam, ifp, jfp, g, sOps);
+        visc_point<N, R, VM>(a, sMu + c * 4, fam, ifp, jfp, R(s1d), R(s2d), Qown, g, fv);
+        R *dst = fvi + ((size_t)(c0 + c) * 2 + fam) * NV * NIT + k;
 #pragma unroll
-    for (int d = 0; d < 2; d++) {
-#pragma unroll
-        for (int w = 0; w < 8; w++) g[d * NV + w] = R(0);
-        g[d * NV + 8] = R(0);
-    }
-    // contraction index outermost: 16 loads in flight per step
-    constexpr int U = sizeof(R) == 8 ? contraction_unroll<R, N>() : CHORUS_MX_GRAD_UNROLL;
-#pragma unroll U
-    for (int sp = 0; sp < N; sp++) {
-        const int row = fam == 0 ? ifp : jfp;
-        const R l = Lt ? Lt[row * N + sp] : Lop<R>(N, row, sp);
-        const int idx = fam == 0 ? jfp * N + sp : sp * N + ifp;
-#pragma unroll
-        for (int d = 0; d < 2; d++)
-#pragma unroll
-            for (int w = 0; w < 8; w++) g[d * NV + w] = g[d * NV + w] + gq[(d * 8 + w) * NP + idx] * l;
-    }
-}
-
-// viscous + AV contravariant flux at one flux point (comp_viscous_flux, compflux.f90:205-256)
-// VM: bit 0 physical viscosity/resistivity (mu or eta0 nonzero), bit 1 AV
-template <int N, class R, int VM>
-__device__ __forceinline__ void visc_point(const ViscArgs &v, const double *mu4, int fam, int ifp, int jfp, R s1,
-                                           R s2, const R *qv, const R *g, R *fv) {
-    const R gam = R(v.gam), lam = R(-2.0 / 3.0), prandt = R(0.72);
-    R Fv[NV], Gv[NV];
-    if constexpr ((VM & 1) != 0) {
-        vflux<R>(gam, lam, prandt, R(v.mu), R(v.eta0), qv, g, Fv, Gv);
-#pragma unroll
-        for (int w = 0; w < NV; w++) fv[w] = Fv[w] * s1 + Gv[w] * s2;
-    } else {
-#pragma unroll
-        for (int w = 0; w < NV; w++) fv[w] = R(0);
-    }
-    if constexpr ((VM & 2) != 0) {
-        const int lo = fam == 0 ? 3 : 0, hi = fam == 0 ? 1 : 2;   // sides 1 and 2 of the family
-        const double x = fam == 0 ? c_Xf[N][ifp] : c_Xf[N][jfp];
-        const R mu_art = R(mu4[lo] * (1.0 - x) + mu4[hi] * x);
-        vflux<R>(gam, lam, prandt, mu_art, R(0), qv, g, Fv, Gv);
-#pragma unroll
-        for (int w = 0; w < NV; w++) fv[w] = fv[w] + Fv[w] * s1 + Gv[w] * s2;
+        for (int w = 0; w < NV; w++) dst[w * NIT] = fv[w];
     }
 }
 
@@ -440,7 +471,7 @@ __global__ void __launch_bounds__(256) k_face_visc(FaceArgs a) {
 
 // ---------------------------------------------------------------------------
 template <int N, class R, int VM, int NG = 4>
-__global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(CHORUS_VISC_MINW))) k_stage_visc(StageArgs a, ViscArgs v) {
+__global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_waves_per_eu(CHORUS_VISC_MINW(N)))) k_stage_visc(StageArgs a, ViscArgs v) {
     using S = VShape<N>;
     using T = V2Shape<N>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ;
@@ -448,9 +479,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
     constexpr int UST = 20 * NF + 1;     // per-cell stride: F (8) + Fv (9) + A_z + metric rows (2)
     __shared__ double sQ[CPB * NQ];
     __shared__ double sX[CPB * 2 * NG];
-    __shared__ R sG[CPB * 16 * NP];      // nablaQs from k_grad
     __shared__ R sU[CPB * UST];          // [c][plane][line][r], one family at a time
-    __shared__ double sMu[CPB * 4];      // muAVfi/fj(1,side) per local face
     __shared__ double sAV[CPB * NP];     // AV indicator work
     __shared__ double sOps[Ops64Layout<N>::SIZE];   // FP64 operator tables (OpTables64 layout)
     constexpr int OM = Ops64Layout<N>::M, OXF = Ops64Layout<N>::XF, OXS = Ops64Layout<N>::XS;
@@ -461,22 +490,14 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
     const int ncl = min(CPB, a.ncell - c0);
     {
         constexpr int PQ = per_thread<CPB * NQ, T::THREADS>(), PX = per_thread<CPB * 2 * NG, T::THREADS>();
-        constexpr int PG = per_thread<CPB * 16 * NP, T::THREADS>(), PM = per_thread<CPB * 4, T::THREADS>();
-        double hq[PQ], hx[PX], hm[PM];
-        R hg[PG];
+        double hq[PQ], hx[PX];
         gather_regs<PQ, T::THREADS>(hq, a.Qin + (size_t)c0 * NQ, ncl * NQ);
         gather_regs<PX, T::THREADS>(hx, a.cellxy + (size_t)c0 * 2 * NG, ncl * 2 * NG);
-        gather_regs<PG, T::THREADS>(hg, v.gsol_r<R>() + (size_t)c0 * 16 * NP, ncl * 16 * NP);
         OpTables64<N, T::THREADS> ops;
         ops.gather();
-        // face averages of muAV(1) (compAVflux/interfaceAVflux/cyclocAVflux,
-        // AV.f90:199-303, 494-564), evaluated by k_face_visc
-        if constexpr ((VM & 2) != 0) gather_regs<PM, T::THREADS>(hm, v.mu4 + (size_t)c0 * 4, ncl * 4);
         scatter_lds<PQ, T::THREADS>(sQ, hq, ncl * NQ);
         scatter_lds<PX, T::THREADS>(sX, hx, ncl * 2 * NG);
-        scatter_lds<PG, T::THREADS>(sG, hg, ncl * 16 * NP);
         ops.scatter(sOps);
-        if constexpr ((VM & 2) != 0) scatter_lds<PM, T::THREADS>(sMu, hm, ncl * 4);
     }
     if constexpr ((VM & 2) != 0) {
         if (blockIdx.x == 0)
@@ -493,7 +514,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
     const int pt = tid - grp * T::G1;
     const int pc = pt / NP, pk = pt % NP, pjs = pk / N, pis = pk % N;
     const bool pact = (pt < CPB * NP) && (pc < ncl);
-    const R *FF = (const R *)a.FF, *FFv = (const R *)a.FFv;
+    const R *FF = (const R *)a.FF, *FFv = (const R *)a.FFv, *FVI = v.fvi_r<R>();
 
     R acc8[8], accv[NV], dA1 = R(0), dA2 = R(0);   // group 0: acc8 = dF; group 1: accv = dFv
     const size_t base = (size_t)(c0 + pc) * NQ + pk;
@@ -505,8 +526,8 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
             else if (t < T::S1) { kind = 2; item = t - T::NI; }
             else if (t < T::S1 + T::NI) { kind = 1; item = t - T::S1; }
             else { kind = 2; item = (T::S1 - T::NI) + (t - T::S1 - T::NI); }
-            if (kind != 2) {
-                // interior flux point
+            if (kind == 0) {
+                // interior flux point: inviscid flux of the own trace
                 const int c = item / NIT, k = item % NIT;
                 if (c >= ncl) continue;
                 const int line = k / (N - 1), r = 1 + k % (N - 1);
@@ -516,24 +537,23 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
                 R Qown[NV];
                 trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown, sOps);
                 R *u = sU + c * UST + line * (N + 1) + r;
-                if (kind == 0) {
-                    // inviscid flux
-                    const R s1 = R(s1d), s2 = R(s2d);
-                    R F[8], G[8];
-                    iflux<R>(R(a.gam), Qown, F, G);
+                const R s1 = R(s1d), s2 = R(s2d);
+                R F[8], G[8];
+                iflux<R>(R(a.gam), Qown, F, G);
 #pragma unroll
-                    for (int w = 0; w < 8; w++) u[w * NF] = F[w] * s1 + G[w] * s2;
-                    u[17 * NF] = Qown[8];
-                    u[18 * NF] = s1;
-                    u[19 * NF] = s2;
-                } else {
-                    // viscous / AV flux
-                    R g[2 * NV], fv[NV];
-                    grad_trace<N, R>(sG + c * 16 * NP, fam, ifp, jfp, g, sOps);
-                    visc_point<N, R, VM>(v, sMu + c * 4, fam, ifp, jfp, R(s1d), R(s2d), Qown, g, fv);
+                for (int w = 0; w < 8; w++) u[w * NF] = F[w] * s1 + G[w] * s2;
+                u[17 * NF] = Qown[8];
+                u[18 * NF] = s1;
+                u[19 * NF] = s2;
+            } else if (kind == 1) {
+                // interior flux point: the viscous / AV flux formed by k_grad
+                const int c = item / NIT, k = item % NIT;
+                if (c >= ncl) continue;
+                const int line = k / (N - 1), r = 1 + k % (N - 1);
+                const R *src = FVI + ((size_t)(c0 + c) * 2 + fam) * NV * NIT + k;
+                R *u = sU + c * UST + line * (N + 1) + r;
 #pragma unroll
-                    for (int w = 0; w < NV; w++) u[(8 + w) * NF] = fv[w];
-                }
+                for (int w = 0; w < NV; w++) u[(8 + w) * NF] = src[w * NIT];
             } else {
                 // face point: the stored common inviscid, A_z and viscous fluxes
                 const int c = item / (2 * N), k = item % (2 * N);
