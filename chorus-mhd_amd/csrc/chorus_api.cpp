@@ -55,7 +55,7 @@ struct chorus_ctx {
     double *FA = nullptr;          // mixed form: per cell face point: FP64 upwind A_z
     void *FFv = nullptr;           // viscous path: per cell face point: common viscous flux
     void *gtr = nullptr;           // viscous path: own nablaQs face traces
-    void *gsol = nullptr;          // viscous path: nablaQs at the solution points (mixed), interior viscous fluxes (FP64)
+    void *fvi = nullptr;           // viscous path: the interior viscous/AV fluxes of the stage
     double *sm = nullptr;          // viscous path: AV indicator per cell
     double *mu4 = nullptr;         // viscous path: face-averaged muAV per cell and local face
     double *eig = nullptr;         // two slots (of ninst values) of the per-stage maximum wave speed
@@ -165,8 +165,7 @@ int launch_stage(chorus_ctx *c, const StageArgs &a) {
         v.nbr = c->nbr;
         v.Qin = a.Qin;
         v.gtr = c->gtr;
-        v.gsol = c->gsol;
-        v.fvi = c->gsol;
+        v.fvi = c->fvi;
         v.inst = c->inst;
         v.hinst = c->hinst;
         v.sm = c->sm;
@@ -642,7 +641,7 @@ int init_ctx(chorus_ctx *c, const chorus_mesh *m, const chorus_params *p, const 
     if (!c->ideal) {
         const size_t nf = nfp;
         HIP_TRY(c, hipMalloc(&c->gtr, nf * 16 * sizeof(double)));
-        HIP_TRY(c, hipMalloc(&c->gsol, (size_t)c->ncell * 16 * c->N * c->N * sizeof(double)));
+        HIP_TRY(c, hipMalloc(&c->fvi, (size_t)c->ncell * 2 * 9 * c->N * (c->N - 1) * sizeof(double)));
         HIP_TRY(c, hipMalloc(&c->sm, (size_t)c->ncell * sizeof(double)));
         HIP_TRY(c, hipMalloc(&c->mu4, (size_t)c->ncell * 4 * sizeof(double)));
     }
@@ -669,7 +668,7 @@ void chorus_free(chorus_ctx *c) {
     (void)hipFree(c->FA);
     (void)hipFree(c->FFv);
     (void)hipFree(c->gtr);
-    (void)hipFree(c->gsol);
+    (void)hipFree(c->fvi);
     (void)hipFree(c->sm);
     (void)hipFree(c->mu4);
     (void)hipFree(c->inst);

@@ -116,8 +116,7 @@ struct ViscArgs {
     const int2 *nbr;
     const double *Qin;
     void *gtr;          // own nablaQs traces [cell][lf][nfp][16]  (Real)
-    void *gsol;         // mixed form: nablaQs at the solution points [cell][16][N*N] (float)
-    void *fvi;          // FP64 form: interior viscous/AV fluxes [cell][fam][9][N*(N-1)] (the gsol allocation)
+    void *fvi;          // interior viscous/AV fluxes [cell][fam][9][N*(N-1)] (Real), written by the gradient kernel
     const int *inst;    // batched instances: instance of each cell (null: one instance)
     const double *hinst;   // batched instances: AV length scale per instance
     double *sm;         // log10 smoothness indicator of density per cell
@@ -130,7 +129,6 @@ struct ViscArgs {
     int artiv;
     double mu, eta0, gam, h_msh, pi;
     template <class R> __host__ __device__ R *gtr_r() const { return (R *)gtr; }
-    template <class R> __host__ __device__ R *gsol_r() const { return (R *)gsol; }
     template <class R> __host__ __device__ R *fvi_r() const { return (R *)fvi; }
 };
 

@@ -52,7 +52,7 @@ template <int n, int TBV, int VM>
 static hipError_t visc_stage(const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s) {
     using S = CellShape<n, TBV>;
     hipLaunchKernelGGL((k_face_inv_mx<n>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);
-    hipLaunchKernelGGL((k_grad_mx<n, TBV>), dim3(blocks_of<S>(a.ncell)), dim3(V2ShapeT<n, TBV>::THREADS), 0, s, v);
+    hipLaunchKernelGGL((k_grad_mx<n, TBV, VM>), dim3(blocks_of<S>(a.ncell)), dim3(V2ShapeT<n, TBV>::THREADS), 0, s, v);
     hipLaunchKernelGGL((k_face_visc_mx<n, VM>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);
     hipLaunchKernelGGL((k_stage_visc_mx<n, TBV, VM>), dim3(blocks_of<S>(a.ncell)), dim3(MxVShape<n, TBV>::THREADS), 0, s, a, v);
     return hipGetLastError();

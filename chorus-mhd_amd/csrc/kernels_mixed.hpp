@@ -518,9 +518,11 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
 // deviation from the cell's reference state (own delta trace plus half the jump of the two
 // FP64 traces at face points); nablaQs, its traces and the viscous fluxes are
 // plain FP32 (they carry no large constant part); the viscous face flux is
-// evaluated once per face point from the FP64 trace average.
+// evaluated once per face point from the FP64 trace average.  k_grad_mx also
+// forms the viscous and AV fluxes of the interior flux points, which the stage
+// kernel copies (fvi, as in the FP64 path).
 // ---------------------------------------------------------------------------
-template <int N, int TB>
+template <int N, int TB, int VM>
 __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs a) {
     using S = CellShape<N, TB>;
     using TS = V2ShapeT<N, TB>;
@@ -533,6 +535,7 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
     __shared__ double sBar[CPB * NV];
     __shared__ Ref sRef[CPB];
     __shared__ int2 sNbr[CPB * 4];
+    __shared__ double sMu[CPB * 4];      // muAVfi/fj(1,side) per local face
     __shared__ float sLf[(N + 1) * N], sMf[(N + 1) * N];
     __shared__ double sOps[Ops64Layout<N>::SIZE];   // FP64 operator tables (OpTables64 layout)
     float *sD = (float *)U, *sV = sD + CPB * NQ, *sG = sV + CPB * VST;
@@ -556,6 +559,17 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
         ops.scatter(sLf, sMf);
         ops64.scatter(sOps);
         __syncthreads();
+    }
+    // face averages of muAV(1) of the block's local faces, as in k_grad
+    if constexpr ((VM & 2) != 0) {
+        for (int i = tid; i < ncl * 4; i += T) {
+            const int cell = c0 + i / 4;
+            const int2 info = sNbr[i];
+            const int in = a.inst ? a.inst[cell] : 0;
+            const double eps0 = a.eig[in] * (a.inst ? a.hinst[in] : a.h_msh) / (double)(N - 1) * 1.0;
+            const double mo = mu_av(a.sm[cell], eps0, a.pi);
+            sMu[i] = ((info.y >> 4) & 1) ? mo : 0.5 * (mo + mu_av(a.sm[info.x], eps0, a.pi));
+        }
     }
     const int grp = tid >= TS::G1 ? 1 : 0;
     const int pt = tid - grp * TS::G1;
@@ -674,7 +688,6 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
 #endif
         __syncthreads();
     }
-    for (int i = tid; i < ncl * 16 * NP; i += T) a.gsol_r<float>()[(size_t)c0 * 16 * NP + i] = sG[i];
     // own face traces of nablaQs: a thread per face point and half of the 16
     // components, the L row in registers
     for (int it = tid; it < ncl * 4 * N * 2; it += T) {
@@ -696,6 +709,25 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
             for (int sp = 0; sp < N; sp++) acc += gq[sp * stride] * Lr[sp];
             dst[gg] = acc;
         }
+    }
+    // viscous / AV flux of the own traces at the interior flux points of both
+    // families (comp_viscous_flux, compflux.f90:205-256), for the stage kernel
+    float *fvi = a.fvi_r<float>();
+    for (int it = tid; it < ncl * 2 * NIT; it += T) {
+        const int c = it / (2 * NIT), k2 = it % (2 * NIT), fam = k2 / NIT, k = k2 % NIT;
+        const int line = k / (N - 1), r = 1 + k % (N - 1);
+        const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+        double s1d, s2d;
+        metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d, sOps + Ops64Layout<N>::XF, sOps + Ops64Layout<N>::XS);
+        float d[NV], qv[NV], g[2 * NV], fv[NV];
+        dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d, sLf);
+#pragma unroll
+        for (int w = 0; w < NV; w++) qv[w] = sRef[c].q[w] + d[w];
+        grad_trace<N, float>(sG + c * 16 * NP, fam, ifp, jfp, g, sLf);
+        visc_point<N, float, VM>(a, sMu + c * 4, fam, ifp, jfp, float(s1d), float(s2d), qv, g, fv);
+        float *dst = fvi + ((size_t)(c0 + c) * 2 + fam) * NV * NIT + k;
+#pragma unroll
+        for (int w = 0; w < NV; w++) dst[w * NIT] = fv[w];
     }
 }
 
@@ -804,20 +836,19 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ, T = TS::THREADS, NIT = TS::NIT;
     constexpr int UST = 17 * NF + 1;     // dF1 (8) + Fv1 (9)
     constexpr int AST = NF + 1;          // per-cell stride of the FP64 A_z flux-point values
-    constexpr int UF = CPB * NQ + CPB * UST + CPB * 16 * NP;
+    constexpr int UF = CPB * NQ + CPB * UST;
     constexpr int UD = (UF + 1) / 2;
     __shared__ double U[UD];
     __shared__ double sQ64[CPB * NQ];    // FP64 state (combined in place into the new state)
     __shared__ double sX[CPB * 8];
     __shared__ double sBar[CPB * NV];
     __shared__ Ref sRef[CPB];
-    __shared__ double sMu[CPB * 4];
     __shared__ double sAV[CPB * NP];
     __shared__ double sAz[CPB * NP];     // FP64 A_z plane of each cell
     __shared__ double sA[2 * CPB * AST]; // FP64 A_z at the flux points of both families
     __shared__ float sLf[(N + 1) * N], sMf[(N + 1) * N];
     __shared__ double sOps[Ops64Layout<N>::SIZE];   // FP64 operator tables (OpTables64 layout)
-    float *sD = (float *)U, *sU = sD + CPB * NQ, *sG = sU + CPB * UST;
+    float *sD = (float *)U, *sU = sD + CPB * NQ;
     static_assert(9 * CPB * NP <= CPB * UST, "exchange area");
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -829,21 +860,15 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     }
     {
         // the per-cell arrays are loaded with the state (one memory latency)
-        constexpr int PG = per_thread<CPB * 16 * NP, T>(), PX = per_thread<CPB * 8, T>();
-        constexpr int PM = per_thread<CPB * 4, T>();
-        float hg[PG];
-        double hx[PX], hm[PM];
+        constexpr int PX = per_thread<CPB * 8, T>();
+        double hx[PX];
         OpTables<N, T> ops;
         ops.gather();
         OpTables64<N, T> ops64;
         ops64.gather();
-        gather_regs<PG, T>(hg, v.gsol_r<float>() + (size_t)c0 * 16 * NP, ncl * 16 * NP);
         gather_regs<PX, T>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
-        if constexpr ((VM & 2) != 0) gather_regs<PM, T>(hm, v.mu4 + (size_t)c0 * 4, ncl * 4);
         stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, sBar, sRef, sD, sAz, 8, sQ64);
-        scatter_lds<PG, T>(sG, hg, ncl * 16 * NP);
         scatter_lds<PX, T>(sX, hx, ncl * 8);
-        if constexpr ((VM & 2) != 0) scatter_lds<PM, T>(sMu, hm, ncl * 4);
         ops.scatter(sLf, sMf);
         ops64.scatter(sOps);
     }
@@ -858,7 +883,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     const int pc = pt / NP, pk = pt % NP, pjs = pk / N, pis = pk % N;
     const bool pact = (pt < CPB * NP) && (pc < ncl);
     const size_t base = (size_t)(c0 + pc) * NQ + pk;
-    const float *FF = (const float *)a.FF, *FFv = (const float *)a.FFv;
+    const float *FF = (const float *)a.FF, *FFv = (const float *)a.FFv, *FVI = v.fvi_r<float>();
     float acc8[8], accv[NV];
     double qa[NV];
 #pragma unroll 1
@@ -894,23 +919,15 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
                 for (int w = 0; w < 8; w++) u[w * NF] = F[w] * s1 + G[w] * s2 + jump[w];
                 sA[(fam * CPB + c) * AST + line * (N + 1) + r] = az_at<N>(a.FA, sAz, c0 + c, c, fam, r, line, sOps);
             } else if (t >= TS::S1) {
-                // viscous / AV flux of the own trace at an interior point
+                // viscous / AV flux at an interior point, formed by k_grad_mx
                 const int item = t - TS::S1;
                 const int c = item / NIT, k = item % NIT;
                 if (c >= ncl) continue;
                 const int line = k / (N - 1), r = 1 + k % (N - 1);
-                const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
-                double s1d, s2d;
-                metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d, sOps + Ops64Layout<N>::XF, sOps + Ops64Layout<N>::XS);
-                float d[NV], qv[NV], g[2 * NV], fv[NV];
-                dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d, sLf);
-#pragma unroll
-                for (int w = 0; w < NV; w++) qv[w] = sRef[c].q[w] + d[w];
-                grad_trace<N, float>(sG + c * 16 * NP, fam, ifp, jfp, g, sLf);
-                visc_point<N, float, VM>(v, sMu + c * 4, fam, ifp, jfp, float(s1d), float(s2d), qv, g, fv);
+                const float *src = FVI + ((size_t)(c0 + c) * 2 + fam) * NV * NIT + k;
                 float *u = sU + c * UST + line * (N + 1) + r;
 #pragma unroll
-                for (int w = 0; w < NV; w++) u[(8 + w) * NF] = fv[w];
+                for (int w = 0; w < NV; w++) u[(8 + w) * NF] = src[w * NIT];
             }
         }
         __syncthreads();
