@@ -725,7 +725,6 @@ int chorus_compresid(chorus_ctx *c, double *resid) {
 
 int chorus_set_nodes8(chorus_ctx *c, const double *xm, const double *ym) {
     if (!c || !xm || !ym) return fail(c, CHORUS_ERR_ARG, "null argument");
-    if (c->p.precision != CHORUS_FP64) return fail(c, CHORUS_ERR_ARG, "8-node cells need the FP64 form");
     if (c->plot.t) return fail(c, CHORUS_ERR_ARG, "Tecplot output of 8-node cells is not supported");
     const size_t nc = (size_t)c->ncell;
     for (size_t i = 0; i < 4 * nc; i++)

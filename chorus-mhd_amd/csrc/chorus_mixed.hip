@@ -18,21 +18,28 @@ static int fblocks(int N, int nface) { return (nface * N + 255) / 256; }
 //   tile 0: 256 / 128, tile 1: 128 / 64, tile 2: 512 / 256 (default)
 int tile_count() { return 3; }
 
-template <int n, int TBI>
+template <int n, int TBI, int NG>
 static hipError_t ideal_stage(const StageArgs &a, const FaceArgs &f, hipStream_t s) {
     using S = FluxCellShape<n, TBI>;
-    hipLaunchKernelGGL((k_face_inv_mx<n>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);
-    hipLaunchKernelGGL((k_stage_ideal_mx<n, TBI>), dim3(blocks_of<S>(a.ncell)), dim3(S::THREADS), 0, s, a);
+    hipLaunchKernelGGL((k_face_inv_mx<n, NG>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);
+    hipLaunchKernelGGL((k_stage_ideal_mx<n, TBI, NG>), dim3(blocks_of<S>(a.ncell)), dim3(S::THREADS), 0, s, a);
     return hipGetLastError();
 }
 
+template <int n, int NG>
+static hipError_t ideal_stage_ng(const StageArgs &a, const FaceArgs &f, hipStream_t s, int tile) {
+    switch (tile) {
+    case 1: return ideal_stage<n, 128, NG>(a, f, s);
+    case 2: return ideal_stage<n, 512, NG>(a, f, s);
+    default: return ideal_stage<n, 256, NG>(a, f, s);
+    }
+}
+
+// cells with 8 geometry nodes (curved edges) take the NG = 8 instantiations
 template <int n>
 static hipError_t ideal_stage_tile(const StageArgs &a, const FaceArgs &f, hipStream_t s, int tile) {
-    switch (tile) {
-    case 1: return ideal_stage<n, 128>(a, f, s);
-    case 2: return ideal_stage<n, 512>(a, f, s);
-    default: return ideal_stage<n, 256>(a, f, s);
-    }
+    if ((a.ng == 8) != (f.ng == 8)) return hipErrorInvalidValue;
+    return a.ng == 8 ? ideal_stage_ng<n, 8>(a, f, s, tile) : ideal_stage_ng<n, 4>(a, f, s, tile);
 }
 
 hipError_t launch_stage_ideal_mixed(int N, const StageArgs &a, const FaceArgs &f, hipStream_t s, int tile) {
@@ -48,25 +55,33 @@ hipError_t launch_stage_ideal_mixed(int N, const StageArgs &a, const FaceArgs &f
     }
 }
 
-template <int n, int TBV, int VM>
+template <int n, int TBV, int VM, int NG>
 static hipError_t visc_stage(const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s) {
     using S = CellShape<n, TBV>;
-    hipLaunchKernelGGL((k_face_inv_mx<n>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);
-    hipLaunchKernelGGL((k_grad_mx<n, TBV, VM>), dim3(blocks_of<S>(a.ncell)), dim3(V2ShapeT<n, TBV>::THREADS), 0, s, v);
-    hipLaunchKernelGGL((k_face_visc_mx<n, VM>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);
-    hipLaunchKernelGGL((k_stage_visc_mx<n, TBV, VM>), dim3(blocks_of<S>(a.ncell)), dim3(MxVShape<n, TBV>::THREADS), 0, s, a, v);
+    hipLaunchKernelGGL((k_face_inv_mx<n, NG>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);
+    hipLaunchKernelGGL((k_grad_mx<n, TBV, VM, NG>), dim3(blocks_of<S>(a.ncell)), dim3(V2ShapeT<n, TBV>::THREADS), 0, s,
+                       v);
+    hipLaunchKernelGGL((k_face_visc_mx<n, VM, NG>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);
+    hipLaunchKernelGGL((k_stage_visc_mx<n, TBV, VM, NG>), dim3(blocks_of<S>(a.ncell)), dim3(MxVShape<n, TBV>::THREADS),
+                       0, s, a, v);
     return hipGetLastError();
+}
+
+template <int n, int TBV, int NG>
+static hipError_t visc_stage_vm_ng(const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s) {
+    const int vm = ((v.mu != 0.0 || v.eta0 != 0.0) ? 1 : 0) | (v.artiv ? 2 : 0);
+    switch (vm) {
+    case 1: return visc_stage<n, TBV, 1, NG>(a, v, f, s);
+    case 2: return visc_stage<n, TBV, 2, NG>(a, v, f, s);
+    case 3: return visc_stage<n, TBV, 3, NG>(a, v, f, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 template <int n, int TBV>
 static hipError_t visc_stage_vm(const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s) {
-    const int vm = ((v.mu != 0.0 || v.eta0 != 0.0) ? 1 : 0) | (v.artiv ? 2 : 0);
-    switch (vm) {
-    case 1: return visc_stage<n, TBV, 1>(a, v, f, s);
-    case 2: return visc_stage<n, TBV, 2>(a, v, f, s);
-    case 3: return visc_stage<n, TBV, 3>(a, v, f, s);
-    default: return hipErrorInvalidValue;
-    }
+    if ((a.ng == 8) != (f.ng == 8)) return hipErrorInvalidValue;
+    return a.ng == 8 ? visc_stage_vm_ng<n, TBV, 8>(a, v, f, s) : visc_stage_vm_ng<n, TBV, 4>(a, v, f, s);
 }
 
 template <int n>

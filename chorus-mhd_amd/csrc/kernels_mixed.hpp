@@ -227,9 +227,25 @@ __device__ __forceinline__ void dtrace9(const float *dq, int fam, int ifp, int j
 //   fam 1: s = (-y_psi, x_psi),  s(0) = (y1-y2, x2-x1), s(N) = (y4-y3, x3-x4)
 // so sum_r A(r) M(r,pos) s(r) = s(0) T0 + s(N) T1 with
 // T0 = sum_r A(r) M(r,pos) (1 - Xf(r)), T1 = sum_r A(r) M(r,pos) Xf(r).
-template <int N>
-__device__ __forceinline__ void az_derivative(const double *A, const double *xy, int fam, int pos, double &e1,
-                                              double &e2, const double *ops) {
+// The metric rows of an 8-node cell (NG = 8) are quadratic along a line: there
+// the products are summed point by point, with s(r) of the 8-node map.
+template <int N, int NG = 4>
+__device__ __forceinline__ void az_derivative(const double *A, const double *xy, int fam, int pos, int line,
+                                              double &e1, double &e2, const double *ops) {
+    if constexpr (NG == 8) {
+        e1 = 0.0;
+        e2 = 0.0;
+#pragma unroll
+        for (int r = 0; r <= N; r++) {
+            const double am = A[r] * ops[Ops64Layout<N>::M + r * N + pos];
+            double s1, s2;
+            metric_row_g<N, 8>(xy, fam, fam == 0 ? r : line, fam == 0 ? line : r, s1, s2,
+                               ops + Ops64Layout<N>::XF, ops + Ops64Layout<N>::XS);
+            e1 += am * s1;
+            e2 += am * s2;
+        }
+        return;
+    }
     double T0 = 0.0, T1 = 0.0;
 #pragma unroll
     for (int r = 0; r <= N; r++) {
@@ -251,12 +267,12 @@ __device__ __forceinline__ void az_derivative(const double *A, const double *xy,
 // -(u J dA/dx + v J dA/dy) at solution point (pis, pjs) from the FP64 A_z
 // flux-point values A[fam * FAMST + line * (N + 1) + r] of the point's cell
 // (magnetic_potential.f90:1-42, compresid.f90:40-47); q is the point's state
-template <int N, int FAMST>
+template <int N, int FAMST, int NG = 4>
 __device__ __forceinline__ double az_residual(const double *A, const double *xy, int pis, int pjs, const double *q,
                                               int qstride, const double *ops) {
     double e1a, e2a, e1b, e2b;
-    az_derivative<N>(A + pjs * (N + 1), xy, 0, pis, e1a, e2a, ops);
-    az_derivative<N>(A + FAMST + pis * (N + 1), xy, 1, pjs, e1b, e2b, ops);
+    az_derivative<N, NG>(A + pjs * (N + 1), xy, 0, pis, pjs, e1a, e2a, ops);
+    az_derivative<N, NG>(A + FAMST + pis * (N + 1), xy, 1, pjs, pis, e1b, e2b, ops);
     const double au = q[qstride] / q[0], av = q[2 * qstride] / q[0];
     return -(au * (e1a + e1b) + av * (e2a + e2b));
 }
@@ -280,7 +296,7 @@ __device__ __forceinline__ double az_at(const double *FA, const double *sAz, int
 // of the FP32 wave speed with ARTIV.  A FREE face has no jump (zeros).
 // FF[cell][lfi][nfp][9] (float): 8 flux jump terms + A_z jump.
 // ---------------------------------------------------------------------------
-template <int N>
+template <int N, int NG = 4>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_face_inv_mx(FaceArgs a) {
     // the block's 256 face points move through LDS as in k_face_inv: traces in
     // with every load of a thread issued first, jump terms out, both with
@@ -307,11 +323,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         free_face = fc.z < 0;
         int ifp, jfp;
         face2fp<N>(fc.y + 1, k, ifp, jfp);
-        double xy[8];
+        double xy[2 * NG];
 #pragma unroll
-        for (int q = 0; q < 8; q++) xy[q] = a.cellxy[(size_t)fc.x * 8 + q];
+        for (int q = 0; q < 2 * NG; q++) xy[q] = a.cellxy[(size_t)fc.x * (2 * NG) + q];
         double m1, m2;
-        metric_row<N>(xy, lf_family(fc.y + 1), ifp, jfp, m1, m2);
+        metric_row_g<N, NG>(xy, lf_family(fc.y + 1), ifp, jfp, m1, m2);
         const float s1 = float(m1), s2 = float(m2);
         // the right state: the other cell's trace; at an INLE / SYMP / WALL face
         // that of the boundary kind; at a FREE / OUTL face the own trace
@@ -405,7 +421,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 #ifndef CHORUS_MX_IDEAL_MINW
 #define CHORUS_MX_IDEAL_MINW(N) ((N) <= 5 ? 4 : 3)
 #endif
-template <int N, int TB>
+template <int N, int TB, int NG = 4>
 __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__((amdgpu_waves_per_eu(CHORUS_MX_IDEAL_MINW(N)))) k_stage_ideal_mx(StageArgs a) {
     using S = FluxCellShape<N, TB>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ, T = S::THREADS;
@@ -415,7 +431,7 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
     constexpr int UD = (UF + 1) / 2;
     __shared__ double U[UD];
     __shared__ double sQ64[CPB * NQ];    // FP64 state (combined in place into the new state)
-    __shared__ double sX[CPB * 8];
+    __shared__ double sX[CPB * 2 * NG];
     __shared__ double sBar[CPB * NV];
     __shared__ Ref sRef[CPB];
     __shared__ double sAz[CPB * NP];     // FP64 A_z plane of each cell
@@ -429,15 +445,15 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
     const int ncl = min(CPB, a.ncell - c0);
     {
         // the small per-cell arrays are loaded with the state (one memory latency)
-        constexpr int PX = per_thread<CPB * 8, T>();
+        constexpr int PX = per_thread<CPB * 2 * NG, T>();
         double hx[PX];
         OpTables<N, T> ops;
-        gather_regs<PX, T>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
+        gather_regs<PX, T>(hx, a.cellxy + (size_t)c0 * 2 * NG, ncl * 2 * NG);
         ops.gather();
         OpTables64<N, T> ops64;
         ops64.gather();
         stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, sBar, sRef, sD, sAz, 8, sQ64);
-        scatter_lds<PX, T>(sX, hx, ncl * 8);
+        scatter_lds<PX, T>(sX, hx, ncl * 2 * NG);
         ops.scatter(sLf, sMf);
         ops64.scatter(sOps);
         __syncthreads();
@@ -466,7 +482,7 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
                 for (int v = 0; v < 8; v++) jump[v] = src[v];
             }
             double s1d, s2d;
-            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d, sOps + Ops64Layout<N>::XF, sOps + Ops64Layout<N>::XS);
+            metric_row_g<N, NG>(sX + c * 2 * NG, fam, ifp, jfp, s1d, s2d, sOps + Ops64Layout<N>::XF, sOps + Ops64Layout<N>::XS);
             const float s1 = float(s1d), s2 = float(s2d);
             float d[NV], F[8], G[8];
             dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d, sLf);
@@ -499,12 +515,12 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
     // A_z residual in FP64 from the flux-point values of both families
     double res8 = 0.0;
     if (pact) {
-        const double iJd = 1.0 / jacobian<N>(sX + pc * 8, pis, pjs, sOps + Ops64Layout<N>::XS);
+        const double iJd = 1.0 / jacobian_g<N, NG>(sX + pc * 2 * NG, pis, pjs, sOps + Ops64Layout<N>::XS);
         const float iJ = float(iJd);
 #pragma unroll
         for (int v = 0; v < 8; v++) res[v] = dF[v] * iJ;
         res[8] = 0.f;
-        res8 = az_residual<N, CPB * AST>(sA + pc * AST, sX + pc * 8, pis, pjs, sQ64 + pc * NQ + pk, NP, sOps) * iJd;
+        res8 = az_residual<N, CPB * AST, NG>(sA + pc * AST, sX + pc * 2 * NG, pis, pjs, sQ64 + pc * NQ + pk, NP, sOps) * iJd;
     }
     if (pact) combine_point(a, base, NP, sQ64 + pc * NQ + pk, qa, res, res8);
     if (a.kind == KIND_RESID) return;
@@ -522,7 +538,7 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
 // forms the viscous and AV fluxes of the interior flux points, which the stage
 // kernel copies (fvi, as in the FP64 path).
 // ---------------------------------------------------------------------------
-template <int N, int TB, int VM>
+template <int N, int TB, int VM, int NG = 4>
 __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs a) {
     using S = CellShape<N, TB>;
     using TS = V2ShapeT<N, TB>;
@@ -531,7 +547,7 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
     constexpr int UF = CPB * NQ + CPB * VST + CPB * 16 * NP;
     constexpr int UD = (UF + 1) / 2 > CPB * NQ ? (UF + 1) / 2 : CPB * NQ;
     __shared__ double U[UD];
-    __shared__ double sX[CPB * 8];
+    __shared__ double sX[CPB * 2 * NG];
     __shared__ double sBar[CPB * NV];
     __shared__ Ref sRef[CPB];
     __shared__ int2 sNbr[CPB * 4];
@@ -544,17 +560,17 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
     const int c0 = blk * CPB;
     const int ncl = min(CPB, a.ncell - c0);
     {
-        constexpr int PX = per_thread<CPB * 8, T>(), PN = per_thread<CPB * 4, T>();
+        constexpr int PX = per_thread<CPB * 2 * NG, T>(), PN = per_thread<CPB * 4, T>();
         double hx[PX];
         int2 hn[PN];
         OpTables<N, T> ops;
-        gather_regs<PX, T>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
+        gather_regs<PX, T>(hx, a.cellxy + (size_t)c0 * 2 * NG, ncl * 2 * NG);
         gather_regs<PN, T>(hn, a.nbr + (size_t)c0 * 4, ncl * 4);
         ops.gather();
         OpTables64<N, T> ops64;
         ops64.gather();
         stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, sBar, sRef, sD, nullptr);
-        scatter_lds<PX, T>(sX, hx, ncl * 8);
+        scatter_lds<PX, T>(sX, hx, ncl * 2 * NG);
         scatter_lds<PN, T>(sNbr, hn, ncl * 4);
         ops.scatter(sLf, sMf);
         ops64.scatter(sOps);
@@ -581,7 +597,7 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
     __shared__ float sIJ[CPB * NP];
     for (int i = tid; i < ncl * NP; i += T) {
         const int k = i % NP;
-        sIJ[i] = float(1.0 / jacobian<N>(sX + (i / NP) * 8, k % N, k / N, sOps + Ops64Layout<N>::XS));
+        sIJ[i] = float(1.0 / jacobian_g<N, NG>(sX + (i / NP) * 2 * NG, k % N, k / N, sOps + Ops64Layout<N>::XS));
     }
 #endif
 #pragma unroll 1
@@ -615,7 +631,7 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
             if (c >= ncl) continue;
             const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
             double s1d, s2d;
-            metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d, sOps + Ops64Layout<N>::XF, sOps + Ops64Layout<N>::XS);
+            metric_row_g<N, NG>(sX + c * 2 * NG, fam, ifp, jfp, s1d, s2d, sOps + Ops64Layout<N>::XF, sOps + Ops64Layout<N>::XS);
             float d[NV];
             dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d, sLf);
             float *q = sV + c * VST + line * (N + 1) + r;
@@ -680,7 +696,7 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
 #pragma unroll
                 for (int v = 0; v < 8; v++) gp[v] = acc[v];
             } else {
-                const float iJ = float(1.0 / jacobian<N>(sX + pc * 8, pis, pjs, sOps + Ops64Layout<N>::XS));
+                const float iJ = float(1.0 / jacobian_g<N, NG>(sX + pc * 2 * NG, pis, pjs, sOps + Ops64Layout<N>::XS));
 #pragma unroll
                 for (int v = 0; v < 8; v++) sG[(pc * 16 + grp * 8 + v) * NP + pk] = (gp[v] + acc[v]) * iJ;
             }
@@ -718,7 +734,7 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
         const int line = k / (N - 1), r = 1 + k % (N - 1);
         const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
         double s1d, s2d;
-        metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d, sOps + Ops64Layout<N>::XF, sOps + Ops64Layout<N>::XS);
+        metric_row_g<N, NG>(sX + c * 2 * NG, fam, ifp, jfp, s1d, s2d, sOps + Ops64Layout<N>::XF, sOps + Ops64Layout<N>::XS);
         float d[NV], qv[NV], g[2 * NV], fv[NV];
         dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d, sLf);
 #pragma unroll
@@ -733,7 +749,7 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
 
 // viscous and AV face flux, mixed form: k_face_visc in FP32 from the FP64
 // trace average
-template <int N, int VM>
+template <int N, int VM, int NG = 4>
 __global__ void __launch_bounds__(256) k_face_visc_mx(FaceArgs a) {
     const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= a.nface * N) return;
@@ -744,17 +760,17 @@ __global__ void __launch_bounds__(256) k_face_visc_mx(FaceArgs a) {
     float sl1, sl2, sr1 = 0.f, sr2 = 0.f;
     {
         int ifp, jfp;
-        double xy[8], m1, m2;
+        double xy[2 * NG], m1, m2;
         face2fp<N>(fc.y + 1, k, ifp, jfp);
 #pragma unroll
-        for (int q = 0; q < 8; q++) xy[q] = a.cellxy[(size_t)fc.x * 8 + q];
-        metric_row<N>(xy, lf_family(fc.y + 1), ifp, jfp, m1, m2);
+        for (int q = 0; q < 2 * NG; q++) xy[q] = a.cellxy[(size_t)fc.x * (2 * NG) + q];
+        metric_row_g<N, NG>(xy, lf_family(fc.y + 1), ifp, jfp, m1, m2);
         sl1 = float(m1); sl2 = float(m2);
         if (!free_face) {
             face2fp<N>(fc.w + 1, kr, ifp, jfp);
 #pragma unroll
-            for (int q = 0; q < 8; q++) xy[q] = a.cellxy[(size_t)fc.z * 8 + q];
-            metric_row<N>(xy, lf_family(fc.w + 1), ifp, jfp, m1, m2);
+            for (int q = 0; q < 2 * NG; q++) xy[q] = a.cellxy[(size_t)fc.z * (2 * NG) + q];
+            metric_row_g<N, NG>(xy, lf_family(fc.w + 1), ifp, jfp, m1, m2);
             sr1 = float(m1); sr2 = float(m2);
         }
     }
@@ -829,7 +845,7 @@ template <int N, int TB> struct MxVShape {
 #ifndef CHORUS_MX_VISC_MINW
 #define CHORUS_MX_VISC_MINW(N) ((N) <= 6 ? 4 : 3)
 #endif
-template <int N, int TB, int VM>
+template <int N, int TB, int VM, int NG = 4>
 __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amdgpu_waves_per_eu(CHORUS_MX_VISC_MINW(N)))) k_stage_visc_mx(StageArgs a, ViscArgs v) {
     using S = CellShape<N, TB>;
     using TS = MxVShape<N, TB>;
@@ -840,7 +856,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     constexpr int UD = (UF + 1) / 2;
     __shared__ double U[UD];
     __shared__ double sQ64[CPB * NQ];    // FP64 state (combined in place into the new state)
-    __shared__ double sX[CPB * 8];
+    __shared__ double sX[CPB * 2 * NG];
     __shared__ double sBar[CPB * NV];
     __shared__ Ref sRef[CPB];
     __shared__ double sAV[CPB * NP];
@@ -860,15 +876,15 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     }
     {
         // the per-cell arrays are loaded with the state (one memory latency)
-        constexpr int PX = per_thread<CPB * 8, T>();
+        constexpr int PX = per_thread<CPB * 2 * NG, T>();
         double hx[PX];
         OpTables<N, T> ops;
         ops.gather();
         OpTables64<N, T> ops64;
         ops64.gather();
-        gather_regs<PX, T>(hx, a.cellxy + (size_t)c0 * 8, ncl * 8);
+        gather_regs<PX, T>(hx, a.cellxy + (size_t)c0 * 2 * NG, ncl * 2 * NG);
         stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, sBar, sRef, sD, sAz, 8, sQ64);
-        scatter_lds<PX, T>(sX, hx, ncl * 8);
+        scatter_lds<PX, T>(sX, hx, ncl * 2 * NG);
         ops.scatter(sLf, sMf);
         ops64.scatter(sOps);
     }
@@ -910,7 +926,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
                     for (int w = 0; w < NV; w++) u[(8 + w) * NF] = FFv[o + w];
                 }
                 double s1d, s2d;
-                metric_row<N>(sX + c * 8, fam, ifp, jfp, s1d, s2d, sOps + Ops64Layout<N>::XF, sOps + Ops64Layout<N>::XS);
+                metric_row_g<N, NG>(sX + c * 2 * NG, fam, ifp, jfp, s1d, s2d, sOps + Ops64Layout<N>::XF, sOps + Ops64Layout<N>::XS);
                 const float s1 = float(s1d), s2 = float(s2d);
                 float d[NV], F[8], G[8];
                 dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d, sLf);
@@ -973,12 +989,12 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     double res8 = 0.0;
     if (pact && grp == 0) {
         prefetch_qa(a, base, NP, qa);
-        const double iJd = 1.0 / jacobian<N>(sX + pc * 8, pis, pjs, sOps + Ops64Layout<N>::XS);
+        const double iJd = 1.0 / jacobian_g<N, NG>(sX + pc * 2 * NG, pis, pjs, sOps + Ops64Layout<N>::XS);
         const float iJ = float(iJd);
 #pragma unroll
         for (int w = 0; w < 8; w++) res[w] = (-acc8[w] + xv[w * CPB * NP + pt]) * iJ;
         res[8] = 0.f;
-        res8 = (az_residual<N, CPB * AST>(sA + pc * AST, sX + pc * 8, pis, pjs, sQ64 + pc * NQ + pk, NP, sOps) +
+        res8 = (az_residual<N, CPB * AST, NG>(sA + pc * AST, sX + pc * 2 * NG, pis, pjs, sQ64 + pc * NQ + pk, NP, sOps) +
                 double(xv[8 * CPB * NP + pt])) * iJd;
         combine_point(a, base, NP, sQ64 + pc * NQ + pk, qa, res, res8);
     }

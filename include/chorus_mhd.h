@@ -109,10 +109,9 @@ int chorus_init_bc(const chorus_mesh *mesh, const chorus_params *params, const c
  * terms and Jacobians of the residual and of the CT correction then come
  * from the 8-node derivatives (the sums of calcjacob.f90:20-28 over 8
  * nodes).  The reference prepares these tables and stops
- * (initsetup.f90:71-78); this completes that branch.  FP64 form only (the
- * ideal, viscous and AV paths; CHORUS_ERR_ARG for the mixed form), no Tecplot
- * output (chorus_plot_setup returns CHORUS_ERR_ARG); call before the first
- * step. */
+ * (initsetup.f90:71-78); this completes that branch.  Both precision forms,
+ * the ideal, viscous and AV paths; no Tecplot output (chorus_plot_setup
+ * returns CHORUS_ERR_ARG); call before the first step. */
 int chorus_set_nodes8(chorus_ctx *ctx, const double *xm, const double *ym);
 
 /* Q(9,N,N,NCELL) in the reference layout (setup2d.f90:54).  Upload/download. */
