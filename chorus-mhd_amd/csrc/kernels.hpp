@@ -240,47 +240,37 @@ __device__ __forceinline__ int side_sign(int lfi) { return (lfi == 0 || lfi == 3
 
 // traces of all 9 variables of the block's cells at their 4N face points, from
 // the LDS copy q[c][var][js][is]; same summation order as trace9.  Called by
-// all threads of the block; with THREADS = blockDim.x given, the items of a
-// thread are unrolled so that their LDS reads overlap.  TQ[cell][lfi][nfp][var].
+// all threads of the block: one thread per (cell, face point) forms the nine
+// variables of its record (the point's index arithmetic and its row of L are
+// shared by the nine sums).  TQ[cell][lfi][nfp][var].
 template <int N>
-__device__ __forceinline__ double trace_item(const double *q, int it, const double *L) {
-    constexpr int NP = N * N, NQ = NV * NP, PT = 4 * N * NV;
-    const int c = it / PT, k = it % PT, fpi = k / NV, v = k % NV;
-    const int lfi = fpi / N, nfp = fpi % N;
-    int ifp, jfp;
-    face2fp<N>(lfi + 1, nfp, ifp, jfp);
-    const double *qv = q + c * NQ + v * NP;
-    double acc = 0.0;
-    if (lf_family(lfi + 1) == 0) {
-#pragma unroll
-        for (int sp = 0; sp < N; sp++) acc = acc + qv[jfp * N + sp] * L[ifp * N + sp];
-    } else {
-#pragma unroll
-        for (int sp = 0; sp < N; sp++) acc = acc + qv[sp * N + ifp] * L[jfp * N + sp];
-    }
-    return acc;
-}
-
-template <int N, int THREADS = 0, int CPB = 0>
 __device__ __forceinline__ void store_traces(const double *q, int ncl, int c0, double *TQ, const double *Lt = nullptr) {
-    constexpr int PT = 4 * N * NV;
+    constexpr int NP = N * N, NQ = NV * NP, FP = 4 * N, PT = FP * NV;
     const double *L = Lt ? Lt : c_L[N];
-    if constexpr (THREADS > 0) {
-        constexpr int PER = (CPB * PT + THREADS - 1) / THREADS;
-        const int n = ncl * PT;
-        double t[PER];
+    for (int it = threadIdx.x; it < ncl * FP; it += blockDim.x) {
+        const int c = it / FP, fpi = it % FP, lfi = fpi / N, nfp = fpi % N;
+        int ifp, jfp;
+        face2fp<N>(lfi + 1, nfp, ifp, jfp);
+        const bool fam0 = lf_family(lfi + 1) == 0;
+        // family 0: the line js = jfp, extrapolated to ifp; family 1: the
+        // column is = ifp, extrapolated to jfp
+        const double *qp = q + c * NQ + (fam0 ? jfp * N : ifp);
+        const int stride = fam0 ? 1 : N;
+        const double *Lrow = L + (fam0 ? ifp : jfp) * N;
+        double Lr[N], acc[NV];
 #pragma unroll
-        for (int k = 0; k < PER; k++) {
-            const int it = (int)threadIdx.x + k * THREADS;
-            t[k] = it < n ? trace_item<N>(q, it, L) : 0.0;
-        }
+        for (int sp = 0; sp < N; sp++) Lr[sp] = Lrow[sp];
 #pragma unroll
-        for (int k = 0; k < PER; k++) {
-            const int it = (int)threadIdx.x + k * THREADS;
-            if (it < n) TQ[(size_t)c0 * PT + it] = t[k];
+        for (int v = 0; v < NV; v++) acc[v] = 0.0;
+#pragma unroll
+        for (int sp = 0; sp < N; sp++) {
+            const double *qs = qp + sp * stride;
+#pragma unroll
+            for (int v = 0; v < NV; v++) acc[v] = acc[v] + qs[v * NP] * Lr[sp];
         }
-    } else {
-        for (int it = threadIdx.x; it < ncl * PT; it += blockDim.x) TQ[(size_t)c0 * PT + it] = trace_item<N>(q, it, L);
+        double *dst = TQ + (size_t)(c0 + c) * PT + fpi * NV;
+#pragma unroll
+        for (int v = 0; v < NV; v++) dst[v] = acc[v];
     }
 }
 
@@ -591,7 +581,7 @@ __global__ void __launch_bounds__(IShape<N>::THREADS) __attribute__((amdgpu_wave
         __syncthreads();
     }
     // D: traces of the new state for the next stage's face pass
-    if (a.kind != KIND_RESID) store_traces<N, S::THREADS, CPB>(sQ, ncl, c0, a.TQ, sOps);
+    if (a.kind != KIND_RESID) store_traces<N>(sQ, ncl, c0, a.TQ, sOps);
 }
 
 // ---------------------------------------------------------------------------

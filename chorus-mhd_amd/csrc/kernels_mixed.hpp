@@ -525,7 +525,7 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
     if (pact) combine_point(a, base, NP, sQ64 + pc * NQ + pk, qa, res, res8);
     if (a.kind == KIND_RESID) return;
     __syncthreads();
-    store_traces<N, T, CPB>(sQ64, ncl, c0, a.TQ, sOps);
+    store_traces<N>(sQ64, ncl, c0, a.TQ, sOps);
 }
 
 // ---------------------------------------------------------------------------

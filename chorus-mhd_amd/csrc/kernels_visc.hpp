@@ -145,7 +145,7 @@ __global__ void __launch_bounds__(256) k_trace(int ncell, const double *Q, doubl
         ops.scatter(sOps);
     }
     __syncthreads();
-    store_traces<N, S::THREADS, CPB>(sQ, ncl, c0, TQ, sOps);
+    store_traces<N>(sQ, ncl, c0, TQ, sOps);
     if (sm) av_sensor<N, CPB>(sm, sQ, sAV, ncl, c0, sOps);
 }
 
