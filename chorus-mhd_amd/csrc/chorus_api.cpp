@@ -124,6 +124,13 @@ void from_device_layout(const chorus_ctx *c, const std::vector<double> &in, doub
 
 bool g_ops_uploaded = false;
 
+// mixed viscous path: the half trace jumps (8 floats per cell face point) live
+// in the upper half of FF, whose FP32 flux records use 9 of its 18 words
+float *half_jumps(chorus_ctx *c) {
+    if (c->ideal || c->p.precision != CHORUS_MIXED) return nullptr;
+    return (float *)c->FF + (size_t)c->ncell * 4 * c->N * 9;
+}
+
 FaceArgs face_args(chorus_ctx *c) {
     FaceArgs f{};
     f.nface = c->nface;
@@ -132,6 +139,7 @@ FaceArgs face_args(chorus_ctx *c) {
     f.TQ = c->TQ;
     f.FF = c->FF;
     f.FA = c->FA;
+    f.HJ = half_jumps(c);
     f.FFv = c->FFv;
     f.gtr = c->gtr;
     f.sm = c->sm;
@@ -173,6 +181,7 @@ int launch_stage(chorus_ctx *c, const StageArgs &a) {
         v.eig_next = c->eig + (size_t)(1 - c->stage_parity) * c->ninst;
         v.ninst = c->ninst;
         v.TQ = c->TQ;
+        v.HJ = half_jumps(c);
         v.mu4 = c->mu4;
         v.sm6 = c->sm6;
         v.artiv = c->p.artiv;

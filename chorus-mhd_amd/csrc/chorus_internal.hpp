@@ -86,6 +86,8 @@ struct FaceArgs {
     const double *TQ;
     void *FF;
     double *FA;              // mixed form: FP64 upwind A_z per cell face point
+    float *HJ;               // mixed viscous path: half the jump of the FP64 traces,
+                             // (other - own)/2 rounded to FP32, per cell face point [8]
     void *FFv;
     const void *gtr;
     const double *sm;        // AV indicator per cell
@@ -124,6 +126,7 @@ struct ViscArgs {
     double *eig_next;   // the next stage's slots (one per instance), zeroed by k_stage_visc
     int ninst;          // instances sharing the launch
     const double *TQ;   // traces of Qin at the face points
+    const float *HJ;    // mixed form: half trace jumps per cell face point (k_face_inv_mx)
     const double *mu4;  // face-averaged muAV(1) per cell and local face (k_face_visc)
     double *sm6;        // when set (Tecplot output): log10 indicator of By of the stage's input
     int artiv;

@@ -7,7 +7,8 @@ namespace chorus {
 
 hipError_t upload_operators_mixed() {
     hipError_t e = upload_tables();
-    return e != hipSuccess ? e : upload_av_tables();
+    if (e == hipSuccess) e = upload_av_tables();
+    return e != hipSuccess ? e : upload_av_roundtrip();
 }
 
 template <class S> static int blocks_of(int ncell) { return (ncell + S::CPB - 1) / S::CPB; }

@@ -243,34 +243,74 @@ __device__ __forceinline__ int side_sign(int lfi) { return (lfi == 0 || lfi == 3
 // all threads of the block: one thread per (cell, face point) forms the nine
 // variables of its record (the point's index arithmetic and its row of L are
 // shared by the nine sums).  TQ[cell][lfi][nfp][var].
+// With `stage` (LDS, trace_stage_doubles(CPB) doubles, free for the call), a
+// wave's 64 records are exchanged through its own part of it, so that each
+// of its nine global stores writes 64 consecutive doubles; without it, each
+// thread stores its own 72-byte record.
+template <int N> constexpr int trace_stage_doubles(int cpb) { return (cpb * 4 * N + 63) / 64 * 64 * NV; }
+
 template <int N>
-__device__ __forceinline__ void store_traces(const double *q, int ncl, int c0, double *TQ, const double *Lt = nullptr) {
+__device__ __forceinline__ void store_traces(const double *q, int ncl, int c0, double *TQ, const double *Lt = nullptr,
+                                             double *stage = nullptr) {
     constexpr int NP = N * N, NQ = NV * NP, FP = 4 * N, PT = FP * NV;
     const double *L = Lt ? Lt : c_L[N];
-    for (int it = threadIdx.x; it < ncl * FP; it += blockDim.x) {
-        const int c = it / FP, fpi = it % FP, lfi = fpi / N, nfp = fpi % N;
-        int ifp, jfp;
-        face2fp<N>(lfi + 1, nfp, ifp, jfp);
-        const bool fam0 = lf_family(lfi + 1) == 0;
-        // family 0: the line js = jfp, extrapolated to ifp; family 1: the
-        // column is = ifp, extrapolated to jfp
-        const double *qp = q + c * NQ + (fam0 ? jfp * N : ifp);
-        const int stride = fam0 ? 1 : N;
-        const double *Lrow = L + (fam0 ? ifp : jfp) * N;
-        double Lr[N], acc[NV];
+    const int nrec = ncl * FP;
+    for (int base = 0; base < nrec; base += blockDim.x) {
+        const int it = base + (int)threadIdx.x;
+        double acc[NV];
+        if (it < nrec) {
+            const int c = it / FP, fpi = it % FP, lfi = fpi / N, nfp = fpi % N;
+            int ifp, jfp;
+            face2fp<N>(lfi + 1, nfp, ifp, jfp);
+            const bool fam0 = lf_family(lfi + 1) == 0;
+            // family 0: the line js = jfp, extrapolated to ifp; family 1: the
+            // column is = ifp, extrapolated to jfp
+            const double *qp = q + c * NQ + (fam0 ? jfp * N : ifp);
+            const int stride = fam0 ? 1 : N;
+            const double *Lrow = L + (fam0 ? ifp : jfp) * N;
+            double Lr[N];
 #pragma unroll
-        for (int sp = 0; sp < N; sp++) Lr[sp] = Lrow[sp];
+            for (int sp = 0; sp < N; sp++) Lr[sp] = Lrow[sp];
 #pragma unroll
-        for (int v = 0; v < NV; v++) acc[v] = 0.0;
+            for (int v = 0; v < NV; v++) acc[v] = 0.0;
 #pragma unroll
-        for (int sp = 0; sp < N; sp++) {
-            const double *qs = qp + sp * stride;
+            for (int sp = 0; sp < N; sp++) {
+                const double *qs = qp + sp * stride;
 #pragma unroll
-            for (int v = 0; v < NV; v++) acc[v] = acc[v] + qs[v * NP] * Lr[sp];
+                for (int v = 0; v < NV; v++) acc[v] = acc[v] + qs[v * NP] * Lr[sp];
+            }
+            if (!stage) {
+                double *dst = TQ + (size_t)c0 * PT + (size_t)it * NV;
+#pragma unroll
+                for (int v = 0; v < NV; v++) dst[v] = acc[v];
+            }
         }
-        double *dst = TQ + (size_t)(c0 + c) * PT + fpi * NV;
+        if (stage) {
+            // the wave's records [w0, w0 + nw) leave as nw*9 consecutive doubles
+            const int lane = (int)threadIdx.x & 63, w0 = base + ((int)threadIdx.x & ~63);
+            const int nw = min(64, nrec - w0);
+            if (nw > 0) {
+                double *st = stage + (((int)threadIdx.x >> 6) << 6) * NV;
+                if (it < nrec) {
 #pragma unroll
-        for (int v = 0; v < NV; v++) dst[v] = acc[v];
+                    for (int v = 0; v < NV; v++) st[lane * NV + v] = acc[v];
+                }
+                // the wave reads what its own lanes wrote: LDS operations of a
+                // wave complete in order; keep the compiler from moving them
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                double *dst = TQ + (size_t)c0 * PT + (size_t)w0 * NV;
+#pragma unroll
+                for (int j = 0; j < NV; j++) {
+                    const int k = j * 64 + lane;
+                    if (k < nw * NV) dst[k] = st[k];
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+        }
     }
 }
 
@@ -580,8 +620,10 @@ __global__ void __launch_bounds__(IShape<N>::THREADS) __attribute__((amdgpu_wave
         }
         __syncthreads();
     }
-    // D: traces of the new state for the next stage's face pass
-    if (a.kind != KIND_RESID) store_traces<N>(sQ, ncl, c0, a.TQ, sOps);
+    // D: traces of the new state for the next stage's face pass (the flux
+    // planes, free now, stage the stores)
+    constexpr bool STG = trace_stage_doubles<N>(CPB) * sizeof(double) <= CPB * FST * sizeof(R);
+    if (a.kind != KIND_RESID) store_traces<N>(sQ, ncl, c0, a.TQ, sOps, STG ? reinterpret_cast<double *>(sF) : nullptr);
 }
 
 // ---------------------------------------------------------------------------

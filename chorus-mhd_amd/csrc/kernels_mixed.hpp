@@ -42,6 +42,25 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 
 namespace chorus {
 
+// round trip of the AV sensor's interpolation per direction, N -> N-1 -> N
+// points (AV.f90:15-58): R = H2 H1 (N x N, FP32; the 2-D round trip is R (x) R)
+static __constant__ float c_Rav[kMaxN + 1][kMaxN * kMaxN];
+
+static hipError_t upload_av_roundtrip() {
+    static float R[kMaxN + 1][kMaxN * kMaxN];
+    static double H1[kMaxN * kMaxN], H2[kMaxN * kMaxN];
+    for (int n = 2; n <= kMaxN; n++) {
+        host_av_tables(n, H1, H2);
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                double acc = 0.0;
+                for (int l = 0; l < n - 1; l++) acc += H2[i * (n - 1) + l] * H1[l * n + j];
+                R[n][i * n + j] = float(acc);
+            }
+    }
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_Rav), R, sizeof(R));
+}
+
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
@@ -178,6 +197,58 @@ template <int N, int THREADS> struct OpTables {
         scatter_lds<PO, THREADS>(sM, m, K);
     }
 };
+
+// AV smoothness indicator of density, mixed form: the same quantity as
+// av_sensor (AV.f90:15-82), log10 of sum_p (Qsm - Q)^2 / Q^2 with Qsm the
+// N -> N-1 -> N interpolation round trip R (x) R of the cell's density, here in
+// FP32 on the deviation d = rho - rhobar from a constant of the cell (the
+// round trip reproduces constants, so Qsm - Q = (R (x) R - I) d): one pass per
+// direction, then the per-cell sum and a FP32 logarithm.
+// rho[c * CS + p]: the block's FP64 densities; rbar[c * NV]: the constant;
+// work: 2 * CPB * N^2 floats of LDS; Rt: the LDS copy of c_Rav[N].
+// Called by all threads of the block.
+template <int N, int CPB, int CS = NV * N * N>
+__device__ __forceinline__ void av_sensor_mx(double *sm, const double *rho, const double *rbar, float *work,
+                                             int ncl, int c0, const float *Rt) {
+    constexpr int NP = N * N;
+    const int tid = threadIdx.x;
+    const int pc = tid / NP, pk = tid % NP, pjs = pk / N, pis = pk % N;
+    const bool pact = (tid < CPB * NP) && (pc < ncl);
+    float *d = work, *t = work + CPB * NP;
+    float dq = 0.f, r = 1.f;
+    if (pact) {
+        const double q = rho[pc * CS + pk];
+        dq = float(q - rbar[pc * NV]);
+        r = float(q);
+        d[tid] = dq;
+    }
+    __syncthreads();
+    if (pact) {
+        // along is: t[js][is] = sum_k R[is][k] d[js][k]
+        const float *dr = d + pc * NP + pjs * N;
+        float acc = 0.f;
+#pragma unroll
+        for (int k = 0; k < N; k++) acc += Rt[pis * N + k] * dr[k];
+        t[tid] = acc;
+    }
+    __syncthreads();
+    if (pact) {
+        // along js, minus the point's own deviation
+        const float *tc = t + pc * NP + pis;
+        float acc = 0.f;
+#pragma unroll
+        for (int k = 0; k < N; k++) acc += Rt[pjs * N + k] * tc[k * N];
+        const float dev = acc - dq;
+        d[tid] = dev * dev * frcp(r * r);
+    }
+    __syncthreads();
+    if (tid < ncl) {
+        float se = 0.f;
+#pragma unroll
+        for (int p = 0; p < NP; p++) se += d[tid * NP + p];
+        sm[c0 + tid] = double(__builtin_amdgcn_logf(se) * 0.30102999566398120f);
+    }
+}
 
 // SSP stage combination at one solution point: the point's FP64 state qs[v*NP]
 // (the block's LDS copy) and a register copy of Qa (when the stage needs it);
@@ -355,6 +426,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         const float eig = wave_speed(s.q, Qr, s1 * im, s2 * im, gam);
         const float dis = 0.5f * sl * mag * eig;
         const size_t rl = face_rec<N>(fc, k, false);
+        if (a.HJ) {
+            // the half trace jumps the gradient kernel's face points use (the
+            // central flux-point value is own + (other - own)/2); a boundary
+            // face has none: its central value is the own trace
+            float4 *hl = reinterpret_cast<float4 *>(a.HJ + rl * 8);
+            if (free_face) {
+                hl[0] = make_float4(0.f, 0.f, 0.f, 0.f);
+                hl[1] = hl[0];
+            } else {
+                float4 *hr = reinterpret_cast<float4 *>(a.HJ + face_rec<N>(fc, k, true) * 8);
+                hl[0] = make_float4(0.5f * d[0], 0.5f * d[1], 0.5f * d[2], 0.5f * d[3]);
+                hl[1] = make_float4(0.5f * d[4], 0.5f * d[5], 0.5f * d[6], 0.5f * d[7]);
+                hr[0] = make_float4(-0.5f * d[0], -0.5f * d[1], -0.5f * d[2], -0.5f * d[3]);
+                hr[1] = make_float4(-0.5f * d[4], -0.5f * d[5], -0.5f * d[6], -0.5f * d[7]);
+            }
+        }
         if (free_face) {
 #pragma unroll
             for (int v = 0; v < RW; v++) out[v] = 0.f;
@@ -525,7 +612,9 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
     if (pact) combine_point(a, base, NP, sQ64 + pc * NQ + pk, qa, res, res8);
     if (a.kind == KIND_RESID) return;
     __syncthreads();
-    store_traces<N>(sQ64, ncl, c0, a.TQ, sOps);
+    // (the flux area is free now: it stages the trace stores)
+    constexpr bool STG = trace_stage_doubles<N>(CPB) <= UD;
+    store_traces<N>(sQ64, ncl, c0, a.TQ, sOps, STG ? U : nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -616,16 +705,13 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
                 c = item / (2 * N);
                 line = k >> 1; r = (k & 1) ? N : 0;
                 if (c < ncl) {
+                    // (other - own)/2 of the FP64 traces, formed by k_face_inv_mx
                     int lfi, nfp;
                     face_of<N>(fam, r, line, lfi, nfp);
-                    const int cell = c0 + c;
-                    const int2 info = sNbr[c * 4 + lfi];
-                    if (!((info.y >> 4) & 1)) {
-                        const double *own = a.TQ + (((size_t)cell * 4 + lfi) * N + nfp) * NV;
-                        const double *oth = a.TQ + (((size_t)info.x * 4 + (info.y & 7) - 1) * N + partner_nfp<N>(nfp)) * NV;
-#pragma unroll
-                        for (int v = 0; v < 8; v++) half_jump[v] = 0.5f * float(oth[v] - own[v]);
-                    }
+                    const float4 *h = reinterpret_cast<const float4 *>(a.HJ + (((size_t)(c0 + c) * 4 + lfi) * N + nfp) * 8);
+                    const float4 h0 = h[0], h1 = h[1];
+                    half_jump[0] = h0.x; half_jump[1] = h0.y; half_jump[2] = h0.z; half_jump[3] = h0.w;
+                    half_jump[4] = h1.x; half_jump[5] = h1.y; half_jump[6] = h1.z; half_jump[7] = h1.w;
                 }
             }
             if (c >= ncl) continue;
@@ -728,19 +814,33 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
     }
     // viscous / AV flux of the own traces at the interior flux points of both
     // families (comp_viscous_flux, compflux.f90:205-256), for the stage kernel
+    // (tasks family-major, so that a wave's tasks share a family: those of
+    // family 1 read the delta traces and metric rows its flux phase left in sV)
     float *fvi = a.fvi_r<float>();
-    for (int it = tid; it < ncl * 2 * NIT; it += T) {
-        const int c = it / (2 * NIT), k2 = it % (2 * NIT), fam = k2 / NIT, k = k2 % NIT;
+    for (int it = tid; it < 2 * CPB * NIT; it += T) {
+        const int fam = it / (CPB * NIT), k1 = it % (CPB * NIT), c = k1 / NIT, k = k1 % NIT;
+        if (c >= ncl) continue;
         const int line = k / (N - 1), r = 1 + k % (N - 1);
         const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
-        double s1d, s2d;
-        metric_row_g<N, NG>(sX + c * 2 * NG, fam, ifp, jfp, s1d, s2d, sOps + Ops64Layout<N>::XF, sOps + Ops64Layout<N>::XS);
-        float d[NV], qv[NV], g[2 * NV], fv[NV];
-        dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d, sLf);
+        float d[NV], qv[NV], g[2 * NV], fv[NV], s1, s2;
+        if (fam == 1) {
+            const float *q = sV + c * VST + line * (N + 1) + r;
+#pragma unroll
+            for (int w = 0; w < 8; w++) d[w] = q[w * NF];
+            d[8] = 0.f;   // (A_z does not enter the viscous flux)
+            s1 = q[8 * NF];
+            s2 = q[9 * NF];
+        } else {
+            double s1d, s2d;
+            metric_row_g<N, NG>(sX + c * 2 * NG, fam, ifp, jfp, s1d, s2d, sOps + Ops64Layout<N>::XF, sOps + Ops64Layout<N>::XS);
+            s1 = float(s1d);
+            s2 = float(s2d);
+            dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d, sLf);
+        }
 #pragma unroll
         for (int w = 0; w < NV; w++) qv[w] = sRef[c].q[w] + d[w];
         grad_trace<N, float>(sG + c * 16 * NP, fam, ifp, jfp, g, sLf);
-        visc_point<N, float, VM>(a, sMu + c * 4, fam, ifp, jfp, float(s1d), float(s2d), qv, g, fv);
+        visc_point<N, float, VM>(a, sMu + c * 4, fam, ifp, jfp, s1, s2, qv, g, fv);
         float *dst = fvi + ((size_t)(c0 + c) * 2 + fam) * NV * NIT + k;
 #pragma unroll
         for (int w = 0; w < NV; w++) dst[w * NIT] = fv[w];
@@ -864,8 +964,12 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     __shared__ double sA[2 * CPB * AST]; // FP64 A_z at the flux points of both families
     __shared__ float sLf[(N + 1) * N], sMf[(N + 1) * N];
     __shared__ double sOps[Ops64Layout<N>::SIZE];   // FP64 operator tables (OpTables64 layout)
+    __shared__ float sRav[N * N];        // AV interpolation round trip (c_Rav)
     float *sD = (float *)U, *sU = sD + CPB * NQ;
     static_assert(9 * CPB * NP <= CPB * UST, "exchange area");
+    if constexpr ((VM & 2) != 0) {
+        if (threadIdx.x < N * N) sRav[threadIdx.x] = c_Rav[N][threadIdx.x];
+    }
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = blk * CPB;
@@ -1000,8 +1104,10 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     }
     if (a.kind == KIND_RESID) return;
     __syncthreads();
-    store_traces<N>(sQ64, ncl, c0, a.TQ, sOps);
-    if constexpr ((VM & 2) != 0) av_sensor<N, CPB>(a.sm, sQ64, sAV, ncl, c0, sOps);
+    // (the flux area is free now: it stages the trace stores)
+    constexpr bool STG = trace_stage_doubles<N>(CPB) <= UD;
+    store_traces<N>(sQ64, ncl, c0, a.TQ, sOps, STG ? U : nullptr);
+    if constexpr ((VM & 2) != 0) av_sensor_mx<N, CPB>(a.sm, sQ64, sBar, reinterpret_cast<float *>(sAV), ncl, c0, sRav);
 }
 
 }  // namespace chorus

@@ -363,17 +363,31 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
     }
     // viscous / AV flux of the own traces at the interior flux points of both
     // families (comp_viscous_flux, compflux.f90:205-256), for the stage kernel
+    // (tasks family-major, so that a wave's tasks share a family: those of
+    // family 1 read the traces and metric rows its flux phase left in sV)
     R *fvi = a.fvi_r<R>();
-    for (int it = tid; it < ncl * 2 * NIT; it += blockDim.x) {
-        const int c = it / (2 * NIT), k2 = it % (2 * NIT), fam = k2 / NIT, k = k2 % NIT;
+    for (int it = tid; it < 2 * CPB * NIT; it += blockDim.x) {
+        const int fam = it / (CPB * NIT), k1 = it % (CPB * NIT), c = k1 / NIT, k = k1 % NIT;
+        if (c >= ncl) continue;
         const int line = k / (N - 1), r = 1 + k % (N - 1);
         const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
-        double s1d, s2d;
-        metric_row_g<N, NG>(sX + c * 2 * NG, fam, ifp, jfp, s1d, s2d, sOps + OXF, sOps + OXS);
-        R Qown[NV], g[2 * NV], fv[NV];
-        trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown, sOps);
+        R Qown[NV], g[2 * NV], fv[NV], s1, s2;
+        if (fam == 1) {
+            const R *q = sV + c * VST + line * (N + 1) + r;
+#pragma unroll
+            for (int v = 0; v < 8; v++) Qown[v] = q[v * NF];
+            Qown[8] = R(0);   // (A_z does not enter the viscous flux)
+            s1 = q[8 * NF];
+            s2 = q[9 * NF];
+        } else {
+            double s1d, s2d;
+            metric_row_g<N, NG>(sX + c * 2 * NG, fam, ifp, jfp, s1d, s2d, sOps + OXF, sOps + OXS);
+            s1 = R(s1d);
+            s2 = R(s2d);
+            trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown, sOps);
+        }
         grad_trace<N, R>(sG + c * 16 * NP, fam, ifp, jfp, g, sOps);
-        visc_point<N, R, VM>(a, sMu + c * 4, fam, ifp, jfp, R(s1d), R(s2d), Qown, g, fv);
+        visc_point<N, R, VM>(a, sMu + c * 4, fam, ifp, jfp, s1, s2, Qown, g, fv);
         R *dst = fvi + ((size_t)(c0 + c) * 2 + fam) * NV * NIT + k;
 #pragma unroll
         for (int w = 0; w < NV; w++) dst[w * NIT] = fv[w];
@@ -648,7 +662,9 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
     __syncthreads();
     // traces of the new state for the next stage's face passes, and its AV
     // indicator (AV.f90:15-82) for the next stage
-    store_traces<N>(sQ, ncl, c0, a.TQ, sOps);
+    // (the flux planes are free now: they stage the trace stores)
+    constexpr bool STG = trace_stage_doubles<N>(CPB) * sizeof(double) <= CPB * UST * sizeof(R);
+    store_traces<N>(sQ, ncl, c0, a.TQ, sOps, STG ? reinterpret_cast<double *>(sU) : nullptr);
     if constexpr ((VM & 2) != 0) av_sensor<N, CPB>(a.sm, sQ, sAV, ncl, c0, sOps);
 }
 
