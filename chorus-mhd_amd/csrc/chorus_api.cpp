@@ -57,7 +57,6 @@ struct chorus_ctx {
     void *gtr = nullptr;           // viscous path: own nablaQs face traces
     void *fvi = nullptr;           // viscous path: the interior viscous/AV fluxes of the stage
     double *sm = nullptr;          // viscous path: AV indicator per cell
-    double *mu4 = nullptr;         // viscous path: face-averaged muAV per cell and local face
     double *eig = nullptr;         // two slots (of ninst values) of the per-stage maximum wave speed
     int ninst = 1;                 // instances sharing this context (batched groups)
     int *inst = nullptr;           // batched groups: instance of each cell (device)
@@ -78,7 +77,7 @@ struct chorus_ctx {
     // Tecplot output (chorus_plot_setup / chorus_write_tecplot)
     PlotState plot;
     double *mucell = nullptr;      // viscous path with AV: muAV(1) per cell of the last stage's input
-    double *sm6 = nullptr;         // viscous path with AV: By indicator of the last stage's input
+    double *sm6 = nullptr;         // after chorus_plot_setup (AV): By indicator of the last stage's input
     // restart dump image (chorus_write_restart), allocated on first use
     unsigned *rs_img = nullptr;
     // one time step captured as a hipGraph, per buffer rotation, tile shape and
@@ -143,7 +142,6 @@ FaceArgs face_args(chorus_ctx *c) {
     f.FFv = c->FFv;
     f.gtr = c->gtr;
     f.sm = c->sm;
-    f.mu4 = c->mu4;
     f.mucell = c->mucell;
     f.inst = c->inst;
     f.hinst = c->hinst;
@@ -182,7 +180,7 @@ int launch_stage(chorus_ctx *c, const StageArgs &a) {
         v.ninst = c->ninst;
         v.TQ = c->TQ;
         v.HJ = half_jumps(c);
-        v.mu4 = c->mu4;
+        v.mucell = c->mucell;
         v.sm6 = c->sm6;
         v.artiv = c->p.artiv;
         v.mu = c->p.mu;
@@ -191,7 +189,6 @@ int launch_stage(chorus_ctx *c, const StageArgs &a) {
         v.h_msh = c->h_msh;
         v.pi = std::atan(1.0) * 4.0;
         f.eig = v.eig;
-        f.eig_now = v.eig;
         c->last_parity = c->stage_parity;
         c->stage_parity = 1 - c->stage_parity;
         if (c->p.precision == CHORUS_MIXED) e = launch_stage_visc_mixed(c->N, a, v, f, c->stream, c->tile);
@@ -308,7 +305,7 @@ int one_step(chorus_ctx *c) {
 // one_step as one hipGraph launch: the step's 11-26 kernel launches cost one
 // launch, which matters for small meshes; captured on first use per key
 int graph_step(chorus_ctx *c) {
-    const std::array<int, 4> key{c->cur, c->stage_parity, c->tile, c->mucell ? 1 : 0};
+    const std::array<int, 4> key{c->cur, c->stage_parity, c->tile, c->sm6 ? 1 : 0};
     auto it = c->step_graphs.find(key);
     if (it == c->step_graphs.end()) {
         chorus_ctx::StepGraph g;
@@ -528,6 +525,9 @@ int init_ctx(chorus_ctx *c, const chorus_mesh *m, const chorus_params *p, const 
     if (!m || !p) return fail(c, CHORUS_ERR_ARG, "null mesh or params");
     if (m->N < 2 || m->N > kMaxN) return fail(c, CHORUS_ERR_ARG, "N must be in 2..8");
     if (m->ncell <= 0) return fail(c, CHORUS_ERR_ARG, "empty mesh");
+    // the kernels number face-point records (cell, local face, point) in 32 bits
+    if ((long long)m->ncell * 4 * m->N >= (1LL << 31))
+        return fail(c, CHORUS_ERR_ARG, "mesh too large: NCELL*4*N face-point records must stay below 2^31");
     if (p->vismode != 1)
         return fail(c, CHORUS_ERR_ARG, "vismode must be 1: with vismode=0 the reference reads uninitialised "
                                       "flux-point A_z (compflux.f90:72-75, magnetic_potential.f90:17)");
@@ -652,7 +652,7 @@ int init_ctx(chorus_ctx *c, const chorus_mesh *m, const chorus_params *p, const 
         HIP_TRY(c, hipMalloc(&c->gtr, nf * 16 * sizeof(double)));
         HIP_TRY(c, hipMalloc(&c->fvi, (size_t)c->ncell * 2 * 9 * c->N * (c->N - 1) * sizeof(double)));
         HIP_TRY(c, hipMalloc(&c->sm, (size_t)c->ncell * sizeof(double)));
-        HIP_TRY(c, hipMalloc(&c->mu4, (size_t)c->ncell * 4 * sizeof(double)));
+        if (c->p.artiv) HIP_TRY(c, hipMalloc(&c->mucell, (size_t)c->ncell * sizeof(double)));
     }
     HIP_TRY(c, hipMemcpyAsync(c->cellxy, xy.data(), xy.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->nbr, nbr.data(), nbr.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
@@ -679,7 +679,6 @@ void chorus_free(chorus_ctx *c) {
     (void)hipFree(c->gtr);
     (void)hipFree(c->fvi);
     (void)hipFree(c->sm);
-    (void)hipFree(c->mu4);
     (void)hipFree(c->inst);
     (void)hipFree(c->hinst);
     c->plot.release();
@@ -931,13 +930,12 @@ int plot_state_setup(PlotState &ps, const chorus_mesh *m, int N, int nre, std::s
     return 0;
 }
 
-// from here on every viscous stage of c also keeps muAV(1) per cell and the By
-// indicator of its input, which the output of the last stage's values needs
+// from here on every viscous stage of c also keeps the By indicator of its
+// input, which the output of the last stage's values needs (muAV(1) per cell
+// is kept by every AV stage)
 hipError_t plot_av_columns(chorus_ctx *c) {
-    if (c->ideal || !c->p.artiv || c->mucell) return hipSuccess;
-    hipError_t e = hipMalloc(&c->mucell, (size_t)c->ncell * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&c->sm6, (size_t)c->ncell * sizeof(double));
-    if (e == hipSuccess) e = hipMemset(c->mucell, 0, (size_t)c->ncell * sizeof(double));
+    if (c->ideal || !c->p.artiv || c->sm6) return hipSuccess;
+    hipError_t e = hipMalloc(&c->sm6, (size_t)c->ncell * sizeof(double));
     if (e == hipSuccess) e = hipMemset(c->sm6, 0, (size_t)c->ncell * sizeof(double));
     return e;
 }
@@ -1360,7 +1358,7 @@ int chorus_batch_plot_setup(chorus_batch *b, int i, const chorus_mesh *mesh, int
     std::string err;
     const int rc = plot_state_setup(m.plot, mesh, m.N, nre, err);
     if (rc) return bfail(b, rc, err);
-    if (!c->ideal && c->p.artiv && !c->mucell) {
+    if (!c->ideal && c->p.artiv && !c->sm6) {
         HIP_TRYB(b, plot_av_columns(c));
         drop_graphs(b);   // the captured stages do not yet write the new AV columns
     }
@@ -1373,7 +1371,7 @@ int chorus_batch_write_tecplot(chorus_batch *b, int i, const char *path, double 
     if (!m.plot.t) return bfail(b, CHORUS_ERR_ARG, "chorus_batch_plot_setup was not called for this instance");
     chorus_ctx *c = b->groups[m.group];
     HIP_TRYB(b, hipStreamSynchronize(b->stream));
-    const bool av = !c->ideal && c->p.artiv && c->last_parity >= 0 && c->mucell && m.av_valid;
+    const bool av = !c->ideal && c->p.artiv && c->last_parity >= 0 && c->sm6 && m.av_valid;
     PlotArgs a{};
     a.ncell = m.ncell;
     a.artiv = av ? 1 : 0;

@@ -91,10 +91,8 @@ struct FaceArgs {
     void *FFv;
     const void *gtr;
     const double *sm;        // AV indicator per cell
-    double *mu4;             // k_face_visc: face-averaged muAV(1) per cell and local face
-    double *mucell;          // k_face_visc, when set (Tecplot output): muAV(1) per cell
+    const double *mucell;    // k_face_visc: muAV(1) per cell of this stage's input (k_grad)
     double *eig;             // k_face_inv: atomic max of the Rusanov wave speed (AV), per instance
-    const double *eig_now;   // k_face_visc: this stage's maximum (epsilon0), per instance
     const int *inst;         // batched instances: instance of each cell (null: one instance)
     const double *hinst;     // batched instances: AV length scale per instance
     int artiv;
@@ -127,7 +125,7 @@ struct ViscArgs {
     int ninst;          // instances sharing the launch
     const double *TQ;   // traces of Qin at the face points
     const float *HJ;    // mixed form: half trace jumps per cell face point (k_face_inv_mx)
-    const double *mu4;  // face-averaged muAV(1) per cell and local face (k_face_visc)
+    double *mucell;     // AV: muAV(1) per cell of this stage's input, written by k_grad
     double *sm6;        // when set (Tecplot output): log10 indicator of By of the stage's input
     int artiv;
     double mu, eta0, gam, h_msh, pi;

@@ -357,8 +357,10 @@ __device__ __forceinline__ void instance_max_atomic(double v, int inst, double *
 // right) are gathered with consecutive threads on consecutive values of a
 // record, and the fluxes leave the same way, so every global access of a wave
 // covers a few contiguous runs instead of 64 records 72 bytes apart.
-template <int N> __device__ __forceinline__ size_t face_rec(const int4 &fc, int k, bool right_side) {
-    return (right_side && fc.z >= 0) ? ((size_t)fc.z * 4 + fc.w) * N + (N - 1 - k) : ((size_t)fc.x * 4 + fc.y) * N + k;
+// (record numbers fit 32 bits: ncell * 4 * N < 2^31 for every mesh a context
+// can hold; only byte offsets need 64 bits)
+template <int N> __device__ __forceinline__ unsigned face_rec(const int4 &fc, int k, bool right_side) {
+    return (right_side && fc.z >= 0) ? unsigned((fc.z * 4 + fc.w) * N + (N - 1 - k)) : unsigned((fc.x * 4 + fc.y) * N + k);
 }
 
 // Slot of a thread in the block-wide copies of 2*NV values per face point
@@ -386,7 +388,7 @@ __device__ __forceinline__ void gather_face_traces(const double *__restrict__ TQ
         const int p = sl.pb + j * SL::PSTEP;
         if (p < npt) {
             const int gp = p0 + p, f = gp / N;
-            val[j] = TQ[face_rec<N>(sF[f - f0], gp - f * N, sl.side) * NV + sl.wv()];
+            val[j] = TQ[(size_t)face_rec<N>(sF[f - f0], gp - f * N, sl.side) * NV + sl.wv()];
         }
     }
 #pragma unroll
@@ -409,7 +411,7 @@ __device__ __forceinline__ void scatter_face_values(T *__restrict__ out, const i
         if (p < npt) {
             const int gp = p0 + p, f = gp / N;
             const int4 fc = sF[f - f0];
-            if (keep(fc, sl.w)) out[face_rec<N>(fc, gp - f * N, sl.side) * NV + sl.wv()] = T(sT[p * SL::RW + sl.w]);
+            if (keep(fc, sl.w)) out[(size_t)face_rec<N>(fc, gp - f * N, sl.side) * NV + sl.wv()] = T(sT[p * SL::RW + sl.w]);
         }
     }
 }
@@ -560,7 +562,7 @@ __global__ void __launch_bounds__(IShape<N>::THREADS) __attribute__((amdgpu_wave
             int lfi, nfp;
             if (fam == 0) { lfi = (r == 0) ? 3 : 1; nfp = (r == 0) ? (N - 1 - line) : line; }
             else          { lfi = (r == 0) ? 0 : 2; nfp = (r == 0) ? line : (N - 1 - line); }
-            const R *src = FF + (((size_t)(c0 + c) * 4 + lfi) * N + nfp) * NV;
+            const R *src = FF + (size_t)(((c0 + c) * 4 + lfi) * N + nfp) * NV;
             double s1d, s2d;
             metric_row_g<N, NG>(sX + c * 2 * NG, fam, ifp, jfp, s1d, s2d, sOps + OXF, sOps + OXS);
             R *f = sF + c * FST + line * (N + 1) + r;
@@ -681,7 +683,7 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
             if (i < ncl * 4 * N) {
                 const int2 info = a.nbr[(size_t)(c0 + c) * 4 + lfi];
                 if (!((info.y >> 4) & 1)) {
-                    const double *t = a.TQ + (((size_t)info.x * 4 + (info.y & 7) - 1) * N + (N - 1 - nfp)) * NV + VAR0;
+                    const double *t = a.TQ + (size_t)((info.x * 4 + (info.y & 7) - 1) * N + (N - 1 - nfp)) * NV + VAR0;
 #pragma unroll
                     for (int w = 0; w < NVAR; w++) hp[w][k] = t[w];
                 }
@@ -764,7 +766,7 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
             int ifp, jfp;
             face2fp<N>(lfi + 1, nfp, ifp, jfp);
             const double v = trace1<N>(sV[w] + c * NP, lf_family(lfi + 1), ifp, jfp, sOps);
-            a.TQout[(((size_t)(c0 + c) * 4 + lfi) * N + nfp) * NV + 5 + w] = v;
+            a.TQout[(size_t)(((c0 + c) * 4 + lfi) * N + nfp) * NV + 5 + w] = v;
         }
     } else {
         double sumsq = 0.0, dmax = 0.0;

@@ -356,7 +356,7 @@ __device__ __forceinline__ double az_at(const double *FA, const double *sAz, int
     if (r == 0 || r == N) {
         int lfi, nfp;
         face_of<N>(fam, r, line, lfi, nfp);
-        return FA[((size_t)cell * 4 + lfi) * N + nfp];
+        return FA[(size_t)((cell * 4 + lfi) * N + nfp)];
     }
     return trace1<N>(sAz + c * N * N, fam, fam == 0 ? r : line, fam == 0 ? line : r, ops);
 }
@@ -430,12 +430,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             // the half trace jumps the gradient kernel's face points use (the
             // central flux-point value is own + (other - own)/2); a boundary
             // face has none: its central value is the own trace
-            float4 *hl = reinterpret_cast<float4 *>(a.HJ + rl * 8);
+            float4 *hl = reinterpret_cast<float4 *>(a.HJ + (size_t)rl * 8);
             if (free_face) {
                 hl[0] = make_float4(0.f, 0.f, 0.f, 0.f);
                 hl[1] = hl[0];
             } else {
-                float4 *hr = reinterpret_cast<float4 *>(a.HJ + face_rec<N>(fc, k, true) * 8);
+                float4 *hr = reinterpret_cast<float4 *>(a.HJ + (size_t)face_rec<N>(fc, k, true) * 8);
                 hl[0] = make_float4(0.5f * d[0], 0.5f * d[1], 0.5f * d[2], 0.5f * d[3]);
                 hl[1] = make_float4(0.5f * d[4], 0.5f * d[5], 0.5f * d[6], 0.5f * d[7]);
                 hr[0] = make_float4(-0.5f * d[0], -0.5f * d[1], -0.5f * d[2], -0.5f * d[3]);
@@ -564,7 +564,7 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
             if (r == 0 || r == N) {
                 int lfi, nfp;
                 face_of<N>(fam, r, line, lfi, nfp);
-                const float *src = FF + (((size_t)(c0 + c) * 4 + lfi) * N + nfp) * NV;
+                const float *src = FF + (size_t)(((c0 + c) * 4 + lfi) * N + nfp) * NV;
 #pragma unroll
                 for (int v = 0; v < 8; v++) jump[v] = src[v];
             }
@@ -674,6 +674,7 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
             const double eps0 = a.eig[in] * (a.inst ? a.hinst[in] : a.h_msh) / (double)(N - 1) * 1.0;
             const double mo = mu_av(a.sm[cell], eps0, a.pi);
             sMu[i] = ((info.y >> 4) & 1) ? mo : 0.5 * (mo + mu_av(a.sm[info.x], eps0, a.pi));
+            if ((i & 3) == 0) a.mucell[cell] = mo;   // for the viscous face kernel (and the output)
         }
     }
     const int grp = tid >= TS::G1 ? 1 : 0;
@@ -708,7 +709,7 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
                     // (other - own)/2 of the FP64 traces, formed by k_face_inv_mx
                     int lfi, nfp;
                     face_of<N>(fam, r, line, lfi, nfp);
-                    const float4 *h = reinterpret_cast<const float4 *>(a.HJ + (((size_t)(c0 + c) * 4 + lfi) * N + nfp) * 8);
+                    const float4 *h = reinterpret_cast<const float4 *>(a.HJ + (size_t)(((c0 + c) * 4 + lfi) * N + nfp) * 8);
                     const float4 h0 = h[0], h1 = h[1];
                     half_jump[0] = h0.x; half_jump[1] = h0.y; half_jump[2] = h0.z; half_jump[3] = h0.w;
                     half_jump[4] = h1.x; half_jump[5] = h1.y; half_jump[6] = h1.z; half_jump[7] = h1.w;
@@ -802,7 +803,7 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
         float Lr[N];
 #pragma unroll
         for (int sp = 0; sp < N; sp++) Lr[sp] = sLf[lrow * N + sp];
-        float *dst = a.gtr_r<float>() + ((size_t)(c0 + c) * 4 * N + fpi) * 16 + h * 8;
+        float *dst = a.gtr_r<float>() + (size_t)((c0 + c) * 4 * N + fpi) * 16 + h * 8;
 #pragma unroll
         for (int gg = 0; gg < 8; gg++) {
             const float *gq = sG + (c * 16 + h * 8 + gg) * NP + base;
@@ -841,7 +842,7 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
         for (int w = 0; w < NV; w++) qv[w] = sRef[c].q[w] + d[w];
         grad_trace<N, float>(sG + c * 16 * NP, fam, ifp, jfp, g, sLf);
         visc_point<N, float, VM>(a, sMu + c * 4, fam, ifp, jfp, s1, s2, qv, g, fv);
-        float *dst = fvi + ((size_t)(c0 + c) * 2 + fam) * NV * NIT + k;
+        float *dst = fvi + (size_t)((c0 + c) * 2 + fam) * (NV * NIT) + k;
 #pragma unroll
         for (int w = 0; w < NV; w++) dst[w * NIT] = fv[w];
     }
@@ -874,8 +875,8 @@ __global__ void __launch_bounds__(256) k_face_visc_mx(FaceArgs a) {
             sr1 = float(m1); sr2 = float(m2);
         }
     }
-    const size_t pl = ((size_t)fc.x * 4 + fc.y) * N + k;
-    const size_t pr = free_face ? pl : ((size_t)fc.z * 4 + fc.w) * N + kr;
+    const size_t pl = (size_t)((fc.x * 4 + fc.y) * N + k);
+    const size_t pr = free_face ? pl : (size_t)((fc.z * 4 + fc.w) * N + kr);
     float qv[NV], g[2 * NV];
 #pragma unroll
     for (int v = 0; v < 8; v++) qv[v] = float(0.5 * a.TQ[pl * NV + v] + 0.5 * a.TQ[pr * NV + v]);
@@ -897,19 +898,10 @@ __global__ void __launch_bounds__(256) k_face_visc_mx(FaceArgs a) {
         for (int w = 0; w < NV; w++) { fl[w] = Fv[w] * sl1 + Gv[w] * sl2; fr[w] = Fv[w] * sr1 + Gv[w] * sr2; }
     }
     if constexpr ((VM & 2) != 0) {
-        const int in = a.inst ? a.inst[fc.x] : 0;
-        const double eps0 = a.eig_now[in] * (a.inst ? a.hinst[in] : a.h_msh) / (double)(N - 1);
-        const double ml = mu_av(a.sm[fc.x], eps0, a.pi);
-        const double mr = free_face ? ml : mu_av(a.sm[fc.z], eps0, a.pi);
+        // muAV(1) of both cells (mu_av of this stage's epsilon0, kept by k_grad_mx)
+        const double ml = a.mucell[fc.x];
+        const double mr = free_face ? ml : a.mucell[fc.z];
         const double mf = free_face ? ml : 0.5 * (ml + mr);
-        if (k == 0) {
-            a.mu4[(size_t)fc.x * 4 + fc.y] = mf;
-            if (!free_face) a.mu4[(size_t)fc.z * 4 + fc.w] = mf;
-            if (a.mucell) {
-                a.mucell[fc.x] = ml;
-                if (!free_face) a.mucell[fc.z] = mr;
-            }
-        }
         vflux<float>(gam, lam, prandt, float(mf), 0.f, qv, g, Fv, Gv);
 #pragma unroll
         for (int w = 0; w < NV; w++) {
@@ -1023,7 +1015,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
                 if (r == 0 || r == N) {
                     int lfi, nfp;
                     face_of<N>(fam, r, line, lfi, nfp);
-                    const size_t o = (((size_t)(c0 + c) * 4 + lfi) * N + nfp) * NV;
+                    const size_t o = (size_t)(((c0 + c) * 4 + lfi) * N + nfp) * NV;
 #pragma unroll
                     for (int w = 0; w < 8; w++) jump[w] = FF[o + w];
 #pragma unroll
@@ -1044,7 +1036,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
                 const int c = item / NIT, k = item % NIT;
                 if (c >= ncl) continue;
                 const int line = k / (N - 1), r = 1 + k % (N - 1);
-                const float *src = FVI + ((size_t)(c0 + c) * 2 + fam) * NV * NIT + k;
+                const float *src = FVI + (size_t)((c0 + c) * 2 + fam) * (NV * NIT) + k;
                 float *u = sU + c * UST + line * (N + 1) + r;
 #pragma unroll
                 for (int w = 0; w < NV; w++) u[(8 + w) * NF] = src[w * NIT];

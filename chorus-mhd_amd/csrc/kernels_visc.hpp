@@ -259,6 +259,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
             const double eps0 = a.eig[in] * (a.inst ? a.hinst[in] : a.h_msh) / (double)(N - 1) * 1.0;
             const double mo = mu_av(a.sm[cell], eps0, a.pi);
             sMu[i] = ((info.y >> 4) & 1) ? mo : 0.5 * (mo + mu_av(a.sm[info.x], eps0, a.pi));
+            if ((i & 3) == 0) a.mucell[cell] = mo;   // for the viscous face kernel (and the output)
         }
     }
     const int grp = tid >= T::G1 ? 1 : 0;
@@ -298,9 +299,9 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
                 int lfi, nfp;
                 face_of<N>(fam, r, line, lfi, nfp);
                 const int2 info = sNbr[c * 4 + lfi];
-                const double *own = a.TQ + (((size_t)cell * 4 + lfi) * N + nfp) * NV;
+                const double *own = a.TQ + (size_t)((cell * 4 + lfi) * N + nfp) * NV;
                 const double *oth = ((info.y >> 4) & 1) ? own
-                                    : a.TQ + (((size_t)info.x * 4 + (info.y & 7) - 1) * N + partner_nfp<N>(nfp)) * NV;
+                                    : a.TQ + (size_t)((info.x * 4 + (info.y & 7) - 1) * N + partner_nfp<N>(nfp)) * NV;
                 R *q = sV + c * VST + line * (N + 1) + r;
 #pragma unroll
                 for (int v = 0; v < 8; v++) q[v * NF] = R(0.5) * R(own[v]) + R(0.5) * R(oth[v]);
@@ -351,7 +352,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
         R Lr[N];
 #pragma unroll
         for (int sp = 0; sp < N; sp++) Lr[sp] = R(sOps[lrow * N + sp]);
-        R *dst = a.gtr_r<R>() + ((size_t)(c0 + c) * 4 * N + fpi) * 16 + h * 8;
+        R *dst = a.gtr_r<R>() + (size_t)((c0 + c) * 4 * N + fpi) * 16 + h * 8;
 #pragma unroll
         for (int gg = 0; gg < 8; gg++) {
             const R *gq = sG + (c * 16 + h * 8 + gg) * NP + base;
@@ -388,7 +389,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
         }
         grad_trace<N, R>(sG + c * 16 * NP, fam, ifp, jfp, g, sOps);
         visc_point<N, R, VM>(a, sMu + c * 4, fam, ifp, jfp, s1, s2, Qown, g, fv);
-        R *dst = fvi + ((size_t)(c0 + c) * 2 + fam) * NV * NIT + k;
+        R *dst = fvi + (size_t)((c0 + c) * 2 + fam) * (NV * NIT) + k;
 #pragma unroll
         for (int w = 0; w < NV; w++) dst[w * NIT] = fv[w];
     }
@@ -430,8 +431,8 @@ __global__ void __launch_bounds__(256) k_face_visc(FaceArgs a) {
             sr1 = R(m1); sr2 = R(m2);
         }
     }
-    const size_t pl = ((size_t)fc.x * 4 + fc.y) * N + k;
-    const size_t pr = free_face ? pl : ((size_t)fc.z * 4 + fc.w) * N + kr;
+    const size_t pl = (size_t)((fc.x * 4 + fc.y) * N + k);
+    const size_t pr = free_face ? pl : (size_t)((fc.z * 4 + fc.w) * N + kr);
     R qv[NV], g[2 * NV];
 #pragma unroll
     for (int v = 0; v < 8; v++) qv[v] = R(0.5) * R(a.TQ[pl * NV + v]) + R(0.5) * R(a.TQ[pr * NV + v]);
@@ -454,19 +455,10 @@ __global__ void __launch_bounds__(256) k_face_visc(FaceArgs a) {
         for (int w = 0; w < NV; w++) { fl[w] = R(0); fr[w] = R(0); }
     }
     if constexpr ((VM & 2) != 0) {
-        const int in = a.inst ? a.inst[fc.x] : 0;
-        const double eps0 = a.eig_now[in] * (a.inst ? a.hinst[in] : a.h_msh) / (double)(N - 1) * 1.0;
-        const double ml = mu_av(a.sm[fc.x], eps0, a.pi);
-        const double mr = free_face ? ml : mu_av(a.sm[fc.z], eps0, a.pi);
+        // muAV(1) of both cells (mu_av of this stage's epsilon0, kept by k_grad)
+        const double ml = a.mucell[fc.x];
+        const double mr = free_face ? ml : a.mucell[fc.z];
         const double mf = free_face ? ml : 0.5 * (ml + mr);
-        if (k == 0) {
-            a.mu4[(size_t)fc.x * 4 + fc.y] = mf;
-            if (!free_face) a.mu4[(size_t)fc.z * 4 + fc.w] = mf;
-            if (a.mucell) {
-                a.mucell[fc.x] = ml;
-                if (!free_face) a.mucell[fc.z] = mr;
-            }
-        }
         vflux<R>(gam, lam, prandt, R(mf), R(0), qv, g, Fv, Gv);
 #pragma unroll
         for (int w = 0; w < NV; w++) {
@@ -564,7 +556,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
                 const int c = item / NIT, k = item % NIT;
                 if (c >= ncl) continue;
                 const int line = k / (N - 1), r = 1 + k % (N - 1);
-                const R *src = FVI + ((size_t)(c0 + c) * 2 + fam) * NV * NIT + k;
+                const R *src = FVI + (size_t)((c0 + c) * 2 + fam) * (NV * NIT) + k;
                 R *u = sU + c * UST + line * (N + 1) + r;
 #pragma unroll
                 for (int w = 0; w < NV; w++) u[(8 + w) * NF] = src[w * NIT];
@@ -576,7 +568,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
                 const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
                 int lfi, nfp;
                 face_of<N>(fam, r, line, lfi, nfp);
-                const size_t o = (((size_t)(c0 + c) * 4 + lfi) * N + nfp) * NV;
+                const size_t o = (size_t)(((c0 + c) * 4 + lfi) * N + nfp) * NV;
                 double s1d, s2d;
                 metric_row_g<N, NG>(sX + c * 2 * NG, fam, ifp, jfp, s1d, s2d, sOps + OXF, sOps + OXS);
                 R *u = sU + c * UST + line * (N + 1) + r;
