@@ -918,22 +918,53 @@ __global__ void __launch_bounds__(256) k_face_visc_mx(FaceArgs a) {
     }
 }
 
-// task layout of the mixed viscous stage kernel: inviscid delta flux at every
-// flux point (face points also copy the viscous face flux), then the viscous
-// flux at the interior points; derivative groups as in V2Shape
+// task layout of the mixed viscous stage kernel: one flux task per flux point
+// of a family, then two threads per solution point for the derivative and
+// the stage combination (group 0: variables 0-3 and A_z, group 1: 4-7)
 template <int N, int TB> struct MxVShape {
     using S = CellShape<N, TB>;
     static constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF;
     static constexpr int NIT = N * (N - 1);
-    static constexpr int NA = CPB * NF;                      // inviscid tasks (all flux points)
-    static constexpr int S1 = NA;
-    static constexpr int NI = CPB * NIT;                     // viscous interior tasks
-    static constexpr int BT = S1 + NI;
-    static constexpr int G1 = ((CPB * NP + 63) / 64) * 64;
+    static constexpr int NA = CPB * NF;                      // flux tasks (all flux points of a family)
+    static constexpr int G1 = ((CPB * NP + 63) / 64) * 64;   // first thread of group 1
     static constexpr int CT = G1 + CPB * NP;
-    static constexpr int THREADS = (((BT > CT ? BT : CT) + 63) / 64) * 64;
+    static constexpr int THREADS = (((NA > CT ? NA : CT) + 63) / 64) * 64;
 };
 
+// combination of the variables v0 .. v0+NVG-1 of a point (combine_point for a
+// subset): qs the point's FP64 state in LDS, qa its prefetched Qa, res the
+// residuals of the subset, r8 the A_z residual when the subset holds var 8
+template <int NVG>
+__device__ __forceinline__ void combine_vars(const StageArgs &a, size_t base, int NP, double *qs, int v0,
+                                             const double *qa, const float *res, double r8 = 0.0) {
+#pragma unroll
+    for (int k = 0; k < NVG; k++) {
+        const int v = v0 + k;
+        const double r = v == 8 ? r8 : double(res[k]);
+        const size_t o = base + (size_t)v * NP;
+        const double q = qs[v * NP];
+        double qn;
+        switch (a.kind) {
+        case KIND_RESID: a.Rout[o] = r; continue;
+        case KIND_AXPY: qn = q + a.c0 * r; break;
+        case KIND_3A: qn = a.c1 * qa[k] + a.c2 * q + a.c0 * r; break;
+        case KIND_3B: qn = a.c1 * qa[k] + a.c2 * q + a.c3 * r * a.c4; break;
+        case KIND_S54_4: qn = a.c1 * qa[k] + a.c2 * q + a.c0 * r; a.Rout[o] = r; break;
+        default: qn = a.c1 * qa[k] + a.c2 * a.Qb[o] + a.c3 * a.Qc[o] + a.c4 * q + a.c5 * a.Rprev[o] + a.c0 * r; break;
+        }
+        a.Qout[o] = qn;
+        qs[v * NP] = qn;
+    }
+}
+
+// Viscous / AV stage kernel, mixed form.  The mixed form keeps no reference
+// association, so the inviscid and viscous contributions of a flux point are
+// combined before the derivative: every flux task forms
+//   u = dF1(own delta trace) + jump - Fv        (8 components)
+// with the jump terms of k_face_inv_mx and the viscous face flux of
+// k_face_visc_mx at face points, the interior viscous flux of k_grad_mx (fvi)
+// elsewhere; the viscous flux of A_z is zero (getrusanov.f90:157-361), so A_z
+// keeps its FP64 upwind/trace values alone.  Then one sum per component.
 #ifndef CHORUS_MX_VISC_MINW
 #define CHORUS_MX_VISC_MINW(N) ((N) <= 6 ? 4 : 3)
 #endif
@@ -942,7 +973,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     using S = CellShape<N, TB>;
     using TS = MxVShape<N, TB>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ, T = TS::THREADS, NIT = TS::NIT;
-    constexpr int UST = 17 * NF + 1;     // dF1 (8) + Fv1 (9)
+    constexpr int UST = 8 * NF + 1;      // per-cell stride of the combined flux planes
     constexpr int AST = NF + 1;          // per-cell stride of the FP64 A_z flux-point values
     constexpr int UF = CPB * NQ + CPB * UST;
     constexpr int UD = (UF + 1) / 2;
@@ -958,7 +989,6 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     __shared__ double sOps[Ops64Layout<N>::SIZE];   // FP64 operator tables (OpTables64 layout)
     __shared__ float sRav[N * N];        // AV interpolation round trip (c_Rav)
     float *sD = (float *)U, *sU = sD + CPB * NQ;
-    static_assert(9 * CPB * NP <= CPB * UST, "exchange area");
     if constexpr ((VM & 2) != 0) {
         if (threadIdx.x < N * N) sRav[threadIdx.x] = c_Rav[N][threadIdx.x];
     }
@@ -995,104 +1025,83 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     const int pc = pt / NP, pk = pt % NP, pjs = pk / N, pis = pk % N;
     const bool pact = (pt < CPB * NP) && (pc < ncl);
     const size_t base = (size_t)(c0 + pc) * NQ + pk;
+    const int vg = 4 * grp;              // first variable of the thread's group
     const float *FF = (const float *)a.FF, *FFv = (const float *)a.FFv, *FVI = v.fvi_r<float>();
-    float acc8[8], accv[NV];
-    double qa[NV];
+    float acc[4];
+    double qa[5];
 #pragma unroll 1
     for (int fam = 0; fam < 2; fam++) {
-        for (int t = tid; t < TS::BT; t += T) {
-            if (t < TS::NA) {
-                // inviscid delta flux of the own trace, + jump terms and the
-                // viscous face flux at face points
-                const int c = t / NF, k = t % NF;
-                if (c >= ncl) continue;
-                const int line = k / (N + 1), r = k % (N + 1);
-                const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
-                float *u = sU + c * UST + line * (N + 1) + r;
-                float jump[8];
+        for (int t = tid; t < TS::NA; t += T) {
+            const int c = t / NF, k = t % NF;
+            if (c >= ncl) continue;
+            const int line = k / (N + 1), r = k % (N + 1);
+            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+            float *u = sU + c * UST + line * (N + 1) + r;
+            // the terms beside the own delta flux: jump - viscous face flux at a
+            // face point, minus the interior viscous flux elsewhere
+            float extra[8];
+            if (r == 0 || r == N) {
+                int lfi, nfp;
+                face_of<N>(fam, r, line, lfi, nfp);
+                const size_t o = (size_t)(((c0 + c) * 4 + lfi) * N + nfp) * NV;
 #pragma unroll
-                for (int w = 0; w < 8; w++) jump[w] = 0.f;
-                if (r == 0 || r == N) {
-                    int lfi, nfp;
-                    face_of<N>(fam, r, line, lfi, nfp);
-                    const size_t o = (size_t)(((c0 + c) * 4 + lfi) * N + nfp) * NV;
+                for (int w = 0; w < 8; w++) extra[w] = FF[o + w] - FFv[o + w];
+            } else {
+                const float *src = FVI + (size_t)((c0 + c) * 2 + fam) * (NV * NIT) + line * (N - 1) + r - 1;
 #pragma unroll
-                    for (int w = 0; w < 8; w++) jump[w] = FF[o + w];
-#pragma unroll
-                    for (int w = 0; w < NV; w++) u[(8 + w) * NF] = FFv[o + w];
-                }
-                double s1d, s2d;
-                metric_row_g<N, NG>(sX + c * 2 * NG, fam, ifp, jfp, s1d, s2d, sOps + Ops64Layout<N>::XF, sOps + Ops64Layout<N>::XS);
-                const float s1 = float(s1d), s2 = float(s2d);
-                float d[NV], F[8], G[8];
-                dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d, sLf);
-                dflux(sRef[c], d, gam - 1.f, F, G);
-#pragma unroll
-                for (int w = 0; w < 8; w++) u[w * NF] = F[w] * s1 + G[w] * s2 + jump[w];
-                sA[(fam * CPB + c) * AST + line * (N + 1) + r] = az_at<N>(a.FA, sAz, c0 + c, c, fam, r, line, sOps);
-            } else if (t >= TS::S1) {
-                // viscous / AV flux at an interior point, formed by k_grad_mx
-                const int item = t - TS::S1;
-                const int c = item / NIT, k = item % NIT;
-                if (c >= ncl) continue;
-                const int line = k / (N - 1), r = 1 + k % (N - 1);
-                const float *src = FVI + (size_t)((c0 + c) * 2 + fam) * (NV * NIT) + k;
-                float *u = sU + c * UST + line * (N + 1) + r;
-#pragma unroll
-                for (int w = 0; w < NV; w++) u[(8 + w) * NF] = src[w * NIT];
+                for (int w = 0; w < 8; w++) extra[w] = -src[w * NIT];
             }
+            double s1d, s2d;
+            metric_row_g<N, NG>(sX + c * 2 * NG, fam, ifp, jfp, s1d, s2d, sOps + Ops64Layout<N>::XF, sOps + Ops64Layout<N>::XS);
+            const float s1 = float(s1d), s2 = float(s2d);
+            float d[NV], F[8], G[8];
+            dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d, sLf);
+            dflux(sRef[c], d, gam - 1.f, F, G);
+#pragma unroll
+            for (int w = 0; w < 8; w++) u[w * NF] = F[w] * s1 + G[w] * s2 + extra[w];
+            sA[(fam * CPB + c) * AST + line * (N + 1) + r] = az_at<N>(a.FA, sAz, c0 + c, c, fam, r, line, sOps);
         }
         __syncthreads();
+        // the point's Qa for the combination, loaded while the derivative is summed
+        if (fam == 1 && pact) {
+            const bool need = a.kind != KIND_RESID && a.kind != KIND_AXPY;
+#pragma unroll
+            for (int k = 0; k < 4; k++) qa[k] = need ? a.Qa[base + (size_t)(vg + k) * NP] : 0.0;
+            qa[4] = need && grp == 0 ? a.Qa[base + (size_t)8 * NP] : 0.0;
+        }
         if (pact) {
             const int line = fam == 0 ? pjs : pis, pos = fam == 0 ? pis : pjs;
-            const float *f = sU + pc * UST + line * (N + 1);
-            if (grp == 0) {
-                float d[8];
+            const float *f = sU + pc * UST + line * (N + 1) + vg * NF;
+            float d4[4];
 #pragma unroll
-                for (int w = 0; w < 8; w++) d[w] = 0.f;
+            for (int w = 0; w < 4; w++) d4[w] = 0.f;
 #pragma unroll CHORUS_MX_CUNROLL
-                for (int r = 0; r <= N; r++) {
-                    const float m = sMf[r * N + pos];
+            for (int r = 0; r <= N; r++) {
+                const float m = sMf[r * N + pos];
 #pragma unroll
-                    for (int w = 0; w < 8; w++) d[w] += f[w * NF + r] * m;
-                }
-#pragma unroll
-                for (int w = 0; w < 8; w++) acc8[w] = fam == 0 ? d[w] : acc8[w] + d[w];
-            } else {
-                float dv[NV];
-#pragma unroll
-                for (int w = 0; w < NV; w++) dv[w] = 0.f;
-#pragma unroll CHORUS_MX_CUNROLL
-                for (int r = 0; r <= N; r++) {
-                    const float m = sMf[r * N + pos];
-#pragma unroll
-                    for (int w = 0; w < NV; w++) dv[w] += f[(8 + w) * NF + r] * m;
-                }
-#pragma unroll
-                for (int w = 0; w < NV; w++) accv[w] = fam == 0 ? dv[w] : accv[w] + dv[w];
+                for (int w = 0; w < 4; w++) d4[w] += f[w * NF + r] * m;
             }
+#pragma unroll
+            for (int w = 0; w < 4; w++) acc[w] = fam == 0 ? d4[w] : acc[w] + d4[w];
         }
         __syncthreads();
     }
-    // group 1 hands its viscous sums to group 0 through the flux area
-    float *xv = sU;
-    if (pact && grp == 1) {
-#pragma unroll
-        for (int w = 0; w < NV; w++) xv[w * CPB * NP + pt] = accv[w];
-    }
-    __syncthreads();
-    float res[NV];
-    double res8 = 0.0;
-    if (pact && grp == 0) {
-        prefetch_qa(a, base, NP, qa);
+    if (pact) {
         const double iJd = 1.0 / jacobian_g<N, NG>(sX + pc * 2 * NG, pis, pjs, sOps + Ops64Layout<N>::XS);
         const float iJ = float(iJd);
+        float res[4];
 #pragma unroll
-        for (int w = 0; w < 8; w++) res[w] = (-acc8[w] + xv[w * CPB * NP + pt]) * iJ;
-        res[8] = 0.f;
-        res8 = (az_residual<N, CPB * AST, NG>(sA + pc * AST, sX + pc * 2 * NG, pis, pjs, sQ64 + pc * NQ + pk, NP, sOps) +
-                double(xv[8 * CPB * NP + pt])) * iJd;
-        combine_point(a, base, NP, sQ64 + pc * NQ + pk, qa, res, res8);
+        for (int w = 0; w < 4; w++) res[w] = -acc[w] * iJ;
+        double *qs = sQ64 + pc * NQ + pk;
+        if (grp == 0) {
+            // A_z residual in FP64 from the flux-point values of both families
+            // (before the combination overwrites the point's density and momenta)
+            const double r8 = az_residual<N, CPB * AST, NG>(sA + pc * AST, sX + pc * 2 * NG, pis, pjs, qs, NP, sOps) * iJd;
+            combine_vars<4>(a, base, NP, qs, 0, qa, res);
+            combine_vars<1>(a, base, NP, qs, 8, qa + 4, res, r8);
+        } else {
+            combine_vars<4>(a, base, NP, qs, 4, qa, res);
+        }
     }
     if (a.kind == KIND_RESID) return;
     __syncthreads();
