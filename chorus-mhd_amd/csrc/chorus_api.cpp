@@ -645,7 +645,8 @@ int init_ctx(chorus_ctx *c, const chorus_mesh *m, const chorus_params *p, const 
     HIP_TRY(c, hipMalloc(&c->TQ, nfp * 9 * sizeof(double)));
     HIP_TRY(c, hipMalloc(&c->FF, nfp * 9 * sizeof(double)));
     if (c->p.precision == CHORUS_MIXED) HIP_TRY(c, hipMalloc(&c->FA, nfp * sizeof(double)));
-    if (!c->ideal) HIP_TRY(c, hipMalloc(&c->FFv, nfp * 9 * sizeof(double)));
+    // (the mixed form folds the viscous face flux into its FF records)
+    if (!c->ideal && c->p.precision != CHORUS_MIXED) HIP_TRY(c, hipMalloc(&c->FFv, nfp * 9 * sizeof(double)));
     HIP_TRY(c, hipMemcpyAsync(c->faces, faces.data(), faces.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
     if (!c->ideal) {
         const size_t nf = nfp;

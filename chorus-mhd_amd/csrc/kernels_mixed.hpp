@@ -144,6 +144,33 @@ __device__ __forceinline__ float wave_speed(const float *Ql, const float *Qr, fl
     return 0.5f * fabsf((ul + ur) * nx + (vl + vr) * ny) + c_f;
 }
 
+// Rusanov jump terms of one face point for its left (ol) and right (orr)
+// cells from the two FP64 traces (see the header); sl = sign_l, slr =
+// sign_l sign_r; returns the FP32 wave speed.  d: the FP32 trace jump qr - ql.
+__device__ __forceinline__ float rusanov_jumps(const double *ql, const float *d, float s1, float s2, float sl,
+                                               float slr, double gam, float *ol, float *orr) {
+    Ref s;
+    ref_from(ql, gam, s);
+    const float gamf = float(gam);
+    float dF[8], dG[8];
+    dflux(s, d, gamf - 1.f, dF, dG);
+    const float mag = fsqrt(s1 * s1 + s2 * s2);
+    const float im = frcp(mag);
+    float Qr[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) Qr[v] = s.q[v] + d[v];
+    const float eig = wave_speed(s.q, Qr, s1 * im, s2 * im, gamf);
+    const float dis = 0.5f * sl * mag * eig;
+#pragma unroll
+    for (int v = 0; v < 8; v++) {
+        const float h = 0.5f * (dF[v] * s1 + dG[v] * s2);
+        const float e = dis * d[v];
+        ol[v] = h - e;
+        orr[v] = slr * (-h - e);
+    }
+    return eig;
+}
+
 // Phase A of the mixed cell kernels: the block's cells (contiguous in HBM)
 // become FP32 deviations from their reference states Qbar (the FP64 state at
 // the central solution point).  Each thread keeps the values it loaded in
@@ -363,20 +390,24 @@ __device__ __forceinline__ double az_at(const double *FA, const double *sAz, int
 
 // ---------------------------------------------------------------------------
 // Face pass, mixed form: the jump terms of the Rusanov flux for both cells of
-// each face point (see the header) and the A_z upwind jump; the stage maximum
-// of the FP32 wave speed with ARTIV.  A FREE face has no jump (zeros).
-// FF[cell][lfi][nfp][9] (float): 8 flux jump terms + A_z jump.
+// each face point (see the header), the FP64 upwind A_z, and on the viscous
+// path the half trace jumps; the stage maximum of the FP32 wave speed with
+// ARTIV.  A FREE / OUTL face has no jump (zeros).
+// FF[cell][lfi][nfp][8] (float): the 8 flux jump terms of the cell's side
+// (the viscous face kernel subtracts its flux from them in place).
 // ---------------------------------------------------------------------------
-template <int N, int NG = 4>
+// JUMPS = false (viscous path): the jump terms are formed by k_face_visc_mx
+// beside the viscous flux, and this pass keeps the wave-speed maximum, the
+// half trace jumps and the upwind A_z.
+template <int N, int NG = 4, bool JUMPS = true>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_face_inv_mx(FaceArgs a) {
-    // the block's 256 face points move through LDS as in k_face_inv: traces in
-    // with every load of a thread issued first, jump terms out, both with
-    // consecutive threads on consecutive values of a record
+    // the block's 256 face points' traces move through LDS as in k_face_inv
+    // (every load of a thread issued first, consecutive threads on consecutive
+    // values of a record); the 32-byte output records leave directly
     constexpr int FPB = 256, RW = 2 * NV, NFB = FPB / N + 2;
-    __shared__ double sT[FPB * RW];               // [p][left 9 | right 9] traces; then float jump terms
+    __shared__ double sT[FPB * RW];               // [p][left 9 | right 9] traces
     __shared__ int4 sF[NFB];
     __shared__ double red[256];
-    float *sO = reinterpret_cast<float *>(sT);
     const int tid = threadIdx.x;
     const int p0 = xcd_block(blockIdx.x, gridDim.x) * FPB;
     const int npt = min(FPB, a.nface * N - p0);
@@ -386,12 +417,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     gather_face_traces<N, FPB>(a.TQ, sF, f0, p0, npt, sT);
     __syncthreads();
     double emax = 0.0;
-    float out[RW];
-    bool free_face = true;
     if (tid < npt) {
         const int gp = p0 + tid, k = gp % N;
         const int4 fc = sF[gp / N - f0];
-        free_face = fc.z < 0;
+        const bool free_face = fc.z < 0;
         int ifp, jfp;
         face2fp<N>(fc.y + 1, k, ifp, jfp);
         double xy[2 * NG];
@@ -412,19 +441,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         if (free_face && fc.w != BK_ZEROGRAD) boundary_state(fc.w, ql, m1, m2, a.qin, qr);
 #pragma unroll
         for (int v = 0; v < NV; v++) d[v] = float(qr[v] - ql[v]);
-        Ref s;
-        ref_from(ql, a.gam, s);
-        const float gam = float(a.gam);
-        float dF[8], dG[8];
-        dflux(s, d, gam - 1.f, dF, dG);
         const float sl = float(side_sign(fc.y));
-        const float mag = fsqrt(s1 * s1 + s2 * s2);
-        const float im = frcp(mag);
-        float Qr[NV];
+        const float slr = free_face ? 0.f : sl * float(-side_sign(fc.w));
+        float ol[8], orr[8], eig;
+        if constexpr (JUMPS) {
+            eig = rusanov_jumps(ql, d, s1, s2, sl, slr, a.gam, ol, orr);
+        } else {
+            float Ql[NV], Qr[NV];
 #pragma unroll
-        for (int v = 0; v < NV; v++) Qr[v] = s.q[v] + d[v];
-        const float eig = wave_speed(s.q, Qr, s1 * im, s2 * im, gam);
-        const float dis = 0.5f * sl * mag * eig;
+            for (int v = 0; v < NV; v++) {
+                Ql[v] = float(ql[v]);
+                Qr[v] = Ql[v] + d[v];
+            }
+            const float im = frcp(fsqrt(s1 * s1 + s2 * s2));
+            eig = wave_speed(Ql, Qr, s1 * im, s2 * im, float(a.gam));
+        }
         const size_t rl = face_rec<N>(fc, k, false);
         if (a.HJ) {
             // the half trace jumps the gradient kernel's face points use (the
@@ -442,26 +473,29 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                 hr[1] = make_float4(-0.5f * d[4], -0.5f * d[5], -0.5f * d[6], -0.5f * d[7]);
             }
         }
+        float4 *fl = reinterpret_cast<float4 *>((float *)a.FF + (size_t)rl * 8);
         if (free_face) {
+            if constexpr (JUMPS) {
+                // the left cell's jump term against the boundary state (none
+                // at a FREE / OUTL face)
+                if (fc.w == BK_ZEROGRAD) {
 #pragma unroll
-            for (int v = 0; v < RW; v++) out[v] = 0.f;
-            if (fc.w != BK_ZEROGRAD) {
-                // the left cell's jump term against the boundary state
-#pragma unroll
-                for (int v = 0; v < 8; v++) out[v] = 0.5f * (dF[v] * s1 + dG[v] * s2) - dis * d[v];
+                    for (int v = 0; v < 8; v++) ol[v] = 0.f;
+                }
+                fl[0] = make_float4(ol[0], ol[1], ol[2], ol[3]);
+                fl[1] = make_float4(ol[4], ol[5], ol[6], ol[7]);
             }
             // A_z of the boundary state is the cell's own: the upwind value is too
             a.FA[rl] = ql[8];
         } else {
-            const float slr = sl * float(-side_sign(fc.w));
-#pragma unroll
-            for (int v = 0; v < 8; v++) {
-                const float h = 0.5f * (dF[v] * s1 + dG[v] * s2);
-                const float e = dis * d[v];
-                out[v] = h - e;
-                out[NV + v] = slr * (-h - e);
+            const unsigned rr = face_rec<N>(fc, k, true);
+            if constexpr (JUMPS) {
+                float4 *fr = reinterpret_cast<float4 *>((float *)a.FF + (size_t)rr * 8);
+                fl[0] = make_float4(ol[0], ol[1], ol[2], ol[3]);
+                fl[1] = make_float4(ol[4], ol[5], ol[6], ol[7]);
+                fr[0] = make_float4(orr[0], orr[1], orr[2], orr[3]);
+                fr[1] = make_float4(orr[4], orr[5], orr[6], orr[7]);
             }
-            out[8] = out[NV + 8] = 0.f;
             // upwind A_z by the left cell's outward normal velocity, evaluated in FP64
             // as the FP64 form does (getrusanov.f90:61-66, 98-103): near-zero normal
             // velocities then pick the same side, so A_z keeps the FP64 form's face values
@@ -471,21 +505,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             const bool take_left = rhoi * ql[1] * ux + rhoi * ql[2] * uy > 0.0;
             const double az = take_left ? ql[8] : sT[tid * RW + NV + 8];
             a.FA[rl] = az;
-            a.FA[face_rec<N>(fc, k, true)] = az;
+            a.FA[rr] = az;
         }
         emax = double(eig);
     }
-    __syncthreads();
-    if (tid < npt) {
-#pragma unroll
-        for (int v = 0; v < RW; v++) sO[tid * RW + v] = out[v];
-    }
-    __syncthreads();
-    // FF[cell][lfi][nfp][9] (float): the 8 jump terms of each side; a FREE face
-    // writes zeros (all 9) on its only side; the A_z slot of an interior face is
-    // not used
-    scatter_face_values<N, FPB>((float *)a.FF, sF, f0, p0, npt, sO,
-                                [](const int4 &fc, int w) { return fc.z < 0 ? w < NV : (w % NV) < 8; });
     if (a.artiv) {
         if (a.inst) {
             const int tt = min(p0 + tid, a.nface * N - 1);
@@ -564,9 +587,10 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
             if (r == 0 || r == N) {
                 int lfi, nfp;
                 face_of<N>(fam, r, line, lfi, nfp);
-                const float *src = FF + (size_t)(((c0 + c) * 4 + lfi) * N + nfp) * NV;
-#pragma unroll
-                for (int v = 0; v < 8; v++) jump[v] = src[v];
+                const float4 *src = reinterpret_cast<const float4 *>(FF + (size_t)(((c0 + c) * 4 + lfi) * N + nfp) * 8);
+                const float4 j0 = src[0], j1 = src[1];
+                jump[0] = j0.x; jump[1] = j0.y; jump[2] = j0.z; jump[3] = j0.w;
+                jump[4] = j1.x; jump[5] = j1.y; jump[6] = j1.z; jump[7] = j1.w;
             }
             double s1d, s2d;
             metric_row_g<N, NG>(sX + c * 2 * NG, fam, ifp, jfp, s1d, s2d, sOps + Ops64Layout<N>::XF, sOps + Ops64Layout<N>::XS);
@@ -842,14 +866,17 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
         for (int w = 0; w < NV; w++) qv[w] = sRef[c].q[w] + d[w];
         grad_trace<N, float>(sG + c * 16 * NP, fam, ifp, jfp, g, sLf);
         visc_point<N, float, VM>(a, sMu + c * 4, fam, ifp, jfp, s1, s2, qv, g, fv);
-        float *dst = fvi + (size_t)((c0 + c) * 2 + fam) * (NV * NIT) + k;
+        // (8 components: the viscous flux of A_z is zero)
+        float *dst = fvi + (size_t)((c0 + c) * 2 + fam) * (8 * NIT) + k;
 #pragma unroll
-        for (int w = 0; w < NV; w++) dst[w * NIT] = fv[w];
+        for (int w = 0; w < 8; w++) dst[w * NIT] = fv[w];
     }
 }
 
 // viscous and AV face flux, mixed form: k_face_visc in FP32 from the FP64
-// trace average
+// trace average, together with the Rusanov jump terms of the face point (the
+// viscous path's k_face_inv_mx leaves them to this pass): FF[cell side] =
+// jump - viscous flux
 template <int N, int VM, int NG = 4>
 __global__ void __launch_bounds__(256) k_face_visc_mx(FaceArgs a) {
     const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -859,14 +886,15 @@ __global__ void __launch_bounds__(256) k_face_visc_mx(FaceArgs a) {
     const bool free_face = fc.z < 0;
     const int kr = N - 1 - k;
     float sl1, sl2, sr1 = 0.f, sr2 = 0.f;
+    double ml1, ml2;
     {
         int ifp, jfp;
         double xy[2 * NG], m1, m2;
         face2fp<N>(fc.y + 1, k, ifp, jfp);
 #pragma unroll
         for (int q = 0; q < 2 * NG; q++) xy[q] = a.cellxy[(size_t)fc.x * (2 * NG) + q];
-        metric_row_g<N, NG>(xy, lf_family(fc.y + 1), ifp, jfp, m1, m2);
-        sl1 = float(m1); sl2 = float(m2);
+        metric_row_g<N, NG>(xy, lf_family(fc.y + 1), ifp, jfp, ml1, ml2);
+        sl1 = float(ml1); sl2 = float(ml2);
         if (!free_face) {
             face2fp<N>(fc.w + 1, kr, ifp, jfp);
 #pragma unroll
@@ -877,10 +905,39 @@ __global__ void __launch_bounds__(256) k_face_visc_mx(FaceArgs a) {
     }
     const size_t pl = (size_t)((fc.x * 4 + fc.y) * N + k);
     const size_t pr = free_face ? pl : (size_t)((fc.z * 4 + fc.w) * N + kr);
+    // both FP64 traces (a boundary face repeats its own): the central value of
+    // the viscous flux, and the Rusanov jump terms of the inviscid flux
+    double ql[NV], qt[NV];
+#pragma unroll
+    for (int v = 0; v < 8; v++) {
+        ql[v] = a.TQ[pl * NV + v];
+        qt[v] = a.TQ[pr * NV + v];
+    }
+    ql[8] = qt[8] = 0.0;
     float qv[NV], g[2 * NV];
 #pragma unroll
-    for (int v = 0; v < 8; v++) qv[v] = float(0.5 * a.TQ[pl * NV + v] + 0.5 * a.TQ[pr * NV + v]);
+    for (int v = 0; v < 8; v++) qv[v] = float(0.5 * ql[v] + 0.5 * qt[v]);
     qv[8] = 0.f;
+    float jl[8], jr[8];
+    {
+        // the right state of the inviscid flux: the other cell's trace; at an
+        // INLE / SYMP / WALL face that of the boundary kind; at a FREE / OUTL
+        // face the own trace (no jump)
+        double qr[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) qr[v] = qt[v];
+        if (free_face && fc.w != BK_ZEROGRAD) boundary_state(fc.w, ql, ml1, ml2, a.qin, qr);
+        float d[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) d[v] = float(qr[v] - ql[v]);
+        const float sl = float(side_sign(fc.y));
+        const float slr = free_face ? 0.f : sl * float(-side_sign(fc.w));
+        rusanov_jumps(ql, d, sl1, sl2, sl, slr, a.gam, jl, jr);
+        if (free_face && fc.w == BK_ZEROGRAD) {
+#pragma unroll
+            for (int v = 0; v < 8; v++) jl[v] = 0.f;
+        }
+    }
     const float *gl = (const float *)a.gtr + pl * 16, *gr = (const float *)a.gtr + pr * 16;
 #pragma unroll
     for (int d = 0; d < 2; d++) {
@@ -909,12 +966,15 @@ __global__ void __launch_bounds__(256) k_face_visc_mx(FaceArgs a) {
             fr[w] += Fv[w] * sr1 + Gv[w] * sr2;
         }
     }
-    float *FFv = (float *)a.FFv;
-#pragma unroll
-    for (int w = 0; w < NV; w++) FFv[pl * NV + w] = fl[w];
+    // each cell side's record: jump - viscous flux (components 0-7; the
+    // viscous flux of A_z is zero)
+    float4 *rl = reinterpret_cast<float4 *>((float *)a.FF + pl * 8);
+    rl[0] = make_float4(jl[0] - fl[0], jl[1] - fl[1], jl[2] - fl[2], jl[3] - fl[3]);
+    rl[1] = make_float4(jl[4] - fl[4], jl[5] - fl[5], jl[6] - fl[6], jl[7] - fl[7]);
     if (!free_face) {
-#pragma unroll
-        for (int w = 0; w < NV; w++) FFv[pr * NV + w] = fr[w];
+        float4 *rr = reinterpret_cast<float4 *>((float *)a.FF + pr * 8);
+        rr[0] = make_float4(jr[0] - fr[0], jr[1] - fr[1], jr[2] - fr[2], jr[3] - fr[3]);
+        rr[1] = make_float4(jr[4] - fr[4], jr[5] - fr[5], jr[6] - fr[6], jr[7] - fr[7]);
     }
 }
 
@@ -1026,7 +1086,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     const bool pact = (pt < CPB * NP) && (pc < ncl);
     const size_t base = (size_t)(c0 + pc) * NQ + pk;
     const int vg = 4 * grp;              // first variable of the thread's group
-    const float *FF = (const float *)a.FF, *FFv = (const float *)a.FFv, *FVI = v.fvi_r<float>();
+    const float *FF = (const float *)a.FF, *FVI = v.fvi_r<float>();
     float acc[4];
     double qa[5];
 #pragma unroll 1
@@ -1043,11 +1103,12 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
             if (r == 0 || r == N) {
                 int lfi, nfp;
                 face_of<N>(fam, r, line, lfi, nfp);
-                const size_t o = (size_t)(((c0 + c) * 4 + lfi) * N + nfp) * NV;
-#pragma unroll
-                for (int w = 0; w < 8; w++) extra[w] = FF[o + w] - FFv[o + w];
+                const float4 *src = reinterpret_cast<const float4 *>(FF + (size_t)(((c0 + c) * 4 + lfi) * N + nfp) * 8);
+                const float4 e0 = src[0], e1 = src[1];
+                extra[0] = e0.x; extra[1] = e0.y; extra[2] = e0.z; extra[3] = e0.w;
+                extra[4] = e1.x; extra[5] = e1.y; extra[6] = e1.z; extra[7] = e1.w;
             } else {
-                const float *src = FVI + (size_t)((c0 + c) * 2 + fam) * (NV * NIT) + line * (N - 1) + r - 1;
+                const float *src = FVI + (size_t)((c0 + c) * 2 + fam) * (8 * NIT) + line * (N - 1) + r - 1;
 #pragma unroll
                 for (int w = 0; w < 8; w++) extra[w] = -src[w * NIT];
             }
