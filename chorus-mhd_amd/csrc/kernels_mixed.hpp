@@ -171,6 +171,76 @@ __device__ __forceinline__ float rusanov_jumps(const double *ql, const float *d,
     return eig;
 }
 
+// FP32 metric of a straight-sided cell for the delta fluxes.  The bilinear
+// map's metric rows (mapder.f90:1-76) are linear along a line of flux points:
+//   fam 0: (y_eta, -x_eta) = (1 - psi) A + psi B,   psi = Xf(ifp)
+//   fam 1: (-y_psi, x_psi) from (1 - eta) C + eta D, eta = Xf(jfp)
+// with the edge vectors A = v4 - v1, B = v3 - v2, C = v2 - v1, D = v3 - v4,
+// differenced in FP64 and rounded once: e[8] = {Ax, Ay, Bx, By, Cx, Cy, Dx, Dy}.
+// The block's edge vectors are staged with the state (ncl*8 values from the
+// FP64 vertex coordinates of cells c0.., 8 doubles per cell).
+// gather() issues the coordinate loads with the rest of the staging; scatter()
+// writes the rounded differences to LDS before the block's staging barrier
+template <int CPB, int T> struct EdgeStage {
+    static constexpr int PER = (CPB * 8 + T - 1) / T;
+    double p[PER], m[PER];
+    __device__ __forceinline__ void gather(const double *xy, int ncl) {
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const int i = (int)threadIdx.x + j * T, c = i >> 3, k = i & 7;
+            // (plus, minus) coordinate of component k: x1..x4 at 0..3, y1..y4 at 4..7
+            const int pl = (0x62516273 >> (4 * k)) & 15, mi = (0x73405140 >> (4 * k)) & 15;
+            p[j] = i < ncl * 8 ? xy[c * 8 + pl] : 0.0;
+            m[j] = i < ncl * 8 ? xy[c * 8 + mi] : 0.0;
+        }
+    }
+    __device__ __forceinline__ void scatter(float *sE, int ncl) const {
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const int i = (int)threadIdx.x + j * T;
+            if (i < ncl * 8) sE[i] = float(p[j] - m[j]);
+        }
+    }
+};
+__device__ __forceinline__ void metric_row_f(const float *e, int fam, float x, float &s1, float &s2) {
+    if (fam == 0) {
+        s1 = (1.f - x) * e[1] + x * e[3];
+        s2 = -((1.f - x) * e[0] + x * e[2]);
+    } else {
+        s1 = -((1.f - x) * e[5] + x * e[7]);
+        s2 = (1.f - x) * e[4] + x * e[6];
+    }
+}
+// metric row of flux point (fam, r along the family, ifp, jfp) of block cell c
+// in FP32: from the staged edge vectors, or the FP64 8-node map (NG = 8)
+template <int N, int NG>
+__device__ __forceinline__ void metric_row_mx(const float *sE, const double *sX, const double *ops, int c, int fam,
+                                              int r, int ifp, int jfp, float &s1, float &s2) {
+    if constexpr (NG == 4) {
+        metric_row_f(sE + c * 8, fam, float(ops[Ops64Layout<N>::XF + r]), s1, s2);
+    } else {
+        double s1d, s2d;
+        metric_row_g<N, NG>(sX + c * 2 * NG, fam, ifp, jfp, s1d, s2d, ops + Ops64Layout<N>::XF,
+                            ops + Ops64Layout<N>::XS);
+        s1 = float(s1d);
+        s2 = float(s2d);
+    }
+}
+// 1/J at solution point (is, js) of block cell c in FP32 (J = x_psi y_eta - x_eta y_psi)
+template <int N, int NG>
+__device__ __forceinline__ float inv_jacobian_mx(const float *sE, const double *sX, const double *ops, int c, int is,
+                                                 int js) {
+    if constexpr (NG == 4) {
+        const float *e = sE + c * 8;
+        const float psi = float(ops[Ops64Layout<N>::XS + is]), eta = float(ops[Ops64Layout<N>::XS + js]);
+        const float xpsi = (1.f - eta) * e[4] + eta * e[6], ypsi = (1.f - eta) * e[5] + eta * e[7];
+        const float xeta = (1.f - psi) * e[0] + psi * e[2], yeta = (1.f - psi) * e[1] + psi * e[3];
+        return frcp(xpsi * yeta - xeta * ypsi);
+    } else {
+        return float(1.0 / jacobian_g<N, NG>(sX + c * 2 * NG, is, js, ops + Ops64Layout<N>::XS));
+    }
+}
+
 // Phase A of the mixed cell kernels: the block's cells (contiguous in HBM)
 // become FP32 deviations from their reference states Qbar (the FP64 state at
 // the central solution point).  Each thread keeps the values it loaded in
@@ -542,6 +612,7 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
     __shared__ double U[UD];
     __shared__ double sQ64[CPB * NQ];    // FP64 state (combined in place into the new state)
     __shared__ double sX[CPB * 2 * NG];
+    __shared__ float sE[NG == 4 ? CPB * 8 : 1];   // FP32 edge vectors of the block's cells
     __shared__ double sBar[CPB * NV];
     __shared__ Ref sRef[CPB];
     __shared__ double sAz[CPB * NP];     // FP64 A_z plane of each cell
@@ -562,8 +633,11 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
         ops.gather();
         OpTables64<N, T> ops64;
         ops64.gather();
+        EdgeStage<CPB, T> edges;
+        if constexpr (NG == 4) edges.gather(a.cellxy + (size_t)c0 * 2 * NG, ncl);
         stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, sBar, sRef, sD, sAz, 8, sQ64);
         scatter_lds<PX, T>(sX, hx, ncl * 2 * NG);
+        if constexpr (NG == 4) edges.scatter(sE, ncl);
         ops.scatter(sLf, sMf);
         ops64.scatter(sOps);
         __syncthreads();
@@ -592,9 +666,8 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
                 jump[0] = j0.x; jump[1] = j0.y; jump[2] = j0.z; jump[3] = j0.w;
                 jump[4] = j1.x; jump[5] = j1.y; jump[6] = j1.z; jump[7] = j1.w;
             }
-            double s1d, s2d;
-            metric_row_g<N, NG>(sX + c * 2 * NG, fam, ifp, jfp, s1d, s2d, sOps + Ops64Layout<N>::XF, sOps + Ops64Layout<N>::XS);
-            const float s1 = float(s1d), s2 = float(s2d);
+            float s1, s2;
+            metric_row_mx<N, NG>(sE, sX, sOps, c, fam, r, ifp, jfp, s1, s2);
             float d[NV], F[8], G[8];
             dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d, sLf);
             dflux(sRef[c], d, gam - 1.f, F, G);
@@ -657,17 +730,19 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
     using TS = V2ShapeT<N, TB>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ, T = TS::THREADS, NIT = TS::NIT;
     constexpr int VST = 10 * NF + 1;
-    constexpr int UF = CPB * NQ + CPB * VST + CPB * 16 * NP;
+    constexpr int GO = (CPB * NQ + CPB * VST + 3) / 4 * 4;   // sG offset (16-byte aligned)
+    constexpr int UF = GO + CPB * 16 * NP;
     constexpr int UD = (UF + 1) / 2 > CPB * NQ ? (UF + 1) / 2 : CPB * NQ;
-    __shared__ double U[UD];
+    __shared__ __align__(16) double U[UD];
     __shared__ double sX[CPB * 2 * NG];
+    __shared__ float sE[NG == 4 ? CPB * 8 : 1];   // FP32 edge vectors of the block's cells
     __shared__ double sBar[CPB * NV];
     __shared__ Ref sRef[CPB];
     __shared__ int2 sNbr[CPB * 4];
     __shared__ double sMu[CPB * 4];      // muAVfi/fj(1,side) per local face
     __shared__ float sLf[(N + 1) * N], sMf[(N + 1) * N];
     __shared__ double sOps[Ops64Layout<N>::SIZE];   // FP64 operator tables (OpTables64 layout)
-    float *sD = (float *)U, *sV = sD + CPB * NQ, *sG = sV + CPB * VST;
+    float *sD = (float *)U, *sV = sD + CPB * NQ, *sG = sD + GO;   // sG: nablaQs, point-major [c][p][16]
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = blk * CPB;
@@ -682,8 +757,11 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
         ops.gather();
         OpTables64<N, T> ops64;
         ops64.gather();
+        EdgeStage<CPB, T> edges;
+        if constexpr (NG == 4) edges.gather(a.cellxy + (size_t)c0 * 2 * NG, ncl);
         stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, sBar, sRef, sD, nullptr);
         scatter_lds<PX, T>(sX, hx, ncl * 2 * NG);
+        if constexpr (NG == 4) edges.scatter(sE, ncl);
         scatter_lds<PN, T>(sNbr, hn, ncl * 4);
         ops.scatter(sLf, sMf);
         ops64.scatter(sOps);
@@ -741,8 +819,8 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
             }
             if (c >= ncl) continue;
             const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
-            double s1d, s2d;
-            metric_row_g<N, NG>(sX + c * 2 * NG, fam, ifp, jfp, s1d, s2d, sOps + Ops64Layout<N>::XF, sOps + Ops64Layout<N>::XS);
+            float s1d, s2d;
+            metric_row_mx<N, NG>(sE, sX, sOps, c, fam, r, ifp, jfp, s1d, s2d);
             float d[NV];
             dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d, sLf);
             float *q = sV + c * VST + line * (N + 1) + r;
@@ -782,7 +860,7 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
                     const int pos = 4 * kq + j;
                     if (pos < N) {
                         const int k = fam == 0 ? line * N + pos : pos * N + line;
-                        float *g = sG + (c * 16 + cgrp * 8 + cv) * NP + k;
+                        float *g = sG + (c * NP + k) * 16 + cgrp * 8 + cv;
                         *g = fam == 0 ? acc[j] : (*g + acc[j]) * sIJ[c * NP + k];
                     }
                 }
@@ -807,9 +885,10 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
 #pragma unroll
                 for (int v = 0; v < 8; v++) gp[v] = acc[v];
             } else {
-                const float iJ = float(1.0 / jacobian_g<N, NG>(sX + pc * 2 * NG, pis, pjs, sOps + Ops64Layout<N>::XS));
+                const float iJ = inv_jacobian_mx<N, NG>(sE, sX, sOps, pc, pis, pjs);
 #pragma unroll
-                for (int v = 0; v < 8; v++) sG[(pc * 16 + grp * 8 + v) * NP + pk] = (gp[v] + acc[v]) * iJ;
+                for (int v = 0; v < 8; v++) gp[v] = (gp[v] + acc[v]) * iJ;
+                lds_store8(sG + (pc * NP + pk) * 16 + grp * 8, gp);
             }
         }
 #endif
@@ -828,14 +907,18 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
 #pragma unroll
         for (int sp = 0; sp < N; sp++) Lr[sp] = sLf[lrow * N + sp];
         float *dst = a.gtr_r<float>() + (size_t)((c0 + c) * 4 * N + fpi) * 16 + h * 8;
+        float acc[8];
 #pragma unroll
-        for (int gg = 0; gg < 8; gg++) {
-            const float *gq = sG + (c * 16 + h * 8 + gg) * NP + base;
-            float acc = 0.f;
+        for (int gg = 0; gg < 8; gg++) acc[gg] = 0.f;
 #pragma unroll
-            for (int sp = 0; sp < N; sp++) acc += gq[sp * stride] * Lr[sp];
-            dst[gg] = acc;
+        for (int sp = 0; sp < N; sp++) {
+            float v[8];
+            lds_load8(sG + (c * NP + base + sp * stride) * 16 + h * 8, v);
+#pragma unroll
+            for (int gg = 0; gg < 8; gg++) acc[gg] += v[gg] * Lr[sp];
         }
+        reinterpret_cast<float4 *>(dst)[0] = make_float4(acc[0], acc[1], acc[2], acc[3]);
+        reinterpret_cast<float4 *>(dst)[1] = make_float4(acc[4], acc[5], acc[6], acc[7]);
     }
     // viscous / AV flux of the own traces at the interior flux points of both
     // families (comp_viscous_flux, compflux.f90:205-256), for the stage kernel
@@ -856,15 +939,12 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
             s1 = q[8 * NF];
             s2 = q[9 * NF];
         } else {
-            double s1d, s2d;
-            metric_row_g<N, NG>(sX + c * 2 * NG, fam, ifp, jfp, s1d, s2d, sOps + Ops64Layout<N>::XF, sOps + Ops64Layout<N>::XS);
-            s1 = float(s1d);
-            s2 = float(s2d);
+            metric_row_mx<N, NG>(sE, sX, sOps, c, fam, r, ifp, jfp, s1, s2);
             dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d, sLf);
         }
 #pragma unroll
         for (int w = 0; w < NV; w++) qv[w] = sRef[c].q[w] + d[w];
-        grad_trace<N, float>(sG + c * 16 * NP, fam, ifp, jfp, g, sLf);
+        grad_trace_pm<N, float>(sG + c * NP * 16, fam, ifp, jfp, g, sLf);
         visc_point<N, float, VM>(a, sMu + c * 4, fam, ifp, jfp, s1, s2, qv, g, fv);
         // (8 components: the viscous flux of A_z is zero)
         float *dst = fvi + (size_t)((c0 + c) * 2 + fam) * (8 * NIT) + k;
@@ -1040,6 +1120,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     __shared__ double U[UD];
     __shared__ double sQ64[CPB * NQ];    // FP64 state (combined in place into the new state)
     __shared__ double sX[CPB * 2 * NG];
+    __shared__ float sE[NG == 4 ? CPB * 8 : 1];   // FP32 edge vectors of the block's cells
     __shared__ double sBar[CPB * NV];
     __shared__ Ref sRef[CPB];
     __shared__ double sAV[CPB * NP];
@@ -1069,8 +1150,11 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
         OpTables64<N, T> ops64;
         ops64.gather();
         gather_regs<PX, T>(hx, a.cellxy + (size_t)c0 * 2 * NG, ncl * 2 * NG);
+        EdgeStage<CPB, T> edges;
+        if constexpr (NG == 4) edges.gather(a.cellxy + (size_t)c0 * 2 * NG, ncl);
         stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, sBar, sRef, sD, sAz, 8, sQ64);
         scatter_lds<PX, T>(sX, hx, ncl * 2 * NG);
+        if constexpr (NG == 4) edges.scatter(sE, ncl);
         ops.scatter(sLf, sMf);
         ops64.scatter(sOps);
     }
@@ -1112,9 +1196,8 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
 #pragma unroll
                 for (int w = 0; w < 8; w++) extra[w] = -src[w * NIT];
             }
-            double s1d, s2d;
-            metric_row_g<N, NG>(sX + c * 2 * NG, fam, ifp, jfp, s1d, s2d, sOps + Ops64Layout<N>::XF, sOps + Ops64Layout<N>::XS);
-            const float s1 = float(s1d), s2 = float(s2d);
+            float s1, s2;
+            metric_row_mx<N, NG>(sE, sX, sOps, c, fam, r, ifp, jfp, s1, s2);
             float d[NV], F[8], G[8];
             dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d, sLf);
             dflux(sRef[c], d, gam - 1.f, F, G);
@@ -1148,8 +1231,9 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
         __syncthreads();
     }
     if (pact) {
-        const double iJd = 1.0 / jacobian_g<N, NG>(sX + pc * 2 * NG, pis, pjs, sOps + Ops64Layout<N>::XS);
-        const float iJ = float(iJd);
+        // 1/J: FP64 for the A_z residual (group 0), FP32 for the rest
+        const double iJd = grp == 0 ? 1.0 / jacobian_g<N, NG>(sX + pc * 2 * NG, pis, pjs, sOps + Ops64Layout<N>::XS) : 0.0;
+        const float iJ = grp == 0 ? float(iJd) : inv_jacobian_mx<N, NG>(sE, sX, sOps, pc, pis, pjs);
         float res[4];
 #pragma unroll
         for (int w = 0; w < 4; w++) res[w] = -acc[w] * iJ;

@@ -183,6 +183,55 @@ __device__ __forceinline__ void grad_trace(const R *gq, int fam, int ifp, int jf
     }
 }
 
+// 8 consecutive values of LDS at p (16-byte aligned) with 16-byte accesses
+__device__ __forceinline__ void lds_load8(const float *p, float *v) {
+    const float4 a = reinterpret_cast<const float4 *>(p)[0], b = reinterpret_cast<const float4 *>(p)[1];
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+__device__ __forceinline__ void lds_load8(const double *p, double *v) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double2 a = reinterpret_cast<const double2 *>(p)[k];
+        v[2 * k] = a.x;
+        v[2 * k + 1] = a.y;
+    }
+}
+__device__ __forceinline__ void lds_store8(float *p, const float *v) {
+    reinterpret_cast<float4 *>(p)[0] = make_float4(v[0], v[1], v[2], v[3]);
+    reinterpret_cast<float4 *>(p)[1] = make_float4(v[4], v[5], v[6], v[7]);
+}
+__device__ __forceinline__ void lds_store8(double *p, const double *v) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) reinterpret_cast<double2 *>(p)[k] = make_double2(v[2 * k], v[2 * k + 1]);
+}
+
+// the same with the cell's nablaQs point-major (mixed form: 16-byte LDS reads)
+template <int N, class R>
+__device__ __forceinline__ void grad_trace_pm(const R *gc, int fam, int ifp, int jfp, R *g, const R *Lt = nullptr) {
+    // gc: the cell's nablaQs, point-major: [p][d*8+v] (16 values per point)
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+#pragma unroll
+        for (int w = 0; w < 8; w++) g[d * NV + w] = R(0);
+        g[d * NV + 8] = R(0);
+    }
+    // contraction index outermost: 16 loads in flight per step
+    constexpr int U = sizeof(R) == 8 ? contraction_unroll<R, N>() : CHORUS_MX_GRAD_UNROLL;
+#pragma unroll U
+    for (int sp = 0; sp < N; sp++) {
+        const int row = fam == 0 ? ifp : jfp;
+        const R l = Lt ? Lt[row * N + sp] : Lop<R>(N, row, sp);
+        const int idx = fam == 0 ? jfp * N + sp : sp * N + ifp;
+        R v[16];
+        lds_load8(gc + idx * 16, v);
+        lds_load8(gc + idx * 16 + 8, v + 8);
+#pragma unroll
+        for (int d = 0; d < 2; d++)
+#pragma unroll
+            for (int w = 0; w < 8; w++) g[d * NV + w] = g[d * NV + w] + v[d * 8 + w] * l;
+    }
+}
+
 // viscous + AV contravariant flux at one flux point (comp_viscous_flux, compflux.f90:205-256)
 // VM: bit 0 physical viscosity/resistivity (mu or eta0 nonzero), bit 1 AV
 template <int N, class R, int VM>
