@@ -1105,8 +1105,11 @@ __device__ __forceinline__ void combine_vars(const StageArgs &a, size_t base, in
 // k_face_visc_mx at face points, the interior viscous flux of k_grad_mx (fvi)
 // elsewhere; the viscous flux of A_z is zero (getrusanov.f90:157-361), so A_z
 // keeps its FP64 upwind/trace values alone.  Then one sum per component.
+// waves per SIMD requested from the register allocator: 5 (at most 96 VGPRs)
+// at N = 5, where it spills nothing; N = 3, 4 need fewer registers anyway and
+// N = 6 would spill (measured per order and tile shape)
 #ifndef CHORUS_MX_VISC_MINW
-#define CHORUS_MX_VISC_MINW(N) ((N) <= 6 ? 4 : 3)
+#define CHORUS_MX_VISC_MINW(N) ((N) == 5 ? 5 : (N) <= 6 ? 4 : 3)
 #endif
 template <int N, int TB, int VM, int NG = 4>
 __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amdgpu_waves_per_eu(CHORUS_MX_VISC_MINW(N)))) k_stage_visc_mx(StageArgs a, ViscArgs v) {
