@@ -177,6 +177,32 @@ __device__ __forceinline__ void stage_combine(const StageArgs &a, size_t base, i
     }
 }
 
+// combination of the variables v0 .. v0+NVG-1 of a point (stage_combine for a
+// subset): qs the point's FP64 state in LDS, qa its prefetched Qa, res the
+// residuals of the subset, r8 the A_z residual when the subset holds var 8
+template <int NVG, class R>
+__device__ __forceinline__ void combine_vars(const StageArgs &a, size_t base, int NP, double *qs, int v0,
+                                             const double *qa, const R *res, double r8 = 0.0) {
+#pragma unroll
+    for (int k = 0; k < NVG; k++) {
+        const int v = v0 + k;
+        const double r = v == 8 ? r8 : double(res[k]);
+        const size_t o = base + (size_t)v * NP;
+        const double q = qs[v * NP];
+        double qn;
+        switch (a.kind) {
+        case KIND_RESID: a.Rout[o] = r; continue;
+        case KIND_AXPY: qn = q + a.c0 * r; break;
+        case KIND_3A: qn = a.c1 * qa[k] + a.c2 * q + a.c0 * r; break;
+        case KIND_3B: qn = a.c1 * qa[k] + a.c2 * q + a.c3 * r * a.c4; break;
+        case KIND_S54_4: qn = a.c1 * qa[k] + a.c2 * q + a.c0 * r; a.Rout[o] = r; break;
+        default: qn = a.c1 * qa[k] + a.c2 * a.Qb[o] + a.c3 * a.Qc[o] + a.c4 * q + a.c5 * a.Rprev[o] + a.c0 * r; break;
+        }
+        a.Qout[o] = qn;
+        qs[v * NP] = qn;
+    }
+}
+
 // register prefetch of a point's Qa for the stage kinds that combine with it
 __device__ __forceinline__ void prefetch_qa(const StageArgs &a, size_t base, int NP, double *qa) {
     const bool need = a.kind != KIND_RESID && a.kind != KIND_AXPY;

@@ -1071,31 +1071,6 @@ template <int N, int TB> struct MxVShape {
     static constexpr int THREADS = (((NA > CT ? NA : CT) + 63) / 64) * 64;
 };
 
-// combination of the variables v0 .. v0+NVG-1 of a point (combine_point for a
-// subset): qs the point's FP64 state in LDS, qa its prefetched Qa, res the
-// residuals of the subset, r8 the A_z residual when the subset holds var 8
-template <int NVG>
-__device__ __forceinline__ void combine_vars(const StageArgs &a, size_t base, int NP, double *qs, int v0,
-                                             const double *qa, const float *res, double r8 = 0.0) {
-#pragma unroll
-    for (int k = 0; k < NVG; k++) {
-        const int v = v0 + k;
-        const double r = v == 8 ? r8 : double(res[k]);
-        const size_t o = base + (size_t)v * NP;
-        const double q = qs[v * NP];
-        double qn;
-        switch (a.kind) {
-        case KIND_RESID: a.Rout[o] = r; continue;
-        case KIND_AXPY: qn = q + a.c0 * r; break;
-        case KIND_3A: qn = a.c1 * qa[k] + a.c2 * q + a.c0 * r; break;
-        case KIND_3B: qn = a.c1 * qa[k] + a.c2 * q + a.c3 * r * a.c4; break;
-        case KIND_S54_4: qn = a.c1 * qa[k] + a.c2 * q + a.c0 * r; a.Rout[o] = r; break;
-        default: qn = a.c1 * qa[k] + a.c2 * a.Qb[o] + a.c3 * a.Qc[o] + a.c4 * q + a.c5 * a.Rprev[o] + a.c0 * r; break;
-        }
-        a.Qout[o] = qn;
-        qs[v * NP] = qn;
-    }
-}
 
 // Viscous / AV stage kernel, mixed form.  The mixed form keeps no reference
 // association, so the inviscid and viscous contributions of a flux point are

@@ -674,30 +674,52 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
         }
         __syncthreads();
     }
-    // group 1 hands its viscous sums to group 0 through the (now free) flux area
+    // the residual and the stage combination are split over both groups
+    // (group 0: variables 0-3 and A_z, group 1: variables 4-7); each hands the
+    // sums of the other group's variables through the (now free) flux area
     R *xv = sU;
-    if (pact && grp == 1) {
+    if (pact) {
+        if (grp == 0) {
 #pragma unroll
-        for (int w = 0; w < NV; w++) xv[w * CPB * NP + pt] = accv[w];
+            for (int w = 4; w < 8; w++) xv[w * CPB * NP + pt] = acc8[w];
+        } else {
+#pragma unroll
+            for (int w = 0; w < 4; w++) xv[w * CPB * NP + pt] = accv[w];
+            xv[8 * CPB * NP + pt] = accv[8];
+        }
+    }
+    // the point's Qa, loaded across the barrier
+    double qa[5];
+    if (pact) {
+        const bool need = a.kind != KIND_RESID && a.kind != KIND_AXPY;
+        const int vg = 4 * grp;
+#pragma unroll
+        for (int k = 0; k < 4; k++) qa[k] = need ? a.Qa[base + (size_t)(vg + k) * NP] : 0.0;
+        qa[4] = need && grp == 0 ? a.Qa[base + (size_t)8 * NP] : 0.0;
     }
     __syncthreads();
-    if (pact && grp == 0) {
+    if (pact) {
         // residual at the solution point (compresid.f90:12-65)
         const R J = R(jacobian_g<N, NG>(sX + pc * 2 * NG, pis, pjs, sOps + OXS));
         double *qs = sQ + pc * NQ + pk;
-        const R q0 = R(qs[0]), q1 = R(qs[NP]), q2 = R(qs[2 * NP]);
-        const R au = q1 / q0, av = q2 / q0;
-        const R nA1 = dA1 / J, nA2 = dA2 / J;
-        R res[NV];
+        if (grp == 0) {
+            const R q0 = R(qs[0]), q1 = R(qs[NP]), q2 = R(qs[2 * NP]);
+            const R au = q1 / q0, av = q2 / q0;
+            const R nA1 = dA1 / J, nA2 = dA2 / J;
+            R res[4];
 #pragma unroll
-        for (int w = 0; w < 8; w++) res[w] = acc8[w];
-        res[8] = -(au * nA1) - av * nA2;
-        res[8] = res[8] * J;
+            for (int w = 0; w < 4; w++) res[w] = (acc8[w] + xv[w * CPB * NP + pt]) / J;
+            R r8 = -(au * nA1) - av * nA2;
+            r8 = r8 * J;
+            r8 = (r8 + xv[8 * CPB * NP + pt]) / J;
+            combine_vars<4>(a, base, NP, qs, 0, qa, res);
+            combine_vars<1>(a, base, NP, qs, 8, qa + 4, res, r8);
+        } else {
+            R res[4];
 #pragma unroll
-        for (int w = 0; w < NV; w++) res[w] = res[w] + xv[w * CPB * NP + pt];
-#pragma unroll
-        for (int w = 0; w < NV; w++) res[w] = res[w] / J;
-        stage_combine(a, base, NP, qs, res);
+            for (int w = 0; w < 4; w++) res[w] = (xv[(4 + w) * CPB * NP + pt] + accv[4 + w]) / J;
+            combine_vars<4>(a, base, NP, qs, 4, qa, res);
+        }
     }
     if (a.kind == KIND_RESID) return;
     __syncthreads();
