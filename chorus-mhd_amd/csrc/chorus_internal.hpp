@@ -116,7 +116,8 @@ struct ViscArgs {
     const int2 *nbr;
     const double *Qin;
     void *gtr;          // own nablaQs traces [cell][lf][nfp][16]  (Real)
-    void *fvi;          // interior viscous/AV fluxes [cell][fam][9][N*(N-1)] (Real), written by the gradient kernel
+    void *fvi;          // written by the gradient kernel: FP64 form, the interior viscous/AV fluxes
+                        // [cell][fam][9][N*(N-1)]; mixed form, their divergence part [cell][point][8] (FP32)
     const int *inst;    // batched instances: instance of each cell (null: one instance)
     const double *hinst;   // batched instances: AV length scale per instance
     double *sm;         // log10 smoothness indicator of density per cell

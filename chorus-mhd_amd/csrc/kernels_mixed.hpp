@@ -721,8 +721,8 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
 // FP64 traces at face points); nablaQs, its traces and the viscous fluxes are
 // plain FP32 (they carry no large constant part); the viscous face flux is
 // evaluated once per face point from the FP64 trace average.  k_grad_mx also
-// forms the viscous and AV fluxes of the interior flux points, which the stage
-// kernel copies (fvi, as in the FP64 path).
+// forms the viscous and AV fluxes of the interior flux points and their part
+// of the divergence at the solution points (fvi), which the stage kernel adds.
 // ---------------------------------------------------------------------------
 template <int N, int TB, int VM, int NG = 4>
 __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs a) {
@@ -921,13 +921,20 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
         reinterpret_cast<float4 *>(dst)[1] = make_float4(acc[4], acc[5], acc[6], acc[7]);
     }
     // viscous / AV flux of the own traces at the interior flux points of both
-    // families (comp_viscous_flux, compflux.f90:205-256), for the stage kernel
+    // families (comp_viscous_flux, compflux.f90:205-256), one task per thread
     // (tasks family-major, so that a wave's tasks share a family: those of
-    // family 1 read the delta traces and metric rows its flux phase left in sV)
-    float *fvi = a.fvi_r<float>();
-    for (int it = tid; it < 2 * CPB * NIT; it += T) {
+    // family 1 read the delta traces and metric rows its flux phase left in sV);
+    // then their part of the divergence at the solution points, for the stage
+    // kernel: fvi[cell][point][8] = sum over interior r of M(r, pos) Fv(r), both
+    // families (the face points' viscous flux joins the stage kernel's sum
+    // through the face records)
+    static_assert(2 * CPB * NIT <= T, "one interior flux task per thread");
+    float fvr[8];
+    int tfam = -1, tc = 0, tk = 0;
+    {
+        const int it = tid;
         const int fam = it / (CPB * NIT), k1 = it % (CPB * NIT), c = k1 / NIT, k = k1 % NIT;
-        if (c >= ncl) continue;
+        if (it < 2 * CPB * NIT && c < ncl) {
         const int line = k / (N - 1), r = 1 + k % (N - 1);
         const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
         float d[NV], qv[NV], g[2 * NV], fv[NV], s1, s2;
@@ -947,9 +954,33 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
         grad_trace_pm<N, float>(sG + c * NP * 16, fam, ifp, jfp, g, sLf);
         visc_point<N, float, VM>(a, sMu + c * 4, fam, ifp, jfp, s1, s2, qv, g, fv);
         // (8 components: the viscous flux of A_z is zero)
-        float *dst = fvi + (size_t)((c0 + c) * 2 + fam) * (8 * NIT) + k;
 #pragma unroll
-        for (int w = 0; w < 8; w++) dst[w * NIT] = fv[w];
+        for (int w = 0; w < 8; w++) fvr[w] = fv[w];
+        tfam = fam;
+        tc = c;
+        tk = k;
+        }
+    }
+    __syncthreads();   // the deviations, flux planes and gradient in LDS are free now
+    float *sFv = sD;   // [fam][c][w][line][r-1]
+    if (tfam >= 0) {
+#pragma unroll
+        for (int w = 0; w < 8; w++) sFv[((tfam * CPB + tc) * 8 + w) * NIT + tk] = fvr[w];
+    }
+    __syncthreads();
+    if (pact) {
+        const int vg = 4 * grp;
+        float p4[4] = {0.f, 0.f, 0.f, 0.f};
+        const float *f0 = sFv + ((0 * CPB + pc) * 8 + vg) * NIT + pjs * (N - 1);
+        const float *f1 = sFv + ((1 * CPB + pc) * 8 + vg) * NIT + pis * (N - 1);
+#pragma unroll
+        for (int r = 1; r < N; r++) {
+            const float m0 = sMf[r * N + pis], m1 = sMf[r * N + pjs];
+#pragma unroll
+            for (int w = 0; w < 4; w++) p4[w] += f0[w * NIT + r - 1] * m0 + f1[w * NIT + r - 1] * m1;
+        }
+        float *dst = a.fvi_r<float>() + ((size_t)(c0 + pc) * NP + pk) * 8 + vg;
+        reinterpret_cast<float4 *>(dst)[0] = make_float4(p4[0], p4[1], p4[2], p4[3]);
     }
 }
 
@@ -1077,9 +1108,10 @@ template <int N, int TB> struct MxVShape {
 // combined before the derivative: every flux task forms
 //   u = dF1(own delta trace) + jump - Fv        (8 components)
 // with the jump terms of k_face_inv_mx and the viscous face flux of
-// k_face_visc_mx at face points, the interior viscous flux of k_grad_mx (fvi)
-// elsewhere; the viscous flux of A_z is zero (getrusanov.f90:157-361), so A_z
-// keeps its FP64 upwind/trace values alone.  Then one sum per component.
+// k_face_visc_mx at face points; the interior viscous fluxes enter as their
+// divergence at the solution points, formed by k_grad_mx (fvi).  The viscous
+// flux of A_z is zero (getrusanov.f90:157-361), so A_z keeps its FP64
+// upwind/trace values alone.  Then one sum per component.
 // waves per SIMD requested from the register allocator: 5 (at most 96 VGPRs)
 // at N = 5, where it spills nothing; N = 3, 4 need fewer registers anyway and
 // N = 6 would spill (measured per order and tile shape)
@@ -1151,6 +1183,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     const float *FF = (const float *)a.FF, *FVI = v.fvi_r<float>();
     float acc[4];
     double qa[5];
+    float4 pv = make_float4(0.f, 0.f, 0.f, 0.f);   // the interior viscous fluxes' divergence (k_grad_mx)
 #pragma unroll 1
     for (int fam = 0; fam < 2; fam++) {
         for (int t = tid; t < TS::NA; t += T) {
@@ -1170,9 +1203,9 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
                 extra[0] = e0.x; extra[1] = e0.y; extra[2] = e0.z; extra[3] = e0.w;
                 extra[4] = e1.x; extra[5] = e1.y; extra[6] = e1.z; extra[7] = e1.w;
             } else {
-                const float *src = FVI + (size_t)((c0 + c) * 2 + fam) * (8 * NIT) + line * (N - 1) + r - 1;
+                // (the interior viscous flux enters through k_grad_mx's partial divergence)
 #pragma unroll
-                for (int w = 0; w < 8; w++) extra[w] = -src[w * NIT];
+                for (int w = 0; w < 8; w++) extra[w] = 0.f;
             }
             float s1, s2;
             metric_row_mx<N, NG>(sE, sX, sOps, c, fam, r, ifp, jfp, s1, s2);
@@ -1190,6 +1223,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
 #pragma unroll
             for (int k = 0; k < 4; k++) qa[k] = need ? a.Qa[base + (size_t)(vg + k) * NP] : 0.0;
             qa[4] = need && grp == 0 ? a.Qa[base + (size_t)8 * NP] : 0.0;
+            pv = reinterpret_cast<const float4 *>(FVI + ((size_t)(c0 + pc) * NP + pk) * 8 + vg)[0];
         }
         if (pact) {
             const int line = fam == 0 ? pjs : pis, pos = fam == 0 ? pis : pjs;
@@ -1213,8 +1247,10 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
         const double iJd = grp == 0 ? 1.0 / jacobian_g<N, NG>(sX + pc * 2 * NG, pis, pjs, sOps + Ops64Layout<N>::XS) : 0.0;
         const float iJ = grp == 0 ? float(iJd) : inv_jacobian_mx<N, NG>(sE, sX, sOps, pc, pis, pjs);
         float res[4];
-#pragma unroll
-        for (int w = 0; w < 4; w++) res[w] = -acc[w] * iJ;
+        res[0] = (pv.x - acc[0]) * iJ;
+        res[1] = (pv.y - acc[1]) * iJ;
+        res[2] = (pv.z - acc[2]) * iJ;
+        res[3] = (pv.w - acc[3]) * iJ;
         double *qs = sQ64 + pc * NQ + pk;
         if (grp == 0) {
             // A_z residual in FP64 from the flux-point values of both families
