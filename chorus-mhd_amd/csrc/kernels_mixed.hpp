@@ -952,7 +952,7 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
 #pragma unroll
         for (int w = 0; w < NV; w++) qv[w] = sRef[c].q[w] + d[w];
         grad_trace_pm<N, float>(sG + c * NP * 16, fam, ifp, jfp, g, sLf);
-        visc_point<N, float, VM>(a, sMu + c * 4, fam, ifp, jfp, s1, s2, qv, g, fv);
+        visc_point<N, float, VM>(a, sMu + c * 4, fam, ifp, jfp, s1, s2, qv, g, fv, sOps + Ops64Layout<N>::XF);
         // (8 components: the viscous flux of A_z is zero)
 #pragma unroll
         for (int w = 0; w < 8; w++) fvr[w] = fv[w];
@@ -1016,31 +1016,39 @@ __global__ void __launch_bounds__(256) k_face_visc_mx(FaceArgs a) {
     }
     const size_t pl = (size_t)((fc.x * 4 + fc.y) * N + k);
     const size_t pr = free_face ? pl : (size_t)((fc.z * 4 + fc.w) * N + kr);
-    // both FP64 traces (a boundary face repeats its own): the central value of
-    // the viscous flux, and the Rusanov jump terms of the inviscid flux
-    double ql[NV], qt[NV];
+    // the own FP64 trace and the half trace jump (other - own)/2 of the
+    // inviscid face pass (zero at a boundary face): the central value of the
+    // viscous flux, and the Rusanov jump terms of the inviscid flux
+    double ql[NV];
 #pragma unroll
-    for (int v = 0; v < 8; v++) {
-        ql[v] = a.TQ[pl * NV + v];
-        qt[v] = a.TQ[pr * NV + v];
+    for (int v = 0; v < 8; v++) ql[v] = a.TQ[pl * NV + v];
+    ql[8] = 0.0;
+    float hj[8];
+    {
+        const float4 *h = reinterpret_cast<const float4 *>(a.HJ + pl * 8);
+        const float4 h0 = h[0], h1 = h[1];
+        hj[0] = h0.x; hj[1] = h0.y; hj[2] = h0.z; hj[3] = h0.w;
+        hj[4] = h1.x; hj[5] = h1.y; hj[6] = h1.z; hj[7] = h1.w;
     }
-    ql[8] = qt[8] = 0.0;
     float qv[NV], g[2 * NV];
 #pragma unroll
-    for (int v = 0; v < 8; v++) qv[v] = float(0.5 * ql[v] + 0.5 * qt[v]);
+    for (int v = 0; v < 8; v++) qv[v] = float(ql[v]) + hj[v];
     qv[8] = 0.f;
     float jl[8], jr[8];
     {
-        // the right state of the inviscid flux: the other cell's trace; at an
-        // INLE / SYMP / WALL face that of the boundary kind; at a FREE / OUTL
-        // face the own trace (no jump)
-        double qr[NV];
-#pragma unroll
-        for (int v = 0; v < NV; v++) qr[v] = qt[v];
-        if (free_face && fc.w != BK_ZEROGRAD) boundary_state(fc.w, ql, ml1, ml2, a.qin, qr);
+        // the jump to the right state of the inviscid flux: twice the half
+        // jump (exact); at an INLE / SYMP / WALL face that of the boundary
+        // kind's state; none at a FREE / OUTL face
         float d[NV];
 #pragma unroll
-        for (int v = 0; v < NV; v++) d[v] = float(qr[v] - ql[v]);
+        for (int v = 0; v < 8; v++) d[v] = 2.f * hj[v];
+        d[8] = 0.f;
+        if (free_face && fc.w != BK_ZEROGRAD) {
+            double qr[NV];
+            boundary_state(fc.w, ql, ml1, ml2, a.qin, qr);
+#pragma unroll
+            for (int v = 0; v < NV; v++) d[v] = float(qr[v] - ql[v]);
+        }
         const float sl = float(side_sign(fc.y));
         const float slr = free_face ? 0.f : sl * float(-side_sign(fc.w));
         rusanov_jumps(ql, d, sl1, sl2, sl, slr, a.gam, jl, jr);

@@ -234,9 +234,10 @@ __device__ __forceinline__ void grad_trace_pm(const R *gc, int fam, int ifp, int
 
 // viscous + AV contravariant flux at one flux point (comp_viscous_flux, compflux.f90:205-256)
 // VM: bit 0 physical viscosity/resistivity (mu or eta0 nonzero), bit 1 AV
+// (xf: the LDS copy of the flux-point coordinates Xf)
 template <int N, class R, int VM>
 __device__ __forceinline__ void visc_point(const ViscArgs &v, const double *mu4, int fam, int ifp, int jfp, R s1,
-                                           R s2, const R *qv, const R *g, R *fv) {
+                                           R s2, const R *qv, const R *g, R *fv, const double *xf) {
     const R gam = R(v.gam), lam = R(-2.0 / 3.0), prandt = R(0.72);
     R Fv[NV], Gv[NV];
     if constexpr ((VM & 1) != 0) {
@@ -249,7 +250,7 @@ __device__ __forceinline__ void visc_point(const ViscArgs &v, const double *mu4,
     }
     if constexpr ((VM & 2) != 0) {
         const int lo = fam == 0 ? 3 : 0, hi = fam == 0 ? 1 : 2;   // sides 1 and 2 of the family
-        const double x = fam == 0 ? c_Xf[N][ifp] : c_Xf[N][jfp];
+        const double x = fam == 0 ? xf[ifp] : xf[jfp];
         const R mu_art = R(mu4[lo] * (1.0 - x) + mu4[hi] * x);
         vflux<R>(gam, lam, prandt, mu_art, R(0), qv, g, Fv, Gv);
 #pragma unroll
@@ -437,7 +438,7 @@ __global__ void __launch_bounds__(V2Shape<N>::THREADS) __attribute__((amdgpu_wav
             trace9<N, R>(sQ + c * NQ, fam, ifp, jfp, Qown, sOps);
         }
         grad_trace<N, R>(sG + c * 16 * NP, fam, ifp, jfp, g, sOps);
-        visc_point<N, R, VM>(a, sMu + c * 4, fam, ifp, jfp, s1, s2, Qown, g, fv);
+        visc_point<N, R, VM>(a, sMu + c * 4, fam, ifp, jfp, s1, s2, Qown, g, fv, sOps + OXF);
         R *dst = fvi + (size_t)((c0 + c) * 2 + fam) * (NV * NIT) + k;
 #pragma unroll
         for (int w = 0; w < NV; w++) dst[w * NIT] = fv[w];
