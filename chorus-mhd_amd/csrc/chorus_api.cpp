@@ -50,7 +50,10 @@ struct chorus_ctx {
     double *partial = nullptr;     // per-block reduction partials
     int nface = 0;                 // faces carrying a flux (interior, CYCL pairs, FREE)
     int4 *faces = nullptr;         // {cellL, lfiL, cellR (-1: FREE), lfiR} ordered by left cell
-    double *TQ = nullptr;          // traces of the current state at the face points
+    double *TQ = nullptr;          // traces of the current state at the face points (mixed form:
+                                   // written only for the div B diagnostic)
+    MxTraces mt{};                 // mixed form: the traces the face passes read (compact form)
+    void *mt_buf = nullptr;        // mixed form: the allocation holding mt
     void *FF = nullptr;            // per cell face point: common inviscid flux + A_z
     double *FA = nullptr;          // mixed form: per cell face point: FP64 upwind A_z
     void *FFv = nullptr;           // viscous path: per cell face point: common viscous flux
@@ -136,6 +139,7 @@ FaceArgs face_args(chorus_ctx *c) {
     f.faces = c->faces;
     f.cellxy = c->cellxy;
     f.TQ = c->TQ;
+    f.mt = c->mt;
     f.FF = c->FF;
     f.FA = c->FA;
     f.HJ = half_jumps(c);
@@ -209,13 +213,19 @@ StageArgs base_args(chorus_ctx *c) {
     a.FA = c->FA;
     a.FFv = c->FFv;
     a.TQ = c->TQ;
+    a.mt = c->mt;
     a.sm = c->sm;
     a.ng = c->ng;
     return a;
 }
 
+bool mixed_traces(const chorus_ctx *c) { return c->mt.dev != nullptr; }
+
+// the traces of Q that the face passes read (and, with AV, its indicator)
 int run_trace(chorus_ctx *c, const double *Q) {
-    hipError_t e = launch_trace(c->N, c->ncell, Q, c->TQ, c->p.artiv ? c->sm : nullptr, c->stream);
+    double *sm = c->p.artiv ? c->sm : nullptr;
+    hipError_t e = mixed_traces(c) ? launch_trace_mixed(c->N, c->ncell, Q, c->mt, sm, c->stream)
+                                   : launch_trace(c->N, c->ncell, Q, c->TQ, sm, c->stream);
     if (e != hipSuccess) return fail(c, CHORUS_ERR_HIP, std::string("trace launch: ") + hipGetErrorString(e));
     return 0;
 }
@@ -231,7 +241,14 @@ int run_ct(chorus_ctx *c, double *Q, int which, double *partial = nullptr) {
     a.TQ = c->TQ;
     a.TQout = c->TQ;
     a.ng = c->ng;
-    hipError_t e = launch_ct(c->N, a, c->stream);
+    hipError_t e = hipSuccess;
+    if (mixed_traces(c)) {
+        // the correction reads the A_z traces and writes the Bx, By traces of
+        // the compact form; the diagnostic reads FP64 Bx, By traces, made here
+        if (which == 0) a.mt = c->mt;
+        else e = launch_trace(c->N, c->ncell, Q, c->TQ, nullptr, c->stream);
+    }
+    if (e == hipSuccess) e = launch_ct(c->N, a, c->stream);
     if (e != hipSuccess) return fail(c, CHORUS_ERR_HIP, std::string("ct launch: ") + hipGetErrorString(e));
     return 0;
 }
@@ -643,6 +660,14 @@ int init_ctx(chorus_ctx *c, const chorus_mesh *m, const chorus_params *p, const 
     const size_t nfp = (size_t)c->ncell * 4 * c->N;
     HIP_TRY(c, hipMalloc(&c->faces, faces.size() * sizeof(int4)));
     HIP_TRY(c, hipMalloc(&c->TQ, nfp * 9 * sizeof(double)));
+    if (c->p.precision == CHORUS_MIXED) {
+        // dev [nfp][8] float, az [nfp] double, ref [ncell][8] double
+        HIP_TRY(c, hipMalloc(&c->mt_buf, nfp * 8 * sizeof(float) + nfp * sizeof(double) +
+                                             (size_t)c->ncell * 8 * sizeof(double)));
+        c->mt.dev = (float *)c->mt_buf;
+        c->mt.az = (double *)(c->mt.dev + nfp * 8);
+        c->mt.ref = c->mt.az + nfp;
+    }
     HIP_TRY(c, hipMalloc(&c->FF, nfp * 9 * sizeof(double)));
     if (c->p.precision == CHORUS_MIXED) HIP_TRY(c, hipMalloc(&c->FA, nfp * sizeof(double)));
     // (the mixed form folds the viscous face flux into its FF records)
@@ -674,6 +699,7 @@ void chorus_free(chorus_ctx *c) {
     (void)hipFree(c->eig);
     (void)hipFree(c->faces);
     (void)hipFree(c->TQ);
+    (void)hipFree(c->mt_buf);
     (void)hipFree(c->FF);
     (void)hipFree(c->FA);
     (void)hipFree(c->FFv);
@@ -1450,6 +1476,9 @@ int chorus_batch_divb(chorus_batch *b, double *divbmax) {
     int rc = refresh(b);
     if (rc) return rc;
     HIP_TRYB(b, hipStreamSynchronize(b->stream));
+    // the diagnostic reads FP64 Bx, By traces: the mixed form's groups make them
+    for (chorus_ctx *c : b->groups)
+        if (mixed_traces(c)) HIP_TRYB(b, launch_trace(c->N, c->ncell, c->X[c->cur], c->TQ, nullptr, c->stream));
     for (int i = 0; i < b->ninst; i++) {
         const chorus_batch::Member &m = b->mem[i];
         chorus_ctx *c = b->groups[m.group];

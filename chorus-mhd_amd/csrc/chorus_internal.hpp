@@ -49,6 +49,21 @@ enum StageKind : int {
     KIND_S54_5 = 5,   // ((((c1*Qa + c2*Qb) + c3*Qc) + c4*Q) + c5*Rprev) + c0*R  (FiveStageRk s5)
 };
 
+// Mixed form: the traces of the state at the face points in the compact form
+// the face passes read (the FP64 TQ is then written only for the div B
+// diagnostic).  Per cell face point (TQ order): dev, the FP32 difference of
+// variables 1-8 from the cell's FP64 reference state ref (any state of the
+// cell's own scale: the jump of two traces is formed as the rounded
+// difference of the two references plus the difference of the deviations,
+// so it keeps the precision of a difference of FP64 traces to FP32 rounding
+// of terms the size of a cell's variation), and az, the FP64 trace of A_z.
+//   dev[cell][lfi][nfp][8] (float), az[cell][lfi][nfp] (double), ref[cell][8] (double)
+struct MxTraces {
+    float *dev;
+    double *az;
+    double *ref;
+};
+
 struct StageArgs {
     int ncell;
     const double *cellxy;
@@ -64,6 +79,7 @@ struct StageArgs {
     const double *FA;   // mixed form: the FP64 upwind A_z of each cell's face points
     const void *FFv;    // viscous face fluxes (k_face_visc)
     double *TQ;         // traces of Qout at the face points, written by the stage kernel
+    MxTraces mt;        // mixed form: the same traces in the compact form
     double *sm;         // AV: indicator of Qout per cell, written by the viscous stage kernel
     int ng;             // geometry nodes per cell in cellxy: 4 (0 taken as 4) or 8
 };
@@ -84,6 +100,7 @@ struct FaceArgs {
     const int4 *faces;
     const double *cellxy;
     const double *TQ;
+    MxTraces mt;             // mixed form: the traces in the compact form (TQ unused)
     void *FF;
     double *FA;              // mixed form: FP64 upwind A_z per cell face point
     float *HJ;               // mixed viscous path: half the jump of the FP64 traces,
@@ -144,6 +161,8 @@ struct CTArgs {
     double *partial;    // which=1: per block {sum divB^2, max |divB|}
     const double *TQ;   // traces of A_z (which=0) / Bx, By (which=1) at the face points
     double *TQout;      // which=0: the traces of the corrected Bx, By are written here
+    MxTraces mt;        // which=0, mixed form: A_z traces read from mt.az and the corrected
+                        // Bx, By traces written to mt.dev (TQ, TQout unused)
     int ng;             // geometry nodes per cell in cellxy: 4 (0 taken as 4) or 8
 };
 
@@ -214,6 +233,7 @@ __device__ __forceinline__ void prefetch_qa(const StageArgs &a, size_t base, int
 // launchers (defined in the kernel translation units); return hipError_t
 hipError_t launch_stage_ideal_fp64(int N, const StageArgs &a, const FaceArgs &f, hipStream_t s);
 hipError_t launch_trace(int N, int ncell, const double *Q, double *TQ, double *sm, hipStream_t s);
+hipError_t launch_trace_mixed(int N, int ncell, const double *Q, const MxTraces &mt, double *sm, hipStream_t s);
 hipError_t launch_stage_ideal_mixed(int N, const StageArgs &a, const FaceArgs &f, hipStream_t s, int tile);
 int tile_count();
 hipError_t launch_ct(int N, const CTArgs &a, hipStream_t s);

@@ -12,6 +12,25 @@ hipError_t upload_operators_mixed() {
 }
 
 template <class S> static int blocks_of(int ncell) { return (ncell + S::CPB - 1) / S::CPB; }
+
+template <int n> static hipError_t trace_mx(int ncell, const double *Q, const MxTraces &mt, double *sm, hipStream_t s) {
+    hipLaunchKernelGGL((k_trace_mx<n>), dim3(blocks_of<Shape<n>>(ncell)), dim3(Shape<n>::THREADS), 0, s, ncell, Q, mt,
+                       sm);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_mixed(int N, int ncell, const double *Q, const MxTraces &mt, double *sm, hipStream_t s) {
+    switch (N) {
+    case 2: return trace_mx<2>(ncell, Q, mt, sm, s);
+    case 3: return trace_mx<3>(ncell, Q, mt, sm, s);
+    case 4: return trace_mx<4>(ncell, Q, mt, sm, s);
+    case 5: return trace_mx<5>(ncell, Q, mt, sm, s);
+    case 6: return trace_mx<6>(ncell, Q, mt, sm, s);
+    case 7: return trace_mx<7>(ncell, Q, mt, sm, s);
+    case 8: return trace_mx<8>(ncell, Q, mt, sm, s);
+    default: return hipErrorInvalidValue;
+    }
+}
 static int fblocks(int N, int nface) { return (nface * N + 255) / 256; }
 
 // tile shapes of the mixed cell kernels (threads-per-block targets of the

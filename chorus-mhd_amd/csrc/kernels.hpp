@@ -314,6 +314,49 @@ __device__ __forceinline__ void store_traces(const double *q, int ncl, int c0, d
     }
 }
 
+// The same traces in the mixed form's compact form (MxTraces): the FP64 sums
+// of store_traces, less the cell's reference state ref(c, v) = ref[c*rs + v*vs]
+// for variables 1-8 and rounded to FP32, A_z in FP64; the reference states of
+// the block's cells are stored beside them.  One thread per record: its 32
+// deviation bytes leave as two 16-byte stores, consecutive threads on
+// consecutive records.  Lt: the FP64 extrapolation table (c_L layout).
+template <int N>
+__device__ __forceinline__ void store_traces_mx(const double *q, int ncl, int c0, const MxTraces &mt,
+                                                const double *ref, int rs, int vs, const double *Lt) {
+    constexpr int NP = N * N, NQ = NV * NP, FP = 4 * N;
+    const int nrec = ncl * FP;
+    for (int it = (int)threadIdx.x; it < nrec; it += blockDim.x) {
+        const int c = it / FP, fpi = it % FP, lfi = fpi / N, nfp = fpi % N;
+        int ifp, jfp;
+        face2fp<N>(lfi + 1, nfp, ifp, jfp);
+        const bool fam0 = lf_family(lfi + 1) == 0;
+        const double *qp = q + c * NQ + (fam0 ? jfp * N : ifp);
+        const int stride = fam0 ? 1 : N;
+        const double *Lrow = Lt + (fam0 ? ifp : jfp) * N;
+        double Lr[N], acc[NV];
+#pragma unroll
+        for (int sp = 0; sp < N; sp++) Lr[sp] = Lrow[sp];
+#pragma unroll
+        for (int v = 0; v < NV; v++) acc[v] = 0.0;
+#pragma unroll
+        for (int sp = 0; sp < N; sp++) {
+            const double *qs = qp + sp * stride;
+#pragma unroll
+            for (int v = 0; v < NV; v++) acc[v] = acc[v] + qs[v * NP] * Lr[sp];
+        }
+        float dv[8];
+#pragma unroll
+        for (int v = 0; v < 8; v++) dv[v] = float(acc[v] - ref[c * rs + v * vs]);
+        const size_t rec = (size_t)c0 * FP + it;
+        float4 *d = reinterpret_cast<float4 *>(mt.dev + rec * 8);
+        d[0] = make_float4(dv[0], dv[1], dv[2], dv[3]);
+        d[1] = make_float4(dv[4], dv[5], dv[6], dv[7]);
+        mt.az[rec] = acc[8];
+    }
+    for (int i = (int)threadIdx.x; i < ncl * 8; i += blockDim.x)
+        mt.ref[(size_t)c0 * 8 + i] = ref[(i >> 3) * rs + (i & 7) * vs];
+}
+
 template <class R>
 __device__ __forceinline__ void block_max_atomic(double v, double *red, double *dst) {
     const int tid = threadIdx.x, n = blockDim.x;
@@ -683,9 +726,14 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
             if (i < ncl * 4 * N) {
                 const int2 info = a.nbr[(size_t)(c0 + c) * 4 + lfi];
                 if (!((info.y >> 4) & 1)) {
-                    const double *t = a.TQ + (size_t)((info.x * 4 + (info.y & 7) - 1) * N + (N - 1 - nfp)) * NV + VAR0;
+                    const size_t prec = (size_t)((info.x * 4 + (info.y & 7) - 1) * N + (N - 1 - nfp));
+                    if (WHICH == 0 && a.mt.az) {
+                        hp[0][k] = a.mt.az[prec];
+                    } else {
+                        const double *t = a.TQ + prec * NV + VAR0;
 #pragma unroll
-                    for (int w = 0; w < NVAR; w++) hp[w][k] = t[w];
+                        for (int w = 0; w < NVAR; w++) hp[w][k] = t[w];
+                    }
                 }
             }
         }
@@ -766,7 +814,9 @@ __global__ void __launch_bounds__(256) k_ct(CTArgs a) {
             int ifp, jfp;
             face2fp<N>(lfi + 1, nfp, ifp, jfp);
             const double v = trace1<N>(sV[w] + c * NP, lf_family(lfi + 1), ifp, jfp, sOps);
-            a.TQout[(size_t)(((c0 + c) * 4 + lfi) * N + nfp) * NV + 5 + w] = v;
+            const size_t rec = (size_t)(((c0 + c) * 4 + lfi) * N + nfp);
+            if (a.mt.dev) a.mt.dev[rec * 8 + 5 + w] = float(v - a.mt.ref[(size_t)(c0 + c) * 8 + 5 + w]);
+            else a.TQout[rec * NV + 5 + w] = v;
         }
     } else {
         double sumsq = 0.0, dmax = 0.0;

@@ -458,6 +458,22 @@ __device__ __forceinline__ double az_at(const double *FA, const double *sAz, int
     return trace1<N>(sAz + c * N * N, fam, fam == 0 ? r : line, fam == 0 ? line : r, ops);
 }
 
+// a cell face point's trace in the compact form: the FP32 deviations of
+// variables 1-8 and the cell's FP64 reference state
+__device__ __forceinline__ void mx_trace(const MxTraces &mt, size_t rec, int cell, float *dv, double *ref) {
+    const float4 *d = reinterpret_cast<const float4 *>(mt.dev + rec * 8);
+    const float4 d0 = d[0], d1 = d[1];
+    dv[0] = d0.x; dv[1] = d0.y; dv[2] = d0.z; dv[3] = d0.w;
+    dv[4] = d1.x; dv[5] = d1.y; dv[6] = d1.z; dv[7] = d1.w;
+    const double2 *r = reinterpret_cast<const double2 *>(mt.ref + (size_t)cell * 8);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const double2 x = r[j];
+        ref[2 * j] = x.x;
+        ref[2 * j + 1] = x.y;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Face pass, mixed form: the jump terms of the Rusanov flux for both cells of
 // each face point (see the header), the FP64 upwind A_z, and on the viscous
@@ -471,25 +487,16 @@ __device__ __forceinline__ double az_at(const double *FA, const double *sAz, int
 // half trace jumps and the upwind A_z.
 template <int N, int NG = 4, bool JUMPS = true>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_face_inv_mx(FaceArgs a) {
-    // the block's 256 face points' traces move through LDS as in k_face_inv
-    // (every load of a thread issued first, consecutive threads on consecutive
-    // values of a record); the 32-byte output records leave directly
-    constexpr int FPB = 256, RW = 2 * NV, NFB = FPB / N + 2;
-    __shared__ double sT[FPB * RW];               // [p][left 9 | right 9] traces
-    __shared__ int4 sF[NFB];
+    // one thread per face point: both sides' compact traces are loaded
+    // directly (the face's points share the cells' reference states through
+    // the cache), and the 32-byte output records leave directly
     __shared__ double red[256];
     const int tid = threadIdx.x;
-    const int p0 = xcd_block(blockIdx.x, gridDim.x) * FPB;
-    const int npt = min(FPB, a.nface * N - p0);
-    const int f0 = p0 / N, nf = (p0 + npt - 1) / N - f0 + 1;
-    if (tid < nf) sF[tid] = a.faces[f0 + tid];
-    __syncthreads();
-    gather_face_traces<N, FPB>(a.TQ, sF, f0, p0, npt, sT);
-    __syncthreads();
+    const int gp = xcd_block(blockIdx.x, gridDim.x) * 256 + tid;
     double emax = 0.0;
-    if (tid < npt) {
-        const int gp = p0 + tid, k = gp % N;
-        const int4 fc = sF[gp / N - f0];
+    if (gp < a.nface * N) {
+        const int k = gp % N;
+        const int4 fc = a.faces[gp / N];
         const bool free_face = fc.z < 0;
         int ifp, jfp;
         face2fp<N>(fc.y + 1, k, ifp, jfp);
@@ -499,18 +506,35 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         double m1, m2;
         metric_row_g<N, NG>(xy, lf_family(fc.y + 1), ifp, jfp, m1, m2);
         const float s1 = float(m1), s2 = float(m2);
-        // the right state: the other cell's trace; at an INLE / SYMP / WALL face
-        // that of the boundary kind; at a FREE / OUTL face the own trace
-        double ql[NV], qr[NV];
+        const unsigned rl = face_rec<N>(fc, k, false);
+        const unsigned rr = free_face ? rl : face_rec<N>(fc, k, true);
+        // the own trace in FP64 (reference plus deviation) and the jump to the
+        // right state: the other cell's trace (the rounded difference of the
+        // two references plus that of the deviations); at an INLE / SYMP /
+        // WALL face the boundary kind's state; none at a FREE / OUTL face
+        float dl[8];
+        double bl[8], ql[NV];
+        mx_trace(a.mt, rl, fc.x, dl, bl);
+#pragma unroll
+        for (int v = 0; v < 8; v++) ql[v] = bl[v] + double(dl[v]);
+        ql[8] = a.mt.az[rl];
         float d[NV];
+        d[8] = 0.f;
+        if (!free_face) {
+            float dr[8];
+            double br[8];
+            mx_trace(a.mt, rr, fc.z, dr, br);
 #pragma unroll
-        for (int v = 0; v < NV; v++) {
-            ql[v] = sT[tid * RW + v];
-            qr[v] = sT[tid * RW + NV + v];
+            for (int v = 0; v < 8; v++) d[v] = float(br[v] - bl[v]) + (dr[v] - dl[v]);
+        } else if (fc.w != BK_ZEROGRAD) {
+            double qr[NV];
+            boundary_state(fc.w, ql, m1, m2, a.qin, qr);
+#pragma unroll
+            for (int v = 0; v < 8; v++) d[v] = float(qr[v] - ql[v]);
+        } else {
+#pragma unroll
+            for (int v = 0; v < 8; v++) d[v] = 0.f;
         }
-        if (free_face && fc.w != BK_ZEROGRAD) boundary_state(fc.w, ql, m1, m2, a.qin, qr);
-#pragma unroll
-        for (int v = 0; v < NV; v++) d[v] = float(qr[v] - ql[v]);
         const float sl = float(side_sign(fc.y));
         const float slr = free_face ? 0.f : sl * float(-side_sign(fc.w));
         float ol[8], orr[8], eig;
@@ -526,7 +550,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             const float im = frcp(fsqrt(s1 * s1 + s2 * s2));
             eig = wave_speed(Ql, Qr, s1 * im, s2 * im, float(a.gam));
         }
-        const size_t rl = face_rec<N>(fc, k, false);
         if (a.HJ) {
             // the half trace jumps the gradient kernel's face points use (the
             // central flux-point value is own + (other - own)/2); a boundary
@@ -536,7 +559,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                 hl[0] = make_float4(0.f, 0.f, 0.f, 0.f);
                 hl[1] = hl[0];
             } else {
-                float4 *hr = reinterpret_cast<float4 *>(a.HJ + (size_t)face_rec<N>(fc, k, true) * 8);
+                float4 *hr = reinterpret_cast<float4 *>(a.HJ + (size_t)rr * 8);
                 hl[0] = make_float4(0.5f * d[0], 0.5f * d[1], 0.5f * d[2], 0.5f * d[3]);
                 hl[1] = make_float4(0.5f * d[4], 0.5f * d[5], 0.5f * d[6], 0.5f * d[7]);
                 hr[0] = make_float4(-0.5f * d[0], -0.5f * d[1], -0.5f * d[2], -0.5f * d[3]);
@@ -558,7 +581,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             // A_z of the boundary state is the cell's own: the upwind value is too
             a.FA[rl] = ql[8];
         } else {
-            const unsigned rr = face_rec<N>(fc, k, true);
             if constexpr (JUMPS) {
                 float4 *fr = reinterpret_cast<float4 *>((float *)a.FF + (size_t)rr * 8);
                 fl[0] = make_float4(ol[0], ol[1], ol[2], ol[3]);
@@ -573,7 +595,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             const double mn = sqrt(m1 * m1 + m2 * m2);
             const double ux = m1 / mn * double(side_sign(fc.y)), uy = m2 / mn * double(side_sign(fc.y));
             const bool take_left = rhoi * ql[1] * ux + rhoi * ql[2] * uy > 0.0;
-            const double az = take_left ? ql[8] : sT[tid * RW + NV + 8];
+            const double az = take_left ? ql[8] : a.mt.az[rr];
             a.FA[rl] = az;
             a.FA[rr] = az;
         }
@@ -581,7 +603,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     }
     if (a.artiv) {
         if (a.inst) {
-            const int tt = min(p0 + tid, a.nface * N - 1);
+            const int tt = min(gp, a.nface * N - 1);
             instance_max_atomic(emax, a.inst[a.faces[tt / N].x], a.eig);
         } else {
             block_max_atomic<float>(emax, red, a.eig);
@@ -709,9 +731,7 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
     if (pact) combine_point(a, base, NP, sQ64 + pc * NQ + pk, qa, res, res8);
     if (a.kind == KIND_RESID) return;
     __syncthreads();
-    // (the flux area is free now: it stages the trace stores)
-    constexpr bool STG = trace_stage_doubles<N>(CPB) <= UD;
-    store_traces<N>(sQ64, ncl, c0, a.TQ, sOps, STG ? U : nullptr);
+    store_traces_mx<N>(sQ64, ncl, c0, a.mt, sBar, NV, 1, sOps);
 }
 
 // ---------------------------------------------------------------------------
@@ -1016,12 +1036,17 @@ __global__ void __launch_bounds__(256) k_face_visc_mx(FaceArgs a) {
     }
     const size_t pl = (size_t)((fc.x * 4 + fc.y) * N + k);
     const size_t pr = free_face ? pl : (size_t)((fc.z * 4 + fc.w) * N + kr);
-    // the own FP64 trace and the half trace jump (other - own)/2 of the
+    // the own trace in FP64 and the half trace jump (other - own)/2 of the
     // inviscid face pass (zero at a boundary face): the central value of the
     // viscous flux, and the Rusanov jump terms of the inviscid flux
     double ql[NV];
+    {
+        float dl[8];
+        double bl[8];
+        mx_trace(a.mt, pl, fc.x, dl, bl);
 #pragma unroll
-    for (int v = 0; v < 8; v++) ql[v] = a.TQ[pl * NV + v];
+        for (int v = 0; v < 8; v++) ql[v] = bl[v] + double(dl[v]);
+    }
     ql[8] = 0.0;
     float hj[8];
     {
@@ -1130,7 +1155,7 @@ template <int N, int TB, int VM, int NG = 4>
 __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amdgpu_waves_per_eu(CHORUS_MX_VISC_MINW(N)))) k_stage_visc_mx(StageArgs a, ViscArgs v) {
     using S = CellShape<N, TB>;
     using TS = MxVShape<N, TB>;
-    constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ, T = TS::THREADS, NIT = TS::NIT;
+    constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ, T = TS::THREADS;
     constexpr int UST = 8 * NF + 1;      // per-cell stride of the combined flux planes
     constexpr int AST = NF + 1;          // per-cell stride of the FP64 A_z flux-point values
     constexpr int UF = CPB * NQ + CPB * UST;
@@ -1272,9 +1297,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     }
     if (a.kind == KIND_RESID) return;
     __syncthreads();
-    // (the flux area is free now: it stages the trace stores)
-    constexpr bool STG = trace_stage_doubles<N>(CPB) <= UD;
-    store_traces<N>(sQ64, ncl, c0, a.TQ, sOps, STG ? U : nullptr);
+    store_traces_mx<N>(sQ64, ncl, c0, a.mt, sBar, NV, 1, sOps);
     if constexpr ((VM & 2) != 0) av_sensor_mx<N, CPB>(a.sm, sQ64, sBar, reinterpret_cast<float *>(sAV), ncl, c0, sRav);
 }
 

@@ -149,6 +149,32 @@ __global__ void __launch_bounds__(256) k_trace(int ncell, const double *Q, doubl
     if (sm) av_sensor<N, CPB>(sm, sQ, sAV, ncl, c0, sOps);
 }
 
+// the same in the mixed form's compact form, each cell's reference state its
+// central solution point (the stage kernels' choice)
+template <int N>
+__global__ void __launch_bounds__(256) k_trace_mx(int ncell, const double *Q, MxTraces mt, double *sm) {
+    using S = Shape<N>;
+    constexpr int CPB = S::CPB, NP = S::NP, NQ = S::NQ, PC = (N / 2) * N + N / 2;
+    __shared__ double sQ[CPB * NQ];
+    __shared__ double sAV[CPB * NP];
+    __shared__ double sOps[Ops64Layout<N>::SIZE];   // FP64 operator tables (OpTables64 layout)
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int c0 = blk * CPB;
+    const int ncl = min(CPB, ncell - c0);
+    {
+        constexpr int PQ = per_thread<CPB * NQ, S::THREADS>();
+        double hq[PQ];
+        gather_regs<PQ, S::THREADS>(hq, Q + (size_t)c0 * NQ, ncl * NQ);
+        OpTables64<N, S::THREADS> ops;
+        ops.gather();
+        scatter_lds<PQ, S::THREADS>(sQ, hq, ncl * NQ);
+        ops.scatter(sOps);
+    }
+    __syncthreads();
+    store_traces_mx<N>(sQ, ncl, c0, mt, sQ + PC, NQ, NP, sOps);
+    if (sm) av_sensor<N, CPB>(sm, sQ, sAV, ncl, c0, sOps);
+}
+
 // muAV(1,:) of one cell from its indicator and epsilon0 (AV.f90:120-172)
 __device__ __forceinline__ double mu_av(double sm, double eps0, double pi) {
     const double se_low = -6.0, se_high = -1.0;

@@ -16,6 +16,9 @@ namespace chorus {
 
 hipError_t launch_stage_ideal_fp64(int, const StageArgs &, const FaceArgs &, hipStream_t) { unreachable(__func__); }
 hipError_t launch_trace(int, int, const double *, double *, double *, hipStream_t) { unreachable(__func__); }
+hipError_t launch_trace_mixed(int, int, const double *, const MxTraces &, double *, hipStream_t) {
+    unreachable(__func__);
+}
 hipError_t launch_stage_ideal_mixed(int, const StageArgs &, const FaceArgs &, hipStream_t, int) { unreachable(__func__); }
 int tile_count() { unreachable(__func__); }
 hipError_t launch_ct(int, const CTArgs &, hipStream_t) { unreachable(__func__); }
