@@ -484,7 +484,7 @@ __device__ __forceinline__ void mx_trace(const MxTraces &mt, size_t rec, int cel
 // ---------------------------------------------------------------------------
 // JUMPS = false (viscous path): the jump terms are formed by k_face_visc_mx
 // beside the viscous flux, and this pass keeps the wave-speed maximum, the
-// half trace jumps and the upwind A_z.
+// half trace jumps (for the gradient kernel) and the upwind A_z.
 template <int N, int NG = 4, bool JUMPS = true>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_face_inv_mx(FaceArgs a) {
     // one thread per face point: both sides' compact traces are loaded
@@ -1036,37 +1036,42 @@ __global__ void __launch_bounds__(256) k_face_visc_mx(FaceArgs a) {
     }
     const size_t pl = (size_t)((fc.x * 4 + fc.y) * N + k);
     const size_t pr = free_face ? pl : (size_t)((fc.z * 4 + fc.w) * N + kr);
-    // the own trace in FP64 and the half trace jump (other - own)/2 of the
-    // inviscid face pass (zero at a boundary face): the central value of the
-    // viscous flux, and the Rusanov jump terms of the inviscid flux
+    // the own trace in FP64 and the jump to the other cell's trace (the
+    // rounded difference of the two references plus that of the deviations;
+    // none at a boundary face): the central value own + jump/2 of the viscous
+    // flux, and the Rusanov jump terms of the inviscid flux
     double ql[NV];
+    float dj[8];
     {
         float dl[8];
         double bl[8];
         mx_trace(a.mt, pl, fc.x, dl, bl);
 #pragma unroll
         for (int v = 0; v < 8; v++) ql[v] = bl[v] + double(dl[v]);
+        if (!free_face) {
+            float dr[8];
+            double br[8];
+            mx_trace(a.mt, pr, fc.z, dr, br);
+#pragma unroll
+            for (int v = 0; v < 8; v++) dj[v] = float(br[v] - bl[v]) + (dr[v] - dl[v]);
+        } else {
+#pragma unroll
+            for (int v = 0; v < 8; v++) dj[v] = 0.f;
+        }
     }
     ql[8] = 0.0;
-    float hj[8];
-    {
-        const float4 *h = reinterpret_cast<const float4 *>(a.HJ + pl * 8);
-        const float4 h0 = h[0], h1 = h[1];
-        hj[0] = h0.x; hj[1] = h0.y; hj[2] = h0.z; hj[3] = h0.w;
-        hj[4] = h1.x; hj[5] = h1.y; hj[6] = h1.z; hj[7] = h1.w;
-    }
     float qv[NV], g[2 * NV];
 #pragma unroll
-    for (int v = 0; v < 8; v++) qv[v] = float(ql[v]) + hj[v];
+    for (int v = 0; v < 8; v++) qv[v] = float(ql[v]) + 0.5f * dj[v];
     qv[8] = 0.f;
     float jl[8], jr[8];
     {
-        // the jump to the right state of the inviscid flux: twice the half
-        // jump (exact); at an INLE / SYMP / WALL face that of the boundary
-        // kind's state; none at a FREE / OUTL face
+        // the jump to the right state of the inviscid flux: to the other
+        // cell's trace; at an INLE / SYMP / WALL face to the boundary kind's
+        // state; none at a FREE / OUTL face
         float d[NV];
 #pragma unroll
-        for (int v = 0; v < 8; v++) d[v] = 2.f * hj[v];
+        for (int v = 0; v < 8; v++) d[v] = dj[v];
         d[8] = 0.f;
         if (free_face && fc.w != BK_ZEROGRAD) {
             double qr[NV];
