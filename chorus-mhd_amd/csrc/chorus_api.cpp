@@ -126,13 +126,6 @@ void from_device_layout(const chorus_ctx *c, const std::vector<double> &in, doub
 
 bool g_ops_uploaded = false;
 
-// mixed viscous path: the half trace jumps (8 floats per cell face point) live
-// in the upper half of FF, whose FP32 flux records use 9 of its 18 words
-float *half_jumps(chorus_ctx *c) {
-    if (c->ideal || c->p.precision != CHORUS_MIXED) return nullptr;
-    return (float *)c->FF + (size_t)c->ncell * 4 * c->N * 9;
-}
-
 FaceArgs face_args(chorus_ctx *c) {
     FaceArgs f{};
     f.nface = c->nface;
@@ -142,7 +135,6 @@ FaceArgs face_args(chorus_ctx *c) {
     f.mt = c->mt;
     f.FF = c->FF;
     f.FA = c->FA;
-    f.HJ = half_jumps(c);
     f.FFv = c->FFv;
     f.gtr = c->gtr;
     f.sm = c->sm;
@@ -183,7 +175,7 @@ int launch_stage(chorus_ctx *c, const StageArgs &a) {
         v.eig_next = c->eig + (size_t)(1 - c->stage_parity) * c->ninst;
         v.ninst = c->ninst;
         v.TQ = c->TQ;
-        v.HJ = half_jumps(c);
+        v.mt = c->mt;
         v.mucell = c->mucell;
         v.sm6 = c->sm6;
         v.artiv = c->p.artiv;
@@ -192,6 +184,7 @@ int launch_stage(chorus_ctx *c, const StageArgs &a) {
         v.gam = c->p.gam;
         v.h_msh = c->h_msh;
         v.pi = std::atan(1.0) * 4.0;
+        for (int q = 0; q < 8; q++) v.qin[q] = c->qin[q];
         f.eig = v.eig;
         c->last_parity = c->stage_parity;
         c->stage_parity = 1 - c->stage_parity;
@@ -565,6 +558,8 @@ int init_ctx(chorus_ctx *c, const chorus_mesh *m, const chorus_params *p, const 
 
     // on-device face connectivity: per cell and local face, the partner cell,
     // its local face and whether this cell is the reference's left or right cell
+    // ({partner, lf | right << 3}); a face without a right cell holds
+    // {-1, 1 << 4 | kind << 5} with its BoundaryKind
     std::vector<int2> nbr((size_t)m->ncell * 4, make_int2(-2, 0));
     auto IF = [&](int f, int k) { return m->if2c[(size_t)k * m->nface + f]; };
     auto link = [&](int l, int lfl, int r, int lfr) -> bool {
@@ -594,7 +589,7 @@ int init_ctx(chorus_ctx *c, const chorus_mesh *m, const chorus_params *p, const 
             int l = IF(f, 0) - 1, lf = IF(f, 2);
             if (l < 0 || l >= m->ncell || lf < 1 || lf > 4)
                 return fail(c, CHORUS_ERR_TOPOLOGY, std::string("bad ") + what + " face");
-            nbr[(size_t)l * 4 + lf - 1] = make_int2(-1, 1 << 4);
+            nbr[(size_t)l * 4 + lf - 1] = make_int2(-1, (1 << 4) | (kind << 5));
             bkind[(size_t)l * 4 + lf - 1] = kind;
         }
         return 0;

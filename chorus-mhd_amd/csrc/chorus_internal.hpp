@@ -103,8 +103,6 @@ struct FaceArgs {
     MxTraces mt;             // mixed form: the traces in the compact form (TQ unused)
     void *FF;
     double *FA;              // mixed form: FP64 upwind A_z per cell face point
-    float *HJ;               // mixed viscous path: half the jump of the FP64 traces,
-                             // (other - own)/2 rounded to FP32, per cell face point [8]
     void *FFv;
     const void *gtr;
     const double *sm;        // AV indicator per cell
@@ -142,11 +140,14 @@ struct ViscArgs {
     double *eig_next;   // the next stage's slots (one per instance), zeroed by k_stage_visc
     int ninst;          // instances sharing the launch
     const double *TQ;   // traces of Qin at the face points
-    const float *HJ;    // mixed form: half trace jumps per cell face point (k_face_inv_mx)
+    MxTraces mt;        // mixed form: the same traces in the compact form
     double *mucell;     // AV: muAV(1) per cell of this stage's input, written by k_grad
+                        // (mixed form with AV alone: for a unit wave speed until the stage kernel
+                        // scales it by the stage maximum)
     double *sm6;        // when set (Tecplot output): log10 indicator of By of the stage's input
     int artiv;
     double mu, eta0, gam, h_msh, pi;
+    double qin[8];      // INLE faces: the prescribed state (chorus_bc)
     template <class R> __host__ __device__ R *gtr_r() const { return (R *)gtr; }
     template <class R> __host__ __device__ R *fvi_r() const { return (R *)fvi; }
 };

@@ -78,7 +78,11 @@ hipError_t launch_stage_ideal_mixed(int N, const StageArgs &a, const FaceArgs &f
 template <int n, int TBV, int VM, int NG>
 static hipError_t visc_stage(const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s) {
     using S = CellShape<n, TBV>;
-    hipLaunchKernelGGL((k_face_inv_mx<n, NG, false>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);
+    // the gradient kernel reads the other cells' traces itself; only physical
+    // viscosity with AV needs the wave-speed maximum before it (with AV alone
+    // the gradient kernel forms it)
+    if constexpr (VM == 3)
+        hipLaunchKernelGGL((k_face_inv_mx<n, NG, false>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);
     hipLaunchKernelGGL((k_grad_mx<n, TBV, VM, NG>), dim3(blocks_of<S>(a.ncell)), dim3(V2ShapeT<n, TBV>::THREADS), 0, s,
                        v);
     hipLaunchKernelGGL((k_face_visc_mx<n, VM, NG>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);

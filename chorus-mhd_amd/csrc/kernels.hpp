@@ -388,6 +388,21 @@ __device__ __forceinline__ void instance_max_atomic(double v, int inst, double *
     }
 }
 
+// the same for a kernel with many waves per instance: a wave whose maximum does
+// not exceed the value already stored issues no atomic (the stored value only
+// grows during the kernel, so a stale read costs an atomic, never a result)
+__device__ __forceinline__ void instance_max_atomic_filtered(double v, int inst, double *dst) {
+    const int first = __builtin_amdgcn_readfirstlane(inst);
+    const bool uniform = __all(inst == first);
+    if (uniform)
+        for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    if (uniform && (threadIdx.x & 63) != 0) return;
+    // (non-negative doubles order like their bit patterns)
+    unsigned long long *p = reinterpret_cast<unsigned long long *>(dst + (uniform ? first : inst));
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    if (b > __atomic_load_n(p, __ATOMIC_RELAXED)) atomicMax(p, b);
+}
+
 // ---------------------------------------------------------------------------
 // Rusanov flux of every face point, once per face (getrusanov.f90:1-109 called
 // from interfaceflux_all.f90:16-114, cyclocflux_all.f90:15-114, bcflux.f90:19-73):

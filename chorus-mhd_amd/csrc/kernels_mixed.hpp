@@ -202,6 +202,65 @@ template <int CPB, int T> struct EdgeStage {
         }
     }
 };
+// The other cells' compact traces at the face points of the block's cells
+// (cells c0.., ncl of them), staged with the rest of the block: per own face
+// point the other cell's 8 FP32 deviations (in the own point order), per own
+// local face the difference of the other cell's FP64 reference state and the
+// own one (sBar), rounded to FP32; zeros at boundary faces.  gather_nbr()
+// loads the neighbour entries first, gather() the traces (after the other
+// staging loads are issued), scatter() writes them to LDS once sBar is there.
+template <int N, int CPB, int T> struct PartnerStage {
+    static constexpr int ND = CPB * 8 * N, NR = CPB * 16;   // float4 and double2 items
+    static constexpr int PD = (ND + T - 1) / T, PR = (NR + T - 1) / T;
+    int2 nd[PD], nr[PR];
+    float4 d[PD];
+    double2 r[PR];
+    __device__ __forceinline__ void gather_nbr(const int2 *nbr, int c0, int ncl) {
+#pragma unroll
+        for (int k = 0; k < PD; k++) {
+            const int i = (int)threadIdx.x + k * T;
+            nd[k] = i < ncl * 8 * N ? nbr[(c0 + i / (8 * N)) * 4 + (i % (8 * N)) / (2 * N)] : make_int2(-1, 1 << 4);
+        }
+#pragma unroll
+        for (int k = 0; k < PR; k++) {
+            const int i = (int)threadIdx.x + k * T;
+            nr[k] = i < ncl * 16 ? nbr[(c0 + i / 16) * 4 + (i / 4) % 4] : make_int2(-1, 1 << 4);
+        }
+    }
+    __device__ __forceinline__ void gather(const MxTraces &mt) {
+#pragma unroll
+        for (int k = 0; k < PD; k++) {
+            const int i = (int)threadIdx.x + k * T, rem = i % (8 * N), nfp = (rem >> 1) % N;
+            const bool other = !((nd[k].y >> 4) & 1);
+            const size_t rec = (size_t)((nd[k].x * 4 + (nd[k].y & 7) - 1) * N + (N - 1 - nfp));
+            d[k] = other ? reinterpret_cast<const float4 *>(mt.dev)[rec * 2 + (rem & 1)] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int k = 0; k < PR; k++) {
+            const int i = (int)threadIdx.x + k * T;
+            const bool other = !((nr[k].y >> 4) & 1);
+            r[k] = other ? reinterpret_cast<const double2 *>(mt.ref)[(size_t)nr[k].x * 4 + i % 4] : make_double2(0.0, 0.0);
+        }
+    }
+    __device__ __forceinline__ void scatter(float *sPD, float *sPR, const double *sBar, int ncl) const {
+#pragma unroll
+        for (int k = 0; k < PD; k++) {
+            const int i = (int)threadIdx.x + k * T;
+            if (i < ncl * 8 * N) reinterpret_cast<float4 *>(sPD)[i] = d[k];
+        }
+#pragma unroll
+        for (int k = 0; k < PR; k++) {
+            const int i = (int)threadIdx.x + k * T;
+            if (i < ncl * 16) {
+                const bool other = !((nr[k].y >> 4) & 1);
+                const double *b = sBar + (i / 16) * NV + 2 * (i % 4);
+                reinterpret_cast<float2 *>(sPR)[i] =
+                    other ? make_float2(float(r[k].x - b[0]), float(r[k].y - b[1])) : make_float2(0.f, 0.f);
+            }
+        }
+    }
+};
+
 __device__ __forceinline__ void metric_row_f(const float *e, int fam, float x, float &s1, float &s2) {
     if (fam == 0) {
         s1 = (1.f - x) * e[1] + x * e[3];
@@ -277,6 +336,39 @@ __device__ __forceinline__ void stage_deviations(const double *Qg, int ncl, doub
     }
     __syncthreads();
 }
+
+// stage_deviations in two parts, so that loads which depend on other staged
+// values can be issued between them: load() issues the state loads, finish()
+// does the rest (with its two barriers)
+template <int N, int CPB, int THREADS> struct DevStage {
+    static constexpr int NP = N * N, NQ = NV * NP, PER = (CPB * NQ + THREADS - 1) / THREADS;
+    double held[PER];
+    __device__ __forceinline__ void load(const double *Qg, int ncl) {
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = (int)threadIdx.x + k * THREADS;
+            held[k] = i < ncl * NQ ? Qg[i] : 0.0;
+        }
+    }
+    __device__ __forceinline__ void finish(int ncl, double gam, double *sBar, Ref *sRef, float *dq) const {
+        constexpr int PC = (N / 2) * N + N / 2;   // central solution point
+        const int tid = threadIdx.x, n = ncl * NQ;
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = tid + k * THREADS;
+            const int c = i / NQ, r = i % NQ, v = r / NP, p = r % NP;
+            if (i < n && p == PC) sBar[c * NV + v] = held[k];
+        }
+        __syncthreads();
+        for (int c = tid; c < ncl; c += THREADS) ref_from(sBar + c * NV, gam, sRef[c]);
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int i = tid + k * THREADS;
+            if (i < n) dq[i] = float(held[k] - sBar[(i / NQ) * NV + (i % NQ) / NP]);
+        }
+        __syncthreads();
+    }
+};
 
 // FP32 extrapolation and derivative operators of order N staged in LDS
 // (the per-thread indices of the contractions would otherwise make every
@@ -474,17 +566,49 @@ __device__ __forceinline__ void mx_trace(const MxTraces &mt, size_t rec, int cel
     }
 }
 
+// upwind side of A_z at a face point: true for the left cell's trace, by the
+// left cell's outward normal velocity, evaluated in FP64 as the FP64 form does
+// (getrusanov.f90:61-66, 98-103): near-zero normal velocities then pick the
+// same side, so A_z keeps the FP64 form's face values.  ql: the left trace
+// (FP64), (m1, m2): its metric row, lfl: its local face.
+__device__ __forceinline__ bool upwind_az(const double *ql, double m1, double m2, int lfl) {
+    const double rhoi = 1.0 / ql[0];
+    const double mn = sqrt(m1 * m1 + m2 * m2);
+    const double ux = m1 / mn * double(side_sign(lfl)), uy = m2 / mn * double(side_sign(lfl));
+    return rhoi * ql[1] * ux + rhoi * ql[2] * uy > 0.0;
+}
+
+// boundary_state in FP32 (for the wave speed of a boundary face point)
+__device__ __forceinline__ void boundary_state_f(int kind, const float *ql, float s1, float s2, const double *qin,
+                                                 float *qr) {
+#pragma unroll
+    for (int v = 0; v < NV; v++) qr[v] = ql[v];
+    if (kind == BK_MIRROR) {
+        const float im = frcp(fsqrt(s1 * s1 + s2 * s2));
+        const float nx = s1 * im, ny = s2 * im;
+        const float mn = ql[1] * nx + ql[2] * ny, bn = ql[5] * nx + ql[6] * ny;
+        qr[1] = ql[1] - 2.f * mn * nx;
+        qr[2] = ql[2] - 2.f * mn * ny;
+        qr[5] = ql[5] - 2.f * bn * nx;
+        qr[6] = ql[6] - 2.f * bn * ny;
+    } else if (kind == BK_INLET) {
+#pragma unroll
+        for (int v = 0; v < 8; v++) qr[v] = float(qin[v]);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Face pass, mixed form: the jump terms of the Rusanov flux for both cells of
-// each face point (see the header), the FP64 upwind A_z, and on the viscous
-// path the half trace jumps; the stage maximum of the FP32 wave speed with
-// ARTIV.  A FREE / OUTL face has no jump (zeros).
+// each face point (see the header) and the FP64 upwind A_z; the stage maximum
+// of the FP32 wave speed with ARTIV.  A FREE / OUTL face has no jump (zeros).
 // FF[cell][lfi][nfp][8] (float): the 8 flux jump terms of the cell's side
 // (the viscous face kernel subtracts its flux from them in place).
 // ---------------------------------------------------------------------------
-// JUMPS = false (viscous path): the jump terms are formed by k_face_visc_mx
-// beside the viscous flux, and this pass keeps the wave-speed maximum, the
-// half trace jumps (for the gradient kernel) and the upwind A_z.
+// JUMPS = false (viscous path with physical viscosity and AV together): only
+// the wave-speed maximum, which the gradient kernel's AV viscosity needs
+// before its interior fluxes.  The viscous face kernel forms the jump terms
+// and the upwind A_z, and with AV alone the gradient kernel forms the maximum
+// itself (the AV fluxes are linear in it).
 template <int N, int NG = 4, bool JUMPS = true>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_face_inv_mx(FaceArgs a) {
     // one thread per face point: both sides' compact traces are loaded
@@ -550,25 +674,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             const float im = frcp(fsqrt(s1 * s1 + s2 * s2));
             eig = wave_speed(Ql, Qr, s1 * im, s2 * im, float(a.gam));
         }
-        if (a.HJ) {
-            // the half trace jumps the gradient kernel's face points use (the
-            // central flux-point value is own + (other - own)/2); a boundary
-            // face has none: its central value is the own trace
-            float4 *hl = reinterpret_cast<float4 *>(a.HJ + (size_t)rl * 8);
+        if constexpr (JUMPS) {
+            float4 *fl = reinterpret_cast<float4 *>((float *)a.FF + (size_t)rl * 8);
             if (free_face) {
-                hl[0] = make_float4(0.f, 0.f, 0.f, 0.f);
-                hl[1] = hl[0];
-            } else {
-                float4 *hr = reinterpret_cast<float4 *>(a.HJ + (size_t)rr * 8);
-                hl[0] = make_float4(0.5f * d[0], 0.5f * d[1], 0.5f * d[2], 0.5f * d[3]);
-                hl[1] = make_float4(0.5f * d[4], 0.5f * d[5], 0.5f * d[6], 0.5f * d[7]);
-                hr[0] = make_float4(-0.5f * d[0], -0.5f * d[1], -0.5f * d[2], -0.5f * d[3]);
-                hr[1] = make_float4(-0.5f * d[4], -0.5f * d[5], -0.5f * d[6], -0.5f * d[7]);
-            }
-        }
-        float4 *fl = reinterpret_cast<float4 *>((float *)a.FF + (size_t)rl * 8);
-        if (free_face) {
-            if constexpr (JUMPS) {
                 // the left cell's jump term against the boundary state (none
                 // at a FREE / OUTL face)
                 if (fc.w == BK_ZEROGRAD) {
@@ -577,27 +685,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                 }
                 fl[0] = make_float4(ol[0], ol[1], ol[2], ol[3]);
                 fl[1] = make_float4(ol[4], ol[5], ol[6], ol[7]);
-            }
-            // A_z of the boundary state is the cell's own: the upwind value is too
-            a.FA[rl] = ql[8];
-        } else {
-            if constexpr (JUMPS) {
+                // A_z of the boundary state is the cell's own: the upwind value is too
+                a.FA[rl] = ql[8];
+            } else {
                 float4 *fr = reinterpret_cast<float4 *>((float *)a.FF + (size_t)rr * 8);
                 fl[0] = make_float4(ol[0], ol[1], ol[2], ol[3]);
                 fl[1] = make_float4(ol[4], ol[5], ol[6], ol[7]);
                 fr[0] = make_float4(orr[0], orr[1], orr[2], orr[3]);
                 fr[1] = make_float4(orr[4], orr[5], orr[6], orr[7]);
+                const double az = upwind_az(ql, m1, m2, fc.y) ? ql[8] : a.mt.az[rr];
+                a.FA[rl] = az;
+                a.FA[rr] = az;
             }
-            // upwind A_z by the left cell's outward normal velocity, evaluated in FP64
-            // as the FP64 form does (getrusanov.f90:61-66, 98-103): near-zero normal
-            // velocities then pick the same side, so A_z keeps the FP64 form's face values
-            const double rhoi = 1.0 / ql[0];
-            const double mn = sqrt(m1 * m1 + m2 * m2);
-            const double ux = m1 / mn * double(side_sign(fc.y)), uy = m2 / mn * double(side_sign(fc.y));
-            const bool take_left = rhoi * ql[1] * ux + rhoi * ql[2] * uy > 0.0;
-            const double az = take_left ? ql[8] : a.mt.az[rr];
-            a.FA[rl] = az;
-            a.FA[rr] = az;
         }
         emax = double(eig);
     }
@@ -735,14 +834,19 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
 }
 
 // ---------------------------------------------------------------------------
-// Viscous / AV path, mixed form: the launches of the FP64 path, two threads
-// per solution point as there.  The central Qv of the gradient pass is a
-// deviation from the cell's reference state (own delta trace plus half the jump of the two
-// FP64 traces at face points); nablaQs, its traces and the viscous fluxes are
-// plain FP32 (they carry no large constant part); the viscous face flux is
-// evaluated once per face point from the FP64 trace average.  k_grad_mx also
-// forms the viscous and AV fluxes of the interior flux points and their part
-// of the divergence at the solution points (fvi), which the stage kernel adds.
+// Viscous / AV path, mixed form: two threads per solution point as in the
+// FP64 path, and no inviscid face pass before the gradient kernel (only with
+// physical viscosity and AV together, for the wave-speed maximum).  The
+// central Qv of the gradient pass is a deviation from the cell's reference
+// state (at face points the mean of the own delta trace and the other cell's
+// compact trace); nablaQs, its traces and the viscous fluxes are plain FP32
+// (they carry no large constant part); the viscous face flux is evaluated once
+// per face point from the trace average, beside the Rusanov jump terms and the
+// upwind A_z.  k_grad_mx also forms the viscous and AV fluxes of the interior
+// flux points and their part of the divergence at the solution points (fvi),
+// which the stage kernel adds.  With AV alone, k_grad_mx forms the stage
+// maximum of the wave speed from its face points and the AV viscosity for a
+// unit wave speed; the viscous face and stage kernels scale by the maximum.
 // ---------------------------------------------------------------------------
 template <int N, int TB, int VM, int NG = 4>
 __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs a) {
@@ -763,6 +867,16 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
     __shared__ float sLf[(N + 1) * N], sMf[(N + 1) * N];
     __shared__ double sOps[Ops64Layout<N>::SIZE];   // FP64 operator tables (OpTables64 layout)
     float *sD = (float *)U, *sV = sD + CPB * NQ, *sG = sD + GO;   // sG: nablaQs, point-major [c][p][16]
+    // the other cells' traces (PartnerStage): their deviations [c][face point][8]
+    // and reference differences [c][lf][8]; they live in sG, which is written
+    // only by the second family's derivative, where N allows
+#if CHORUS_GRAD_MFMA
+    constexpr bool IN_G = false;
+#else
+    constexpr bool IN_G = 32 * (N + 1) <= 16 * N * N;
+#endif
+    __shared__ __align__(16) float sPX[IN_G ? 1 : CPB * 32 * (N + 1)];
+    float *sPD = IN_G ? sG : sPX, *sPR = sPD + CPB * 4 * N * 8;
     const int tid = threadIdx.x;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int c0 = blk * CPB;
@@ -772,6 +886,8 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
         double hx[PX];
         int2 hn[PN];
         OpTables<N, T> ops;
+        PartnerStage<N, CPB, T> partners;
+        partners.gather_nbr(a.nbr, c0, ncl);
         gather_regs<PX, T>(hx, a.cellxy + (size_t)c0 * 2 * NG, ncl * 2 * NG);
         gather_regs<PN, T>(hn, a.nbr + (size_t)c0 * 4, ncl * 4);
         ops.gather();
@@ -779,10 +895,15 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
         ops64.gather();
         EdgeStage<CPB, T> edges;
         if constexpr (NG == 4) edges.gather(a.cellxy + (size_t)c0 * 2 * NG, ncl);
-        stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, sBar, sRef, sD, nullptr);
+        DevStage<N, CPB, T> dev;
+        dev.load(a.Qin + (size_t)c0 * NQ, ncl);
+        // (the neighbour entries were the first loads issued)
+        partners.gather(a.mt);
+        dev.finish(ncl, a.gam, sBar, sRef, sD);
         scatter_lds<PX, T>(sX, hx, ncl * 2 * NG);
         if constexpr (NG == 4) edges.scatter(sE, ncl);
         scatter_lds<PN, T>(sNbr, hn, ncl * 4);
+        partners.scatter(sPD, sPR, sBar, ncl);
         ops.scatter(sLf, sMf);
         ops64.scatter(sOps);
         __syncthreads();
@@ -793,7 +914,11 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
             const int cell = c0 + i / 4;
             const int2 info = sNbr[i];
             const int in = a.inst ? a.inst[cell] : 0;
-            const double eps0 = a.eig[in] * (a.inst ? a.hinst[in] : a.h_msh) / (double)(N - 1) * 1.0;
+            // with AV alone, for a unit wave speed: the AV fluxes are linear in
+            // it, and the viscous face and stage kernels scale them by the stage
+            // maximum that this kernel forms
+            const double ev = VM == 2 ? 1.0 : a.eig[in];
+            const double eps0 = ev * (a.inst ? a.hinst[in] : a.h_msh) / (double)(N - 1) * 1.0;
             const double mo = mu_av(a.sm[cell], eps0, a.pi);
             sMu[i] = ((info.y >> 4) & 1) ? mo : 0.5 * (mo + mu_av(a.sm[info.x], eps0, a.pi));
             if ((i & 3) == 0) a.mucell[cell] = mo;   // for the viscous face kernel (and the output)
@@ -812,14 +937,19 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
         sIJ[i] = float(1.0 / jacobian_g<N, NG>(sX + (i / NP) * 2 * NG, k % N, k / N, sOps + Ops64Layout<N>::XS));
     }
 #endif
+    // with AV alone the stage maximum of the Rusanov wave speed is formed here,
+    // from the face tasks (each of the cell's face points is one of them, in
+    // one family or the other; the speed is symmetric in the two sides)
+    float emax = 0.f;
+    const float gamf = float(a.gam);
 #pragma unroll 1
     for (int fam = 0; fam < 2; fam++) {
         for (int t = tid; t < TS::NI + TS::NFC; t += T) {
             int c, line, r;
-            float half_jump[8];
-#pragma unroll
-            for (int v = 0; v < 8; v++) half_jump[v] = 0.f;
-            if (t < TS::NI) {
+            const bool face_task = t >= TS::NI;
+            bool bnd = true;
+            int kind = BK_ZEROGRAD, pfp = 0;
+            if (!face_task) {
                 c = t / NIT;
                 const int k = t % NIT;
                 line = k / (N - 1); r = 1 + k % (N - 1);
@@ -828,13 +958,12 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
                 c = item / (2 * N);
                 line = k >> 1; r = (k & 1) ? N : 0;
                 if (c < ncl) {
-                    // (other - own)/2 of the FP64 traces, formed by k_face_inv_mx
                     int lfi, nfp;
                     face_of<N>(fam, r, line, lfi, nfp);
-                    const float4 *h = reinterpret_cast<const float4 *>(a.HJ + (size_t)(((c0 + c) * 4 + lfi) * N + nfp) * 8);
-                    const float4 h0 = h[0], h1 = h[1];
-                    half_jump[0] = h0.x; half_jump[1] = h0.y; half_jump[2] = h0.z; half_jump[3] = h0.w;
-                    half_jump[4] = h1.x; half_jump[5] = h1.y; half_jump[6] = h1.z; half_jump[7] = h1.w;
+                    const int2 info = sNbr[c * 4 + lfi];
+                    bnd = (info.y >> 4) & 1;
+                    kind = (info.y >> 5) & 3;
+                    pfp = lfi * N + nfp;
                 }
             }
             if (c >= ncl) continue;
@@ -844,10 +973,39 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
             float d[NV];
             dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d, sLf);
             float *q = sV + c * VST + line * (N + 1) + r;
+            // the other cell's trace as a deviation from this cell's reference
+            // state (the own one at a boundary face)
+            float dj[8];
+            if (face_task && !bnd) {
+                const float *pd = sPD + (c * 4 * N + pfp) * 8, *pr = sPR + (c * 4 + pfp / N) * 8;
 #pragma unroll
-            for (int v = 0; v < 8; v++) q[v * NF] = d[v] + half_jump[v];
+                for (int v = 0; v < 8; v++) dj[v] = pr[v] + pd[v];
+                // the central value (own + other)/2
+#pragma unroll
+                for (int v = 0; v < 8; v++) q[v * NF] = 0.5f * (d[v] + dj[v]);
+            } else {
+#pragma unroll
+                for (int v = 0; v < 8; v++) {
+                    dj[v] = d[v];
+                    q[v * NF] = d[v];
+                }
+            }
             q[8 * NF] = float(s1d);
             q[9 * NF] = float(s2d);
+            if constexpr (VM == 2) {
+                if (face_task) {
+                    float Ql[NV], Qr[NV];
+#pragma unroll
+                    for (int v = 0; v < 8; v++) {
+                        Ql[v] = sRef[c].q[v] + d[v];
+                        Qr[v] = sRef[c].q[v] + dj[v];
+                    }
+                    Ql[8] = Qr[8] = 0.f;
+                    if (bnd && kind != BK_ZEROGRAD) boundary_state_f(kind, Ql, s1d, s2d, a.qin, Qr);
+                    const float im = frcp(fsqrt(s1d * s1d + s2d * s2d));
+                    emax = fmaxf(emax, wave_speed(Ql, Qr, s1d * im, s2d * im, gamf));
+                }
+            }
         }
         __syncthreads();
 #if CHORUS_GRAD_MFMA
@@ -913,6 +1071,11 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
         }
 #endif
         __syncthreads();
+    }
+    if constexpr (VM == 2) {
+        // (every thread: a wave-wide maximum per instance)
+        const int tc = tid >= TS::NI ? min((tid - TS::NI) / (2 * N), ncl - 1) : 0;
+        instance_max_atomic_filtered(double(emax), a.inst ? a.inst[c0 + tc] : 0, a.eig);
     }
     // own face traces of nablaQs: a thread per face point and half of the 16
     // components, the L row in registers
@@ -1008,8 +1171,9 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
 // trace average, together with the Rusanov jump terms of the face point (the
 // viscous path's k_face_inv_mx leaves them to this pass): FF[cell side] =
 // jump - viscous flux
+// (5 waves per SIMD: at most 96 VGPRs, which it reaches without spilling)
 template <int N, int VM, int NG = 4>
-__global__ void __launch_bounds__(256) k_face_visc_mx(FaceArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_face_visc_mx(FaceArgs a) {
     const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= a.nface * N) return;
     const int f = t / N, k = t % N;
@@ -1059,6 +1223,9 @@ __global__ void __launch_bounds__(256) k_face_visc_mx(FaceArgs a) {
             for (int v = 0; v < 8; v++) dj[v] = 0.f;
         }
     }
+    // the FP64 upwind A_z of the face point for both cells (the own one at a
+    // boundary face: the boundary state keeps the cell's A_z), stored last
+    const double az = a.mt.az[free_face || upwind_az(ql, ml1, ml2, fc.y) ? pl : pr];
     ql[8] = 0.0;
     float qv[NV], g[2 * NV];
 #pragma unroll
@@ -1107,7 +1274,10 @@ __global__ void __launch_bounds__(256) k_face_visc_mx(FaceArgs a) {
         // muAV(1) of both cells (mu_av of this stage's epsilon0, kept by k_grad_mx)
         const double ml = a.mucell[fc.x];
         const double mr = free_face ? ml : a.mucell[fc.z];
-        const double mf = free_face ? ml : 0.5 * (ml + mr);
+        double mf = free_face ? ml : 0.5 * (ml + mr);
+        // with AV alone the gradient kernel formed muAV for a unit wave speed
+        // and the stage maximum beside it
+        if constexpr (VM == 2) mf = mf * a.eig[a.inst ? a.inst[fc.x] : 0];
         vflux<float>(gam, lam, prandt, float(mf), 0.f, qv, g, Fv, Gv);
 #pragma unroll
         for (int w = 0; w < NV; w++) {
@@ -1120,10 +1290,12 @@ __global__ void __launch_bounds__(256) k_face_visc_mx(FaceArgs a) {
     float4 *rl = reinterpret_cast<float4 *>((float *)a.FF + pl * 8);
     rl[0] = make_float4(jl[0] - fl[0], jl[1] - fl[1], jl[2] - fl[2], jl[3] - fl[3]);
     rl[1] = make_float4(jl[4] - fl[4], jl[5] - fl[5], jl[6] - fl[6], jl[7] - fl[7]);
+    a.FA[pl] = az;
     if (!free_face) {
         float4 *rr = reinterpret_cast<float4 *>((float *)a.FF + pr * 8);
         rr[0] = make_float4(jr[0] - fr[0], jr[1] - fr[1], jr[2] - fr[2], jr[3] - fr[3]);
         rr[1] = make_float4(jr[4] - fr[4], jr[5] - fr[5], jr[6] - fr[6], jr[7] - fr[7]);
+        a.FA[pr] = az;
     }
 }
 
@@ -1284,6 +1456,15 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
         // 1/J: FP64 for the A_z residual (group 0), FP32 for the rest
         const double iJd = grp == 0 ? 1.0 / jacobian_g<N, NG>(sX + pc * 2 * NG, pis, pjs, sOps + Ops64Layout<N>::XS) : 0.0;
         const float iJ = grp == 0 ? float(iJd) : inv_jacobian_mx<N, NG>(sE, sX, sOps, pc, pis, pjs);
+        if constexpr (VM == 2) {
+            // AV alone: the gradient kernel formed the interior AV fluxes for a
+            // unit wave speed; the stage maximum scales them (and muAV, kept
+            // for the output)
+            const double e = v.eig[v.inst ? v.inst[c0 + pc] : 0];
+            const float ef = float(e);
+            pv.x *= ef; pv.y *= ef; pv.z *= ef; pv.w *= ef;
+            if (pk == 0 && grp == 0) v.mucell[c0 + pc] *= e;
+        }
         float res[4];
         res[0] = (pv.x - acc[0]) * iJ;
         res[1] = (pv.y - acc[1]) * iJ;
