@@ -337,6 +337,42 @@ __device__ __forceinline__ void stage_deviations(const double *Qg, int ncl, doub
     __syncthreads();
 }
 
+// The face-point records of the block's cells (cells c0.., ncl of them) read
+// by the mixed stage kernels: the 8-float records of FF and the FP64 upwind
+// A_z of FA, staged into LDS with the state, so that the flux tasks wait for
+// no global load
+template <int N, int CPB, int T> struct FaceRecStage {
+    static constexpr int N4 = CPB * 8 * N, NA = CPB * 4 * N;   // float4 items, doubles
+    static constexpr int P4 = (N4 + T - 1) / T, PA = (NA + T - 1) / T;
+    float4 f[P4];
+    double az[PA];
+    __device__ __forceinline__ void gather(const float *FF, const double *FA, int c0, int ncl) {
+        const float4 *src = reinterpret_cast<const float4 *>(FF) + (size_t)c0 * 8 * N;
+#pragma unroll
+        for (int k = 0; k < P4; k++) {
+            const int i = (int)threadIdx.x + k * T;
+            f[k] = i < ncl * 8 * N ? src[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int k = 0; k < PA; k++) {
+            const int i = (int)threadIdx.x + k * T;
+            az[k] = i < ncl * 4 * N ? FA[(size_t)c0 * 4 * N + i] : 0.0;
+        }
+    }
+    __device__ __forceinline__ void scatter(float *sFF, double *sFA, int ncl) const {
+#pragma unroll
+        for (int k = 0; k < P4; k++) {
+            const int i = (int)threadIdx.x + k * T;
+            if (i < ncl * 8 * N) reinterpret_cast<float4 *>(sFF)[i] = f[k];
+        }
+#pragma unroll
+        for (int k = 0; k < PA; k++) {
+            const int i = (int)threadIdx.x + k * T;
+            if (i < ncl * 4 * N) sFA[i] = az[k];
+        }
+    }
+};
+
 // stage_deviations in two parts, so that loads which depend on other staged
 // values can be issued between them: load() issues the state loads, finish()
 // does the rest (with its two barriers)
@@ -1323,10 +1359,10 @@ template <int N, int TB> struct MxVShape {
 // flux of A_z is zero (getrusanov.f90:157-361), so A_z keeps its FP64
 // upwind/trace values alone.  Then one sum per component.
 // waves per SIMD requested from the register allocator: 5 (at most 96 VGPRs)
-// at N = 5, where it spills nothing; N = 3, 4 need fewer registers anyway and
-// N = 6 would spill (measured per order and tile shape)
+// up to N = 5, where it spills at most a few words; N = 6 would spill
+// (measured per order and tile shape)
 #ifndef CHORUS_MX_VISC_MINW
-#define CHORUS_MX_VISC_MINW(N) ((N) == 5 ? 5 : (N) <= 6 ? 4 : 3)
+#define CHORUS_MX_VISC_MINW(N) ((N) <= 5 ? 5 : (N) <= 6 ? 4 : 3)
 #endif
 template <int N, int TB, int VM, int NG = 4>
 __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amdgpu_waves_per_eu(CHORUS_MX_VISC_MINW(N)))) k_stage_visc_mx(StageArgs a, ViscArgs v) {
@@ -1349,6 +1385,8 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     __shared__ float sLf[(N + 1) * N], sMf[(N + 1) * N];
     __shared__ double sOps[Ops64Layout<N>::SIZE];   // FP64 operator tables (OpTables64 layout)
     __shared__ float sRav[N * N];        // AV interpolation round trip (c_Rav)
+    __shared__ __align__(16) float sFF[CPB * 4 * N * 8];   // the cells' face records (FaceRecStage)
+    __shared__ double sFA[CPB * 4 * N];
     float *sD = (float *)U, *sU = sD + CPB * NQ;
     if constexpr ((VM & 2) != 0) {
         if (threadIdx.x < N * N) sRav[threadIdx.x] = c_Rav[N][threadIdx.x];
@@ -1372,9 +1410,12 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
         gather_regs<PX, T>(hx, a.cellxy + (size_t)c0 * 2 * NG, ncl * 2 * NG);
         EdgeStage<CPB, T> edges;
         if constexpr (NG == 4) edges.gather(a.cellxy + (size_t)c0 * 2 * NG, ncl);
+        FaceRecStage<N, CPB, T> recs;
+        recs.gather((const float *)a.FF, a.FA, c0, ncl);
         stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, sBar, sRef, sD, sAz, 8, sQ64);
         scatter_lds<PX, T>(sX, hx, ncl * 2 * NG);
         if constexpr (NG == 4) edges.scatter(sE, ncl);
+        recs.scatter(sFF, sFA, ncl);
         ops.scatter(sLf, sMf);
         ops64.scatter(sOps);
     }
@@ -1390,7 +1431,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     const bool pact = (pt < CPB * NP) && (pc < ncl);
     const size_t base = (size_t)(c0 + pc) * NQ + pk;
     const int vg = 4 * grp;              // first variable of the thread's group
-    const float *FF = (const float *)a.FF, *FVI = v.fvi_r<float>();
+    const float *FVI = v.fvi_r<float>();
     float acc[4];
     double qa[5];
     float4 pv = make_float4(0.f, 0.f, 0.f, 0.f);   // the interior viscous fluxes' divergence (k_grad_mx)
@@ -1408,7 +1449,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
             if (r == 0 || r == N) {
                 int lfi, nfp;
                 face_of<N>(fam, r, line, lfi, nfp);
-                const float4 *src = reinterpret_cast<const float4 *>(FF + (size_t)(((c0 + c) * 4 + lfi) * N + nfp) * 8);
+                const float4 *src = reinterpret_cast<const float4 *>(sFF + ((c * 4 + lfi) * N + nfp) * 8);
                 const float4 e0 = src[0], e1 = src[1];
                 extra[0] = e0.x; extra[1] = e0.y; extra[2] = e0.z; extra[3] = e0.w;
                 extra[4] = e1.x; extra[5] = e1.y; extra[6] = e1.z; extra[7] = e1.w;
@@ -1424,7 +1465,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
             dflux(sRef[c], d, gam - 1.f, F, G);
 #pragma unroll
             for (int w = 0; w < 8; w++) u[w * NF] = F[w] * s1 + G[w] * s2 + extra[w];
-            sA[(fam * CPB + c) * AST + line * (N + 1) + r] = az_at<N>(a.FA, sAz, c0 + c, c, fam, r, line, sOps);
+            sA[(fam * CPB + c) * AST + line * (N + 1) + r] = az_at<N>(sFA, sAz, c, c, fam, r, line, sOps);
         }
         __syncthreads();
         // the point's Qa for the combination, loaded while the derivative is summed
