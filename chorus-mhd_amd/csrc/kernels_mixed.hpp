@@ -1435,6 +1435,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     float acc[4];
     double qa[5];
     float4 pv = make_float4(0.f, 0.f, 0.f, 0.f);   // the interior viscous fluxes' divergence (k_grad_mx)
+    double eg = 0.0;                                 // AV alone: the stage maximum of the wave speed
 #pragma unroll 1
     for (int fam = 0; fam < 2; fam++) {
         for (int t = tid; t < TS::NA; t += T) {
@@ -1475,6 +1476,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
             for (int k = 0; k < 4; k++) qa[k] = need ? a.Qa[base + (size_t)(vg + k) * NP] : 0.0;
             qa[4] = need && grp == 0 ? a.Qa[base + (size_t)8 * NP] : 0.0;
             pv = reinterpret_cast<const float4 *>(FVI + ((size_t)(c0 + pc) * NP + pk) * 8 + vg)[0];
+            if constexpr (VM == 2) eg = v.eig[v.inst ? v.inst[c0 + pc] : 0];
         }
         if (pact) {
             const int line = fam == 0 ? pjs : pis, pos = fam == 0 ? pis : pjs;
@@ -1501,10 +1503,9 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
             // AV alone: the gradient kernel formed the interior AV fluxes for a
             // unit wave speed; the stage maximum scales them (and muAV, kept
             // for the output)
-            const double e = v.eig[v.inst ? v.inst[c0 + pc] : 0];
-            const float ef = float(e);
+            const float ef = float(eg);
             pv.x *= ef; pv.y *= ef; pv.z *= ef; pv.w *= ef;
-            if (pk == 0 && grp == 0) v.mucell[c0 + pc] *= e;
+            if (pk == 0 && grp == 0) v.mucell[c0 + pc] *= eg;
         }
         float res[4];
         res[0] = (pv.x - acc[0]) * iJ;
