@@ -101,6 +101,14 @@ def test_mixed_vs_reference(name):
     s = _solver(g, "mixed")
     s.set_state(g["q0"])
     rr = _rel_l2(s.compresid(), g["resid"])
+    if int(g["artiv"]):
+        # the stage maximum of the wave speed and the AV scale epsilon0 it sets
+        # (with AV alone the mixed gradient kernel forms the maximum from its
+        # face points, in FP32) against the reference's
+        d = s.diag()
+        _say(name, "mixed eigvmax", d["eigvmax"], "reference", float(g["eigvmax"]))
+        assert d["eigvmax"] == pytest.approx(float(g["eigvmax"]), rel=1e-6)
+        assert d["epsilon0"] == pytest.approx(float(g["epsilon0"]), rel=1e-6)
     s.step(int(g["steps"]))
     q = s.get_state()
     s.close()
