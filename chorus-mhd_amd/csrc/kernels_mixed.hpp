@@ -209,6 +209,12 @@ template <int CPB, int T> struct EdgeStage {
 // own one (sBar), rounded to FP32; zeros at boundary faces.  gather_nbr()
 // loads the neighbour entries first, gather() the traces (after the other
 // staging loads are issued), scatter() writes them to LDS once sBar is there.
+// point stride of the gradient kernel's point-major nablaQs in LDS: 16 values
+// padded to 20 floats (80 bytes, still 16-byte aligned), so that the 16-byte
+// accesses of neighbouring points start on different banks
+#ifndef CHORUS_MX_GS
+#define CHORUS_MX_GS 20
+#endif
 template <int N, int CPB, int T> struct PartnerStage {
     static constexpr int ND = CPB * 8 * N, NR = CPB * 16;   // float4 and double2 items
     static constexpr int PD = (ND + T - 1) / T, PR = (NR + T - 1) / T;
@@ -890,8 +896,9 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
     using TS = V2ShapeT<N, TB>;
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ, T = TS::THREADS, NIT = TS::NIT;
     constexpr int VST = 10 * NF + 1;
+    constexpr int GS = CHORUS_MX_GS;                          // sG point stride (floats)
     constexpr int GO = (CPB * NQ + CPB * VST + 3) / 4 * 4;   // sG offset (16-byte aligned)
-    constexpr int UF = GO + CPB * 16 * NP;
+    constexpr int UF = GO + CPB * GS * NP;
     constexpr int UD = (UF + 1) / 2 > CPB * NQ ? (UF + 1) / 2 : CPB * NQ;
     __shared__ __align__(16) double U[UD];
     __shared__ double sX[CPB * 2 * NG];
@@ -902,14 +909,14 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
     __shared__ double sMu[CPB * 4];      // muAVfi/fj(1,side) per local face
     __shared__ float sLf[(N + 1) * N], sMf[(N + 1) * N];
     __shared__ double sOps[Ops64Layout<N>::SIZE];   // FP64 operator tables (OpTables64 layout)
-    float *sD = (float *)U, *sV = sD + CPB * NQ, *sG = sD + GO;   // sG: nablaQs, point-major [c][p][16]
+    float *sD = (float *)U, *sV = sD + CPB * NQ, *sG = sD + GO;   // sG: nablaQs, point-major [c][p][GS]
     // the other cells' traces (PartnerStage): their deviations [c][face point][8]
     // and reference differences [c][lf][8]; they live in sG, which is written
     // only by the second family's derivative, where N allows
 #if CHORUS_GRAD_MFMA
     constexpr bool IN_G = false;
 #else
-    constexpr bool IN_G = 32 * (N + 1) <= 16 * N * N;
+    constexpr bool IN_G = 32 * (N + 1) <= GS * N * N;
 #endif
     __shared__ __align__(16) float sPX[IN_G ? 1 : CPB * 32 * (N + 1)];
     float *sPD = IN_G ? sG : sPX, *sPR = sPD + CPB * 4 * N * 8;
@@ -1074,7 +1081,7 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
                     const int pos = 4 * kq + j;
                     if (pos < N) {
                         const int k = fam == 0 ? line * N + pos : pos * N + line;
-                        float *g = sG + (c * NP + k) * 16 + cgrp * 8 + cv;
+                        float *g = sG + (c * NP + k) * GS + cgrp * 8 + cv;
                         *g = fam == 0 ? acc[j] : (*g + acc[j]) * sIJ[c * NP + k];
                     }
                 }
@@ -1102,7 +1109,7 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
                 const float iJ = inv_jacobian_mx<N, NG>(sE, sX, sOps, pc, pis, pjs);
 #pragma unroll
                 for (int v = 0; v < 8; v++) gp[v] = (gp[v] + acc[v]) * iJ;
-                lds_store8(sG + (pc * NP + pk) * 16 + grp * 8, gp);
+                lds_store8(sG + (pc * NP + pk) * GS + grp * 8, gp);
             }
         }
 #endif
@@ -1132,7 +1139,7 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
 #pragma unroll
         for (int sp = 0; sp < N; sp++) {
             float v[8];
-            lds_load8(sG + (c * NP + base + sp * stride) * 16 + h * 8, v);
+            lds_load8(sG + (c * NP + base + sp * stride) * GS + h * 8, v);
 #pragma unroll
             for (int gg = 0; gg < 8; gg++) acc[gg] += v[gg] * Lr[sp];
         }
@@ -1170,7 +1177,7 @@ __global__ void __launch_bounds__((V2ShapeT<N, TB>::THREADS)) k_grad_mx(ViscArgs
         }
 #pragma unroll
         for (int w = 0; w < NV; w++) qv[w] = sRef[c].q[w] + d[w];
-        grad_trace_pm<N, float>(sG + c * NP * 16, fam, ifp, jfp, g, sLf);
+        grad_trace_pm<N, float, GS>(sG + c * NP * GS, fam, ifp, jfp, g, sLf);
         visc_point<N, float, VM>(a, sMu + c * 4, fam, ifp, jfp, s1, s2, qv, g, fv, sOps + Ops64Layout<N>::XF);
         // (8 components: the viscous flux of A_z is zero)
 #pragma unroll

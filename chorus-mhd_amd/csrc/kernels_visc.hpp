@@ -232,9 +232,9 @@ __device__ __forceinline__ void lds_store8(double *p, const double *v) {
 }
 
 // the same with the cell's nablaQs point-major (mixed form: 16-byte LDS reads)
-template <int N, class R>
+template <int N, class R, int PS = 16>
 __device__ __forceinline__ void grad_trace_pm(const R *gc, int fam, int ifp, int jfp, R *g, const R *Lt = nullptr) {
-    // gc: the cell's nablaQs, point-major: [p][d*8+v] (16 values per point)
+    // gc: the cell's nablaQs, point-major: [p][d*8+v] (16 values per point, PS apart)
 #pragma unroll
     for (int d = 0; d < 2; d++) {
 #pragma unroll
@@ -249,8 +249,8 @@ __device__ __forceinline__ void grad_trace_pm(const R *gc, int fam, int ifp, int
         const R l = Lt ? Lt[row * N + sp] : Lop<R>(N, row, sp);
         const int idx = fam == 0 ? jfp * N + sp : sp * N + ifp;
         R v[16];
-        lds_load8(gc + idx * 16, v);
-        lds_load8(gc + idx * 16 + 8, v + 8);
+        lds_load8(gc + idx * PS, v);
+        lds_load8(gc + idx * PS + 8, v + 8);
 #pragma unroll
         for (int d = 0; d < 2; d++)
 #pragma unroll
