@@ -1378,7 +1378,11 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ, T = TS::THREADS;
     constexpr int UST = 8 * NF + 1;      // per-cell stride of the combined flux planes
     constexpr int AST = NF + 1;          // per-cell stride of the FP64 A_z flux-point values
-    constexpr int UF = CPB * NQ + CPB * UST;
+    // the flux planes of both families side by side where the LDS allows the
+    // same blocks per CU (N >= 6: two 512-thread blocks either way), of one
+    // family at a time otherwise
+    constexpr bool MERGE = N >= 6;
+    constexpr int UF = CPB * NQ + (MERGE ? 2 : 1) * CPB * UST;   // deviations + flux planes
     constexpr int UD = (UF + 1) / 2;
     __shared__ double U[UD];
     __shared__ double sQ64[CPB * NQ];    // FP64 state (combined in place into the new state)
@@ -1443,64 +1447,85 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     double qa[5];
     float4 pv = make_float4(0.f, 0.f, 0.f, 0.f);   // the interior viscous fluxes' divergence (k_grad_mx)
     double eg = 0.0;                                 // AV alone: the stage maximum of the wave speed
-#pragma unroll 1
-    for (int fam = 0; fam < 2; fam++) {
-        for (int t = tid; t < TS::NA; t += T) {
-            const int c = t / NF, k = t % NF;
-            if (c >= ncl) continue;
-            const int line = k / (N + 1), r = k % (N + 1);
-            const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
-            float *u = sU + c * UST + line * (N + 1) + r;
-            // the terms beside the own delta flux: jump - viscous face flux at a
-            // face point, minus the interior viscous flux elsewhere
-            float extra[8];
-            if (r == 0 || r == N) {
-                int lfi, nfp;
-                face_of<N>(fam, r, line, lfi, nfp);
-                const float4 *src = reinterpret_cast<const float4 *>(sFF + ((c * 4 + lfi) * N + nfp) * 8);
-                const float4 e0 = src[0], e1 = src[1];
-                extra[0] = e0.x; extra[1] = e0.y; extra[2] = e0.z; extra[3] = e0.w;
-                extra[4] = e1.x; extra[5] = e1.y; extra[6] = e1.z; extra[7] = e1.w;
-            } else {
-                // (the interior viscous flux enters through k_grad_mx's partial divergence)
+    // one flux task per flux point: the own delta flux plus, at a face point,
+    // the jump - viscous face flux record (the interior viscous flux enters
+    // through k_grad_mx's partial divergence); and the FP64 A_z of the point
+    auto flux_task = [&](int fam, int c, int k) {
+        const int line = k / (N + 1), r = k % (N + 1);
+        const int ifp = fam == 0 ? r : line, jfp = fam == 0 ? line : r;
+        float *u = sU + ((MERGE ? fam : 0) * CPB + c) * UST + line * (N + 1) + r;
+        float extra[8];
+        if (r == 0 || r == N) {
+            int lfi, nfp;
+            face_of<N>(fam, r, line, lfi, nfp);
+            const float4 *src = reinterpret_cast<const float4 *>(sFF + ((c * 4 + lfi) * N + nfp) * 8);
+            const float4 e0 = src[0], e1 = src[1];
+            extra[0] = e0.x; extra[1] = e0.y; extra[2] = e0.z; extra[3] = e0.w;
+            extra[4] = e1.x; extra[5] = e1.y; extra[6] = e1.z; extra[7] = e1.w;
+        } else {
 #pragma unroll
-                for (int w = 0; w < 8; w++) extra[w] = 0.f;
-            }
-            float s1, s2;
-            metric_row_mx<N, NG>(sE, sX, sOps, c, fam, r, ifp, jfp, s1, s2);
-            float d[NV], F[8], G[8];
-            dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d, sLf);
-            dflux(sRef[c], d, gam - 1.f, F, G);
-#pragma unroll
-            for (int w = 0; w < 8; w++) u[w * NF] = F[w] * s1 + G[w] * s2 + extra[w];
-            sA[(fam * CPB + c) * AST + line * (N + 1) + r] = az_at<N>(sFA, sAz, c, c, fam, r, line, sOps);
+            for (int w = 0; w < 8; w++) extra[w] = 0.f;
         }
-        __syncthreads();
-        // the point's Qa for the combination, loaded while the derivative is summed
-        if (fam == 1 && pact) {
-            const bool need = a.kind != KIND_RESID && a.kind != KIND_AXPY;
+        float s1, s2;
+        metric_row_mx<N, NG>(sE, sX, sOps, c, fam, r, ifp, jfp, s1, s2);
+        float d[NV], F[8], G[8];
+        dtrace9<N>(sD + c * NQ, fam, ifp, jfp, d, sLf);
+        dflux(sRef[c], d, gam - 1.f, F, G);
 #pragma unroll
-            for (int k = 0; k < 4; k++) qa[k] = need ? a.Qa[base + (size_t)(vg + k) * NP] : 0.0;
-            qa[4] = need && grp == 0 ? a.Qa[base + (size_t)8 * NP] : 0.0;
-            pv = reinterpret_cast<const float4 *>(FVI + ((size_t)(c0 + pc) * NP + pk) * 8 + vg)[0];
-            if constexpr (VM == 2) eg = v.eig[v.inst ? v.inst[c0 + pc] : 0];
-        }
-        if (pact) {
-            const int line = fam == 0 ? pjs : pis, pos = fam == 0 ? pis : pjs;
-            const float *f = sU + pc * UST + line * (N + 1) + vg * NF;
-            float d4[4];
+        for (int w = 0; w < 8; w++) u[w * NF] = F[w] * s1 + G[w] * s2 + extra[w];
+        sA[(fam * CPB + c) * AST + line * (N + 1) + r] = az_at<N>(sFA, sAz, c, c, fam, r, line, sOps);
+    };
+    // the point's Qa for the combination, loaded while the derivative is summed
+    auto load_point = [&]() {
+        const bool need = a.kind != KIND_RESID && a.kind != KIND_AXPY;
 #pragma unroll
-            for (int w = 0; w < 4; w++) d4[w] = 0.f;
+        for (int k = 0; k < 4; k++) qa[k] = need ? a.Qa[base + (size_t)(vg + k) * NP] : 0.0;
+        qa[4] = need && grp == 0 ? a.Qa[base + (size_t)8 * NP] : 0.0;
+        pv = reinterpret_cast<const float4 *>(FVI + ((size_t)(c0 + pc) * NP + pk) * 8 + vg)[0];
+        if constexpr (VM == 2) eg = v.eig[v.inst ? v.inst[c0 + pc] : 0];
+    };
+    // the derivative of one family's planes (the family-1 sum added to the family-0 one)
+    auto derivative = [&](int fam) {
+        const int line = fam == 0 ? pjs : pis, pos = fam == 0 ? pis : pjs;
+        const float *f = sU + ((MERGE ? fam : 0) * CPB + pc) * UST + line * (N + 1) + vg * NF;
+        float d4[4];
+#pragma unroll
+        for (int w = 0; w < 4; w++) d4[w] = 0.f;
 #pragma unroll CHORUS_MX_CUNROLL
-            for (int r = 0; r <= N; r++) {
-                const float m = sMf[r * N + pos];
+        for (int r = 0; r <= N; r++) {
+            const float m = sMf[r * N + pos];
 #pragma unroll
-                for (int w = 0; w < 4; w++) d4[w] += f[w * NF + r] * m;
-            }
+            for (int w = 0; w < 4; w++) d4[w] += f[w * NF + r] * m;
+        }
 #pragma unroll
-            for (int w = 0; w < 4; w++) acc[w] = fam == 0 ? d4[w] : acc[w] + d4[w];
+        for (int w = 0; w < 4; w++) acc[w] = fam == 0 ? d4[w] : acc[w] + d4[w];
+    };
+    if constexpr (MERGE) {
+        // both families' flux tasks in one phase (their planes side by side):
+        // they fill the threads more evenly than a family per round, and two
+        // barriers go
+        for (int t = tid; t < 2 * TS::NA; t += T) {
+            const int fam = t >= TS::NA ? 1 : 0, tf = t - fam * TS::NA;
+            if (tf / NF < ncl) flux_task(fam, tf / NF, tf % NF);
         }
         __syncthreads();
+        if (pact) {
+            load_point();
+            derivative(0);
+            derivative(1);
+        }
+    } else {
+#pragma unroll 1
+        for (int fam = 0; fam < 2; fam++) {
+            for (int t = tid; t < TS::NA; t += T)
+                if (t / NF < ncl) flux_task(fam, t / NF, t % NF);
+            __syncthreads();
+            if (pact) {
+                if (fam == 1) load_point();
+                derivative(fam);
+            }
+            __syncthreads();
+        }
     }
     if (pact) {
         // 1/J: FP64 for the A_z residual (group 0), FP32 for the rest
