@@ -41,7 +41,12 @@ METRIC = "DOF-updates/s per RK stage"
 # host cores of one GPU's share of the GPU box (the CPU baseline's process count)
 CPU_SHARE = 16
 PROFILES = os.path.join(REPO, "profiles")
-TILES = ["256/128", "128/64", "512/256"]
+# tile index -> (stage shape, gradient shape) of the mixed cell kernels; a shape
+# is the threads-per-block target of the ideal / viscous kernels
+_SHAPES = ["256/128", "128/64", "512/256"]
+_TILE_PAIRS = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 0), (1, 2), (2, 0), (2, 1)]
+TILES = [_SHAPES[s] if s == g else "%s, gradient %s" % (_SHAPES[s], _SHAPES[g].split("/")[1])
+         for s, g in _TILE_PAIRS]
 
 
 def measured_traffic(key: str):

@@ -33,10 +33,17 @@ hipError_t launch_trace_mixed(int N, int ncell, const double *Q, const MxTraces 
 }
 static int fblocks(int N, int nface) { return (nface * N + 255) / 256; }
 
-// tile shapes of the mixed cell kernels (threads-per-block targets of the
-// ideal and viscous kernels), chosen per order by the autotuner:
-//   tile 0: 256 / 128, tile 1: 128 / 64, tile 2: 512 / 256 (default)
-int tile_count() { return 3; }
+// tile shapes of the mixed cell kernels, chosen per order by the autotuner.
+// A shape is a threads-per-block target of the ideal / viscous kernels:
+//   shape 0: 256 / 128, shape 1: 128 / 64, shape 2: 512 / 256.
+// Tile t gives the ideal or viscous stage kernel shape kStage[t] and the
+// gradient kernel shape kGrad[t]: tiles 0-2 one shape for both, 3-8 the other
+// pairs (the two kernels of an order need not prefer the same shape).
+namespace {
+constexpr int kStage[9] = {0, 1, 2, 0, 0, 1, 1, 2, 2};
+constexpr int kGrad[9] = {0, 1, 2, 1, 2, 0, 2, 0, 1};
+}
+int tile_count() { return 9; }
 
 template <int n, int TBI, int NG>
 static hipError_t ideal_stage(const StageArgs &a, const FaceArgs &f, hipStream_t s) {
@@ -48,7 +55,7 @@ static hipError_t ideal_stage(const StageArgs &a, const FaceArgs &f, hipStream_t
 
 template <int n, int NG>
 static hipError_t ideal_stage_ng(const StageArgs &a, const FaceArgs &f, hipStream_t s, int tile) {
-    switch (tile) {
+    switch (kStage[tile]) {
     case 1: return ideal_stage<n, 128, NG>(a, f, s);
     case 2: return ideal_stage<n, 512, NG>(a, f, s);
     default: return ideal_stage<n, 256, NG>(a, f, s);
@@ -75,15 +82,16 @@ hipError_t launch_stage_ideal_mixed(int N, const StageArgs &a, const FaceArgs &f
     }
 }
 
-template <int n, int TBV, int VM, int NG>
+template <int n, int TBG, int TBV, int VM, int NG>
 static hipError_t visc_stage(const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s) {
     using S = CellShape<n, TBV>;
+    using SG = CellShape<n, TBG>;
     // the gradient kernel reads the other cells' traces itself; only physical
     // viscosity with AV needs the wave-speed maximum before it (with AV alone
     // the gradient kernel forms it)
     if constexpr (VM == 3)
         hipLaunchKernelGGL((k_face_inv_mx<n, NG, false>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);
-    hipLaunchKernelGGL((k_grad_mx<n, TBV, VM, NG>), dim3(blocks_of<S>(a.ncell)), dim3(V2ShapeT<n, TBV>::THREADS), 0, s,
+    hipLaunchKernelGGL((k_grad_mx<n, TBG, VM, NG>), dim3(blocks_of<SG>(a.ncell)), dim3(V2ShapeT<n, TBG>::THREADS), 0, s,
                        v);
     hipLaunchKernelGGL((k_face_visc_mx<n, VM, NG>), dim3(fblocks(n, f.nface)), dim3(256), 0, s, f);
     hipLaunchKernelGGL((k_stage_visc_mx<n, TBV, VM, NG>), dim3(blocks_of<S>(a.ncell)), dim3(MxVShape<n, TBV>::THREADS),
@@ -91,29 +99,39 @@ static hipError_t visc_stage(const StageArgs &a, const ViscArgs &v, const FaceAr
     return hipGetLastError();
 }
 
-template <int n, int TBV, int NG>
+template <int n, int TBG, int TBV, int NG>
 static hipError_t visc_stage_vm_ng(const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s) {
     const int vm = ((v.mu != 0.0 || v.eta0 != 0.0) ? 1 : 0) | (v.artiv ? 2 : 0);
     switch (vm) {
-    case 1: return visc_stage<n, TBV, 1, NG>(a, v, f, s);
-    case 2: return visc_stage<n, TBV, 2, NG>(a, v, f, s);
-    case 3: return visc_stage<n, TBV, 3, NG>(a, v, f, s);
+    case 1: return visc_stage<n, TBG, TBV, 1, NG>(a, v, f, s);
+    case 2: return visc_stage<n, TBG, TBV, 2, NG>(a, v, f, s);
+    case 3: return visc_stage<n, TBG, TBV, 3, NG>(a, v, f, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-template <int n, int TBV>
+template <int n, int TBG, int TBV>
 static hipError_t visc_stage_vm(const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s) {
     if ((a.ng == 8) != (f.ng == 8)) return hipErrorInvalidValue;
-    return a.ng == 8 ? visc_stage_vm_ng<n, TBV, 8>(a, v, f, s) : visc_stage_vm_ng<n, TBV, 4>(a, v, f, s);
+    return a.ng == 8 ? visc_stage_vm_ng<n, TBG, TBV, 8>(a, v, f, s) : visc_stage_vm_ng<n, TBG, TBV, 4>(a, v, f, s);
+}
+
+// the viscous kernels' TB of each shape (shape 0: 128, 1: 64, 2: 256)
+template <int n, int TBV>
+static hipError_t visc_stage_grad(const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s, int g) {
+    switch (g) {
+    case 1: return visc_stage_vm<n, 64, TBV>(a, v, f, s);
+    case 2: return visc_stage_vm<n, 256, TBV>(a, v, f, s);
+    default: return visc_stage_vm<n, 128, TBV>(a, v, f, s);
+    }
 }
 
 template <int n>
 static hipError_t visc_stage_tile(const StageArgs &a, const ViscArgs &v, const FaceArgs &f, hipStream_t s, int tile) {
-    switch (tile) {
-    case 1: return visc_stage_vm<n, 64>(a, v, f, s);
-    case 2: return visc_stage_vm<n, 256>(a, v, f, s);
-    default: return visc_stage_vm<n, 128>(a, v, f, s);
+    switch (kStage[tile]) {
+    case 1: return visc_stage_grad<n, 64>(a, v, f, s, kGrad[tile]);
+    case 2: return visc_stage_grad<n, 256>(a, v, f, s, kGrad[tile]);
+    default: return visc_stage_grad<n, 128>(a, v, f, s, kGrad[tile]);
     }
 }
 

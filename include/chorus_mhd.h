@@ -204,11 +204,13 @@ int chorus_write_tecplot(chorus_ctx *ctx, const char *path, double ctime);
 int chorus_write_restart(chorus_ctx *ctx, const char *path, int iter, double ctime, const int *icg);
 
 /* Tile-shape autotuner (no reference counterpart): times the mixed form's
- * cell kernels in every compiled tile shape on this context's own mesh and
- * state (residual evaluations, no state change), keeps the fastest for later
- * steps and returns its index in *tile (0: 256/128, 1: 128/64, 2: 512/256
- * threads of the ideal/viscous cell kernels).  The FP64 form has one shape
- * (*tile = 0). */
+ * cell kernels in every compiled tile on this context's own mesh and state
+ * (residual evaluations, no state change), keeps the fastest for later steps
+ * and returns its index in *tile.  A tile gives the shape of the stage kernel
+ * and that of the gradient kernel, each one of three threads-per-block
+ * targets (ideal/viscous kernels: 256/128, 128/64, 512/256): tiles 0-2 use
+ * shape 0, 1 or 2 for both, tiles 3-8 the pairs (stage, gradient) = (0,1),
+ * (0,2), (1,0), (1,2), (2,0), (2,1).  The FP64 form has one shape (*tile = 0). */
 int chorus_autotune(chorus_ctx *ctx, int *tile);
 
 /* Fixes the tile shape of the mixed form's cell kernels (an index as returned
