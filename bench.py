@@ -656,8 +656,8 @@ def main():
                     help="single-instance workloads: skip the per-launch event pass (e.g. under a profiler)")
     ap.add_argument("--divb-report", action="store_true", help="ensemble: max |div B| of every instance")
     ap.add_argument("--tile", type=int, default=None,
-                    help="fixed tile shape of the mixed form (0: 256/128, 1: 128/64, 2: 512/256) instead of "
-                         "autotuning, e.g. for profiles without the tuner's launches")
+                    help="single-instance workloads: fixed tile of the mixed form (0-8, see TILES; 0: 256/128, "
+                         "1: 128/64, 2: 512/256) instead of autotuning, e.g. for profiles without the tuner's launches")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run two kernels of known HBM bytes (a 256 MiB fill: 256 MiB written; an "
                          "in-place add of two 256 MiB arrays: 512 MiB read, 256 MiB written) so that "
