@@ -579,17 +579,18 @@ __device__ __forceinline__ double az_residual(const double *A, const double *xy,
     return -(au * (e1a + e1b) + av * (e2a + e2b));
 }
 
-// FP64 A_z at flux point (fam, r, line) of block cell c: the upwind face value
-// of k_face_inv_mx at face points, the trace of the FP64 plane elsewhere
+// FP64 A_z at flux point (fam, r, line) of a cell: the upwind face value of a
+// face pass at face points (FA: the records of the cell `cell`), the trace of
+// the cell's FP64 A_z plane (az: N^2 values) elsewhere
 template <int N>
-__device__ __forceinline__ double az_at(const double *FA, const double *sAz, int cell, int c, int fam, int r,
-                                        int line, const double *ops) {
+__device__ __forceinline__ double az_at(const double *FA, const double *az, int cell, int fam, int r, int line,
+                                        const double *ops) {
     if (r == 0 || r == N) {
         int lfi, nfp;
         face_of<N>(fam, r, line, lfi, nfp);
         return FA[(size_t)((cell * 4 + lfi) * N + nfp)];
     }
-    return trace1<N>(sAz + c * N * N, fam, fam == 0 ? r : line, fam == 0 ? line : r, ops);
+    return trace1<N>(az, fam, fam == 0 ? r : line, fam == 0 ? line : r, ops);
 }
 
 // a cell face point's trace in the compact form: the FP32 deviations of
@@ -837,7 +838,7 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
             float *f = sF + c * FST + line * (N + 1) + r;
 #pragma unroll
             for (int v = 0; v < 8; v++) f[v * NF] = F[v] * s1 + G[v] * s2 + jump[v];
-            sA[(fam * CPB + c) * AST + line * (N + 1) + r] = az_at<N>(a.FA, sAz, c0 + c, c, fam, r, line, sOps);
+            sA[(fam * CPB + c) * AST + line * (N + 1) + r] = az_at<N>(a.FA, sAz + c * NP, c0 + c, fam, r, line, sOps);
         }
         __syncthreads();
         // the point's Qa for the combination, loaded while the derivative is summed
@@ -1378,9 +1379,10 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     constexpr int CPB = S::CPB, NP = S::NP, NF = S::NF, NQ = S::NQ, T = TS::THREADS;
     constexpr int UST = 8 * NF + 1;      // per-cell stride of the combined flux planes
     constexpr int AST = NF + 1;          // per-cell stride of the FP64 A_z flux-point values
-    // the flux planes of both families side by side where the LDS allows the
-    // same blocks per CU (N >= 6: two 512-thread blocks either way), of one
-    // family at a time otherwise
+    // the flux planes of both families side by side at N >= 6 (two 512-thread
+    // blocks per CU either way there), of one family at a time below (at N = 5
+    // both side by side measured slower even where the LDS kept five blocks
+    // per CU)
     constexpr bool MERGE = N >= 6;
     constexpr int UF = CPB * NQ + (MERGE ? 2 : 1) * CPB * UST;   // deviations + flux planes
     constexpr int UD = (UF + 1) / 2;
@@ -1390,8 +1392,6 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     __shared__ float sE[NG == 4 ? CPB * 8 : 1];   // FP32 edge vectors of the block's cells
     __shared__ double sBar[CPB * NV];
     __shared__ Ref sRef[CPB];
-    __shared__ double sAV[CPB * NP];
-    __shared__ double sAz[CPB * NP];     // FP64 A_z plane of each cell
     __shared__ double sA[2 * CPB * AST]; // FP64 A_z at the flux points of both families
     __shared__ float sLf[(N + 1) * N], sMf[(N + 1) * N];
     __shared__ double sOps[Ops64Layout<N>::SIZE];   // FP64 operator tables (OpTables64 layout)
@@ -1423,7 +1423,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
         if constexpr (NG == 4) edges.gather(a.cellxy + (size_t)c0 * 2 * NG, ncl);
         FaceRecStage<N, CPB, T> recs;
         recs.gather((const float *)a.FF, a.FA, c0, ncl);
-        stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, sBar, sRef, sD, sAz, 8, sQ64);
+        stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, sBar, sRef, sD, nullptr, 8, sQ64);
         scatter_lds<PX, T>(sX, hx, ncl * 2 * NG);
         if constexpr (NG == 4) edges.scatter(sE, ncl);
         recs.scatter(sFF, sFA, ncl);
@@ -1433,7 +1433,11 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     __syncthreads();
     // the By indicator of the input state, for the Tecplot output's muAV(6)
     if constexpr ((VM & 2) != 0) {
-        if (v.sm6) av_sensor<N, CPB>(v.sm6, sQ64 + 5 * NP, sAV, ncl, c0, sOps);
+        if (v.sm6) {
+            // (its work area: the flux planes, unused until the flux phase)
+            av_sensor<N, CPB>(v.sm6, sQ64 + 5 * NP, U + (CPB * NQ + 1) / 2, ncl, c0, sOps);
+            __syncthreads();
+        }
     }
     const float gam = float(a.gam);
     const int grp = tid >= TS::G1 ? 1 : 0;
@@ -1473,7 +1477,8 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
         dflux(sRef[c], d, gam - 1.f, F, G);
 #pragma unroll
         for (int w = 0; w < 8; w++) u[w * NF] = F[w] * s1 + G[w] * s2 + extra[w];
-        sA[(fam * CPB + c) * AST + line * (N + 1) + r] = az_at<N>(sFA, sAz, c, c, fam, r, line, sOps);
+        // (the FP64 A_z plane is the state's, combined only after the flux phase)
+        sA[(fam * CPB + c) * AST + line * (N + 1) + r] = az_at<N>(sFA, sQ64 + c * NQ + 8 * NP, c, fam, r, line, sOps);
     };
     // the point's Qa for the combination, loaded while the derivative is summed
     auto load_point = [&]() {
@@ -1558,7 +1563,8 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
     if (a.kind == KIND_RESID) return;
     __syncthreads();
     store_traces_mx<N>(sQ64, ncl, c0, a.mt, sBar, NV, 1, sOps);
-    if constexpr ((VM & 2) != 0) av_sensor_mx<N, CPB>(a.sm, sQ64, sBar, reinterpret_cast<float *>(sAV), ncl, c0, sRav);
+    // (its work area: the deviations and flux planes, free since the derivative)
+    if constexpr ((VM & 2) != 0) av_sensor_mx<N, CPB>(a.sm, sQ64, sBar, reinterpret_cast<float *>(U), ncl, c0, sRav);
 }
 
 }  // namespace chorus
