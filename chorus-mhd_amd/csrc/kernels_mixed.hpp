@@ -309,13 +309,13 @@ __device__ __forceinline__ float inv_jacobian_mx(const float *sE, const double *
 // Phase A of the mixed cell kernels: the block's cells (contiguous in HBM)
 // become FP32 deviations from their reference states Qbar (the FP64 state at
 // the central solution point).  Each thread keeps the values it loaded in
-// registers; the threads holding a central point publish it as Qbar, and keep,
-// if given, receives the FP64 plane of variable keep_var of each cell, and q64,
-// if given, the FP64 state itself (the combination reads it from there).
+// registers; the threads holding a central point publish it as Qbar, and q64,
+// if given, receives the FP64 state itself (the combination, the FP64 A_z
+// derivative and the traces read it from there).
 // Called by all THREADS threads of the block.
 template <int N, int CPB, int THREADS>
 __device__ __forceinline__ void stage_deviations(const double *Qg, int ncl, double gam, double *sBar, Ref *sRef,
-                                                 float *dq, double *keep, int keep_var = 0, double *q64 = nullptr) {
+                                                 float *dq, double *q64 = nullptr) {
     constexpr int NP = N * N, NQ = NV * NP, PER = (CPB * NQ + THREADS - 1) / THREADS;
     constexpr int PC = (N / 2) * N + N / 2;   // central solution point
     const int tid = threadIdx.x, n = ncl * NQ;
@@ -330,7 +330,6 @@ __device__ __forceinline__ void stage_deviations(const double *Qg, int ncl, doub
         const int i = tid + k * THREADS;
         const int c = i / NQ, r = i % NQ, v = r / NP, p = r % NP;
         if (i < n && p == PC) sBar[c * NV + v] = held[k];
-        if (keep && i < n && v == keep_var) keep[c * NP + p] = held[k];
         if (q64 && i < n) q64[i] = held[k];
     }
     __syncthreads();
@@ -779,7 +778,6 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
     __shared__ float sE[NG == 4 ? CPB * 8 : 1];   // FP32 edge vectors of the block's cells
     __shared__ double sBar[CPB * NV];
     __shared__ Ref sRef[CPB];
-    __shared__ double sAz[CPB * NP];     // FP64 A_z plane of each cell
     __shared__ double sA[2 * CPB * AST]; // FP64 A_z at the flux points of both families
     __shared__ float sLf[(N + 1) * N], sMf[(N + 1) * N];
     __shared__ double sOps[Ops64Layout<N>::SIZE];   // FP64 operator tables (OpTables64 layout)
@@ -799,7 +797,7 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
         ops64.gather();
         EdgeStage<CPB, T> edges;
         if constexpr (NG == 4) edges.gather(a.cellxy + (size_t)c0 * 2 * NG, ncl);
-        stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, sBar, sRef, sD, sAz, 8, sQ64);
+        stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, sBar, sRef, sD, sQ64);
         scatter_lds<PX, T>(sX, hx, ncl * 2 * NG);
         if constexpr (NG == 4) edges.scatter(sE, ncl);
         ops.scatter(sLf, sMf);
@@ -838,7 +836,8 @@ __global__ void __launch_bounds__((FluxCellShape<N, TB>::THREADS)) __attribute__
             float *f = sF + c * FST + line * (N + 1) + r;
 #pragma unroll
             for (int v = 0; v < 8; v++) f[v * NF] = F[v] * s1 + G[v] * s2 + jump[v];
-            sA[(fam * CPB + c) * AST + line * (N + 1) + r] = az_at<N>(a.FA, sAz + c * NP, c0 + c, fam, r, line, sOps);
+            // (the FP64 A_z plane is the state's, combined only after the flux phase)
+            sA[(fam * CPB + c) * AST + line * (N + 1) + r] = az_at<N>(a.FA, sQ64 + c * NQ + 8 * NP, c0 + c, fam, r, line, sOps);
         }
         __syncthreads();
         // the point's Qa for the combination, loaded while the derivative is summed
@@ -1423,7 +1422,7 @@ __global__ void __launch_bounds__((MxVShape<N, TB>::THREADS)) __attribute__((amd
         if constexpr (NG == 4) edges.gather(a.cellxy + (size_t)c0 * 2 * NG, ncl);
         FaceRecStage<N, CPB, T> recs;
         recs.gather((const float *)a.FF, a.FA, c0, ncl);
-        stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, sBar, sRef, sD, nullptr, 8, sQ64);
+        stage_deviations<N, CPB, T>(a.Qin + (size_t)c0 * NQ, ncl, a.gam, sBar, sRef, sD, sQ64);
         scatter_lds<PX, T>(sX, hx, ncl * 2 * NG);
         if constexpr (NG == 4) edges.scatter(sE, ncl);
         recs.scatter(sFF, sFA, ncl);
