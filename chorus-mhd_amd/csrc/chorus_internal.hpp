@@ -107,7 +107,8 @@ struct FaceArgs {
     const void *gtr;
     const double *sm;        // AV indicator per cell
     const double *mucell;    // k_face_visc: muAV(1) per cell of this stage's input (k_grad)
-    double *eig;             // k_face_inv: atomic max of the Rusanov wave speed (AV), per instance
+    double *eig;             // AV: the stage maximum of the Rusanov wave speed, per instance (k_face_inv
+                             // forms it by atomic max; the mixed viscous face kernel reads it)
     const int *inst;         // batched instances: instance of each cell (null: one instance)
     const double *hinst;     // batched instances: AV length scale per instance
     int artiv;
@@ -136,7 +137,8 @@ struct ViscArgs {
     const int *inst;    // batched instances: instance of each cell (null: one instance)
     const double *hinst;   // batched instances: AV length scale per instance
     double *sm;         // log10 smoothness indicator of density per cell
-    double *eig;        // this stage's maximum Rusanov wave speed (atomic max)
+    double *eig;        // this stage's maximum Rusanov wave speed (atomic max; with AV alone the
+                        // mixed gradient kernel forms it, otherwise a face pass before it)
     double *eig_next;   // the next stage's slots (one per instance), zeroed by k_stage_visc
     int ninst;          // instances sharing the launch
     const double *TQ;   // traces of Qin at the face points
