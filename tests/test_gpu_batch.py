@@ -177,6 +177,49 @@ def test_batch_members_vs_reference(precision):
     b.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_ssp54_viscous_and_av_vs_reference(precision):
+    """FiveStageRk batches with physical viscosity, resistivity and AV together
+    (VM=3) and AV alone (VM=2), N=5 on FREE faces, against the reference's
+    fixtures; an oracle-made Orszag-Tang member shares the VM=3 group, so that
+    group's stage maximum of the wave speed is formed per instance."""
+    _say("batch SSP(5,4) vs reference", precision)
+    names = ["visc_n6_N5_free_av", "smooth_n6_N5_free_av"]
+    gs = [load_golden(n) for n in names]
+    mem = [Instance(g["mesh"], int(g["N"]), float(g["dt"]), precision=precision, **golden_params(g))
+           for g in gs]
+    kw = golden_params(gs[0])
+    assert kw["k_stage"] == 5 and kw["artiv"] == 1 and kw["mu"] > 0 and kw["eta0"] > 0
+    om = quad_grid(7, 5, 0.0, 2 * math.pi, 0.0, 2 * math.pi, jitter=0.1, seed=33, bc="FREE")
+    o = orc.Oracle(om, 5, dt=float(gs[0]["dt"]), **kw)
+    o.init_ot()
+    oq0 = o.get_q()
+    o.steps(2)
+    oq2 = o.get_q()
+    mem.append(Instance(om, 5, float(gs[0]["dt"]), precision=precision, **kw))
+    b = Batch(mem)
+    groups = b.groups()
+    assert sorted(x["ninst"] for x in groups) == [1, 2]
+    for i, g in enumerate(gs):
+        b.set_state(i, g["q0"])
+    b.set_state(2, oq0)
+    assert all(int(g["steps"]) == 2 for g in gs)
+    b.step(2)
+    for i, (got, ref, what) in enumerate(((b.get_state(0), gs[0]["qend"], names[0]),
+                                          (b.get_state(1), gs[1]["qend"], names[1]),
+                                          (b.get_state(2), oq2, "oracle member"))):
+        if precision == "mixed":
+            rel = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+            _say(what, "mixed rel L2", rel)
+            assert rel <= 1e-6, what
+        else:
+            scale = np.abs(ref).max(axis=(0, 1, 2))
+            err = np.abs(got - ref).max(axis=(0, 1, 2))
+            _say(what, "fp64 max err / scale", float((err / np.where(scale > 0, scale, 1.0)).max()))
+            assert (err <= 1e-13 * scale).all(), what
+    b.close()
+
+
 def test_batch_rejects_mixed_steppers():
     mesh = quad_grid(4, 4, 0, 1, 0, 1)
     with pytest.raises(Exception):
